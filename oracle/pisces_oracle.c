@@ -1,0 +1,976 @@
+/*
+ * pisces_oracle.c -- CPU restatement of the PISCES excess-electron eigensolve
+ * path (sine-DVR H*x, DPP electron-water potential, rigid DPP water model with
+ * self-consistent induced dipoles, reverse-communication Davidson).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library's parity tests and
+ * the CPU baseline of the benchmark.  Nothing in the product links this file.
+ *
+ * Each function names the reference lines it restates.  The arithmetic keeps
+ * the reference's operation order where that is cheap (so FP64 results agree to
+ * a few ulps); it is written from the algorithm, not copied.
+ */
+#include "pisces_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Constants as the reference defines them.  PI is the 15-digit value of
+ * src/constants.h:4 and src/sine_dvr.cpp:24 (not M_PI): keeping it keeps the
+ * T matrices and the Euler rotations identical to the reference. */
+#define OR_PI 3.14159265358979
+#define OR_BOHR2ANGS 0.52917720859                 /* src/constants.h:17 */
+#define OR_ANGS2BOHR (1.0 / OR_BOHR2ANGS)
+/* Rigid TTM/DPP monomer, src/ttm_data.h:8-58 */
+#define OR_TTM_QH 0.5742185854046506
+#define OR_TTM_OZ 0.0650980307
+#define OR_TTM_HZ (-0.5207842453)
+#define OR_TTM_HY 0.756950327
+#define OR_DPP_DELTAM 0.25
+/* DPP damping constants of the water model (chargeselector 2),
+ * src/Water.cpp:241-243: Damping[1] (charge-dipole), Damping[2] (dipole-dipole) */
+#define OR_DAMP_CD 0.21
+#define OR_DAMP_DD 0.3
+
+static int g_threads = 0;
+void or_set_threads(int n) { g_threads = n; }
+static int nthreads(void)
+{
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ======================================================================= a1 */
+static void sine_dvr_impl(int n, double mass, double x0, double xnp1, double *x,
+                          double *T, int packed)
+{
+    /* src/sine_dvr.cpp:47-111 */
+    double dx = (xnp1 - x0) / (double)(n + 1);
+    for (int j = 0; j < n; ++j) x[j] = x0 + dx * (double)(j + 1);
+    double f1 = 0.5 * OR_PI * OR_PI / (mass * dx * dx);
+    double f2 = OR_PI / (double)(n + 1);
+    double f3 = (double)(n + 1) * (double)(n + 1);
+    for (int a = 0; a < n; ++a) {
+        double s = sin((a + 1) * f2);
+        double taa = f1 * (1.0 / 3.0 + 1.0 / (6.0 * f3) - 0.5 / (f3 * (s * s)));
+        if (packed) T[a * (a + 1) / 2 + a] = taa;
+        else T[a + n * a] = taa;
+    }
+    for (int a = 0; a < n; ++a) {
+        for (int b = 0; b < a; ++b) {
+            double dc = cos(f2 * (a + 1)) - cos(f2 * (b + 1));
+            double tab = f1 * 2.0 / f3 * sin(f2 * (a + 1)) * sin(f2 * (b + 1)) / (dc * dc);
+            if ((a - b) % 2 == 1) tab = -tab;
+            if (packed) {
+                T[a * (a + 1) / 2 + b] = tab;
+            } else {
+                T[a + n * b] = tab;
+                T[b + n * a] = tab;
+            }
+        }
+    }
+}
+
+void or_sine_dvr(int n, double mass, double x0, double xnp1, double *x, double *T)
+{
+    sine_dvr_impl(n, mass, x0, xnp1, x, T, 0);
+}
+
+void or_sine_dvr_packed(int n, double mass, double x0, double xnp1, double *x, double *Tp)
+{
+    sine_dvr_impl(n, mass, x0, xnp1, x, Tp, 1);
+}
+
+/* ======================================================================= a3 */
+void or_apply_h(int nx, int ny, int nz, const double *Tx, const double *Ty,
+                const double *Tz, const double *V, const double *X, double *Y,
+                int nvec)
+{
+    const long ng = (long)nx * ny * nz;
+    const long sy = nx, sz = (long)nx * ny;
+    for (int v = 0; v < nvec; ++v) {
+        const double *x = X + (long)v * ng;
+        double *y = Y + (long)v * ng;
+        /* y = V o x  (src/mtx.cpp:34-38) */
+#pragma omp parallel for num_threads(nthreads()) schedule(static)
+        for (long g = 0; g < ng; ++g) y[g] = V[g] * x[g];
+        /* x sweep (src/mtx.cpp:58-73): y(:,j,k) += Tx x(:,j,k) */
+#pragma omp parallel for collapse(2) num_threads(nthreads()) schedule(static)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j) {
+                const double *xl = x + j * sy + k * sz;
+                double *yl = y + j * sy + k * sz;
+                for (int a = 0; a < nx; ++a) {
+                    double s = 0.0;
+                    for (int b = 0; b < nx; ++b) s += Tx[a + nx * b] * xl[b];
+                    yl[a] += s;
+                }
+            }
+        /* y sweep (src/mtx.cpp:77-88): stride nx */
+#pragma omp parallel for collapse(2) num_threads(nthreads()) schedule(static)
+        for (int k = 0; k < nz; ++k)
+            for (int i = 0; i < nx; ++i) {
+                const double *xl = x + i + k * sz;
+                double *yl = y + i + k * sz;
+                for (int a = 0; a < ny; ++a) {
+                    double s = 0.0;
+                    for (int b = 0; b < ny; ++b) s += Ty[a + ny * b] * xl[b * sy];
+                    yl[a * sy] += s;
+                }
+            }
+        /* z sweep (src/mtx.cpp:92-103): stride nx*ny */
+#pragma omp parallel for collapse(2) num_threads(nthreads()) schedule(static)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const double *xl = x + i + j * sy;
+                double *yl = y + i + j * sy;
+                for (int a = 0; a < nz; ++a) {
+                    double s = 0.0;
+                    for (int b = 0; b < nz; ++b) s += Tz[a + nz * b] * xl[b * sz];
+                    yl[a * sz] += s;
+                }
+            }
+    }
+}
+
+/* ======================================================================= a4 */
+void or_diag(int nx, int ny, int nz, const double *Tx, const double *Ty,
+             const double *Tz, const double *V, double *D)
+{
+    /* src/DVR.cpp:1808-1881: diag = V, then += Tx_ii, Ty_jj, Tz_kk in turn */
+#pragma omp parallel for collapse(2) num_threads(nthreads()) schedule(static)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                long g = i + (long)nx * (j + (long)ny * k);
+                double d = V[g];
+                d += Tx[i + nx * i];
+                d += Ty[j + ny * j];
+                d += Tz[k + nz * k];
+                D[g] = d;
+            }
+}
+
+/* ====================================================================== a11 */
+void or_piab_start(int nx, int ny, int nz, const double *xg, const double *yg,
+                   const double *zg, double *wf)
+{
+    /* src/DVR.cpp:1725-1805: x0 = first grid point, L = last - first; the
+     * function is the lowest state of the central half box, zero outside. */
+    const double *g[3] = {xg, yg, zg};
+    int n[3] = {nx, ny, nz};
+    double x0[3], L[3];
+    for (int d = 0; d < 3; ++d) {
+        x0[d] = g[d][0];
+        L[d] = g[d][n[d] - 1] - x0[d];
+    }
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                int ii[3] = {i, j, k};
+                double w = 1.0;
+                for (int d = 0; d < 3; ++d) {
+                    double q = g[d][ii[d]];
+                    if (q < x0[d] / 2 || q > -x0[d] / 2) w = 0.0;
+                    else w *= sin(OR_PI / L[d] * 2.0 * (q - x0[d] / 2.0));
+                }
+                wf[i + (long)nx * (j + (long)ny * k)] = w;
+            }
+}
+
+/* ======================================================================= a8 */
+/* Reference (standard-orientation) positions of one monomer relative to its
+ * centre of mass, bohr: Water::SetupCharge case 2, src/Water.cpp:216-236. */
+static void monomer_ref(double ref[4][3])
+{
+    const double a2b = OR_ANGS2BOHR;
+    ref[0][0] = 0; ref[0][1] = 0;                 ref[0][2] = OR_TTM_OZ * a2b;
+    ref[1][0] = 0; ref[1][1] = OR_TTM_HY * a2b;   ref[1][2] = OR_TTM_HZ * a2b;
+    ref[2][0] = 0; ref[2][1] = -OR_TTM_HY * a2b;  ref[2][2] = OR_TTM_HZ * a2b;
+    ref[3][0] = 0; ref[3][1] = 0;                 ref[3][2] = (OR_TTM_OZ - OR_DPP_DELTAM) * a2b;
+}
+
+/* Water::UpdatePositions -> ResetPositions + ApplyEuler, src/Water.cpp:837-943 */
+static void monomer_place(const double conf[6], double out[4][3])
+{
+    double ref[4][3];
+    monomer_ref(ref);
+    double c1 = cos(conf[3]), c2 = cos(conf[4]), c3 = cos(conf[5]);
+    double s1 = sin(conf[3]), s2 = sin(conf[4]), s3 = sin(conf[5]);
+    double R[9];
+    R[0] = c1 * c2 * c3 - s1 * s3; R[1] = -c1 * c2 * s3 - s1 * c3; R[2] = c1 * s2;
+    R[3] = s1 * c2 * c3 + c1 * s3; R[4] = -s1 * c2 * s3 + c1 * c3; R[5] = s1 * s2;
+    R[6] = -s2 * c3;               R[7] = s2 * s3;                 R[8] = c2;
+    for (int a = 0; a < 4; ++a) {
+        /* positions are first stored absolute (origin + ref) and the rotation
+         * acts on (position - origin), as the reference does */
+        double p0 = conf[0] + ref[a][0], p1 = conf[1] + ref[a][1], p2 = conf[2] + ref[a][2];
+        double o0 = p0 - conf[0], o1 = p1 - conf[1], o2 = p2 - conf[2];
+        double n0 = R[0] * o0 + R[1] * o1 + R[2] * o2;
+        double n1 = R[3] * o0 + R[4] * o1 + R[5] * o2;
+        double n2 = R[6] * o0 + R[7] * o1 + R[8] * o2;
+        out[a][0] = n0 + conf[0];
+        out[a][1] = n1 + conf[1];
+        out[a][2] = n2 + conf[2];
+    }
+}
+
+/* Water::Reposition, src/Water.cpp:695-825 (input in bohr) */
+static void monomer_project(const double *w, double conf[6])
+{
+    double o[3] = {w[0], w[1], w[2]}, h1[3] = {w[3], w[4], w[5]}, h2[3] = {w[6], w[7], w[8]};
+    double r1[3], r3[3], z[3], y[3], x[3];
+    for (int a = 0; a < 3; ++a) { r1[a] = h1[a] - o[a]; r3[a] = h2[a] - o[a]; }
+    for (int a = 0; a < 3; ++a) z[a] = r1[a] + r3[a];
+    double zm = sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+    for (int a = 0; a < 3; ++a) { z[a] /= zm; z[a] = -z[a]; }
+    double p = r1[0] * z[0] + r1[1] * z[1] + r1[2] * z[2];
+    for (int a = 0; a < 3; ++a) y[a] = r1[a] - p * z[a];
+    double ym = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+    for (int a = 0; a < 3; ++a) y[a] /= ym;
+    x[0] = y[1] * z[2] - y[2] * z[1];
+    x[1] = y[2] * z[0] - y[0] * z[2];
+    x[2] = y[0] * z[1] - y[1] * z[0];
+    double phi, theta, psi;
+    theta = acos(z[2]);
+    double st = sin(theta);
+    if (fabs(st) > 1e-5) {
+        double tc = -x[2] / st;
+        double ac = acos(tc);
+        if (tc <= -1) ac = OR_PI;
+        psi = (y[2] >= 0) ? ac : 2 * OR_PI - ac;
+        tc = z[0] / st;
+        if (tc > 1) tc = 1;
+        if (tc < -1) tc = -1;
+        phi = (z[1] >= 0) ? acos(tc) : 2 * OR_PI - acos(tc);
+    } else {
+        phi = 0.0;
+        psi = (x[1] >= 0) ? acos(y[1]) : 2 * OR_PI - acos(y[1]);
+    }
+    conf[0] = (h1[0] + h2[0] + 16.0 * o[0]) / 18.0;
+    conf[1] = (h1[1] + h2[1] + 16.0 * o[1]) / 18.0;
+    conf[2] = (h1[2] + h2[2] + 16.0 * o[2]) / 18.0;
+    conf[3] = phi;
+    conf[4] = theta;
+    conf[5] = psi;
+}
+
+int or_water_sites(int nw, const double *xyz_ang, double *sites, double *conf)
+{
+    for (int w = 0; w < nw; ++w) {
+        double b[9], c[6], pos[4][3];
+        /* Water::Reposition_A converts to bohr, src/Water.cpp:679-687 */
+        for (int a = 0; a < 9; ++a) b[a] = xyz_ang[9 * w + a] * OR_ANGS2BOHR;
+        monomer_project(b, c);
+        monomer_place(c, pos);
+        for (int s = 0; s < 4; ++s)
+            for (int a = 0; a < 3; ++a) {
+                sites[12 * w + 3 * s + a] = pos[s][a];
+                if (!isfinite(pos[s][a])) return -1;
+            }
+        if (conf)
+            for (int a = 0; a < 6; ++a) conf[6 * w + a] = c[a];
+    }
+    return 0;
+}
+
+/* per-site charge, damping polarizability ("sigma") and polarizability,
+ * src/Water.cpp:233-236 */
+static void site_params(int s, double *q, double *sig, double *alpha)
+{
+    const double a3 = OR_ANGS2BOHR * OR_ANGS2BOHR * OR_ANGS2BOHR;
+    const double polO = 1.22 * a3, polH = 0.28 * a3;
+    switch (s) {
+    case 0: *q = 0.0; *sig = 0.0; *alpha = polO; break;
+    case 1: case 2: *q = OR_TTM_QH; *sig = polH; *alpha = polH; break;
+    default: *q = -2.0 * OR_TTM_QH; *sig = polO; *alpha = 0.0; break;
+    }
+}
+
+/* ElectrostaticDCDamp / ElectrostaticDDDamp, src/DPP.cpp:1798-1812 */
+static double damp_cd(double r, double ai, double aj, double a)
+{
+    double A = pow(ai * aj, 1.0 / 6.0);
+    double x = a * pow(r / A, 3);
+    return 1.0 - exp(-x);
+}
+static double damp_dd(double r, double ai, double aj, double a)
+{
+    double A = pow(ai * aj, 1.0 / 6.0);
+    double x = a * pow(r / A, 3);
+    return 1.0 - (1 + x) * exp(-x);
+}
+
+int or_water_induced(int nw, const double *sites, double *mu, double *epc,
+                     double *e_pol, int *iters)
+{
+    const int ns = 3 * nw; /* polarizable sites O,H1,H2 per water */
+    double *al = malloc(sizeof(double) * ns), *xs = malloc(sizeof(double) * 3 * ns);
+    double *f1 = calloc(3 * ns, sizeof(double)), *f2 = malloc(sizeof(double) * 3 * ns);
+    double *dp = malloc(sizeof(double) * 3 * ns), *od = malloc(sizeof(double) * 3 * ns);
+    for (int w = 0; w < nw; ++w)
+        for (int s = 0; s < 3; ++s) {
+            double q, sg, a;
+            site_params(s, &q, &sg, &a);
+            int i = 3 * w + s;
+            al[i] = a;
+            for (int c = 0; c < 3; ++c) xs[3 * i + c] = sites[12 * w + 3 * s + c];
+        }
+    /* field of the permanent charges (src/DPP.cpp:830-865), skipping charges of
+     * the site's own water */
+    for (int w = 0; w < nw; ++w)
+        for (int s = 0; s < 4; ++s) {
+            double q, sg, a;
+            site_params(s, &q, &sg, &a);
+            if (q == 0) continue;
+            const double *pc = sites + 12 * w + 3 * s;
+            for (int i = 0; i < ns; ++i) {
+                if (i / 3 == w) continue;
+                double rx = xs[3 * i] - pc[0], ry = xs[3 * i + 1] - pc[1], rz = xs[3 * i + 2] - pc[2];
+                double t = q * pow(rx * rx + ry * ry + rz * rz, -1.5);
+                double dist = sqrt(rx * rx + ry * ry + rz * rz);
+                double dmp = damp_cd(dist, al[i], sg, OR_DAMP_CD);
+                f1[3 * i] += t * rx * dmp;
+                f1[3 * i + 1] += t * ry * dmp;
+                f1[3 * i + 2] += t * rz * dmp;
+            }
+        }
+    const double newfac = 0.5, oldfac = 1 - newfac;
+    for (int i = 0; i < ns; ++i)
+        for (int c = 0; c < 3; ++c) dp[3 * i + c] = newfac * al[i] * f1[3 * i + c];
+    /* 0.5-mixed fixed-point iteration (src/DPP.cpp:953-1006) */
+    int maxit = 5 * (ns + 10), conv = 0, it;
+    double tester = 0;
+    for (it = 0; it < maxit; ++it) {
+        for (int k = 0; k < 3 * ns; ++k) f2[k] = 0;
+        for (int i = 0; i < ns; ++i)
+            for (int j = 0; j < ns; ++j) {
+                if (j == i) continue;
+                /* FieldOfDipole(dipole j at site j, evaluated at site i),
+                 * src/DPP.cpp:2377-2410 */
+                double rx = xs[3 * i] - xs[3 * j], ry = xs[3 * i + 1] - xs[3 * j + 1],
+                       rz = xs[3 * i + 2] - xs[3 * j + 2];
+                double r2 = rx * rx + ry * ry + rz * rz;
+                double r5 = pow(r2, -2.5);
+                double d1 = damp_dd(sqrt(r2), al[i], al[j], OR_DAMP_DD);
+                double d2 = damp_cd(sqrt(r2), al[i], al[j], OR_DAMP_DD);
+                double txx = (d1 * 3. * rx * rx - d2 * r2) * r5;
+                double tyy = (d1 * 3. * ry * ry - d2 * r2) * r5;
+                double tzz = (d1 * 3. * rz * rz - d2 * r2) * r5;
+                double txy = d1 * 3. * rx * ry * r5;
+                double txz = d1 * 3. * rx * rz * r5;
+                double tyz = d1 * 3. * ry * rz * r5;
+                double mx = dp[3 * j], my = dp[3 * j + 1], mz = dp[3 * j + 2];
+                f2[3 * i] += txx * mx + txy * my + txz * mz;
+                f2[3 * i + 1] += txy * mx + tyy * my + tyz * mz;
+                f2[3 * i + 2] += txz * mx + tyz * my + tzz * mz;
+            }
+        for (int i = 0; i < ns; ++i)
+            for (int c = 0; c < 3; ++c) {
+                od[3 * i + c] = dp[3 * i + c];
+                dp[3 * i + c] = oldfac * od[3 * i + c] + newfac * al[i] * (f1[3 * i + c] + f2[3 * i + c]);
+            }
+        tester = 0;
+        for (int i = 0; i < ns; ++i) {
+            double tx = od[3 * i] - dp[3 * i], ty = od[3 * i + 1] - dp[3 * i + 1],
+                   tz = od[3 * i + 2] - dp[3 * i + 2];
+            tester += tx * tx + ty * ty + tz * tz;
+        }
+        tester = sqrt(tester) / 3 / ns;
+        if (tester < 1e-12) { conv = 1; break; }
+    }
+    if (iters) *iters = conv ? it + 1 : it;
+    if (mu) memcpy(mu, dp, sizeof(double) * 3 * ns);
+    if (epc) memcpy(epc, f1, sizeof(double) * 3 * ns);
+    if (e_pol) {
+        /* sum over pairs of WaterWaterPolarizationEnergy, src/DPP.cpp:1935-2024 */
+        double e = 0;
+        for (int wi = 0; wi < nw - 1; ++wi)
+            for (int wj = wi + 1; wj < nw; ++wj)
+                for (int k = 0; k < 3; ++k) { /* polarizable centres O,H1,H2 */
+                    double q, sg, a;
+                    site_params(k, &q, &sg, &a);
+                    const double *x1 = sites + 12 * wi + 3 * k, *x2 = sites + 12 * wj + 3 * k;
+                    const double *m1 = dp + 3 * (3 * wi + k), *m2 = dp + 3 * (3 * wj + k);
+                    for (int j = 0; j < 4; ++j) { /* W2 polarizing W1 */
+                        double q2, s2, a2;
+                        site_params(j, &q2, &s2, &a2);
+                        if (q2 == 0) continue;
+                        const double *p2 = sites + 12 * wj + 3 * j;
+                        double rx = x1[0] - p2[0], ry = x1[1] - p2[1], rz = x1[2] - p2[2];
+                        double dist = sqrt(rx * rx + ry * ry + rz * rz);
+                        double dmp = damp_cd(dist, a, s2, OR_DAMP_CD);
+                        e -= q2 * (m1[0] * rx + m1[1] * ry + m1[2] * rz) * dmp *
+                             pow(rx * rx + ry * ry + rz * rz, -1.5) / 2.0;
+                    }
+                    for (int j = 0; j < 4; ++j) { /* W1 polarizing W2 */
+                        double q1, s1, a1;
+                        site_params(j, &q1, &s1, &a1);
+                        if (q1 == 0) continue;
+                        const double *p1 = sites + 12 * wi + 3 * j;
+                        double rx = x2[0] - p1[0], ry = x2[1] - p1[1], rz = x2[2] - p1[2];
+                        double dist = sqrt(rx * rx + ry * ry + rz * rz);
+                        double dmp = damp_cd(dist, a, s1, OR_DAMP_CD);
+                        e -= q1 * (m2[0] * rx + m2[1] * ry + m2[2] * rz) * dmp *
+                             pow(rx * rx + ry * ry + rz * rz, -1.5) / 2.0;
+                    }
+                }
+        *e_pol = e;
+    }
+    free(al); free(xs); free(f1); free(f2); free(dp); free(od);
+    return conv ? 0 : 1;
+}
+
+int or_water_lists(int nw, const double *sites, const double *mu, double *q,
+                   int *iq, double *d, int *id)
+{
+    (void)sites;
+    int nq = 0;
+    for (int w = 0; w < nw; ++w)
+        for (int s = 0; s < 4; ++s) {
+            double qq, sg, a;
+            site_params(s, &qq, &sg, &a);
+            if (qq != 0) { q[nq] = qq; iq[nq] = 4 * w + s; ++nq; }
+        }
+    int nd = 0;
+    for (int w = 0; w < nw; ++w)
+        for (int s = 0; s < 4; ++s) {
+            double qq, sg, a;
+            site_params(s, &qq, &sg, &a);
+            if (a != 0) {
+                for (int c = 0; c < 3; ++c) d[3 * nd + c] = mu[3 * nd + c];
+                id[nd] = 4 * w + s;
+                ++nd;
+            }
+        }
+    return nq;
+}
+
+/* ==================================================================== a5 a6 */
+typedef struct {
+    int damp_type, rep_type, pol_type, adiabatic;
+    double qdamp, ddamp, pdamp, alpha, eps;
+    int ns, nq, nd, npps, nrep, nion;
+    const double *sites, *q, *d, *ion_xyz, *ion_q;
+    const int *iq, *id;
+    double ion_r;
+    int *ipps;
+    double *rep;  /* exponent, coefficient */
+    int *irep;
+} or_pot;
+
+static double reff(double r, double r0)
+{
+    /* effective distance, src/Potential.cpp:1191-1196 */
+    if (r < r0) {
+        double t = r / r0;
+        return r0 * (0.5 + t * t * t * (1.0 - 0.5 * t));
+    }
+    return r;
+}
+
+static int pot_setup(or_pot *P, const int *flags, const double *pp, int nsites)
+{
+    /* Potential::SetupDPPGTOP, src/Potential.cpp:234-593 */
+    int pf = flags[0];
+    if (pf < 1 || pf > 4) return -2;
+    P->damp_type = flags[1];
+    P->pol_type = flags[3];
+    P->adiabatic = flags[4];
+    if (P->pol_type != 0 && P->pol_type != 1) return -2;
+    P->alpha = pp[0];
+    double vrep = pp[2], vrepO = (flags[2] == 1) ? pp[5] : pp[2];
+    if (P->damp_type == 1) {
+        P->qdamp = 1.0 / (pp[3] * pp[3]);
+        P->ddamp = 1.0 / (pp[4] * pp[4]);
+        P->pdamp = 1.0 / (pp[1] * pp[1]);
+    } else if (P->damp_type == 2) {
+        P->qdamp = fabs(pp[3]);
+        P->ddamp = fabs(pp[4]);
+        P->pdamp = fabs(pp[1]);
+    } else return -2;
+    P->eps = pp[6];
+    if (nsites % 4) return -3;
+    int nw = nsites / 4;
+    P->npps = (P->pol_type == 1) ? nw : 0;
+    P->ipps = malloc(sizeof(int) * (nw + 1));
+    for (int i = 0; i < nw; ++i) P->ipps[i] = 4 * i + 3; /* M sites */
+    /* repulsive cores: src/Potential.cpp:598-728 */
+    double sH = 1.0 / vrep, sO = 1.0 / vrepO;
+    int per = (pf == 1) ? 6 : (pf == 2) ? 12 : (pf == 3) ? 4 : 0;
+    P->nrep = per * nw;
+    P->rep = malloc(sizeof(double) * 2 * (P->nrep + 1));
+    P->irep = malloc(sizeof(int) * (P->nrep + 1));
+    P->rep_type = (pf == 3) ? 2 : (pf == 4) ? 0 : 1;
+    for (int w = 0; w < nw; ++w) {
+        double *g = P->rep + 2 * per * w;
+        int *s = P->irep + per * w;
+        if (pf == 1) {
+            static const double e6[3] = {33.87, 5.095, 1.159};
+            static const double c6[3] = {2.84230876640242, 4.55401933663180, 6.91243747981459};
+            for (int h = 0; h < 2; ++h)
+                for (int k = 0; k < 3; ++k) {
+                    s[3 * h + k] = 4 * w + 1 + h;
+                    g[2 * (3 * h + k)] = e6[k];
+                    g[2 * (3 * h + k) + 1] = c6[k] * (1.0 / vrep);
+                }
+        } else if (pf == 2) {
+            static const double eo[6] = {322.037, 48.4308, 10.4206, 7.40294, 1.5762, 0.373684};
+            static const double co[6] = {41.69216064, 58.35946922, 35.19878892, -3.27944113, 4.44843633, 2.36213092};
+            static const double eh[3] = {5.447178, 0.824547, 0.183192};
+            static const double ch[3] = {0.69233452, 0.66686241, 0.07193908};
+            for (int k = 0; k < 6; ++k) { s[k] = 4 * w; g[2 * k] = eo[k]; g[2 * k + 1] = co[k] * sO; }
+            for (int h = 0; h < 2; ++h)
+                for (int k = 0; k < 3; ++k) {
+                    int m = 6 + 3 * h + k;
+                    s[m] = 4 * w + 1 + h;
+                    g[2 * m] = eh[k];
+                    g[2 * m + 1] = ch[k] * sH;
+                }
+        } else if (pf == 3) {
+            s[0] = 4 * w; s[1] = 4 * w; s[2] = 4 * w + 1; s[3] = 4 * w + 2;
+            g[0] = 9.98526; g[1] = 192.608 * sO;
+            g[2] = 1.67075; g[3] = 13.8937 * sO;
+            g[4] = 1.64121; g[5] = 1.7643 * sH;
+            g[6] = 1.64121; g[7] = 1.7643 * sH;
+        }
+    }
+    return 0;
+}
+
+static void pot_free(or_pot *P)
+{
+    free(P->ipps); free(P->rep); free(P->irep);
+}
+
+/* EvaluateDPPGTOP at one point with the per-site distance table of
+ * Potential::Evaluate (src/Potential.cpp:136-146). */
+static double pot_eval(const or_pot *P, const double *r, double *R, double *R2,
+                       double *Rx, double *Ry, double *Rz, double *Rm3, double *parts)
+{
+    for (int i = 0; i < P->ns; ++i) {
+        Rx[i] = r[0] - P->sites[3 * i];
+        Ry[i] = r[1] - P->sites[3 * i + 1];
+        Rz[i] = r[2] - P->sites[3 * i + 2];
+        R2[i] = Rx[i] * Rx[i] + Ry[i] * Ry[i] + Rz[i] * Rz[i];
+        R[i] = sqrt(R2[i]);
+        Rm3[i] = 1.0 / (R2[i] * R[i]);
+    }
+    /* charges, src/Potential.cpp:1168-1210 */
+    double vpc = 0;
+    for (int i = 0; i < P->nq; ++i) {
+        int s = P->iq[i];
+        if (P->damp_type == 1) vpc += -P->q[i] / R[s] * (1.0 - exp(-P->qdamp * R2[s]));
+        else vpc += -P->q[i] / reff(R[s], P->qdamp);
+    }
+    /* dipoles, src/Potential.cpp:1222-1270 */
+    double vind = 0;
+    for (int i = 0; i < P->nd; ++i) {
+        int s = P->id[i];
+        const double *m = P->d + 3 * i;
+        double dot = Rx[s] * m[0] + Ry[s] * m[1] + Rz[s] * m[2];
+        if (P->damp_type == 1) {
+            double dm = 1.0 - exp(-P->ddamp * R2[s]);
+            vind += -dot * Rm3[s] * dm * dm;
+        } else {
+            double rm3 = Rm3[s];
+            if (R[s] < P->ddamp) {
+                double re = reff(R[s], P->ddamp);
+                rm3 = 1.0 / (re * re * re);
+            }
+            vind += -dot * rm3;
+        }
+    }
+    /* repulsive cores, src/Potential.cpp:1282-1315 */
+    double vrep = 0;
+    for (int i = 0; i < P->nrep; ++i) {
+        int s = P->irep[i];
+        if (P->rep_type == 1) vrep += P->rep[2 * i + 1] * exp(-P->rep[2 * i] * R2[s]);
+        else if (P->rep_type == 2) vrep += P->rep[2 * i + 1] * exp(-P->rep[2 * i] * R[s]);
+    }
+    /* polarization, src/Potential.cpp:1331-1374 and :875-887 */
+    double vpol = 0;
+    if (P->pol_type == 1) {
+        double sp = 0;
+        for (int i = 0; i < P->npps; ++i) {
+            int s = P->ipps[i];
+            double rsq = R2[s];
+            if (P->damp_type == 1) {
+                double dm = 1.0 - exp(-P->pdamp * rsq);
+                sp += P->alpha * dm * dm / (rsq * rsq);
+            } else {
+                if (R[s] < P->pdamp) {
+                    double re = reff(R[s], P->pdamp);
+                    double re2 = re * re;
+                    sp += P->alpha * rsq / (re2 * re2 * re2);
+                } else sp += P->alpha / (rsq * rsq);
+            }
+        }
+        if (P->adiabatic) vpol = P->eps - sqrt(P->eps * (P->eps + sp));
+        else vpol = -0.5 * sp;
+    }
+    /* ions (Bloomfield erf-damped charge, src/NaCl.cpp:300-307) */
+    double vion = 0;
+    for (int i = 0; i < P->nion; ++i) {
+        double dx = r[0] - P->ion_xyz[3 * i], dy = r[1] - P->ion_xyz[3 * i + 1],
+               dz = r[2] - P->ion_xyz[3 * i + 2];
+        double rr = sqrt(dx * dx + dy * dy + dz * dz);
+        vion += -P->ion_q[i] * erf(rr / P->ion_r) / rr;
+    }
+    double vt = vpc + vind + vrep + vpol + vion;
+    if (parts) {
+        parts[0] = vpc + vion; parts[1] = vind; parts[2] = vrep; parts[3] = vpol; parts[4] = vt;
+    }
+    return vt;
+}
+
+static int pot_init(or_pot *P, const int *flags, const double *pp, int nsites,
+                    const double *sites, int nq, const double *q, const int *iq,
+                    int nd, const double *d, const int *id, int nion,
+                    const double *ion_xyz, const double *ion_q, double ion_r)
+{
+    memset(P, 0, sizeof(*P));
+    int rc = pot_setup(P, flags, pp, nsites);
+    if (rc) return rc;
+    P->ns = nsites; P->sites = sites;
+    P->nq = nq; P->q = q; P->iq = iq;
+    P->nd = nd; P->d = d; P->id = id;
+    P->nion = nion; P->ion_xyz = ion_xyz; P->ion_q = ion_q; P->ion_r = ion_r;
+    return 0;
+}
+
+int or_potential_points(const int *flags, const double *potpara, int nsites,
+                        const double *sites, int nq, const double *q, const int *iq,
+                        int nd, const double *d, const int *id, int nion,
+                        const double *ion_xyz, const double *ion_q, double ion_r,
+                        int npts, const double *pts, double *v, double *parts)
+{
+    or_pot P;
+    int rc = pot_init(&P, flags, potpara, nsites, sites, nq, q, iq, nd, d, id, nion,
+                      ion_xyz, ion_q, ion_r);
+    if (rc) { pot_free(&P); return rc; }
+#pragma omp parallel num_threads(nthreads())
+    {
+        double *buf = malloc(sizeof(double) * 6 * (nsites + 1));
+        double *R = buf, *R2 = buf + nsites, *Rx = buf + 2 * nsites, *Ry = buf + 3 * nsites,
+               *Rz = buf + 4 * nsites, *Rm3 = buf + 5 * nsites;
+#pragma omp for schedule(static)
+        for (int p = 0; p < npts; ++p)
+            v[p] = pot_eval(&P, pts + 3 * p, R, R2, Rx, Ry, Rz, Rm3, parts ? parts + 5 * p : NULL);
+        free(buf);
+    }
+    pot_free(&P);
+    return 0;
+}
+
+int or_potential_grid(const int *flags, const double *potpara, int nsites,
+                      const double *sites, int nq, const double *q, const int *iq,
+                      int nd, const double *d, const int *id, int nion,
+                      const double *ion_xyz, const double *ion_q, double ion_r,
+                      int nx, int ny, int nz, const double *xg, const double *yg,
+                      const double *zg, double *v)
+{
+    or_pot P;
+    int rc = pot_init(&P, flags, potpara, nsites, sites, nq, q, iq, nd, d, id, nion,
+                      ion_xyz, ion_q, ion_r);
+    if (rc) { pot_free(&P); return rc; }
+    const long ng = (long)nx * ny * nz;
+#pragma omp parallel num_threads(nthreads())
+    {
+        double *buf = malloc(sizeof(double) * 6 * (nsites + 1));
+        double *R = buf, *R2 = buf + nsites, *Rx = buf + 2 * nsites, *Ry = buf + 3 * nsites,
+               *Rz = buf + 4 * nsites, *Rm3 = buf + 5 * nsites;
+#pragma omp for schedule(static)
+        for (long g = 0; g < ng; ++g) {
+            int i = (int)(g % nx), j = (int)((g / nx) % ny), k = (int)(g / ((long)nx * ny));
+            double r[3] = {xg[i], yg[j], zg[k]};
+            v[g] = pot_eval(&P, r, R, R2, Rx, Ry, Rz, Rm3, NULL);
+        }
+        free(buf);
+    }
+    pot_free(&P);
+    return 0;
+}
+
+/* ================================================================ a9, a10 */
+static double dot(long n, const double *a, const double *b)
+{
+    double s = 0;
+#pragma omp parallel for reduction(+ : s) num_threads(nthreads()) schedule(static)
+    for (long i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+static double nrm2(long n, const double *a) { return sqrt(dot(n, a, a)); }
+static void scal(long n, double f, double *a)
+{
+#pragma omp parallel for num_threads(nthreads()) schedule(static)
+    for (long i = 0; i < n; ++i) a[i] *= f;
+}
+static void axpy(long n, double f, const double *x, double *y)
+{
+#pragma omp parallel for num_threads(nthreads()) schedule(static)
+    for (long i = 0; i < n; ++i) y[i] += f * x[i];
+}
+
+int or_syev(int n, const double *A, int lda, double *W, double *Vec, int ldv)
+{
+    /* cyclic Jacobi on a copy symmetrised from the upper triangle */
+    double *a = malloc(sizeof(double) * n * n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i <= j; ++i) {
+            double x = A[i + (long)lda * j];
+            a[i + n * j] = x;
+            a[j + n * i] = x;
+        }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) Vec[i + (long)ldv * j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, tot = 0;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                tot += a[i + n * j] * a[i + n * j];
+                if (i != j) off += a[i + n * j] * a[i + n * j];
+            }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = a[p + n * q];
+                if (apq == 0.0) continue;
+                double app = a[p + n * p], aqq = a[q + n * q];
+                double th = (aqq - app) / (2.0 * apq);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) { /* columns p,q */
+                    double akp = a[k + n * p], akq = a[k + n * q];
+                    a[k + n * p] = c * akp - s * akq;
+                    a[k + n * q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) { /* rows p,q */
+                    double apk = a[p + n * k], aqk = a[q + n * k];
+                    a[p + n * k] = c * apk - s * aqk;
+                    a[q + n * k] = s * apk + c * aqk;
+                }
+                a[p + n * q] = a[q + n * p] = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    double vp = Vec[k + (long)ldv * p], vq = Vec[k + (long)ldv * q];
+                    Vec[k + (long)ldv * p] = c * vp - s * vq;
+                    Vec[k + (long)ldv * q] = s * vp + c * vq;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) W[i] = a[i + n * i];
+    /* ascending order, eigenvectors follow */
+    for (int i = 0; i < n - 1; ++i) {
+        int m = i;
+        for (int j = i + 1; j < n; ++j)
+            if (W[j] < W[m]) m = j;
+        if (m != i) {
+            double t = W[i]; W[i] = W[m]; W[m] = t;
+            for (int k = 0; k < n; ++k) {
+                double x = Vec[k + (long)ldv * i];
+                Vec[k + (long)ldv * i] = Vec[k + (long)ldv * m];
+                Vec[k + (long)ldv * m] = x;
+            }
+        }
+    }
+    free(a);
+    return 0;
+}
+
+int or_davidson(int ndim, int maxsub, int nroots, int maxmacro, int ptol,
+                int corrflag, or_matvec_fn mv, void *ctx, const double *diag,
+                double *B, double *evals, int *n_mtx, int *n_macro, double *resid)
+{
+    const long N = ndim;
+    const double thresh = pow(10.0, -ptol);
+    double *Z = calloc(N * maxsub, sizeof(double));
+    double *S = calloc((long)maxsub * maxsub, sizeof(double));
+    double *Vv = calloc((long)maxsub * maxsub, sizeof(double));
+    double *sse = calloc(maxsub, sizeof(double));
+    double *ritz = calloc(N * nroots, sizeof(double));
+    double *res = calloc(N * nroots, sizeof(double));
+    double *jd = calloc(N, sizeof(double));
+    double *residuals = calloc(nroots, sizeof(double));
+    double *cw = calloc(nroots, sizeof(double));
+    int *ccode = calloc(nroots, sizeof(int)), *bidx = calloc(nroots, sizeof(int));
+    int nmtx = 0, nconv = 0, jmacro = 1, nsub = 0, nnewB = nroots, nnewZ = 0, done = 0;
+
+    /* GramSchmidt on the start vectors, src/Davidson.cpp:482-514 */
+    for (int i = 0; i < nroots; ++i) {
+        double *vi = B + N * i;
+        scal(N, 1.0 / nrm2(N, vi), vi);
+        for (int j = 0; j < i; ++j)
+            for (int r = 0; r < 2; ++r) {
+                axpy(N, -dot(N, vi, B + N * j), B + N * j, vi);
+                scal(N, 1.0 / nrm2(N, vi), vi);
+            }
+    }
+    while (!done) {
+        if (nnewB > 0) { /* return 1 of the reverse interface: Z = H B for new columns */
+            mv(B + N * nsub, Z + N * nsub, nnewB, ctx);
+            nmtx += nnewB;
+            nnewZ = nnewB;
+            nnewB = 0;
+        }
+        /* ComputeS: S = Z^T B over the whole subspace (src/Davidson.cpp:385-410) */
+        int m = nsub + nnewZ;
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i <= j; ++i) S[i + (long)maxsub * j] = dot(N, Z + N * i, B + N * j);
+        nsub = m;
+        or_syev(nsub, S, maxsub, sse, Vv, maxsub);
+        for (int r = 0; r < nroots; ++r) {
+            if (nconv > 0) {
+                double s = 0;
+                for (int k = 0; k < nconv; ++k) s += Vv[k + (long)maxsub * r] * Vv[k + (long)maxsub * r];
+                cw[r] = sqrt(s);
+            } else cw[r] = 0;
+        }
+        /* Ritz vectors u = B v, renormalised (src/Davidson.cpp:240-246) */
+        for (int r = 0; r < nroots; ++r) {
+            double *u = ritz + N * r;
+#pragma omp parallel for num_threads(nthreads()) schedule(static)
+            for (long g = 0; g < N; ++g) {
+                double s = 0;
+                for (int k = 0; k < nsub; ++k) s += B[g + N * k] * Vv[k + (long)maxsub * r];
+                u[g] = s;
+            }
+            scal(N, 1.0 / nrm2(N, u), u);
+        }
+        /* residuals r = Z v - lambda u (src/Davidson.cpp:251-269) */
+        for (int r = 0; r < nroots; ++r) {
+            double *rc = res + N * r, lam = sse[r];
+            const double *u = ritz + N * r;
+#pragma omp parallel for num_threads(nthreads()) schedule(static)
+            for (long g = 0; g < N; ++g) {
+                double s = 0;
+                for (int k = 0; k < nsub; ++k) s += Z[g + N * k] * Vv[k + (long)maxsub * r];
+                rc[g] = s - lam * u[g];
+            }
+            residuals[r] = nrm2(N, rc);
+        }
+        /* convergence bookkeeping (src/Davidson.cpp:272-297) */
+        int nnewconv = 0;
+        for (int r = 0; r < nroots; ++r) {
+            if (cw[r] > 0.95) ccode[r] = 0;
+            else if (residuals[r] < thresh) {
+                ccode[r] = 1;
+                bidx[r] = nconv;
+                memcpy(B + N * nconv, ritz + N * r, sizeof(double) * N);
+                ++nconv;
+                ++nnewconv;
+            } else ccode[r] = 2;
+        }
+        if (nconv == nroots) {
+            for (int r = 0; r < nroots; ++r) evals[r] = sse[r];
+            memcpy(B, ritz, sizeof(double) * N * nroots);
+            done = 1;
+            break;
+        }
+        /* restart (src/Davidson.cpp:303-346) */
+        if (nsub == maxsub || nnewconv > 0) {
+            nnewB = 0;
+            for (int r = 0; r < nroots; ++r)
+                if (ccode[r] == 2) {
+                    memcpy(B + N * (nnewB + nconv), ritz + N * r, sizeof(double) * N);
+                    bidx[r] = nconv + nnewB;
+                    ++nnewB;
+                }
+            for (int r = 0; r < nroots; ++r)
+                if (ccode[r] > 0) {
+                    double *dst = ritz + N * bidx[r];
+#pragma omp parallel for num_threads(nthreads()) schedule(static)
+                    for (long g = 0; g < N; ++g) {
+                        double s = 0;
+                        for (int k = 0; k < nsub; ++k) s += Z[g + N * k] * Vv[k + (long)maxsub * r];
+                        dst[g] = s;
+                    }
+                }
+            long off = (long)(nconv - nnewconv) * N;
+            memcpy(Z + off, ritz + off, sizeof(double) * N * (nnewB + nnewconv));
+            nsub = nroots;
+            jmacro += 1;
+            if (jmacro >= maxmacro) {
+                /* maxmacro exhausted: return without touching evals, as the
+                 * reference does (src/Davidson.cpp:332-341); B holds the
+                 * current Ritz vectors */
+                break;
+            }
+        }
+        /* correction vectors (src/Davidson.cpp:353-371) */
+        nnewB = 0;
+        for (int r = 0; r < nroots; ++r) {
+            if (nsub + nnewB == maxsub) break;
+            if (ccode[r] < 2) continue;
+            double *rc = res + N * r, lam = sse[r];
+            if (corrflag == 1) {
+                for (long g = 0; g < N; ++g) {
+                    double dv = lam - diag[g];
+                    if (fabs(dv) < 1e-12) dv = 1e-12;
+                    rc[g] /= dv;
+                }
+            } else if (corrflag == 2) {
+                const double *u = ritz + N * r;
+                for (long g = 0; g < N; ++g) {
+                    double mk = diag[g] - lam;
+                    if (fabs(mk) < 1e-12) mk = 1e-12;
+                    mk = 1.0 / mk;
+                    rc[g] *= -mk;
+                    jd[g] = mk * u[g];
+                }
+                double f = dot(N, u, rc) / dot(N, u, jd);
+                axpy(N, f, jd, rc);
+            }
+            /* OrthoVecOnB (src/Davidson.cpp:447-474) */
+            int nb = nsub + nnewB;
+            if (nb > 0) {
+                scal(N, 1.0 / nrm2(N, rc), rc);
+                for (int i = 0; i < nb; ++i)
+                    for (int t = 0; t < 2; ++t) {
+                        axpy(N, -dot(N, rc, B + N * i), B + N * i, rc);
+                        scal(N, 1.0 / nrm2(N, rc), rc);
+                    }
+            }
+            memcpy(B + N * (nsub + nnewB), rc, sizeof(double) * N);
+            ++nnewB;
+        }
+    }
+    if (n_mtx) *n_mtx = nmtx;
+    if (n_macro) *n_macro = jmacro;
+    if (resid) memcpy(resid, residuals, sizeof(double) * nroots);
+    free(Z); free(S); free(Vv); free(sse); free(ritz); free(res); free(jd);
+    free(residuals); free(cw); free(ccode); free(bidx);
+    return nconv;
+}
+
+typedef struct {
+    int nx, ny, nz;
+    const double *Tx, *Ty, *Tz, *V;
+} grid_ctx;
+
+static void grid_mv(const double *x, double *y, int nvec, void *c)
+{
+    grid_ctx *g = (grid_ctx *)c;
+    or_apply_h(g->nx, g->ny, g->nz, g->Tx, g->Ty, g->Tz, g->V, x, y, nvec);
+}
+
+int or_solve_grid(int nx, int ny, int nz, const double *Tx, const double *Ty,
+                  const double *Tz, const double *V, int nroots, int maxsub,
+                  int maxmacro, int ptol, int corrflag, double *B, double *evals,
+                  int *n_mtx, int *n_macro, double *resid)
+{
+    long N = (long)nx * ny * nz;
+    double *D = malloc(sizeof(double) * N);
+    or_diag(nx, ny, nz, Tx, Ty, Tz, V, D);
+    grid_ctx g = {nx, ny, nz, Tx, Ty, Tz, V};
+    int rc = or_davidson((int)N, maxsub, nroots, maxmacro, ptol, corrflag, grid_mv, &g,
+                         D, B, evals, n_mtx, n_macro, resid);
+    free(D);
+    return rc;
+}

@@ -1,0 +1,133 @@
+/*
+ * pisces_oracle.h -- CPU restatement of the PISCES excess-electron eigensolve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: the parity tests, the
+ * smoke check and the CPU-baseline leg of the benchmark call it; the product
+ * (the HIP library under pisces-1_amd/) never links or calls it.
+ *
+ * Every function restates the reference algorithm named in its comment (file:line
+ * in the reference tree).  Plain C99 + OpenMP, FP64 throughout.
+ *
+ * Parity status of each function (what pins it) is listed in oracle/README.md.
+ */
+#ifndef PISCES_ORACLE_H
+#define PISCES_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- a1, a2 */
+/* sine-DVR grid and kinetic matrix (full, symmetric, row-major n*n).
+ * Restates src/sine_dvr.cpp:45-131 (ldt > 0 branch, which fills the full
+ * matrix); the grid is called with x0 = -L/2, xN+1 = L/2, mass 1 by
+ * src/DVR.cpp:602-605. */
+void or_sine_dvr(int n, double mass, double x0, double xnp1, double *x, double *T);
+/* Same matrix in the reference's packed layout (ldt == 0: a*(a+1)/2 + b, b <= a),
+ * src/sine_dvr.cpp:84,106. */
+void or_sine_dvr_packed(int n, double mass, double x0, double xnp1, double *x, double *Tp);
+
+/* ---------------------------------------------------------------- a3, a4 */
+/* Y[:,v] = V o X[:,v] + (Tx (x) I (x) I + I (x) Ty (x) I + I (x) I (x) Tz) X[:,v]
+ * for v < nvec; vectors are columns of length nx*ny*nz, x fastest.
+ * Restates DVR::MatrixTimesVector, src/mtx.cpp:22-108 (V first, then the x, y
+ * and z sweeps, each accumulating into y). */
+void or_apply_h(int nx, int ny, int nz, const double *Tx, const double *Ty,
+                const double *Tz, const double *V, const double *X, double *Y,
+                int nvec);
+/* diag(H) = V + Tx_ii + Ty_jj + Tz_kk; src/DVR.cpp:1808-1881. */
+void or_diag(int nx, int ny, int nz, const double *Tx, const double *Ty,
+             const double *Tz, const double *V, double *D);
+/* Half-box particle-in-a-box start vector; src/DVR.cpp:1725-1805. */
+void or_piab_start(int nx, int ny, int nz, const double *xg, const double *yg,
+                   const double *zg, double *wf);
+
+/* ---------------------------------------------------------------- a8 */
+/* Rigid DPP waters from O,H1,H2 Cartesian input (Angstrom, 9 doubles/water).
+ * Projection onto the rigid monomer (Water::Reposition, src/Water.cpp:695-825)
+ * then rebuild from centre + Euler angles (Water::UpdatePositions/ApplyEuler,
+ * src/Water.cpp:837-943).  Output: 4 sites per water (O,H1,H2,M) in bohr
+ * (12 doubles/water) and the configuration (x,y,z,phi,theta,psi).
+ * Returns 0, or -1 if a projection produced a non-finite value. */
+int or_water_sites(int nw, const double *xyz_ang, double *sites, double *conf);
+
+/* Self-consistent induced dipoles on the polarizable sites (O,H1,H2 of each
+ * water): WaterCluster::CalcInducedDipoles, src/DPP.cpp:760-1065.
+ * sites: 12*nw bohr (from or_water_sites).  Outputs:
+ *   mu[9*nw]  induced dipoles (3 sites x 3 components per water)
+ *   epc[9*nw] field of the permanent charges at those sites
+ *             (ReportPolSitesAndField, src/DPP.cpp:540-642)
+ *   *e_pol    water-water polarization energy (CalcIntermolecularPotential,
+ *             src/DPP.cpp:1648-1700 with WaterWaterPolarizationEnergy :1935-2024)
+ *   *iters    SCF iterations used.  Returns 0 if converged, 1 if not. */
+int or_water_induced(int nw, const double *sites, double *mu, double *epc,
+                     double *e_pol, int *iters);
+
+/* ---------------------------------------------------------------- a5, a6 */
+/* The electron-water potential at npts points, for the site/charge/dipole/
+ * polarizable-site lists that ClusterAnion hands to Potential::Setup
+ * (src/ClusterAnion.cpp:186-246).  Restates Potential::SetupDPPGTOP
+ * (src/Potential.cpp:234-593), Set4STORepCore/Set6GTORepCore/Set12GTORepCore
+ * (:598-728), Evaluate (:128-182) and EvaluateDPPGTOP with its terms
+ * (:844-956, :1169-1374).  PolType 1 (and its adiabatic form) and PolType 0
+ * are covered; other PolTypes return -2.
+ *   flags[6]   = Potential, DampType, SigmaOFlag, Polarization, AdiabaticPolPot, InternalParam
+ *   potpara[11]= PotPara[0..10] as in src/GetInput.cpp:53-61
+ *   nion ions with charge ion_q at ion_xyz add -q erf(R/r_ion)/R (Bloomfield,
+ *   src/NaCl.cpp:285-318) -- an extension, no reference combines it with waters.
+ *   parts (optional, 5*npts): Vpc, Vind, Vrep, Vpol, Vtotal per point. */
+int or_potential_points(const int *flags, const double *potpara, int nsites,
+                        const double *sites, int nq, const double *q, const int *iq,
+                        int nd, const double *d, const int *id, int nion,
+                        const double *ion_xyz, const double *ion_q, double ion_r,
+                        int npts, const double *pts, double *v, double *parts);
+
+/* Whole grid, x fastest: points x_i, y_j, z_k from the 1-D grids. */
+int or_potential_grid(const int *flags, const double *potpara, int nsites,
+                      const double *sites, int nq, const double *q, const int *iq,
+                      int nd, const double *d, const int *id, int nion,
+                      const double *ion_xyz, const double *ion_q, double ion_r,
+                      int nx, int ny, int nz, const double *xg, const double *yg,
+                      const double *zg, double *v);
+
+/* Lists exactly as ClusterAnion::EnergyFromConfiguration builds them
+ * (GetLists, src/DPP.cpp:402-498; ReportInducedDipoles :505-529):
+ *   q[3*nw], iq[3*nw]  charges on H1,H2,M;  d[9*nw], id[3*nw] induced dipoles
+ *   on O,H1,H2.  Returns the number of charges (3*nw). */
+int or_water_lists(int nw, const double *sites, const double *mu, double *q,
+                   int *iq, double *d, int *id);
+
+/* ---------------------------------------------------------------- a9, a10 */
+typedef void (*or_matvec_fn)(const double *x, double *y, int nvec, void *ctx);
+
+/* Reverse-communication Davidson restated as a driver loop:
+ * Davidson(), src/Davidson.cpp:61-376 with ComputeS :385-410, OrthoVecOnB
+ * :447-474, GramSchmidt :482-514, correction vectors :530-566; driver loop of
+ * DVR::davdriver, src/davdriver.cpp:28-96.
+ * B holds ndim*maxsub doubles, start vectors in the first nroots columns; on
+ * return its first nroots columns hold the Ritz vectors.  evals[nroots].
+ * Returns the number of converged roots; *n_mtx counts H*x products and
+ * *n_macro the macro iterations. */
+int or_davidson(int ndim, int maxsub, int nroots, int maxmacro, int ptol,
+                int corrflag, or_matvec_fn mv, void *ctx, const double *diag,
+                double *B, double *evals, int *n_mtx, int *n_macro,
+                double *resid);
+
+/* Convenience: Davidson on the sine-DVR Hamiltonian of a grid. */
+int or_solve_grid(int nx, int ny, int nz, const double *Tx, const double *Ty,
+                  const double *Tz, const double *V, int nroots, int maxsub,
+                  int maxmacro, int ptol, int corrflag, double *B, double *evals,
+                  int *n_mtx, int *n_macro, double *resid);
+
+/* Symmetric eigensolver (cyclic Jacobi), ascending eigenvalues; A is n*n
+ * column-major (read: upper triangle), W[n], Vec n*n column-major.  Stands in
+ * for LAPACK dsyev at src/Davidson.cpp:224. */
+int or_syev(int n, const double *A, int lda, double *W, double *Vec, int ldv);
+
+/* Threads used by the OpenMP loops (0 = library default). */
+void or_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

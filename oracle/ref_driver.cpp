@@ -1,0 +1,195 @@
+// ref_driver.cpp -- harness that runs the reference PISCES functions that build
+// in this image (src/sine_dvr.cpp, src/Davidson.cpp, src/Potential.cpp with
+// src/MolPolAux.cpp and src/DerivDDTensor.cpp) on inputs written by
+// tests/golden/make_golden.py, so that the CPU restatement in pisces_oracle.c
+// can be checked against the reference itself.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile.ref into oracle/_ref/;
+// the reference sources are compiled where they lie and never copied.
+//
+// File format (little endian): a sequence of records, each an int64 element
+// count followed by that many float64 values (integers are stored as float64).
+//
+// Commands:
+//   ref_driver tkin      <in> <out>   in: [n, L]            out: [x(n)], [Tpacked]
+//   ref_driver potential <in> <out>   in: flags, potpara, sites, q, iq, d, id,
+//                                         alphas, ips, epc, [vpolww], pts
+//                                     out: 5 energies per point
+//   ref_driver davidson  <in> <out>   in: [nx,ny,nz,Lx,Ly,Lz], V, [nroots,maxsub,
+//                                         maxmacro,ptol,corrflag], start vectors
+//                                     out: [nconv,n_mtx], evals, vectors
+#include <mpi.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "vecdefs.h"
+#include "Potential.h"
+#include "sine_dvr.h"
+#include "Davidson.h"
+
+typedef std::vector<double> rec;
+
+static std::vector<rec> read_records(const char *path)
+{
+    std::vector<rec> out;
+    FILE *f = fopen(path, "rb");
+    if (!f) { perror(path); exit(2); }
+    int64_t n;
+    while (fread(&n, sizeof(n), 1, f) == 1) {
+        rec r(n);
+        if (n && fread(r.data(), sizeof(double), n, f) != (size_t)n) { fprintf(stderr, "short read\n"); exit(2); }
+        out.push_back(r);
+    }
+    fclose(f);
+    return out;
+}
+
+static void write_records(const char *path, const std::vector<rec> &rs)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) { perror(path); exit(2); }
+    for (const rec &r : rs) {
+        int64_t n = (int64_t)r.size();
+        fwrite(&n, sizeof(n), 1, f);
+        if (n) fwrite(r.data(), sizeof(double), n, f);
+    }
+    fclose(f);
+}
+
+static std::vector<int> as_int(const rec &r)
+{
+    std::vector<int> v(r.size());
+    for (size_t i = 0; i < r.size(); ++i) v[i] = (int)llround(r[i]);
+    return v;
+}
+
+static int cmd_tkin(const char *in, const char *out)
+{
+    std::vector<rec> r = read_records(in);
+    int n = (int)r[0][0];
+    double L = r[0][1];
+    rec x(n), t(n * (n + 1) / 2);
+    sine_dvr(n, 1.0, -0.5 * L, 0.5 * L, x.data(), t.data(), 0, 0);
+    write_records(out, {x, t});
+    return 0;
+}
+
+static int cmd_potential(const char *in, const char *out)
+{
+    std::vector<rec> r = read_records(in);
+    std::vector<int> flags = as_int(r[0]);
+    const rec &potpara = r[1], &sites = r[2], &q = r[3];
+    std::vector<int> iq = as_int(r[4]);
+    const rec &d = r[5];
+    std::vector<int> id = as_int(r[6]);
+    const rec &alphas = r[7];
+    std::vector<int> ips = as_int(r[8]);
+    const rec &epc = r[9];
+    double vpolww = r[10].empty() ? 0.0 : r[10][0];
+    const rec &pts = r[11];
+    iVec pf(flags.begin(), flags.end());
+    rec dmu(3 * (sites.size() / 3) * 3 * (d.size() / 3) + 1, 0.0);
+    rec pp(potpara);
+    pp.resize(32, 0.0);
+    Potential V;
+    V.VpolWaterWater = vpolww;
+    V.Setup(pf, (int)sites.size() / 3, sites.data(), (int)q.size(), q.data(), iq.data(),
+            (int)d.size() / 3, d.data(), id.data(), (int)alphas.size(), alphas.data(),
+            ips.data(), epc.data(), dmu.data(), pp.data());
+    size_t npts = pts.size() / 3;
+    rec e(5 * npts);
+    for (size_t p = 0; p < npts; ++p) {
+        V.Evaluate(&pts[3 * p]);
+        V.ReportEnergies(5, &e[5 * p]);
+    }
+    write_records(out, {e});
+    return 0;
+}
+
+// H*x with full T matrices (harness code; the reference's own
+// DVR::MatrixTimesVector lives in a translation unit that needs FFTW).
+static void hx(int n[3], double *T[3], const double *V, const double *x, double *y)
+{
+    long ng = (long)n[0] * n[1] * n[2];
+    for (long g = 0; g < ng; ++g) y[g] = V[g] * x[g];
+    long st[3] = {1, n[0], (long)n[0] * n[1]};
+    for (int ax = 0; ax < 3; ++ax) {
+        int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+        for (int b = 0; b < n[o2]; ++b)
+            for (int a = 0; a < n[o1]; ++a) {
+                long base = a * st[o1] + b * st[o2];
+                for (int i = 0; i < n[ax]; ++i) {
+                    double s = 0;
+                    for (int j = 0; j < n[ax]; ++j) s += T[ax][i + n[ax] * j] * x[base + j * st[ax]];
+                    y[base + i * st[ax]] += s;
+                }
+            }
+    }
+}
+
+static int cmd_davidson(const char *in, const char *out)
+{
+    std::vector<rec> r = read_records(in);
+    int n[3] = {(int)r[0][0], (int)r[0][1], (int)r[0][2]};
+    double L[3] = {r[0][3], r[0][4], r[0][5]};
+    const rec &V = r[1];
+    std::vector<int> p = as_int(r[2]);
+    int nroots = p[0], maxsub = p[1], maxmacro = p[2], ptol = p[3], corrflag = p[4];
+    const rec &start = r[3];
+    long ng = (long)n[0] * n[1] * n[2];
+    double *T[3];
+    rec Tm[3], xg[3];
+    for (int a = 0; a < 3; ++a) {
+        Tm[a].resize((size_t)n[a] * n[a]);
+        xg[a].resize(n[a]);
+        sine_dvr(n[a], 1.0, -0.5 * L[a], 0.5 * L[a], xg[a].data(), Tm[a].data(), n[a], 0);
+        T[a] = Tm[a].data();
+    }
+    rec diag(ng);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                long g = i + (long)n[0] * (j + (long)n[1] * k);
+                diag[g] = V[g] + T[0][i * (n[0] + 1)] + T[1][j * (n[1] + 1)] + T[2][k * (n[2] + 1)];
+            }
+    rec B(ng * maxsub, 0.0), Z(ng * maxsub, 0.0);
+    std::copy(start.begin(), start.begin() + ng * nroots, B.begin());
+    rec work(DavidsonWorkSize((int)ng, maxsub, nroots, corrflag));
+    rec ev(nroots, 0.0);
+    int inout[3] = {0, 0, 0}, nconv = 0, nmtx = 0, ido = 1;
+    // driver loop in the manner of DVR::davdriver (src/davdriver.cpp:58-86)
+    while (ido != 0) {
+        ido = Davidson((int)ng, maxsub, nroots, maxmacro, ptol, corrflag, 0, ev.data(), nconv,
+                       B.data(), Z.data(), diag.data(), work.data(), inout);
+        if (ido == 1) {
+            for (int v = 0; v < inout[2]; ++v) {
+                hx(n, T, V.data(), &B[(inout[0] + v) * ng], &Z[(inout[1] + v) * ng]);
+                ++nmtx;
+            }
+        }
+    }
+    rec meta = {(double)nconv, (double)nmtx};
+    rec vecs(B.begin(), B.begin() + ng * nroots);
+    write_records(out, {meta, ev, vecs});
+    return 0;
+}
+
+int main(int argc, char **argv)
+{
+    MPI_Init(&argc, &argv);
+    int rc = 1;
+    if (argc == 4) {
+        if (!strcmp(argv[1], "tkin")) rc = cmd_tkin(argv[2], argv[3]);
+        else if (!strcmp(argv[1], "potential")) rc = cmd_potential(argv[2], argv[3]);
+        else if (!strcmp(argv[1], "davidson")) rc = cmd_davidson(argv[2], argv[3]);
+    } else {
+        fprintf(stderr, "usage: ref_driver {tkin|potential|davidson} <in> <out>\n");
+    }
+    MPI_Finalize();
+    return rc;
+}

@@ -1,0 +1,132 @@
+"""Write the golden fixtures of tests/golden/ by running the reference.
+
+Runs oracle/_ref/ref_driver (built by oracle/Makefile.ref from the reference
+sources in /root/reference) on seeded inputs and stores inputs and reference
+outputs as small .npz files.  Run in the build container:
+
+    make -C oracle -f Makefile.ref && python tests/golden/make_golden.py
+
+The fixtures are data only (inputs and the reference's outputs).  The site,
+charge and dipole lists fed to the reference Potential come from the oracle's
+water model (the reference's Water.cpp does not compile in this image).
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "pisces-1_amd"))
+import oracle  # noqa: E402
+import workload  # noqa: E402
+
+ANGS2BOHR = workload.ANGS2BOHR
+
+
+def run_ref(cmd, recs):
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        oracle.write_records(fin, recs)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([oracle.REF_DRIVER, cmd, fin, fout], check=True, env=env,
+                       stdout=subprocess.DEVNULL)
+        return oracle.read_records(fout)
+
+
+def golden_tkin():
+    cases = [(8, 10.0), (16, 20.0), (32, 36.0), (33, 36.0), (64, 40.0), (96, 48.0)]
+    out = {}
+    for i, (n, L) in enumerate(cases):
+        x, T = run_ref("tkin", [[n, L]])
+        out["n%d" % i] = np.array([n, L])
+        out["x%d" % i] = x
+        out["T%d" % i] = T
+    np.savez_compressed(os.path.join(HERE, "tkin.npz"), ncases=len(cases), **out)
+
+
+def sample_points(sites, rng, box, nbox=400, nnear=40):
+    pts = [rng.uniform(-box / 2, box / 2, size=(nbox, 3))]
+    for s in sites:  # points near every site, 0.05 .. 4 bohr away (damping branches)
+        d = rng.normal(size=(nnear // 8, 3))
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        r = rng.uniform(0.05, 4.0, size=(nnear // 8, 1))
+        pts.append(s + d * r)
+    return np.concatenate(pts)
+
+
+FLAG_SETS = {
+    "default": ((3, 2, 0, 1, 0, 0), None),
+    "adiabatic": ((3, 2, 0, 1, 1, 0), None),
+    "gaussdamp": ((3, 1, 0, 1, 0, 0), None),
+    "gto6": ((1, 2, 0, 1, 0, 0), None),
+    "gto12": ((2, 2, 0, 1, 0, 0), None),
+    "gto12_sigo": ((2, 2, 1, 1, 0, 0), {5: 3.9}),
+    "norep": ((4, 2, 0, 1, 0, 0), None),
+    "nopol": ((3, 2, 0, 0, 0, 0), None),
+    "internal1": ((3, 2, 0, 1, 0, 1), {1: 5.011, 4: 5.011, 2: 6.6396}),
+}
+
+
+def golden_potential(tag, nw, radius, seed, box):
+    xyz, _ = workload.water_cluster(nw, radius, seed)
+    cl = oracle.Cluster(xyz)
+    rng = np.random.default_rng(seed + 7)
+    pts = sample_points(cl.sites, rng, box)
+    nw3 = 3 * nw
+    alphas = np.tile([1.22 * ANGS2BOHR ** 3, 0.28 * ANGS2BOHR ** 3, 0.28 * ANGS2BOHR ** 3], nw)
+    ips = np.array([4 * w + s for w in range(nw) for s in range(3)], dtype=np.float64)
+    out = dict(xyz=xyz, pts=pts, sites=cl.sites, mu=cl.mu, epc=cl.epc, e_pol=cl.e_pol)
+    for name, (flags, over) in FLAG_SETS.items():
+        pp = oracle.default_potpara()
+        if over:
+            for k, v in over.items():
+                pp[k] = v
+        recs = [flags, pp, cl.sites, cl.q, cl.iq, cl.d, cl.id, alphas, ips,
+                cl.epc[:3 * nw3], [cl.e_pol], pts]
+        (e,) = run_ref("potential", recs)
+        out["flags_" + name] = np.array(flags)
+        out["potpara_" + name] = pp
+        out["E_" + name] = e.reshape(-1, 5)
+    np.savez_compressed(os.path.join(HERE, "potential_%s.npz" % tag), **out)
+
+
+def golden_davidson(tag, n, L, nw, radius, seed, params, store_vecs=True):
+    xyz, _ = workload.water_cluster(nw, radius, seed)
+    cl = oracle.Cluster(xyz)
+    g = oracle.Grid(n, L)
+    V = cl.potential_grid(g)
+    nroots, maxsub, maxmacro, ptol, corrflag = params
+    rng = np.random.default_rng(seed + 11)
+    start = np.zeros((nroots, g.ngp))
+    start[0] = g.piab()
+    for r in range(1, nroots):
+        start[r] = rng.uniform(-1, 1, size=g.ngp)
+    meta, ev, vecs = run_ref("davidson", [list(n) + list(L), V, list(params), start])
+    out = dict(n=np.array(n), L=np.array(L), xyz=xyz, params=np.array(params), start=start,
+               nconv=int(meta[0]), n_mtx=int(meta[1]), evals=ev)
+    if store_vecs:
+        out["vecs"] = vecs.reshape(nroots, -1)
+    np.savez_compressed(os.path.join(HERE, "davidson_%s.npz" % tag), **out)
+
+
+def main():
+    if not oracle.ref_available():
+        sys.exit("oracle/_ref/ref_driver is not built")
+    oracle.set_threads(1)
+    golden_tkin()
+    golden_potential("c1", 6, 3.5, 1000, 36.0)
+    golden_potential("c2", 24, 6.0, 2000, 40.0)
+    golden_davidson("g10_r1", (10, 10, 10), (20.0, 20.0, 20.0), 6, 3.5, 1000, (1, 20, 100, 9, 1))
+    golden_davidson("g12_r4", (12, 11, 10), (22.0, 21.0, 20.0), 6, 3.5, 1001, (4, 16, 100, 8, 1))
+    golden_davidson("g10_r3_small", (10, 10, 10), (20.0, 20.0, 20.0), 6, 3.5, 1002, (3, 6, 200, 8, 1))
+    golden_davidson("g10_r2_jd", (10, 10, 10), (20.0, 20.0, 20.0), 6, 3.5, 1003, (2, 12, 100, 8, 2))
+    golden_davidson("c1", (32, 32, 32), (36.0, 36.0, 36.0), 6, 3.5, 1000, (1, 20, 100, 9, 1))
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
