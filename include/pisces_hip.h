@@ -1,0 +1,219 @@
+/*
+ * pisces_hip.h -- C ABI of the MI355X (gfx950) implementation of the PISCES
+ * excess-electron eigensolve: sine-DVR H*X, the DPP electron-water potential on
+ * the grid, and the Davidson eigensolver with its block operations.
+ *
+ * Conventions shared by every entry point:
+ *   - Plain C types only; the caller owns host buffers, a context owns its
+ *     device buffers.  No global state: contexts are independent and may be
+ *     used from different threads concurrently (one thread per context).
+ *   - Grid vectors are x-fastest, index i + nx*(j + ny*k), exactly the layout of
+ *     DVR::wavefn / v_diag in the reference (src/mtx.cpp:15-19).  A block of
+ *     nvec vectors is nvec such columns back to back (the reference's B and Z,
+ *     src/davdriver.cpp:34-39).
+ *   - Lengths are bohr, energies hartree.
+ *   - Return codes: 0 ok; < 0 error (see pisces_strerror); > 0 "not converged"
+ *     where documented.  No entry point calls exit().
+ *   - Pointers named d_* are device pointers (hipMalloc'd or torch CUDA
+ *     tensors); all others are host pointers.
+ */
+#ifndef PISCES_HIP_H
+#define PISCES_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pisces_ctx pisces_ctx;
+
+/* precision of the H*X products inside a solve */
+enum {
+    PISCES_FP64 = 0,   /* plain form: FP64 everywhere                         */
+    PISCES_FP32 = 1,   /* FP32 MFMA contractions, FP64 block ops and residuals */
+    PISCES_BF16X2 = 2  /* split hi/lo BF16 MFMA contractions, FP32 accumulate  */
+};
+
+enum {
+    PISCES_OK = 0,
+    PISCES_ERR_ARG = -1,
+    PISCES_ERR_HIP = -2,
+    PISCES_ERR_NOMEM = -3,
+    PISCES_ERR_UNSUPPORTED = -4,
+    PISCES_ERR_STATE = -5,
+    PISCES_ERR_NUMERIC = -6
+};
+
+/* Version / build string of the library. */
+const char *pisces_version(void);
+const char *pisces_strerror(int code);
+/* Message of the last failing call on this thread (empty if none). */
+const char *pisces_last_error(void);
+/* Selects the H*X kernel family: 0 LDS-tiled VALU, 1 matrix cores (default).
+ * For cross-checks; no reference counterpart. */
+int pisces_debug_set_hx_variant(int variant);
+
+/* Create a context for one sine-DVR grid of n[0] x n[1] x n[2] points on boxes
+ * of length L[3] (bohr), centred at the origin; computes the 1-D grids and
+ * kinetic matrices (sine_dvr, src/sine_dvr.cpp:45-131) and uploads them.
+ * Replaces DVR::SetupDVR for DVRType 0 (src/DVR.cpp:64-161, :524-651).
+ * device: HIP device ordinal. */
+int pisces_create(pisces_ctx **ctx, const int n[3], const double L[3], int device);
+void pisces_destroy(pisces_ctx *ctx);
+
+/* Grid points (x[n0], y[n1], z[n2]) and kinetic matrices (n_a*n_a each,
+ * symmetric) of the context; either pointer may be NULL. */
+int pisces_grid(const pisces_ctx *ctx, double *xyz, double *T);
+
+/* Set the diagonal potential from a host array of ngp doubles (x fastest).
+ * Replaces DVR::ComputePotential's product (v_diag) when the caller builds the
+ * potential itself. */
+int pisces_set_potential(pisces_ctx *ctx, const double *v_diag);
+int pisces_get_potential(const pisces_ctx *ctx, double *v_diag);
+
+/* Build the electron-water potential on the grid on the GPU from the lists
+ * that ClusterAnion hands to Potential::Setup (src/ClusterAnion.cpp:186-246;
+ * Potential::Setup src/Potential.h:38-42) and evaluate it at every grid point
+ * (DVR::ComputePotential sampling 1, src/DVR.cpp:661-859, with
+ * Potential::Evaluate / EvaluateDPPGTOP, src/Potential.cpp:128-182, 844-956).
+ *   flags[6]  Potential, DampType, SigmaOFlag, Polarization, AdiabaticPolPot,
+ *             InternalParam (PotFlag of src/GetInput.cpp:43-48)
+ *   potpara   PotPara[0..10] (src/GetInput.cpp:53-61)
+ *   sites     3*nsites, 4 per water (O,H1,H2,M)
+ *   q/iq      charges and their site indices; d/id dipoles and site indices
+ *   ion_*     optional erf-damped point charges (Bloomfield form,
+ *             src/NaCl.cpp:285-318); nion may be 0.
+ * Supported: Potential 1-4, DampType 1-2, Polarization 0-1 (and adiabatic). */
+int pisces_build_potential(pisces_ctx *ctx, const int flags[6], const double *potpara,
+                           int nsites, const double *sites, int nq, const double *q,
+                           const int *iq, int nd, const double *d, const int *id,
+                           int nion, const double *ion_xyz, const double *ion_q,
+                           double ion_r);
+
+/* Water model on the GPU: rigid DPP monomers from O,H1,H2 Cartesian input in
+ * Angstrom (Water::Reposition + UpdatePositions, src/Water.cpp:695-943), the
+ * FP64 self-consistent induced dipoles (WaterCluster::CalcInducedDipoles,
+ * src/DPP.cpp:760-1065), and the lists above (GetLists :402-498,
+ * ReportInducedDipoles :505-529), then the grid potential.
+ * Outputs (optional, may be NULL): sites[12*nw], mu[9*nw].
+ * Returns > 0 if the dipole SCF did not converge (the potential is still built,
+ * as in the reference, which only warns). */
+int pisces_build_cluster_potential(pisces_ctx *ctx, int nw, const double *xyz_ang,
+                                   const int flags[6], const double *potpara, int nion,
+                                   const double *ion_xyz_bohr, const double *ion_q,
+                                   double ion_r, double *sites, double *mu);
+
+/* Y[:,v] = H X[:,v] for v < nvec, host buffers (ngp*nvec doubles each).
+ * Replaces DVR::MatrixTimesVector (src/mtx.cpp:22-108) called once per vector
+ * by davdriver (src/davdriver.cpp:67-78); the context's potential is used. */
+int pisces_apply_h(pisces_ctx *ctx, const double *X, double *Y, int nvec);
+
+/* Same on device buffers: d_X, d_Y hold nvec vectors with leading dimension
+ * ngp, in the given precision (FP64: double; FP32 and BF16X2: float). */
+int pisces_apply_h_device(pisces_ctx *ctx, const void *d_X, void *d_Y, int nvec,
+                          int precision);
+
+/* diag(H) on the host, ngp doubles.  Replaces DVR::ComputeDiagonal
+ * (src/DVR.cpp:1808-1881). */
+int pisces_diag(pisces_ctx *ctx, double *diag);
+
+/* Half-box particle-in-a-box start vector (DVR::ParticleInAnDBoxWf,
+ * src/DVR.cpp:1725-1805), ngp doubles on the host. */
+int pisces_piab(pisces_ctx *ctx, double *wf);
+
+typedef struct {
+    int nconv;        /* converged roots                                      */
+    int n_matvec;     /* H*x products (vectors) in the final precision         */
+    int n_matvec_lo;  /* H*x products in the low precision (mixed forms)       */
+    int n_iter;       /* Davidson iterations (subspace expansions)             */
+    int n_macro;      /* macro iterations (restarts + 1)                      */
+    double seconds;   /* wall time of the solve                                */
+    double resid[64]; /* final FP64 residual norms ||H u - lambda u||, per root */
+} pisces_stats;
+
+/* Davidson eigensolve on the GPU with the context's potential.
+ * Replaces DVR::Diagonalize case 2 -> davdriver -> Davidson (src/DVR.cpp:
+ * 490-504, src/davdriver.cpp:28-96, src/Davidson.cpp:61-376):
+ *   nroots    eigenpairs wanted (nStates)
+ *   maxsub    maximal subspace size (maxSubspace)
+ *   maxmacro  maximal macro iterations (maxIter)
+ *   ptol      residual threshold 10^-ptol (pTol)
+ *   corrflag  0 residual, 1 Davidson, 2 zeroth-order Jacobi-Davidson correction
+ *   vecs      in: start vectors (nroots*ngp); out: Ritz vectors
+ *   evals     out: eigenvalues (nroots)
+ *   precision PISCES_FP64 follows the reference algorithm step by step;
+ *             PISCES_FP32/BF16X2 run the low-precision contractions with FP64
+ *             defect correction (FP64 residuals every refresh_every iterations
+ *             and at convergence).
+ * Returns the number of converged roots (>= 0) or an error code. */
+int pisces_solve(pisces_ctx *ctx, int nroots, int maxsub, int maxmacro, int ptol,
+                 int corrflag, int precision, double *vecs, double *evals,
+                 pisces_stats *stats);
+
+/* Reverse-communication Davidson with the reference's exact calling
+ * convention and semantics (src/Davidson.h:2-15): return 1 asks the caller to
+ * set Z[inout[1]*ndim ..] = H B[inout[0]*ndim ..] for inout[2] vectors, return
+ * 0 means done (converged vectors in B[0..nroots), eigenvalues in evals).
+ * B, Z, diag and davwork are host arrays owned by the caller (davwork of
+ * pisces_davidson_work_size doubles); the block operations run on the GPU of
+ * ctx.  Unlike the reference it keeps its state in ctx (re-entrant). */
+int pisces_davidson_work_size(int ndim, int maxsub, int nroots, int corrflag);
+int pisces_davidson(pisces_ctx *ctx, int ndim, int maxsub, int nroots, int maxmacro,
+                    int tol, int corrflag, int verbose, double *evals, int *nConv,
+                    double *B, double *Z, const double *diag, double *davwork,
+                    int *inout);
+
+/* ---------------------------------------------------------------------------
+ * Batched / served solver (no reference counterpart: the reference solves one
+ * cluster at a time).  Many independent instances of (possibly) different grid
+ * sizes are advanced together, one set of batched kernels per Davidson step,
+ * with per-instance convergence masks.
+ * ------------------------------------------------------------------------- */
+typedef struct pisces_server pisces_server;
+
+typedef struct {
+    int n[3];            /* grid points                                   */
+    double L[3];         /* box lengths (bohr)                            */
+    int nw;              /* waters                                        */
+    const double *xyz;   /* 9*nw, Angstrom (O,H1,H2)                      */
+    int nion;            /* optional ions (bohr positions, charges)       */
+    const double *ion_xyz;
+    const double *ion_q;
+    int nroots;          /* states wanted                                 */
+    int warm;            /* 1: start from the chain's previous subspace   */
+    long long chain;     /* chain id for warm starts (-1: none)           */
+} pisces_request;
+
+typedef struct {
+    long long id;        /* request id returned by pisces_server_submit   */
+    int status;          /* nconv >= 0, or an error code                  */
+    int nroots;
+    double evals[8];
+    double resid[8];
+    int n_iter;
+    int n_matvec;
+    double latency_s;    /* submit -> completion                          */
+} pisces_result;
+
+/* hbm_cap_bytes: admission cap on device memory used by live instances;
+ * max_slots: instances advanced together; precision as for pisces_solve. */
+int pisces_server_create(pisces_server **srv, int device, size_t hbm_cap_bytes,
+                         int max_slots, int precision, int maxsub, int ptol);
+void pisces_server_destroy(pisces_server *srv);
+/* Queue a request; returns its id (>= 0) or an error code. */
+long long pisces_server_submit(pisces_server *srv, const pisces_request *req);
+/* Advance every live instance by one Davidson step, admitting queued requests
+ * as slots and memory free up.  Returns the number of live instances. */
+int pisces_server_step(pisces_server *srv);
+/* Copy out up to max finished results; returns how many were copied. */
+int pisces_server_poll(pisces_server *srv, pisces_result *out, int max);
+/* Device bytes an instance of this request would hold (the admission cost). */
+size_t pisces_server_footprint(const pisces_server *srv, const pisces_request *req);
+/* Fetch the Ritz vectors of a finished request (nroots*ngp doubles). */
+int pisces_server_vectors(pisces_server *srv, long long id, double *vecs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -794,9 +794,13 @@ int or_davidson(int ndim, int maxsub, int nroots, int maxmacro, int ptol,
     double *S = calloc((long)maxsub * maxsub, sizeof(double));
     double *Vv = calloc((long)maxsub * maxsub, sizeof(double));
     double *sse = calloc(maxsub, sizeof(double));
-    double *ritz = calloc(N * nroots, sizeof(double));
-    double *res = calloc(N * nroots, sizeof(double));
-    double *jd = calloc(N, sizeof(double));
+    /* Ritz vectors, residual vectors and the Jacobi-Davidson vector are one
+     * contiguous block, as in the reference's davwork (src/Davidson.cpp:181-187):
+     * the reference's bookkeeping can index past the Ritz block into the
+     * residual block, and this layout reproduces what it then computes. */
+    double *ritz = calloc(N * (2 * nroots + 1), sizeof(double));
+    double *res = ritz + N * nroots;
+    double *jd = res + N * nroots;
     double *residuals = calloc(nroots, sizeof(double));
     double *cw = calloc(nroots, sizeof(double));
     int *ccode = calloc(nroots, sizeof(int)), *bidx = calloc(nroots, sizeof(int));
@@ -944,7 +948,7 @@ int or_davidson(int ndim, int maxsub, int nroots, int maxmacro, int ptol,
     if (n_mtx) *n_mtx = nmtx;
     if (n_macro) *n_macro = jmacro;
     if (resid) memcpy(resid, residuals, sizeof(double) * nroots);
-    free(Z); free(S); free(Vv); free(sse); free(ritz); free(res); free(jd);
+    free(Z); free(S); free(Vv); free(sse); free(ritz);
     free(residuals); free(cw); free(ccode); free(bidx);
     return nconv;
 }

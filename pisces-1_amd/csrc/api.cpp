@@ -1,0 +1,377 @@
+// api.cpp -- the C ABI (include/pisces_hip.h): contexts, H*X, potential builds,
+// the water model and the single-instance solves.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+
+using namespace pisces;
+
+namespace pisces {
+const char *last_error();
+}
+
+// ------------------------------------------------------------------ context
+pisces_ctx::~pisces_ctx()
+{
+    if (device >= 0) static_cast<void>(hipSetDevice(device));
+    if (stream) static_cast<void>(hipStreamSynchronize(stream));
+    if (hjobs) static_cast<void>(hipHostFree(hjobs));
+    if (hsmall) static_cast<void>(hipHostFree(hsmall));
+    if (stream) static_cast<void>(hipStreamDestroy(stream));
+}
+
+const HxJob *pisces_ctx::push_job(const HxJob &j)
+{
+    if (job_slot == kJobRing) {  // host staging slots are reused: drain first
+        PC_HIP(hipStreamSynchronize(stream));
+        job_slot = 0;
+    }
+    const int s = job_slot++;
+    hjobs[s] = j;
+    HxJob *d = djobs.as<HxJob>() + s;
+    PC_HIP(hipMemcpyAsync(d, &hjobs[s], sizeof(HxJob), hipMemcpyHostToDevice, stream));
+    return d;
+}
+
+void pisces_ctx::hx(const void *X, void *Y, long long ld, int nvec, int precision)
+{
+    if (nvec <= 0) return;
+    HxJob j;
+    j.nx = n[0]; j.ny = n[1]; j.nz = n[2]; j.nvec = nvec;
+    const bool f64 = (precision == PISCES_FP64);
+    for (int a = 0; a < 3; ++a) j.T[a] = f64 ? dT64[a].p : dT32[a].p;
+    j.V = f64 ? dV64.p : dV32.p;
+    j.X = X; j.Y = Y; j.ld = ld;
+    const HxJob *d = push_job(j);
+    hx_launch(d, 1, n[2] * nvec, n[1] * nvec, maxn, precision, stream);
+}
+
+void pisces_ctx::sync() { PC_HIP(hipStreamSynchronize(stream)); }
+
+// ------------------------------------------------------------------ helpers
+namespace {
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        return f();
+    } catch (const hip_error &e) {
+        set_last_error(std::string("HIP error ") + hipGetErrorString(e.err) + " in " + e.what);
+        return PISCES_ERR_HIP;
+    } catch (const api_error &e) {
+        set_last_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return PISCES_ERR_NOMEM;
+    }
+}
+
+void upload(dbuf &b, const void *h, size_t bytes, hipStream_t s)
+{
+    b.ensure(bytes);
+    PC_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, s));
+}
+
+void set_potential_device(pisces_ctx *c)
+{
+    // the FP32 copy for the low-precision H*X forms
+    c->dV32.ensure(sizeof(float) * c->ngp);
+    to_f32(c->dV64.as<double>(), c->dV32.as<float>(), c->ngp, c->stream);
+    c->has_v = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pisces_version(void) { return "pisces-mi355x 0.1 (gfx950)"; }
+
+const char *pisces_strerror(int code)
+{
+    switch (code) {
+    case PISCES_OK: return "ok";
+    case PISCES_ERR_ARG: return "invalid argument";
+    case PISCES_ERR_HIP: return "HIP runtime error";
+    case PISCES_ERR_NOMEM: return "out of memory";
+    case PISCES_ERR_UNSUPPORTED: return "unsupported option";
+    case PISCES_ERR_STATE: return "invalid state";
+    case PISCES_ERR_NUMERIC: return "numerical failure";
+    default: return code > 0 ? "not converged" : "unknown error";
+    }
+}
+
+const char *pisces_last_error(void) { return pisces::last_error(); }
+
+int pisces_create(pisces_ctx **out, const int n[3], const double L[3], int device)
+{
+    if (!out || !n || !L) return PISCES_ERR_ARG;
+    *out = nullptr;
+    for (int a = 0; a < 3; ++a)
+        if (n[a] < 2 || n[a] > 128 || !(L[a] > 0)) {
+            set_last_error("grid dimensions must be 2..128 and lengths positive");
+            return PISCES_ERR_ARG;
+        }
+    pisces_ctx *c = new (std::nothrow) pisces_ctx();
+    if (!c) return PISCES_ERR_NOMEM;
+    int rc = guarded([&] {
+        c->device = device;
+        PC_HIP(hipSetDevice(device));
+        PC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->ngp = 1;
+        for (int a = 0; a < 3; ++a) {
+            c->n[a] = n[a];
+            c->L[a] = L[a];
+            c->ngp *= n[a];
+            c->maxn = std::max(c->maxn, n[a]);
+            c->x[a].resize(n[a]);
+            c->T[a].resize((size_t)n[a] * n[a]);
+            // DVR::ComputeGridPointsAndKineticEnergy: sine_dvr on [-L/2, L/2], mass 1
+            sine_dvr_full(n[a], 1.0, -0.5 * L[a], 0.5 * L[a], c->x[a].data(), c->T[a].data());
+            upload(c->dx[a], c->x[a].data(), sizeof(double) * n[a], c->stream);
+            upload(c->dT64[a], c->T[a].data(), sizeof(double) * n[a] * n[a], c->stream);
+            std::vector<float> t32(c->T[a].begin(), c->T[a].end());
+            upload(c->dT32[a], t32.data(), sizeof(float) * n[a] * n[a], c->stream);
+            PC_HIP(hipStreamSynchronize(c->stream));
+        }
+        PC_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->hjobs),
+                             sizeof(HxJob) * pisces_ctx::kJobRing, hipHostMallocDefault));
+        PC_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->hsmall),
+                             sizeof(double) * pisces_ctx::kSmall, hipHostMallocDefault));
+        c->djobs.alloc(sizeof(HxJob) * pisces_ctx::kJobRing);
+        c->dV64.alloc(sizeof(double) * c->ngp);
+        return 0;
+    });
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return PISCES_OK;
+}
+
+void pisces_destroy(pisces_ctx *c) { delete c; }
+
+int pisces_grid(const pisces_ctx *c, double *xyz, double *T)
+{
+    if (!c) return PISCES_ERR_ARG;
+    size_t ox = 0, ot = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (xyz) std::memcpy(xyz + ox, c->x[a].data(), sizeof(double) * c->n[a]);
+        if (T) std::memcpy(T + ot, c->T[a].data(), sizeof(double) * c->n[a] * c->n[a]);
+        ox += c->n[a];
+        ot += (size_t)c->n[a] * c->n[a];
+    }
+    return PISCES_OK;
+}
+
+int pisces_set_potential(pisces_ctx *c, const double *v)
+{
+    if (!c || !v) return PISCES_ERR_ARG;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        PC_HIP(hipMemcpyAsync(c->dV64.p, v, sizeof(double) * c->ngp, hipMemcpyHostToDevice, c->stream));
+        set_potential_device(c);
+        c->sync();
+        return 0;
+    });
+}
+
+int pisces_get_potential(const pisces_ctx *c, double *v)
+{
+    if (!c || !v) return PISCES_ERR_ARG;
+    if (!c->has_v) return PISCES_ERR_STATE;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        PC_HIP(hipMemcpyAsync(v, c->dV64.p, sizeof(double) * c->ngp, hipMemcpyDeviceToHost, c->stream));
+        PC_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+int pisces_build_potential(pisces_ctx *c, const int flags[6], const double *potpara, int nsites,
+                           const double *sites, int nq, const double *q, const int *iq, int nd,
+                           const double *d, const int *id, int nion, const double *ion_xyz,
+                           const double *ion_q, double ion_r)
+{
+    if (!c || !flags || !potpara || nsites <= 0 || !sites || nq < 0 || nd < 0 || nion < 0)
+        return PISCES_ERR_ARG;
+    if ((nq && (!q || !iq)) || (nd && (!d || !id)) || (nion && (!ion_xyz || !ion_q)))
+        return PISCES_ERR_ARG;
+    for (int i = 0; i < nq; ++i)
+        if (iq[i] < 0 || iq[i] >= nsites) return PISCES_ERR_ARG;
+    for (int i = 0; i < nd; ++i)
+        if (id[i] < 0 || id[i] >= nsites) return PISCES_ERR_ARG;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        PotParams P{};
+        std::vector<double> rep;
+        std::vector<int> irep, ipps;
+        PC_REQUIRE(potential_setup(flags, potpara, nsites, P, rep, irep, ipps),
+                   PISCES_ERR_UNSUPPORTED, "unsupported potential flags");
+        P.ion_r = ion_r;
+        dbuf ds, dq, diq, dd, did, drep, direp, dipps, dix, diq2;
+        upload(ds, sites, sizeof(double) * 3 * nsites, c->stream);
+        if (nq) { upload(dq, q, sizeof(double) * nq, c->stream); upload(diq, iq, sizeof(int) * nq, c->stream); }
+        if (nd) { upload(dd, d, sizeof(double) * 3 * nd, c->stream); upload(did, id, sizeof(int) * nd, c->stream); }
+        if (!rep.empty()) {
+            upload(drep, rep.data(), sizeof(double) * rep.size(), c->stream);
+            upload(direp, irep.data(), sizeof(int) * irep.size(), c->stream);
+        }
+        if (!ipps.empty()) upload(dipps, ipps.data(), sizeof(int) * ipps.size(), c->stream);
+        if (nion) {
+            upload(dix, ion_xyz, sizeof(double) * 3 * nion, c->stream);
+            upload(diq2, ion_q, sizeof(double) * nion, c->stream);
+        }
+        PotDev D{nsites, nq, nd, (int)irep.size(), (int)ipps.size(), nion,
+                 ds.as<double>(), dq.as<double>(), diq.as<int>(), dd.as<double>(), did.as<int>(),
+                 drep.as<double>(), direp.as<int>(), dipps.as<int>(), dix.as<double>(),
+                 diq2.as<double>()};
+        potential_grid_launch(P, D, c->n[0], c->n[1], c->n[2], c->dx[0].as<double>(),
+                              c->dx[1].as<double>(), c->dx[2].as<double>(), c->dV64.as<double>(),
+                              nullptr, c->stream);
+        set_potential_device(c);
+        c->sync();
+        return 0;
+    });
+}
+
+int pisces_build_cluster_potential(pisces_ctx *c, int nw, const double *xyz_ang,
+                                   const int flags[6], const double *potpara, int nion,
+                                   const double *ion_xyz_bohr, const double *ion_q, double ion_r,
+                                   double *sites_out, double *mu_out)
+{
+    if (!c || nw <= 0 || !xyz_ang || !flags || !potpara || nion < 0) return PISCES_ERR_ARG;
+    if (nion && (!ion_xyz_bohr || !ion_q)) return PISCES_ERR_ARG;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        std::vector<double> sites((size_t)12 * nw);
+        PC_REQUIRE(water_sites_host(nw, xyz_ang, sites.data()), PISCES_ERR_NUMERIC,
+                   "water projection produced non-finite coordinates");
+        const int ns = 3 * nw;
+        dbuf dsites, dmu, depc, dtens, dinfo;
+        upload(dsites, sites.data(), sizeof(double) * sites.size(), c->stream);
+        dmu.alloc(sizeof(double) * 3 * ns);
+        depc.alloc(sizeof(double) * 3 * ns);
+        dtens.alloc(sizeof(double) * 6 * (size_t)ns * ns);
+        dinfo.alloc(sizeof(int) * 2);
+        water_scf_launch(nw, dsites.as<double>(), dmu.as<double>(), depc.as<double>(),
+                         dtens.as<double>(), dinfo.as<int>(), c->stream);
+        int info[2] = {0, 0};
+        PC_HIP(hipMemcpyAsync(info, dinfo.p, sizeof(info), hipMemcpyDeviceToHost, c->stream));
+        std::vector<double> mu((size_t)3 * ns);
+        PC_HIP(hipMemcpyAsync(mu.data(), dmu.p, sizeof(double) * mu.size(), hipMemcpyDeviceToHost, c->stream));
+        PC_HIP(hipStreamSynchronize(c->stream));
+        if (sites_out) std::memcpy(sites_out, sites.data(), sizeof(double) * sites.size());
+        if (mu_out) std::memcpy(mu_out, mu.data(), sizeof(double) * mu.size());
+        std::vector<double> q;
+        std::vector<int> iq, id;
+        water_lists_host(nw, q, iq, id);
+        const int rc = pisces_build_potential(c, flags, potpara, 4 * nw, sites.data(), (int)q.size(),
+                                              q.data(), iq.data(), (int)id.size(), mu.data(),
+                                              id.data(), nion, ion_xyz_bohr, ion_q, ion_r);
+        if (rc < 0) return rc;
+        return info[0] ? 0 : 1;
+    });
+}
+
+int pisces_apply_h(pisces_ctx *c, const double *X, double *Y, int nvec)
+{
+    if (!c || !X || !Y || nvec <= 0) return PISCES_ERR_ARG;
+    if (!c->has_v) return PISCES_ERR_STATE;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        const size_t bytes = sizeof(double) * c->ngp * nvec;
+        c->wX.ensure(bytes);
+        c->wY.ensure(bytes);
+        PC_HIP(hipMemcpyAsync(c->wX.p, X, bytes, hipMemcpyHostToDevice, c->stream));
+        c->hx(c->wX.p, c->wY.p, c->ngp, nvec, PISCES_FP64);
+        PC_HIP(hipMemcpyAsync(Y, c->wY.p, bytes, hipMemcpyDeviceToHost, c->stream));
+        c->sync();
+        return 0;
+    });
+}
+
+int pisces_apply_h_device(pisces_ctx *c, const void *dX, void *dY, int nvec, int precision)
+{
+    if (!c || !dX || !dY || nvec <= 0) return PISCES_ERR_ARG;
+    if (precision < PISCES_FP64 || precision > PISCES_BF16X2) return PISCES_ERR_ARG;
+    if (!c->has_v) return PISCES_ERR_STATE;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        c->hx(dX, dY, c->ngp, nvec, precision);
+        c->sync();
+        return 0;
+    });
+}
+
+int pisces_diag(pisces_ctx *c, double *diag)
+{
+    if (!c || !diag) return PISCES_ERR_ARG;
+    if (!c->has_v) return PISCES_ERR_STATE;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        c->wdiag.ensure(sizeof(double) * c->ngp);
+        diag_h(c->dV64.as<double>(), c->dT64[0].as<double>(), c->dT64[1].as<double>(),
+               c->dT64[2].as<double>(), c->n[0], c->n[1], c->n[2], c->wdiag.as<double>(), c->stream);
+        PC_HIP(hipMemcpyAsync(diag, c->wdiag.p, sizeof(double) * c->ngp, hipMemcpyDeviceToHost, c->stream));
+        c->sync();
+        return 0;
+    });
+}
+
+int pisces_piab(pisces_ctx *c, double *wf)
+{
+    if (!c || !wf) return PISCES_ERR_ARG;
+    // DVR::ParticleInAnDBoxWf (src/DVR.cpp:1725-1805): x0 = first grid point,
+    // L = last - first; the lowest state of the central half box.
+    const double pi = 3.14159265358979;
+    double x0[3], Lb[3];
+    for (int a = 0; a < 3; ++a) {
+        x0[a] = c->x[a][0];
+        Lb[a] = c->x[a][c->n[a] - 1] - x0[a];
+    }
+    for (int k = 0; k < c->n[2]; ++k)
+        for (int j = 0; j < c->n[1]; ++j)
+            for (int i = 0; i < c->n[0]; ++i) {
+                const int ii[3] = {i, j, k};
+                double w = 1.0;
+                for (int a = 0; a < 3; ++a) {
+                    const double q = c->x[a][ii[a]];
+                    if (q < x0[a] / 2 || q > -x0[a] / 2) w = 0.0;
+                    else w *= std::sin(pi / Lb[a] * 2.0 * (q - x0[a] / 2.0));
+                }
+                wf[i + (long long)c->n[0] * (j + (long long)c->n[1] * k)] = w;
+            }
+    return PISCES_OK;
+}
+
+int pisces_solve(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, int corrflag,
+                 int precision, double *vecs, double *evals, pisces_stats *st)
+{
+    if (!c || !vecs || !evals) return PISCES_ERR_ARG;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        if (precision == PISCES_FP64)
+            return solve_fp64(c, nroots, maxsub, maxmacro, ptol, corrflag, vecs, evals, st);
+        PC_REQUIRE(precision == PISCES_FP32 || precision == PISCES_BF16X2, PISCES_ERR_ARG,
+                   "unknown precision");
+        return solve_mixed(c, nroots, maxsub, maxmacro, ptol, corrflag, precision, vecs, evals, st);
+    });
+}
+
+// internal test hook: select the H*X kernel family (0 VALU, 1 MFMA)
+int pisces_debug_set_hx_variant(int v)
+{
+    if (v < 0 || v > 1) return PISCES_ERR_ARG;
+    hx_set_variant(v);
+    return PISCES_OK;
+}
+
+}  // extern "C"

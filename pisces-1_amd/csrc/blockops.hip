@@ -1,0 +1,395 @@
+// blockops.hip -- Davidson block operations on grid vectors for gfx950.
+//
+// The vectors are long (N = ngp up to ~10^6) and the blocks narrow (m <= 64
+// basis vectors, k <= 16 new vectors), so every operation is a tall-skinny
+// product that streams the basis once from HBM:
+//   tsgram:  G = A^T B        (m x k Gram / projection blocks, FP64 accumulation)
+//   combine: Y = A W  (+/-)   (Ritz vectors B V, Z V; CGS updates B h)
+// plus pointwise kernels (residual, Davidson / Jacobi-Davidson correction).
+// Reductions are deterministic: per-chunk partial sums in a fixed layout, then a
+// fixed-order reduction, so a solve is bitwise reproducible run to run.
+#include "blockops.hpp"
+
+namespace pisces {
+
+namespace {
+
+constexpr int kT = 256;
+
+// ---------------------------------------------------------------- tsgram
+// partial[o * nchunks + chunk], o = a + ma * b, over elements of one chunk.
+// The chunk of every A and B column is staged in LDS (coalesced loads); each
+// thread then forms whole dot products of one (a, b) pair from LDS.
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(kT)
+tsgram_kernel(const TA *__restrict__ A, long long lda, int ma, const TB *__restrict__ B,
+              long long ldb, int mb, long long N, int chunk, double *__restrict__ partial,
+              int nchunks)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *sa = reinterpret_cast<double *>(smem);
+    const int pitch = chunk + 1;
+    double *sb = sa + (size_t)ma * pitch;
+    const long long g0 = (long long)blockIdx.x * chunk;
+    const int len = (int)((N - g0) < chunk ? (N - g0) : chunk);
+    for (int a = 0; a < ma; ++a)
+        for (int e = threadIdx.x; e < chunk; e += kT)
+            sa[a * pitch + e] = (e < len) ? (double)A[a * lda + g0 + e] : 0.0;
+    for (int b = 0; b < mb; ++b)
+        for (int e = threadIdx.x; e < chunk; e += kT)
+            sb[b * pitch + e] = (e < len) ? (double)B[b * ldb + g0 + e] : 0.0;
+    __syncthreads();
+    for (int o = threadIdx.x; o < ma * mb; o += kT) {
+        const int a = o % ma, b = o / ma;
+        const double *pa = sa + a * pitch, *pb = sb + b * pitch;
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        int e = 0;
+        for (; e + 3 < len; e += 4) {
+            s0 += pa[e] * pb[e];
+            s1 += pa[e + 1] * pb[e + 1];
+            s2 += pa[e + 2] * pb[e + 2];
+            s3 += pa[e + 3] * pb[e + 3];
+        }
+        for (; e < len; ++e) s0 += pa[e] * pb[e];
+        partial[(long long)o * nchunks + blockIdx.x] = (s0 + s1) + (s2 + s3);
+    }
+}
+
+// out[o] = sum over chunks of partial[o * nchunks + c]; one block per output.
+__global__ void __launch_bounds__(kT)
+reduce_kernel(const double *__restrict__ partial, int nchunks, double *__restrict__ out,
+              int out_ld, int ma)
+{
+    __shared__ double red[kT];
+    const int o = blockIdx.x;
+    double s = 0;
+    for (int c = threadIdx.x; c < nchunks; c += kT) s += partial[(long long)o * nchunks + c];
+    red[threadIdx.x] = s;
+    for (int w = kT / 2; w > 0; w >>= 1) {
+        __syncthreads();
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    }
+    if (threadIdx.x == 0) {
+        const int a = o % ma, b = o / ma;
+        out[a + (long long)out_ld * b] = red[0];
+    }
+}
+
+// ---------------------------------------------------------------- combine
+// Y[b] = op(Y[b], sum_a A[a] W[a + ldw*b]) for b < mb; W staged in LDS.
+// mode 0: Y = AW, mode 1: Y = Y - AW, mode 2: Y = Y + AW.
+template <typename TA, typename TY>
+__global__ void __launch_bounds__(kT)
+combine_kernel(const TA *__restrict__ A, long long lda, int ma, const double *__restrict__ W,
+               int ldw, int mb, TY *__restrict__ Y, long long ldy, long long N, int mode)
+{
+    __shared__ double w[64 * 16];
+    for (int t = threadIdx.x; t < ma * mb; t += kT) {
+        const int a = t % ma, b = t / ma;
+        w[t] = W[a + (long long)ldw * b];
+    }
+    __syncthreads();
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= N) return;
+    double acc[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) acc[b] = 0.0;
+    for (int a = 0; a < ma; ++a) {
+        const double x = (double)A[a * lda + g];
+#pragma unroll
+        for (int b = 0; b < 16; ++b)
+            if (b < mb) acc[b] += x * w[a + ma * b];
+    }
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+        if (b >= mb) break;
+        TY *y = Y + b * ldy + g;
+        if (mode == 0) *y = (TY)acc[b];
+        else if (mode == 1) *y = (TY)((double)*y - acc[b]);
+        else *y = (TY)((double)*y + acc[b]);
+    }
+}
+
+// ---------------------------------------------------------------- pointwise
+// U[r] *= s[r];  R[r] = ZV[r] - lam[r] * U[r]  (src/Davidson.cpp:242-259)
+__global__ void __launch_bounds__(kT)
+residual_kernel(double *__restrict__ U, const double *__restrict__ ZV, double *__restrict__ R,
+                long long ld, long long N, int nr, const double *__restrict__ scale,
+                const double *__restrict__ lam)
+{
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= N) return;
+    for (int r = 0; r < nr; ++r) {
+        const double u = U[r * ld + g] * scale[r];
+        U[r * ld + g] = u;
+        R[r * ld + g] = ZV[r * ld + g] - lam[r] * u;
+    }
+}
+
+// Davidson correction c = r / (lambda - diag), |lambda - diag| >= 1e-12
+// (src/Davidson.cpp:557-566); writes C[t] from R[src[t]].
+__global__ void __launch_bounds__(kT)
+precond_kernel(const double *__restrict__ R, long long ldr, const double *__restrict__ diag,
+               double *__restrict__ C, long long ldc, long long N, int k,
+               const int *__restrict__ src, const double *__restrict__ lam)
+{
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= N) return;
+    const double dg = diag[g];
+    for (int t = 0; t < k; ++t) {
+        double dv = lam[t] - dg;
+        if (fabs(dv) < 1e-12) dv = 1e-12;
+        C[t * ldc + g] = R[src[t] * ldr + g] / dv;
+    }
+}
+
+// Jacobi-Davidson step 1 (src/Davidson.cpp:530-549): c = -M r, j = M u,
+// M = 1/(diag - lambda) with the same 1e-12 guard.
+__global__ void __launch_bounds__(kT)
+jd_kernel(const double *__restrict__ R, long long ldr, const double *__restrict__ U,
+          const double *__restrict__ diag, double *__restrict__ C, double *__restrict__ Jv,
+          long long ldc, long long N, int k, const int *__restrict__ src,
+          const int *__restrict__ usrc, const double *__restrict__ lam)
+{
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= N) return;
+    const double dg = diag[g];
+    for (int t = 0; t < k; ++t) {
+        double mk = dg - lam[t];
+        if (fabs(mk) < 1e-12) mk = 1e-12;
+        mk = 1.0 / mk;
+        C[t * ldc + g] = -R[src[t] * ldr + g] * mk;
+        Jv[t * ldc + g] = mk * U[usrc[t] * ldr + g];
+    }
+}
+
+// c += f * j with f = (u.c)/(u.j) formed on the device from reduced dots.
+__global__ void __launch_bounds__(kT)
+jd_finish_kernel(double *__restrict__ C, const double *__restrict__ Jv, long long ldc,
+                 long long N, int k, const double *__restrict__ dots)
+{
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= N) return;
+    for (int t = 0; t < k; ++t) {
+        const double f = dots[2 * t] / dots[2 * t + 1];
+        C[t * ldc + g] += f * Jv[t * ldc + g];
+    }
+}
+
+// Cholesky of the k x k Gram matrix G (column-major, ld k) and W = L^{-T}, so
+// that C W has orthonormal columns (CholQR); a non-positive pivot marks the
+// column as dependent: its column of W is zeroed and info[0] counts it.
+__global__ void cholinv_kernel(const double *__restrict__ G, int k, double *__restrict__ W,
+                               int *__restrict__ info, double rel_tol)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double L[16 * 16];
+    int bad = 0;
+    int keep[16];
+    for (int j = 0; j < k; ++j) {
+        for (int i = 0; i < k; ++i) L[i + 16 * j] = 0.0;
+    }
+    for (int j = 0; j < k; ++j) {
+        double d = G[j + k * j];
+        for (int p = 0; p < j; ++p) d -= L[j + 16 * p] * L[j + 16 * p];
+        keep[j] = (d > rel_tol * G[j + k * j] && d > 0.0) ? 1 : 0;
+        if (!keep[j]) { ++bad; continue; }
+        const double ljj = sqrt(d);
+        L[j + 16 * j] = ljj;
+        for (int i = j + 1; i < k; ++i) {
+            double s = G[i + k * j];
+            for (int p = 0; p < j; ++p) s -= L[i + 16 * p] * L[j + 16 * p];
+            L[i + 16 * j] = s / ljj;
+        }
+    }
+    // W = L^{-T}: solve L^T W = I column by column (upper-triangular system)
+    for (int c = 0; c < k; ++c) {
+        for (int i = k - 1; i >= 0; --i) {
+            double s = (i == c) ? 1.0 : 0.0;
+            if (!keep[i]) { W[i + k * c] = 0.0; continue; }
+            for (int p = i + 1; p < k; ++p) s -= L[p + 16 * i] * W[p + k * c];
+            W[i + k * c] = s / L[i + 16 * i];
+        }
+    }
+    for (int c = 0; c < k; ++c)
+        if (!keep[c])
+            for (int i = 0; i < k; ++i) W[i + k * c] = 0.0;
+    info[0] = bad;
+    for (int c = 0; c < k; ++c) info[1 + c] = keep[c];
+}
+
+// diag(H) = V + Tx_ii + Ty_jj + Tz_kk (DVR::ComputeDiagonal, src/DVR.cpp:1808-1881)
+__global__ void __launch_bounds__(kT)
+diag_kernel(const double *__restrict__ V, const double *__restrict__ Tx,
+            const double *__restrict__ Ty, const double *__restrict__ Tz, int nx, int ny,
+            int nz, double *__restrict__ D)
+{
+    const long long N = (long long)nx * ny * nz;
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= N) return;
+    const int i = (int)(g % nx), j = (int)((g / nx) % ny), k = (int)(g / ((long long)nx * ny));
+    double d = V[g];
+    d += Tx[i + (long long)nx * i];
+    d += Ty[j + (long long)ny * j];
+    d += Tz[k + (long long)nz * k];
+    D[g] = d;
+}
+
+__global__ void __launch_bounds__(kT)
+convert_kernel(const double *__restrict__ x, float *__restrict__ y, long long n)
+{
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g < n) y[g] = (float)x[g];
+}
+
+inline unsigned nblk(long long N) { return (unsigned)((N + kT - 1) / kT); }
+
+}  // namespace
+
+int gram_chunk(int ma, int mb)
+{
+    // LDS budget: (ma + mb) * (chunk + 1) doubles <= 96 KB
+    int chunk = 256;
+    while (chunk > 32 && (size_t)(ma + mb) * (chunk + 1) * 8 > 96 * 1024) chunk /= 2;
+    return chunk;
+}
+
+template <typename TA, typename TB>
+static void tsgram_impl(const TA *A, long long lda, int ma, const TB *B, long long ldb, int mb,
+                        long long N, double *d_partial, size_t partial_cap, double *d_out,
+                        int out_ld, hipStream_t s)
+{
+    if (ma <= 0 || mb <= 0) return;
+    const int chunk = gram_chunk(ma, mb);
+    const int nchunks = (int)((N + chunk - 1) / chunk);
+    PC_REQUIRE((size_t)nchunks * ma * mb <= partial_cap, PISCES_ERR_ARG, "gram scratch too small");
+    const size_t lds = (size_t)(ma + mb) * (chunk + 1) * sizeof(double);
+    static bool once = false;
+    if (!once) {
+        static_cast<void>(hipFuncSetAttribute(
+            reinterpret_cast<const void *>(tsgram_kernel<TA, TB>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        once = true;
+    }
+    hipLaunchKernelGGL((tsgram_kernel<TA, TB>), dim3(nchunks), dim3(kT), lds, s, A, lda, ma, B,
+                       ldb, mb, N, chunk, d_partial, nchunks);
+    hipLaunchKernelGGL(reduce_kernel, dim3(ma * mb), dim3(kT), 0, s, d_partial, nchunks, d_out,
+                       out_ld, ma);
+    PC_HIP(hipGetLastError());
+}
+
+void tsgram(const double *A, long long lda, int ma, const double *B, long long ldb, int mb,
+            long long N, double *d_partial, size_t partial_cap, double *d_out, int out_ld,
+            hipStream_t s)
+{
+    tsgram_impl<double, double>(A, lda, ma, B, ldb, mb, N, d_partial, partial_cap, d_out, out_ld, s);
+}
+void tsgram(const float *A, long long lda, int ma, const double *B, long long ldb, int mb,
+            long long N, double *d_partial, size_t partial_cap, double *d_out, int out_ld,
+            hipStream_t s)
+{
+    tsgram_impl<float, double>(A, lda, ma, B, ldb, mb, N, d_partial, partial_cap, d_out, out_ld, s);
+}
+void tsgram(const float *A, long long lda, int ma, const float *B, long long ldb, int mb,
+            long long N, double *d_partial, size_t partial_cap, double *d_out, int out_ld,
+            hipStream_t s)
+{
+    tsgram_impl<float, float>(A, lda, ma, B, ldb, mb, N, d_partial, partial_cap, d_out, out_ld, s);
+}
+void tsgram(const double *A, long long lda, int ma, const float *B, long long ldb, int mb,
+            long long N, double *d_partial, size_t partial_cap, double *d_out, int out_ld,
+            hipStream_t s)
+{
+    tsgram_impl<double, float>(A, lda, ma, B, ldb, mb, N, d_partial, partial_cap, d_out, out_ld, s);
+}
+
+template <typename TA, typename TY>
+static void combine_impl(const TA *A, long long lda, int ma, const double *d_W, int ldw, int mb,
+                         TY *Y, long long ldy, long long N, int mode, hipStream_t s)
+{
+    if (mb <= 0) return;
+    PC_REQUIRE(ma <= 64 && mb <= 16, PISCES_ERR_ARG, "combine block too wide");
+    if (ma <= 0) {
+        PC_REQUIRE(mode != 0, PISCES_ERR_ARG, "empty combine");
+        return;
+    }
+    hipLaunchKernelGGL((combine_kernel<TA, TY>), dim3(nblk(N)), dim3(kT), 0, s, A, lda, ma, d_W,
+                       ldw, mb, Y, ldy, N, mode);
+    PC_HIP(hipGetLastError());
+}
+
+void combine(const double *A, long long lda, int ma, const double *d_W, int ldw, int mb,
+             double *Y, long long ldy, long long N, int mode, hipStream_t s)
+{
+    combine_impl<double, double>(A, lda, ma, d_W, ldw, mb, Y, ldy, N, mode, s);
+}
+void combine(const float *A, long long lda, int ma, const double *d_W, int ldw, int mb,
+             double *Y, long long ldy, long long N, int mode, hipStream_t s)
+{
+    combine_impl<float, double>(A, lda, ma, d_W, ldw, mb, Y, ldy, N, mode, s);
+}
+void combine(const float *A, long long lda, int ma, const double *d_W, int ldw, int mb,
+             float *Y, long long ldy, long long N, int mode, hipStream_t s)
+{
+    combine_impl<float, float>(A, lda, ma, d_W, ldw, mb, Y, ldy, N, mode, s);
+}
+void combine(const double *A, long long lda, int ma, const double *d_W, int ldw, int mb,
+             float *Y, long long ldy, long long N, int mode, hipStream_t s)
+{
+    combine_impl<double, float>(A, lda, ma, d_W, ldw, mb, Y, ldy, N, mode, s);
+}
+
+void residual(double *U, const double *ZV, double *R, long long ld, long long N, int nr,
+              const double *d_scale, const double *d_lam, hipStream_t s)
+{
+    hipLaunchKernelGGL(residual_kernel, dim3(nblk(N)), dim3(kT), 0, s, U, ZV, R, ld, N, nr,
+                       d_scale, d_lam);
+    PC_HIP(hipGetLastError());
+}
+
+void precond(const double *R, long long ldr, const double *diag, double *C, long long ldc,
+             long long N, int k, const int *d_src, const double *d_lam, hipStream_t s)
+{
+    hipLaunchKernelGGL(precond_kernel, dim3(nblk(N)), dim3(kT), 0, s, R, ldr, diag, C, ldc, N,
+                       k, d_src, d_lam);
+    PC_HIP(hipGetLastError());
+}
+
+void jd_precond(const double *R, long long ldr, const double *U, const double *diag, double *C,
+                double *J, long long ldc, long long N, int k, const int *d_src,
+                const int *d_usrc, const double *d_lam, hipStream_t s)
+{
+    hipLaunchKernelGGL(jd_kernel, dim3(nblk(N)), dim3(kT), 0, s, R, ldr, U, diag, C, J, ldc, N,
+                       k, d_src, d_usrc, d_lam);
+    PC_HIP(hipGetLastError());
+}
+
+void jd_finish(double *C, const double *J, long long ldc, long long N, int k,
+               const double *d_dots, hipStream_t s)
+{
+    hipLaunchKernelGGL(jd_finish_kernel, dim3(nblk(N)), dim3(kT), 0, s, C, J, ldc, N, k, d_dots);
+    PC_HIP(hipGetLastError());
+}
+
+void cholinv(const double *d_G, int k, double *d_W, int *d_info, double rel_tol, hipStream_t s)
+{
+    PC_REQUIRE(k <= 16, PISCES_ERR_ARG, "block too wide for cholinv");
+    hipLaunchKernelGGL(cholinv_kernel, dim3(1), dim3(64), 0, s, d_G, k, d_W, d_info, rel_tol);
+    PC_HIP(hipGetLastError());
+}
+
+void diag_h(const double *V, const double *Tx, const double *Ty, const double *Tz, int nx,
+            int ny, int nz, double *D, hipStream_t s)
+{
+    hipLaunchKernelGGL(diag_kernel, dim3(nblk((long long)nx * ny * nz)), dim3(kT), 0, s, V, Tx,
+                       Ty, Tz, nx, ny, nz, D);
+    PC_HIP(hipGetLastError());
+}
+
+void to_f32(const double *x, float *y, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(convert_kernel, dim3(nblk(n)), dim3(kT), 0, s, x, y, n);
+    PC_HIP(hipGetLastError());
+}
+
+}  // namespace pisces

@@ -1,0 +1,127 @@
+// common.hpp -- shared declarations of the PISCES MI355X library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pisces_hip.h"
+
+namespace pisces {
+
+// Error handling: every HIP call goes through PC_HIP, which turns a failure into
+// a thrown hip_error; the C-ABI layer converts exceptions into return codes.
+struct hip_error {
+    hipError_t err;
+    const char *what;
+    int line;
+};
+struct api_error {
+    int code;
+    std::string msg;
+};
+
+#define PC_HIP(call)                                                     \
+    do {                                                                 \
+        hipError_t e_ = (call);                                          \
+        if (e_ != hipSuccess) throw ::pisces::hip_error{e_, #call, __LINE__}; \
+    } while (0)
+
+#define PC_REQUIRE(cond, code, msg)                                      \
+    do {                                                                 \
+        if (!(cond)) throw ::pisces::api_error{(code), (msg)};          \
+    } while (0)
+
+void set_last_error(const std::string &s);
+
+// Device buffer owned by RAII.
+struct dbuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    dbuf() = default;
+    explicit dbuf(size_t b) { alloc(b); }
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    dbuf(dbuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    dbuf &operator=(dbuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~dbuf() { release(); }
+    void alloc(size_t b) {
+        release();
+        if (b) PC_HIP(hipMalloc(&p, b));
+        bytes = b;
+    }
+    void ensure(size_t b) { if (b > bytes) alloc(b); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// --------------------------------------------------------------- H*X
+// One instance's share of an H*X launch: Y[:,v] = H X[:,v], v < nvec.
+struct HxJob {
+    int nx, ny, nz, nvec;
+    const void *T[3];   // full symmetric kinetic matrices, n_a*n_a, in the job precision
+    const void *V;      // diagonal potential (ngp), job precision
+    const void *X;      // vectors, X + v*ld
+    void *Y;
+    long long ld;
+};
+
+// precision codes: PISCES_FP64 / PISCES_FP32 / PISCES_BF16X2
+void hx_launch(const HxJob *d_jobs, int njobs, int max_planes_x, int max_planes_z,
+               int max_n, int precision, hipStream_t s);
+// Kernel variant selector (0 = VALU LDS-tiled, 1 = MFMA); default 1 where available.
+void hx_set_variant(int v);
+int hx_get_variant();
+
+// --------------------------------------------------------------- potential
+struct PotParams {
+    int damp_type;     // 1 Gaussian, 2 effective distance
+    int rep_type;      // 0 none, 1 Gaussians exp(-a R^2), 2 Slaters exp(-a R)
+    int pol_type;      // 0 none, 1 non-interacting polarizable M sites
+    int adiabatic;
+    double qdamp, ddamp, pdamp, alpha, eps;
+    double ion_r;
+};
+// Builds the repulsive-core table and PotParams from PotFlag/PotPara exactly as
+// Potential::SetupDPPGTOP does; returns false for unsupported flag sets.
+bool potential_setup(const int *flags, const double *pp, int nsites, PotParams &P,
+                     std::vector<double> &rep, std::vector<int> &irep,
+                     std::vector<int> &ipps);
+
+struct PotDev {
+    int nsites, nq, nd, nrep, npps, nion;
+    const double *sites;   // 3*nsites
+    const double *q; const int *iq;
+    const double *d; const int *id;
+    const double *rep; const int *irep;
+    const int *ipps;
+    const double *ion_xyz; const double *ion_q;
+};
+void potential_grid_launch(const PotParams &P, const PotDev &D, int nx, int ny, int nz,
+                           const double *d_x, const double *d_y, const double *d_z,
+                           double *d_v, float *d_v32, hipStream_t s);
+
+// --------------------------------------------------------------- water model
+// Host: rigid-monomer projection of O,H1,H2 (Angstrom) to 4 sites (bohr).
+bool water_sites_host(int nw, const double *xyz_ang, double *sites);
+// Device: induced-dipole SCF; d_sites 12*nw, outputs d_mu/d_epc 9*nw.
+void water_scf_launch(int nw, const double *d_sites, double *d_mu, double *d_epc,
+                      double *d_tensor, int *d_info, hipStream_t s);
+// Charges of the DPP monomer (H1, H2, M) and polarizabilities (O, H1, H2).
+void water_lists_host(int nw, std::vector<double> &q, std::vector<int> &iq,
+                      std::vector<int> &id);
+
+// --------------------------------------------------------------- host helpers
+void sine_dvr_full(int n, double mass, double x0, double xnp1, double *x, double *T);
+
+}  // namespace pisces

@@ -1,0 +1,56 @@
+// context.hpp -- the per-grid context behind the C ABI.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "blockops.hpp"
+#include "common.hpp"
+
+// One sine-DVR grid on one device, with its potential and solver workspace.
+// Re-entrant: everything lives here, nothing in globals.
+struct pisces_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n[3] = {0, 0, 0};
+    double L[3] = {0, 0, 0};
+    long long ngp = 0;
+    int maxn = 0;
+    std::vector<double> x[3], T[3];  // host copies
+    pisces::dbuf dx[3], dT64[3], dT32[3];
+    pisces::dbuf dV64, dV32;
+    bool has_v = false;
+
+    // job ring for H*X launches
+    static constexpr int kJobRing = 256;
+    pisces::dbuf djobs;
+    pisces::HxJob *hjobs = nullptr;  // pinned
+    int job_slot = 0;
+
+    // solver workspace (grown on demand)
+    pisces::dbuf wB, wZ, wU, wZV, wR, wC, wJ, wdiag, wpart, wsmall, winfo, wX, wY;
+    pisces::dbuf wB32, wZ32, wC32;
+    double *hsmall = nullptr;  // pinned, kSmall doubles
+    static constexpr int kSmall = 1 << 16;
+
+    // reverse-communication Davidson state (pisces_davidson); type-erased so
+    // that the state's definition stays in its own translation unit
+    std::shared_ptr<void> rc;
+
+    pisces_ctx() = default;
+    ~pisces_ctx();
+    pisces_ctx(const pisces_ctx &) = delete;
+    pisces_ctx &operator=(const pisces_ctx &) = delete;
+
+    // device job slot for one H*X launch of this grid
+    const pisces::HxJob *push_job(const pisces::HxJob &j);
+    void hx(const void *X, void *Y, long long ld, int nvec, int precision);
+    void sync();
+};
+
+namespace pisces {
+int solve_fp64(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, int corrflag,
+               double *vecs, double *evals, pisces_stats *st);
+int solve_mixed(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, int corrflag,
+                int precision, double *vecs, double *evals, pisces_stats *st);
+}  // namespace pisces

@@ -1,0 +1,278 @@
+// host.cpp -- host-side FP64 pieces of the PISCES MI355X library: sine-DVR
+// kinetic matrices, the rigid-water projection, the potential parameter tables
+// and the small symmetric eigensolver of the projected Davidson matrix.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "blockops.hpp"
+#include "common.hpp"
+
+namespace pisces {
+
+namespace {
+// 15-digit pi of the reference (src/sine_dvr.cpp:24, src/constants.h:4); using
+// it keeps the kinetic matrices and Euler rotations identical to the reference.
+constexpr double kPi = 3.14159265358979;
+constexpr double kBohr2Angs = 0.52917720859;  // src/constants.h:17
+thread_local std::string g_last_error;
+}  // namespace
+
+void set_last_error(const std::string &s) { g_last_error = s; }
+const char *last_error() { return g_last_error.c_str(); }
+
+// sine-DVR grid and full kinetic matrix: src/sine_dvr.cpp:45-131, eq. (B.62),
+// (B.65) of Phys. Rep. 324, 1 (2000).
+void sine_dvr_full(int n, double mass, double x0, double xnp1, double *x, double *T)
+{
+    const double dx = (xnp1 - x0) / (double)(n + 1);
+    for (int j = 0; j < n; ++j) x[j] = x0 + dx * (double)(j + 1);
+    const double f1 = 0.5 * kPi * kPi / (mass * dx * dx);
+    const double f2 = kPi / (double)(n + 1);
+    const double f3 = (double)(n + 1) * (double)(n + 1);
+    for (int a = 0; a < n; ++a) {
+        const double s = std::sin((a + 1) * f2);
+        T[a + n * a] = f1 * (1.0 / 3.0 + 1.0 / (6.0 * f3) - 0.5 / (f3 * (s * s)));
+    }
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b < a; ++b) {
+            const double dc = std::cos(f2 * (a + 1)) - std::cos(f2 * (b + 1));
+            double t = f1 * 2.0 / f3 * std::sin(f2 * (a + 1)) * std::sin(f2 * (b + 1)) / (dc * dc);
+            if ((a - b) % 2 == 1) t = -t;
+            T[a + n * b] = t;
+            T[b + n * a] = t;
+        }
+}
+
+// ------------------------------------------------------------------ water
+// Standard orientation of the DPP monomer relative to its centre of mass, bohr:
+// Water::SetupCharge case 2 (src/Water.cpp:216-236) with src/ttm_data.h:29-44.
+static void monomer_ref(double ref[4][3])
+{
+    const double a2b = 1.0 / kBohr2Angs;
+    const double oz = 0.0650980307, hz = -0.5207842453, hy = 0.756950327, dm = 0.25;
+    const double r[4][3] = {{0, 0, oz * a2b}, {0, hy * a2b, hz * a2b}, {0, -hy * a2b, hz * a2b},
+                            {0, 0, (oz - dm) * a2b}};
+    std::memcpy(ref, r, sizeof(r));
+}
+
+// Water::Reposition (src/Water.cpp:695-825): molecular frame from the O->H
+// vectors, Euler angles, centre of mass with masses 16/1/1.
+static void monomer_project(const double *w, double conf[6])
+{
+    const double o[3] = {w[0], w[1], w[2]};
+    double r1[3], r3[3], z[3], y[3], x[3];
+    for (int a = 0; a < 3; ++a) { r1[a] = w[3 + a] - o[a]; r3[a] = w[6 + a] - o[a]; }
+    for (int a = 0; a < 3; ++a) z[a] = r1[a] + r3[a];
+    const double zm = std::sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+    for (int a = 0; a < 3; ++a) { z[a] /= zm; z[a] = -z[a]; }
+    const double p = r1[0] * z[0] + r1[1] * z[1] + r1[2] * z[2];
+    for (int a = 0; a < 3; ++a) y[a] = r1[a] - p * z[a];
+    const double ym = std::sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+    for (int a = 0; a < 3; ++a) y[a] /= ym;
+    x[0] = y[1] * z[2] - y[2] * z[1];
+    x[1] = y[2] * z[0] - y[0] * z[2];
+    x[2] = y[0] * z[1] - y[1] * z[0];
+    double phi, theta, psi;
+    theta = std::acos(z[2]);
+    const double st = std::sin(theta);
+    if (std::fabs(st) > 1e-5) {
+        double tc = -x[2] / st;
+        double ac = std::acos(tc);
+        if (tc <= -1) ac = kPi;
+        psi = (y[2] >= 0) ? ac : 2 * kPi - ac;
+        tc = z[0] / st;
+        if (tc > 1) tc = 1;
+        if (tc < -1) tc = -1;
+        phi = (z[1] >= 0) ? std::acos(tc) : 2 * kPi - std::acos(tc);
+    } else {
+        phi = 0.0;
+        psi = (x[1] >= 0) ? std::acos(y[1]) : 2 * kPi - std::acos(y[1]);
+    }
+    conf[0] = (w[3] + w[6] + 16.0 * o[0]) / 18.0;
+    conf[1] = (w[4] + w[7] + 16.0 * o[1]) / 18.0;
+    conf[2] = (w[5] + w[8] + 16.0 * o[2]) / 18.0;
+    conf[3] = phi;
+    conf[4] = theta;
+    conf[5] = psi;
+}
+
+// Water::UpdatePositions = ResetPositions + ApplyEuler (src/Water.cpp:837-943)
+static void monomer_place(const double conf[6], double out[4][3])
+{
+    double ref[4][3];
+    monomer_ref(ref);
+    const double c1 = std::cos(conf[3]), c2 = std::cos(conf[4]), c3 = std::cos(conf[5]);
+    const double s1 = std::sin(conf[3]), s2 = std::sin(conf[4]), s3 = std::sin(conf[5]);
+    const double R[9] = {c1 * c2 * c3 - s1 * s3, -c1 * c2 * s3 - s1 * c3, c1 * s2,
+                         s1 * c2 * c3 + c1 * s3, -s1 * c2 * s3 + c1 * c3, s1 * s2,
+                         -s2 * c3,               s2 * s3,                 c2};
+    for (int a = 0; a < 4; ++a) {
+        const double p0 = conf[0] + ref[a][0], p1 = conf[1] + ref[a][1], p2 = conf[2] + ref[a][2];
+        const double o0 = p0 - conf[0], o1 = p1 - conf[1], o2 = p2 - conf[2];
+        out[a][0] = R[0] * o0 + R[1] * o1 + R[2] * o2 + conf[0];
+        out[a][1] = R[3] * o0 + R[4] * o1 + R[5] * o2 + conf[1];
+        out[a][2] = R[6] * o0 + R[7] * o1 + R[8] * o2 + conf[2];
+    }
+}
+
+bool water_sites_host(int nw, const double *xyz_ang, double *sites)
+{
+    const double a2b = 1.0 / kBohr2Angs;
+    for (int w = 0; w < nw; ++w) {
+        double b[9], c[6], pos[4][3];
+        for (int a = 0; a < 9; ++a) b[a] = xyz_ang[9 * w + a] * a2b;
+        monomer_project(b, c);
+        monomer_place(c, pos);
+        for (int s = 0; s < 4; ++s)
+            for (int a = 0; a < 3; ++a) {
+                if (!std::isfinite(pos[s][a])) return false;
+                sites[12 * w + 3 * s + a] = pos[s][a];
+            }
+    }
+    return true;
+}
+
+// GetLists / ReportInducedDipoles ordering (src/DPP.cpp:402-529): charges on
+// H1, H2, M of each water; induced dipoles on O, H1, H2.
+void water_lists_host(int nw, std::vector<double> &q, std::vector<int> &iq, std::vector<int> &id)
+{
+    const double qh = 0.5742185854046506;  // src/ttm_data.h:14
+    q.clear(); iq.clear(); id.clear();
+    for (int w = 0; w < nw; ++w) {
+        q.push_back(qh); iq.push_back(4 * w + 1);
+        q.push_back(qh); iq.push_back(4 * w + 2);
+        q.push_back(-2.0 * qh); iq.push_back(4 * w + 3);
+        for (int s = 0; s < 3; ++s) id.push_back(4 * w + s);
+    }
+}
+
+// ------------------------------------------------------------------ potential
+// Potential::SetupDPPGTOP (src/Potential.cpp:234-593) with the repulsive cores
+// Set6GTORepCore / Set12GTORepCore / Set4STORepCore (:598-728).  The numeric
+// tables are the reference's published model parameters.
+bool potential_setup(const int *flags, const double *pp, int nsites, PotParams &P,
+                     std::vector<double> &rep, std::vector<int> &irep, std::vector<int> &ipps)
+{
+    const int pf = flags[0];
+    if (pf < 1 || pf > 4) return false;
+    if (nsites % 4) return false;
+    P.damp_type = flags[1];
+    P.pol_type = flags[3];
+    P.adiabatic = flags[4];
+    if (P.pol_type != 0 && P.pol_type != 1) return false;
+    P.alpha = pp[0];
+    const double vrep = pp[2], vrepO = (flags[2] == 1) ? pp[5] : pp[2];
+    if (P.damp_type == 1) {
+        P.qdamp = 1.0 / (pp[3] * pp[3]);
+        P.ddamp = 1.0 / (pp[4] * pp[4]);
+        P.pdamp = 1.0 / (pp[1] * pp[1]);
+    } else if (P.damp_type == 2) {
+        P.qdamp = std::fabs(pp[3]);
+        P.ddamp = std::fabs(pp[4]);
+        P.pdamp = std::fabs(pp[1]);
+    } else {
+        return false;
+    }
+    P.eps = pp[6];
+    const int nw = nsites / 4;
+    ipps.clear();
+    if (P.pol_type == 1)
+        for (int w = 0; w < nw; ++w) ipps.push_back(4 * w + 3);  // M sites
+    rep.clear(); irep.clear();
+    P.rep_type = (pf == 3) ? 2 : (pf == 4) ? 0 : 1;
+    const double sH = 1.0 / vrep, sO = 1.0 / vrepO;
+    for (int w = 0; w < nw; ++w) {
+        if (pf == 1) {
+            const double e6[3] = {33.87, 5.095, 1.159};
+            const double c6[3] = {2.84230876640242, 4.55401933663180, 6.91243747981459};
+            for (int h = 0; h < 2; ++h)
+                for (int k = 0; k < 3; ++k) {
+                    irep.push_back(4 * w + 1 + h);
+                    rep.push_back(e6[k]);
+                    rep.push_back(c6[k] * (1.0 / vrep));
+                }
+        } else if (pf == 2) {
+            const double eo[6] = {322.037, 48.4308, 10.4206, 7.40294, 1.5762, 0.373684};
+            const double co[6] = {41.69216064, 58.35946922, 35.19878892, -3.27944113, 4.44843633, 2.36213092};
+            const double eh[3] = {5.447178, 0.824547, 0.183192};
+            const double ch[3] = {0.69233452, 0.66686241, 0.07193908};
+            for (int k = 0; k < 6; ++k) { irep.push_back(4 * w); rep.push_back(eo[k]); rep.push_back(co[k] * sO); }
+            for (int h = 0; h < 2; ++h)
+                for (int k = 0; k < 3; ++k) {
+                    irep.push_back(4 * w + 1 + h);
+                    rep.push_back(eh[k]);
+                    rep.push_back(ch[k] * sH);
+                }
+        } else if (pf == 3) {
+            irep.push_back(4 * w); rep.push_back(9.98526); rep.push_back(192.608 * sO);
+            irep.push_back(4 * w); rep.push_back(1.67075); rep.push_back(13.8937 * sO);
+            irep.push_back(4 * w + 1); rep.push_back(1.64121); rep.push_back(1.7643 * sH);
+            irep.push_back(4 * w + 2); rep.push_back(1.64121); rep.push_back(1.7643 * sH);
+        }
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ eigensolver
+void syev_host(int n, const double *A, int lda, double *W, double *V, int ldv)
+{
+    std::vector<double> a((size_t)n * n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i <= j; ++i) {
+            const double x = A[i + (size_t)lda * j];
+            a[i + (size_t)n * j] = x;
+            a[j + (size_t)n * i] = x;
+        }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) V[i + (size_t)ldv * j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, tot = 0;
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const double v = a[i + (size_t)n * j];
+                tot += v * v;
+                if (i != j) off += v * v;
+            }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a[p + (size_t)n * q];
+                if (apq == 0.0) continue;
+                const double app = a[p + (size_t)n * p], aqq = a[q + (size_t)n * q];
+                const double th = (aqq - app) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = a[k + (size_t)n * p], akq = a[k + (size_t)n * q];
+                    a[k + (size_t)n * p] = c * akp - s * akq;
+                    a[k + (size_t)n * q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = a[p + (size_t)n * k], aqk = a[q + (size_t)n * k];
+                    a[p + (size_t)n * k] = c * apk - s * aqk;
+                    a[q + (size_t)n * k] = s * apk + c * aqk;
+                }
+                a[p + (size_t)n * q] = a[q + (size_t)n * p] = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    const double vp = V[k + (size_t)ldv * p], vq = V[k + (size_t)ldv * q];
+                    V[k + (size_t)ldv * p] = c * vp - s * vq;
+                    V[k + (size_t)ldv * q] = s * vp + c * vq;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) W[i] = a[i + (size_t)n * i];
+    for (int i = 0; i < n - 1; ++i) {
+        int m = i;
+        for (int j = i + 1; j < n; ++j)
+            if (W[j] < W[m]) m = j;
+        if (m != i) {
+            std::swap(W[i], W[m]);
+            for (int k = 0; k < n; ++k) std::swap(V[k + (size_t)ldv * i], V[k + (size_t)ldv * m]);
+        }
+    }
+}
+
+}  // namespace pisces

@@ -1,0 +1,233 @@
+"""Python binding of the MI355X PISCES library (lib/libpisces_hip.so) via ctypes.
+
+The library is the product; this module only marshals numpy arrays through the
+C ABI of include/pisces_hip.h.  It raises if the library is not built: there is
+no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpisces_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "pisces_hip.h")
+
+FP64, FP32, BF16X2 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x2": BF16X2}
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+class PiscesError(RuntimeError):
+    pass
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("nconv", ctypes.c_int),
+        ("n_matvec", ctypes.c_int),
+        ("n_matvec_lo", ctypes.c_int),
+        ("n_iter", ctypes.c_int),
+        ("n_macro", ctypes.c_int),
+        ("seconds", ctypes.c_double),
+        ("resid", ctypes.c_double * 64),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PiscesError("pisces-1_amd/lib/libpisces_hip.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.pisces_version.restype = ctypes.c_char_p
+        L.pisces_strerror.restype = ctypes.c_char_p
+        L.pisces_last_error.restype = ctypes.c_char_p
+        L.pisces_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), _I, _D, ctypes.c_int]
+        L.pisces_destroy.argtypes = [ctypes.c_void_p]
+        L.pisces_destroy.restype = None
+        for name in ("pisces_grid", "pisces_set_potential", "pisces_get_potential",
+                     "pisces_apply_h", "pisces_diag", "pisces_piab", "pisces_solve",
+                     "pisces_build_potential", "pisces_build_cluster_potential",
+                     "pisces_apply_h_device"):
+            getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_I)
+
+
+def check(rc, what=""):
+    if rc < 0:
+        msg = lib().pisces_last_error().decode()
+        raise PiscesError("%s failed: %s (%s)" % (what, lib().pisces_strerror(rc).decode(), msg))
+    return rc
+
+
+def exported_symbols():
+    """Function names declared in include/pisces_hip.h."""
+    import re
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pisces_[a-z0-9_]+)\s*\(", txt)))
+
+
+def set_hx_variant(v):
+    check(lib().pisces_debug_set_hx_variant(int(v)), "set_hx_variant")
+
+
+class Context:
+    """One sine-DVR grid on one GPU (pisces_create)."""
+
+    def __init__(self, n, L, device=0):
+        self.n = tuple(int(v) for v in n)
+        self.L = tuple(float(v) for v in L)
+        self.ngp = self.n[0] * self.n[1] * self.n[2]
+        h = ctypes.c_void_p()
+        na = np.array(self.n, dtype=np.int32)
+        La = np.array(self.L, dtype=np.float64)
+        check(lib().pisces_create(ctypes.byref(h), _pi(na), _p(La), int(device)), "pisces_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pisces_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def grid(self):
+        xyz = np.zeros(sum(self.n))
+        T = np.zeros(sum(v * v for v in self.n))
+        check(lib().pisces_grid(self.h, _p(xyz), _p(T)), "pisces_grid")
+        xs, Ts, ox, ot = [], [], 0, 0
+        for v in self.n:
+            xs.append(xyz[ox:ox + v].copy())
+            Ts.append(T[ot:ot + v * v].reshape(v, v).copy())
+            ox += v
+            ot += v * v
+        return xs, Ts
+
+    def set_potential(self, V):
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
+        assert V.size == self.ngp
+        check(lib().pisces_set_potential(self.h, _p(V)), "pisces_set_potential")
+
+    def get_potential(self):
+        V = np.zeros(self.ngp)
+        check(lib().pisces_get_potential(self.h, _p(V)), "pisces_get_potential")
+        return V
+
+    def build_potential(self, flags, potpara, sites, q, iq, d, idd, ions=None, ion_r=2.9):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        pp = np.ascontiguousarray(potpara, dtype=np.float64)
+        s = np.ascontiguousarray(sites, dtype=np.float64).reshape(-1)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        iq = np.ascontiguousarray(iq, dtype=np.int32)
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1)
+        idd = np.ascontiguousarray(idd, dtype=np.int32)
+        if ions is None:
+            ix, iqq, nion = np.zeros(3), np.zeros(1), 0
+        else:
+            ix = np.ascontiguousarray(ions[0], dtype=np.float64).reshape(-1)
+            iqq = np.ascontiguousarray(ions[1], dtype=np.float64).reshape(-1)
+            nion = iqq.size
+        check(lib().pisces_build_potential(
+            self.h, _pi(f), _p(pp), s.size // 3, _p(s), q.size, _p(q), _pi(iq), idd.size, _p(d),
+            _pi(idd), nion, _p(ix), _p(iqq), ctypes.c_double(ion_r)), "pisces_build_potential")
+
+    def build_cluster_potential(self, xyz_ang, flags=(3, 2, 0, 1, 0, 0), potpara=None,
+                                ions=None, ion_r=2.9):
+        """Water model (GPU dipole SCF) + grid potential; returns (sites, mu, rc)."""
+        xyz = np.ascontiguousarray(xyz_ang, dtype=np.float64).reshape(-1)
+        nw = xyz.size // 9
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        pp = np.ascontiguousarray(default_potpara() if potpara is None else potpara, dtype=np.float64)
+        sites = np.zeros(12 * nw)
+        mu = np.zeros(9 * nw)
+        if ions is None:
+            ix, iqq, nion = np.zeros(3), np.zeros(1), 0
+        else:
+            ix = np.ascontiguousarray(ions[0], dtype=np.float64).reshape(-1)
+            iqq = np.ascontiguousarray(ions[1], dtype=np.float64).reshape(-1)
+            nion = iqq.size
+        rc = check(lib().pisces_build_cluster_potential(
+            self.h, nw, _p(xyz), _pi(f), _p(pp), nion, _p(ix), _p(iqq), ctypes.c_double(ion_r),
+            _p(sites), _p(mu)), "pisces_build_cluster_potential")
+        return sites.reshape(-1, 3), mu, rc
+
+    def apply_h(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        nvec = 1 if X.ndim == 1 else X.shape[0]
+        Y = np.zeros_like(X)
+        check(lib().pisces_apply_h(self.h, _p(X), _p(Y), nvec), "pisces_apply_h")
+        return Y
+
+    def apply_h_device(self, dX, dY, nvec, precision):
+        """dX, dY: torch CUDA tensors (or raw device pointers as ints)."""
+        px = dX if isinstance(dX, int) else dX.data_ptr()
+        py = dY if isinstance(dY, int) else dY.data_ptr()
+        check(lib().pisces_apply_h_device(self.h, ctypes.c_void_p(px), ctypes.c_void_p(py),
+                                          int(nvec), int(precision)), "pisces_apply_h_device")
+
+    def diag(self):
+        D = np.zeros(self.ngp)
+        check(lib().pisces_diag(self.h, _p(D)), "pisces_diag")
+        return D
+
+    def piab(self):
+        wf = np.zeros(self.ngp)
+        check(lib().pisces_piab(self.h, _p(wf)), "pisces_piab")
+        return wf
+
+    def solve(self, nroots, start, maxsub=20, maxmacro=100, ptol=7, corrflag=1,
+              precision=FP64):
+        vecs = np.ascontiguousarray(np.asarray(start, dtype=np.float64)[:nroots]).copy()
+        assert vecs.shape == (nroots, self.ngp)
+        ev = np.zeros(nroots)
+        st = Stats()
+        rc = check(lib().pisces_solve(self.h, nroots, maxsub, maxmacro, ptol, corrflag,
+                                      int(precision), _p(vecs), _p(ev), ctypes.byref(st)),
+                   "pisces_solve")
+        return dict(nconv=rc, evals=ev, vecs=vecs, n_matvec=st.n_matvec,
+                    n_matvec_lo=st.n_matvec_lo, n_iter=st.n_iter, n_macro=st.n_macro,
+                    seconds=st.seconds, resid=np.array(st.resid[:nroots]))
+
+
+def default_potpara():
+    """PotPara defaults for Potential 1-4 (src/GetInput.cpp:53-61)."""
+    pp = np.zeros(11)
+    pp[:7] = [9.745, 2.536, 4.4837, 1.7724, 1.7724, 4.4837, 0.0]
+    pp[6] = 0.5 / np.sqrt(pp[0])
+    pp[10] = 20.0
+    return pp
+
+
+def start_vectors(ctx, nroots, seed=0):
+    """Half-box particle-in-a-box vector plus seeded uniform [-1,1] vectors, as
+    DVR::Diagonalize prepares them for StartVector=1 (src/DVR.cpp:408-453)."""
+    rng = np.random.default_rng(seed)
+    S = np.zeros((nroots, ctx.ngp))
+    S[0] = ctx.piab()
+    for r in range(1, nroots):
+        S[r] = rng.uniform(-1, 1, size=ctx.ngp)
+    return S

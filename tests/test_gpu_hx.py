@@ -1,0 +1,82 @@
+"""H*X on the GPU (pisces_apply_h / pisces_apply_h_device) against the oracle's
+restatement of DVR::MatrixTimesVector (src/mtx.cpp:22-108)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [((12, 11, 10), (22.0, 21.0, 20.0)), ((32, 32, 32), (36.0, 36.0, 36.0)),
+          ((64, 64, 64), (40.0, 40.0, 40.0)), ((48, 40, 33), (44.0, 40.0, 36.0))]
+
+
+def _case(oracle_mod, n, L, nvec, seed):
+    g = oracle_mod.Grid(n, L)
+    rng = np.random.default_rng(seed)
+    V = rng.uniform(-1.0, 0.5, g.ngp)
+    X = rng.uniform(-1.0, 1.0, (nvec, g.ngp))
+    return g, V, X, g.apply_h(V, X)
+
+
+@pytest.mark.parametrize("variant", [1, 0])
+@pytest.mark.parametrize("n,L", SHAPES)
+def test_apply_h_fp64(pisces, oracle_mod, n, L, variant):
+    print("hx fp64", n, "variant", variant, flush=True)
+    g, V, X, Yo = _case(oracle_mod, n, L, 3, 1)
+    pisces.set_hx_variant(variant)
+    try:
+        with pisces.Context(n, L) as ctx:
+            ctx.set_potential(V)
+            Y = ctx.apply_h(X)
+    finally:
+        pisces.set_hx_variant(1)
+    # FP64 with a different summation order: a few ulps of the row sums
+    assert np.abs(Y - Yo).max() <= 1e-13 * np.abs(Yo).max() * 10
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-6), ("bf16x2", 2e-4)])
+@pytest.mark.parametrize("n,L", SHAPES)
+def test_apply_h_low_precision(pisces, oracle_mod, n, L, precision, tol):
+    import torch
+    print("hx", precision, n, flush=True)
+    g, V, X, Yo = _case(oracle_mod, n, L, 4, 2)
+    with pisces.Context(n, L) as ctx:
+        ctx.set_potential(V)
+        dX = torch.tensor(X, dtype=torch.float32, device="cuda")
+        dY = torch.zeros_like(dX)
+        torch.cuda.synchronize()
+        ctx.apply_h_device(dX, dY, X.shape[0], pisces.PRECISIONS[precision])
+        Y = dY.cpu().numpy().astype(np.float64)
+    # reference for the rounded input: the FP32 input itself is exact in FP64
+    Yr = g.apply_h(V, X.astype(np.float32).astype(np.float64))
+    err = np.abs(Y - Yr).max() / np.abs(Yr).max()
+    assert err <= tol, err
+
+
+def test_apply_h_fp64_device_matches_host(pisces, oracle_mod):
+    import torch
+    n, L = (32, 32, 32), (36.0, 36.0, 36.0)
+    g, V, X, Yo = _case(oracle_mod, n, L, 2, 3)
+    with pisces.Context(n, L) as ctx:
+        ctx.set_potential(V)
+        dX = torch.tensor(X, dtype=torch.float64, device="cuda")
+        dY = torch.zeros_like(dX)
+        torch.cuda.synchronize()
+        ctx.apply_h_device(dX, dY, 2, pisces.FP64)
+        Yd = dY.cpu().numpy()
+        Yh = ctx.apply_h(X)
+    np.testing.assert_array_equal(Yd, Yh)
+
+
+def test_apply_h_linearity_and_symmetry(pisces, oracle_mod):
+    """Size-independent properties at the benchmark size: H(ax+by) = aHx + bHy
+    and <u, Hv> = <Hu, v>."""
+    n, L = (64, 64, 64), (40.0, 40.0, 40.0)
+    rng = np.random.default_rng(9)
+    ng = 64 ** 3
+    V = rng.uniform(-1, 0.5, ng)
+    u, v = rng.uniform(-1, 1, (2, ng))
+    with pisces.Context(n, L) as ctx:
+        ctx.set_potential(V)
+        Hu, Hv, Hw = ctx.apply_h(np.stack([u, v, 0.3 * u - 1.7 * v]))
+    np.testing.assert_allclose(Hw, 0.3 * Hu - 1.7 * Hv, rtol=0, atol=1e-12 * np.abs(Hw).max())
+    assert abs(u @ Hv - Hu @ v) <= 1e-12 * abs(u @ Hv)

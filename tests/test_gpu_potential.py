@@ -1,0 +1,74 @@
+"""Grid potential and water model on the GPU against the oracle (whose
+Potential restatement is pinned bit-for-bit to the reference's
+Potential::Evaluate by tests/test_oracle_golden.py)."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+FLAGSETS = ["default", "adiabatic", "gaussdamp", "gto6", "gto12", "gto12_sigo", "norep",
+            "nopol", "internal1"]
+
+
+@pytest.mark.parametrize("flagset", FLAGSETS)
+def test_build_potential_lists(pisces, oracle_mod, flagset):
+    f = golden("potential_c1.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    n, L = (32, 32, 32), (36.0, 36.0, 36.0)
+    g = oracle_mod.Grid(n, L)
+    flags, pp = f["flags_" + flagset], f["potpara_" + flagset]
+    Vo = cl.potential_grid(g, flags, pp)
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential(flags, pp, cl.sites, cl.q, cl.iq, cl.d, cl.id)
+        V = ctx.get_potential()
+    assert np.abs(V - Vo).max() <= 1e-13 * np.abs(Vo).max()
+
+
+@pytest.mark.parametrize("nw,rad,seed,n,L", [(6, 3.5, 1000, (32, 32, 32), 36.0),
+                                             (24, 6.0, 2000, (64, 64, 64), 40.0),
+                                             (45, 8.0, 3000, (48, 48, 48), 48.0)])
+def test_cluster_potential_gpu_water_model(pisces, oracle_mod, nw, rad, seed, n, L):
+    import workload
+    print("cluster", nw, n, flush=True)
+    xyz, _ = workload.water_cluster(nw, rad, seed)
+    cl = oracle_mod.Cluster(xyz)
+    g = oracle_mod.Grid(n, (L, L, L))
+    Vo = cl.potential_grid(g)
+    with pisces.Context(n, (L, L, L)) as ctx:
+        sites, mu, rc = ctx.build_cluster_potential(xyz)
+        V = ctx.get_potential()
+    assert rc == 0
+    np.testing.assert_array_equal(sites, cl.sites)
+    assert np.abs(mu - cl.mu).max() <= 1e-10 * np.abs(cl.mu).max()
+    assert np.abs(V - Vo).max() <= 1e-11 * np.abs(Vo).max()
+
+
+def test_cluster_potential_with_ion(pisces, oracle_mod):
+    import workload
+    xyz, ion = workload.water_cluster(10, 5.0, 3100, ion_min=2.3)
+    ion_b = np.asarray(ion) * workload.ANGS2BOHR
+    n, L = (32, 32, 32), (40.0, 40.0, 40.0)
+    cl = oracle_mod.Cluster(xyz, ions=(ion_b, [1.0]), ion_r=2.9)
+    g = oracle_mod.Grid(n, L)
+    Vo = cl.potential_grid(g)
+    with pisces.Context(n, L) as ctx:
+        ctx.build_cluster_potential(xyz, ions=(ion_b, [1.0]), ion_r=2.9)
+        V = ctx.get_potential()
+    assert np.abs(V - Vo).max() <= 1e-11 * np.abs(Vo).max()
+
+
+def test_diag_and_piab(pisces, oracle_mod):
+    n, L = (20, 18, 16), (30.0, 28.0, 26.0)
+    g = oracle_mod.Grid(n, L)
+    rng = np.random.default_rng(4)
+    V = rng.normal(size=g.ngp)
+    with pisces.Context(n, L) as ctx:
+        ctx.set_potential(V)
+        np.testing.assert_array_equal(ctx.diag(), g.diag(V))
+        np.testing.assert_array_equal(ctx.piab(), g.piab())
+        xs, Ts = ctx.grid()
+    for a in range(3):
+        np.testing.assert_array_equal(xs[a], g.x[a])
+        np.testing.assert_array_equal(Ts[a], g.T[a])
