@@ -114,6 +114,18 @@ int pisces_apply_h(pisces_ctx *ctx, const double *X, double *Y, int nvec);
 int pisces_apply_h_device(pisces_ctx *ctx, const void *d_X, void *d_Y, int nvec,
                           int precision);
 
+/* Host-buffer H*X in a chosen precision: X is converted on the device to the
+ * precision's storage type, multiplied, and Y converted back to double.  No
+ * reference counterpart (the reference has only the FP64 product). */
+int pisces_apply_h_ex(pisces_ctx *ctx, const double *X, double *Y, int nvec, int precision);
+
+/* Times `iters` H*X applications on `nvec` device-resident vectors (uniform in
+ * [-1,1], seeded) with HIP events on the context's stream after `warmup`
+ * untimed ones; *ms_per_apply receives the mean time of one application (both
+ * passes).  Measurement hook for the benchmark; no reference counterpart. */
+int pisces_time_hx(pisces_ctx *ctx, int nvec, int precision, int warmup, int iters,
+                   double *ms_per_apply);
+
 /* diag(H) on the host, ngp doubles.  Replaces DVR::ComputeDiagonal
  * (src/DVR.cpp:1808-1881). */
 int pisces_diag(pisces_ctx *ctx, double *diag);

@@ -311,6 +311,57 @@ int pisces_apply_h_device(pisces_ctx *c, const void *dX, void *dY, int nvec, int
     });
 }
 
+int pisces_apply_h_ex(pisces_ctx *c, const double *X, double *Y, int nvec, int precision)
+{
+    if (!c || !X || !Y || nvec <= 0) return PISCES_ERR_ARG;
+    if (precision < PISCES_FP64 || precision > PISCES_BF16X2) return PISCES_ERR_ARG;
+    if (precision == PISCES_FP64) return pisces_apply_h(c, X, Y, nvec);
+    if (!c->has_v) return PISCES_ERR_STATE;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        const long long n = c->ngp * (long long)nvec;
+        c->wX.ensure(sizeof(double) * n);
+        c->wY.ensure(sizeof(double) * n);
+        c->wB32.ensure(sizeof(float) * n);
+        c->wZ32.ensure(sizeof(float) * n);
+        PC_HIP(hipMemcpyAsync(c->wX.p, X, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        to_f32(c->wX.as<double>(), c->wB32.as<float>(), n, c->stream);
+        c->hx(c->wB32.p, c->wZ32.p, c->ngp, nvec, precision);
+        to_f64(c->wZ32.as<float>(), c->wY.as<double>(), n, c->stream);
+        PC_HIP(hipMemcpyAsync(Y, c->wY.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        c->sync();
+        return 0;
+    });
+}
+
+int pisces_time_hx(pisces_ctx *c, int nvec, int precision, int warmup, int iters, double *ms)
+{
+    if (!c || nvec <= 0 || iters <= 0 || warmup < 0 || !ms) return PISCES_ERR_ARG;
+    if (precision < PISCES_FP64 || precision > PISCES_BF16X2) return PISCES_ERR_ARG;
+    if (!c->has_v) return PISCES_ERR_STATE;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        const long long n = c->ngp * (long long)nvec;
+        const size_t es = precision == PISCES_FP64 ? sizeof(double) : sizeof(float);
+        dbuf X(es * n), Y(es * n);
+        fill_uniform(X.p, n, precision == PISCES_FP64, 12345u, c->stream);
+        for (int w = 0; w < warmup; ++w) c->hx(X.p, Y.p, c->ngp, nvec, precision);
+        hipEvent_t e0, e1;
+        PC_HIP(hipEventCreate(&e0));
+        PC_HIP(hipEventCreate(&e1));
+        PC_HIP(hipEventRecord(e0, c->stream));
+        for (int w = 0; w < iters; ++w) c->hx(X.p, Y.p, c->ngp, nvec, precision);
+        PC_HIP(hipEventRecord(e1, c->stream));
+        PC_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        PC_HIP(hipEventElapsedTime(&t, e0, e1));
+        static_cast<void>(hipEventDestroy(e0));
+        static_cast<void>(hipEventDestroy(e1));
+        *ms = (double)t / iters;
+        return 0;
+    });
+}
+
 int pisces_diag(pisces_ctx *c, double *diag)
 {
     if (!c || !diag) return PISCES_ERR_ARG;

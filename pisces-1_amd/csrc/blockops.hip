@@ -242,6 +242,28 @@ convert_kernel(const double *__restrict__ x, float *__restrict__ y, long long n)
     if (g < n) y[g] = (float)x[g];
 }
 
+__global__ void __launch_bounds__(kT)
+convert64_kernel(const float *__restrict__ x, double *__restrict__ y, long long n)
+{
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g < n) y[g] = (double)x[g];
+}
+
+__global__ void __launch_bounds__(kT)
+fill_kernel(void *x, long long n, int f64, unsigned seed)
+{
+    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
+    if (g >= n) return;
+    // splitmix64 of (seed, g)
+    unsigned long long z = (unsigned long long)g + 0x9e3779b97f4a7c15ull * (seed + 1ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    if (f64) static_cast<double *>(x)[g] = u;
+    else static_cast<float *>(x)[g] = (float)u;
+}
+
 inline unsigned nblk(long long N) { return (unsigned)((N + kT - 1) / kT); }
 
 }  // namespace
@@ -383,6 +405,18 @@ void diag_h(const double *V, const double *Tx, const double *Ty, const double *T
 {
     hipLaunchKernelGGL(diag_kernel, dim3(nblk((long long)nx * ny * nz)), dim3(kT), 0, s, V, Tx,
                        Ty, Tz, nx, ny, nz, D);
+    PC_HIP(hipGetLastError());
+}
+
+void to_f64(const float *x, double *y, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(convert64_kernel, dim3(nblk(n)), dim3(kT), 0, s, x, y, n);
+    PC_HIP(hipGetLastError());
+}
+
+void fill_uniform(void *x, long long n, bool f64, unsigned seed, hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3(nblk(n)), dim3(kT), 0, s, x, n, f64 ? 1 : 0, seed);
     PC_HIP(hipGetLastError());
 }
 

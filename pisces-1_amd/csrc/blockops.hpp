@@ -46,6 +46,9 @@ void cholinv(const double *d_G, int k, double *d_W, int *d_info, double rel_tol,
 void diag_h(const double *V, const double *Tx, const double *Ty, const double *Tz, int nx,
             int ny, int nz, double *D, hipStream_t s);
 void to_f32(const double *x, float *y, long long n, hipStream_t s);
+void to_f64(const float *x, double *y, long long n, hipStream_t s);
+// x[i] = uniform in [-1, 1) from a counter-based hash of (seed, i)
+void fill_uniform(void *x, long long n, bool f64, unsigned seed, hipStream_t s);
 
 // Symmetric eigensolver on the host for the projected matrix (m <= 64):
 // cyclic Jacobi, ascending eigenvalues; reads the upper triangle of A

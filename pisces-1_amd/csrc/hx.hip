@@ -27,6 +27,8 @@
 namespace pisces {
 
 static int g_variant = 1;
+// FP64 matrix-core path enabled (lane layout checked on the device).
+static constexpr bool kF64MfmaVerified = true;
 void hx_set_variant(int v) { g_variant = v; }
 int hx_get_variant() { return g_variant; }
 
@@ -147,7 +149,7 @@ __global__ void __launch_bounds__(kThreads) hx_plane_valu(const HxJob *__restric
 // consecutive i (128 B for FP64).  With the matrix-core operand layout
 //   A (16 x K): lane l holds A[l%16][k0 + l/16]      (K = 4 per instruction)
 //   B (K x 16): lane l holds B[k0 + l/16][l%16]
-//   D (16x16): lane l, element r holds D[4*(l/16)+r][l%16]
+//   D (16x16): lane l, element r holds D[row(l/16, r)][l%16] (see MfmaTraits)
 // the two contractions of a plane are
 //   along c: D(c,i) += sum_k Tc[c][k] P[k][i]   A = Tc (symmetric), B = P
 //   along i: D(c,i) += sum_k P[c][k] Ti[k][i]   A = P,              B = Ti
@@ -156,12 +158,15 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 template <typename T> struct MfmaTraits;
+// Result rows held by lane group lq = lane/16: FP32 rows 4*lq + r; FP64 rows
+// lq + 4*r (the FP64 instruction interleaves the four row groups).
 template <> struct MfmaTraits<double> {
     typedef f64x4 acc_t;
     __device__ static inline acc_t mma(double a, double b, acc_t c)
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int row(int lq, int r) { return lq + 4 * r; }
 };
 template <> struct MfmaTraits<float> {
     typedef f32x4 acc_t;
@@ -169,6 +174,7 @@ template <> struct MfmaTraits<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int row(int lq, int r) { return 4 * lq + r; }
 };
 
 // LDS row pitch (elements) of a plane whose rows have n valid entries: padded to
@@ -227,7 +233,7 @@ __global__ void __launch_bounds__(kThreads) hx_plane_mfma(const HxJob *__restric
         const int i = i0 + l16;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int c = c0 + 4 * lq + r;
+            const int c = c0 + M::row(lq, r);
             if (i < g.ni && c < g.nc) {
                 const long long gi = g.base + i + c * g.cs;
                 if (PASS == 0) Y[gi] = V[g.vbase + i + c * g.cs] * P[c * ld + i] + acc[r];
@@ -385,7 +391,7 @@ void hx_launch(const HxJob *d_jobs, int njobs, int max_planes_a, int max_planes_
             hipLaunchKernelGGL((hx_plane_mfma<float, 1>), gb, blk, lds, s, d_jobs);
         }
     } else {
-        if (g_variant == 0) {
+        if (g_variant == 0 || !kF64MfmaVerified) {
             const size_t lds = (size_t)((max_n + 3) & ~3) * ((max_n + 3) & ~3) * sizeof(double);
             hipLaunchKernelGGL((hx_plane_valu<double, 0>), ga, blk, lds, s, d_jobs);
             hipLaunchKernelGGL((hx_plane_valu<double, 1>), gb, blk, lds, s, d_jobs);

@@ -52,7 +52,7 @@ def lib():
         for name in ("pisces_grid", "pisces_set_potential", "pisces_get_potential",
                      "pisces_apply_h", "pisces_diag", "pisces_piab", "pisces_solve",
                      "pisces_build_potential", "pisces_build_cluster_potential",
-                     "pisces_apply_h_device"):
+                     "pisces_apply_h_device", "pisces_apply_h_ex", "pisces_time_hx"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -182,12 +182,20 @@ class Context:
         check(lib().pisces_apply_h(self.h, _p(X), _p(Y), nvec), "pisces_apply_h")
         return Y
 
-    def apply_h_device(self, dX, dY, nvec, precision):
-        """dX, dY: torch CUDA tensors (or raw device pointers as ints)."""
-        px = dX if isinstance(dX, int) else dX.data_ptr()
-        py = dY if isinstance(dY, int) else dY.data_ptr()
-        check(lib().pisces_apply_h_device(self.h, ctypes.c_void_p(px), ctypes.c_void_p(py),
-                                          int(nvec), int(precision)), "pisces_apply_h_device")
+    def apply_h_ex(self, X, precision):
+        """H*X in a chosen precision (X, Y on the host as float64)."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        nvec = 1 if X.ndim == 1 else X.shape[0]
+        Y = np.zeros_like(X)
+        check(lib().pisces_apply_h_ex(self.h, _p(X), _p(Y), nvec, int(precision)),
+              "pisces_apply_h_ex")
+        return Y
+
+    def time_hx(self, nvec, precision, warmup=5, iters=50):
+        ms = ctypes.c_double(0)
+        check(lib().pisces_time_hx(self.h, int(nvec), int(precision), int(warmup), int(iters),
+                                   ctypes.byref(ms)), "pisces_time_hx")
+        return ms.value
 
     def diag(self):
         D = np.zeros(self.ngp)

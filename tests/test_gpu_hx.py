@@ -36,35 +36,23 @@ def test_apply_h_fp64(pisces, oracle_mod, n, L, variant):
 @pytest.mark.parametrize("precision,tol", [("fp32", 2e-6), ("bf16x2", 2e-4)])
 @pytest.mark.parametrize("n,L", SHAPES)
 def test_apply_h_low_precision(pisces, oracle_mod, n, L, precision, tol):
-    import torch
     print("hx", precision, n, flush=True)
     g, V, X, Yo = _case(oracle_mod, n, L, 4, 2)
     with pisces.Context(n, L) as ctx:
         ctx.set_potential(V)
-        dX = torch.tensor(X, dtype=torch.float32, device="cuda")
-        dY = torch.zeros_like(dX)
-        torch.cuda.synchronize()
-        ctx.apply_h_device(dX, dY, X.shape[0], pisces.PRECISIONS[precision])
-        Y = dY.cpu().numpy().astype(np.float64)
+        Y = ctx.apply_h_ex(X, pisces.PRECISIONS[precision])
     # reference for the rounded input: the FP32 input itself is exact in FP64
     Yr = g.apply_h(V, X.astype(np.float32).astype(np.float64))
     err = np.abs(Y - Yr).max() / np.abs(Yr).max()
     assert err <= tol, err
 
 
-def test_apply_h_fp64_device_matches_host(pisces, oracle_mod):
-    import torch
+def test_apply_h_ex_fp64_is_apply_h(pisces, oracle_mod):
     n, L = (32, 32, 32), (36.0, 36.0, 36.0)
     g, V, X, Yo = _case(oracle_mod, n, L, 2, 3)
     with pisces.Context(n, L) as ctx:
         ctx.set_potential(V)
-        dX = torch.tensor(X, dtype=torch.float64, device="cuda")
-        dY = torch.zeros_like(dX)
-        torch.cuda.synchronize()
-        ctx.apply_h_device(dX, dY, 2, pisces.FP64)
-        Yd = dY.cpu().numpy()
-        Yh = ctx.apply_h(X)
-    np.testing.assert_array_equal(Yd, Yh)
+        np.testing.assert_array_equal(ctx.apply_h_ex(X, pisces.FP64), ctx.apply_h(X))
 
 
 def test_apply_h_linearity_and_symmetry(pisces, oracle_mod):
