@@ -163,6 +163,24 @@ int pisces_solve(pisces_ctx *ctx, int nroots, int maxsub, int maxmacro, int ptol
                  int corrflag, int precision, double *vecs, double *evals,
                  pisces_stats *stats);
 
+/* Block Davidson on the GPU (the solver behind the benchmark and the server):
+ * `block` Ritz pairs are tracked and corrected per iteration (block >= nroots;
+ * the extra ones guard the wanted roots against near-degenerate neighbours),
+ * the subspace holds at most maxsub vectors and is thick-restarted from the
+ * block's Ritz vectors.  precision selects the storage of the correction
+ * basis and the H*X form used on it (FP64, FP32 MFMA or split-BF16 MFMA); the
+ * Ritz block and its H-image are FP64 throughout and, in the low-precision
+ * forms, H*X of the Ritz block is recomputed in FP64 at every restart (defect
+ * correction), so reported eigenvalues and residuals are FP64-exact.
+ * Converged when ||H u_i - theta_i u_i|| < 10^-ptol for i < nroots.
+ * vecs: in nroots start vectors (ngp doubles each; the rest of the block is
+ * seeded random), out nroots Ritz vectors.  Returns converged roots or an
+ * error.  Generalises the reference's one-vector-per-root Davidson
+ * (src/Davidson.cpp:61-376) to blocks; it has no direct counterpart. */
+int pisces_solve_block(pisces_ctx *ctx, int nroots, int block, int maxsub, int max_iter,
+                       int ptol, int precision, double *vecs, double *evals,
+                       pisces_stats *stats);
+
 /* Reverse-communication Davidson with the reference's exact calling
  * convention and semantics (src/Davidson.h:2-15): return 1 asks the caller to
  * set Z[inout[1]*ndim ..] = H B[inout[0]*ndim ..] for inout[2] vectors, return

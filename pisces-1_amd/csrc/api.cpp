@@ -59,18 +59,7 @@ namespace {
 template <typename F>
 int guarded(F &&f)
 {
-    try {
-        return f();
-    } catch (const hip_error &e) {
-        set_last_error(std::string("HIP error ") + hipGetErrorString(e.err) + " in " + e.what);
-        return PISCES_ERR_HIP;
-    } catch (const api_error &e) {
-        set_last_error(e.msg);
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        set_last_error("host allocation failed");
-        return PISCES_ERR_NOMEM;
-    }
+    return guarded_call(std::forward<F>(f));
 }
 
 void upload(dbuf &b, const void *h, size_t bytes, hipStream_t s)

@@ -1,0 +1,77 @@
+// bd.hpp -- batched kernels of the block-Davidson engine.
+//
+// Every launch covers a list of independent tasks (one or more per problem
+// instance); blockIdx.y selects the task.  Vectors are FP64 or FP32 columns
+// (a32/b32/y32 flags, uniform per task), accumulation is always FP64.
+#pragma once
+
+#include "common.hpp"
+
+namespace pisces {
+
+constexpr int kBdMaxM = 64;   // max subspace dimension (block + corrections)
+constexpr int kBdMaxK = 16;   // max block size
+
+// partial Gram sums of A^T B over one chunk of rows; out[a + out_ld * b]
+struct GramTask {
+    const void *A; long long lda; int ma; int a32;
+    const void *B; long long ldb; int mb; int b32;
+    long long N;
+    double *out; int out_ld;
+    long long part_off;  // this task's region of the partial buffer (doubles)
+    int nch;             // workgroups of this task (each sums cpb chunks)
+    int cpb;             // chunks of kBdChunk rows per workgroup
+};
+
+// Y (set|-=|+=) A1 W1 + A2 W2, W in device memory (column-major)
+struct CombTask {
+    const void *A1; long long lda1; int ma1; int a1_32; const double *W1; int ldw1;
+    const void *A2; long long lda2; int ma2; int a2_32; const double *W2; int ldw2;
+    void *Y; long long ldy; int y32; int mb; int mode;
+    long long N;
+};
+
+// Symmetric eigenproblem of the projected matrix: G (m x m, upper triangle,
+// leading dimension ldg) -> lowest k eigenvalues theta and eigenvectors Y (m x k,
+// leading dimension m).  Columns flagged dead (keep[c] == 0 for correction
+// column c) are excluded.
+struct EigTask {
+    const double *G; int ldg; int m; int k;
+    double *theta; double *Y;
+    const int *keep;  // keep mask of all m columns (may be null)
+    int pad0, pad;
+};
+
+// R = HU - theta U for k columns, and partial sums of ||R_i||^2
+struct ResTask {
+    const double *U; const double *HU; double *R; long long ld; int k; int pad;
+    const double *theta; long long N; long long part_off; int nch; int pad2;
+    double *out;     // ||R_i||^2 after reduction
+    double *gather;  // host-bound copy: theta[0..16) then ||R_i||^2 [16..32)
+};
+
+// T[t] = R[src[t]] / (theta[src[t]] - D), |.| >= 1e-12 (Davidson correction)
+struct PrecTask {
+    const double *R; long long ld; const double *theta; const double *D;
+    double *T; long long N; int nt; int src[kBdMaxK];
+};
+
+// W = L^{-T} of the k x k Gram G (CholQR).  keep_out[c] = 0 marks a column
+// whose remaining norm is below tol (relative): its column of W is zero.
+// keep_in (may be null = all kept) carries the mask of an earlier pass.
+struct CholTask {
+    const double *G; int k; int pad; double *W; const int *keep_in; int *keep_out; double tol;
+};
+
+void bd_gram(const GramTask *d_tasks, int ntasks, int max_nch, int max_mamb, int max_cols,
+             double *d_partial, hipStream_t s);
+void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, hipStream_t s);
+void bd_eig(const EigTask *d_tasks, int ntasks, hipStream_t s);
+void bd_resid(const ResTask *d_tasks, int ntasks, int max_nch, double *d_partial, hipStream_t s);
+void bd_precond(const PrecTask *d_tasks, int ntasks, long long max_N, hipStream_t s);
+void bd_cholinv(const CholTask *d_tasks, int ntasks, hipStream_t s);
+
+constexpr int kBdChunk = 128;     // rows per Gram chunk
+constexpr int kBdResChunk = 1024; // rows per residual chunk
+
+}  // namespace pisces

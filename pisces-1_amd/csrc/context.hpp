@@ -2,6 +2,8 @@
 #pragma once
 
 #include <memory>
+#include <new>
+#include <string>
 #include <vector>
 
 #include "blockops.hpp"
@@ -49,6 +51,23 @@ struct pisces_ctx {
 };
 
 namespace pisces {
+// Runs f, converting library exceptions into C-ABI return codes.
+template <typename F>
+int guarded_call(F &&f)
+{
+    try {
+        return f();
+    } catch (const hip_error &e) {
+        set_last_error(std::string("HIP error ") + hipGetErrorString(e.err) + " in " + e.what);
+        return PISCES_ERR_HIP;
+    } catch (const api_error &e) {
+        set_last_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return PISCES_ERR_NOMEM;
+    }
+}
 int solve_fp64(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, int corrflag,
                double *vecs, double *evals, pisces_stats *st);
 int solve_mixed(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, int corrflag,

@@ -1,0 +1,94 @@
+// engine.hpp -- the batched block-Davidson engine.
+//
+// Many independent eigenproblems (grids of different sizes, different
+// potentials) advance together: each Davidson step is one fixed sequence of
+// batched launches over every live instance, with a single device-to-host
+// transfer of the per-root Ritz values and residual norms.  The same engine
+// runs one instance (pisces_solve_block) or many (the server).
+//
+// Algorithm (per instance; k = block size, nroots <= k wanted):
+//   basis  = [X | C]: X the k current Ritz vectors (FP64, orthonormal) with
+//            HX = H X kept in FP64; C the corrections of this macro iteration,
+//            stored in the solve precision with HC = H C from the matching
+//            H*X form (FP64, FP32 MFMA or split-BF16 MFMA).
+//   step   : G = [X|C]^T H [X|C] extended by its new columns (FP64 sums);
+//            lowest k eigenpairs of G; U = [X|C] y, HU = [HX|HC] y (FP64);
+//            R = HU - theta U and ||R|| (FP64); Davidson corrections
+//            R_i / (theta_i - diag H) for the unconverged columns,
+//            orthogonalised against [X|C] (two CGS passes) and among
+//            themselves (Cholesky QR, twice), appended to C.
+//   restart: when C is full: X = U (re-orthonormalised), HX = HU; the
+//            low-precision forms then recompute HX = H X in FP64 (defect
+//            correction), so residuals and eigenvalues are FP64-exact.
+//   converged: ||R_i|| < 10^-ptol for i < nroots, judged on an FP64-fresh HX
+//            in the low-precision forms.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "bd.hpp"
+#include "common.hpp"
+
+namespace pisces {
+
+// Grid data of one instance on the device (non-owning).
+struct BdGrid {
+    int n[3] = {0, 0, 0};
+    long long N = 0;
+    const void *T64[3] = {nullptr, nullptr, nullptr};
+    const void *T32[3] = {nullptr, nullptr, nullptr};
+    const double *V64 = nullptr;
+    const float *V32 = nullptr;
+    const double *D = nullptr;  // diag(H)
+};
+
+struct BdResult {
+    int status = 0;   // >= 0: converged roots; < 0 error
+    int nroots = 0;
+    std::vector<double> evals, resid;
+    int n_iter = 0, n_mv64 = 0, n_mvlo = 0, n_restart = 0;
+};
+
+class BdEngine {
+public:
+    BdEngine(int device, hipStream_t s, int precision, int block, int maxsub, int ptol,
+             int max_iter);
+    ~BdEngine();
+    // Adds an instance; start: nroots host vectors (more columns are filled
+    // with seeded random vectors up to the block size).  Returns its slot.
+    int add(const BdGrid &g, int nroots, const double *start, int nstart, unsigned seed,
+            std::shared_ptr<void> keepalive = nullptr);
+    // Advances every live instance by one step; returns the number still live.
+    int step();
+    bool finished(int slot) const;
+    const BdResult &result(int slot) const;
+    // Ritz vectors of a finished instance (nroots x N doubles, host).
+    void vectors(int slot, double *out);
+    void release(int slot);
+    size_t bytes_per_instance(long long N) const;
+    int live() const;
+    int precision() const { return prec_; }
+
+private:
+    struct Inst;
+    void run_restarts(std::vector<Inst *> &v);
+    std::vector<std::unique_ptr<Inst>> inst_;
+    int dev_;
+    hipStream_t s_;
+    int prec_, k_, maxsub_, ptol_, max_iter_;
+    // device task arrays and their pinned host staging, one region per stage
+    dbuf dtask_;
+    unsigned char *htask_ = nullptr;
+    size_t task_cap_ = 0;
+    dbuf dpart_;
+    dbuf dgather_;
+    double *hsmall_ = nullptr;
+    size_t hsmall_cap_ = 0;
+    template <typename T> T *stage(int region, int count, T **dptr);
+    void ensure_tasks(size_t per_region_bytes);
+    void grow_part(size_t bytes);
+    size_t region_bytes_ = 0;
+};
+
+}  // namespace pisces

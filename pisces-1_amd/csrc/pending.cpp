@@ -2,13 +2,6 @@
 // order; they fail loudly instead of silently doing nothing.
 #include "context.hpp"
 
-namespace pisces {
-int solve_mixed(pisces_ctx *, int, int, int, int, int, int, double *, double *, pisces_stats *)
-{
-    throw api_error{PISCES_ERR_UNSUPPORTED, "mixed-precision solve not built"};
-}
-}  // namespace pisces
-
 extern "C" {
 int pisces_davidson_work_size(int, int, int, int) { return PISCES_ERR_UNSUPPORTED; }
 int pisces_davidson(pisces_ctx *, int, int, int, int, int, int, int, double *, int *, double *,

@@ -287,8 +287,9 @@ int solve_fp64(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, in
     }
     if (!converged) {
         // maxmacro exhausted: like the reference, leave evals untouched and
-        // return the current Ritz vectors
-        PC_HIP(hipMemcpyAsync(vecs, d.U, sizeof(double) * N * nroots, hipMemcpyDeviceToHost, d.s));
+        // return B's first columns (converged vectors first, then the current
+        // Ritz vectors of the unconverged roots; src/Davidson.cpp:303-341)
+        PC_HIP(hipMemcpyAsync(vecs, d.B, sizeof(double) * N * nroots, hipMemcpyDeviceToHost, d.s));
         PC_HIP(hipStreamSynchronize(d.s));
     }
     return nconv;

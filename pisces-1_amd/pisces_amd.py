@@ -52,7 +52,8 @@ def lib():
         for name in ("pisces_grid", "pisces_set_potential", "pisces_get_potential",
                      "pisces_apply_h", "pisces_diag", "pisces_piab", "pisces_solve",
                      "pisces_build_potential", "pisces_build_cluster_potential",
-                     "pisces_apply_h_device", "pisces_apply_h_ex", "pisces_time_hx"):
+                     "pisces_apply_h_device", "pisces_apply_h_ex", "pisces_time_hx",
+                     "pisces_solve_block"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -219,6 +220,25 @@ class Context:
         return dict(nconv=rc, evals=ev, vecs=vecs, n_matvec=st.n_matvec,
                     n_matvec_lo=st.n_matvec_lo, n_iter=st.n_iter, n_macro=st.n_macro,
                     seconds=st.seconds, resid=np.array(st.resid[:nroots]))
+
+
+def _solve_block(self, nroots, start, block=None, maxsub=32, max_iter=2000, ptol=7,
+                 precision=FP64):
+    """Block Davidson (pisces_solve_block)."""
+    block = nroots if block is None else block
+    vecs = np.ascontiguousarray(np.asarray(start, dtype=np.float64)[:nroots]).copy()
+    assert vecs.shape == (nroots, self.ngp)
+    ev = np.zeros(nroots)
+    st = Stats()
+    rc = check(lib().pisces_solve_block(self.h, nroots, block, maxsub, max_iter, ptol,
+                                        int(precision), _p(vecs), _p(ev), ctypes.byref(st)),
+               "pisces_solve_block")
+    return dict(nconv=rc, evals=ev, vecs=vecs, n_matvec=st.n_matvec,
+                n_matvec_lo=st.n_matvec_lo, n_iter=st.n_iter, n_macro=st.n_macro,
+                seconds=st.seconds, resid=np.array(st.resid[:nroots]))
+
+
+Context.solve_block = _solve_block
 
 
 def default_potpara():
