@@ -35,6 +35,9 @@ enum {
     PISCES_BF16X2 = 2  /* split hi/lo BF16 MFMA contractions, FP32 accumulate  */
 };
 
+/* correction (preconditioner) choices of pisces_solve_block */
+enum { PISCES_PRECOND_DIAG = 1, PISCES_PRECOND_KINETIC = 3 };
+
 enum {
     PISCES_OK = 0,
     PISCES_ERR_ARG = -1,
@@ -173,12 +176,16 @@ int pisces_solve(pisces_ctx *ctx, int nroots, int maxsub, int maxmacro, int ptol
  * forms, H*X of the Ritz block is recomputed in FP64 at every restart (defect
  * correction), so reported eigenvalues and residuals are FP64-exact.
  * Converged when ||H u_i - theta_i u_i|| < 10^-ptol for i < nroots.
+ * precond: PISCES_PRECOND_DIAG, the reference's Davidson correction
+ * r / (theta - diag H) (src/Davidson.cpp:557-566), or PISCES_PRECOND_KINETIC,
+ * t = -(T - theta)^{-1} r applied exactly through the eigenbases of the 1-D
+ * sine-DVR kinetic matrices (fast diagonalisation; an extension).
  * vecs: in nroots start vectors (ngp doubles each; the rest of the block is
  * seeded random), out nroots Ritz vectors.  Returns converged roots or an
  * error.  Generalises the reference's one-vector-per-root Davidson
  * (src/Davidson.cpp:61-376) to blocks; it has no direct counterpart. */
 int pisces_solve_block(pisces_ctx *ctx, int nroots, int block, int maxsub, int max_iter,
-                       int ptol, int precision, double *vecs, double *evals,
+                       int ptol, int precond, int precision, double *vecs, double *evals,
                        pisces_stats *stats);
 
 /* Reverse-communication Davidson with the reference's exact calling

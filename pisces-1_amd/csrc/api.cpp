@@ -127,6 +127,10 @@ int pisces_create(pisces_ctx **out, const int n[3], const double L[3], int devic
             upload(c->dT64[a], c->T[a].data(), sizeof(double) * n[a] * n[a], c->stream);
             std::vector<float> t32(c->T[a].begin(), c->T[a].end());
             upload(c->dT32[a], t32.data(), sizeof(float) * n[a] * n[a], c->stream);
+            std::vector<double> Q((size_t)n[a] * n[a]), lam(n[a]);
+            syev_host(n[a], c->T[a].data(), n[a], lam.data(), Q.data(), n[a]);
+            upload(c->dQ[a], Q.data(), sizeof(double) * Q.size(), c->stream);
+            upload(c->dlam[a], lam.data(), sizeof(double) * lam.size(), c->stream);
             PC_HIP(hipStreamSynchronize(c->stream));
         }
         PC_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->hjobs),

@@ -136,7 +136,9 @@ __device__ inline int rr_index(int i, int r, int mp)
     return (i == 0) ? 0 : ((i - 1 + r) % (mp - 1)) + 1;
 }
 
-__global__ void __launch_bounds__(kT) eig_kernel(const EigTask *__restrict__ tasks)
+constexpr int kEigT = 64;
+
+__global__ void __launch_bounds__(kEigT) eig_kernel(const EigTask *__restrict__ tasks)
 {
     const EigTask t = tasks[blockIdx.x];
     constexpr int LD = kBdMaxM + 1;
@@ -144,11 +146,11 @@ __global__ void __launch_bounds__(kT) eig_kernel(const EigTask *__restrict__ tas
     __shared__ double V[kBdMaxM * LD];
     __shared__ double cs[kBdMaxM / 2], sn[kBdMaxM / 2];
     __shared__ int pp[kBdMaxM / 2], qq[kBdMaxM / 2];
-    __shared__ double red[kT];
+    __shared__ double red[kEigT];
     __shared__ int order[kBdMaxM];
     __shared__ int s_stop;
     const int m = t.m;
-    for (int idx = threadIdx.x; idx < m * m; idx += kT) {
+    for (int idx = threadIdx.x; idx < m * m; idx += kEigT) {
         const int i = idx % m, j = idx / m;
         double g = (i <= j) ? t.G[i + (long long)t.ldg * j] : t.G[j + (long long)t.ldg * i];
         const bool di = t.keep && !t.keep[i];
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(kT) eig_kernel(const EigTask *__restrict__ tas
     const int npair = mp / 2;
     for (int sweep = 0; sweep < 40; ++sweep) {
         double off = 0.0, tot = 0.0;
-        for (int idx = threadIdx.x; idx < m * m; idx += kT) {
+        for (int idx = threadIdx.x; idx < m * m; idx += kEigT) {
             const int i = idx % m, j = idx / m;
             const double a = A[i + LD * j];
             if (i != j && fabs(A[i + LD * i]) < 1e29 && fabs(A[j + LD * j]) < 1e29) off += a * a;
@@ -170,7 +172,7 @@ __global__ void __launch_bounds__(kT) eig_kernel(const EigTask *__restrict__ tas
         }
         red[threadIdx.x] = off;
         __syncthreads();
-        for (int w = kT / 2; w > 0; w >>= 1) {
+        for (int w = kEigT / 2; w > 0; w >>= 1) {
             if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
@@ -178,7 +180,7 @@ __global__ void __launch_bounds__(kT) eig_kernel(const EigTask *__restrict__ tas
         __syncthreads();
         red[threadIdx.x] = tot;
         __syncthreads();
-        for (int w = kT / 2; w > 0; w >>= 1) {
+        for (int w = kEigT / 2; w > 0; w >>= 1) {
             if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
@@ -207,7 +209,7 @@ __global__ void __launch_bounds__(kT) eig_kernel(const EigTask *__restrict__ tas
                 sn[threadIdx.x] = s;
             }
             __syncthreads();
-            for (int item = threadIdx.x; item < m * npair; item += kT) {
+            for (int item = threadIdx.x; item < m * npair; item += kEigT) {
                 const int k = item % m, pr = item / m;
                 const int p = pp[pr], q = qq[pr];
                 if (q >= m || sn[pr] == 0.0) continue;
@@ -220,7 +222,7 @@ __global__ void __launch_bounds__(kT) eig_kernel(const EigTask *__restrict__ tas
                 V[k + LD * q] = s * vkp + c * vkq;
             }
             __syncthreads();
-            for (int item = threadIdx.x; item < m * npair; item += kT) {
+            for (int item = threadIdx.x; item < m * npair; item += kEigT) {
                 const int k = item % m, pr = item / m;
                 const int p = pp[pr], q = qq[pr];
                 if (q >= m || sn[pr] == 0.0) continue;
@@ -242,7 +244,7 @@ __global__ void __launch_bounds__(kT) eig_kernel(const EigTask *__restrict__ tas
         }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < m * t.k; idx += kT) {
+    for (int idx = threadIdx.x; idx < m * t.k; idx += kEigT) {
         const int i = idx % m, c = idx / m;
         t.Y[i + (long long)m * c] = V[i + LD * order[c]];
     }
@@ -345,7 +347,10 @@ __global__ void cholinv_batched_kernel(const CholTask *__restrict__ tasks)
             for (int p = i + 1; p < k; ++p) s -= L[p + kBdMaxK * i] * t.W[p + k * c];
             t.W[i + k * c] = s / L[i + kBdMaxK * i];
         }
-    for (int c = 0; c < k; ++c) t.keep_out[c] = keep[c];
+    for (int c = 0; c < k; ++c) {
+        t.keep_out[c] = keep[c];
+        if (t.keep_copy) t.keep_copy[c] = keep[c];
+    }
 }
 
 inline unsigned nblk(long long N) { return (unsigned)((N + kT - 1) / kT); }
@@ -385,7 +390,7 @@ void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, hipStream_
 void bd_eig(const EigTask *d_tasks, int ntasks, hipStream_t s)
 {
     if (ntasks <= 0) return;
-    hipLaunchKernelGGL(eig_kernel, dim3(ntasks), dim3(kT), 0, s, d_tasks);
+    hipLaunchKernelGGL(eig_kernel, dim3(ntasks), dim3(kEigT), 0, s, d_tasks);
     PC_HIP(hipGetLastError());
 }
 

@@ -61,6 +61,7 @@ struct PrecTask {
 // keep_in (may be null = all kept) carries the mask of an earlier pass.
 struct CholTask {
     const double *G; int k; int pad; double *W; const int *keep_in; int *keep_out; double tol;
+    int *keep_copy;  // optional second destination of keep_out
 };
 
 void bd_gram(const GramTask *d_tasks, int ntasks, int max_nch, int max_mamb, int max_cols,

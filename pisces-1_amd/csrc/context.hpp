@@ -20,6 +20,7 @@ struct pisces_ctx {
     int maxn = 0;
     std::vector<double> x[3], T[3];  // host copies
     pisces::dbuf dx[3], dT64[3], dT32[3];
+    pisces::dbuf dQ[3], dlam[3];  // eigenvectors / eigenvalues of T_a (kinetic preconditioner)
     pisces::dbuf dV64, dV32;
     bool has_v = false;
 

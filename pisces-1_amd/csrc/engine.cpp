@@ -1,12 +1,24 @@
 // engine.cpp -- host control of the batched block-Davidson engine.
+//
+// A step is two launch phases separated by one device-to-host transfer:
+//   A: restarts (keeping the previous Ritz block, "GD+k"), FP64 refresh of
+//      H X, H C of the new corrections, new columns of G, eigenpairs, Ritz
+//      vectors, residuals and their norms;
+//   B: corrections for the unconverged columns (precondition, orthogonalise,
+//      append).
+// All task descriptors of a phase are written to one pinned arena and sent
+// with a single copy before the phase's first launch; the two phases use
+// separate arenas so that a phase never overwrites descriptors still in flight.
 #include "engine.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
-#include <random>
 
 #include "blockops.hpp"
+#include "fdm.hpp"
 
 namespace pisces {
 
@@ -19,23 +31,28 @@ constexpr int oRn = 5136;    // ||R||^2 (k)
 constexpr int oH = 5152;     // CGS coefficients ((k + mc) x nt)
 constexpr int oWc = 6176;    // CholQR factor (k x k)
 constexpr int oGt = 6432;    // Gram of the new block (k x k)
-constexpr int oKeep = 6688;  // int keep mask of all columns (as ints)
+constexpr int oKeep = 6688;  // int keep mask of all columns
 constexpr int oKeepT = 6752; // int keep mask of the new block
 constexpr int oGu = 6784;    // Gram of U at restart (k x k)
 constexpr int oWu = 7040;    // restart factor (k x k)
+constexpr int oHp = 7296;    // X^T U_prev (k x k)
 constexpr int kSmall = 8192;
-constexpr int kRegions = 24;
 constexpr int kCpb = 4;      // Gram chunks per workgroup
+
+inline int nch_of(long long N)
+{
+    return (int)((N + (long long)kBdChunk * kCpb - 1) / ((long long)kBdChunk * kCpb));
+}
 }  // namespace
 
 struct BdEngine::Inst {
     BdGrid g;
     int nroots = 0, k = 0, mcmax = 0;
-    dbuf X, HX, U, HU, R, T, T2, C, HC, small;
-    int mc = 0, nnew = 0, nt = 0;
+    dbuf X, HX, U, HU, Up, HUp, R, T, T2, C, HC, small;
+    int mc = 0, nnew = 0, gnew = 0, nt = 0;
     int src[kBdMaxK];
     bool restart = true, init = true, refresh = false, fresh = false, done = false;
-    bool released = false;
+    bool have_prev = false;
     BdResult res;
     double theta[kBdMaxK], rn[kBdMaxK];
     std::shared_ptr<void> keepalive;
@@ -44,22 +61,61 @@ struct BdEngine::Inst {
     int *keepT() const { return reinterpret_cast<int *>(small.as<double>() + oKeepT); }
 };
 
+// ------------------------------------------------------------------ arena
+struct BdEngine::Arena {
+    unsigned char *h = nullptr;
+    dbuf d;
+    size_t cap = 0, used = 0;
+    ~Arena()
+    {
+        if (h) static_cast<void>(hipHostFree(h));
+    }
+    void reserve(size_t bytes, hipStream_t s)
+    {
+        if (bytes <= cap) return;
+        PC_HIP(hipStreamSynchronize(s));
+        if (h) PC_HIP(hipHostFree(h));
+        h = nullptr;
+        cap = bytes + bytes / 2;
+        PC_HIP(hipHostMalloc(reinterpret_cast<void **>(&h), cap, hipHostMallocDefault));
+        d.alloc(cap);
+    }
+    template <typename T> T *take(int count, T **dp)
+    {
+        used = (used + 63) & ~(size_t)63;
+        PC_REQUIRE(used + sizeof(T) * (size_t)count <= cap, PISCES_ERR_ARG, "task arena overflow");
+        T *hp = reinterpret_cast<T *>(h + used);
+        *dp = reinterpret_cast<T *>(d.as<unsigned char>() + used);
+        used += sizeof(T) * (size_t)count;
+        return hp;
+    }
+    void flush(hipStream_t s)
+    {
+        if (used) PC_HIP(hipMemcpyAsync(d.p, h, used, hipMemcpyHostToDevice, s));
+    }
+};
+
 BdEngine::BdEngine(int device, hipStream_t s, int precision, int block, int maxsub, int ptol,
-                   int max_iter)
+                   int max_iter, int precond)
     : dev_(device), s_(s), prec_(precision), k_(block), maxsub_(maxsub), ptol_(ptol),
-      max_iter_(max_iter)
+      max_iter_(max_iter), precond_(precond)
 {
+    PC_REQUIRE(precond == 1 || precond == 3, PISCES_ERR_ARG,
+               "preconditioner must be 1 (diagonal) or 3 (kinetic)");
     PC_REQUIRE(block >= 1 && block <= kBdMaxK, PISCES_ERR_ARG, "block size must be 1..16");
-    PC_REQUIRE(maxsub >= 2 * block && maxsub <= kBdMaxM, PISCES_ERR_ARG,
-               "need 2*block <= maxsub <= 64");
+    PC_REQUIRE(maxsub >= 3 * block && maxsub <= kBdMaxM, PISCES_ERR_ARG,
+               "need 3*block <= maxsub <= 64");
     PC_REQUIRE(precision >= PISCES_FP64 && precision <= PISCES_BF16X2, PISCES_ERR_ARG,
                "unknown precision");
+    arena_[0] = std::make_unique<Arena>();
+    arena_[1] = std::make_unique<Arena>();
+    const char *tr = std::getenv("PISCES_TRACE");
+    trace_ = tr && tr[0] == '1';
 }
 
 BdEngine::~BdEngine()
 {
     if (s_) static_cast<void>(hipStreamSynchronize(s_));
-    if (htask_) static_cast<void>(hipHostFree(htask_));
     if (hsmall_) static_cast<void>(hipHostFree(hsmall_));
 }
 
@@ -67,7 +123,7 @@ size_t BdEngine::bytes_per_instance(long long N) const
 {
     const size_t e = (prec_ == PISCES_FP64) ? 8 : 4;
     const size_t mcmax = (size_t)(maxsub_ - k_);
-    return (size_t)N * (7 * k_ * 8 + 2 * mcmax * e) + kSmall * 8;
+    return (size_t)N * (9 * k_ * 8 + 2 * mcmax * e) + kSmall * 8;
 }
 
 int BdEngine::live() const
@@ -78,40 +134,12 @@ int BdEngine::live() const
     return n;
 }
 
-void BdEngine::ensure_tasks(size_t per_region)
-{
-    if (per_region <= region_bytes_) return;
-    PC_HIP(hipStreamSynchronize(s_));
-    per_region = std::max(per_region, (size_t)4096);
-    if (htask_) PC_HIP(hipHostFree(htask_));
-    PC_HIP(hipHostMalloc(reinterpret_cast<void **>(&htask_), per_region * kRegions, hipHostMallocDefault));
-    dtask_.alloc(per_region * kRegions);
-    region_bytes_ = per_region;
-}
-
 void BdEngine::grow_part(size_t bytes)
 {
     if (bytes <= dpart_.bytes) return;
     PC_HIP(hipStreamSynchronize(s_));  // the old buffer may still be read
     dpart_.alloc(bytes + bytes / 4);
 }
-
-template <typename T>
-T *BdEngine::stage(int region, int count, T **dptr)
-{
-    PC_REQUIRE((size_t)count * sizeof(T) <= region_bytes_, PISCES_ERR_ARG, "task region overflow");
-    *dptr = reinterpret_cast<T *>(dtask_.as<unsigned char>() + (size_t)region * region_bytes_);
-    return reinterpret_cast<T *>(htask_ + (size_t)region * region_bytes_);
-}
-
-namespace {
-template <typename T>
-const T *upload(BdEngine *, hipStream_t s, T *h, T *d, int count)
-{
-    if (count > 0) PC_HIP(hipMemcpyAsync(d, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
-    return d;
-}
-}  // namespace
 
 int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, unsigned seed,
                   std::shared_ptr<void> keepalive)
@@ -126,18 +154,13 @@ int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, 
     in->keepalive = std::move(keepalive);
     const long long N = g.N;
     const size_t e = (prec_ == PISCES_FP64) ? 8 : 4;
-    in->X.alloc(8 * N * k_);
-    in->HX.alloc(8 * N * k_);
-    in->U.alloc(8 * N * k_);
-    in->HU.alloc(8 * N * k_);
-    in->R.alloc(8 * N * k_);
-    in->T.alloc(8 * N * k_);
-    in->T2.alloc(8 * N * k_);
+    for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->Up, &in->HUp, &in->R, &in->T, &in->T2})
+        b->alloc(8 * N * k_);
     in->C.alloc(e * N * in->mcmax);
     in->HC.alloc(e * N * in->mcmax);
     in->small.alloc(8 * kSmall);
-    // start vectors in U (restart path orthonormalises them): given ones, then
-    // seeded uniform [-1, 1] fill for the rest of the block
+    // start vectors in U (the first restart orthonormalises them): the given
+    // ones, then a seeded uniform [-1, 1] fill for the rest of the block
     if (nstart > 0)
         PC_HIP(hipMemcpyAsync(in->U.p, start, 8 * N * nstart, hipMemcpyHostToDevice, s_));
     if (nstart < k_)
@@ -172,256 +195,299 @@ void BdEngine::release(int slot)
     inst_.at(slot).reset();
 }
 
-// X = orthonormalised U (two Cholesky-QR passes), HX = HU W (or marked for an
-// FP64 recompute).  U holds the start vectors on the first call.
-void BdEngine::run_restarts(std::vector<Inst *> &v)
+// ------------------------------------------------------------------ launches
+namespace {
+
+struct GramBatch {
+    GramTask *h = nullptr, *d = nullptr;
+    int n = 0, max_nch = 0, max_mamb = 0, max_cols = 0;
+    long long part = 0;
+    void add(const void *A, int ma, int a32, const void *B, int mb, int b32, long long N,
+             double *out, int out_ld)
+    {
+        GramTask t{A, N, ma, a32, B, N, mb, b32, N, out, out_ld, part, nch_of(N), kCpb};
+        part += (long long)ma * mb * t.nch;
+        max_nch = std::max(max_nch, t.nch);
+        max_mamb = std::max(max_mamb, ma * mb);
+        max_cols = std::max(max_cols, ma + mb);
+        h[n++] = t;
+    }
+    void launch(double *dp, hipStream_t s) const
+    {
+        if (n) bd_gram(d, n, max_nch, max_mamb, max_cols, dp, s);
+    }
+};
+
+struct CombBatch {
+    CombTask *h = nullptr, *d = nullptr;
+    int n = 0;
+    long long maxN = 0;
+    void add(const CombTask &t)
+    {
+        h[n++] = t;
+        maxN = std::max(maxN, t.N);
+    }
+    void launch(hipStream_t s) const
+    {
+        if (n) bd_combine(d, n, maxN, s);
+    }
+};
+
+struct HxBatch {
+    HxJob *h = nullptr, *d = nullptr;
+    int n = 0, pa = 0, pb = 0, mx = 0;
+    void add(const HxJob &j)
+    {
+        h[n++] = j;
+        pa = std::max(pa, j.nz * j.nvec);
+        pb = std::max(pb, j.ny * j.nvec);
+        mx = std::max({mx, j.nx, j.ny, j.nz});
+    }
+};
+
+CombTask comb1(const void *A, long long N, int ma, int a32, const double *W, int ldw, void *Y,
+               int y32, int mb, int mode)
 {
-    if (v.empty()) return;
-    const int n = (int)v.size();
-    for (int pass = 0; pass < 2; ++pass) {
-        GramTask *dg;
-        GramTask *hg = stage<GramTask>(pass * 3 + 0, n, &dg);
-        CholTask *dc;
-        CholTask *hc = stage<CholTask>(pass * 3 + 1, n, &dc);
-        int max_nch = 0;
-        long long part = 0;
-        for (int i = 0; i < n; ++i) {
-            Inst *in = v[i];
-            const double *src = (pass == 0) ? in->U.as<double>() : in->X.as<double>();
-            GramTask &t = hg[i];
-            t = GramTask{src, in->g.N, k_, 0, src, in->g.N, k_, 0, in->g.N,
-                         in->sm() + oGu, k_, part, 0, kCpb};
-            t.nch = (int)((in->g.N + (long long)kBdChunk * kCpb - 1) / ((long long)kBdChunk * kCpb));
-            part += (long long)k_ * k_ * t.nch;
-            max_nch = std::max(max_nch, t.nch);
-            hc[i] = CholTask{in->sm() + oGu, k_, 0, in->sm() + oWu, pass ? in->keep() : nullptr,
-                             in->keep(), 1e-12};
-        }
-        grow_part(sizeof(double) * part);
-        upload(this, s_, hg, dg, n);
-        upload(this, s_, hc, dc, n);
-        bd_gram(dg, n, max_nch, k_ * k_, 2 * k_, dpart_.as<double>(), s_);
-        bd_cholinv(dc, n, s_);
-        CombTask *dm;
-        CombTask *hm = stage<CombTask>(pass * 3 + 2, 2 * n, &dm);
-        int nm = 0;
-        long long maxN = 0;
-        for (int i = 0; i < n; ++i) {
-            Inst *in = v[i];
-            const double *src = (pass == 0) ? in->U.as<double>() : in->X.as<double>();
-            hm[nm++] = CombTask{src, in->g.N, k_, 0, in->sm() + oWu, k_, nullptr, 0, 0, 0, nullptr,
-                                0, in->X.p, in->g.N, 0, k_, 0, in->g.N};
-            if (!in->init) {
-                const double *hsrc = (pass == 0) ? in->HU.as<double>() : in->HX.as<double>();
-                hm[nm++] = CombTask{hsrc, in->g.N, k_, 0, in->sm() + oWu, k_, nullptr, 0, 0, 0,
-                                    nullptr, 0, in->HX.p, in->g.N, 0, k_, 0, in->g.N};
-            }
-            maxN = std::max(maxN, in->g.N);
-        }
-        upload(this, s_, hm, dm, nm);
-        bd_combine(dm, nm, maxN, s_);
-    }
-    for (Inst *in : v) {
-        in->refresh = in->init || prec_ != PISCES_FP64;
-        in->init = false;
-        in->restart = false;
-        in->fresh = true;
-        in->mc = 0;
-        in->nnew = 0;
-        in->res.n_restart++;
-    }
+    return CombTask{A, N, ma, a32, W, ldw, nullptr, 0, 0, 0, nullptr, 0, Y, N, y32, mb, mode, N};
 }
+
+}  // namespace
 
 int BdEngine::step()
 {
-    std::vector<Inst *> live;
-    int maxn = 0;
+    std::vector<Inst *> act;
     for (auto &p : inst_)
-        if (p && !p->done) {
-            live.push_back(p.get());
-            maxn = std::max({maxn, p->g.n[0], p->g.n[1], p->g.n[2]});
-        }
-    if (live.empty()) return 0;
-    const int n = (int)live.size();
-    const size_t tmax = std::max({sizeof(GramTask), sizeof(CombTask), sizeof(PrecTask),
-                                  sizeof(HxJob), sizeof(EigTask), sizeof(ResTask),
-                                  sizeof(CholTask)});
-    ensure_tasks(tmax * 2 * (n + 1) + 256);
+        if (p && !p->done) act.push_back(p.get());
+    if (act.empty()) return 0;
+    const int n = (int)act.size();
     const double tol = std::pow(10.0, -ptol_);
     const bool lo = prec_ != PISCES_FP64;
     const int e32 = lo ? 1 : 0;
-
-    // 1. restarts (and the first orthonormalisation of the start block)
-    std::vector<Inst *> rs;
-    for (Inst *in : live)
-        if (in->restart) rs.push_back(in);
-    run_restarts(rs);
-
-    // 2. FP64 H X where X changed without an exact HX (start, low-precision restart)
-    // 3. H C for the corrections appended in the previous step
-    for (int phase = 0; phase < 2; ++phase) {
-        HxJob *dj;
-        HxJob *hj = stage<HxJob>(6 + phase, n, &dj);
-        int nj = 0, pa = 0, pb = 0, mx = 0;
-        for (Inst *in : live) {
-            HxJob j{};
-            j.nx = in->g.n[0]; j.ny = in->g.n[1]; j.nz = in->g.n[2];
-            if (phase == 0) {
-                if (!in->refresh) continue;
-                j.nvec = k_;
-                for (int a = 0; a < 3; ++a) j.T[a] = in->g.T64[a];
-                j.V = in->g.V64;
-                j.X = in->X.p; j.Y = in->HX.p;
-                in->res.n_mv64 += k_;
-                in->refresh = false;
-            } else {
-                if (in->nnew <= 0) continue;
-                j.nvec = in->nnew;
-                const size_t e = lo ? 4 : 8;
-                for (int a = 0; a < 3; ++a) j.T[a] = lo ? in->g.T32[a] : in->g.T64[a];
-                j.V = lo ? (const void *)in->g.V32 : (const void *)in->g.V64;
-                j.X = in->C.as<unsigned char>() + e * in->g.N * (in->mc - in->nnew);
-                j.Y = in->HC.as<unsigned char>() + e * in->g.N * (in->mc - in->nnew);
-                if (lo) in->res.n_mvlo += in->nnew;
-                else in->res.n_mv64 += in->nnew;
-            }
-            j.ld = in->g.N;
-            hj[nj++] = j;
-            pa = std::max(pa, j.nz * j.nvec);
-            pb = std::max(pb, j.ny * j.nvec);
-            mx = std::max({mx, j.nx, j.ny, j.nz});
-        }
-        if (nj) {
-            upload(this, s_, hj, dj, nj);
-            hx_launch(dj, nj, pa, pb, mx, phase == 0 ? PISCES_FP64 : prec_, s_);
-        }
-    }
-
-    // 4. new columns of G
-    {
-        GramTask *dg;
-        GramTask *hg = stage<GramTask>(8, 2 * n, &dg);
-        int ng = 0, max_nch = 0, max_mamb = 0, max_cols = 0;
-        long long part = 0;
-        auto addg = [&](const void *A, int ma, int a32, const void *B, int mb, int b32, long long N,
-                        double *out, int out_ld) {
-            GramTask t{A, N, ma, a32, B, N, mb, b32, N, out, out_ld, part, 0, kCpb};
-            t.nch = (int)((N + (long long)kBdChunk * kCpb - 1) / ((long long)kBdChunk * kCpb));
-            part += (long long)ma * mb * t.nch;
-            max_nch = std::max(max_nch, t.nch);
-            max_mamb = std::max(max_mamb, ma * mb);
-            max_cols = std::max(max_cols, ma + mb);
-            hg[ng++] = t;
-        };
-        const size_t e = lo ? 4 : 8;
-        for (Inst *in : live) {
-            const long long N = in->g.N;
-            if (in->fresh)
-                addg(in->X.p, k_, 0, in->HX.p, k_, 0, N, in->sm() + oG, kBdMaxM);
-            if (in->nnew > 0) {
-                const int c0 = in->mc - in->nnew;
-                const void *Cn = in->C.as<unsigned char>() + e * N * c0;
-                const void *HCn = in->HC.as<unsigned char>() + e * N * c0;
-                double *col = in->sm() + oG + (size_t)kBdMaxM * (k_ + c0);
-                // rows of X: HX^T C_new (FP64 HX keeps these entries FP64-exact)
-                addg(in->HX.p, k_, 0, Cn, in->nnew, e32, N, col, kBdMaxM);
-                // rows of C (including the new ones): C^T HC_new
-                addg(in->C.p, in->mc, e32, HCn, in->nnew, e32, N, col + k_, kBdMaxM);
-            }
-        }
-        if (ng) {
-            grow_part(sizeof(double) * part);
-            upload(this, s_, hg, dg, ng);
-            bd_gram(dg, ng, max_nch, max_mamb, max_cols, dpart_.as<double>(), s_);
-        }
-    }
-
-    // 5. lowest k eigenpairs of G
-    {
-        EigTask *de;
-        EigTask *he = stage<EigTask>(9, n, &de);
-        for (int i = 0; i < n; ++i) {
-            Inst *in = live[i];
-            he[i] = EigTask{in->sm() + oG, kBdMaxM, k_ + in->mc, k_, in->sm() + oTh, in->sm() + oY,
-                            in->keep(), 0, 0};
-        }
-        upload(this, s_, he, de, n);
-        bd_eig(de, n, s_);
-    }
-
-    // 6. Ritz vectors U = [X|C] y and HU = [HX|HC] y
-    {
-        CombTask *dm;
-        CombTask *hm = stage<CombTask>(10, 2 * n, &dm);
-        long long maxN = 0;
-        for (int i = 0; i < n; ++i) {
-            Inst *in = live[i];
-            const long long N = in->g.N;
-            const int m = k_ + in->mc;
-            const double *Y = in->sm() + oY;
-            hm[2 * i] = CombTask{in->X.p, N, k_, 0, Y, m, in->C.p, N, in->mc, e32, Y + k_, m,
-                                 in->U.p, N, 0, k_, 0, N};
-            hm[2 * i + 1] = CombTask{in->HX.p, N, k_, 0, Y, m, in->HC.p, N, in->mc, e32, Y + k_, m,
-                                     in->HU.p, N, 0, k_, 0, N};
-            maxN = std::max(maxN, N);
-        }
-        upload(this, s_, hm, dm, 2 * n);
-        bd_combine(dm, 2 * n, maxN, s_);
-    }
-
-    // 7. residuals R = HU - theta U and their norms
+    const size_t e = lo ? 4 : 8;
+    const size_t tmax = std::max({sizeof(GramTask), sizeof(CombTask), sizeof(PrecTask),
+                                  sizeof(HxJob), sizeof(EigTask), sizeof(ResTask),
+                                  sizeof(CholTask), sizeof(FdmTask)});
+    // at most ~30 descriptors per instance and phase, plus alignment slack
+    arena_[0]->reserve((tmax * 32 + 64 * 24) * (size_t)(n + 1), s_);
+    arena_[1]->reserve((tmax * 32 + 64 * 24) * (size_t)(n + 1), s_);
     if (dgather_.bytes < sizeof(double) * 2 * kBdMaxK * (size_t)n) {
         PC_HIP(hipStreamSynchronize(s_));
         dgather_.alloc(sizeof(double) * 2 * kBdMaxK * (size_t)std::max(n, 64));
     }
-    {
-        ResTask *dr;
-        ResTask *hr = stage<ResTask>(11, n, &dr);
-        int max_nch = 0;
-        long long part = 0;
-        for (int i = 0; i < n; ++i) {
-            Inst *in = live[i];
-            const long long N = in->g.N;
-            ResTask t{in->U.as<double>(), in->HU.as<double>(), in->R.as<double>(), N, k_, 0,
-                      in->sm() + oTh, N, part, 0, 0, in->sm() + oRn,
-                      dgather_.as<double>() + (size_t)2 * kBdMaxK * i};
-            t.nch = (int)((N + kBdResChunk - 1) / kBdResChunk);
-            part += (long long)k_ * t.nch;
-            max_nch = std::max(max_nch, t.nch);
-            hr[i] = t;
-        }
-        grow_part(sizeof(double) * part);
-        upload(this, s_, hr, dr, n);
-        bd_resid(dr, n, max_nch, dpart_.as<double>(), s_);
-    }
-    // one transfer of theta and ||R||^2 for every live instance
     if (hsmall_cap_ < (size_t)n * 2 * kBdMaxK) {
         PC_HIP(hipStreamSynchronize(s_));
         if (hsmall_) PC_HIP(hipHostFree(hsmall_));
+        hsmall_ = nullptr;
         hsmall_cap_ = (size_t)std::max(n, 64) * 2 * kBdMaxK;
         PC_HIP(hipHostMalloc(reinterpret_cast<void **>(&hsmall_), sizeof(double) * hsmall_cap_,
                              hipHostMallocDefault));
     }
+
+    // ============================ phase A: descriptors
+    Arena &A = *arena_[0];
+    A.used = 0;
+    std::vector<Inst *> rs;
+    for (Inst *in : act)
+        if (in->restart) rs.push_back(in);
+    const int nr = (int)rs.size();
+    // restart: X = orth(U) (two Cholesky-QR passes), HX = HU W
+    GramBatch rg[2];
+    CholTask *rch[2] = {nullptr, nullptr}, *rcd[2] = {nullptr, nullptr};
+    CombBatch rc[2];
+    for (int pass = 0; pass < 2; ++pass) {
+        rg[pass].h = A.take<GramTask>(nr, &rg[pass].d);
+        rch[pass] = A.take<CholTask>(nr, &rcd[pass]);
+        rc[pass].h = A.take<CombTask>(2 * nr, &rc[pass].d);
+        for (int i = 0; i < nr; ++i) {
+            Inst *in = rs[i];
+            const long long N = in->g.N;
+            const double *src = pass == 0 ? in->U.as<double>() : in->X.as<double>();
+            rg[pass].add(src, k_, 0, src, k_, 0, N, in->sm() + oGu, k_);
+            rch[pass][i] = CholTask{in->sm() + oGu, k_, 0, in->sm() + oWu,
+                                    pass ? in->keep() : nullptr, in->keep(), 1e-12, nullptr};
+            rc[pass].add(comb1(src, N, k_, 0, in->sm() + oWu, k_, in->X.p, 0, k_, 0));
+            if (!in->init) {
+                const double *hsrc = pass == 0 ? in->HU.as<double>() : in->HX.as<double>();
+                rc[pass].add(comb1(hsrc, N, k_, 0, in->sm() + oWu, k_, in->HX.p, 0, k_, 0));
+            }
+        }
+    }
+    // GD+k: the previous Ritz block, orthogonalised against X, opens the new
+    // correction basis C[0..k) with its H-image from HU_prev
+    std::vector<Inst *> gk;
+    for (Inst *in : rs)
+        if (!in->init && in->have_prev) gk.push_back(in);
+    const int ngk = (int)gk.size();
+    GramBatch pg, pg2;
+    CombBatch pc, pc2;
+    CholTask *pch = nullptr, *pcd = nullptr;
+    pg.h = A.take<GramTask>(ngk, &pg.d);
+    pc.h = A.take<CombTask>(2 * ngk, &pc.d);
+    pg2.h = A.take<GramTask>(ngk, &pg2.d);
+    pch = A.take<CholTask>(ngk, &pcd);
+    pc2.h = A.take<CombTask>(2 * ngk, &pc2.d);
+    // (pointers captured now: U/Up swap roles below, before these run)
+    std::vector<const void *> gk_up(ngk), gk_hup(ngk);
+    for (int i = 0; i < ngk; ++i) {
+        Inst *in = gk[i];
+        const long long N = in->g.N;
+        double *h = in->sm() + oHp;
+        gk_up[i] = in->Up.p;
+        gk_hup[i] = in->HUp.p;
+        pg.add(in->X.p, k_, 0, in->Up.p, k_, 0, N, h, k_);
+        // T = Up - X h, T2 = HUp - HX h  (T, T2 first receive copies of Up, HUp)
+        pc.add(comb1(in->X.p, N, k_, 0, h, k_, in->T.p, 0, k_, 1));
+        pc.add(comb1(in->HX.p, N, k_, 0, h, k_, in->T2.p, 0, k_, 1));
+        pg2.add(in->T.p, k_, 0, in->T.p, k_, 0, N, in->sm() + oGt, k_);
+        pch[i] = CholTask{in->sm() + oGt, k_, 0, in->sm() + oWc, nullptr, in->keepT(), 1e-10,
+                          in->keep() + k_};
+        pc2.add(comb1(in->T.p, N, k_, 0, in->sm() + oWc, k_, in->C.p, e32, k_, 0));
+        pc2.add(comb1(in->T2.p, N, k_, 0, in->sm() + oWc, k_, in->HC.p, e32, k_, 0));
+    }
+    for (Inst *in : rs) {
+        in->refresh = in->init || lo;
+        in->mc = 0;
+        in->nnew = 0;
+        in->gnew = 0;
+        in->fresh = true;
+        in->res.n_restart++;
+    }
+    for (Inst *in : gk) {
+        in->mc = k_;
+        in->gnew = k_;
+    }
+    // FP64 refresh of H X, and H C of last step's corrections
+    HxBatch hr, hc;
+    hr.h = A.take<HxJob>(n, &hr.d);
+    hc.h = A.take<HxJob>(n, &hc.d);
+    for (Inst *in : act) {
+        HxJob j{};
+        j.nx = in->g.n[0]; j.ny = in->g.n[1]; j.nz = in->g.n[2];
+        j.ld = in->g.N;
+        if (in->refresh) {
+            j.nvec = k_;
+            for (int a = 0; a < 3; ++a) j.T[a] = in->g.T64[a];
+            j.V = in->g.V64;
+            j.X = in->X.p;
+            j.Y = in->HX.p;
+            hr.add(j);
+            in->res.n_mv64 += k_;
+            in->refresh = false;
+        }
+        if (in->nnew > 0) {
+            j.nvec = in->nnew;
+            for (int a = 0; a < 3; ++a) j.T[a] = lo ? in->g.T32[a] : in->g.T64[a];
+            j.V = lo ? (const void *)in->g.V32 : (const void *)in->g.V64;
+            j.X = in->C.as<unsigned char>() + e * in->g.N * (in->mc - in->nnew);
+            j.Y = in->HC.as<unsigned char>() + e * in->g.N * (in->mc - in->nnew);
+            hc.add(j);
+            if (lo) in->res.n_mvlo += in->nnew;
+            else in->res.n_mv64 += in->nnew;
+        }
+    }
+    // new columns of G
+    GramBatch gg;
+    gg.h = A.take<GramTask>(3 * n, &gg.d);
+    for (Inst *in : act) {
+        const long long N = in->g.N;
+        if (in->fresh) gg.add(in->X.p, k_, 0, in->HX.p, k_, 0, N, in->sm() + oG, kBdMaxM);
+        if (in->gnew > 0) {
+            const int c0 = in->mc - in->gnew;
+            const void *Cn = in->C.as<unsigned char>() + e * N * c0;
+            const void *HCn = in->HC.as<unsigned char>() + e * N * c0;
+            double *col = in->sm() + oG + (size_t)kBdMaxM * (k_ + c0);
+            gg.add(in->HX.p, k_, 0, Cn, in->gnew, e32, N, col, kBdMaxM);
+            gg.add(in->C.p, in->mc, e32, HCn, in->gnew, e32, N, col + k_, kBdMaxM);
+        }
+    }
+    EigTask *he, *de;
+    he = A.take<EigTask>(n, &de);
+    CombBatch rz;
+    rz.h = A.take<CombTask>(2 * n, &rz.d);
+    ResTask *hrs, *drs;
+    hrs = A.take<ResTask>(n, &drs);
+    int res_nch = 0;
+    long long res_part = 0;
+    for (int i = 0; i < n; ++i) {
+        Inst *in = act[i];
+        const long long N = in->g.N;
+        const int m = k_ + in->mc;
+        he[i] = EigTask{in->sm() + oG, kBdMaxM, m, k_, in->sm() + oTh, in->sm() + oY, in->keep(), 0, 0};
+        // the previous Ritz block moves to Up/HUp; the new one is written to U/HU
+        std::swap(in->U, in->Up);
+        std::swap(in->HU, in->HUp);
+        const double *Y = in->sm() + oY;
+        rz.add(CombTask{in->X.p, N, k_, 0, Y, m, in->C.p, N, in->mc, e32, Y + k_, m, in->U.p, N, 0,
+                        k_, 0, N});
+        rz.add(CombTask{in->HX.p, N, k_, 0, Y, m, in->HC.p, N, in->mc, e32, Y + k_, m, in->HU.p, N,
+                        0, k_, 0, N});
+        ResTask t{in->U.as<double>(), in->HU.as<double>(), in->R.as<double>(), N, k_, 0,
+                  in->sm() + oTh, N, res_part, (int)((N + kBdResChunk - 1) / kBdResChunk), 0,
+                  in->sm() + oRn, dgather_.as<double>() + (size_t)2 * kBdMaxK * i};
+        res_part += (long long)k_ * t.nch;
+        res_nch = std::max(res_nch, t.nch);
+        hrs[i] = t;
+    }
+
+    // ============================ phase A: launches
+    const long long part = std::max({rg[0].part, rg[1].part, pg.part, pg2.part, gg.part, res_part});
+    grow_part(sizeof(double) * (size_t)std::max(part, 1LL));
+    A.flush(s_);
+    double *dp = dpart_.as<double>();
+    for (int pass = 0; pass < 2; ++pass) {
+        rg[pass].launch(dp, s_);
+        if (nr) bd_cholinv(rcd[pass], nr, s_);
+        rc[pass].launch(s_);
+    }
+    if (ngk) {
+        for (int i = 0; i < ngk; ++i) {
+            Inst *in = gk[i];
+            PC_HIP(hipMemcpyAsync(in->T.p, gk_up[i], 8 * in->g.N * k_, hipMemcpyDeviceToDevice, s_));
+            PC_HIP(hipMemcpyAsync(in->T2.p, gk_hup[i], 8 * in->g.N * k_, hipMemcpyDeviceToDevice, s_));
+        }
+        pg.launch(dp, s_);
+        pc.launch(s_);
+        pg2.launch(dp, s_);
+        bd_cholinv(pcd, ngk, s_);
+        pc2.launch(s_);
+    }
+    if (hr.n) hx_launch(hr.d, hr.n, hr.pa, hr.pb, hr.mx, PISCES_FP64, s_);
+    if (hc.n) hx_launch(hc.d, hc.n, hc.pa, hc.pb, hc.mx, prec_, s_);
+    gg.launch(dp, s_);
+    bd_eig(de, n, s_);
+    rz.launch(s_);
+    bd_resid(drs, n, res_nch, dp, s_);
     PC_HIP(hipMemcpyAsync(hsmall_, dgather_.p, sizeof(double) * 2 * kBdMaxK * n,
                           hipMemcpyDeviceToHost, s_));
     PC_HIP(hipStreamSynchronize(s_));
+    for (Inst *in : act) in->have_prev = true;
+    for (Inst *in : rs) {
+        in->init = false;
+        in->restart = false;
+    }
 
-    // 8. control
+    // ============================ control
     std::vector<Inst *> corr;
     for (int i = 0; i < n; ++i) {
-        Inst *in = live[i];
+        Inst *in = act[i];
         const double *h = hsmall_ + (size_t)2 * kBdMaxK * i;
         for (int r = 0; r < k_; ++r) {
             in->theta[r] = h[r];
-            in->rn[r] = std::sqrt(std::max(0.0, h[16 + r]));
+            in->rn[r] = std::sqrt(std::max(0.0, h[kBdMaxK + r]));
         }
         in->res.n_iter++;
+        if (trace_ && i == 0) {
+            std::fprintf(stderr, "[bd] it %d mc %d fresh %d |", in->res.n_iter, in->mc, (int)in->fresh);
+            for (int r = 0; r < k_; ++r) std::fprintf(stderr, " %.9f:%.1e", in->theta[r], in->rn[r]);
+            std::fprintf(stderr, "\n");
+        }
         bool wanted = true;
         for (int r = 0; r < in->nroots; ++r)
             if (!(in->rn[r] < tol)) wanted = false;
         const bool was_fresh = in->fresh;
         in->fresh = false;
         in->nnew = 0;
+        in->gnew = 0;
         in->nt = 0;
         if (wanted && (!lo || was_fresh)) {
             in->done = true;
@@ -453,101 +519,104 @@ int BdEngine::step()
         in->nt = cnt;
         corr.push_back(in);
     }
+    if (corr.empty()) return this->live();
 
-    // 9-11. corrections: precondition, orthogonalise, store in C
-    if (!corr.empty()) {
-        const int nc = (int)corr.size();
-        long long maxN = 0;
-        {
-            PrecTask *dp;
-            PrecTask *hp = stage<PrecTask>(12, nc, &dp);
-            for (int i = 0; i < nc; ++i) {
-                Inst *in = corr[i];
-                PrecTask t{};
-                t.R = in->R.as<double>(); t.ld = in->g.N; t.theta = in->sm() + oTh; t.D = in->g.D;
-                t.T = in->T.as<double>(); t.N = in->g.N; t.nt = in->nt;
-                std::memcpy(t.src, in->src, sizeof(in->src));
-                hp[i] = t;
-                maxN = std::max(maxN, in->g.N);
-            }
-            upload(this, s_, hp, dp, nc);
-            bd_precond(dp, nc, maxN, s_);
+    // ============================ phase B: corrections
+    Arena &B = *arena_[1];
+    B.used = 0;
+    const int nc = (int)corr.size();
+    PrecTask *hp = nullptr, *dpc = nullptr;
+    FdmTask *hf = nullptr, *dfd = nullptr;
+    int nfd = 0, fd_maxn = 0;
+    long long maxN = 0;
+    if (precond_ == 1) {
+        hp = B.take<PrecTask>(nc, &dpc);
+        for (int i = 0; i < nc; ++i) {
+            Inst *in = corr[i];
+            PrecTask t{};
+            t.R = in->R.as<double>(); t.ld = in->g.N; t.theta = in->sm() + oTh; t.D = in->g.D;
+            t.T = in->T.as<double>(); t.N = in->g.N; t.nt = in->nt;
+            std::memcpy(t.src, in->src, sizeof(in->src));
+            hp[i] = t;
+            maxN = std::max(maxN, in->g.N);
         }
-        const size_t e = lo ? 4 : 8;
-        for (int pass = 0; pass < 2; ++pass) {
-            GramTask *dg;
-            GramTask *hg = stage<GramTask>(13 + 2 * pass, 2 * nc, &dg);
-            CombTask *dm;
-            CombTask *hm = stage<CombTask>(14 + 2 * pass, nc, &dm);
-            int ng = 0, max_nch = 0, max_mamb = 0, max_cols = 0;
-            long long part = 0;
-            for (int i = 0; i < nc; ++i) {
-                Inst *in = corr[i];
-                const long long N = in->g.N;
-                double *h = in->sm() + oH;
-                const int m = k_ + in->mc;
-                auto mk = [&](const void *A, int ma, int a32, double *out) {
-                    GramTask t{A, N, ma, a32, in->T.p, N, in->nt, 0, N, out, m, part, 0, kCpb};
-                    t.nch = (int)((N + (long long)kBdChunk * kCpb - 1) / ((long long)kBdChunk * kCpb));
-                    part += (long long)ma * in->nt * t.nch;
-                    max_nch = std::max(max_nch, t.nch);
-                    max_mamb = std::max(max_mamb, ma * in->nt);
-                    max_cols = std::max(max_cols, ma + in->nt);
-                    hg[ng++] = t;
-                };
-                mk(in->X.p, k_, 0, h);
-                if (in->mc > 0) mk(in->C.p, in->mc, e32, h + k_);
-                hm[i] = CombTask{in->X.p, N, k_, 0, h, m, in->C.p, N, in->mc, e32, h + k_, m,
-                                 in->T.p, N, 0, in->nt, 1, N};
-            }
-            grow_part(sizeof(double) * part);
-            upload(this, s_, hg, dg, ng);
-            bd_gram(dg, ng, max_nch, max_mamb, max_cols, dpart_.as<double>(), s_);
-            upload(this, s_, hm, dm, nc);
-            bd_combine(dm, nc, maxN, s_);
-        }
-        // Cholesky QR of the new block, twice: T -> T2 -> C[mc .. mc+nt)
-        for (int pass = 0; pass < 2; ++pass) {
-            GramTask *dg;
-            GramTask *hg = stage<GramTask>(17 + 3 * pass, nc, &dg);
-            CholTask *dc;
-            CholTask *hc = stage<CholTask>(18 + 3 * pass, nc, &dc);
-            CombTask *dm;
-            CombTask *hm = stage<CombTask>(19 + 3 * pass, nc, &dm);
-            int max_nch = 0, max_mamb = 0, max_cols = 0;
-            long long part = 0;
-            for (int i = 0; i < nc; ++i) {
-                Inst *in = corr[i];
-                const long long N = in->g.N;
-                const void *src = pass == 0 ? in->T.p : in->T2.p;
-                GramTask t{src, N, in->nt, 0, src, N, in->nt, 0, N, in->sm() + oGt, in->nt, part, 0, kCpb};
-                t.nch = (int)((N + (long long)kBdChunk * kCpb - 1) / ((long long)kBdChunk * kCpb));
-                part += (long long)in->nt * in->nt * t.nch;
-                max_nch = std::max(max_nch, t.nch);
-                max_mamb = std::max(max_mamb, in->nt * in->nt);
-                max_cols = std::max(max_cols, 2 * in->nt);
-                hg[i] = t;
-                hc[i] = CholTask{in->sm() + oGt, in->nt, 0, in->sm() + oWc,
-                                 pass ? in->keepT() : nullptr, in->keepT(), 1e-10};
-                void *dst = pass == 0 ? in->T2.p
-                                      : (void *)(in->C.as<unsigned char>() + e * N * in->mc);
-                hm[i] = CombTask{src, N, in->nt, 0, in->sm() + oWc, in->nt, nullptr, 0, 0, 0,
-                                 nullptr, 0, dst, N, pass == 0 ? 0 : e32, in->nt, 0, N};
-            }
-            grow_part(sizeof(double) * part);
-            upload(this, s_, hg, dg, nc);
-            upload(this, s_, hc, dc, nc);
-            upload(this, s_, hm, dm, nc);
-            bd_gram(dg, nc, max_nch, max_mamb, max_cols, dpart_.as<double>(), s_);
-            bd_cholinv(dc, nc, s_);
-            bd_combine(dm, nc, maxN, s_);
-        }
+    } else {
+        int tot = 0;
+        for (Inst *in : corr) tot += in->nt;
+        hf = B.take<FdmTask>(tot, &dfd);
         for (Inst *in : corr) {
-            PC_HIP(hipMemcpyAsync(in->keep() + k_ + in->mc, in->keepT(), sizeof(int) * in->nt,
-                                  hipMemcpyDeviceToDevice, s_));
-            in->mc += in->nt;
-            in->nnew = in->nt;
+            maxN = std::max(maxN, in->g.N);
+            fd_maxn = std::max({fd_maxn, in->g.n[0], in->g.n[1], in->g.n[2]});
+            for (int t = 0; t < in->nt; ++t) {
+                FdmTask f{};
+                for (int a = 0; a < 3; ++a) {
+                    f.n[a] = in->g.n[a];
+                    f.Q[a] = in->g.Q[a];
+                    f.lam[a] = in->g.lam[a];
+                }
+                f.src = in->R.as<double>() + in->g.N * in->src[t];
+                f.dst = in->T.as<double>() + in->g.N * t;
+                // t = -(T - theta)^{-1} r
+                f.shift = in->theta[in->src[t]];
+                f.scale = -1.0;
+                f.floor = 1e-3;
+                hf[nfd++] = f;
+            }
         }
+    }
+    GramBatch og[2], cg[2];
+    CombBatch oc[2], cc[2];
+    CholTask *chh[2], *chd[2];
+    for (int pass = 0; pass < 2; ++pass) {
+        og[pass].h = B.take<GramTask>(2 * nc, &og[pass].d);
+        oc[pass].h = B.take<CombTask>(nc, &oc[pass].d);
+        for (Inst *in : corr) {
+            const long long N = in->g.N;
+            const int m = k_ + in->mc;
+            double *h = in->sm() + oH;
+            og[pass].add(in->X.p, k_, 0, in->T.p, in->nt, 0, N, h, m);
+            if (in->mc > 0) og[pass].add(in->C.p, in->mc, e32, in->T.p, in->nt, 0, N, h + k_, m);
+            oc[pass].add(CombTask{in->X.p, N, k_, 0, h, m, in->C.p, N, in->mc, e32, h + k_, m,
+                                  in->T.p, N, 0, in->nt, 1, N});
+        }
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        cg[pass].h = B.take<GramTask>(nc, &cg[pass].d);
+        chh[pass] = B.take<CholTask>(nc, &chd[pass]);
+        cc[pass].h = B.take<CombTask>(nc, &cc[pass].d);
+        for (int i = 0; i < nc; ++i) {
+            Inst *in = corr[i];
+            const long long N = in->g.N;
+            const void *src = pass == 0 ? in->T.p : in->T2.p;
+            cg[pass].add(src, in->nt, 0, src, in->nt, 0, N, in->sm() + oGt, in->nt);
+            chh[pass][i] = CholTask{in->sm() + oGt, in->nt, 0, in->sm() + oWc,
+                                    pass ? in->keepT() : nullptr, in->keepT(), 1e-10,
+                                    pass ? in->keep() + k_ + in->mc : nullptr};
+            void *dst = pass == 0 ? in->T2.p : (void *)(in->C.as<unsigned char>() + e * N * in->mc);
+            cc[pass].add(comb1(src, N, in->nt, 0, in->sm() + oWc, in->nt, dst, pass ? e32 : 0,
+                               in->nt, 0));
+        }
+    }
+    long long partB = 1;
+    for (int pass = 0; pass < 2; ++pass) partB = std::max({partB, og[pass].part, cg[pass].part});
+    grow_part(sizeof(double) * (size_t)partB);
+    dp = dpart_.as<double>();
+    B.flush(s_);
+    if (precond_ == 1) bd_precond(dpc, nc, maxN, s_);
+    else fdm_launch(dfd, nfd, fd_maxn, s_);
+    for (int pass = 0; pass < 2; ++pass) {
+        og[pass].launch(dp, s_);
+        oc[pass].launch(s_);
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        cg[pass].launch(dp, s_);
+        bd_cholinv(chd[pass], nc, s_);
+        cc[pass].launch(s_);
+    }
+    for (Inst *in : corr) {
+        in->mc += in->nt;
+        in->nnew = in->nt;
+        in->gnew = in->nt;
     }
     return this->live();
 }

@@ -41,6 +41,8 @@ struct BdGrid {
     const double *V64 = nullptr;
     const float *V32 = nullptr;
     const double *D = nullptr;  // diag(H)
+    const double *Q[3] = {nullptr, nullptr, nullptr};    // eigenvectors of T_a
+    const double *lam[3] = {nullptr, nullptr, nullptr};  // eigenvalues of T_a
 };
 
 struct BdResult {
@@ -52,8 +54,9 @@ struct BdResult {
 
 class BdEngine {
 public:
+    // precond: 1 Davidson diagonal correction, 3 kinetic (fast diagonalisation)
     BdEngine(int device, hipStream_t s, int precision, int block, int maxsub, int ptol,
-             int max_iter);
+             int max_iter, int precond);
     ~BdEngine();
     // Adds an instance; start: nroots host vectors (more columns are filled
     // with seeded random vectors up to the block size).  Returns its slot.
@@ -72,23 +75,18 @@ public:
 
 private:
     struct Inst;
-    void run_restarts(std::vector<Inst *> &v);
+    struct Arena;
     std::vector<std::unique_ptr<Inst>> inst_;
     int dev_;
     hipStream_t s_;
-    int prec_, k_, maxsub_, ptol_, max_iter_;
-    // device task arrays and their pinned host staging, one region per stage
-    dbuf dtask_;
-    unsigned char *htask_ = nullptr;
-    size_t task_cap_ = 0;
+    int prec_, k_, maxsub_, ptol_, max_iter_, precond_;
+    std::unique_ptr<Arena> arena_[2];  // task descriptors of phases A and B
     dbuf dpart_;
     dbuf dgather_;
     double *hsmall_ = nullptr;
     size_t hsmall_cap_ = 0;
-    template <typename T> T *stage(int region, int count, T **dptr);
-    void ensure_tasks(size_t per_region_bytes);
+    bool trace_ = false;  // PISCES_TRACE=1: per-step Ritz values of the first instance
     void grow_part(size_t bytes);
-    size_t region_bytes_ = 0;
 };
 
 }  // namespace pisces

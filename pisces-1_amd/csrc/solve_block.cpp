@@ -22,15 +22,19 @@ BdGrid grid_of(pisces_ctx *c)
     diag_h(c->dV64.as<double>(), c->dT64[0].as<double>(), c->dT64[1].as<double>(),
            c->dT64[2].as<double>(), c->n[0], c->n[1], c->n[2], c->wdiag.as<double>(), c->stream);
     g.D = c->wdiag.as<double>();
+    for (int a = 0; a < 3; ++a) {
+        g.Q[a] = c->dQ[a].as<double>();
+        g.lam[a] = c->dlam[a].as<double>();
+    }
     return g;
 }
 
 int solve_block(pisces_ctx *c, int nroots, int block, int maxsub, int max_iter, int ptol,
-                int precision, double *vecs, double *evals, pisces_stats *st)
+                int precond, int precision, double *vecs, double *evals, pisces_stats *st)
 {
     PC_REQUIRE(c->has_v, PISCES_ERR_STATE, "no potential set");
     auto t0 = std::chrono::steady_clock::now();
-    BdEngine eng(c->device, c->stream, precision, block, maxsub, ptol, max_iter);
+    BdEngine eng(c->device, c->stream, precision, block, maxsub, ptol, max_iter, precond);
     const BdGrid g = grid_of(c);
     const int slot = eng.add(g, nroots, vecs, nroots, 7u, nullptr);
     while (eng.step() > 0) {
@@ -59,24 +63,25 @@ int solve_mixed(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, i
     PC_REQUIRE(corrflag == 1, PISCES_ERR_UNSUPPORTED,
                "the low-precision forms use the Davidson correction (corrflag 1)");
     const int block = nroots;
-    PC_REQUIRE(maxsub >= 2 * block, PISCES_ERR_ARG, "maxsub must be at least 2*nroots");
+    PC_REQUIRE(maxsub >= 3 * block, PISCES_ERR_ARG, "maxsub must be at least 3*nroots");
     const int per_macro = (maxsub - block) / block + 1;
-    return solve_block(c, nroots, block, maxsub, maxmacro * per_macro, ptol, precision, vecs,
+    return solve_block(c, nroots, block, maxsub, maxmacro * per_macro, ptol, 1, precision, vecs,
                        evals, st);
 }
 
 }  // namespace pisces
 
 extern "C" int pisces_solve_block(pisces_ctx *c, int nroots, int block, int maxsub, int max_iter,
-                                  int ptol, int precision, double *vecs, double *evals,
-                                  pisces_stats *st)
+                                  int ptol, int precond, int precision, double *vecs,
+                                  double *evals, pisces_stats *st)
 {
     if (!c || !vecs || !evals) return PISCES_ERR_ARG;
     if (nroots < 1 || block < nroots || precision < PISCES_FP64 || precision > PISCES_BF16X2)
         return PISCES_ERR_ARG;
+    if (precond != 1 && precond != 3) return PISCES_ERR_ARG;
     return pisces::guarded_call([&] {
         PC_HIP(hipSetDevice(c->device));
-        return pisces::solve_block(c, nroots, block, maxsub, max_iter, ptol, precision, vecs,
-                                   evals, st);
+        return pisces::solve_block(c, nroots, block, maxsub, max_iter, ptol, precond, precision,
+                                   vecs, evals, st);
     });
 }

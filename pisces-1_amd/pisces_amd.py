@@ -222,8 +222,11 @@ class Context:
                     seconds=st.seconds, resid=np.array(st.resid[:nroots]))
 
 
+PRECOND_DIAG, PRECOND_KINETIC = 1, 3
+
+
 def _solve_block(self, nroots, start, block=None, maxsub=32, max_iter=2000, ptol=7,
-                 precision=FP64):
+                 precision=FP64, precond=PRECOND_KINETIC):
     """Block Davidson (pisces_solve_block)."""
     block = nroots if block is None else block
     vecs = np.ascontiguousarray(np.asarray(start, dtype=np.float64)[:nroots]).copy()
@@ -231,7 +234,8 @@ def _solve_block(self, nroots, start, block=None, maxsub=32, max_iter=2000, ptol
     ev = np.zeros(nroots)
     st = Stats()
     rc = check(lib().pisces_solve_block(self.h, nroots, block, maxsub, max_iter, ptol,
-                                        int(precision), _p(vecs), _p(ev), ctypes.byref(st)),
+                                        int(precond), int(precision), _p(vecs), _p(ev),
+                                        ctypes.byref(st)),
                "pisces_solve_block")
     return dict(nconv=rc, evals=ev, vecs=vecs, n_matvec=st.n_matvec,
                 n_matvec_lo=st.n_matvec_lo, n_iter=st.n_iter, n_macro=st.n_macro,

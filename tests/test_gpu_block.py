@@ -18,7 +18,7 @@ def _golden_case(oracle_mod, tag):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x2"])
 @pytest.mark.parametrize("tag,block,maxsub", [("g12_r4", 4, 16), ("g12_r4", 8, 32),
-                                              ("c1", 1, 20), ("c1", 4, 24)])
+                                              ("c1", 1, 20), ("c1", 4, 24), ("g10_r2_jd", 4, 16)])
 def test_block_solve_matches_reference(pisces, oracle_mod, tag, block, maxsub, precision):
     print("block", tag, block, precision, flush=True)
     f, n, L, V = _golden_case(oracle_mod, tag)
