@@ -220,6 +220,7 @@ typedef struct {
     int nroots;          /* states wanted                                 */
     int warm;            /* 1: start from the chain's previous subspace   */
     long long chain;     /* chain id for warm starts (-1: none)           */
+    int want_vectors;    /* 1: keep the Ritz vectors for pisces_server_vectors */
 } pisces_request;
 
 typedef struct {
@@ -247,8 +248,12 @@ int pisces_server_step(pisces_server *srv);
 int pisces_server_poll(pisces_server *srv, pisces_result *out, int max);
 /* Device bytes an instance of this request would hold (the admission cost). */
 size_t pisces_server_footprint(const pisces_server *srv, const pisces_request *req);
-/* Fetch the Ritz vectors of a finished request (nroots*ngp doubles). */
+/* Fetch the Ritz vectors of a finished request (nroots*ngp doubles); the
+ * vectors of the 16 most recently finished requests are kept. */
 int pisces_server_vectors(pisces_server *srv, long long id, double *vecs);
+/* Requests queued and not yet admitted; device bytes held by live instances. */
+int pisces_server_pending(const pisces_server *srv);
+size_t pisces_server_used_bytes(const pisces_server *srv);
 
 #ifdef __cplusplus
 }

@@ -141,8 +141,19 @@ void BdEngine::grow_part(size_t bytes)
     dpart_.alloc(bytes + bytes / 4);
 }
 
+dbuf BdEngine::take_buf(size_t bytes)
+{
+    for (size_t i = 0; i < pool_.size(); ++i)
+        if (pool_[i].bytes == bytes) {
+            dbuf b = std::move(pool_[i]);
+            pool_.erase(pool_.begin() + (long)i);
+            return b;
+        }
+    return dbuf(bytes);
+}
+
 int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, unsigned seed,
-                  std::shared_ptr<void> keepalive)
+                  std::shared_ptr<void> keepalive, bool start_on_device)
 {
     PC_REQUIRE(nroots >= 1 && nroots <= k_, PISCES_ERR_ARG, "nroots must be 1..block");
     PC_REQUIRE(nstart >= 0 && nstart <= k_, PISCES_ERR_ARG, "too many start vectors");
@@ -155,20 +166,22 @@ int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, 
     const long long N = g.N;
     const size_t e = (prec_ == PISCES_FP64) ? 8 : 4;
     for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->Up, &in->HUp, &in->R, &in->T, &in->T2})
-        b->alloc(8 * N * k_);
-    in->C.alloc(e * N * in->mcmax);
-    in->HC.alloc(e * N * in->mcmax);
-    in->small.alloc(8 * kSmall);
+        *b = take_buf(8 * N * k_);
+    in->C = take_buf(e * N * in->mcmax);
+    in->HC = take_buf(e * N * in->mcmax);
+    in->small = take_buf(8 * kSmall);
     // start vectors in U (the first restart orthonormalises them): the given
     // ones, then a seeded uniform [-1, 1] fill for the rest of the block
     if (nstart > 0)
-        PC_HIP(hipMemcpyAsync(in->U.p, start, 8 * N * nstart, hipMemcpyHostToDevice, s_));
+        PC_HIP(hipMemcpyAsync(in->U.p, start, 8 * N * nstart,
+                              start_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s_));
     if (nstart < k_)
         fill_uniform(in->U.as<double>() + N * nstart, N * (k_ - nstart), true, seed, s_);
     std::vector<int> ones(kBdMaxM + kBdMaxK, 1);
+    static_assert(kBdMaxM + kBdMaxK <= 128, "keep mask size");
     PC_HIP(hipMemcpyAsync(in->keep(), ones.data(), sizeof(int) * (kBdMaxM + kBdMaxK),
                           hipMemcpyHostToDevice, s_));
-    PC_HIP(hipStreamSynchronize(s_));
+    PC_HIP(hipStreamSynchronize(s_));  // `ones` and the host start vectors go out of scope
     in->res.nroots = nroots;
     for (size_t i = 0; i < inst_.size(); ++i)
         if (!inst_[i]) {
@@ -189,9 +202,34 @@ void BdEngine::vectors(int slot, double *out)
     PC_HIP(hipStreamSynchronize(s_));
 }
 
+size_t BdEngine::pool_bytes() const
+{
+    size_t t = 0;
+    for (const dbuf &b : pool_) t += b.bytes;
+    return t;
+}
+
+void BdEngine::trim_pool(size_t max_bytes)
+{
+    size_t t = pool_bytes();
+    if (t <= max_bytes) return;
+    PC_HIP(hipStreamSynchronize(s_));
+    while (t > max_bytes && !pool_.empty()) {
+        t -= pool_.front().bytes;
+        pool_.erase(pool_.begin());
+    }
+}
+
+const double *BdEngine::vectors_device(int slot) const { return inst_.at(slot)->U.as<double>(); }
+
 void BdEngine::release(int slot)
 {
-    PC_HIP(hipStreamSynchronize(s_));
+    // buffers go to the pool (no hipFree: it would stall the device); they are
+    // reused by later add() calls, in stream order
+    Inst *in = inst_.at(slot).get();
+    for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->Up, &in->HUp, &in->R, &in->T, &in->T2,
+                    &in->C, &in->HC, &in->small})
+        if (b->p) pool_.push_back(std::move(*b));
     inst_.at(slot).reset();
 }
 

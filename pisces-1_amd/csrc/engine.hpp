@@ -61,13 +61,18 @@ public:
     // Adds an instance; start: nroots host vectors (more columns are filled
     // with seeded random vectors up to the block size).  Returns its slot.
     int add(const BdGrid &g, int nroots, const double *start, int nstart, unsigned seed,
-            std::shared_ptr<void> keepalive = nullptr);
+            std::shared_ptr<void> keepalive = nullptr, bool start_on_device = false);
     // Advances every live instance by one step; returns the number still live.
     int step();
     bool finished(int slot) const;
     const BdResult &result(int slot) const;
     // Ritz vectors of a finished instance (nroots x N doubles, host).
     void vectors(int slot, double *out);
+    // Device pointer to the Ritz vectors of a finished instance (valid until release).
+    const double *vectors_device(int slot) const;
+    // Frees pooled buffers (of released instances) beyond max_bytes in total.
+    void trim_pool(size_t max_bytes);
+    size_t pool_bytes() const;
     void release(int slot);
     size_t bytes_per_instance(long long N) const;
     int live() const;
@@ -87,6 +92,8 @@ private:
     size_t hsmall_cap_ = 0;
     bool trace_ = false;  // PISCES_TRACE=1: per-step Ritz values of the first instance
     void grow_part(size_t bytes);
+    std::vector<dbuf> pool_;  // released instance buffers, reused by add()
+    dbuf take_buf(size_t bytes);
 };
 
 }  // namespace pisces

@@ -263,3 +263,148 @@ def start_vectors(ctx, nroots, seed=0):
     for r in range(1, nroots):
         S[r] = rng.uniform(-1, 1, size=ctx.ngp)
     return S
+
+
+# ---------------------------------------------------------------- server
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("L", ctypes.c_double * 3),
+        ("nw", ctypes.c_int),
+        ("xyz", _D),
+        ("nion", ctypes.c_int),
+        ("ion_xyz", _D),
+        ("ion_q", _D),
+        ("nroots", ctypes.c_int),
+        ("warm", ctypes.c_int),
+        ("chain", ctypes.c_longlong),
+        ("want_vectors", ctypes.c_int),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_longlong),
+        ("status", ctypes.c_int),
+        ("nroots", ctypes.c_int),
+        ("evals", ctypes.c_double * 8),
+        ("resid", ctypes.c_double * 8),
+        ("n_iter", ctypes.c_int),
+        ("n_matvec", ctypes.c_int),
+        ("latency_s", ctypes.c_double),
+    ]
+
+
+def _server_lib():
+    L = lib()
+    if not getattr(L, "_server_typed", False):
+        L.pisces_server_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                           ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int]
+        L.pisces_server_destroy.argtypes = [ctypes.c_void_p]
+        L.pisces_server_destroy.restype = None
+        L.pisces_server_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request)]
+        L.pisces_server_submit.restype = ctypes.c_longlong
+        L.pisces_server_step.argtypes = [ctypes.c_void_p]
+        L.pisces_server_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(Result), ctypes.c_int]
+        L.pisces_server_footprint.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request)]
+        L.pisces_server_footprint.restype = ctypes.c_size_t
+        L.pisces_server_vectors.argtypes = [ctypes.c_void_p, ctypes.c_longlong, _D]
+        L.pisces_server_pending.argtypes = [ctypes.c_void_p]
+        L.pisces_server_used_bytes.argtypes = [ctypes.c_void_p]
+        L.pisces_server_used_bytes.restype = ctypes.c_size_t
+        L._server_typed = True
+    return L
+
+
+class Server:
+    """Continuous-batching solver service (pisces_server_*)."""
+
+    def __init__(self, hbm_cap_bytes=200 << 30, max_slots=64, precision=BF16X2, maxsub=32,
+                 ptol=7, device=0):
+        L = _server_lib()
+        h = ctypes.c_void_p()
+        check(L.pisces_server_create(ctypes.byref(h), int(device), int(hbm_cap_bytes),
+                                     int(max_slots), int(precision), int(maxsub), int(ptol)),
+              "pisces_server_create")
+        self.h = h
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            _server_lib().pisces_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _req(self, n, L, xyz, nroots, ions=None, chain=-1, warm=False, want_vectors=False):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+        r = Request()
+        for a in range(3):
+            r.n[a] = int(n[a])
+            r.L[a] = float(L[a])
+        r.nw = xyz.size // 9
+        r.xyz = _p(xyz)
+        keep = [xyz]
+        if ions is not None:
+            ix = np.ascontiguousarray(ions[0], dtype=np.float64).reshape(-1)
+            iq = np.ascontiguousarray(ions[1], dtype=np.float64).reshape(-1)
+            r.nion = iq.size
+            r.ion_xyz = _p(ix)
+            r.ion_q = _p(iq)
+            keep += [ix, iq]
+        r.nroots = int(nroots)
+        r.warm = 1 if warm else 0
+        r.chain = int(chain)
+        r.want_vectors = 1 if want_vectors else 0
+        return r, keep
+
+    def submit(self, n, L, xyz, nroots, ions=None, chain=-1, warm=False, want_vectors=False):
+        r, keep = self._req(n, L, xyz, nroots, ions, chain, warm, want_vectors)
+        rid = _server_lib().pisces_server_submit(self.h, ctypes.byref(r))
+        check(int(rid), "pisces_server_submit")
+        self.shapes[int(rid)] = (tuple(n), int(nroots))
+        return int(rid)
+
+    def footprint(self, n, L, xyz, nroots):
+        r, keep = self._req(n, L, xyz, nroots)
+        return int(_server_lib().pisces_server_footprint(self.h, ctypes.byref(r)))
+
+    def step(self):
+        return check(_server_lib().pisces_server_step(self.h), "pisces_server_step")
+
+    def pending(self):
+        return _server_lib().pisces_server_pending(self.h)
+
+    def used_bytes(self):
+        return int(_server_lib().pisces_server_used_bytes(self.h))
+
+    def poll(self, max_results=1024):
+        buf = (Result * max_results)()
+        k = check(_server_lib().pisces_server_poll(self.h, buf, max_results), "pisces_server_poll")
+        out = []
+        for i in range(k):
+            r = buf[i]
+            out.append(dict(id=r.id, status=r.status, evals=np.array(r.evals[:r.nroots]),
+                            resid=np.array(r.resid[:r.nroots]), n_iter=r.n_iter,
+                            n_matvec=r.n_matvec, latency_s=r.latency_s))
+        return out
+
+    def vectors(self, rid):
+        n, nroots = self.shapes[rid]
+        v = np.zeros((nroots, n[0] * n[1] * n[2]))
+        check(_server_lib().pisces_server_vectors(self.h, int(rid), _p(v)), "pisces_server_vectors")
+        return v
+
+    def run_until_done(self, max_steps=100000):
+        results = []
+        for _ in range(max_steps):
+            live = self.step()
+            results += self.poll()
+            if live == 0 and self.pending() == 0:
+                break
+        return results
