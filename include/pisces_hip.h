@@ -57,6 +57,18 @@ const char *pisces_last_error(void);
  * For cross-checks; no reference counterpart. */
 int pisces_debug_set_hx_variant(int variant);
 
+/* Launch timing: when enabled, the solver brackets each of its batched launch
+ * families with HIP events on the stream they run on and accumulates time,
+ * launch count and algorithmic HBM bytes (operands read once, results written
+ * once) per family.  kind: 0 H*X FP64, 1 H*X FP32/BF16X2, 2 Gram, 3 block
+ * combination, 4 projected eigensolve, 5 residuals, 6 preconditioner,
+ * 7 Cholesky-QR, 8 grid potential, 9 induced-dipole solve.  Measurement hook
+ * for the benchmark; no reference counterpart. */
+int pisces_prof_enable(int on);
+int pisces_prof_reset(void);
+int pisces_prof_read(int kind, double *ms_total, long long *launches, double *bytes_total);
+const char *pisces_prof_name(int kind);
+
 /* Create a context for one sine-DVR grid of n[0] x n[1] x n[2] points on boxes
  * of length L[3] (bohr), centred at the origin; computes the 1-D grids and
  * kinetic matrices (sine_dvr, src/sine_dvr.cpp:45-131) and uploads them.

@@ -24,69 +24,117 @@ __device__ inline void st_any(void *p, long long i, int is32, double v)
 }
 
 // ------------------------------------------------------------------ Gram
+// A^T B of two tall column blocks (rows = grid points).  Workgroup = one row
+// chunk x one 8 x 8 tile of (A column, B column) pairs.  Each thread streams
+// rows g = chunk start + tid + 256 s (consecutive lanes on consecutive rows,
+// so every column load is one coalesced 256/512-byte wave access) and keeps
+// the 64 products of its tile in registers; a butterfly reduce-scatter over
+// the wave (63 shuffles) leaves tile entry `lane` on lane `lane`, and the four
+// waves are summed through LDS into this chunk's partial.
+template <typename TA, typename TB>
+__device__ __forceinline__ void gram_rows(const TA *__restrict__ A, long long lda, int na,
+                                          const TB *__restrict__ B, long long ldb, int nb,
+                                          long long r0, long long r1, double (&acc)[kGTile * kGTile])
+{
+    if (na == kGTile && nb == kGTile) {
+#pragma unroll 2
+        for (long long g = r0 + threadIdx.x; g < r1; g += kT) {
+            double a[kGTile], b[kGTile];
+#pragma unroll
+            for (int i = 0; i < kGTile; ++i) a[i] = (double)A[g + i * lda];
+#pragma unroll
+            for (int j = 0; j < kGTile; ++j) b[j] = (double)B[g + j * ldb];
+#pragma unroll
+            for (int i = 0; i < kGTile; ++i)
+#pragma unroll
+                for (int j = 0; j < kGTile; ++j) acc[i * kGTile + j] = fma(a[i], b[j], acc[i * kGTile + j]);
+        }
+    } else {
+        // partial tile: columns past na / nb read column 0 and are weighted 0
+        for (long long g = r0 + threadIdx.x; g < r1; g += kT) {
+            double a[kGTile], b[kGTile];
+#pragma unroll
+            for (int i = 0; i < kGTile; ++i) a[i] = (i < na) ? (double)A[g + i * lda] : 0.0;
+#pragma unroll
+            for (int j = 0; j < kGTile; ++j) b[j] = (j < nb) ? (double)B[g + j * ldb] : 0.0;
+#pragma unroll
+            for (int i = 0; i < kGTile; ++i)
+#pragma unroll
+                for (int j = 0; j < kGTile; ++j) acc[i * kGTile + j] = fma(a[i], b[j], acc[i * kGTile + j]);
+        }
+    }
+}
+
+// Butterfly reduce-scatter of 64 values per lane over a 64-lane wave: after it,
+// lane L holds the wave-wide sum of value L.
+__device__ __forceinline__ double wave_reduce_scatter64(double (&v)[64])
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const bool up = (lane & off) != 0;
+#pragma unroll
+        for (int k = 0; k < off; ++k) {
+            const double keep = up ? v[k + off] : v[k];
+            const double give = up ? v[k] : v[k + off];
+            v[k] = keep + __shfl_xor(give, off, 64);
+        }
+    }
+    return v[0];
+}
+
 __global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ tasks,
                                                   double *__restrict__ partial)
 {
     const GramTask t = tasks[blockIdx.y];
-    if ((int)blockIdx.x >= t.nch) return;
-    extern __shared__ __align__(16) unsigned char smem[];
-    double *sa = reinterpret_cast<double *>(smem);
-    constexpr int P = kBdChunk + 1;
-    double *sb = sa + t.ma * P;
-    // each thread owns up to 4 (a, b) pairs and accumulates over cpb chunks
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int cc = 0; cc < t.cpb; ++cc) {
-        const long long g0 = ((long long)blockIdx.x * t.cpb + cc) * kBdChunk;
-        if (g0 >= t.N) break;
-        const int len = (int)((t.N - g0) < kBdChunk ? (t.N - g0) : kBdChunk);
-        __syncthreads();  // the previous chunk is consumed
-        for (int idx = threadIdx.x; idx < t.ma * kBdChunk; idx += kT) {
-            const int a = idx / kBdChunk, e = idx - a * kBdChunk;
-            sa[a * P + e] = (e < len) ? ld_any(t.A, a * t.lda + g0 + e, t.a32) : 0.0;
-        }
-        for (int idx = threadIdx.x; idx < t.mb * kBdChunk; idx += kT) {
-            const int b = idx / kBdChunk, e = idx - b * kBdChunk;
-            sb[b * P + e] = (e < len) ? ld_any(t.B, b * t.ldb + g0 + e, t.b32) : 0.0;
-        }
-        __syncthreads();
+    const int ntile = t.cpb;  // tiles of this task
+    if ((int)blockIdx.x >= t.nch * ntile) return;
+    const int chunk = blockIdx.x / ntile, tile = blockIdx.x - chunk * ntile;
+    const int nta = (t.ma + kGTile - 1) / kGTile;
+    const int a0 = (tile % nta) * kGTile, b0 = (tile / nta) * kGTile;
+    const int na = min(kGTile, t.ma - a0), nb = min(kGTile, t.mb - b0);
+    const long long r0 = (long long)chunk * kGRows;
+    const long long r1 = min(r0 + kGRows, t.N);
+    double acc[kGTile * kGTile];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int o = threadIdx.x + u * kT;
-            if (o >= t.ma * t.mb) break;
-            const int a = o % t.ma, b = o / t.ma;
-            const double *pa = sa + a * P, *pb = sb + b * P;
-            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-            for (int e = 0; e < kBdChunk; e += 4) {
-                s0 += pa[e] * pb[e];
-                s1 += pa[e + 1] * pb[e + 1];
-                s2 += pa[e + 2] * pb[e + 2];
-                s3 += pa[e + 3] * pb[e + 3];
-            }
-            acc[u] += (s0 + s1) + (s2 + s3);
-        }
+    for (int i = 0; i < kGTile * kGTile; ++i) acc[i] = 0.0;
+    if (t.a32) {
+        const float *A = static_cast<const float *>(t.A) + a0 * t.lda;
+        if (t.b32) gram_rows(A, t.lda, na, static_cast<const float *>(t.B) + b0 * t.ldb, t.ldb, nb, r0, r1, acc);
+        else gram_rows(A, t.lda, na, static_cast<const double *>(t.B) + b0 * t.ldb, t.ldb, nb, r0, r1, acc);
+    } else {
+        const double *A = static_cast<const double *>(t.A) + a0 * t.lda;
+        if (t.b32) gram_rows(A, t.lda, na, static_cast<const float *>(t.B) + b0 * t.ldb, t.ldb, nb, r0, r1, acc);
+        else gram_rows(A, t.lda, na, static_cast<const double *>(t.B) + b0 * t.ldb, t.ldb, nb, r0, r1, acc);
     }
+    const double w = wave_reduce_scatter64(acc);
+    __shared__ double red[kT / 64][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    red[wv][lane] = w;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        double sum = 0.0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int o = threadIdx.x + u * kT;
-        if (o < t.ma * t.mb) partial[t.part_off + (long long)o * t.nch + blockIdx.x] = acc[u];
+        for (int q = 0; q < kT / 64; ++q) sum += red[q][threadIdx.x];
+        const int i = threadIdx.x / kGTile, j = threadIdx.x - i * kGTile;
+        if (i < na && j < nb) {
+            const long long o = (long long)(a0 + i) + (long long)t.ma * (b0 + j);
+            partial[t.part_off + o * t.nch + chunk] = sum;
+        }
     }
 }
 
+// one thread per Gram entry: sum of its chunk partials (fixed order)
 __global__ void __launch_bounds__(kT) gram_reduce_kernel(const GramTask *__restrict__ tasks,
                                                          const double *__restrict__ partial)
 {
     const GramTask t = tasks[blockIdx.y];
-    const int o = blockIdx.x;
+    const int o = blockIdx.x * kT + threadIdx.x;
     if (o >= t.ma * t.mb) return;
-    __shared__ double red[kT];
-    double s = 0;
-    for (int c = threadIdx.x; c < t.nch; c += kT) s += partial[t.part_off + (long long)o * t.nch + c];
-    red[threadIdx.x] = s;
-    for (int w = kT / 2; w > 0; w >>= 1) {
-        __syncthreads();
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    }
-    if (threadIdx.x == 0) t.out[(o % t.ma) + (long long)t.out_ld * (o / t.ma)] = red[0];
+    const double *p = partial + t.part_off + (long long)o * t.nch;
+    double s = 0.0;
+    for (int c = 0; c < t.nch; ++c) s += p[c];
+    t.out[(o % t.ma) + (long long)t.out_ld * (o / t.ma)] = s;
 }
 
 // ------------------------------------------------------------------ combine
@@ -353,30 +401,76 @@ __global__ void cholinv_batched_kernel(const CholTask *__restrict__ tasks)
     }
 }
 
-inline unsigned nblk(long long N) { return (unsigned)((N + kT - 1) / kT); }
-
-void init_attrs()
+// ------------------------------------------------------------------ restart directions
+// One 64-lane workgroup per task; lane = coefficient row (m <= 64).
+__device__ __forceinline__ double wave_sum(double v)
 {
-    static std::once_flag once;
-    std::call_once(once, [] {
-        static_cast<void>(hipFuncSetAttribute(reinterpret_cast<const void *>(gram_kernel),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              160 * 1024));
-    });
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
+
+__global__ void __launch_bounds__(64) coef_kernel(const CoefTask *__restrict__ tasks)
+{
+    const CoefTask t = tasks[blockIdx.x];
+    const int r = threadIdx.x;
+    const bool in = r < t.m;
+    double y[kBdMaxK];
+#pragma unroll
+    for (int j = 0; j < kBdMaxK; ++j) y[j] = (in && j < t.k) ? t.Y[r + (long long)t.m * j] : 0.0;
+    double q[kBdMaxK];
+#pragma unroll
+    for (int j = 0; j < kBdMaxK; ++j) q[j] = (in && j < t.k && r >= t.k) ? y[j] : 0.0;
+    int keep[kBdMaxK];
+#pragma unroll
+    for (int c = 0; c < kBdMaxK; ++c) {
+        keep[c] = 0;
+        if (c >= t.k) continue;
+        // two passes of Gram-Schmidt against the Ritz coefficients and the
+        // directions kept so far, normalising after each pass
+        double nrm = 0.0;
+        for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+            for (int j = 0; j < kBdMaxK; ++j) {
+                if (j >= t.k) break;
+                const double h = wave_sum(y[j] * q[c]);
+                q[c] -= h * y[j];
+            }
+#pragma unroll
+            for (int j = 0; j < kBdMaxK; ++j) {
+                if (j >= c) break;
+                if (!keep[j]) continue;
+                const double h = wave_sum(q[j] * q[c]);
+                q[c] -= h * q[j];
+            }
+            nrm = sqrt(wave_sum(q[c] * q[c]));
+            if (!(nrm > 1e-14)) break;  // the direction is numerically zero
+            q[c] /= nrm;
+        }
+        keep[c] = (nrm > 0.5 && nrm < 2.0) ? 1 : 0;  // second pass left the unit norm intact
+        if (!keep[c]) q[c] = 0.0;
+        if (in) t.Q[r + (long long)t.m * c] = q[c];
+    }
+    if (r < t.k) {
+        int kv = 0;
+#pragma unroll
+        for (int c = 0; c < kBdMaxK; ++c)
+            if (c == r) kv = keep[c];
+        t.keep_out[r] = kv;
+    }
+}
+
+inline unsigned nblk(long long N) { return (unsigned)((N + kT - 1) / kT); }
 
 }  // namespace
 
-void bd_gram(const GramTask *d_tasks, int ntasks, int max_nch, int max_mamb, int max_cols,
-             double *d_partial, hipStream_t s)
+void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_mamb, double *d_partial,
+             hipStream_t s)
 {
     if (ntasks <= 0) return;
-    init_attrs();
-    const size_t lds = (size_t)max_cols * (kBdChunk + 1) * sizeof(double);
-    PC_REQUIRE(lds <= 160 * 1024, PISCES_ERR_ARG, "gram block too wide");
-    hipLaunchKernelGGL(gram_kernel, dim3(max_nch, ntasks), dim3(kT), lds, s, d_tasks, d_partial);
-    hipLaunchKernelGGL(gram_reduce_kernel, dim3(max_mamb, ntasks), dim3(kT), 0, s, d_tasks,
-                       d_partial);
+    hipLaunchKernelGGL(gram_kernel, dim3(max_blocks, ntasks), dim3(kT), 0, s, d_tasks, d_partial);
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3((max_mamb + kT - 1) / kT, ntasks), dim3(kT), 0, s,
+                       d_tasks, d_partial);
     PC_HIP(hipGetLastError());
 }
 
@@ -384,6 +478,13 @@ void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, hipStream_
 {
     if (ntasks <= 0) return;
     hipLaunchKernelGGL(comb_kernel, dim3(nblk(max_N), ntasks), dim3(kT), 0, s, d_tasks);
+    PC_HIP(hipGetLastError());
+}
+
+void bd_coef(const CoefTask *d_tasks, int ntasks, hipStream_t s)
+{
+    if (ntasks <= 0) return;
+    hipLaunchKernelGGL(coef_kernel, dim3(ntasks), dim3(64), 0, s, d_tasks);
     PC_HIP(hipGetLastError());
 }
 

@@ -19,8 +19,8 @@ struct GramTask {
     long long N;
     double *out; int out_ld;
     long long part_off;  // this task's region of the partial buffer (doubles)
-    int nch;             // workgroups of this task (each sums cpb chunks)
-    int cpb;             // chunks of kBdChunk rows per workgroup
+    int nch;             // row chunks of kGRows rows (gram_chunks(N))
+    int cpb;             // 8 x 8 column tiles (gram_tiles(ma, mb)); workgroups = nch * cpb
 };
 
 // Y (set|-=|+=) A1 W1 + A2 W2, W in device memory (column-major)
@@ -64,15 +64,30 @@ struct CholTask {
     int *keep_copy;  // optional second destination of keep_out
 };
 
-void bd_gram(const GramTask *d_tasks, int ntasks, int max_nch, int max_mamb, int max_cols,
-             double *d_partial, hipStream_t s);
+// max_blocks = max over tasks of nch * cpb; max_mamb = max over tasks of ma * mb
+// Search directions of a restart in the coefficients of the old basis [X|C]:
+// Q = orth([0; Y_C] against Y), Y the m x k Ritz coefficients (ld m) whose
+// first k rows belong to X.  Q (m x k, ld m) has orthonormal columns
+// orthogonal to those of Y; a column whose direction is numerically zero is
+// zeroed with keep_out[c] = 0.  Working on the coefficients avoids forming
+// the difference of two nearly equal Ritz blocks in the full space.
+struct CoefTask {
+    const double *Y; int m; int k; double *Q; int *keep_out;
+};
+
+void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_mamb, double *d_partial,
+             hipStream_t s);
 void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, hipStream_t s);
 void bd_eig(const EigTask *d_tasks, int ntasks, hipStream_t s);
 void bd_resid(const ResTask *d_tasks, int ntasks, int max_nch, double *d_partial, hipStream_t s);
 void bd_precond(const PrecTask *d_tasks, int ntasks, long long max_N, hipStream_t s);
 void bd_cholinv(const CholTask *d_tasks, int ntasks, hipStream_t s);
+void bd_coef(const CoefTask *d_tasks, int ntasks, hipStream_t s);
 
-constexpr int kBdChunk = 128;     // rows per Gram chunk
+constexpr int kGTile = 8;          // Gram tile edge (columns of A, of B)
+constexpr int kGRows = 256 * 32;   // rows per Gram chunk (32 per thread)
+inline int gram_chunks(long long N) { return (int)((N + kGRows - 1) / kGRows); }
+inline int gram_tiles(int ma, int mb) { return ((ma + kGTile - 1) / kGTile) * ((mb + kGTile - 1) / kGTile); }
 constexpr int kBdResChunk = 1024; // rows per residual chunk
 
 }  // namespace pisces

@@ -1,7 +1,7 @@
 // engine.cpp -- host control of the batched block-Davidson engine.
 //
 // A step is two launch phases separated by one device-to-host transfer:
-//   A: restarts (keeping the previous Ritz block, "GD+k"), FP64 refresh of
+//   A: restarts (keeping the last search directions, "GD+k"), FP64 refresh of
 //      H X, H C of the new corrections, new columns of G, eigenpairs, Ritz
 //      vectors, residuals and their norms;
 //   B: corrections for the unconverged columns (precondition, orthogonalise,
@@ -10,6 +10,7 @@
 // with a single copy before the phase's first launch; the two phases use
 // separate arenas so that a phase never overwrites descriptors still in flight.
 #include "engine.hpp"
+#include "prof.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -35,14 +36,9 @@ constexpr int oKeep = 6688;  // int keep mask of all columns
 constexpr int oKeepT = 6752; // int keep mask of the new block
 constexpr int oGu = 6784;    // Gram of U at restart (k x k)
 constexpr int oWu = 7040;    // restart factor (k x k)
-constexpr int oHp = 7296;    // X^T U_prev (k x k)
-constexpr int kSmall = 8192;
-constexpr int kCpb = 4;      // Gram chunks per workgroup
-
-inline int nch_of(long long N)
-{
-    return (int)((N + (long long)kBdChunk * kCpb - 1) / ((long long)kBdChunk * kCpb));
-}
+constexpr int oQp = 7552;    // restart search directions in [X|C] coefficients (m x k)
+constexpr int oI = 8576;     // identity (k x k)
+constexpr int kSmall = 8832;
 }  // namespace
 
 struct BdEngine::Inst {
@@ -53,6 +49,7 @@ struct BdEngine::Inst {
     int src[kBdMaxK];
     bool restart = true, init = true, refresh = false, fresh = false, done = false;
     bool have_prev = false;
+    bool confirm = false;  // restart requested to confirm convergence on an FP64-fresh HX
     BdResult res;
     double theta[kBdMaxK], rn[kBdMaxK];
     std::shared_ptr<void> keepalive;
@@ -181,6 +178,10 @@ int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, 
     static_assert(kBdMaxM + kBdMaxK <= 128, "keep mask size");
     PC_HIP(hipMemcpyAsync(in->keep(), ones.data(), sizeof(int) * (kBdMaxM + kBdMaxK),
                           hipMemcpyHostToDevice, s_));
+    std::vector<double> eye((size_t)k_ * k_, 0.0);
+    for (int i = 0; i < k_; ++i) eye[(size_t)i * k_ + i] = 1.0;
+    PC_HIP(hipMemcpyAsync(in->sm() + oI, eye.data(), sizeof(double) * eye.size(),
+                          hipMemcpyHostToDevice, s_));
     PC_HIP(hipStreamSynchronize(s_));  // `ones` and the host start vectors go out of scope
     in->res.nroots = nroots;
     for (size_t i = 0; i < inst_.size(); ++i)
@@ -238,21 +239,33 @@ namespace {
 
 struct GramBatch {
     GramTask *h = nullptr, *d = nullptr;
-    int n = 0, max_nch = 0, max_mamb = 0, max_cols = 0;
+    int n = 0, max_blocks = 0, max_mamb = 0;
     long long part = 0;
     void add(const void *A, int ma, int a32, const void *B, int mb, int b32, long long N,
              double *out, int out_ld)
     {
-        GramTask t{A, N, ma, a32, B, N, mb, b32, N, out, out_ld, part, nch_of(N), kCpb};
+        GramTask t{A, N, ma, a32, B, N, mb, b32, N, out, out_ld, part, gram_chunks(N),
+                   gram_tiles(ma, mb)};
         part += (long long)ma * mb * t.nch;
-        max_nch = std::max(max_nch, t.nch);
+        max_blocks = std::max(max_blocks, t.nch * t.cpb);
         max_mamb = std::max(max_mamb, ma * mb);
-        max_cols = std::max(max_cols, ma + mb);
         h[n++] = t;
+    }
+    double bytes() const
+    {
+        double b = 0;
+        for (int i = 0; i < n; ++i) {
+            const GramTask &t = h[i];
+            b += (double)t.N * t.ma * (t.a32 ? 4 : 8);
+            if (t.B != t.A) b += (double)t.N * t.mb * (t.b32 ? 4 : 8);
+        }
+        return b;
     }
     void launch(double *dp, hipStream_t s) const
     {
-        if (n) bd_gram(d, n, max_nch, max_mamb, max_cols, dp, s);
+        if (!n) return;
+        ProfScope ps(PK_GRAM, s, prof_enabled() ? bytes() : 0.0);
+        bd_gram(d, n, max_blocks, max_mamb, dp, s);
     }
 };
 
@@ -265,9 +278,21 @@ struct CombBatch {
         h[n++] = t;
         maxN = std::max(maxN, t.N);
     }
+    double bytes() const
+    {
+        double b = 0;
+        for (int i = 0; i < n; ++i) {
+            const CombTask &t = h[i];
+            b += (double)t.N * (t.ma1 * (t.a1_32 ? 4 : 8) + t.ma2 * (t.a2_32 ? 4 : 8));
+            b += (double)t.N * t.mb * (t.y32 ? 4 : 8) * (t.mode ? 2 : 1);
+        }
+        return b;
+    }
     void launch(hipStream_t s) const
     {
-        if (n) bd_combine(d, n, maxN, s);
+        if (!n) return;
+        ProfScope ps(PK_COMB, s, prof_enabled() ? bytes() : 0.0);
+        bd_combine(d, n, maxN, s);
     }
 };
 
@@ -280,6 +305,20 @@ struct HxBatch {
         pa = std::max(pa, j.nz * j.nvec);
         pb = std::max(pb, j.ny * j.nvec);
         mx = std::max({mx, j.nx, j.ny, j.nz});
+    }
+    // X read and Y written once per vector, V read once
+    double bytes(size_t e) const
+    {
+        double b = 0;
+        for (int i = 0; i < n; ++i) b += (double)(2 * h[i].nvec + 1) * h[i].nx * h[i].ny * h[i].nz * e;
+        return b;
+    }
+    void launch(int precision, hipStream_t s) const
+    {
+        if (!n) return;
+        const size_t e = precision == PISCES_FP64 ? 8 : 4;
+        ProfScope ps(precision == PISCES_FP64 ? PK_HX64 : PK_HXLO, s, prof_enabled() ? bytes(e) : 0.0);
+        hx_launch(d, n, pa, pb, mx, precision, s);
     }
 };
 
@@ -350,40 +389,36 @@ int BdEngine::step()
             }
         }
     }
-    // GD+k: the previous Ritz block, orthogonalised against X, opens the new
-    // correction basis C[0..k) with its H-image from HU_prev
+    // GD+k: the search directions of the last Ritz step -- the part of the Ritz
+    // block that came from the corrections, P = C Y_C, orthonormalised against
+    // the Ritz block in the coefficients of the old basis -- open the new
+    // correction basis C[0..k), with H P from [HX|HC] by the same coefficients.
+    // Built from the old X, C before the restart overwrites X.
     std::vector<Inst *> gk;
     for (Inst *in : rs)
         if (!in->init && in->have_prev) gk.push_back(in);
     const int ngk = (int)gk.size();
-    GramBatch pg, pg2;
+    CoefTask *hcf = nullptr, *dcf = nullptr;
+    hcf = A.take<CoefTask>(ngk, &dcf);
     CombBatch pc, pc2;
-    CholTask *pch = nullptr, *pcd = nullptr;
-    pg.h = A.take<GramTask>(ngk, &pg.d);
     pc.h = A.take<CombTask>(2 * ngk, &pc.d);
-    pg2.h = A.take<GramTask>(ngk, &pg2.d);
-    pch = A.take<CholTask>(ngk, &pcd);
     pc2.h = A.take<CombTask>(2 * ngk, &pc2.d);
-    // (pointers captured now: U/Up swap roles below, before these run)
-    std::vector<const void *> gk_up(ngk), gk_hup(ngk);
     for (int i = 0; i < ngk; ++i) {
         Inst *in = gk[i];
         const long long N = in->g.N;
-        double *h = in->sm() + oHp;
-        gk_up[i] = in->Up.p;
-        gk_hup[i] = in->HUp.p;
-        pg.add(in->X.p, k_, 0, in->Up.p, k_, 0, N, h, k_);
-        // T = Up - X h, T2 = HUp - HX h  (T, T2 first receive copies of Up, HUp)
-        pc.add(comb1(in->X.p, N, k_, 0, h, k_, in->T.p, 0, k_, 1));
-        pc.add(comb1(in->HX.p, N, k_, 0, h, k_, in->T2.p, 0, k_, 1));
-        pg2.add(in->T.p, k_, 0, in->T.p, k_, 0, N, in->sm() + oGt, k_);
-        pch[i] = CholTask{in->sm() + oGt, k_, 0, in->sm() + oWc, nullptr, in->keepT(), 1e-10,
-                          in->keep() + k_};
-        pc2.add(comb1(in->T.p, N, k_, 0, in->sm() + oWc, k_, in->C.p, e32, k_, 0));
-        pc2.add(comb1(in->T2.p, N, k_, 0, in->sm() + oWc, k_, in->HC.p, e32, k_, 0));
+        const int m = k_ + in->mc;  // basis of the last Ritz step
+        double *Q = in->sm() + oQp;
+        hcf[i] = CoefTask{in->sm() + oY, m, k_, Q, in->keep() + k_};
+        pc.add(CombTask{in->X.p, N, k_, 0, Q, m, in->C.p, N, in->mc, e32, Q + k_, m, in->T.p, N,
+                        0, k_, 0, N});
+        pc.add(CombTask{in->HX.p, N, k_, 0, Q, m, in->HC.p, N, in->mc, e32, Q + k_, m, in->T2.p,
+                        N, 0, k_, 0, N});
+        pc2.add(comb1(in->T.p, N, k_, 0, in->sm() + oI, k_, in->C.p, e32, k_, 0));
+        pc2.add(comb1(in->T2.p, N, k_, 0, in->sm() + oI, k_, in->HC.p, e32, k_, 0));
     }
     for (Inst *in : rs) {
-        in->refresh = in->init || lo;
+        in->refresh = in->init || lo || in->confirm;
+        in->confirm = false;
         in->mc = 0;
         in->nnew = 0;
         in->gnew = 0;
@@ -468,33 +503,37 @@ int BdEngine::step()
     }
 
     // ============================ phase A: launches
-    const long long part = std::max({rg[0].part, rg[1].part, pg.part, pg2.part, gg.part, res_part});
+    const long long part = std::max({rg[0].part, rg[1].part, gg.part, res_part});
     grow_part(sizeof(double) * (size_t)std::max(part, 1LL));
     A.flush(s_);
     double *dp = dpart_.as<double>();
+    if (ngk) {
+        bd_coef(dcf, ngk, s_);
+        pc.launch(s_);
+    }
     for (int pass = 0; pass < 2; ++pass) {
         rg[pass].launch(dp, s_);
-        if (nr) bd_cholinv(rcd[pass], nr, s_);
+        if (nr) {
+            ProfScope ps(PK_CHOL, s_, 0.0);
+            bd_cholinv(rcd[pass], nr, s_);
+        }
         rc[pass].launch(s_);
     }
-    if (ngk) {
-        for (int i = 0; i < ngk; ++i) {
-            Inst *in = gk[i];
-            PC_HIP(hipMemcpyAsync(in->T.p, gk_up[i], 8 * in->g.N * k_, hipMemcpyDeviceToDevice, s_));
-            PC_HIP(hipMemcpyAsync(in->T2.p, gk_hup[i], 8 * in->g.N * k_, hipMemcpyDeviceToDevice, s_));
-        }
-        pg.launch(dp, s_);
-        pc.launch(s_);
-        pg2.launch(dp, s_);
-        bd_cholinv(pcd, ngk, s_);
-        pc2.launch(s_);
-    }
-    if (hr.n) hx_launch(hr.d, hr.n, hr.pa, hr.pb, hr.mx, PISCES_FP64, s_);
-    if (hc.n) hx_launch(hc.d, hc.n, hc.pa, hc.pb, hc.mx, prec_, s_);
+    if (ngk) pc2.launch(s_);
+    hr.launch(PISCES_FP64, s_);
+    hc.launch(prec_, s_);
     gg.launch(dp, s_);
-    bd_eig(de, n, s_);
+    {
+        ProfScope ps(PK_EIG, s_, 0.0);
+        bd_eig(de, n, s_);
+    }
     rz.launch(s_);
-    bd_resid(drs, n, res_nch, dp, s_);
+    {
+        double rb = 0;
+        for (Inst *in : act) rb += 3.0 * 8 * k_ * in->g.N;  // U, HU read; R written
+        ProfScope ps(PK_RESID, s_, rb);
+        bd_resid(drs, n, res_nch, dp, s_);
+    }
     PC_HIP(hipMemcpyAsync(hsmall_, dgather_.p, sizeof(double) * 2 * kBdMaxK * n,
                           hipMemcpyDeviceToHost, s_));
     PC_HIP(hipStreamSynchronize(s_));
@@ -527,7 +566,7 @@ int BdEngine::step()
         in->nnew = 0;
         in->gnew = 0;
         in->nt = 0;
-        if (wanted && (!lo || was_fresh)) {
+        if (wanted && was_fresh) {
             in->done = true;
             in->res.status = in->nroots;
             in->res.evals.assign(in->theta, in->theta + in->nroots);
@@ -543,8 +582,12 @@ int BdEngine::step()
             in->res.resid.assign(in->rn, in->rn + in->nroots);
             continue;
         }
-        if (wanted) {  // low precision: confirm on an FP64-fresh HX
+        if (wanted) {
+            // H X and the Ritz block are kept by recurrence (and, in the low
+            // precision forms, carry the FP32/BF16 products): confirm on a
+            // freshly computed FP64 H X before declaring convergence
             in->restart = true;
+            in->confirm = true;
             continue;
         }
         int cnt = 0;
@@ -640,15 +683,24 @@ int BdEngine::step()
     grow_part(sizeof(double) * (size_t)partB);
     dp = dpart_.as<double>();
     B.flush(s_);
-    if (precond_ == 1) bd_precond(dpc, nc, maxN, s_);
-    else fdm_launch(dfd, nfd, fd_maxn, s_);
+    {
+        // R columns read once, corrections written once (+ diag H for the diagonal form)
+        double pb = 0;
+        for (Inst *in : corr) pb += 8.0 * in->g.N * (2 * in->nt + (precond_ == 1 ? 1 : 0));
+        ProfScope ps(PK_PREC, s_, pb);
+        if (precond_ == 1) bd_precond(dpc, nc, maxN, s_);
+        else fdm_launch(dfd, nfd, fd_maxn, s_);
+    }
     for (int pass = 0; pass < 2; ++pass) {
         og[pass].launch(dp, s_);
         oc[pass].launch(s_);
     }
     for (int pass = 0; pass < 2; ++pass) {
         cg[pass].launch(dp, s_);
-        bd_cholinv(chd[pass], nc, s_);
+        {
+            ProfScope ps(PK_CHOL, s_, 0.0);
+            bd_cholinv(chd[pass], nc, s_);
+        }
         cc[pass].launch(s_);
     }
     for (Inst *in : corr) {

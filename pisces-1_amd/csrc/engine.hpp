@@ -19,9 +19,12 @@
 //            themselves (Cholesky QR, twice), appended to C.
 //   restart: when C is full: X = U (re-orthonormalised), HX = HU; the
 //            low-precision forms then recompute HX = H X in FP64 (defect
-//            correction), so residuals and eigenvalues are FP64-exact.
+//            correction), so residuals and eigenvalues are FP64-exact.  The
+//            last search directions P = C Y_C, orthonormalised against the
+//            Ritz coefficients in the coefficient space of the old basis, open
+//            the new C (GD+k); H P comes from [HX|HC] by the same coefficients.
 //   converged: ||R_i|| < 10^-ptol for i < nroots, judged on an FP64-fresh HX
-//            in the low-precision forms.
+//            (every form: a restart that recomputes H X in FP64 confirms it).
 #pragma once
 
 #include <memory>

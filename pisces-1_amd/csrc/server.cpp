@@ -21,6 +21,7 @@
 #include "blockops.hpp"
 #include "context.hpp"
 #include "engine.hpp"
+#include "prof.hpp"
 
 using namespace pisces;
 
@@ -174,8 +175,12 @@ void build_potential(pisces_server *sv, const Pending &p, InstRes &res)
     sv->s_epc.ensure(sizeof(double) * 3 * ns);
     sv->s_tens.ensure(sizeof(double) * 6 * (size_t)ns * ns);
     sv->s_info.ensure(sizeof(int) * 2);
-    water_scf_launch(nw, sv->s_sites.as<double>(), sv->s_mu.as<double>(), sv->s_epc.as<double>(),
-                     sv->s_tens.as<double>(), sv->s_info.as<int>(), sv->stream);
+    {
+        ProfScope ps(PK_SCF, sv->stream, 0.0);
+        water_scf_launch(nw, sv->s_sites.as<double>(), sv->s_mu.as<double>(),
+                         sv->s_epc.as<double>(), sv->s_tens.as<double>(), sv->s_info.as<int>(),
+                         sv->stream);
+    }
     std::vector<double> q;
     std::vector<int> iq, id;
     water_lists_host(nw, q, iq, id);
@@ -204,6 +209,8 @@ void build_potential(pisces_server *sv, const Pending &p, InstRes &res)
              sv->s_mu.as<double>(), sv->s_id.as<int>(), sv->s_rep.as<double>(),
              sv->s_irep.as<int>(), sv->s_ipps.as<int>(), sv->s_ion.as<double>(),
              sv->s_ionq.as<double>()};
+    // V written in FP64 and FP32, then read once more with diag H written
+    ProfScope ps(PK_POT, sv->stream, (double)g.N * (8 + 4 + 8 + 8));
     potential_grid_launch(P, D, g.n[0], g.n[1], g.n[2], g.dx[0].as<double>(), g.dx[1].as<double>(),
                           g.dx[2].as<double>(), res.V64.as<double>(), res.V32.as<float>(), sv->stream);
     diag_h(res.V64.as<double>(), g.dT64[0].as<double>(), g.dT64[1].as<double>(),

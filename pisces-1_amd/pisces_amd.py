@@ -265,6 +265,34 @@ def start_vectors(ctx, nroots, seed=0):
     return S
 
 
+# ---------------------------------------------------------------- launch timing
+PROF_KINDS = 10
+
+
+def prof_enable(on=True):
+    L = lib()
+    L.pisces_prof_name.restype = ctypes.c_char_p
+    L.pisces_prof_read.argtypes = [ctypes.c_int, _D, ctypes.POINTER(ctypes.c_longlong), _D]
+    check(L.pisces_prof_enable(1 if on else 0), "pisces_prof_enable")
+
+
+def prof_reset():
+    check(lib().pisces_prof_reset(), "pisces_prof_reset")
+
+
+def prof_read():
+    """{family: (ms_total, launches, algorithmic_bytes_total)} of the timed launches."""
+    L = lib()
+    L.pisces_prof_name.restype = ctypes.c_char_p
+    out = {}
+    for k in range(PROF_KINDS):
+        ms, n, b = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+        check(L.pisces_prof_read(k, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b)),
+              "pisces_prof_read")
+        out[L.pisces_prof_name(k).decode()] = (ms.value, n.value, b.value)
+    return out
+
+
 # ---------------------------------------------------------------- server
 class Request(ctypes.Structure):
     _fields_ = [
