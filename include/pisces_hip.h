@@ -141,6 +141,15 @@ int pisces_apply_h_ex(pisces_ctx *ctx, const double *X, double *Y, int nvec, int
 int pisces_time_hx(pisces_ctx *ctx, int nvec, int precision, int warmup, int iters,
                    double *ms_per_apply);
 
+/* Kinetic preconditioner t = scale * (Tx (+) Ty (+) Tz - shift)^{-1} r for nvec
+ * host vectors (nvec*ngp doubles each way), applied through the eigenbases of
+ * the 1-D kinetic matrices with |lam - shift| floored at `floor`; precision
+ * PISCES_FP64 or PISCES_BF16X2.  The block solver's correction step; exposed
+ * for testing.  No reference counterpart (the reference corrects with the
+ * diagonal, src/Davidson.cpp:557-566). */
+int pisces_kinetic_precond(pisces_ctx *ctx, int nvec, const double *r, double shift,
+                           double scale, double floor, int precision, double *t);
+
 /* diag(H) on the host, ngp doubles.  Replaces DVR::ComputeDiagonal
  * (src/DVR.cpp:1808-1881). */
 int pisces_diag(pisces_ctx *ctx, double *diag);

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "fdm.hpp"
 
 using namespace pisces;
 
@@ -44,7 +45,7 @@ void pisces_ctx::hx(const void *X, void *Y, long long ld, int nvec, int precisio
     HxJob j;
     j.nx = n[0]; j.ny = n[1]; j.nz = n[2]; j.nvec = nvec;
     const bool f64 = (precision == PISCES_FP64);
-    for (int a = 0; a < 3; ++a) j.T[a] = f64 ? dT64[a].p : dT32[a].p;
+    for (int a = 0; a < 3; ++a) j.ax[a] = ax[a].m;
     j.V = f64 ? dV64.p : dV32.p;
     j.X = X; j.Y = Y; j.ld = ld;
     const HxJob *d = push_job(j);
@@ -125,13 +126,7 @@ int pisces_create(pisces_ctx **out, const int n[3], const double L[3], int devic
             sine_dvr_full(n[a], 1.0, -0.5 * L[a], 0.5 * L[a], c->x[a].data(), c->T[a].data());
             upload(c->dx[a], c->x[a].data(), sizeof(double) * n[a], c->stream);
             upload(c->dT64[a], c->T[a].data(), sizeof(double) * n[a] * n[a], c->stream);
-            std::vector<float> t32(c->T[a].begin(), c->T[a].end());
-            upload(c->dT32[a], t32.data(), sizeof(float) * n[a] * n[a], c->stream);
-            std::vector<double> Q((size_t)n[a] * n[a]), lam(n[a]);
-            syev_host(n[a], c->T[a].data(), n[a], lam.data(), Q.data(), n[a]);
-            upload(c->dQ[a], Q.data(), sizeof(double) * Q.size(), c->stream);
-            upload(c->dlam[a], lam.data(), sizeof(double) * lam.size(), c->stream);
-            PC_HIP(hipStreamSynchronize(c->stream));
+            c->ax[a].build(n[a], c->T[a].data(), c->stream);
         }
         PC_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->hjobs),
                              sizeof(HxJob) * pisces_ctx::kJobRing, hipHostMallocDefault));
@@ -351,6 +346,37 @@ int pisces_time_hx(pisces_ctx *c, int nvec, int precision, int warmup, int iters
         static_cast<void>(hipEventDestroy(e0));
         static_cast<void>(hipEventDestroy(e1));
         *ms = (double)t / iters;
+        return 0;
+    });
+}
+
+int pisces_kinetic_precond(pisces_ctx *c, int nvec, const double *r, double shift, double scale,
+                           double floor, int precision, double *t)
+{
+    if (!c || nvec <= 0 || !r || !t || !(floor > 0)) return PISCES_ERR_ARG;
+    if (precision != PISCES_FP64 && precision != PISCES_BF16X2) return PISCES_ERR_ARG;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        const size_t bytes = sizeof(double) * c->ngp * (size_t)nvec;
+        dbuf R(bytes), Tb(bytes), W(sizeof(float) * c->ngp * (size_t)nvec);
+        dbuf tasks(sizeof(FdmTask) * nvec);
+        std::vector<FdmTask> h(nvec);
+        for (int v = 0; v < nvec; ++v) {
+            FdmTask &f = h[v];
+            for (int a = 0; a < 3; ++a) f.ax[a] = c->ax[a].m;
+            f.src = R.as<double>() + c->ngp * v;
+            f.dst = Tb.as<double>() + c->ngp * v;
+            f.work = W.as<float>() + c->ngp * v;
+            f.shift = shift;
+            f.scale = scale;
+            f.floor = floor;
+        }
+        PC_HIP(hipMemcpyAsync(R.p, r, bytes, hipMemcpyHostToDevice, c->stream));
+        PC_HIP(hipMemcpyAsync(tasks.p, h.data(), sizeof(FdmTask) * nvec, hipMemcpyHostToDevice,
+                              c->stream));
+        fdm_launch(tasks.as<FdmTask>(), nvec, c->maxn, precision, c->stream);
+        PC_HIP(hipMemcpyAsync(t, Tb.p, bytes, hipMemcpyDeviceToHost, c->stream));
+        c->sync();
         return 0;
     });
 }

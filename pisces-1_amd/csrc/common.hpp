@@ -65,11 +65,39 @@ struct dbuf {
     template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
+// --------------------------------------------------------------- axis operators
+// Device operand matrices of one grid axis with n points, zero padded to
+// np = n rounded up to a multiple of 32, row-major (element (r, c) at r*np + c):
+// the kinetic matrix T (symmetric) in FP64, FP32 and split BF16 (hi = bf16(t),
+// lo = bf16(t - hi) of the FP32 value), and the eigenvectors Q (T = Q diag(lam)
+// Q^T, eigenvectors in the columns) and Q^T in FP64 and split BF16.  BF16 is
+// held as its 16-bit pattern.
+struct AxisMats {
+    int n = 0, np = 0;
+    const double *t64 = nullptr;
+    const float *t32 = nullptr;
+    const uint16_t *th = nullptr, *tl = nullptr;
+    const double *q64 = nullptr, *qt64 = nullptr;
+    const uint16_t *qh = nullptr, *ql = nullptr, *qth = nullptr, *qtl = nullptr;
+    const double *lam = nullptr;
+};
+
+// Owner of one axis's device matrices.
+struct AxisStore {
+    dbuf buf;
+    AxisMats m;
+    // T: n x n FP64 kinetic matrix (row-major, symmetric); uploads on stream s
+    // and waits for the copy.
+    void build(int n, const double *T, hipStream_t s);
+};
+
+inline int axis_pad(int n) { return (n + 31) & ~31; }
+
 // --------------------------------------------------------------- H*X
 // One instance's share of an H*X launch: Y[:,v] = H X[:,v], v < nvec.
 struct HxJob {
     int nx, ny, nz, nvec;
-    const void *T[3];   // full symmetric kinetic matrices, n_a*n_a, in the job precision
+    AxisMats ax[3];     // kinetic operators of x, y, z
     const void *V;      // diagonal potential (ngp), job precision
     const void *X;      // vectors, X + v*ld
     void *Y;
@@ -79,6 +107,9 @@ struct HxJob {
 // precision codes: PISCES_FP64 / PISCES_FP32 / PISCES_BF16X2
 void hx_launch(const HxJob *d_jobs, int njobs, int max_planes_x, int max_planes_z,
                int max_n, int precision, hipStream_t s);
+// matrix-core plane kernels (both passes)
+void hx_mfma_launch(const HxJob *d_jobs, int njobs, int max_planes_a, int max_planes_b,
+                    int max_n, int precision, hipStream_t s);
 // Kernel variant selector (0 = VALU LDS-tiled, 1 = MFMA); default 1 where available.
 void hx_set_variant(int v);
 int hx_get_variant();

@@ -19,8 +19,8 @@ struct pisces_ctx {
     long long ngp = 0;
     int maxn = 0;
     std::vector<double> x[3], T[3];  // host copies
-    pisces::dbuf dx[3], dT64[3], dT32[3];
-    pisces::dbuf dQ[3], dlam[3];  // eigenvectors / eigenvalues of T_a (kinetic preconditioner)
+    pisces::dbuf dx[3], dT64[3];
+    pisces::AxisStore ax[3];  // padded operand matrices of the plane kernels
     pisces::dbuf dV64, dV32;
     bool has_v = false;
 

@@ -439,7 +439,7 @@ int BdEngine::step()
         j.ld = in->g.N;
         if (in->refresh) {
             j.nvec = k_;
-            for (int a = 0; a < 3; ++a) j.T[a] = in->g.T64[a];
+            for (int a = 0; a < 3; ++a) j.ax[a] = in->g.ax[a];
             j.V = in->g.V64;
             j.X = in->X.p;
             j.Y = in->HX.p;
@@ -449,7 +449,7 @@ int BdEngine::step()
         }
         if (in->nnew > 0) {
             j.nvec = in->nnew;
-            for (int a = 0; a < 3; ++a) j.T[a] = lo ? in->g.T32[a] : in->g.T64[a];
+            for (int a = 0; a < 3; ++a) j.ax[a] = in->g.ax[a];
             j.V = lo ? (const void *)in->g.V32 : (const void *)in->g.V64;
             j.X = in->C.as<unsigned char>() + e * in->g.N * (in->mc - in->nnew);
             j.Y = in->HC.as<unsigned char>() + e * in->g.N * (in->mc - in->nnew);
@@ -630,13 +630,11 @@ int BdEngine::step()
             fd_maxn = std::max({fd_maxn, in->g.n[0], in->g.n[1], in->g.n[2]});
             for (int t = 0; t < in->nt; ++t) {
                 FdmTask f{};
-                for (int a = 0; a < 3; ++a) {
-                    f.n[a] = in->g.n[a];
-                    f.Q[a] = in->g.Q[a];
-                    f.lam[a] = in->g.lam[a];
-                }
+                for (int a = 0; a < 3; ++a) f.ax[a] = in->g.ax[a];
                 f.src = in->R.as<double>() + in->g.N * in->src[t];
                 f.dst = in->T.as<double>() + in->g.N * t;
+                // T2 is free until the Cholesky-QR below: FP32 scratch of the BF16 form
+                f.work = reinterpret_cast<float *>(in->T2.p) + in->g.N * t;
                 // t = -(T - theta)^{-1} r
                 f.shift = in->theta[in->src[t]];
                 f.scale = -1.0;
@@ -689,7 +687,7 @@ int BdEngine::step()
         for (Inst *in : corr) pb += 8.0 * in->g.N * (2 * in->nt + (precond_ == 1 ? 1 : 0));
         ProfScope ps(PK_PREC, s_, pb);
         if (precond_ == 1) bd_precond(dpc, nc, maxN, s_);
-        else fdm_launch(dfd, nfd, fd_maxn, s_);
+        else fdm_launch(dfd, nfd, fd_maxn, lo ? PISCES_BF16X2 : PISCES_FP64, s_);
     }
     for (int pass = 0; pass < 2; ++pass) {
         og[pass].launch(dp, s_);

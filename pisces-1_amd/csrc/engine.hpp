@@ -39,13 +39,10 @@ namespace pisces {
 struct BdGrid {
     int n[3] = {0, 0, 0};
     long long N = 0;
-    const void *T64[3] = {nullptr, nullptr, nullptr};
-    const void *T32[3] = {nullptr, nullptr, nullptr};
+    AxisMats ax[3];             // kinetic operators and eigenbases of the axes
     const double *V64 = nullptr;
     const float *V32 = nullptr;
     const double *D = nullptr;  // diag(H)
-    const double *Q[3] = {nullptr, nullptr, nullptr};    // eigenvectors of T_a
-    const double *lam[3] = {nullptr, nullptr, nullptr};  // eigenvalues of T_a
 };
 
 struct BdResult {

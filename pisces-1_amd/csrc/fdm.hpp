@@ -6,18 +6,19 @@
 namespace pisces {
 
 // dst = scale * (Tx (+) Ty (+) Tz - shift)^{-1} src for one grid vector, with
-// |eigenvalue - shift| floored at `floor`.  Q[a]: eigenvectors of T_a (column
-// major, n_a x n_a), lam[a]: its eigenvalues.
+// |lam_x + lam_y + lam_z - shift| floored at `floor`, applied through the
+// eigenbases of the 1-D kinetic matrices (ax[a].q64 / qt64 / split BF16 and
+// ax[a].lam).  work: ngp floats of scratch (split-BF16 form only).  The FP64
+// form works in place in dst.
 struct FdmTask {
-    int n[3];
-    int pad;
-    const double *Q[3];
-    const double *lam[3];
+    AxisMats ax[3];
     const double *src;
     double *dst;
+    float *work;
     double shift, scale, floor;
 };
 
-void fdm_launch(const FdmTask *d_tasks, int ntasks, int max_n, hipStream_t s);
+// precision: PISCES_FP64 (FP64 matrix cores) or PISCES_BF16X2 (split BF16)
+void fdm_launch(const FdmTask *d_tasks, int ntasks, int max_n, int precision, hipStream_t s);
 
 }  // namespace pisces

@@ -34,7 +34,8 @@ struct GridShape {
     double L[3];
     long long N;
     std::vector<double> x[3];
-    dbuf dx[3], dT64[3], dT32[3], dQ[3], dlam[3];
+    dbuf dx[3], dT64[3];
+    AxisStore ax[3];
     std::vector<double> piab;
 };
 
@@ -120,16 +121,11 @@ std::shared_ptr<GridShape> shape_of(pisces_server *sv, const pisces_request &r)
         g->L[a] = r.L[a];
         g->N *= r.n[a];
         g->x[a].resize(r.n[a]);
-        std::vector<double> T((size_t)r.n[a] * r.n[a]), Q(T.size()), lam(r.n[a]);
+        std::vector<double> T((size_t)r.n[a] * r.n[a]);
         sine_dvr_full(r.n[a], 1.0, -0.5 * r.L[a], 0.5 * r.L[a], g->x[a].data(), T.data());
-        std::vector<float> T32(T.begin(), T.end());
-        syev_host(r.n[a], T.data(), r.n[a], lam.data(), Q.data(), r.n[a]);
         upload(g->dx[a], g->x[a].data(), sizeof(double) * r.n[a], sv->stream);
         upload(g->dT64[a], T.data(), sizeof(double) * T.size(), sv->stream);
-        upload(g->dT32[a], T32.data(), sizeof(float) * T32.size(), sv->stream);
-        upload(g->dQ[a], Q.data(), sizeof(double) * Q.size(), sv->stream);
-        upload(g->dlam[a], lam.data(), sizeof(double) * lam.size(), sv->stream);
-        PC_HIP(hipStreamSynchronize(sv->stream));
+        g->ax[a].build(r.n[a], T.data(), sv->stream);  // waits for the uploads
     }
     // half-box particle-in-a-box start vector (DVR::ParticleInAnDBoxWf,
     // src/DVR.cpp:1725-1805)
@@ -235,10 +231,7 @@ void admit(pisces_server *sv)
         BdGrid g;
         for (int a = 0; a < 3; ++a) {
             g.n[a] = res->shape->n[a];
-            g.T64[a] = res->shape->dT64[a].p;
-            g.T32[a] = res->shape->dT32[a].p;
-            g.Q[a] = res->shape->dQ[a].as<double>();
-            g.lam[a] = res->shape->dlam[a].as<double>();
+            g.ax[a] = res->shape->ax[a].m;
         }
         g.N = N;
         g.V64 = res->V64.as<double>();

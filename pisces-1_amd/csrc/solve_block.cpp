@@ -12,8 +12,7 @@ BdGrid grid_of(pisces_ctx *c)
     BdGrid g;
     for (int a = 0; a < 3; ++a) {
         g.n[a] = c->n[a];
-        g.T64[a] = c->dT64[a].p;
-        g.T32[a] = c->dT32[a].p;
+        g.ax[a] = c->ax[a].m;
     }
     g.N = c->ngp;
     g.V64 = c->dV64.as<double>();
@@ -22,10 +21,6 @@ BdGrid grid_of(pisces_ctx *c)
     diag_h(c->dV64.as<double>(), c->dT64[0].as<double>(), c->dT64[1].as<double>(),
            c->dT64[2].as<double>(), c->n[0], c->n[1], c->n[2], c->wdiag.as<double>(), c->stream);
     g.D = c->wdiag.as<double>();
-    for (int a = 0; a < 3; ++a) {
-        g.Q[a] = c->dQ[a].as<double>();
-        g.lam[a] = c->dlam[a].as<double>();
-    }
     return g;
 }
 

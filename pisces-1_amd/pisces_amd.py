@@ -192,6 +192,18 @@ class Context:
               "pisces_apply_h_ex")
         return Y
 
+    def kinetic_precond(self, r, shift, scale=-1.0, floor=1e-3, precision=FP64):
+        """t = scale (T - shift)^{-1} r through the 1-D eigenbases (pisces_kinetic_precond)."""
+        r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(-1, self.ngp))
+        t = np.zeros_like(r)
+        L = lib()
+        L.pisces_kinetic_precond.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_int, _D]
+        check(L.pisces_kinetic_precond(self.h, r.shape[0], _p(r), float(shift), float(scale),
+                                       float(floor), int(precision), _p(t)),
+              "pisces_kinetic_precond")
+        return t
+
     def time_hx(self, nvec, precision, warmup=5, iters=50):
         ms = ctypes.c_double(0)
         check(lib().pisces_time_hx(self.h, int(nvec), int(precision), int(warmup), int(iters),
