@@ -61,8 +61,9 @@ int pisces_debug_set_hx_variant(int variant);
  * families with HIP events on the stream they run on and accumulates time,
  * launch count and algorithmic HBM bytes (operands read once, results written
  * once) per family.  kind: 0 H*X FP64, 1 H*X FP32/BF16X2, 2 Gram, 3 block
- * combination, 4 projected eigensolve, 5 residuals, 6 preconditioner,
- * 7 Cholesky-QR, 8 grid potential, 9 induced-dipole solve.  Measurement hook
+ * combination, 4 projected eigensolve, 5 Ritz vectors + residuals, 6
+ * preconditioner, 7 small factors (Cholesky, orthogonalisation), 8 grid
+ * potential, 9 induced-dipole solve.  Measurement hook
  * for the benchmark; no reference counterpart. */
 int pisces_prof_enable(int on);
 int pisces_prof_reset(void);

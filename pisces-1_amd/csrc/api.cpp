@@ -365,6 +365,8 @@ int pisces_kinetic_precond(pisces_ctx *c, int nvec, const double *r, double shif
             FdmTask &f = h[v];
             for (int a = 0; a < 3; ++a) f.ax[a] = c->ax[a].m;
             f.src = R.as<double>() + c->ngp * v;
+            f.src32 = 0;
+            f.dst32 = 0;
             f.dst = Tb.as<double>() + c->ngp * v;
             f.work = W.as<float>() + c->ngp * v;
             f.shift = shift;

@@ -66,20 +66,29 @@ __device__ __forceinline__ void gram_rows(const TA *__restrict__ A, long long ld
 }
 
 // Butterfly reduce-scatter of 64 values per lane over a 64-lane wave: after it,
-// lane L holds the wave-wide sum of value L.
+// lane L holds the wave-wide sum of value L.  Each step halves the values a
+// lane keeps: it sends the half its partner keeps and adds the half it gets.
+template <int OFF>
+__device__ __forceinline__ void rs_step(double (&v)[64], int lane)
+{
+    const bool up = (lane & OFF) != 0;
+#pragma unroll
+    for (int k = 0; k < OFF; ++k) {
+        const double keep = up ? v[k + OFF] : v[k];
+        const double give = up ? v[k] : v[k + OFF];
+        v[k] = keep + __shfl_xor(give, OFF, 64);
+    }
+}
+
 __device__ __forceinline__ double wave_reduce_scatter64(double (&v)[64])
 {
     const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const bool up = (lane & off) != 0;
-#pragma unroll
-        for (int k = 0; k < off; ++k) {
-            const double keep = up ? v[k + off] : v[k];
-            const double give = up ? v[k] : v[k + off];
-            v[k] = keep + __shfl_xor(give, off, 64);
-        }
-    }
+    rs_step<32>(v, lane);
+    rs_step<16>(v, lane);
+    rs_step<8>(v, lane);
+    rs_step<4>(v, lane);
+    rs_step<2>(v, lane);
+    rs_step<1>(v, lane);
     return v[0];
 }
 
@@ -141,11 +150,15 @@ __global__ void __launch_bounds__(kT) gram_reduce_kernel(const GramTask *__restr
 __global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ tasks)
 {
     const CombTask t = tasks[blockIdx.y];
-    __shared__ double w[kBdMaxM * kBdMaxK];
-    const int ma = t.ma1 + t.ma2;
+    __shared__ double w[(kBdMaxM + kBdMaxK) * kBdMaxK];
+    const int ma = t.ma1 + t.ma2 + t.ma3;
     for (int idx = threadIdx.x; idx < ma * t.mb; idx += kT) {
         const int a = idx % ma, b = idx / ma;
-        w[idx] = (a < t.ma1) ? t.W1[a + (long long)t.ldw1 * b] : t.W2[(a - t.ma1) + (long long)t.ldw2 * b];
+        double v;
+        if (a < t.ma1) v = t.W1[a + (long long)t.ldw1 * b];
+        else if (a < t.ma1 + t.ma2) v = t.W2[(a - t.ma1) + (long long)t.ldw2 * b];
+        else v = t.W3[(a - t.ma1 - t.ma2) + (long long)t.ldw3 * b];
+        w[idx] = v;
     }
     __syncthreads();
     const long long g = (long long)blockIdx.x * kT + threadIdx.x;
@@ -164,6 +177,12 @@ __global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ t
 #pragma unroll
         for (int b = 0; b < kBdMaxK; ++b)
             if (b < t.mb) acc[b] += x * w[t.ma1 + a + ma * b];
+    }
+    for (int a = 0; a < t.ma3; ++a) {
+        const double x = ld_any(t.A3, a * t.lda3 + g, t.a3_32);
+#pragma unroll
+        for (int b = 0; b < kBdMaxK; ++b)
+            if (b < t.mb) acc[b] += x * w[t.ma1 + t.ma2 + a + ma * b];
     }
 #pragma unroll
     for (int b = 0; b < kBdMaxK; ++b) {
@@ -184,21 +203,27 @@ __device__ inline int rr_index(int i, int r, int mp)
     return (i == 0) ? 0 : ((i - 1 + r) % (mp - 1)) + 1;
 }
 
-constexpr int kEigT = 64;
+constexpr int kEigT = 256;
 
+// Cyclic Jacobi with round-robin pairing: every round applies m/2 disjoint
+// rotations J as A <- J^T A J in one pass over 2 x 2 blocks (rows of one
+// pair, columns of another: each block reads and writes only its own four
+// entries) and V <- V J.  A pair is rotated only while its off-diagonal entry
+// is above 1e-17 (|a_pp| + |a_qq|); sweeps end after one with no rotation or
+// once the off-diagonal Frobenius norm is 1e-14 of the total.
 __global__ void __launch_bounds__(kEigT) eig_kernel(const EigTask *__restrict__ tasks)
 {
-    const EigTask t = tasks[blockIdx.x];
+    const EigTask &t = tasks[blockIdx.x];
     constexpr int LD = kBdMaxM + 1;
     __shared__ double A[kBdMaxM * LD];
     __shared__ double V[kBdMaxM * LD];
     __shared__ double cs[kBdMaxM / 2], sn[kBdMaxM / 2];
     __shared__ int pp[kBdMaxM / 2], qq[kBdMaxM / 2];
-    __shared__ double red[kEigT];
+    __shared__ double red[kEigT / 64][2];
     __shared__ int order[kBdMaxM];
-    __shared__ int s_stop;
-    const int m = t.m;
-    for (int idx = threadIdx.x; idx < m * m; idx += kEigT) {
+    __shared__ int s_stop, s_rot;
+    const int m = t.m, tid = threadIdx.x;
+    for (int idx = tid; idx < m * m; idx += kEigT) {
         const int i = idx % m, j = idx / m;
         double g = (i <= j) ? t.G[i + (long long)t.ldg * j] : t.G[j + (long long)t.ldg * i];
         const bool di = t.keep && !t.keep[i];
@@ -210,93 +235,107 @@ __global__ void __launch_bounds__(kEigT) eig_kernel(const EigTask *__restrict__ 
     __syncthreads();
     const int mp = m + (m & 1);
     const int npair = mp / 2;
+    const int lane = tid & 63, wave = tid >> 6;
     for (int sweep = 0; sweep < 40; ++sweep) {
         double off = 0.0, tot = 0.0;
-        for (int idx = threadIdx.x; idx < m * m; idx += kEigT) {
+        for (int idx = tid; idx < m * m; idx += kEigT) {
             const int i = idx % m, j = idx / m;
             const double a = A[i + LD * j];
             if (i != j && fabs(A[i + LD * i]) < 1e29 && fabs(A[j + LD * j]) < 1e29) off += a * a;
             else if (i == j && fabs(a) < 1e29) tot += a * a;
         }
-        red[threadIdx.x] = off;
-        __syncthreads();
-        for (int w = kEigT / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
+        for (int o = 32; o >= 1; o >>= 1) {
+            off += __shfl_xor(off, o, 64);
+            tot += __shfl_xor(tot, o, 64);
         }
-        const double offs = red[0];
-        __syncthreads();
-        red[threadIdx.x] = tot;
-        __syncthreads();
-        for (int w = kEigT / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
+        if (lane == 0) {
+            red[wave][0] = off;
+            red[wave][1] = tot;
         }
-        if (threadIdx.x == 0) s_stop = (offs <= 1e-30 * (red[0] + offs)) ? 1 : 0;
+        __syncthreads();
+        if (tid == 0) {
+            double so = 0, st = 0;
+            for (int w = 0; w < kEigT / 64; ++w) {
+                so += red[w][0];
+                st += red[w][1];
+            }
+            s_stop = (so <= 1e-28 * (st + so)) ? 1 : 0;
+            s_rot = 0;
+        }
         __syncthreads();
         if (s_stop) break;
         for (int r = 0; r < mp - 1; ++r) {
-            if (threadIdx.x < npair) {
-                int p = rr_index(threadIdx.x, r, mp), q = rr_index(mp - 1 - threadIdx.x, r, mp);
+            if (tid < npair) {
+                int p = rr_index(tid, r, mp), q = rr_index(mp - 1 - tid, r, mp);
                 if (p > q) { const int x = p; p = q; q = x; }
                 double c = 1.0, s = 0.0;
                 if (q < m) {
                     const double apq = A[p + LD * q];
-                    if (apq != 0.0) {
-                        const double th = (A[q + LD * q] - A[p + LD * p]) / (2.0 * apq);
+                    const double app = A[p + LD * p], aqq = A[q + LD * q];
+                    if (fabs(apq) > 1e-17 * (fabs(app) + fabs(aqq)) + 1e-300) {
+                        const double th = (aqq - app) / (2.0 * apq);
                         double tt;
                         if (fabs(th) > 1e150) tt = 0.5 / th;
                         else tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
                         c = 1.0 / sqrt(tt * tt + 1.0);
                         s = tt * c;
+                        s_rot = 1;  // every writer stores 1
                     }
                 }
-                pp[threadIdx.x] = p;
-                qq[threadIdx.x] = q;
-                cs[threadIdx.x] = c;
-                sn[threadIdx.x] = s;
+                pp[tid] = p;
+                qq[tid] = q;
+                cs[tid] = c;
+                sn[tid] = s;
             }
             __syncthreads();
-            for (int item = threadIdx.x; item < m * npair; item += kEigT) {
+            for (int b = tid; b < npair * npair; b += kEigT) {
+                const int P1 = b % npair, P2 = b / npair;
+                const double s1 = sn[P1], s2 = sn[P2];
+                if (s1 == 0.0 && s2 == 0.0) continue;
+                const double c1 = cs[P1], c2 = cs[P2];
+                const int p1 = pp[P1], q1 = qq[P1], p2 = pp[P2], q2 = qq[P2];
+                const bool v1 = q1 < m, v2 = q2 < m;
+                const double b00 = A[p1 + LD * p2];
+                const double b01 = v2 ? A[p1 + LD * q2] : 0.0;
+                const double b10 = v1 ? A[q1 + LD * p2] : 0.0;
+                const double b11 = (v1 && v2) ? A[q1 + LD * q2] : 0.0;
+                const double x00 = b00 * c2 - b01 * s2, x01 = b00 * s2 + b01 * c2;
+                const double x10 = b10 * c2 - b11 * s2, x11 = b10 * s2 + b11 * c2;
+                A[p1 + LD * p2] = c1 * x00 - s1 * x10;
+                if (v2) A[p1 + LD * q2] = c1 * x01 - s1 * x11;
+                if (v1) A[q1 + LD * p2] = s1 * x00 + c1 * x10;
+                if (v1 && v2) A[q1 + LD * q2] = s1 * x01 + c1 * x11;
+            }
+            for (int item = tid; item < m * npair; item += kEigT) {
                 const int k = item % m, pr = item / m;
+                const double s = sn[pr];
+                if (s == 0.0) continue;
                 const int p = pp[pr], q = qq[pr];
-                if (q >= m || sn[pr] == 0.0) continue;
-                const double c = cs[pr], s = sn[pr];
-                const double akp = A[k + LD * p], akq = A[k + LD * q];
-                A[k + LD * p] = c * akp - s * akq;
-                A[k + LD * q] = s * akp + c * akq;
+                const double c = cs[pr];
                 const double vkp = V[k + LD * p], vkq = V[k + LD * q];
                 V[k + LD * p] = c * vkp - s * vkq;
                 V[k + LD * q] = s * vkp + c * vkq;
             }
             __syncthreads();
-            for (int item = threadIdx.x; item < m * npair; item += kEigT) {
-                const int k = item % m, pr = item / m;
-                const int p = pp[pr], q = qq[pr];
-                if (q >= m || sn[pr] == 0.0) continue;
-                const double c = cs[pr], s = sn[pr];
-                const double apk = A[p + LD * k], aqk = A[q + LD * k];
-                A[p + LD * k] = c * apk - s * aqk;
-                A[q + LD * k] = s * apk + c * aqk;
-            }
-            __syncthreads();
         }
+        if (!s_rot) break;  // uniform: read after the round's final barrier
     }
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < m; ++i) order[i] = i;
-        for (int i = 0; i < m - 1; ++i) {
-            int b = i;
-            for (int j = i + 1; j < m; ++j)
-                if (A[order[j] + LD * order[j]] < A[order[b] + LD * order[b]]) b = j;
-            const int x = order[i]; order[i] = order[b]; order[b] = x;
+    // ascending order by rank (ties by index)
+    if (tid < m) {
+        const double di = A[tid + LD * tid];
+        int rank = 0;
+        for (int j = 0; j < m; ++j) {
+            const double dj = A[j + LD * j];
+            rank += (dj < di || (dj == di && j < tid)) ? 1 : 0;
         }
+        order[rank] = tid;
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < m * t.k; idx += kEigT) {
+    for (int idx = tid; idx < m * t.k; idx += kEigT) {
         const int i = idx % m, c = idx / m;
         t.Y[i + (long long)m * c] = V[i + LD * order[c]];
     }
-    if (threadIdx.x < t.k) t.theta[threadIdx.x] = A[order[threadIdx.x] + LD * order[threadIdx.x]];
+    if (tid < t.k) t.theta[tid] = A[order[tid] + LD * order[tid]];
 }
 
 // ------------------------------------------------------------------ residual
@@ -350,6 +389,177 @@ __global__ void __launch_bounds__(64) resid_reduce_kernel(const ResTask *__restr
     }
 }
 
+// ------------------------------------------------------------------ Ritz vectors + residuals
+// KB >= k: block size rounded up; coefficient columns k .. KB-1 are zero.
+template <int KB>
+__global__ void __launch_bounds__(kT) ritz_kernel(const RitzTask *__restrict__ tasks,
+                                                  double *__restrict__ partial)
+{
+    const RitzTask &t = tasks[blockIdx.y];
+    if ((int)blockIdx.x >= t.nch) return;
+    const int k = t.k, mc = t.mc, m = k + mc;
+    __shared__ double y[kBdMaxM * KB];  // y[a][b], b fastest
+    __shared__ double th[KB];
+    __shared__ double red[KB][kT / 64];
+    for (int idx = threadIdx.x; idx < m * KB; idx += kT) {
+        const int a = idx / KB, b = idx - a * KB;
+        y[idx] = b < k ? t.Y[a + (long long)m * b] : 0.0;
+    }
+    if (threadIdx.x < KB) th[threadIdx.x] = threadIdx.x < k ? t.theta[threadIdx.x] : 0.0;
+    __syncthreads();
+    // one row per thread (rows of a block are consecutive: coalesced columns)
+    const long long g = (long long)blockIdx.x * kRitzRows + threadIdx.x;
+    const bool row = g < t.N;
+    double u[KB], hu[KB];
+#pragma unroll
+    for (int b = 0; b < KB; ++b) u[b] = hu[b] = 0.0;
+    if (row) {
+        for (int a = 0; a < k; ++a) {
+            const double x = t.X[a * t.N + g], hx = t.HX[a * t.N + g];
+            const double *w = y + a * KB;
+#pragma unroll
+            for (int b = 0; b < KB; ++b) {
+                u[b] += x * w[b];
+                hu[b] += hx * w[b];
+            }
+        }
+        for (int c = 0; c < mc; ++c) {
+            const double x = ld_any(t.C, c * t.N + g, t.c32), hx = ld_any(t.HC, c * t.N + g, t.c32);
+            const double *w = y + (k + c) * KB;
+#pragma unroll
+            for (int b = 0; b < KB; ++b) {
+                u[b] += x * w[b];
+                hu[b] += hx * w[b];
+            }
+        }
+    }
+    double nrm[KB];
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+        nrm[b] = 0.0;
+        if (b < k && row) {
+            const double r = hu[b] - th[b] * u[b];
+            t.U[b * t.N + g] = u[b];
+            t.HU[b * t.N + g] = hu[b];
+            st_any(t.R, b * t.N + g, t.r32, r);
+            nrm[b] = r * r;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+        double v = nrm[b];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[b][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < k) {
+        double sum = 0;
+        for (int w = 0; w < kT / 64; ++w) sum += red[threadIdx.x][w];
+        partial[t.part_off + (long long)threadIdx.x * t.nch + blockIdx.x] = sum;
+    }
+}
+
+__global__ void __launch_bounds__(64) ritz_reduce_kernel(const RitzTask *__restrict__ tasks,
+                                                         const double *__restrict__ partial)
+{
+    const RitzTask &t = tasks[blockIdx.x];
+    if ((int)threadIdx.x >= t.k) return;
+    double sum = 0;
+    for (int c = 0; c < t.nch; ++c) sum += partial[t.part_off + (long long)threadIdx.x * t.nch + c];
+    t.out[threadIdx.x] = sum;
+    if (t.gather) {
+        t.gather[threadIdx.x] = t.theta[threadIdx.x];
+        t.gather[kBdMaxK + threadIdx.x] = sum;
+    }
+}
+
+// ------------------------------------------------------------------ orthogonalisation factor
+// One 64-lane workgroup per task: S = G - h^T h (h may be absent), Cholesky
+// with dropping, W = L^{-T}, F = -h W.  Shared by the Cholesky-QR factor.
+__device__ void orth_factor(const double *__restrict__ hg, const double *__restrict__ G, int m,
+                            int nt, double *__restrict__ Wo, double *__restrict__ Fo,
+                            const int *kin, int *kout, int *kcopy, double tol)
+{
+    constexpr int K = kBdMaxK;
+    __shared__ double h[kBdMaxM * kBdMaxK];
+    __shared__ double S[K * K], L[K * K], W[K * K];
+    __shared__ int keep[K];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < m * nt; idx += 64) h[idx] = hg[idx];
+    __syncthreads();
+    for (int idx = tid; idx < nt * nt; idx += 64) {
+        const int i = idx % nt, j = idx / nt;
+        double v = G[i + (long long)nt * j];
+        for (int a = 0; a < m; ++a) v -= h[a + m * i] * h[a + m * j];
+        S[i + K * j] = v;
+        L[i + K * j] = 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < nt; ++j) {
+        if (tid == 0) {
+            const double d = S[j + K * j];
+            const int ki = kin ? kin[j] : 1;
+            const int kj = (ki && d > tol * G[j + (long long)nt * j] && d > 0.0) ? 1 : 0;
+            keep[j] = kj;
+            L[j + K * j] = kj ? sqrt(d) : 0.0;
+        }
+        __syncthreads();
+        if (keep[j]) {
+            const double ljj = L[j + K * j];
+            for (int i = j + 1 + tid; i < nt; i += 64) L[i + K * j] = S[i + K * j] / ljj;
+            __syncthreads();
+            // trailing update of the lower triangle
+            for (int idx = tid; idx < nt * nt; idx += 64) {
+                const int i = idx % nt, l = idx / nt;
+                if (l > j && i >= l) S[i + K * l] -= L[i + K * j] * L[l + K * j];
+            }
+        }
+        __syncthreads();
+    }
+    // W = L^{-T} (upper triangular), one column per thread
+    if (tid < nt) {
+        const int c = tid;
+        for (int i = nt - 1; i >= 0; --i) {
+            if (!keep[i] || !keep[c]) {
+                W[i + K * c] = 0.0;
+                continue;
+            }
+            double v = (i == c) ? 1.0 : 0.0;
+            for (int p = i + 1; p < nt; ++p) v -= L[p + K * i] * W[p + K * c];
+            W[i + K * c] = v / L[i + K * i];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nt * nt; idx += 64) {
+        const int i = idx % nt, c = idx / nt;
+        Wo[i + (long long)nt * c] = W[i + K * c];
+    }
+    if (Fo)
+        for (int idx = tid; idx < m * nt; idx += 64) {
+            const int a = idx % m, c = idx / m;
+            double f = 0.0;
+            for (int i = 0; i < nt; ++i) f -= h[a + m * i] * W[i + K * c];
+            Fo[a + (long long)m * c] = f;
+        }
+    if (tid < nt) {
+        kout[tid] = keep[tid];
+        if (kcopy) kcopy[tid] = keep[tid];
+    }
+}
+
+__global__ void __launch_bounds__(64) orth_kernel(const OrthTask *__restrict__ tasks)
+{
+    const OrthTask &t = tasks[blockIdx.x];
+    orth_factor(t.h, t.G, t.m, t.nt, t.W, t.F, t.keep_in, t.keep_out, t.keep_copy, t.tol);
+}
+
+__global__ void __launch_bounds__(64) cholinv_batched_kernel(const CholTask *__restrict__ tasks)
+{
+    const CholTask &t = tasks[blockIdx.x];
+    orth_factor(nullptr, t.G, 0, t.k, t.W, nullptr, t.keep_in, t.keep_out, t.keep_copy, t.tol);
+}
+
 // ------------------------------------------------------------------ correction
 __global__ void __launch_bounds__(kT) prec_kernel(const PrecTask *__restrict__ tasks)
 {
@@ -362,42 +572,6 @@ __global__ void __launch_bounds__(kT) prec_kernel(const PrecTask *__restrict__ t
         double dv = t.theta[r] - dg;
         if (fabs(dv) < 1e-12) dv = 1e-12;
         t.T[j * t.N + g] = t.R[r * t.ld + g] / dv;
-    }
-}
-
-__global__ void cholinv_batched_kernel(const CholTask *__restrict__ tasks)
-{
-    const CholTask t = tasks[blockIdx.x];
-    if (threadIdx.x != 0) return;
-    const int k = t.k;
-    double L[kBdMaxK * kBdMaxK];
-    int keep[kBdMaxK];
-    for (int j = 0; j < k; ++j)
-        for (int i = 0; i < k; ++i) L[i + kBdMaxK * j] = 0.0;
-    for (int j = 0; j < k; ++j) {
-        double d = t.G[j + k * j];
-        for (int p = 0; p < j; ++p) d -= L[j + kBdMaxK * p] * L[j + kBdMaxK * p];
-        const int kin = t.keep_in ? t.keep_in[j] : 1;
-        keep[j] = (kin && d > t.tol * t.G[j + k * j] && d > 0.0) ? 1 : 0;
-        if (!keep[j]) continue;
-        const double ljj = sqrt(d);
-        L[j + kBdMaxK * j] = ljj;
-        for (int i = j + 1; i < k; ++i) {
-            double s = t.G[i + k * j];
-            for (int p = 0; p < j; ++p) s -= L[i + kBdMaxK * p] * L[j + kBdMaxK * p];
-            L[i + kBdMaxK * j] = s / ljj;
-        }
-    }
-    for (int c = 0; c < k; ++c)
-        for (int i = k - 1; i >= 0; --i) {
-            if (!keep[i] || !keep[c]) { t.W[i + k * c] = 0.0; continue; }
-            double s = (i == c) ? 1.0 : 0.0;
-            for (int p = i + 1; p < k; ++p) s -= L[p + kBdMaxK * i] * t.W[p + k * c];
-            t.W[i + k * c] = s / L[i + kBdMaxK * i];
-        }
-    for (int c = 0; c < k; ++c) {
-        t.keep_out[c] = keep[c];
-        if (t.keep_copy) t.keep_copy[c] = keep[c];
     }
 }
 
@@ -485,6 +659,27 @@ void bd_coef(const CoefTask *d_tasks, int ntasks, hipStream_t s)
 {
     if (ntasks <= 0) return;
     hipLaunchKernelGGL(coef_kernel, dim3(ntasks), dim3(64), 0, s, d_tasks);
+    PC_HIP(hipGetLastError());
+}
+
+void bd_ritz(const RitzTask *d_tasks, int ntasks, int max_nch, int k, double *d_partial,
+             hipStream_t s)
+{
+    if (ntasks <= 0) return;
+    PC_REQUIRE(k >= 1 && k <= kBdMaxK, PISCES_ERR_ARG, "block size must be 1..16");
+    const dim3 grid(max_nch, ntasks);
+    if (k <= 2) hipLaunchKernelGGL(ritz_kernel<2>, grid, dim3(kT), 0, s, d_tasks, d_partial);
+    else if (k <= 4) hipLaunchKernelGGL(ritz_kernel<4>, grid, dim3(kT), 0, s, d_tasks, d_partial);
+    else if (k <= 8) hipLaunchKernelGGL(ritz_kernel<8>, grid, dim3(kT), 0, s, d_tasks, d_partial);
+    else hipLaunchKernelGGL(ritz_kernel<16>, grid, dim3(kT), 0, s, d_tasks, d_partial);
+    hipLaunchKernelGGL(ritz_reduce_kernel, dim3(ntasks), dim3(64), 0, s, d_tasks, d_partial);
+    PC_HIP(hipGetLastError());
+}
+
+void bd_orth(const OrthTask *d_tasks, int ntasks, hipStream_t s)
+{
+    if (ntasks <= 0) return;
+    hipLaunchKernelGGL(orth_kernel, dim3(ntasks), dim3(64), 0, s, d_tasks);
     PC_HIP(hipGetLastError());
 }
 

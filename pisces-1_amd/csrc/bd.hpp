@@ -23,12 +23,40 @@ struct GramTask {
     int cpb;             // 8 x 8 column tiles (gram_tiles(ma, mb)); workgroups = nch * cpb
 };
 
-// Y (set|-=|+=) A1 W1 + A2 W2, W in device memory (column-major)
+// Y (set|-=|+=) A1 W1 + A2 W2 + A3 W3, W in device memory (column-major);
+// the third source is optional (ma3 = 0)
 struct CombTask {
     const void *A1; long long lda1; int ma1; int a1_32; const double *W1; int ldw1;
     const void *A2; long long lda2; int ma2; int a2_32; const double *W2; int ldw2;
     void *Y; long long ldy; int y32; int mb; int mode;
     long long N;
+    const void *A3; long long lda3; int ma3; int a3_32; const double *W3; int ldw3;
+};
+
+// Ritz vectors and residuals of one instance in one pass over the basis:
+//   U = X Y_X + C Y_C,  HU = HX Y_X + HC Y_C,  R = HU - U diag(theta),
+// plus partial sums of ||R_i||^2 (reduced into out, and into gather as in
+// ResTask).  X, HX, U, HU FP64; C, HC FP32 if c32; R FP32 if r32.
+struct RitzTask {
+    const double *X, *HX; const void *C, *HC;
+    int k, mc, c32, r32;
+    const double *Y; const double *theta;
+    double *U, *HU; void *R;
+    long long N;
+    long long part_off; int nch; int pad;
+    double *out; double *gather;
+};
+
+// Factor of one orthogonalisation round of a new block T (nt columns) against
+// an orthonormal basis V (m columns), from h = V^T T (m x nt, ld m) and
+// G = T^T T (nt x nt, ld nt): S = G - h^T h = L L^T (columns whose pivot falls
+// below tol * G_jj, or that keep_in marks dead, are dropped), W = L^{-T},
+// F = -h W; then (T - V h) W = T W + V F has orthonormal columns orthogonal to V.
+struct OrthTask {
+    const double *h; const double *G; int m; int nt;
+    double *W; double *F;
+    const int *keep_in; int *keep_out; int *keep_copy;
+    double tol;
 };
 
 // Symmetric eigenproblem of the projected matrix: G (m x m, upper triangle,
@@ -82,6 +110,10 @@ void bd_eig(const EigTask *d_tasks, int ntasks, hipStream_t s);
 void bd_resid(const ResTask *d_tasks, int ntasks, int max_nch, double *d_partial, hipStream_t s);
 void bd_precond(const PrecTask *d_tasks, int ntasks, long long max_N, hipStream_t s);
 void bd_cholinv(const CholTask *d_tasks, int ntasks, hipStream_t s);
+// k: the (common) block size of the tasks
+void bd_ritz(const RitzTask *d_tasks, int ntasks, int max_nch, int k, double *d_partial,
+             hipStream_t s);
+void bd_orth(const OrthTask *d_tasks, int ntasks, hipStream_t s);
 void bd_coef(const CoefTask *d_tasks, int ntasks, hipStream_t s);
 
 constexpr int kGTile = 8;          // Gram tile edge (columns of A, of B)
@@ -89,5 +121,6 @@ constexpr int kGRows = 256 * 32;   // rows per Gram chunk (32 per thread)
 inline int gram_chunks(long long N) { return (int)((N + kGRows - 1) / kGRows); }
 inline int gram_tiles(int ma, int mb) { return ((ma + kGTile - 1) / kGTile) * ((mb + kGTile - 1) / kGTile); }
 constexpr int kBdResChunk = 1024; // rows per residual chunk
+constexpr int kRitzRows = 256;    // rows per Ritz/residual workgroup (one per thread)
 
 }  // namespace pisces

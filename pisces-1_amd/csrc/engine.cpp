@@ -38,7 +38,8 @@ constexpr int oGu = 6784;    // Gram of U at restart (k x k)
 constexpr int oWu = 7040;    // restart factor (k x k)
 constexpr int oQp = 7552;    // restart search directions in [X|C] coefficients (m x k)
 constexpr int oI = 8576;     // identity (k x k)
-constexpr int kSmall = 8832;
+constexpr int oF = 8832;     // orthogonalisation coefficients on [X|C] ((k + mc) x nt)
+constexpr int kSmall = 9856;
 }  // namespace
 
 struct BdEngine::Inst {
@@ -283,7 +284,8 @@ struct CombBatch {
         double b = 0;
         for (int i = 0; i < n; ++i) {
             const CombTask &t = h[i];
-            b += (double)t.N * (t.ma1 * (t.a1_32 ? 4 : 8) + t.ma2 * (t.a2_32 ? 4 : 8));
+            b += (double)t.N * (t.ma1 * (t.a1_32 ? 4 : 8) + t.ma2 * (t.a2_32 ? 4 : 8) +
+                                t.ma3 * (t.a3_32 ? 4 : 8));
             b += (double)t.N * t.mb * (t.y32 ? 4 : 8) * (t.mode ? 2 : 1);
         }
         return b;
@@ -341,8 +343,11 @@ int BdEngine::step()
     const bool lo = prec_ != PISCES_FP64;
     const int e32 = lo ? 1 : 0;
     const size_t e = lo ? 4 : 8;
+    // residuals and raw corrections in FP32 in the low-precision forms with the
+    // kinetic preconditioner (only the split-BF16 preconditioner reads them)
+    const int r32 = (lo && precond_ == 3) ? 1 : 0;
     const size_t tmax = std::max({sizeof(GramTask), sizeof(CombTask), sizeof(PrecTask),
-                                  sizeof(HxJob), sizeof(EigTask), sizeof(ResTask),
+                                  sizeof(HxJob), sizeof(EigTask), sizeof(RitzTask), sizeof(OrthTask),
                                   sizeof(CholTask), sizeof(FdmTask)});
     // at most ~30 descriptors per instance and phase, plus alignment slack
     arena_[0]->reserve((tmax * 32 + 64 * 24) * (size_t)(n + 1), s_);
@@ -475,12 +480,11 @@ int BdEngine::step()
     }
     EigTask *he, *de;
     he = A.take<EigTask>(n, &de);
-    CombBatch rz;
-    rz.h = A.take<CombTask>(2 * n, &rz.d);
-    ResTask *hrs, *drs;
-    hrs = A.take<ResTask>(n, &drs);
+    RitzTask *hrz, *drz;
+    hrz = A.take<RitzTask>(n, &drz);
     int res_nch = 0;
     long long res_part = 0;
+    double ritz_bytes = 0;
     for (int i = 0; i < n; ++i) {
         Inst *in = act[i];
         const long long N = in->g.N;
@@ -490,16 +494,15 @@ int BdEngine::step()
         std::swap(in->U, in->Up);
         std::swap(in->HU, in->HUp);
         const double *Y = in->sm() + oY;
-        rz.add(CombTask{in->X.p, N, k_, 0, Y, m, in->C.p, N, in->mc, e32, Y + k_, m, in->U.p, N, 0,
-                        k_, 0, N});
-        rz.add(CombTask{in->HX.p, N, k_, 0, Y, m, in->HC.p, N, in->mc, e32, Y + k_, m, in->HU.p, N,
-                        0, k_, 0, N});
-        ResTask t{in->U.as<double>(), in->HU.as<double>(), in->R.as<double>(), N, k_, 0,
-                  in->sm() + oTh, N, res_part, (int)((N + kBdResChunk - 1) / kBdResChunk), 0,
-                  in->sm() + oRn, dgather_.as<double>() + (size_t)2 * kBdMaxK * i};
+        RitzTask t{in->X.as<double>(), in->HX.as<double>(), in->C.p, in->HC.p, k_, in->mc, e32,
+                   r32, Y, in->sm() + oTh, in->U.as<double>(), in->HU.as<double>(), in->R.p, N,
+                   res_part, (int)((N + kRitzRows - 1) / kRitzRows), 0, in->sm() + oRn,
+                   dgather_.as<double>() + (size_t)2 * kBdMaxK * i};
+        hrz[i] = t;
+        // X, HX, C, HC read; U, HU, R written
+        ritz_bytes += (double)N * (16.0 * k_ + 2.0 * e * in->mc + (16.0 + (r32 ? 4 : 8)) * k_);
         res_part += (long long)k_ * t.nch;
         res_nch = std::max(res_nch, t.nch);
-        hrs[i] = t;
     }
 
     // ============================ phase A: launches
@@ -527,12 +530,9 @@ int BdEngine::step()
         ProfScope ps(PK_EIG, s_, 0.0);
         bd_eig(de, n, s_);
     }
-    rz.launch(s_);
     {
-        double rb = 0;
-        for (Inst *in : act) rb += 3.0 * 8 * k_ * in->g.N;  // U, HU read; R written
-        ProfScope ps(PK_RESID, s_, rb);
-        bd_resid(drs, n, res_nch, dp, s_);
+        ProfScope ps(PK_RESID, s_, ritz_bytes);
+        bd_ritz(drz, n, res_nch, k_, dp, s_);
     }
     PC_HIP(hipMemcpyAsync(hsmall_, dgather_.p, sizeof(double) * 2 * kBdMaxK * n,
                           hipMemcpyDeviceToHost, s_));
@@ -603,9 +603,15 @@ int BdEngine::step()
     if (corr.empty()) return this->live();
 
     // ============================ phase B: corrections
+    // precondition the residual columns into T, then two orthogonalisation
+    // rounds against V = [X | C] and within the block, each one Gram launch
+    // ([X|C|T]^T T), one factor kernel and one combination T W + V F; the
+    // second round writes C[mc .. mc + nt)
     Arena &B = *arena_[1];
     B.used = 0;
     const int nc = (int)corr.size();
+    const int t32 = r32;
+    const size_t et = t32 ? 4 : 8;
     PrecTask *hp = nullptr, *dpc = nullptr;
     FdmTask *hf = nullptr, *dfd = nullptr;
     int nfd = 0, fd_maxn = 0;
@@ -631,9 +637,12 @@ int BdEngine::step()
             for (int t = 0; t < in->nt; ++t) {
                 FdmTask f{};
                 for (int a = 0; a < 3; ++a) f.ax[a] = in->g.ax[a];
-                f.src = in->R.as<double>() + in->g.N * in->src[t];
-                f.dst = in->T.as<double>() + in->g.N * t;
-                // T2 is free until the Cholesky-QR below: FP32 scratch of the BF16 form
+                f.src = in->R.as<unsigned char>() + (r32 ? 4 : 8) * in->g.N * in->src[t];
+                f.dst = in->T.as<unsigned char>() + et * in->g.N * t;
+                f.src32 = r32;
+                f.dst32 = t32;
+                // T2 is free until the first orthogonalisation round writes it:
+                // FP32 scratch of the split-BF16 form
                 f.work = reinterpret_cast<float *>(in->T2.p) + in->g.N * t;
                 // t = -(T - theta)^{-1} r
                 f.shift = in->theta[in->src[t]];
@@ -643,63 +652,53 @@ int BdEngine::step()
             }
         }
     }
-    GramBatch og[2], cg[2];
-    CombBatch oc[2], cc[2];
-    CholTask *chh[2], *chd[2];
+    GramBatch og[2];
+    CombBatch oc[2];
+    OrthTask *hot[2], *dot[2];
     for (int pass = 0; pass < 2; ++pass) {
-        og[pass].h = B.take<GramTask>(2 * nc, &og[pass].d);
+        og[pass].h = B.take<GramTask>(3 * nc, &og[pass].d);
         oc[pass].h = B.take<CombTask>(nc, &oc[pass].d);
-        for (Inst *in : corr) {
-            const long long N = in->g.N;
-            const int m = k_ + in->mc;
-            double *h = in->sm() + oH;
-            og[pass].add(in->X.p, k_, 0, in->T.p, in->nt, 0, N, h, m);
-            if (in->mc > 0) og[pass].add(in->C.p, in->mc, e32, in->T.p, in->nt, 0, N, h + k_, m);
-            oc[pass].add(CombTask{in->X.p, N, k_, 0, h, m, in->C.p, N, in->mc, e32, h + k_, m,
-                                  in->T.p, N, 0, in->nt, 1, N});
-        }
-    }
-    for (int pass = 0; pass < 2; ++pass) {
-        cg[pass].h = B.take<GramTask>(nc, &cg[pass].d);
-        chh[pass] = B.take<CholTask>(nc, &chd[pass]);
-        cc[pass].h = B.take<CombTask>(nc, &cc[pass].d);
+        hot[pass] = B.take<OrthTask>(nc, &dot[pass]);
         for (int i = 0; i < nc; ++i) {
             Inst *in = corr[i];
             const long long N = in->g.N;
+            const int m = k_ + in->mc, nt = in->nt;
+            double *h = in->sm() + oH, *G = in->sm() + oGt, *W = in->sm() + oWc, *F = in->sm() + oF;
             const void *src = pass == 0 ? in->T.p : in->T2.p;
-            cg[pass].add(src, in->nt, 0, src, in->nt, 0, N, in->sm() + oGt, in->nt);
-            chh[pass][i] = CholTask{in->sm() + oGt, in->nt, 0, in->sm() + oWc,
-                                    pass ? in->keepT() : nullptr, in->keepT(), 1e-10,
-                                    pass ? in->keep() + k_ + in->mc : nullptr};
             void *dst = pass == 0 ? in->T2.p : (void *)(in->C.as<unsigned char>() + e * N * in->mc);
-            cc[pass].add(comb1(src, N, in->nt, 0, in->sm() + oWc, in->nt, dst, pass ? e32 : 0,
-                               in->nt, 0));
+            const int dst32 = pass == 0 ? t32 : e32;
+            og[pass].add(in->X.p, k_, 0, src, nt, t32, N, h, m);
+            if (in->mc > 0) og[pass].add(in->C.p, in->mc, e32, src, nt, t32, N, h + k_, m);
+            og[pass].add(src, nt, t32, src, nt, t32, N, G, nt);
+            hot[pass][i] = OrthTask{h, G, m, nt, W, F, pass ? in->keepT() : nullptr, in->keepT(),
+                                    pass ? in->keep() + k_ + in->mc : nullptr,
+                                    pass ? 1e-10 : 1e-13};
+            CombTask ct{src, N, nt, t32, W, nt, in->X.p, N, k_, 0, F, m, dst, N, dst32, nt, 0, N,
+                        in->C.p, N, in->mc, e32, F + k_, m};
+            oc[pass].add(ct);
         }
     }
     long long partB = 1;
-    for (int pass = 0; pass < 2; ++pass) partB = std::max({partB, og[pass].part, cg[pass].part});
+    for (int pass = 0; pass < 2; ++pass) partB = std::max(partB, og[pass].part);
     grow_part(sizeof(double) * (size_t)partB);
     dp = dpart_.as<double>();
     B.flush(s_);
     {
         // R columns read once, corrections written once (+ diag H for the diagonal form)
         double pb = 0;
-        for (Inst *in : corr) pb += 8.0 * in->g.N * (2 * in->nt + (precond_ == 1 ? 1 : 0));
+        for (Inst *in : corr)
+            pb += (double)in->g.N * in->nt * ((r32 ? 4 : 8) + et) + (precond_ == 1 ? 8.0 * in->g.N : 0.0);
         ProfScope ps(PK_PREC, s_, pb);
         if (precond_ == 1) bd_precond(dpc, nc, maxN, s_);
         else fdm_launch(dfd, nfd, fd_maxn, lo ? PISCES_BF16X2 : PISCES_FP64, s_);
     }
     for (int pass = 0; pass < 2; ++pass) {
         og[pass].launch(dp, s_);
-        oc[pass].launch(s_);
-    }
-    for (int pass = 0; pass < 2; ++pass) {
-        cg[pass].launch(dp, s_);
         {
             ProfScope ps(PK_CHOL, s_, 0.0);
-            bd_cholinv(chd[pass], nc, s_);
+            bd_orth(dot[pass], nc, s_);
         }
-        cc[pass].launch(s_);
+        oc[pass].launch(s_);
     }
     for (Inst *in : corr) {
         in->mc += in->nt;

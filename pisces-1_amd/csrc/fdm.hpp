@@ -12,10 +12,11 @@ namespace pisces {
 // form works in place in dst.
 struct FdmTask {
     AxisMats ax[3];
-    const double *src;
-    double *dst;
+    const void *src;  // FP64, or FP32 if src32 (split-BF16 form only)
+    void *dst;        // FP64, or FP32 if dst32 (split-BF16 form only)
     float *work;
     double shift, scale, floor;
+    int src32, dst32;
 };
 
 // precision: PISCES_FP64 (FP64 matrix cores) or PISCES_BF16X2 (split BF16)

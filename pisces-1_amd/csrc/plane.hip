@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(kPT) fdm_f64_kernel(const FdmTask *__restrict_
     extern __shared__ __align__(16) unsigned char smem[];
     const int pitch = fpitch<double>(g.npi);
     double *P = reinterpret_cast<double *>(smem);
-    const double *src = PASS == 0 ? t.src : t.dst;
+    const double *src = static_cast<const double *>(PASS == 0 ? t.src : t.dst);
     stage_f<double, double>(src, g, false, P, pitch);
     __syncthreads();
     f64x4 acc[NCT][NIT];
@@ -519,7 +519,7 @@ __global__ void __launch_bounds__(kPT) fdm_f64_kernel(const FdmTask *__restrict_
     if (PASS == 0) f64_contract(P, pitch, g, My.qt64, true, acc);         // Qy^T (.)
     else if (PASS == 1) f64_contract(P, pitch, g, Mz.q64, true, acc);     // Qz (.)
     else f64_contract(P, pitch, g, Mx.qt64, false, acc);                  // (.) Qx^T
-    f64_store_global(t.dst, g, acc);
+    f64_store_global(static_cast<double *>(t.dst), g, acc);
 }
 
 // ------------------------------------------------------------------ preconditioner, split-BF16
@@ -675,7 +675,13 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict_
     uint16_t *Pcl = Pch + g.npc * pc;
     f32x4 acc[NCT][NIT];
     if (PASS == 0) {
-        stage_bf2(t.src, g, vec_ok(t.src, g, 2), Pch, Pcl, nullptr, nullptr);
+        if (t.src32) {
+            const float *src = static_cast<const float *>(t.src);
+            stage_bf2(src, g, vec_ok(src, g, 4), Pch, Pcl, nullptr, nullptr);
+        } else {
+            const double *src = static_cast<const double *>(t.src);
+            stage_bf2(src, g, vec_ok(src, g, 2), Pch, Pcl, nullptr, nullptr);
+        }
         __syncthreads();
         bf2_contract(Pch, Pcl, g, Mx.qth, Mx.qtl, false, acc);  // P Qx   (B-form: Qx^T)
         bf2_store_lds(Pth, Ptl, g, acc, true, nullptr);
@@ -698,7 +704,8 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict_
         bf2_store_lds(Pch, Pcl, g, acc, false, nullptr);
         __syncthreads();
         bf2_contract(Pch, Pcl, g, Mx.qh, Mx.ql, false, acc);    // (.) Qx^T  (B-form: Qx)
-        bf2_store_global(t.dst, g, acc);
+        if (t.dst32) bf2_store_global(static_cast<float *>(t.dst), g, acc);
+        else bf2_store_global(static_cast<double *>(t.dst), g, acc);
     }
 }
 

@@ -54,8 +54,8 @@ void resolve()
     g_pending.clear();
 }
 
-const char *kNames[PK_COUNT] = {"hx_fp64", "hx_lo", "gram", "combine", "eig", "resid", "precond",
-                                "cholqr", "potential", "dipole_scf"};
+const char *kNames[PK_COUNT] = {"hx_fp64", "hx_lo", "gram", "combine", "eig", "ritz_resid", "precond",
+                                "factors", "potential", "dipole_scf"};
 
 }  // namespace
 

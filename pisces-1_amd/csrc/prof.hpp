@@ -17,9 +17,9 @@ enum ProfKind {
     PK_GRAM,      // block Gram partial sums + reduction
     PK_COMB,      // block combinations Y = A W
     PK_EIG,       // projected eigensolve
-    PK_RESID,     // residuals and their norms
+    PK_RESID,     // Ritz vectors, residuals and their norms (one fused pass)
     PK_PREC,      // preconditioner (diagonal or kinetic fast diagonalisation)
-    PK_CHOL,      // Cholesky-QR factors
+    PK_CHOL,      // Cholesky / orthogonalisation factors (small)
     PK_POT,       // electron-water potential on the grid (+ diag H)
     PK_SCF,       // induced-dipole solve of the water cluster
     PK_COUNT
