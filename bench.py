@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--fp64-single", type=int, default=8,
                     help="snapshots solved one at a time in FP64 (the plain form)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--served-only", action="store_true",
+                    help="time only the served mixed-precision steps (profiling runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -132,20 +134,22 @@ def main():
     geoms = [workload.water_cluster(N_WATER, RADIUS, SEED + i)[0] for i in range(B)]
     log("batch of %d snapshots generated" % B)
 
-    # ---- plain FP64 GPU form, one solve at a time
     nsingle = max(1, min(a.fp64_single, B))
-    run_single(P, geoms[:1], P.FP64)  # warm-up
-    r64, lat64 = run_single(P, geoms[:nsingle], P.FP64)
-    assert all(r["nconv"] == NROOTS for r in r64), [r["nconv"] for r in r64]
-    fp64_single_sps = nsingle / sum(lat64)
-    log("fp64 single: %.2f solves/s" % fp64_single_sps)
+    if not a.served_only:
+        # ---- plain FP64 GPU form, one solve at a time
+        run_single(P, geoms[:1], P.FP64)  # warm-up
+        r64, lat64 = run_single(P, geoms[:nsingle], P.FP64)
+        assert all(r["nconv"] == NROOTS for r in r64), [r["nconv"] for r in r64]
+        fp64_single_sps = nsingle / sum(lat64)
+        log("fp64 single: %.2f solves/s" % fp64_single_sps)
 
-    # ---- FP64 through the server (same batching as the mixed form)
-    res64b, dt64b, _, _ = run_server(P, geoms, P.FP64, 1, 1)
-    fp64_batched_sps = B / dt64b
-    ev64 = np.array([r["evals"] for r in res64b[0]])
-    assert all(r["status"] == NROOTS for r in res64b[0]), [(r["status"], r["n_iter"]) for r in res64b[0]]
-    log("fp64 batched: %.2f solves/s" % fp64_batched_sps)
+        # ---- FP64 through the server (same batching as the mixed form)
+        res64b, dt64b, _, _ = run_server(P, geoms, P.FP64, 1, 1)
+        fp64_batched_sps = B / dt64b
+        ev64 = np.array([r["evals"] for r in res64b[0]])
+        assert all(r["status"] == NROOTS for r in res64b[0]), \
+            [(r["status"], r["n_iter"]) for r in res64b[0]]
+        log("fp64 batched: %.2f solves/s" % fp64_batched_sps)
 
     # ---- mixed precision through the server: the headline, timed, with launch timing
     res, dt, t_steps, fam = run_server(P, geoms, prec, a.steps, a.warmup, prof=True)
@@ -153,8 +157,13 @@ def main():
     lat = np.array([r["latency_s"] for step in res for r in step])
     n_ok = sum(r["status"] == NROOTS for step in res for r in step)
     evm = np.array([r["evals"] for r in res[-1]])
-    de = np.abs(evm - ev64)
-    de_single = np.abs(evm[:nsingle] - np.array([r["evals"] for r in r64]))
+    if a.served_only:
+        de = de_single = np.zeros(1)
+        fp64_single_sps = fp64_batched_sps = float("nan")
+        lat64 = [float("nan")]
+    else:
+        de = np.abs(evm - ev64)
+        de_single = np.abs(evm[:nsingle] - np.array([r["evals"] for r in r64]))
     log("%s batched: %.2f solves/s, max |dE| %.2e" % (a.precision, sps, de.max()))
 
     # ---- dominant launch family over the timed region and its roofline
@@ -172,6 +181,7 @@ def main():
             "ms_per_launch": round(ms_d / nl_d, 5),
             "bytes_per_launch": round(by_d / nl_d),
             "share_of_timed": round(ms_d * 1e-3 / dt, 3)}
+    roof["traffic"], roof["traffic_over_algorithmic"] = measured_traffic(dom, by_d / nl_d)
 
     # ---- standalone H*X at 64^3 x 8 vectors (entries uniform in [-1, 1])
     hx = {}
@@ -184,7 +194,7 @@ def main():
                         "TFLOPs": round(fl / (ms * 1e-3) / 1e12, 2)}
 
     cpu = None
-    if not a.no_cpu:
+    if not a.no_cpu and not a.served_only:
         cpu = cpu_baseline(P, geoms[0], a.cpu_seconds)
 
     out = {
@@ -220,7 +230,26 @@ def main():
         "hx_standalone_64cube_x8": hx,
         "cpu_baseline": cpu,
     }
+    if a.served_only:
+        for key in ("fp64_gpu", "speedup_vs_fp64_single", "speedup_vs_fp64_batched",
+                    "max_abs_dE_vs_fp64", "dE_ok"):
+            out[key] = None
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(family, algorithmic_bytes_per_launch):
+    """HBM bytes per launch of a launch family from the hardware byte counters
+    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE of the same served run,
+    calibrated on a plain copy kernel), as the measured-to-algorithmic ratio of
+    that run applied to this run's algorithmic bytes per launch."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        ratio = t["families"][family]["ratio"]
+    except (OSError, KeyError, ValueError):
+        return None, None
+    return round(ratio * algorithmic_bytes_per_launch), round(ratio, 3)
 
 
 def cpu_baseline(P, xyz, seconds):

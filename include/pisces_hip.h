@@ -151,6 +151,12 @@ int pisces_time_hx(pisces_ctx *ctx, int nvec, int precision, int warmup, int ite
 int pisces_kinetic_precond(pisces_ctx *ctx, int nvec, const double *r, double shift,
                            double scale, double floor, int precision, double *t);
 
+/* Plain device-to-device copy of `bytes` (multiple of 16) by a streaming
+ * kernel, `reps` timed launches after one warm-up; *ms = mean launch time.
+ * Calibrates hardware byte counters and the achievable HBM bandwidth.
+ * Measurement hook; no reference counterpart. */
+int pisces_copy_probe(int device, long long bytes, int reps, double *ms);
+
 /* diag(H) on the host, ngp doubles.  Replaces DVR::ComputeDiagonal
  * (src/DVR.cpp:1808-1881). */
 int pisces_diag(pisces_ctx *ctx, double *diag);

@@ -147,6 +147,42 @@ __global__ void __launch_bounds__(kT) gram_reduce_kernel(const GramTask *__restr
 }
 
 // ------------------------------------------------------------------ combine
+// acc[b] += sum_a A[a*lda + g] w[(a0 + a) + ma*b]: four column loads in flight
+template <typename T>
+__device__ __forceinline__ void comb_cols(const T *__restrict__ A, long long lda, int na,
+                                          const double *w, int ma, int a0, int mb, long long g,
+                                          double (&acc)[kBdMaxK])
+{
+    int a = 0;
+    for (; a + 4 <= na; a += 4) {
+        const double x0 = A[a * lda + g], x1 = A[(a + 1) * lda + g];
+        const double x2 = A[(a + 2) * lda + g], x3 = A[(a + 3) * lda + g];
+#pragma unroll
+        for (int b = 0; b < kBdMaxK; ++b) {
+            if (b >= mb) break;
+            const double *wb = w + a0 + a + ma * b;
+            acc[b] += x0 * wb[0] + x1 * wb[1] + x2 * wb[2] + x3 * wb[3];
+        }
+    }
+    for (; a < na; ++a) {
+        const double x0 = A[a * lda + g];
+#pragma unroll
+        for (int b = 0; b < kBdMaxK; ++b) {
+            if (b >= mb) break;
+            acc[b] += x0 * w[a0 + a + ma * b];
+        }
+    }
+}
+
+__device__ __forceinline__ void comb_source(const void *A, long long lda, int na, int a32,
+                                            const double *w, int ma, int a0, int mb, long long g,
+                                            double (&acc)[kBdMaxK])
+{
+    if (na <= 0) return;
+    if (a32) comb_cols(static_cast<const float *>(A), lda, na, w, ma, a0, mb, g, acc);
+    else comb_cols(static_cast<const double *>(A), lda, na, w, ma, a0, mb, g, acc);
+}
+
 __global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ tasks)
 {
     const CombTask t = tasks[blockIdx.y];
@@ -166,24 +202,9 @@ __global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ t
     double acc[kBdMaxK];
 #pragma unroll
     for (int b = 0; b < kBdMaxK; ++b) acc[b] = 0.0;
-    for (int a = 0; a < t.ma1; ++a) {
-        const double x = ld_any(t.A1, a * t.lda1 + g, t.a1_32);
-#pragma unroll
-        for (int b = 0; b < kBdMaxK; ++b)
-            if (b < t.mb) acc[b] += x * w[a + ma * b];
-    }
-    for (int a = 0; a < t.ma2; ++a) {
-        const double x = ld_any(t.A2, a * t.lda2 + g, t.a2_32);
-#pragma unroll
-        for (int b = 0; b < kBdMaxK; ++b)
-            if (b < t.mb) acc[b] += x * w[t.ma1 + a + ma * b];
-    }
-    for (int a = 0; a < t.ma3; ++a) {
-        const double x = ld_any(t.A3, a * t.lda3 + g, t.a3_32);
-#pragma unroll
-        for (int b = 0; b < kBdMaxK; ++b)
-            if (b < t.mb) acc[b] += x * w[t.ma1 + t.ma2 + a + ma * b];
-    }
+    comb_source(t.A1, t.lda1, t.ma1, t.a1_32, w, ma, 0, t.mb, g, acc);
+    comb_source(t.A2, t.lda2, t.ma2, t.a2_32, w, ma, t.ma1, t.mb, g, acc);
+    comb_source(t.A3, t.lda3, t.ma3, t.a3_32, w, ma, t.ma1 + t.ma2, t.mb, g, acc);
 #pragma unroll
     for (int b = 0; b < kBdMaxK; ++b) {
         if (b >= t.mb) break;
@@ -390,6 +411,35 @@ __global__ void __launch_bounds__(64) resid_reduce_kernel(const ResTask *__restr
 }
 
 // ------------------------------------------------------------------ Ritz vectors + residuals
+// u[b] += sum_a A[a*N + g] y[(a0 + a)][b], hu likewise with HA: two columns
+// (four loads) in flight
+template <int KB, typename T>
+__device__ __forceinline__ void ritz_cols(const T *__restrict__ A, const T *__restrict__ HA,
+                                          long long N, int na, const double *y, int a0,
+                                          long long g, double (&u)[KB], double (&hu)[KB])
+{
+    int a = 0;
+    for (; a + 2 <= na; a += 2) {
+        const double x0 = A[a * N + g], h0 = HA[a * N + g];
+        const double x1 = A[(a + 1) * N + g], h1 = HA[(a + 1) * N + g];
+        const double *w0 = y + (a0 + a) * KB, *w1 = w0 + KB;
+#pragma unroll
+        for (int b = 0; b < KB; ++b) {
+            u[b] += x0 * w0[b] + x1 * w1[b];
+            hu[b] += h0 * w0[b] + h1 * w1[b];
+        }
+    }
+    if (a < na) {
+        const double x0 = A[a * N + g], h0 = HA[a * N + g];
+        const double *w0 = y + (a0 + a) * KB;
+#pragma unroll
+        for (int b = 0; b < KB; ++b) {
+            u[b] += x0 * w0[b];
+            hu[b] += h0 * w0[b];
+        }
+    }
+}
+
 // KB >= k: block size rounded up; coefficient columns k .. KB-1 are zero.
 template <int KB>
 __global__ void __launch_bounds__(kT) ritz_kernel(const RitzTask *__restrict__ tasks,
@@ -414,24 +464,13 @@ __global__ void __launch_bounds__(kT) ritz_kernel(const RitzTask *__restrict__ t
 #pragma unroll
     for (int b = 0; b < KB; ++b) u[b] = hu[b] = 0.0;
     if (row) {
-        for (int a = 0; a < k; ++a) {
-            const double x = t.X[a * t.N + g], hx = t.HX[a * t.N + g];
-            const double *w = y + a * KB;
-#pragma unroll
-            for (int b = 0; b < KB; ++b) {
-                u[b] += x * w[b];
-                hu[b] += hx * w[b];
-            }
-        }
-        for (int c = 0; c < mc; ++c) {
-            const double x = ld_any(t.C, c * t.N + g, t.c32), hx = ld_any(t.HC, c * t.N + g, t.c32);
-            const double *w = y + (k + c) * KB;
-#pragma unroll
-            for (int b = 0; b < KB; ++b) {
-                u[b] += x * w[b];
-                hu[b] += hx * w[b];
-            }
-        }
+        ritz_cols(t.X, t.HX, t.N, k, y, 0, g, u, hu);
+        if (t.c32)
+            ritz_cols(static_cast<const float *>(t.C), static_cast<const float *>(t.HC), t.N, mc, y,
+                      k, g, u, hu);
+        else
+            ritz_cols(static_cast<const double *>(t.C), static_cast<const double *>(t.HC), t.N, mc,
+                      y, k, g, u, hu);
     }
     double nrm[KB];
 #pragma unroll
@@ -464,13 +503,19 @@ __global__ void __launch_bounds__(64) ritz_reduce_kernel(const RitzTask *__restr
                                                          const double *__restrict__ partial)
 {
     const RitzTask &t = tasks[blockIdx.x];
-    if ((int)threadIdx.x >= t.k) return;
-    double sum = 0;
-    for (int c = 0; c < t.nch; ++c) sum += partial[t.part_off + (long long)threadIdx.x * t.nch + c];
-    t.out[threadIdx.x] = sum;
-    if (t.gather) {
-        t.gather[threadIdx.x] = t.theta[threadIdx.x];
-        t.gather[kBdMaxK + threadIdx.x] = sum;
+    const int lane = threadIdx.x;
+    for (int b = 0; b < t.k; ++b) {
+        const double *p = partial + t.part_off + (long long)b * t.nch;
+        double sum = 0;
+        for (int c = lane; c < t.nch; c += 64) sum += p[c];
+        for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        if (lane == 0) {
+            t.out[b] = sum;
+            if (t.gather) {
+                t.gather[b] = t.theta[b];
+                t.gather[kBdMaxK + b] = sum;
+            }
+        }
     }
 }
 

@@ -10,6 +10,8 @@
 // fixed-order reduction, so a solve is bitwise reproducible run to run.
 #include "blockops.hpp"
 
+#include <algorithm>
+
 namespace pisces {
 
 namespace {
@@ -411,6 +413,38 @@ void diag_h(const double *V, const double *Tx, const double *Ty, const double *T
 void to_f64(const float *x, double *y, long long n, hipStream_t s)
 {
     hipLaunchKernelGGL(convert64_kernel, dim3(nblk(n)), dim3(kT), 0, s, x, y, n);
+    PC_HIP(hipGetLastError());
+}
+
+// plain streaming copy (16-byte vectors, grid-stride): the reference kernel of
+// known traffic for calibrating hardware byte counters
+// four 16-byte loads in flight per thread, each workgroup on a contiguous 16 KiB
+// stretch per iteration
+__global__ void __launch_bounds__(256) copy_probe_kernel(const uint4 *__restrict__ src,
+                                                         uint4 *__restrict__ dst, long long n16)
+{
+    const long long stride = (long long)gridDim.x * 1024;
+    for (long long base = (long long)blockIdx.x * 1024; base < n16; base += stride) {
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long long i = base + u * 256 + threadIdx.x;
+            if (i < n16) v[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long long i = base + u * 256 + threadIdx.x;
+            if (i < n16) dst[i] = v[u];
+        }
+    }
+}
+
+void copy_probe(const void *src, void *dst, long long bytes, hipStream_t s)
+{
+    const long long n16 = bytes / 16;
+    const long long blocks = std::min<long long>((n16 + 1023) / 1024, 8192);
+    hipLaunchKernelGGL(copy_probe_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                       static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
     PC_HIP(hipGetLastError());
 }
 

@@ -49,6 +49,8 @@ void to_f32(const double *x, float *y, long long n, hipStream_t s);
 void to_f64(const float *x, double *y, long long n, hipStream_t s);
 // x[i] = uniform in [-1, 1) from a counter-based hash of (seed, i)
 void fill_uniform(void *x, long long n, bool f64, unsigned seed, hipStream_t s);
+// dst = src, bytes a multiple of 16 (copy_probe_kernel)
+void copy_probe(const void *src, void *dst, long long bytes, hipStream_t s);
 
 // Symmetric eigensolver on the host for the projected matrix (m <= 64):
 // cyclic Jacobi, ascending eigenvalues; reads the upper triangle of A

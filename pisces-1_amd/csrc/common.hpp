@@ -142,12 +142,32 @@ void potential_grid_launch(const PotParams &P, const PotDev &D, int nx, int ny, 
                            const double *d_x, const double *d_y, const double *d_z,
                            double *d_v, float *d_v32, hipStream_t s);
 
+// Several clusters in one launch (one task per cluster, same PotParams).
+struct PotTask {
+    PotDev D;
+    int nx, ny, nz;
+    const double *gx, *gy, *gz;
+    double *v;
+    float *v32;
+};
+void potential_grid_batched_launch(const PotParams &P, const PotTask *d_tasks, int ntasks,
+                                   long long max_ng, hipStream_t s);
+int potential_max_sites();
+
 // --------------------------------------------------------------- water model
 // Host: rigid-monomer projection of O,H1,H2 (Angstrom) to 4 sites (bohr).
 bool water_sites_host(int nw, const double *xyz_ang, double *sites);
 // Device: induced-dipole SCF; d_sites 12*nw, outputs d_mu/d_epc 9*nw.
 void water_scf_launch(int nw, const double *d_sites, double *d_mu, double *d_epc,
                       double *d_tensor, int *d_info, hipStream_t s);
+struct ScfTask {
+    int nw;
+    const double *sites;
+    double *mu, *epc, *tens;
+    int *info;
+};
+void water_scf_batched_launch(const ScfTask *d_tasks, int ntasks, hipStream_t s);
+int water_scf_max_waters();
 // Charges of the DPP monomer (H1, H2, M) and polarizabilities (O, H1, H2).
 void water_lists_host(int nw, std::vector<double> &q, std::vector<int> &iq,
                       std::vector<int> &id);

@@ -34,18 +34,11 @@ __device__ inline double reff(double r, double r0)
     return r;
 }
 
-__global__ void __launch_bounds__(kPotThreads)
-potential_grid_kernel(PotParams P, PotDev D, int nx, int ny, int nz, const double *__restrict__ gx,
-                      const double *__restrict__ gy, const double *__restrict__ gz,
-                      double *__restrict__ v, float *__restrict__ v32)
+__device__ void potential_point(const PotParams &P, const PotDev &D, const SiteLds *__restrict__ S,
+                                int nx, int ny, int nz, const double *__restrict__ gx,
+                                const double *__restrict__ gy, const double *__restrict__ gz,
+                                double *__restrict__ v, float *__restrict__ v32, long long g)
 {
-    __shared__ SiteLds S[kMaxSites];
-    for (int s = threadIdx.x; s < D.nsites; s += blockDim.x)
-        S[s] = SiteLds{D.sites[3 * s], D.sites[3 * s + 1], D.sites[3 * s + 2]};
-    __syncthreads();
-    const long long ng = (long long)nx * ny * nz;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ng) return;
     const int i = (int)(g % nx);
     const int j = (int)((g / nx) % ny);
     const int k = (int)(g / ((long long)nx * ny));
@@ -133,6 +126,36 @@ potential_grid_kernel(PotParams P, PotDev D, int nx, int ny, int nz, const doubl
     if (v32) v32[g] = (float)vt;
 }
 
+__global__ void __launch_bounds__(kPotThreads)
+potential_grid_kernel(PotParams P, PotDev D, int nx, int ny, int nz, const double *__restrict__ gx,
+                      const double *__restrict__ gy, const double *__restrict__ gz,
+                      double *__restrict__ v, float *__restrict__ v32)
+{
+    __shared__ SiteLds S[kMaxSites];
+    for (int s = threadIdx.x; s < D.nsites; s += blockDim.x)
+        S[s] = SiteLds{D.sites[3 * s], D.sites[3 * s + 1], D.sites[3 * s + 2]};
+    __syncthreads();
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long long)nx * ny * nz) return;
+    potential_point(P, D, S, nx, ny, nz, gx, gy, gz, v, v32, g);
+}
+
+// many clusters in one launch: blockIdx.y selects the task
+__global__ void __launch_bounds__(kPotThreads)
+potential_grid_batched_kernel(PotParams P, const PotTask *__restrict__ tasks)
+{
+    const PotTask &t = tasks[blockIdx.y];
+    const long long ng = (long long)t.nx * t.ny * t.nz;
+    if ((long long)blockIdx.x * blockDim.x >= ng) return;
+    __shared__ SiteLds S[kMaxSites];
+    for (int s = threadIdx.x; s < t.D.nsites; s += blockDim.x)
+        S[s] = SiteLds{t.D.sites[3 * s], t.D.sites[3 * s + 1], t.D.sites[3 * s + 2]};
+    __syncthreads();
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    potential_point(P, t.D, S, t.nx, t.ny, t.nz, t.gx, t.gy, t.gz, t.v, t.v32, g);
+}
+
 // --------------------------------------------------------------------------
 // Induced dipoles of the rigid DPP waters, FP64, one workgroup.
 // WaterCluster::CalcInducedDipoles (src/DPP.cpp:760-1065): field of the
@@ -168,9 +191,9 @@ __device__ inline double damp_dd(double r, double ai, double aj, double a)
     return 1.0 - (1 + x) * exp(-x);
 }
 
-__global__ void __launch_bounds__(kScfThreads)
-water_scf_kernel(int nw, const double *__restrict__ sites, double *__restrict__ mu_out,
-                 double *__restrict__ epc_out, double *__restrict__ tens, int *__restrict__ info)
+__device__ void water_scf_body(int nw, const double *__restrict__ sites, double *__restrict__ mu_out,
+                               double *__restrict__ epc_out, double *__restrict__ tens,
+                               int *__restrict__ info)
 {
     const int ns = 3 * nw;
     const double dampCD = 0.21, dampDD = 0.3;  // src/Water.cpp:243
@@ -277,6 +300,20 @@ water_scf_kernel(int nw, const double *__restrict__ sites, double *__restrict__ 
     }
 }
 
+__global__ void __launch_bounds__(kScfThreads)
+water_scf_kernel(int nw, const double *__restrict__ sites, double *__restrict__ mu_out,
+                 double *__restrict__ epc_out, double *__restrict__ tens, int *__restrict__ info)
+{
+    water_scf_body(nw, sites, mu_out, epc_out, tens, info);
+}
+
+// one cluster per workgroup
+__global__ void __launch_bounds__(kScfThreads) water_scf_batched_kernel(const ScfTask *__restrict__ tasks)
+{
+    const ScfTask &t = tasks[blockIdx.x];
+    water_scf_body(t.nw, t.sites, t.mu, t.epc, t.tens, t.info);
+}
+
 }  // namespace
 
 void potential_grid_launch(const PotParams &P, const PotDev &D, int nx, int ny, int nz,
@@ -290,6 +327,26 @@ void potential_grid_launch(const PotParams &P, const PotDev &D, int nx, int ny, 
                        ny, nz, d_x, d_y, d_z, d_v, d_v32);
     PC_HIP(hipGetLastError());
 }
+
+void potential_grid_batched_launch(const PotParams &P, const PotTask *d_tasks, int ntasks,
+                                   long long max_ng, hipStream_t s)
+{
+    if (ntasks <= 0) return;
+    const unsigned blocks = (unsigned)((max_ng + kPotThreads - 1) / kPotThreads);
+    hipLaunchKernelGGL(potential_grid_batched_kernel, dim3(blocks, ntasks), dim3(kPotThreads), 0, s,
+                       P, d_tasks);
+    PC_HIP(hipGetLastError());
+}
+
+void water_scf_batched_launch(const ScfTask *d_tasks, int ntasks, hipStream_t s)
+{
+    if (ntasks <= 0) return;
+    hipLaunchKernelGGL(water_scf_batched_kernel, dim3(ntasks), dim3(kScfThreads), 0, s, d_tasks);
+    PC_HIP(hipGetLastError());
+}
+
+int water_scf_max_waters() { return 4 * kScfThreads / 3; }
+int potential_max_sites() { return kMaxSites; }
 
 void water_scf_launch(int nw, const double *d_sites, double *d_mu, double *d_epc,
                       double *d_tensor, int *d_info, hipStream_t s)

@@ -9,6 +9,7 @@
 // retires the finished ones.  Requests of a chain warm-start from the chain's
 // previous converged Ritz vectors (kept on the device).  Grids of the same
 // shape share their kinetic matrices and eigenbases.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -90,12 +91,15 @@ struct pisces_server {
             }
         return dbuf(bytes);
     }
-    // scratch of the potential build
-    dbuf s_sites, s_mu, s_epc, s_tens, s_info, s_q, s_iq, s_id, s_rep, s_irep, s_ipps, s_ion, s_ionq;
+    // scratch of the potential build: uploaded tables (+ pinned staging), work
+    dbuf s_up, s_wk;
+    unsigned char *h_up = nullptr;
+    size_t h_up_cap = 0;
     ~pisces_server()
     {
         if (stream) static_cast<void>(hipStreamSynchronize(stream));
         eng.reset();
+        if (h_up) static_cast<void>(hipHostFree(h_up));
         if (stream) static_cast<void>(hipStreamDestroy(stream));
     }
 };
@@ -157,77 +161,166 @@ size_t footprint(const pisces_server *sv, long long N)
     return sv->eng->bytes_per_instance(N) + (size_t)N * (8 + 4 + 8);
 }
 
-// potential of one request into res (V64, V32, D), on the server stream
-void build_potential(pisces_server *sv, const Pending &p, InstRes &res)
+// Host-side tables of one admitted request's potential build.
+struct Prep {
+    Pending *p;
+    std::shared_ptr<InstRes> res;
+    std::vector<double> sites, q, rep;
+    std::vector<int> iq, id, irep, ipps;
+    size_t o_sites, o_q, o_iq, o_id, o_rep, o_irep, o_ipps, o_ion, o_ionq;  // upload offsets
+    size_t o_mu, o_epc, o_tens, o_info;                                   // work offsets
+};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Potentials of a batch of admitted requests into their InstRes (V64, V32, D):
+// rigid-water projection and tables on the host, one upload, one induced-dipole
+// launch and one grid-potential launch for the whole batch (one workgroup per
+// cluster / one grid row of workgroups per cluster), then diag H.
+void build_potentials(pisces_server *sv, std::vector<Prep> &batch)
 {
-    const GridShape &g = *res.shape;
-    const int nw = p.req.nw;
-    std::vector<double> sites((size_t)12 * nw);
-    PC_REQUIRE(water_sites_host(nw, p.xyz.data(), sites.data()), PISCES_ERR_NUMERIC,
-               "water projection produced non-finite coordinates");
-    const int ns = 3 * nw;
-    upload(sv->s_sites, sites.data(), sizeof(double) * sites.size(), sv->stream);
-    sv->s_mu.ensure(sizeof(double) * 3 * ns);
-    sv->s_epc.ensure(sizeof(double) * 3 * ns);
-    sv->s_tens.ensure(sizeof(double) * 6 * (size_t)ns * ns);
-    sv->s_info.ensure(sizeof(int) * 2);
-    {
-        ProfScope ps(PK_SCF, sv->stream, 0.0);
-        water_scf_launch(nw, sv->s_sites.as<double>(), sv->s_mu.as<double>(),
-                         sv->s_epc.as<double>(), sv->s_tens.as<double>(), sv->s_info.as<int>(),
-                         sv->stream);
-    }
-    std::vector<double> q;
-    std::vector<int> iq, id;
-    water_lists_host(nw, q, iq, id);
+    const int nb = (int)batch.size();
     // default DPP potential: Potential 3, DampType 2, Polarization 1
     // (src/GetInput.cpp:43-61)
     const int flags[6] = {3, 2, 0, 1, 0, 0};
     const double pp[11] = {9.745, 2.536, 4.4837, 1.7724, 1.7724, 4.4837, 0.5 / std::sqrt(9.745),
                            0, 0, 0, 20.0};
     PotParams P{};
-    std::vector<double> rep;
-    std::vector<int> irep, ipps;
-    PC_REQUIRE(potential_setup(flags, pp, 4 * nw, P, rep, irep, ipps), PISCES_ERR_ARG, "setup");
-    P.ion_r = 2.9;  // Bloomfield r_Na, src/NaCl.cpp:95
-    upload(sv->s_q, q.data(), sizeof(double) * q.size(), sv->stream);
-    upload(sv->s_iq, iq.data(), sizeof(int) * iq.size(), sv->stream);
-    upload(sv->s_id, id.data(), sizeof(int) * id.size(), sv->stream);
-    upload(sv->s_rep, rep.data(), sizeof(double) * rep.size(), sv->stream);
-    upload(sv->s_irep, irep.data(), sizeof(int) * irep.size(), sv->stream);
-    upload(sv->s_ipps, ipps.data(), sizeof(int) * ipps.size(), sv->stream);
-    if (p.req.nion > 0) {
-        upload(sv->s_ion, p.ion_xyz.data(), sizeof(double) * p.ion_xyz.size(), sv->stream);
-        upload(sv->s_ionq, p.ion_q.data(), sizeof(double) * p.ion_q.size(), sv->stream);
+    size_t up = align256(sizeof(ScfTask) * nb) + align256(sizeof(PotTask) * nb), wk = 0;
+    for (Prep &r : batch) {
+        const int nw = r.p->req.nw;
+        PC_REQUIRE(nw <= water_scf_max_waters() && 4 * nw <= potential_max_sites(),
+                   PISCES_ERR_UNSUPPORTED, "too many waters");
+        r.sites.resize((size_t)12 * nw);
+        PC_REQUIRE(water_sites_host(nw, r.p->xyz.data(), r.sites.data()), PISCES_ERR_NUMERIC,
+                   "water projection produced non-finite coordinates");
+        water_lists_host(nw, r.q, r.iq, r.id);
+        PC_REQUIRE(potential_setup(flags, pp, 4 * nw, P, r.rep, r.irep, r.ipps), PISCES_ERR_ARG,
+                   "potential setup");
+        const int ns = 3 * nw, nion = r.p->req.nion;
+        auto put = [&](size_t bytes) { const size_t o = up; up += align256(bytes); return o; };
+        r.o_sites = put(sizeof(double) * r.sites.size());
+        r.o_q = put(sizeof(double) * r.q.size());
+        r.o_iq = put(sizeof(int) * r.iq.size());
+        r.o_id = put(sizeof(int) * r.id.size());
+        r.o_rep = put(sizeof(double) * r.rep.size());
+        r.o_irep = put(sizeof(int) * r.irep.size());
+        r.o_ipps = put(sizeof(int) * r.ipps.size());
+        r.o_ion = put(sizeof(double) * 3 * nion);
+        r.o_ionq = put(sizeof(double) * nion);
+        auto wput = [&](size_t bytes) { const size_t o = wk; wk += align256(bytes); return o; };
+        r.o_mu = wput(sizeof(double) * 3 * ns);
+        r.o_epc = wput(sizeof(double) * 3 * ns);
+        r.o_tens = wput(sizeof(double) * 6 * (size_t)ns * ns);
+        r.o_info = wput(sizeof(int) * 2);
     }
-    PotDev D{4 * nw, (int)q.size(), (int)id.size(), (int)irep.size(), (int)ipps.size(),
-             p.req.nion, sv->s_sites.as<double>(), sv->s_q.as<double>(), sv->s_iq.as<int>(),
-             sv->s_mu.as<double>(), sv->s_id.as<int>(), sv->s_rep.as<double>(),
-             sv->s_irep.as<int>(), sv->s_ipps.as<int>(), sv->s_ion.as<double>(),
-             sv->s_ionq.as<double>()};
-    // V written in FP64 and FP32, then read once more with diag H written
-    ProfScope ps(PK_POT, sv->stream, (double)g.N * (8 + 4 + 8 + 8));
-    potential_grid_launch(P, D, g.n[0], g.n[1], g.n[2], g.dx[0].as<double>(), g.dx[1].as<double>(),
-                          g.dx[2].as<double>(), res.V64.as<double>(), res.V32.as<float>(), sv->stream);
-    diag_h(res.V64.as<double>(), g.dT64[0].as<double>(), g.dT64[1].as<double>(),
-           g.dT64[2].as<double>(), g.n[0], g.n[1], g.n[2], res.D.as<double>(), sv->stream);
-    // the host vectors above are read by the async uploads: wait before they go
+    P.ion_r = 2.9;  // Bloomfield r_Na, src/NaCl.cpp:95
+    sv->s_up.ensure(up);
+    sv->s_wk.ensure(wk);
+    if (sv->h_up_cap < up) {
+        if (sv->h_up) PC_HIP(hipHostFree(sv->h_up));
+        sv->h_up = nullptr;
+        sv->h_up_cap = up + up / 2;
+        PC_HIP(hipHostMalloc(reinterpret_cast<void **>(&sv->h_up), sv->h_up_cap, hipHostMallocDefault));
+    }
+    unsigned char *h = sv->h_up;
+    unsigned char *d = sv->s_up.as<unsigned char>(), *w = sv->s_wk.as<unsigned char>();
+    ScfTask *scf = reinterpret_cast<ScfTask *>(h);
+    PotTask *pot = reinterpret_cast<PotTask *>(h + align256(sizeof(ScfTask) * nb));
+    long long max_ng = 0;
+    for (int i = 0; i < nb; ++i) {
+        Prep &r = batch[i];
+        auto cp = [&](size_t off, const void *src, size_t bytes) {
+            if (bytes) std::memcpy(h + off, src, bytes);
+        };
+        const int nw = r.p->req.nw, nion = r.p->req.nion;
+        cp(r.o_sites, r.sites.data(), sizeof(double) * r.sites.size());
+        cp(r.o_q, r.q.data(), sizeof(double) * r.q.size());
+        cp(r.o_iq, r.iq.data(), sizeof(int) * r.iq.size());
+        cp(r.o_id, r.id.data(), sizeof(int) * r.id.size());
+        cp(r.o_rep, r.rep.data(), sizeof(double) * r.rep.size());
+        cp(r.o_irep, r.irep.data(), sizeof(int) * r.irep.size());
+        cp(r.o_ipps, r.ipps.data(), sizeof(int) * r.ipps.size());
+        cp(r.o_ion, r.p->ion_xyz.data(), sizeof(double) * 3 * nion);
+        cp(r.o_ionq, r.p->ion_q.data(), sizeof(double) * nion);
+        double *mu = reinterpret_cast<double *>(w + r.o_mu);
+        scf[i] = ScfTask{nw, reinterpret_cast<const double *>(d + r.o_sites), mu,
+                         reinterpret_cast<double *>(w + r.o_epc),
+                         reinterpret_cast<double *>(w + r.o_tens),
+                         reinterpret_cast<int *>(w + r.o_info)};
+        const GridShape &g = *r.res->shape;
+        PotTask t{};
+        t.D = PotDev{4 * nw, (int)r.q.size(), (int)r.id.size(), (int)r.irep.size(),
+                     (int)r.ipps.size(), nion, reinterpret_cast<const double *>(d + r.o_sites),
+                     reinterpret_cast<const double *>(d + r.o_q),
+                     reinterpret_cast<const int *>(d + r.o_iq), mu,
+                     reinterpret_cast<const int *>(d + r.o_id),
+                     reinterpret_cast<const double *>(d + r.o_rep),
+                     reinterpret_cast<const int *>(d + r.o_irep),
+                     reinterpret_cast<const int *>(d + r.o_ipps),
+                     reinterpret_cast<const double *>(d + r.o_ion),
+                     reinterpret_cast<const double *>(d + r.o_ionq)};
+        t.nx = g.n[0];
+        t.ny = g.n[1];
+        t.nz = g.n[2];
+        t.gx = g.dx[0].as<double>();
+        t.gy = g.dx[1].as<double>();
+        t.gz = g.dx[2].as<double>();
+        t.v = r.res->V64.as<double>();
+        t.v32 = r.res->V32.as<float>();
+        pot[i] = t;
+        max_ng = std::max(max_ng, g.N);
+    }
+    PC_HIP(hipMemcpyAsync(d, h, up, hipMemcpyHostToDevice, sv->stream));
+    {
+        ProfScope ps(PK_SCF, sv->stream, 0.0);
+        water_scf_batched_launch(reinterpret_cast<const ScfTask *>(d), nb, sv->stream);
+    }
+    {
+        double bytes = 0;  // V written in FP64 and FP32, read once more with diag H written
+        for (Prep &r : batch) bytes += (double)r.res->shape->N * (8 + 4 + 8 + 8);
+        ProfScope ps(PK_POT, sv->stream, bytes);
+        potential_grid_batched_launch(
+            P, reinterpret_cast<const PotTask *>(d + align256(sizeof(ScfTask) * nb)), nb, max_ng,
+            sv->stream);
+        for (Prep &r : batch) {
+            const GridShape &g = *r.res->shape;
+            diag_h(r.res->V64.as<double>(), g.dT64[0].as<double>(), g.dT64[1].as<double>(),
+                   g.dT64[2].as<double>(), g.n[0], g.n[1], g.n[2], r.res->D.as<double>(),
+                   sv->stream);
+        }
+    }
+    // the pinned staging buffer is rewritten by the next batch: wait for the copy
     PC_HIP(hipStreamSynchronize(sv->stream));
 }
 
 void admit(pisces_server *sv)
 {
-    while (!sv->queue.empty() && (int)sv->live.size() < sv->max_slots) {
-        Pending &p = sv->queue.front();
-        long long N = (long long)p.req.n[0] * p.req.n[1] * p.req.n[2];
+    std::vector<Prep> batch;
+    size_t used = sv->used;
+    int live = (int)sv->live.size();
+    for (auto it = sv->queue.begin(); it != sv->queue.end() && live < sv->max_slots; ++it) {
+        const long long N = (long long)it->req.n[0] * it->req.n[1] * it->req.n[2];
         const size_t fp = footprint(sv, N);
-        if (!sv->live.empty() && sv->used + fp > sv->cap) break;  // wait for memory
-        auto res = std::make_shared<InstRes>();
-        res->shape = shape_of(sv, p.req);
-        res->V64 = sv->take(sizeof(double) * N);
-        res->V32 = sv->take(sizeof(float) * N);
-        res->D = sv->take(sizeof(double) * N);
-        build_potential(sv, p, *res);
+        if (live > 0 && used + fp > sv->cap) break;  // wait for memory (first come, first served)
+        Prep r;
+        r.p = &*it;
+        r.res = std::make_shared<InstRes>();
+        r.res->shape = shape_of(sv, it->req);
+        r.res->V64 = sv->take(sizeof(double) * N);
+        r.res->V32 = sv->take(sizeof(float) * N);
+        r.res->D = sv->take(sizeof(double) * N);
+        batch.push_back(std::move(r));
+        used += fp;
+        ++live;
+    }
+    if (batch.empty()) return;
+    build_potentials(sv, batch);
+    for (Prep &r : batch) {
+        Pending &p = *r.p;
+        const long long N = (long long)p.req.n[0] * p.req.n[1] * p.req.n[2];
+        const size_t fp = footprint(sv, N);
+        auto res = r.res;
         BdGrid g;
         for (int a = 0; a < 3; ++a) {
             g.n[a] = res->shape->n[a];
@@ -249,8 +342,8 @@ void admit(pisces_server *sv)
         sv->live.push_back(Live{p.id, slot, fp, p.t_submit, p.req.chain, p.req.nroots, N,
                                 p.req.want_vectors, res});
         sv->used += fp;
-        sv->queue.pop_front();
     }
+    for (size_t i = 0; i < batch.size(); ++i) sv->queue.pop_front();
 }
 
 void retire(pisces_server *sv)
