@@ -19,6 +19,7 @@ const char *last_error();
 // ------------------------------------------------------------------ context
 pisces_ctx::~pisces_ctx()
 {
+    rc.reset();  // stops a reverse-communication solve still waiting for its caller
     if (device >= 0) static_cast<void>(hipSetDevice(device));
     if (stream) static_cast<void>(hipStreamSynchronize(stream));
     if (hjobs) static_cast<void>(hipHostFree(hjobs));

@@ -1,6 +1,7 @@
 // context.hpp -- the per-grid context behind the C ABI.
 #pragma once
 
+#include <functional>
 #include <memory>
 #include <new>
 #include <string>
@@ -69,8 +70,16 @@ int guarded_call(F &&f)
         return PISCES_ERR_NOMEM;
     }
 }
+// Matrix-free hooks of the FP64 reference-mirroring solver: with matvec set,
+// Y = H X for nvec device columns (X at subspace column col0) is obtained from
+// it instead of the context's H*X; with host_diag set, diag(H) is uploaded
+// from it instead of being built from the potential.
+struct Fp64Hooks {
+    std::function<void(const double *X, double *Y, int col0, int nvec)> matvec;
+    const double *host_diag = nullptr;
+};
 int solve_fp64(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, int corrflag,
-               double *vecs, double *evals, pisces_stats *st);
+               double *vecs, double *evals, pisces_stats *st, const Fp64Hooks *hooks = nullptr);
 int solve_mixed(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, int corrflag,
                 int precision, double *vecs, double *evals, pisces_stats *st);
 }  // namespace pisces

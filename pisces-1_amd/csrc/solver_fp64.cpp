@@ -95,9 +95,11 @@ int ortho_block(Dev &d, int nb, int k, double *dst, std::vector<int> &keep)
 }  // namespace
 
 int solve_fp64(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, int corrflag,
-               double *vecs, double *evals, pisces_stats *st)
+               double *vecs, double *evals, pisces_stats *st, const Fp64Hooks *hooks)
 {
-    PC_REQUIRE(c->has_v, PISCES_ERR_STATE, "no potential set");
+    const bool ext = hooks && hooks->matvec;
+    PC_REQUIRE(ext || c->has_v, PISCES_ERR_STATE, "no potential set");
+    PC_REQUIRE(!ext || hooks->host_diag, PISCES_ERR_ARG, "matrix-free solve needs diag(H)");
     PC_REQUIRE(nroots >= 1 && nroots <= 16 && maxsub >= nroots + 1 && maxsub <= 64,
                PISCES_ERR_ARG, "need 1 <= nroots <= 16 and nroots < maxsub <= 64");
     PC_REQUIRE(corrflag >= 0 && corrflag <= 2, PISCES_ERR_ARG, "corrflag must be 0, 1 or 2");
@@ -121,8 +123,12 @@ int solve_fp64(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, in
           c->winfo.as<int>(), part_cap, c->hsmall};
 
     PC_HIP(hipMemcpyAsync(d.B, vecs, sizeof(double) * N * nroots, hipMemcpyHostToDevice, d.s));
-    diag_h(c->dV64.as<double>(), c->dT64[0].as<double>(), c->dT64[1].as<double>(),
-           c->dT64[2].as<double>(), c->n[0], c->n[1], c->n[2], d.diag, d.s);
+    if (ext)
+        PC_HIP(hipMemcpyAsync(d.diag, hooks->host_diag, sizeof(double) * N, hipMemcpyHostToDevice,
+                              d.s));
+    else
+        diag_h(c->dV64.as<double>(), c->dT64[0].as<double>(), c->dT64[1].as<double>(),
+               c->dT64[2].as<double>(), c->n[0], c->n[1], c->n[2], d.diag, d.s);
 
     // GramSchmidt of the start vectors (src/Davidson.cpp:482-514)
     std::vector<int> keep;
@@ -143,7 +149,8 @@ int solve_fp64(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, in
     for (;;) {
         nnewZ = 0;
         if (nnewB > 0) {  // the reverse interface's "return 1": Z = H B (new columns)
-            c->hx(d.B + N * nsub, d.Z + N * nsub, N, nnewB, PISCES_FP64);
+            if (ext) hooks->matvec(d.B + N * nsub, d.Z + N * nsub, nsub, nnewB);
+            else c->hx(d.B + N * nsub, d.Z + N * nsub, N, nnewB, PISCES_FP64);
             nmtx += nnewB;
             nnewZ = nnewB;
             nnewB = 0;

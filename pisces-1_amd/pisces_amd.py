@@ -204,6 +204,34 @@ class Context:
               "pisces_kinetic_precond")
         return t
 
+    def davidson_rc(self, start, diag, matvec, maxsub=20, maxmacro=100, ptol=7, corrflag=1):
+        """Reverse-communication Davidson (pisces_davidson): the caller's
+        matvec(X) -> H X (rows = vectors) is called on every request, exactly as
+        DVR::davdriver calls MatrixTimesVector.  Returns (nconv, evals, vecs, n_mtx)."""
+        L = lib()
+        L.pisces_davidson.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 7 + [
+            _D, ctypes.POINTER(ctypes.c_int), _D, _D, _D, _D, ctypes.POINTER(ctypes.c_int)]
+        nroots, N = start.shape
+        B = np.zeros((maxsub, N))
+        Z = np.zeros((maxsub, N))
+        B[:nroots] = start
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        ev = np.zeros(nroots)
+        work = np.zeros(1)
+        nconv = ctypes.c_int(0)
+        inout = (ctypes.c_int * 3)()
+        n_mtx = 0
+        while True:
+            ido = check(L.pisces_davidson(self.h, N, maxsub, nroots, maxmacro, ptol, corrflag, 0,
+                                          _p(ev), ctypes.byref(nconv), _p(B), _p(Z), _p(diag),
+                                          _p(work), inout), "pisces_davidson")
+            if ido == 0:
+                break
+            b0, z0, nv = inout[0], inout[1], inout[2]
+            Z[z0:z0 + nv] = matvec(B[b0:b0 + nv])
+            n_mtx += nv
+        return nconv.value, ev, B[:nroots].copy(), n_mtx
+
     def time_hx(self, nvec, precision, warmup=5, iters=50):
         ms = ctypes.c_double(0)
         check(lib().pisces_time_hx(self.h, int(nvec), int(precision), int(warmup), int(iters),

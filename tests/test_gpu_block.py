@@ -58,3 +58,28 @@ def test_block_solve_c2(pisces, precision):
     res = np.linalg.norm(HU - r["evals"][:, None] * r["vecs"], axis=1)
     assert np.all(res < 1e-7 * 1.01), res
     np.testing.assert_allclose(r["vecs"] @ r["vecs"].T, np.eye(4), atol=1e-10)
+
+
+def test_c3_mixed_within_1e6_of_fp64(pisces):
+    """(H2O)45 + Na+ on a 96^3 grid (48 bohr), 8 states: the FP32 and split-BF16
+    forms agree with the FP64 form within 1e-6 Eh per eigenvalue."""
+    import workload
+    print("c3", flush=True)
+    xyz, ion = workload.water_cluster(45, 8.0, 3000, ion_min=2.3)
+    ion_b = np.asarray(ion) * workload.ANGS2BOHR
+    n, L = (96, 96, 96), (48.0, 48.0, 48.0)
+    with pisces.Context(n, L) as ctx:
+        ctx.build_cluster_potential(xyz, ions=(ion_b, [1.0]), ion_r=2.9)
+        start = pisces.start_vectors(ctx, 8, seed=3)
+        r64 = ctx.solve_block(8, start, block=12, maxsub=48, max_iter=1500, ptol=7,
+                              precision=pisces.FP64)
+        assert r64["nconv"] == 8, r64
+        HU = ctx.apply_h(r64["vecs"])
+        res = np.linalg.norm(HU - r64["evals"][:, None] * r64["vecs"], axis=1)
+        assert np.all(res < 1e-7 * 1.01), res
+        for prec in ("fp32", "bf16x2"):
+            r = ctx.solve_block(8, start, block=12, maxsub=48, max_iter=1500, ptol=7,
+                                precision=pisces.PRECISIONS[prec])
+            print(prec, r["evals"] - r64["evals"], r["n_iter"], r["seconds"], flush=True)
+            assert r["nconv"] == 8, r
+            np.testing.assert_allclose(r["evals"], r64["evals"], rtol=0, atol=1e-6)

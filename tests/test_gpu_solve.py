@@ -55,3 +55,24 @@ def test_fp64_solve_c2_reproduces_reference(pisces):
     np.testing.assert_allclose(rq[:nc], f["rq"][:nc], rtol=0, atol=1e-8)
     res = np.linalg.norm(HU - rq[:, None] * U, axis=1)
     assert np.all(res[:nc] < 1e-7)
+
+
+@pytest.mark.parametrize("tag", ["g10_r1", "g12_r4", "g10_r2_jd"])
+def test_reverse_communication_matches_reference(pisces, oracle_mod, tag):
+    """pisces_davidson driven like DVR::davdriver, with the caller's own matvec
+    (the oracle's CPU H*x): the reference's eigenvalues and matvec count."""
+    print("rc", tag, flush=True)
+    f = golden("davidson_%s.npz" % tag)
+    n, L = tuple(int(v) for v in f["n"]), tuple(float(v) for v in f["L"])
+    g = oracle_mod.Grid(n, L)
+    V = oracle_mod.Cluster(f["xyz"]).potential_grid(g)
+    nroots, maxsub, maxmacro, ptol, corrflag = (int(v) for v in f["params"])
+    with pisces.Context(n, L) as ctx:
+        nconv, ev, vecs, n_mtx = ctx.davidson_rc(
+            f["start"], g.diag(V), lambda X: g.apply_h(V, X), maxsub=maxsub, maxmacro=maxmacro,
+            ptol=ptol, corrflag=corrflag)
+    assert nconv == int(f["nconv"]) == nroots
+    assert n_mtx == int(f["n_mtx"])
+    np.testing.assert_allclose(ev, f["evals"], rtol=0, atol=1e-8)
+    for i in range(nroots):
+        assert abs(vecs[i] @ f["vecs"][i]) >= 1 - 1e-8
