@@ -8,7 +8,12 @@
  * Every function restates the reference algorithm named in its comment (file:line
  * in the reference tree).  Plain C99 + OpenMP, FP64 throughout.
  *
- * Parity status of each function (what pins it) is listed in oracle/README.md.
+ * Parity: each function's comment ends with what pins it -- the reference's own
+ * outputs on the same inputs (tests/golden/, written by tests/golden/make_golden.py
+ * from oracle/_ref/ref_driver built out of the reference's sine_dvr.cpp,
+ * Davidson.cpp, Potential.cpp, MolPolAux.cpp and DerivDDTensor.cpp), the
+ * mathematics (tests/test_oracle_math.py), or "parity unpinned" where neither
+ * reaches.  The table in oracle/README.md collects them.
  */
 #ifndef PISCES_ORACLE_H
 #define PISCES_ORACLE_H
@@ -23,9 +28,11 @@ extern "C" {
  * matrix); the grid is called with x0 = -L/2, xN+1 = L/2, mass 1 by
  * src/DVR.cpp:602-605. */
 void or_sine_dvr(int n, double mass, double x0, double xnp1, double *x, double *T);
+/* Pinned: reference output (tkin fixture), spectrum pi^2 n^2 / (2 L^2), FBR transform. */
 /* Same matrix in the reference's packed layout (ldt == 0: a*(a+1)/2 + b, b <= a),
  * src/sine_dvr.cpp:84,106. */
 void or_sine_dvr_packed(int n, double mass, double x0, double xnp1, double *x, double *Tp);
+/* Pinned: reference output (tkin fixture, packed layout). */
 
 /* ---------------------------------------------------------------- a3, a4 */
 /* Y[:,v] = V o X[:,v] + (Tx (x) I (x) I + I (x) Ty (x) I + I (x) I (x) Tz) X[:,v]
@@ -35,12 +42,17 @@ void or_sine_dvr_packed(int n, double mass, double x0, double xnp1, double *x, d
 void or_apply_h(int nx, int ny, int nz, const double *Tx, const double *Ty,
                 const double *Tz, const double *V, const double *X, double *Y,
                 int nvec);
+/* Pinned: the dense Kronecker-sum Hamiltonian (DVR::build_h construction),
+ * and the reference Davidson's eigenpairs on it (davidson fixtures). */
 /* diag(H) = V + Tx_ii + Ty_jj + Tz_kk; src/DVR.cpp:1808-1881. */
 void or_diag(int nx, int ny, int nz, const double *Tx, const double *Ty,
              const double *Tz, const double *V, double *D);
+/* Pinned: diagonal of the dense Hamiltonian. */
 /* Half-box particle-in-a-box start vector; src/DVR.cpp:1725-1805. */
 void or_piab_start(int nx, int ny, int nz, const double *xg, const double *yg,
                    const double *zg, double *wf);
+/* Parity unpinned: only a start vector (the reference fixtures were computed
+ * from these same start vectors, so it fixes iteration paths, not results). */
 
 /* ---------------------------------------------------------------- a8 */
 /* Rigid DPP waters from O,H1,H2 Cartesian input (Angstrom, 9 doubles/water).
@@ -50,6 +62,9 @@ void or_piab_start(int nx, int ny, int nz, const double *xg, const double *yg,
  * (12 doubles/water) and the configuration (x,y,z,phi,theta,psi).
  * Returns 0, or -1 if a projection produced a non-finite value. */
 int or_water_sites(int nw, const double *xyz_ang, double *sites, double *conf);
+/* Pinned by invariants only (Water.cpp does not compile here): rigid input is
+ * reproduced, M on the bisector 0.25 A from O, centre of mass with masses 16/1/1.
+ * Parity unpinned: the Euler-angle conventions for non-rigid input. */
 
 /* Self-consistent induced dipoles on the polarizable sites (O,H1,H2 of each
  * water): WaterCluster::CalcInducedDipoles, src/DPP.cpp:760-1065.
@@ -62,6 +77,9 @@ int or_water_sites(int nw, const double *xyz_ang, double *sites, double *conf);
  *   *iters    SCF iterations used.  Returns 0 if converged, 1 if not. */
 int or_water_induced(int nw, const double *sites, double *mu, double *epc,
                      double *e_pol, int *iters);
+/* Pinned: the fixed point solves the dense linear system (1 - alpha T) mu =
+ * alpha E_pc.  Parity unpinned: the damping forms and constants themselves
+ * (restated from src/DPP.cpp:1798-1812, 2377-2410) and e_pol. */
 
 /* ---------------------------------------------------------------- a5, a6 */
 /* The electron-water potential at npts points, for the site/charge/dipole/
@@ -81,6 +99,10 @@ int or_potential_points(const int *flags, const double *potpara, int nsites,
                         int nd, const double *d, const int *id, int nion,
                         const double *ion_xyz, const double *ion_q, double ion_r,
                         int npts, const double *pts, double *v, double *parts);
+/* Pinned: the reference's Potential::Evaluate on the same lists for nine flag
+ * sets (potential fixtures), rigid-motion invariance, far field, Reff C1
+ * continuity.  Parity unpinned: the ion term (no reference combines it with
+ * waters; checked only against its closed form). */
 
 /* Whole grid, x fastest: points x_i, y_j, z_k from the 1-D grids. */
 int or_potential_grid(const int *flags, const double *potpara, int nsites,
@@ -96,6 +118,8 @@ int or_potential_grid(const int *flags, const double *potpara, int nsites,
  *   on O,H1,H2.  Returns the number of charges (3*nw). */
 int or_water_lists(int nw, const double *sites, const double *mu, double *q,
                    int *iq, double *d, int *id);
+/* Parity unpinned beyond charge neutrality of each water: the fixtures pass
+ * these same lists to the reference. */
 
 /* ---------------------------------------------------------------- a9, a10 */
 typedef void (*or_matvec_fn)(const double *x, double *y, int nvec, void *ctx);
@@ -112,6 +136,9 @@ int or_davidson(int ndim, int maxsub, int nroots, int maxmacro, int ptol,
                 int corrflag, or_matvec_fn mv, void *ctx, const double *diag,
                 double *B, double *evals, int *n_mtx, int *n_macro,
                 double *resid);
+/* Pinned: the reference Davidson's eigenvalues, vectors and matvec counts on
+ * the same inputs (davidson fixtures, corrflag 1 and 2, a non-converging case),
+ * and the dense eigensolve. */
 
 /* Convenience: Davidson on the sine-DVR Hamiltonian of a grid. */
 int or_solve_grid(int nx, int ny, int nz, const double *Tx, const double *Ty,
@@ -123,6 +150,7 @@ int or_solve_grid(int nx, int ny, int nz, const double *Tx, const double *Ty,
  * column-major (read: upper triangle), W[n], Vec n*n column-major.  Stands in
  * for LAPACK dsyev at src/Davidson.cpp:224. */
 int or_syev(int n, const double *A, int lda, double *W, double *Vec, int ldv);
+/* Pinned: LAPACK (numpy eigvalsh) and A V = V diag(W). */
 
 /* Threads used by the OpenMP loops (0 = library default). */
 void or_set_threads(int n);

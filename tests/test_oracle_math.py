@@ -161,3 +161,20 @@ def test_effective_distance_damping_is_c1_continuous(oracle_mod):
     d_left = (vp[2] - vp[0]) / (2 * eps)
     d_right = (vp[4] - vp[2]) / (2 * eps)
     assert abs(d_left - d_right) < 1e-4 * abs(d_left) + 1e-9
+
+
+def test_diag_is_dense_diagonal(oracle_mod):
+    g = oracle_mod.Grid((5, 4, 6), (9.0, 8.0, 10.0))
+    V = np.random.default_rng(3).uniform(-1, 1, g.ngp)
+    np.testing.assert_allclose(g.diag(V), np.diag(dense_h(g, V)), rtol=0, atol=1e-13)
+
+
+def test_syev_matches_lapack(oracle_mod):
+    rng = np.random.default_rng(9)
+    for n in (1, 2, 7, 24):
+        A = rng.normal(size=(n, n))
+        A = A + A.T
+        W, Vv = oracle_mod.syev(A)
+        w = np.linalg.eigvalsh(A)
+        np.testing.assert_allclose(W, w, rtol=0, atol=1e-12 * max(1.0, np.abs(w).max()))
+        np.testing.assert_allclose(A @ Vv, Vv * W[None, :], rtol=0, atol=1e-11 * max(1.0, np.abs(w).max()))
