@@ -237,6 +237,7 @@ def main():
         "iters_mean": round(float(np.mean([r["n_iter"] for r in res[-1]])), 1),
         "roofline": roof,
         "launch_families": fam_out,
+        "autotuned": P.tuning(),
         "hx_standalone_64cube_x8": hx,
         "cpu_baseline": cpu,
     }

@@ -69,6 +69,11 @@ int pisces_prof_enable(int on);
 int pisces_prof_reset(void);
 int pisces_prof_read(int kind, double *ms_total, long long *launches, double *bytes_total);
 const char *pisces_prof_name(int kind);
+/* Gram kernel configuration picked by the start-up autotuner of the block
+ * solver (variant 0 VALU register tiles, 1 matrix cores with 16-row blocks,
+ * 2 matrix cores with 64-row runs; rows per workgroup).  Valid after the first
+ * block solve or server creation. */
+int pisces_tuning_gram(int *variant, int *rows);
 
 /* Create a context for one sine-DVR grid of n[0] x n[1] x n[2] points on boxes
  * of length L[3] (bohr), centred at the origin; computes the 1-D grids and

@@ -4,7 +4,11 @@
 // one-workgroup-per-task kernels) picks the task.  Loads of FP32 or FP64 columns
 // go through ld_any(); the flag is uniform per task, so the branch is uniform.
 #include "bd.hpp"
+#include "blockops.hpp"
 
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
 #include <mutex>
 
 namespace pisces {
@@ -24,6 +28,133 @@ __device__ inline void st_any(void *p, long long i, int is32, double v)
 }
 
 // ------------------------------------------------------------------ Gram
+// A^T B of two tall column blocks (rows = grid points) on the FP64 matrix cores.
+// v_mfma_f64_16x16x4f64 takes A^T as a 16 x 4 operand (lane l: row m = l%16,
+// k = l/16) and B as 4 x 16 (k = l/16, column n = l%16) and accumulates a
+// 16 x 16 tile, element r of lane l holding D[l/16 + 4r][l%16].  The k index
+// is a row of the blocks; any assignment of rows to k works as long as A and B
+// use the same one, so lane (n, q) loads rows r0 + 4q .. r0 + 4q + 3 of column
+// n (two 16-byte loads; the wave covers 16 rows x 16 columns, whole cache
+// lines) and feeds them to four consecutive instructions.  All column tiles of
+// A (ma <= 64) against the single tile of B (mb <= 16) are accumulated at once,
+// so every element is read from memory once.  Workgroup = 4 waves on
+// interleaved 16-row blocks of one chunk; the waves' tiles are summed in LDS
+// into the chunk's partial.
+typedef double gram_f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ void gram_load4(const T *__restrict__ col, long long r, long long N,
+                                           bool vec, double (&v)[4])
+{
+    if (vec && r + 4 <= N) {
+        if (sizeof(T) == 8) {
+            const double2 a = *reinterpret_cast<const double2 *>(col + r);
+            const double2 b = *reinterpret_cast<const double2 *>(col + r + 2);
+            v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+        } else {
+            const float4 a = *reinterpret_cast<const float4 *>(col + r);
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = (r + j < N) ? (double)col[r + j] : 0.0;
+    }
+}
+
+template <typename TA, typename TB>
+__device__ __forceinline__ void gram_chunk(const TA *__restrict__ A, long long lda, int ma,
+                                           const TB *__restrict__ B, long long ldb, int mb,
+                                           long long N, long long r_beg, long long r_end,
+                                           bool vec, gram_f64x4 (&acc)[kBdMaxM / 16])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, q = lane >> 4;
+    const int nta = (ma + 15) / 16;
+    const bool bval = n < mb;
+    const TB *bcol = B + (long long)(bval ? n : 0) * ldb;
+    for (long long r0 = r_beg + 16 * wave; r0 < r_end; r0 += 64) {
+        const long long r = r0 + 4 * q;
+        double bv[4];
+        gram_load4(bcol, r, N, vec, bv);
+        if (!bval) bv[0] = bv[1] = bv[2] = bv[3] = 0.0;
+#pragma unroll
+        for (int t = 0; t < kBdMaxM / 16; ++t) {
+            if (t >= nta) break;
+            const int a = 16 * t + n;
+            double av[4];
+            gram_load4(A + (long long)(a < ma ? a : 0) * lda, r, N, vec, av);
+            if (a >= ma) av[0] = av[1] = av[2] = av[3] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j], bv[j], acc[t], 0, 0, 0);
+        }
+    }
+}
+
+// MFMA variant with long runs: per iteration a wave covers 64 consecutive rows,
+// lane (n, q) holding rows r0 + 16q .. r0 + 16q + 15 of column n (four 16-byte
+// loads per column for FP64), fed to 16 instructions; each column is then read
+// in 512-byte runs like the VALU kernel.
+template <typename T>
+__device__ __forceinline__ void gram_load16(const T *__restrict__ col, long long r, long long N,
+                                            bool vec, double (&v)[16])
+{
+    if (vec && r + 16 <= N) {
+        if (sizeof(T) == 8) {
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const double2 a = *reinterpret_cast<const double2 *>(col + r + 2 * h);
+                v[2 * h] = a.x;
+                v[2 * h + 1] = a.y;
+            }
+        } else {
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const float4 a = *reinterpret_cast<const float4 *>(col + r + 4 * h);
+                v[4 * h] = a.x; v[4 * h + 1] = a.y; v[4 * h + 2] = a.z; v[4 * h + 3] = a.w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = (r + j < N) ? (double)col[r + j] : 0.0;
+    }
+}
+
+template <typename TA, typename TB>
+__device__ __forceinline__ void gram_chunk64(const TA *__restrict__ A, long long lda, int ma,
+                                             const TB *__restrict__ B, long long ldb, int mb,
+                                             long long N, long long r_beg, long long r_end,
+                                             bool vec, gram_f64x4 (&acc)[kBdMaxM / 16])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, q = lane >> 4;
+    const int nta = (ma + 15) / 16;
+    const bool bval = n < mb;
+    const TB *bcol = B + (long long)(bval ? n : 0) * ldb;
+    for (long long r0 = r_beg + 64 * wave; r0 < r_end; r0 += 256) {
+        const long long r = r0 + 16 * q;
+        double bv[16];
+        gram_load16(bcol, r, N, vec, bv);
+        if (!bval)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) bv[j] = 0.0;
+#pragma unroll
+        for (int t = 0; t < kBdMaxM / 16; ++t) {
+            if (t >= nta) break;
+            const int a = 16 * t + n;
+            double av[16];
+            gram_load16(A + (long long)(a < ma ? a : 0) * lda, r, N, vec, av);
+            if (a >= ma)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) av[j] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j], bv[j], acc[t], 0, 0, 0);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Gram, VALU variant
 // A^T B of two tall column blocks (rows = grid points).  Workgroup = one row
 // chunk x one 8 x 8 tile of (A column, B column) pairs.  Each thread streams
 // rows g = chunk start + tid + 256 s (consecutive lanes on consecutive rows,
@@ -92,8 +223,8 @@ __device__ __forceinline__ double wave_reduce_scatter64(double (&v)[64])
     return v[0];
 }
 
-__global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ tasks,
-                                                  double *__restrict__ partial)
+__global__ void __launch_bounds__(kT) gram_valu_kernel(const GramTask *__restrict__ tasks,
+                                                       double *__restrict__ partial, int rows)
 {
     const GramTask t = tasks[blockIdx.y];
     const int ntile = t.cpb;  // tiles of this task
@@ -102,8 +233,8 @@ __global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ t
     const int nta = (t.ma + kGTile - 1) / kGTile;
     const int a0 = (tile % nta) * kGTile, b0 = (tile / nta) * kGTile;
     const int na = min(kGTile, t.ma - a0), nb = min(kGTile, t.mb - b0);
-    const long long r0 = (long long)chunk * kGRows;
-    const long long r1 = min(r0 + kGRows, t.N);
+    const long long r0 = (long long)chunk * rows;
+    const long long r1 = min(r0 + rows, t.N);
     double acc[kGTile * kGTile];
 #pragma unroll
     for (int i = 0; i < kGTile * kGTile; ++i) acc[i] = 0.0;
@@ -130,6 +261,60 @@ __global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ t
             const long long o = (long long)(a0 + i) + (long long)t.ma * (b0 + j);
             partial[t.part_off + o * t.nch + chunk] = sum;
         }
+    }
+}
+
+template <int LONG>
+__global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ tasks,
+                                                  double *__restrict__ partial, int rows)
+{
+    const GramTask &t = tasks[blockIdx.y];
+    if ((int)blockIdx.x >= t.nch) return;
+    const long long r_beg = (long long)blockIdx.x * rows;
+    const long long r_end = min(r_beg + rows, t.N);
+    // 16-byte loads need every column start aligned: uniform per task
+    const int ea = t.a32 ? 4 : 8, eb = t.b32 ? 4 : 8;
+    const bool vec = ((reinterpret_cast<uintptr_t>(t.A) | reinterpret_cast<uintptr_t>(t.B)) & 15) == 0 &&
+                     ((t.lda * ea) & 15) == 0 && ((t.ldb * eb) & 15) == 0;
+    gram_f64x4 acc[kBdMaxM / 16];
+#pragma unroll
+    for (int i = 0; i < kBdMaxM / 16; ++i) acc[i] = gram_f64x4{0.0, 0.0, 0.0, 0.0};
+#define PC_GRAM_CALL(TA_, TB_)                                                                  \
+    do {                                                                                        \
+        const TA_ *A = static_cast<const TA_ *>(t.A);                                           \
+        const TB_ *B = static_cast<const TB_ *>(t.B);                                           \
+        if (LONG) gram_chunk64(A, t.lda, t.ma, B, t.ldb, t.mb, t.N, r_beg, r_end, vec, acc);    \
+        else gram_chunk(A, t.lda, t.ma, B, t.ldb, t.mb, t.N, r_beg, r_end, vec, acc);           \
+    } while (0)
+    if (t.a32) {
+        if (t.b32) PC_GRAM_CALL(float, float);
+        else PC_GRAM_CALL(float, double);
+    } else {
+        if (t.b32) PC_GRAM_CALL(double, float);
+        else PC_GRAM_CALL(double, double);
+    }
+#undef PC_GRAM_CALL
+    // sum the four waves' tiles: red[wave][tile][16 x 16]
+    __shared__ double red[kT / 64][kBdMaxM / 16][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, q = lane >> 4;
+    const int nta = (t.ma + 15) / 16;
+#pragma unroll
+    for (int i = 0; i < kBdMaxM / 16; ++i) {
+        if (i >= nta) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave][i][(q + 4 * r) * 16 + n] = acc[i][r];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nta * 256; e += kT) {
+        const int i = e >> 8, m = (e >> 4) & 15, c = e & 15;
+        const int a = 16 * i + m;
+        if (a >= t.ma || c >= t.mb) continue;
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < kT / 64; ++w) sum += red[w][i][m * 16 + c];
+        const long long o = (long long)a + (long long)t.ma * c;
+        partial[t.part_off + o * t.nch + blockIdx.x] = sum;
     }
 }
 
@@ -683,14 +868,50 @@ inline unsigned nblk(long long N) { return (unsigned)((N + kT - 1) / kT); }
 
 }  // namespace
 
+// Gram configuration: kernel variant (0 VALU register tiles, 1 matrix cores
+// with 16-row blocks, 2 matrix cores with 64-row runs) and rows per workgroup.
+// Set by the block engine's online tuner (or PISCES_GRAM=<variant>[:rows]).
+struct GramCfg {
+    int variant, rows;
+};
+static GramCfg g_gram = {2, kGRows};
+
+int gram_variant() { return g_gram.variant; }
+
+int gram_chunks(long long N) { return (int)((N + g_gram.rows - 1) / g_gram.rows); }
+
+int gram_blocks_per_chunk(int ma, int mb)
+{
+    if (g_gram.variant != 0) return 1;
+    return ((ma + kGTile - 1) / kGTile) * ((mb + kGTile - 1) / kGTile);
+}
+
 void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_mamb, double *d_partial,
              hipStream_t s)
 {
     if (ntasks <= 0) return;
-    hipLaunchKernelGGL(gram_kernel, dim3(max_blocks, ntasks), dim3(kT), 0, s, d_tasks, d_partial);
+    const int v = g_gram.variant, rows = g_gram.rows;
+    if (v == 0)
+        hipLaunchKernelGGL(gram_valu_kernel, dim3(max_blocks, ntasks), dim3(kT), 0, s, d_tasks, d_partial, rows);
+    else if (v == 1)
+        hipLaunchKernelGGL(gram_kernel<0>, dim3(max_blocks, ntasks), dim3(kT), 0, s, d_tasks, d_partial, rows);
+    else
+        hipLaunchKernelGGL(gram_kernel<1>, dim3(max_blocks, ntasks), dim3(kT), 0, s, d_tasks, d_partial, rows);
     hipLaunchKernelGGL(gram_reduce_kernel, dim3((max_mamb + kT - 1) / kT, ntasks), dim3(kT), 0, s,
                        d_tasks, d_partial);
     PC_HIP(hipGetLastError());
+}
+
+void gram_set(int variant, int rows)
+{
+    g_gram.variant = variant;
+    g_gram.rows = rows;
+}
+
+int gram_config(int *rows)
+{
+    if (rows) *rows = g_gram.rows;
+    return g_gram.variant;
 }
 
 void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, hipStream_t s)

@@ -19,8 +19,8 @@ struct GramTask {
     long long N;
     double *out; int out_ld;
     long long part_off;  // this task's region of the partial buffer (doubles)
-    int nch;             // row chunks of kGRows rows (gram_chunks(N))
-    int cpb;             // 8 x 8 column tiles (gram_tiles(ma, mb)); workgroups = nch * cpb
+    int nch;             // row chunks (gram_chunks(N))
+    int cpb;             // workgroups per chunk (gram_blocks_per_chunk(ma, mb))
 };
 
 // Y (set|-=|+=) A1 W1 + A2 W2 + A3 W3, W in device memory (column-major);
@@ -92,7 +92,8 @@ struct CholTask {
     int *keep_copy;  // optional second destination of keep_out
 };
 
-// max_blocks = max over tasks of nch * cpb; max_mamb = max over tasks of ma * mb
+// max_blocks = max over tasks of nch; max_mamb = max over tasks of ma * mb;
+// requires ma <= kBdMaxM and mb <= 16
 // Search directions of a restart in the coefficients of the old basis [X|C]:
 // Q = orth([0; Y_C] against Y), Y the m x k Ritz coefficients (ld m) whose
 // first k rows belong to X.  Q (m x k, ld m) has orthonormal columns
@@ -116,10 +117,17 @@ void bd_ritz(const RitzTask *d_tasks, int ntasks, int max_nch, int k, double *d_
 void bd_orth(const OrthTask *d_tasks, int ntasks, hipStream_t s);
 void bd_coef(const CoefTask *d_tasks, int ntasks, hipStream_t s);
 
-constexpr int kGTile = 8;          // Gram tile edge (columns of A, of B)
-constexpr int kGRows = 256 * 32;   // rows per Gram chunk (32 per thread)
-inline int gram_chunks(long long N) { return (int)((N + kGRows - 1) / kGRows); }
-inline int gram_tiles(int ma, int mb) { return ((ma + kGTile - 1) / kGTile) * ((mb + kGTile - 1) / kGTile); }
+constexpr int kGRows = 4096;        // rows per Gram workgroup, matrix-core variants
+constexpr int kGTile = 8;           // VALU variant: 8 x 8 tiles of (A, B) columns
+constexpr int kGRowsValu = 256 * 32; // VALU variant: rows per chunk (32 per thread)
+// Gram kernel variant (0 VALU register tiles, 1 matrix cores with 16-row
+// blocks, 2 matrix cores with 64-row runs) and rows per workgroup, and the
+// task sizing they imply.  The block engine tunes them on its live workload.
+void gram_set(int variant, int rows);
+int gram_config(int *rows);
+int gram_variant();
+int gram_chunks(long long N);
+int gram_blocks_per_chunk(int ma, int mb);
 constexpr int kBdResChunk = 1024; // rows per residual chunk
 constexpr int kRitzRows = 256;    // rows per Ritz/residual workgroup (one per thread)
 

@@ -1,6 +1,7 @@
 // prof.cpp -- HIP-event timing of batched launch families (see prof.hpp).
 #include "prof.hpp"
 
+#include "bd.hpp"
 #include "context.hpp"
 
 #include <cstring>
@@ -114,6 +115,13 @@ int pisces_prof_read(int kind, double *ms_total, long long *launches, double *by
         if (bytes_total) *bytes_total = g_tot[kind].bytes;
         return 0;
     });
+}
+
+int pisces_tuning_gram(int *variant, int *rows)
+{
+    if (!variant || !rows) return PISCES_ERR_ARG;
+    *variant = gram_config(rows);
+    return 0;
 }
 
 const char *pisces_prof_name(int kind)

@@ -333,6 +333,13 @@ def prof_read():
     return out
 
 
+def tuning():
+    """Autotuned kernel configuration ({'gram_variant', 'gram_rows'})."""
+    v, r = ctypes.c_int(), ctypes.c_int()
+    check(lib().pisces_tuning_gram(ctypes.byref(v), ctypes.byref(r)), "pisces_tuning_gram")
+    return {"gram_variant": v.value, "gram_rows": r.value}
+
+
 # ---------------------------------------------------------------- server
 class Request(ctypes.Structure):
     _fields_ = [
