@@ -123,12 +123,19 @@ def main():
                     help="c2: the headline workload; c4: 512 uneven snapshots in one batch; "
                          "c5: open-loop Poisson-served MD chains")
     ap.add_argument("--duration", type=float, default=8.0, help="c5: seconds per offered rate")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the Gram kernel configuration on the served workload and "
+                         "write pisces-1_amd/tuning.txt")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: one GPU only; this build has no multi-GPU path (--gpus %d)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if a.tune:
+        print(json.dumps(tune_gram(a)), flush=True)
+        return
 
     import pisces_amd as P
     import workload
@@ -374,6 +381,42 @@ def bench_c5(P, workload, prec, a, nchains=64):
                                    "rigid moves 0.05 A / 2 deg, warm starts, admission cap "
                                    "200 GB, %.0f s per offered rate" % (nchains, a.duration),
                        "chains_96": sum(ch["n"] == 96 for ch in chains)}}
+
+
+GRAM_CANDIDATES = ["0:8192", "1:4096", "2:2048", "2:4096", "2:8192"]
+
+
+def tune_gram(a):
+    """Autotuner of the Gram kernel (variant x rows per workgroup): one served
+    run of the headline workload per candidate, each in a fresh process with the
+    candidate forced through PISCES_GRAM; the candidate with the highest Gram
+    throughput (algorithmic bytes over the HIP-event time of the Gram launches
+    of the timed region) is written to pisces-1_amd/tuning.txt, which the
+    library reads at start-up.  Whole served runs are the measure because the
+    kernels' speed depends on the cache state the surrounding solver leaves."""
+    import subprocess
+    res = {}
+    for cand in GRAM_CANDIDATES:
+        env = dict(os.environ, PISCES_GRAM=cand)
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--served-only",
+                              "--steps", "2", "--warmup", "1", "--precision", a.precision],
+                             env=env, capture_output=True, text=True, check=True)
+        d = json.loads(out.stdout.strip().splitlines()[-1])
+        res[cand] = {"gram_GBs": d["launch_families"]["gram"]["GBs_algorithmic"],
+                     "solves_per_s": d["value"]}
+        log("tune: gram %s -> %.1f GB/s, %.2f solves/s" % (cand, res[cand]["gram_GBs"],
+                                                            res[cand]["solves_per_s"]))
+    best = max(res, key=lambda c: res[c]["gram_GBs"])
+    v, r = best.split(":")
+    path = os.path.join(ROOT, "pisces-1_amd", "tuning.txt")
+    with open(path, "w") as f:
+        f.write("# Gram kernel configuration: variant rows-per-workgroup\n")
+        f.write("# written by `python bench.py --tune` from served runs of the (H2O)24 64^3 "
+                "workload\n")
+        for c in GRAM_CANDIDATES:
+            f.write("# %s: %.1f GB/s\n" % (c, res[c]["gram_GBs"]))
+        f.write("gram %s %s\n" % (v, r))
+    return {"tuned": "gram", "best": best, "candidates": res, "written": path}
 
 
 def measured_traffic(family, algorithmic_bytes_per_launch):

@@ -6,6 +6,10 @@
 #include "bd.hpp"
 #include "blockops.hpp"
 
+#include <dlfcn.h>
+#include <cstdio>
+#include <cstring>
+#include <string>
 #include <algorithm>
 #include <cstdlib>
 #include <vector>
@@ -875,6 +879,7 @@ struct GramCfg {
     int variant, rows;
 };
 static GramCfg g_gram = {2, kGRows};
+void gram_load_config();
 
 int gram_variant() { return g_gram.variant; }
 
@@ -906,6 +911,45 @@ void gram_set(int variant, int rows)
 {
     g_gram.variant = variant;
     g_gram.rows = rows;
+}
+
+// The configuration comes from PISCES_GRAM=<variant>[:rows], else from the line
+// "gram <variant> <rows>" of tuning.txt in the directory above the library
+// (written by the benchmark's tuning mode from whole served runs), else the
+// default (64-row matrix-core runs, 4096 rows per workgroup).
+void gram_load_config()
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int v = 2, rows = kGRows;
+        const char *e = std::getenv("PISCES_GRAM");
+        if (e && e[0] >= '0' && e[0] <= '2') {
+            v = e[0] - '0';
+            rows = v == 0 ? kGRowsValu : kGRows;
+            if (e[1] == ':') rows = std::atoi(e + 2);
+        } else {
+            Dl_info info;
+            if (dladdr(reinterpret_cast<void *>(&gram_load_config), &info) && info.dli_fname) {
+                std::string path(info.dli_fname);
+                const size_t slash = path.rfind('/');
+                path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
+                       "/../tuning.txt";
+                if (FILE *f = std::fopen(path.c_str(), "r")) {
+                    char key[32];
+                    int fv, fr;
+                    while (std::fscanf(f, "%31s %d %d", key, &fv, &fr) == 3)
+                        if (!std::strcmp(key, "gram") && fv >= 0 && fv <= 2) {
+                            v = fv;
+                            rows = fr;
+                        }
+                    std::fclose(f);
+                }
+            }
+        }
+        rows = std::max(256, std::min(rows, 1 << 20)) & ~255;
+        g_gram.variant = v;
+        g_gram.rows = rows;
+    });
 }
 
 int gram_config(int *rows)

@@ -122,8 +122,10 @@ constexpr int kGTile = 8;           // VALU variant: 8 x 8 tiles of (A, B) colum
 constexpr int kGRowsValu = 256 * 32; // VALU variant: rows per chunk (32 per thread)
 // Gram kernel variant (0 VALU register tiles, 1 matrix cores with 16-row
 // blocks, 2 matrix cores with 64-row runs) and rows per workgroup, and the
-// task sizing they imply.  The block engine tunes them on its live workload.
+// task sizing they imply.
 void gram_set(int variant, int rows);
+// loads the configuration once per process (PISCES_GRAM, tuning.txt or default)
+void gram_load_config();
 int gram_config(int *rows);
 int gram_variant();
 int gram_chunks(long long N);

@@ -107,9 +107,9 @@ BdEngine::BdEngine(int device, hipStream_t s, int precision, int block, int maxs
                "unknown precision");
     arena_[0] = std::make_unique<Arena>();
     arena_[1] = std::make_unique<Arena>();
-
     const char *tr = std::getenv("PISCES_TRACE");
     trace_ = tr && tr[0] == '1';
+    gram_load_config();
 }
 
 BdEngine::~BdEngine()
@@ -236,112 +236,34 @@ void BdEngine::release(int slot)
     inst_.at(slot).reset();
 }
 
-// ------------------------------------------------------------------ Gram tuner
-// Online autotuning of the Gram kernel configuration on the live workload:
-// the first steps of the process rotate through the candidates (one per
-// step), time every Gram launch with HIP events and normalise by its
-// algorithmic bytes; after kTrials steps per candidate the fastest is kept for
-// the rest of the process.  PISCES_GRAM=<variant>:<rows> fixes a configuration.
-namespace {
-
-struct GramTuner {
-    struct Cand {
-        int variant, rows;
-        double ms = 0, bytes = 0;
-        int steps = 0;
-    };
-    std::vector<Cand> cands{{0, kGRowsValu}, {1, 4096}, {2, 2048}, {2, 4096}, {2, 8192}};
-    static constexpr int kTrials = 4;
-    bool fixed = false, done = false;
-    int cur = 0, step = 0;
-    struct Pend {
-        hipEvent_t a, b;
-        double bytes;
-        int cand;
-    };
-    std::vector<Pend> pend;
-    std::vector<hipEvent_t> pool;
-    GramTuner()
-    {
-        const char *e = std::getenv("PISCES_GRAM");
-        if (e && e[0] >= '0' && e[0] <= '2') {
-            int rows = e[0] == '0' ? kGRowsValu : 4096;
-            if (e[1] == ':') rows = std::max(256, std::atoi(e + 2)) & ~255;
-            gram_set(e[0] - '0', rows);
-            fixed = done = true;
-        }
-    }
-    bool active() const { return !done; }
-    // configuration for the next step
-    void begin_step()
-    {
-        if (done) return;
-        cur = step % (int)cands.size();
-        gram_set(cands[cur].variant, cands[cur].rows);
-        ++step;
-    }
-    hipEvent_t ev()
-    {
-        if (!pool.empty()) {
-            hipEvent_t e = pool.back();
-            pool.pop_back();
-            return e;
-        }
-        hipEvent_t e;
-        PC_HIP(hipEventCreate(&e));
-        return e;
-    }
-    // after a synchronisation of the stream: account the finished launches
-    void resolve()
-    {
-        if (pend.empty()) return;
-        std::vector<int> seen(cands.size(), 0);
-        for (Pend &p : pend) {
-            float ms = 0;
-            PC_HIP(hipEventElapsedTime(&ms, p.a, p.b));
-            cands[p.cand].ms += ms;
-            cands[p.cand].bytes += p.bytes;
-            seen[p.cand] = 1;
-            pool.push_back(p.a);
-            pool.push_back(p.b);
-        }
-        pend.clear();
-        for (size_t c = 0; c < cands.size(); ++c) cands[c].steps += seen[c];
-        bool all = true;
-        for (const Cand &c : cands) all = all && c.steps >= kTrials;
-        if (!all) return;
-        int best = 0;
-        for (size_t c = 1; c < cands.size(); ++c)
-            if (cands[c].ms / cands[c].bytes < cands[best].ms / cands[best].bytes) best = (int)c;
-        gram_set(cands[best].variant, cands[best].rows);
-        done = true;
-    }
-};
-
-GramTuner &gram_tuner()
-{
-    static GramTuner t;
-    return t;
-}
-
-}  // namespace
-
 // ------------------------------------------------------------------ launches
 namespace {
+
 
 struct GramBatch {
     GramTask *h = nullptr, *d = nullptr;
     int n = 0, max_blocks = 0, max_mamb = 0;
     long long part = 0;
+    // tasks are sized (row chunks, workgroups, partial offsets) in finish(),
+    // for the Gram configuration in use
     void add(const void *A, int ma, int a32, const void *B, int mb, int b32, long long N,
              double *out, int out_ld)
     {
-        GramTask t{A, N, ma, a32, B, N, mb, b32, N, out, out_ld, part, gram_chunks(N),
-                   gram_blocks_per_chunk(ma, mb)};
-        part += (long long)ma * mb * t.nch;
-        max_blocks = std::max(max_blocks, t.nch * t.cpb);
+        h[n++] = GramTask{A, N, ma, a32, B, N, mb, b32, N, out, out_ld, 0, 0, 0};
         max_mamb = std::max(max_mamb, ma * mb);
-        h[n++] = t;
+    }
+    void finish()
+    {
+        part = 0;
+        max_blocks = 0;
+        for (int i = 0; i < n; ++i) {
+            GramTask &t = h[i];
+            t.part_off = part;
+            t.nch = gram_chunks(t.N);
+            t.cpb = gram_blocks_per_chunk(t.ma, t.mb);
+            part += (long long)t.ma * t.mb * t.nch;
+            max_blocks = std::max(max_blocks, t.nch * t.cpb);
+        }
     }
     double bytes() const
     {
@@ -357,16 +279,7 @@ struct GramBatch {
     {
         if (!n) return;
         ProfScope ps(PK_GRAM, s, prof_enabled() ? bytes() : 0.0);
-        GramTuner &tu = gram_tuner();
-        if (tu.active()) {
-            GramTuner::Pend p{tu.ev(), tu.ev(), bytes(), tu.cur};
-            PC_HIP(hipEventRecord(p.a, s));
-            bd_gram(d, n, max_blocks, max_mamb, dp, s);
-            PC_HIP(hipEventRecord(p.b, s));
-            tu.pend.push_back(p);
-        } else {
-            bd_gram(d, n, max_blocks, max_mamb, dp, s);
-        }
+        bd_gram(d, n, max_blocks, max_mamb, dp, s);
     }
 };
 
@@ -439,7 +352,6 @@ int BdEngine::step()
         if (p && !p->done) act.push_back(p.get());
     if (act.empty()) return 0;
     const int n = (int)act.size();
-    gram_tuner().begin_step();
     const double tol = std::pow(10.0, -ptol_);
     const bool lo = prec_ != PISCES_FP64;
     const int e32 = lo ? 1 : 0;
@@ -607,6 +519,8 @@ int BdEngine::step()
     }
 
     // ============================ phase A: launches
+    for (int pass = 0; pass < 2; ++pass) rg[pass].finish();
+    gg.finish();
     const long long part = std::max({rg[0].part, rg[1].part, gg.part, res_part});
     grow_part(sizeof(double) * (size_t)std::max(part, 1LL));
     A.flush(s_);
@@ -638,7 +552,6 @@ int BdEngine::step()
     PC_HIP(hipMemcpyAsync(hsmall_, dgather_.p, sizeof(double) * 2 * kBdMaxK * n,
                           hipMemcpyDeviceToHost, s_));
     PC_HIP(hipStreamSynchronize(s_));
-    gram_tuner().resolve();
     for (Inst *in : act) in->have_prev = true;
     for (Inst *in : rs) {
         in->init = false;
@@ -781,7 +694,10 @@ int BdEngine::step()
         }
     }
     long long partB = 1;
-    for (int pass = 0; pass < 2; ++pass) partB = std::max(partB, og[pass].part);
+    for (int pass = 0; pass < 2; ++pass) {
+        og[pass].finish();
+        partB = std::max(partB, og[pass].part);
+    }
     grow_part(sizeof(double) * (size_t)partB);
     dp = dpart_.as<double>();
     B.flush(s_);

@@ -751,9 +751,9 @@ template <int NPMAX>
 void hx_bf2_launch(const HxJob *d, int n, int pa, int pb, int np, hipStream_t s)
 {
     init_np<NPMAX>();
-    const size_t lds = bf2_lds(np);
-    hipLaunchKernelGGL((hx_bf2_kernel<0, NPMAX>), dim3(pa, n), dim3(kPT), lds, s, d);
-    hipLaunchKernelGGL((hx_bf2_kernel<1, NPMAX>), dim3(pb, n), dim3(kPT), lds, s, d);
+    // pass A uses only the i-major copy (first half of the layout)
+    hipLaunchKernelGGL((hx_bf2_kernel<0, NPMAX>), dim3(pa, n), dim3(kPT), bf2_lds(np) / 2, s, d);
+    hipLaunchKernelGGL((hx_bf2_kernel<1, NPMAX>), dim3(pb, n), dim3(kPT), bf2_lds(np), s, d);
 }
 
 template <int NPMAX>
@@ -768,7 +768,8 @@ void fdm_np_launch(const FdmTask *d, int n, int nplanes, int np, int precision, 
     } else {
         const size_t lds = bf2_lds(np);
         hipLaunchKernelGGL((fdm_bf2_kernel<0, NPMAX>), dim3(nplanes, n), dim3(kPT), lds, s, d);
-        hipLaunchKernelGGL((fdm_bf2_kernel<1, NPMAX>), dim3(nplanes, n), dim3(kPT), lds, s, d);
+        // pass 1 uses only the i-major copy (first half of the layout)
+        hipLaunchKernelGGL((fdm_bf2_kernel<1, NPMAX>), dim3(nplanes, n), dim3(kPT), lds / 2, s, d);
         hipLaunchKernelGGL((fdm_bf2_kernel<2, NPMAX>), dim3(nplanes, n), dim3(kPT), lds, s, d);
     }
 }
