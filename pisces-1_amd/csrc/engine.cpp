@@ -385,26 +385,32 @@ int BdEngine::step()
     for (Inst *in : act)
         if (in->restart) rs.push_back(in);
     const int nr = (int)rs.size();
-    // restart: X = orth(U) (two Cholesky-QR passes), HX = HU W
+    // restart: X = orth(U) by Cholesky-QR, HX = HU W.  The Ritz block U is
+    // orthonormal up to the rounding of the stored basis (FP32 corrections:
+    // ~1e-7), so one pass leaves ~1e-14; the random start block of a new
+    // instance takes a second pass.  HX is not formed where the FP64 refresh
+    // below recomputes it.
     GramBatch rg[2];
     CholTask *rch[2] = {nullptr, nullptr}, *rcd[2] = {nullptr, nullptr};
     CombBatch rc[2];
+    int nch2[2] = {0, 0};
     for (int pass = 0; pass < 2; ++pass) {
         rg[pass].h = A.take<GramTask>(nr, &rg[pass].d);
         rch[pass] = A.take<CholTask>(nr, &rcd[pass]);
         rc[pass].h = A.take<CombTask>(2 * nr, &rc[pass].d);
         for (int i = 0; i < nr; ++i) {
             Inst *in = rs[i];
+            if (pass == 1 && !in->init) continue;
             const long long N = in->g.N;
             const double *src = pass == 0 ? in->U.as<double>() : in->X.as<double>();
             rg[pass].add(src, k_, 0, src, k_, 0, N, in->sm() + oGu, k_);
-            rch[pass][i] = CholTask{in->sm() + oGu, k_, 0, in->sm() + oWu,
-                                    pass ? in->keep() : nullptr, in->keep(), 1e-12, nullptr};
+            rch[pass][nch2[pass]++] = CholTask{in->sm() + oGu, k_, 0, in->sm() + oWu,
+                                               pass ? in->keep() : nullptr, in->keep(), 1e-12,
+                                               nullptr};
             rc[pass].add(comb1(src, N, k_, 0, in->sm() + oWu, k_, in->X.p, 0, k_, 0));
-            if (!in->init) {
-                const double *hsrc = pass == 0 ? in->HU.as<double>() : in->HX.as<double>();
-                rc[pass].add(comb1(hsrc, N, k_, 0, in->sm() + oWu, k_, in->HX.p, 0, k_, 0));
-            }
+            const bool refresh = in->init || lo || in->confirm;
+            if (!refresh) rc[pass].add(comb1(in->HU.as<double>(), N, k_, 0, in->sm() + oWu, k_,
+                                             in->HX.p, 0, k_, 0));
         }
     }
     // GD+k: the search directions of the last Ritz step -- the part of the Ritz
@@ -531,9 +537,9 @@ int BdEngine::step()
     }
     for (int pass = 0; pass < 2; ++pass) {
         rg[pass].launch(dp, s_);
-        if (nr) {
+        if (nch2[pass]) {
             ProfScope ps(PK_CHOL, s_, 0.0);
-            bd_cholinv(rcd[pass], nr, s_);
+            bd_cholinv(rcd[pass], nch2[pass], s_);
         }
         rc[pass].launch(s_);
     }
