@@ -156,6 +156,13 @@ int pisces_time_hx(pisces_ctx *ctx, int nvec, int precision, int warmup, int ite
 int pisces_kinetic_precond(pisces_ctx *ctx, int nvec, const double *r, double shift,
                            double scale, double floor, int precision, double *t);
 
+/* Times `iters` applications of the kinetic preconditioner (all three passes)
+ * to `nvec` device-resident vectors, as pisces_time_hx does for H*X: FP64
+ * vectors for PISCES_FP64, FP32 vectors with the split-BF16 form for
+ * PISCES_BF16X2.  Measurement hook; no reference counterpart. */
+int pisces_time_precond(pisces_ctx *ctx, int nvec, int precision, int warmup, int iters,
+                        double *ms_per_apply);
+
 /* Plain device-to-device copy of `bytes` (multiple of 16) by a streaming
  * kernel, `reps` timed launches after one warm-up; *ms = mean launch time.
  * Calibrates hardware byte counters and the achievable HBM bandwidth.

@@ -351,6 +351,48 @@ int pisces_time_hx(pisces_ctx *c, int nvec, int precision, int warmup, int iters
     });
 }
 
+int pisces_time_precond(pisces_ctx *c, int nvec, int precision, int warmup, int iters, double *ms)
+{
+    if (!c || nvec <= 0 || iters <= 0 || warmup < 0 || !ms) return PISCES_ERR_ARG;
+    if (precision != PISCES_FP64 && precision != PISCES_BF16X2) return PISCES_ERR_ARG;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        const bool f64 = precision == PISCES_FP64;
+        const long long n = c->ngp * (long long)nvec;
+        const size_t es = f64 ? sizeof(double) : sizeof(float);
+        dbuf R(es * n), Tb(es * n), W(sizeof(float) * n), tasks(sizeof(FdmTask) * nvec);
+        fill_uniform(R.p, n, f64, 4321u, c->stream);
+        std::vector<FdmTask> h(nvec);
+        for (int v = 0; v < nvec; ++v) {
+            FdmTask &f = h[v];
+            for (int a = 0; a < 3; ++a) f.ax[a] = c->ax[a].m;
+            f.src = R.as<unsigned char>() + es * c->ngp * v;
+            f.dst = Tb.as<unsigned char>() + es * c->ngp * v;
+            f.src32 = f.dst32 = f64 ? 0 : 1;
+            f.work = W.as<float>() + c->ngp * v;
+            f.shift = -0.05;
+            f.scale = -1.0;
+            f.floor = 1e-3;
+        }
+        PC_HIP(hipMemcpyAsync(tasks.p, h.data(), sizeof(FdmTask) * nvec, hipMemcpyHostToDevice,
+                              c->stream));
+        for (int w = 0; w < warmup; ++w) fdm_launch(tasks.as<FdmTask>(), nvec, c->maxn, precision, c->stream);
+        hipEvent_t e0, e1;
+        PC_HIP(hipEventCreate(&e0));
+        PC_HIP(hipEventCreate(&e1));
+        PC_HIP(hipEventRecord(e0, c->stream));
+        for (int w = 0; w < iters; ++w) fdm_launch(tasks.as<FdmTask>(), nvec, c->maxn, precision, c->stream);
+        PC_HIP(hipEventRecord(e1, c->stream));
+        PC_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        PC_HIP(hipEventElapsedTime(&t, e0, e1));
+        static_cast<void>(hipEventDestroy(e0));
+        static_cast<void>(hipEventDestroy(e1));
+        *ms = (double)t / iters;
+        return 0;
+    });
+}
+
 int pisces_copy_probe(int device, long long bytes, int reps, double *ms)
 {
     if (bytes <= 0 || bytes % 16 || reps <= 0 || !ms) return PISCES_ERR_ARG;

@@ -34,6 +34,8 @@
 #include "common.hpp"
 #include "fdm.hpp"
 
+#include <algorithm>
+#include <cstdlib>
 #include <mutex>
 
 namespace pisces {
@@ -43,6 +45,7 @@ namespace {
 constexpr int kPT = 256;
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -63,6 +66,16 @@ __device__ __forceinline__ void split2(float x, uint16_t &h, uint16_t &l)
 __device__ __forceinline__ bf16x8 ld8(const uint16_t *p)
 {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(p));
+}
+// Global-memory pointers.  Pointers read from job descriptors are generic to
+// the compiler, which then emits flat accesses (counted against both the
+// vector-memory and the LDS counters); casting them to the global address
+// space gives global_load/global_store with plain vector-memory waits.
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ gptr<T> glb(T *p) { return (gptr<T>)p; }
+__device__ __forceinline__ bf16x8 ld8(gptr<const uint16_t> p)
+{
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<gptr<const u32x4>>(p));
 }
 __device__ __forceinline__ f32x4 mma_bf(bf16x8 a, bf16x8 b, f32x4 c)
 {
@@ -123,68 +136,33 @@ __device__ __forceinline__ bool geo_of(int kind, int nx, int ny, int nz, int nve
 // ------------------------------------------------------------------ staging
 // Plane values as FP32 for items of 2 rows x 4 columns.
 template <typename S>
-__device__ __forceinline__ void load_item(const S *__restrict__ src, const Geo &g, int c, int i,
-                                          bool vec, float (&x)[2][4])
+__device__ __forceinline__ void load_item(const S *src_, const Geo &g, int c, int i, bool vec,
+                                          float (&x)[2][4])
 {
+    const gptr<const S> src = glb(src_) + g.base;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
         const int cc = c + r;
-        if (cc >= g.nc) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) x[r][q] = 0.f;
-            continue;
-        }
-        const S *row = src + g.base + cc * g.cs;
+        const bool okc = cc < g.nc;
+        const gptr<const S> row = src + (okc ? (long long)cc * g.cs : 0);
         if (vec && i + 4 <= g.ni) {
+            float y[4];
             if (sizeof(S) == 4) {
-                const float4 v = *reinterpret_cast<const float4 *>(row + i);
-                x[r][0] = v.x; x[r][1] = v.y; x[r][2] = v.z; x[r][3] = v.w;
+                const f32x4 v = *reinterpret_cast<gptr<const f32x4>>(row + i);
+                y[0] = v[0]; y[1] = v[1]; y[2] = v[2]; y[3] = v[3];
             } else {
-                const double2 a = *reinterpret_cast<const double2 *>(row + i);
-                const double2 b = *reinterpret_cast<const double2 *>(row + i + 2);
-                x[r][0] = (float)a.x; x[r][1] = (float)a.y; x[r][2] = (float)b.x; x[r][3] = (float)b.y;
+                const f64x2 a = *reinterpret_cast<gptr<const f64x2>>(row + i);
+                const f64x2 b = *reinterpret_cast<gptr<const f64x2>>(row + i + 2);
+                y[0] = (float)a[0]; y[1] = (float)a[1]; y[2] = (float)b[0]; y[3] = (float)b[1];
             }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[r][q] = okc ? y[q] : 0.f;
         } else {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) x[r][q] = (i + q < g.ni) ? (float)row[i + q] : 0.f;
-        }
-    }
-}
-
-// Split-BF16 plane into LDS: c-major (rows c, pitch npi + 8) and/or i-major
-// (rows i, pitch npc + 8); null pointers skip a layout.
-template <typename S>
-__device__ void stage_bf2(const S *__restrict__ src, const Geo &g, bool vec, uint16_t *Pch,
-                          uint16_t *Pcl, uint16_t *Pth, uint16_t *Ptl)
-{
-    const int nq = g.npi / 4, items = (g.npc / 2) * nq;
-    const int pc = g.npi + 8, pt = g.npc + 8;
-    for (int it = threadIdx.x; it < items; it += kPT) {
-        const int c = 2 * (it / nq), i = 4 * (it - (it / nq) * nq);
-        float x[2][4];
-        load_item(src, g, c, i, vec, x);
-        uint16_t h[2][4], l[2][4];
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) split2(x[r][q], h[r][q], l[r][q]);
-        if (Pch) {
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                u32x2 vh, vl;
-                vh[0] = h[r][0] | ((unsigned)h[r][1] << 16);
-                vh[1] = h[r][2] | ((unsigned)h[r][3] << 16);
-                vl[0] = l[r][0] | ((unsigned)l[r][1] << 16);
-                vl[1] = l[r][2] | ((unsigned)l[r][3] << 16);
-                *reinterpret_cast<u32x2 *>(Pch + (c + r) * pc + i) = vh;
-                *reinterpret_cast<u32x2 *>(Pcl + (c + r) * pc + i) = vl;
-            }
-        }
-        if (Pth) {
-#pragma unroll
             for (int q = 0; q < 4; ++q) {
-                *reinterpret_cast<unsigned *>(Pth + (i + q) * pt + c) = h[0][q] | ((unsigned)h[1][q] << 16);
-                *reinterpret_cast<unsigned *>(Ptl + (i + q) * pt + c) = l[0][q] | ((unsigned)l[1][q] << 16);
+                const bool ok = okc && i + q < g.ni;
+                const float y = (float)row[ok ? i + q : 0];
+                x[r][q] = ok ? y : 0.f;
             }
         }
     }
@@ -192,20 +170,28 @@ __device__ void stage_bf2(const S *__restrict__ src, const Geo &g, bool vec, uin
 
 // FP32/FP64 plane into LDS, c-major, pitch npi + pad.
 template <typename T, typename S>
-__device__ void stage_f(const S *__restrict__ src, const Geo &g, bool vec, T *P, int pitch)
+__device__ void stage_f(const S *src, const Geo &g, bool vec, T *P, int pitch)
 {
     const int nq = g.npi / 4, items = (g.npc / 2) * nq;
     for (int it = threadIdx.x; it < items; it += kPT) {
         const int c = 2 * (it / nq), i = 4 * (it - (it / nq) * nq);
         if (sizeof(S) == 8 && sizeof(T) == 8) {
+            const gptr<const S> sg = glb(src) + g.base;
+            T y[2][4];
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 const int cc = c + r;
-                const S *row = src + g.base + (long long)cc * g.cs;
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    P[cc * pitch + i + q] = (cc < g.nc && i + q < g.ni) ? (T)row[i + q] : T(0);
+                for (int q = 0; q < 4; ++q) {
+                    const bool ok = cc < g.nc && i + q < g.ni;
+                    const T v = (T)sg[ok ? (long long)cc * g.cs + i + q : 0];
+                    y[r][q] = ok ? v : T(0);
+                }
             }
+#pragma unroll
+            for (int r = 0; r < 2; ++r)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) P[(c + r) * pitch + i + q] = y[r][q];
         } else {
             float x[2][4];
             load_item(src, g, c, i, vec, x);
@@ -226,15 +212,188 @@ __device__ __forceinline__ bool vec_ok(const void *p, const Geo &g, int elems_pe
 // f64/f32 pitch: np + 2 doubles (odd number of 16-byte units per row) or np + 4 floats
 template <typename T> __host__ __device__ inline int fpitch(int np) { return sizeof(T) == 8 ? np + 2 : np + 4; }
 
+// ------------------------------------------------------------------ owned tiles (split-BF16 forms)
+// Wave w owns the 16-row tiles tc = w + 4u (u < NCT) of a plane and all of its
+// 16-column tiles v < NIT; in each, lane (l16, lq) holds rows 4 lq + r (r < 4)
+// of column l16 -- the accumulator layout of the f32-accumulating MFMAs.
+// Global loads, LDS staging and global stores follow this ownership, so a
+// wave combines its results with values it loaded itself, and a plane's rows
+// move HBM -> registers -> LDS without a layout change.  A workgroup walks the
+// planes p = blockIdx.x, blockIdx.x + gridDim.x, ... of one job; the loads of
+// the next plane are issued before the current one is contracted.
+template <int NPMAX> struct Own {
+    static constexpr int NIT = NPMAX / 16;            // column tiles
+    static constexpr int NCT = (NPMAX / 16 + 3) / 4;  // row tiles per wave
+    static constexpr int KS = NPMAX / 32;             // k-steps of the bf16 MFMA
+};
+
+// Owned plane values as FP32 (zero outside the n_c x n_i plane).  Every lane
+// loads unconditionally -- out-of-plane elements from the plane's (0, 0) --
+// so the loads issue back to back, and the value is selected afterwards.
+template <typename S, int NCT, int NIT>
+__device__ __forceinline__ void own_load(const S *src_, long long base, const Geo &g,
+                                         float (&x)[NCT][NIT][4])
+{
+    const gptr<const S> src = glb(src_) + base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int u = 0; u < NCT; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = 16 * (wave + 4 * u) + 4 * lq + r;
+#pragma unroll
+            for (int v = 0; v < NIT; ++v) {
+                const int i = 16 * v + l16;
+                const bool ok = c < g.nc && i < g.ni;
+                const float y = (float)src[ok ? (long long)c * g.cs + i : 0];
+                x[u][v][r] = ok ? y : 0.f;
+            }
+        }
+}
+
+// Exchange a dword with the neighbouring lane (DPP quad_perm 1,0,3,2).
+__device__ __forceinline__ unsigned swap_adj(unsigned x)
+{
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+}
+
+// Split one owned tile column (rows cb .. cb+3 of column i) into the LDS
+// plane: i-major (Pt, pitch pt) and/or c-major (Pc, pitch pc); a null pointer
+// skips a layout.  For the c-major rows the lanes of a column pair (i even,
+// i + 1) meet in one dword: the even lane writes rows cb, cb+1, the odd lane
+// rows cb+2, cb+3.  Every lane of the wave must take part.
+__device__ __forceinline__ void put_tile(const float (&x)[4], int cb, int i, int pt, int pc,
+                                         uint16_t *Pth, uint16_t *Ptl, uint16_t *Pch, uint16_t *Pcl)
+{
+    uint16_t h[4], l[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) split2(x[r], h[r], l[r]);
+    const unsigned h01 = h[0] | ((unsigned)h[1] << 16), h23 = h[2] | ((unsigned)h[3] << 16);
+    const unsigned l01 = l[0] | ((unsigned)l[1] << 16), l23 = l[2] | ((unsigned)l[3] << 16);
+    if (Pth) {
+        u32x2 vh, vl;
+        vh[0] = h01; vh[1] = h23;
+        vl[0] = l01; vl[1] = l23;
+        *reinterpret_cast<u32x2 *>(Pth + i * pt + cb) = vh;
+        *reinterpret_cast<u32x2 *>(Ptl + i * pt + cb) = vl;
+    }
+    if (Pch) {
+        const bool odd = i & 1;
+        const unsigned hr = swap_adj(odd ? h01 : h23), lr = swap_adj(odd ? l01 : l23);
+        const unsigned hm = odd ? h23 : h01, lm = odd ? l23 : l01;
+        const unsigned hlo = odd ? hr : hm, hhi = odd ? hm : hr;  // columns i & ~1, i | 1
+        const unsigned llo = odd ? lr : lm, lhi = odd ? lm : lr;
+        const int row = cb + (odd ? 2 : 0), col = i & ~1;
+        *reinterpret_cast<unsigned *>(Pch + row * pc + col) = (hlo & 0xffffu) | (hhi << 16);
+        *reinterpret_cast<unsigned *>(Pch + (row + 1) * pc + col) = (hlo >> 16) | (hhi & 0xffff0000u);
+        *reinterpret_cast<unsigned *>(Pcl + row * pc + col) = (llo & 0xffffu) | (lhi << 16);
+        *reinterpret_cast<unsigned *>(Pcl + (row + 1) * pc + col) = (llo >> 16) | (lhi & 0xffff0000u);
+    }
+}
+
+template <int NCT, int NIT>
+__device__ __forceinline__ void own_stage(const float (&x)[NCT][NIT][4], const Geo &g,
+                                          uint16_t *Pth, uint16_t *Ptl, uint16_t *Pch, uint16_t *Pcl)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int ntc = g.npc / 16, nti = g.npi / 16, pt = g.npc + 8, pc = g.npi + 8;
+#pragma unroll
+    for (int u = 0; u < NCT; ++u) {
+        const int tc = wave + 4 * u;
+        if (tc >= ntc) break;
+#pragma unroll
+        for (int v = 0; v < NIT; ++v) {
+            if (v >= nti) break;
+            put_tile(x[u][v], 16 * tc + 4 * lq, 16 * v + l16, pt, pc, Pth, Ptl, Pch, Pcl);
+        }
+    }
+}
+
+__device__ __forceinline__ bf16x8 bf_zero()
+{
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    return __builtin_bit_cast(bf16x8, z);
+}
+
+// A-form fragments of a row-major split matrix (leading dimension ld) for the
+// wave's own tile rows: rows 16 tc + l16, columns 32 s + 8 lq .. + 7.
+template <int NCT, int KS>
+__device__ __forceinline__ void afrag(const uint16_t *Mh, const uint16_t *Ml, int ld, int ntc, int ks,
+                                      bf16x8 (&ah)[NCT][KS], bf16x8 (&al)[NCT][KS])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int u = 0; u < NCT; ++u)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int tc = wave + 4 * u;
+            if (tc < ntc && s < ks) {
+                ah[u][s] = ld8(glb(Mh) + (16 * tc + l16) * ld + 32 * s + 8 * lq);
+                al[u][s] = ld8(glb(Ml) + (16 * tc + l16) * ld + 32 * s + 8 * lq);
+            } else {
+                ah[u][s] = bf_zero();
+                al[u][s] = bf_zero();
+            }
+        }
+}
+
+// D(c, i) += sum_k A(c, k) B(k, i) on the column-tile pair (i0, i0 + 16) of
+// one row tile, split-BF16 (hi*hi + hi*lo + lo*hi).  A: held fragments.
+// B(k, i) = Bm[i][k] with Bm row-major of pitch ld: the i-major LDS plane for
+// a contraction along c, or the B-form matrix for one along i.
+template <int KS, typename BP>
+__device__ __forceinline__ void mm_pair(const bf16x8 (&ah)[KS], const bf16x8 (&al)[KS], BP Bh, BP Bl,
+                                        int ld, int ks, int i0, f32x4 &a0, f32x4 &a1)
+{
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lq = lane >> 4;
+    const int o0 = (i0 + l16) * ld + 8 * lq, o1 = o0 + 16 * ld;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        if (s >= ks) break;
+        const bf16x8 b0h = ld8(Bh + o0 + 32 * s), b0l = ld8(Bl + o0 + 32 * s);
+        const bf16x8 b1h = ld8(Bh + o1 + 32 * s), b1l = ld8(Bl + o1 + 32 * s);
+        a0 = mma_bf(ah[s], b0h, a0);
+        a1 = mma_bf(ah[s], b1h, a1);
+        a0 = mma_bf(ah[s], b0l, a0);
+        a1 = mma_bf(ah[s], b1l, a1);
+        a0 = mma_bf(al[s], b0h, a0);
+        a1 = mma_bf(al[s], b1h, a1);
+    }
+}
+
+// A-form fragments of the wave's own rows of the c-major LDS plane.
+template <int KS>
+__device__ __forceinline__ void afrag_lds(const uint16_t *Pch, const uint16_t *Pcl, int pc, int c0, int ks,
+                                          bf16x8 (&ah)[KS], bf16x8 (&al)[KS])
+{
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        if (s >= ks) break;
+        ah[s] = ld8(Pch + (c0 + l16) * pc + 32 * s + 8 * lq);
+        al[s] = ld8(Pcl + (c0 + l16) * pc + 32 * s + 8 * lq);
+    }
+}
+
 // ------------------------------------------------------------------ H*X, split-BF16 form
-template <int PASS, int NPMAX>
+// Pass A (plane (i,j) at k): Y = V o X + Ty X along c = j; the accumulators
+// start from V o X.  Pass B (plane (i,k) at j): Y += Tz X along c = k + X Tx
+// along i; the wave's Y values of the next plane are loaded as soon as those
+// of the current plane have been used.  LDS: the i-major plane (pass A), plus
+// the c-major plane (pass B).
+template <int PASS, int NPMAX, bool MULTI>
 __global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ jobs)
 {
+    constexpr int NCT = Own<NPMAX>::NCT, NIT = Own<NPMAX>::NIT, KS = Own<NPMAX>::KS;
     const HxJob &J = jobs[blockIdx.y];
     const AxisMats &Mi = J.ax[0];
     const AxisMats &Mc = J.ax[PASS == 0 ? 1 : 2];
+    int p = blockIdx.x;
     Geo g;
-    if (!geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, blockIdx.x, g)) return;
+    if (!geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, p, g)) return;
     extern __shared__ __align__(16) unsigned char smem[];
     const int pt = g.npc + 8, pc = g.npi + 8;
     uint16_t *Pth = reinterpret_cast<uint16_t *>(smem);
@@ -242,82 +401,71 @@ __global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ j
     uint16_t *Pch = Ptl + g.npi * pt;
     uint16_t *Pcl = Pch + g.npc * pc;
     const float *X = static_cast<const float *>(J.X);
-    float *Y = static_cast<float *>(J.Y);
-    const bool vec = vec_ok(X, g, 4);
-    stage_bf2(X, g, vec, PASS == 1 ? Pch : nullptr, Pcl, Pth, Ptl);
-    __syncthreads();
     const float *V = static_cast<const float *>(J.V);
+    const gptr<float> Y = glb(static_cast<float *>(J.Y));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int ntc = g.npc / 16, nti = g.npi / 16, ksc = g.npc / 32, ksi = g.npi / 32;
-    constexpr int KS = NPMAX / 32;
-    for (int tc = wave; tc < ntc; tc += 4) {
-        const int c0 = 16 * tc;
-        bf16x8 ach[KS], acl[KS], aih[KS], ail[KS];
+    bf16x8 ach[NCT][KS], acl[NCT][KS];
+    afrag(Mc.th, Mc.tl, g.npc, ntc, ksc, ach, acl);
+    float xb[NCT][NIT][4], wb[NCT][NIT][4];  // X; V (pass A) or Y (pass B)
+    own_load(X, g.base, g, xb);
+    own_load(PASS == 0 ? V : static_cast<const float *>(J.Y), PASS == 0 ? g.vbase : g.base, g, wb);
+    for (;;) {
+        own_stage(xb, g, Pth, Ptl, PASS == 1 ? Pch : nullptr, Pcl);
+        if (PASS == 0) {
 #pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            if (s >= ksc) break;
-            ach[s] = ld8(Mc.th + (c0 + l16) * g.npc + 32 * s + 8 * lq);
-            acl[s] = ld8(Mc.tl + (c0 + l16) * g.npc + 32 * s + 8 * lq);
+            for (int u = 0; u < NCT; ++u)
+#pragma unroll
+                for (int v = 0; v < NIT; ++v)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) wb[u][v][r] *= xb[u][v][r];
         }
-        if (PASS == 1) {
+        __syncthreads();
+        Geo gn;
+        const bool more = MULTI && geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, p + gridDim.x, gn);
+        if (more) own_load(X, gn.base, gn, xb);
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                if (s >= ksi) break;
-                aih[s] = ld8(Pch + (c0 + l16) * pc + 32 * s + 8 * lq);
-                ail[s] = ld8(Pcl + (c0 + l16) * pc + 32 * s + 8 * lq);
-            }
-        }
-        for (int ti = 0; ti < nti; ti += 2) {
-            const int i0 = 16 * ti, i1 = i0 + 16;
-            f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
+        for (int u = 0; u < NCT; ++u) {
+            const int tc = wave + 4 * u;
+            if (tc >= ntc) break;
+            const int c0 = 16 * tc;
+            bf16x8 aih[KS], ail[KS];
+            if (PASS == 1) afrag_lds(Pch, Pcl, pc, c0, ksi, aih, ail);
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                if (s >= ksc) break;
-                const bf16x8 b0h = ld8(Pth + (i0 + l16) * pt + 32 * s + 8 * lq);
-                const bf16x8 b0l = ld8(Ptl + (i0 + l16) * pt + 32 * s + 8 * lq);
-                const bf16x8 b1h = ld8(Pth + (i1 + l16) * pt + 32 * s + 8 * lq);
-                const bf16x8 b1l = ld8(Ptl + (i1 + l16) * pt + 32 * s + 8 * lq);
-                a0 = mma_bf(ach[s], b0h, a0);
-                a1 = mma_bf(ach[s], b1h, a1);
-                a0 = mma_bf(ach[s], b0l, a0);
-                a1 = mma_bf(ach[s], b1l, a1);
-                a0 = mma_bf(acl[s], b0h, a0);
-                a1 = mma_bf(acl[s], b1h, a1);
-            }
-            if (PASS == 1) {
+            for (int v = 0; v < NIT; v += 2) {
+                if (v >= nti) break;
+                f32x4 a0, a1;
 #pragma unroll
-                for (int s = 0; s < KS; ++s) {
-                    if (s >= ksi) break;
-                    // B[k][n] = Tx[k][i0 + n] = Tx[i0 + n][k] (symmetric)
-                    const bf16x8 b0h = ld8(Mi.th + (i0 + l16) * g.npi + 32 * s + 8 * lq);
-                    const bf16x8 b0l = ld8(Mi.tl + (i0 + l16) * g.npi + 32 * s + 8 * lq);
-                    const bf16x8 b1h = ld8(Mi.th + (i1 + l16) * g.npi + 32 * s + 8 * lq);
-                    const bf16x8 b1l = ld8(Mi.tl + (i1 + l16) * g.npi + 32 * s + 8 * lq);
-                    a0 = mma_bf(aih[s], b0h, a0);
-                    a1 = mma_bf(aih[s], b1h, a1);
-                    a0 = mma_bf(aih[s], b0l, a0);
-                    a1 = mma_bf(aih[s], b1l, a1);
-                    a0 = mma_bf(ail[s], b0h, a0);
-                    a1 = mma_bf(ail[s], b1h, a1);
+                for (int r = 0; r < 4; ++r) {
+                    a0[r] = PASS == 0 ? wb[u][v][r] : 0.f;
+                    a1[r] = PASS == 0 ? wb[u][v + 1][r] : 0.f;
                 }
-            }
+                mm_pair(ach[u], acl[u], Pth, Ptl, pt, ksc, 16 * v, a0, a1);
+                // B(k, i) = Tx[k][i] = Tx[i][k] (symmetric)
+                if (PASS == 1) mm_pair(aih, ail, glb(Mi.th), glb(Mi.tl), g.npi, ksi, 16 * v, a0, a1);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int c = c0 + 4 * lq + r;
-                if (c >= g.nc) continue;
-                const long long rowg = g.base + (long long)c * g.cs;
-                const int ia = i0 + l16, ib = i1 + l16;
-                if (PASS == 0) {
-                    const long long rowv = g.vbase + (long long)c * g.cs;
-                    if (ia < g.ni) Y[rowg + ia] = V[rowv + ia] * X[rowg + ia] + a0[r];
-                    if (ib < g.ni) Y[rowg + ib] = V[rowv + ib] * X[rowg + ib] + a1[r];
-                } else {
-                    if (ia < g.ni) Y[rowg + ia] += a0[r];
-                    if (ib < g.ni) Y[rowg + ib] += a1[r];
+                for (int r = 0; r < 4; ++r) {
+                    const int c = c0 + 4 * lq + r;
+                    if (c >= g.nc) continue;
+                    const long long row = (long long)c * g.cs;
+                    const int ia = 16 * v + l16, ib = ia + 16;
+                    if (ia < g.ni) Y[g.base + row + ia] = PASS == 0 ? a0[r] : wb[u][v][r] + a0[r];
+                    if (ib < g.ni) Y[g.base + row + ib] = PASS == 0 ? a1[r] : wb[u][v + 1][r] + a1[r];
+                    if (more) {
+                        const gptr<const float> W =
+                            PASS == 0 ? glb(V) + gn.vbase + row : (gptr<const float>)Y + gn.base + row;
+                        const float wa = W[ia < g.ni ? ia : 0], wv = W[ib < g.ni ? ib : 0];
+                        wb[u][v][r] = ia < g.ni ? wa : 0.f;
+                        wb[u][v + 1][r] = ib < g.ni ? wv : 0.f;
+                    }
                 }
             }
         }
+        if (!more) break;
+        __syncthreads();  // every wave is done with this plane's LDS copy
+        g = gn;
+        p += gridDim.x;
     }
 }
 
@@ -339,11 +487,11 @@ __global__ void __launch_bounds__(kPT) hx_f_kernel(const HxJob *__restrict__ job
     const int pitch = fpitch<T>(g.npi);
     T *P = reinterpret_cast<T *>(smem);
     const T *X = static_cast<const T *>(J.X);
-    T *Y = static_cast<T *>(J.Y);
+    const gptr<T> Y = glb(static_cast<T *>(J.Y));
     stage_f<T, T>(X, g, vec_ok(X, g, 16 / (int)sizeof(T)), P, pitch);
     __syncthreads();
-    const T *V = static_cast<const T *>(J.V);
-    const T *Tc = tmat<T>(Mc), *Ti = tmat<T>(Mi);
+    const gptr<const T> V = glb(static_cast<const T *>(J.V));
+    const gptr<const T> Tc = glb(tmat<T>(Mc)), Ti = glb(tmat<T>(Mi));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int ntc = g.npc / 16, nti = g.npi / 16;
@@ -353,7 +501,7 @@ __global__ void __launch_bounds__(kPT) hx_f_kernel(const HxJob *__restrict__ job
             const int i0 = 16 * ti, i1 = i0 + 16;
             acc_t a0 = {}, a1 = {};
             // along c: A[m][k] = Tc[c0 + m][k], B[k][n] = P[k][i + n]
-            const T *tcr = Tc + (c0 + l16) * g.npc + lq;
+            const gptr<const T> tcr = Tc + (c0 + l16) * g.npc + lq;
 #pragma unroll 4
             for (int k0 = 0; k0 < g.npc; k0 += 4) {
                 const T a = tcr[k0];
@@ -367,7 +515,7 @@ __global__ void __launch_bounds__(kPT) hx_f_kernel(const HxJob *__restrict__ job
 #pragma unroll 4
                 for (int k0 = 0; k0 < g.npi; k0 += 4) {
                     const T a = prow[k0];
-                    const T *tr = Ti + (k0 + lq) * g.npi + l16;
+                    const gptr<const T> tr = Ti + (k0 + lq) * g.npi + l16;
                     a0 = mma(a, tr[i0], a0);
                     a1 = mma(a, tr[i1], a1);
                 }
@@ -396,8 +544,9 @@ __global__ void __launch_bounds__(kPT) hx_f_kernel(const HxJob *__restrict__ job
 // along c with Mc (row-major), or along i with Mi (row-major).
 template <int NCT, int NIT>
 __device__ __forceinline__ void f64_contract(const double *P, int pitch, const Geo &g,
-                                             const double *M, bool along_c, f64x4 (&acc)[NCT][NIT])
+                                             const double *M_, bool along_c, f64x4 (&acc)[NCT][NIT])
 {
+    const gptr<const double> M = glb(M_);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int ntc = g.npc / 16, nti = g.npi / 16;
@@ -412,7 +561,7 @@ __device__ __forceinline__ void f64_contract(const double *P, int pitch, const G
             const int i0 = 16 * v, i1 = i0 + 16;
             f64x4 a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
             if (along_c) {
-                const double *mr = M + (c0 + l16) * g.npc + lq;
+                const gptr<const double> mr = M + (c0 + l16) * g.npc + lq;
 #pragma unroll 4
                 for (int k0 = 0; k0 < g.npc; k0 += 4) {
                     const double a = mr[k0];
@@ -425,7 +574,7 @@ __device__ __forceinline__ void f64_contract(const double *P, int pitch, const G
 #pragma unroll 4
                 for (int k0 = 0; k0 < g.npi; k0 += 4) {
                     const double a = prow[k0];
-                    const double *mr = M + (k0 + lq) * g.npi + l16;
+                    const gptr<const double> mr = M + (k0 + lq) * g.npi + l16;
                     a0 = mma(a, mr[i0], a0);
                     a1 = mma(a, mr[i1], a1);
                 }
@@ -472,8 +621,9 @@ __device__ __forceinline__ void f64_store_lds(double *P, int pitch, const Geo &g
 }
 
 template <typename D, int NCT, int NIT>
-__device__ __forceinline__ void f64_store_global(D *dst, const Geo &g, const f64x4 (&acc)[NCT][NIT])
+__device__ __forceinline__ void f64_store_global(D *dst_, const Geo &g, const f64x4 (&acc)[NCT][NIT])
 {
+    const gptr<D> dst = glb(dst_);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int ntc = g.npc / 16, nti = g.npi / 16;
@@ -523,76 +673,16 @@ __global__ void __launch_bounds__(kPT) fdm_f64_kernel(const FdmTask *__restrict_
 }
 
 // ------------------------------------------------------------------ preconditioner, split-BF16
-// Contract a split-BF16 LDS plane: along c (B from the i-major copy Pt) with
-// the A-form matrix Mc (row-major), or along i (A from the c-major copy Pc)
-// with MiT = Mi^T (row-major).
+// Owned tiles scaled by the preconditioner denominators of plane (i = a,
+// c = z index) at fixed j = g.pl, then split into the i-major plane.
 template <int NCT, int NIT>
-__device__ __forceinline__ void bf2_contract(const uint16_t *Ph, const uint16_t *Pl, const Geo &g,
-                                             const uint16_t *Mh, const uint16_t *Ml, bool along_c,
-                                             f32x4 (&acc)[NCT][NIT])
+__device__ __forceinline__ void put_scaled(const f32x4 (&acc)[NCT][NIT], const Geo &g, const FdmTask &t,
+                                           uint16_t *Pth, uint16_t *Ptl)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
-    const int ntc = g.npc / 16, nti = g.npi / 16;
-    const int pt = g.npc + 8, pc = g.npi + 8;
-#pragma unroll
-    for (int u = 0; u < NCT; ++u) {
-        const int tc = wave + 4 * u;
-        if (tc >= ntc) break;
-        const int c0 = 16 * tc;
-#pragma unroll
-        for (int v = 0; v < NIT; v += 2) {
-            if (v >= nti) break;
-            const int i0 = 16 * v, i1 = i0 + 16;
-            f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
-            if (along_c) {
-                for (int k0 = 0; k0 < g.npc; k0 += 32) {
-                    const bf16x8 ah = ld8(Mh + (c0 + l16) * g.npc + k0 + 8 * lq);
-                    const bf16x8 al = ld8(Ml + (c0 + l16) * g.npc + k0 + 8 * lq);
-                    const bf16x8 b0h = ld8(Ph + (i0 + l16) * pt + k0 + 8 * lq);
-                    const bf16x8 b0l = ld8(Pl + (i0 + l16) * pt + k0 + 8 * lq);
-                    const bf16x8 b1h = ld8(Ph + (i1 + l16) * pt + k0 + 8 * lq);
-                    const bf16x8 b1l = ld8(Pl + (i1 + l16) * pt + k0 + 8 * lq);
-                    a0 = mma_bf(ah, b0h, a0);
-                    a1 = mma_bf(ah, b1h, a1);
-                    a0 = mma_bf(ah, b0l, a0);
-                    a1 = mma_bf(ah, b1l, a1);
-                    a0 = mma_bf(al, b0h, a0);
-                    a1 = mma_bf(al, b1h, a1);
-                }
-            } else {
-                for (int k0 = 0; k0 < g.npi; k0 += 32) {
-                    const bf16x8 ah = ld8(Ph + (c0 + l16) * pc + k0 + 8 * lq);
-                    const bf16x8 al = ld8(Pl + (c0 + l16) * pc + k0 + 8 * lq);
-                    const bf16x8 b0h = ld8(Mh + (i0 + l16) * g.npi + k0 + 8 * lq);
-                    const bf16x8 b0l = ld8(Ml + (i0 + l16) * g.npi + k0 + 8 * lq);
-                    const bf16x8 b1h = ld8(Mh + (i1 + l16) * g.npi + k0 + 8 * lq);
-                    const bf16x8 b1l = ld8(Ml + (i1 + l16) * g.npi + k0 + 8 * lq);
-                    a0 = mma_bf(ah, b0h, a0);
-                    a1 = mma_bf(ah, b1h, a1);
-                    a0 = mma_bf(ah, b0l, a0);
-                    a1 = mma_bf(ah, b1l, a1);
-                    a0 = mma_bf(al, b0h, a0);
-                    a1 = mma_bf(al, b1h, a1);
-                }
-            }
-            acc[u][v] = a0;
-            acc[u][v + 1] = a1;
-        }
-    }
-}
-
-// Write the tiles (optionally scaled) back into LDS as split BF16, i-major
-// (for a following contraction along c) or c-major (along i).
-template <int NCT, int NIT>
-__device__ __forceinline__ void bf2_store_lds(uint16_t *Ph, uint16_t *Pl, const Geo &g,
-                                              const f32x4 (&acc)[NCT][NIT], bool imajor,
-                                              const FdmTask *t)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int l16 = lane & 15, lq = lane >> 4;
-    const int ntc = g.npc / 16, nti = g.npi / 16;
-    const int pt = g.npc + 8, pc = g.npi + 8;
+    const int ntc = g.npc / 16, nti = g.npi / 16, pt = g.npc + 8;
+    const double ly = t.ax[1].lam[g.pl] - t.shift;
 #pragma unroll
     for (int u = 0; u < NCT; ++u) {
         const int tc = wave + 4 * u;
@@ -601,43 +691,40 @@ __device__ __forceinline__ void bf2_store_lds(uint16_t *Ph, uint16_t *Pl, const 
         for (int v = 0; v < NIT; ++v) {
             if (v >= nti) break;
             const int i = 16 * v + l16, cb = 16 * tc + 4 * lq;
-            uint16_t h[4], l[4];
+            float x[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                float x = acc[u][v][r];
-                if (t) {
-                    const int c = cb + r;
-                    double den = 1.0;
-                    if (i < g.ni && c < g.nc) {
-                        den = t->ax[0].lam[i] + t->ax[1].lam[g.pl] + t->ax[2].lam[c] - t->shift;
-                        if (fabs(den) < t->floor) den = den < 0 ? -t->floor : t->floor;
-                    }
-                    x = (float)((double)x * (t->scale / den));
+                const int c = cb + r;
+                double den = 1.0;
+                if (i < g.ni && c < g.nc) {
+                    den = t.ax[0].lam[i] + ly + t.ax[2].lam[c];
+                    if (fabs(den) < t.floor) den = den < 0 ? -t.floor : t.floor;
                 }
-                split2(x, h[r], l[r]);
+                x[r] = (float)((double)acc[u][v][r] * (t.scale / den));
             }
-            if (imajor) {
-                u32x2 vh, vl;
-                vh[0] = h[0] | ((unsigned)h[1] << 16);
-                vh[1] = h[2] | ((unsigned)h[3] << 16);
-                vl[0] = l[0] | ((unsigned)l[1] << 16);
-                vl[1] = l[2] | ((unsigned)l[3] << 16);
-                *reinterpret_cast<u32x2 *>(Ph + i * pt + cb) = vh;
-                *reinterpret_cast<u32x2 *>(Pl + i * pt + cb) = vl;
-            } else {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    Ph[(cb + r) * pc + i] = h[r];
-                    Pl[(cb + r) * pc + i] = l[r];
-                }
-            }
+            put_tile(x, cb, i, pt, 0, Pth, Ptl, nullptr, nullptr);
         }
     }
 }
 
-template <typename D, int NCT, int NIT>
-__device__ __forceinline__ void bf2_store_global(D *dst, const Geo &g, const f32x4 (&acc)[NCT][NIT])
+template <int NCT, int NIT>
+__device__ __forceinline__ void put_acc(const f32x4 (&acc)[NCT][NIT], const Geo &g, uint16_t *Pth,
+                                        uint16_t *Ptl, uint16_t *Pch, uint16_t *Pcl)
 {
+    float x[NCT][NIT][4];
+#pragma unroll
+    for (int u = 0; u < NCT; ++u)
+#pragma unroll
+        for (int v = 0; v < NIT; ++v)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) x[u][v][r] = acc[u][v][r];
+    own_stage(x, g, Pth, Ptl, Pch, Pcl);
+}
+
+template <typename D, int NCT, int NIT>
+__device__ __forceinline__ void bf2_store_global(D *dst_, const Geo &g, const f32x4 (&acc)[NCT][NIT])
+{
+    const gptr<D> dst = glb(dst_);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int ntc = g.npc / 16, nti = g.npi / 16;
@@ -657,55 +744,124 @@ __device__ __forceinline__ void bf2_store_global(D *dst, const Geo &g, const f32
     }
 }
 
-template <int PASS, int NPMAX>
+// All owned tiles of one contraction: along c (A = held fragments of the
+// along-c matrix, B = i-major plane) or along i (A = own rows of the c-major
+// plane, B = B-form matrix from global memory).
+template <int NCT, int NIT, int KS>
+__device__ __forceinline__ void mm_c_all(const bf16x8 (&ah)[NCT][KS], const bf16x8 (&al)[NCT][KS],
+                                         const uint16_t *Pth, const uint16_t *Ptl, const Geo &g,
+                                         f32x4 (&acc)[NCT][NIT])
+{
+    const int wave = threadIdx.x >> 6;
+    const int ntc = g.npc / 16, nti = g.npi / 16, ksc = g.npc / 32, pt = g.npc + 8;
+#pragma unroll
+    for (int u = 0; u < NCT; ++u) {
+        if (wave + 4 * u >= ntc) break;
+#pragma unroll
+        for (int v = 0; v < NIT; v += 2) {
+            if (v >= nti) break;
+            f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
+            mm_pair(ah[u], al[u], Pth, Ptl, pt, ksc, 16 * v, a0, a1);
+            acc[u][v] = a0;
+            acc[u][v + 1] = a1;
+        }
+    }
+}
+
+template <int NCT, int NIT, int KS>
+__device__ __forceinline__ void mm_i_all(const uint16_t *Pch, const uint16_t *Pcl, const uint16_t *Mh,
+                                         const uint16_t *Ml, const Geo &g, f32x4 (&acc)[NCT][NIT])
+{
+    const int wave = threadIdx.x >> 6;
+    const int ntc = g.npc / 16, nti = g.npi / 16, ksi = g.npi / 32, pc = g.npi + 8;
+#pragma unroll
+    for (int u = 0; u < NCT; ++u) {
+        const int tc = wave + 4 * u;
+        if (tc >= ntc) break;
+        bf16x8 ah[KS], al[KS];
+        afrag_lds(Pch, Pcl, pc, 16 * tc, ksi, ah, al);
+#pragma unroll
+        for (int v = 0; v < NIT; v += 2) {
+            if (v >= nti) break;
+            f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
+            mm_pair(ah, al, glb(Mh), glb(Ml), g.npi, ksi, 16 * v, a0, a1);
+            acc[u][v] = a0;
+            acc[u][v + 1] = a1;
+        }
+    }
+}
+
+// Three passes, each walking planes like the H*X kernels:
+//   0, plane (i,j) at k: r -> c-major plane; X Qx along i -> i-major plane;
+//      Qy^T (.) along c -> work
+//   1, plane (i,k) at j: work -> i-major plane A; Qz^T (.) along c, scaled ->
+//      i-major plane B; Qz (.) along c -> work
+//   2, plane (i,j) at k: work -> i-major plane; Qy (.) along c -> c-major
+//      plane; (.) Qx^T along i -> t
+// The c-major plane is only read by the wave that wrote its rows.
+template <int PASS, int NPMAX, bool MULTI>
 __global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict__ tasks)
 {
-    constexpr int NIT = NPMAX / 16, NCT = (NPMAX / 16 + 3) / 4;
+    constexpr int NCT = Own<NPMAX>::NCT, NIT = Own<NPMAX>::NIT, KS = Own<NPMAX>::KS;
     const FdmTask &t = tasks[blockIdx.y];
-    Geo g;
     const int kind = PASS == 1 ? 1 : 0;
     const AxisMats &Mx = t.ax[0], &My = t.ax[1], &Mz = t.ax[2];
     const AxisMats &Mc = PASS == 1 ? Mz : My;
-    if (!geo_of(kind, Mx.n, My.n, Mz.n, 1, 0, Mx.np, Mc.np, blockIdx.x, g)) return;
+    int p = blockIdx.x;
+    Geo g;
+    if (!geo_of(kind, Mx.n, My.n, Mz.n, 1, 0, Mx.np, Mc.np, p, g)) return;
     extern __shared__ __align__(16) unsigned char smem[];
     const int pt = g.npc + 8, pc = g.npi + 8;
     uint16_t *Pth = reinterpret_cast<uint16_t *>(smem);
     uint16_t *Ptl = Pth + g.npi * pt;
-    uint16_t *Pch = Ptl + g.npi * pt;
-    uint16_t *Pcl = Pch + g.npc * pc;
+    uint16_t *P2h = Ptl + g.npi * pt;  // c-major plane (passes 0, 2) or i-major plane B (pass 1)
+    uint16_t *P2l = P2h + (PASS == 1 ? g.npi * pt : g.npc * pc);
+    const int ntc = g.npc / 16, ksc = g.npc / 32;
+    bf16x8 ah[NCT][KS], al[NCT][KS], bh[NCT][KS], bl[NCT][KS];
+    if (PASS == 0) afrag(My.qth, My.qtl, g.npc, ntc, ksc, ah, al);       // Qy^T
+    if (PASS == 1) {
+        afrag(Mz.qth, Mz.qtl, g.npc, ntc, ksc, ah, al);                  // Qz^T
+        afrag(Mz.qh, Mz.ql, g.npc, ntc, ksc, bh, bl);                    // Qz
+    }
+    if (PASS == 2) afrag(My.qh, My.ql, g.npc, ntc, ksc, ah, al);         // Qy
+    const void *src = PASS == 0 ? t.src : t.work;
+    const bool s32 = PASS != 0 || t.src32;
+    float xb[NCT][NIT][4];
+    if (s32) own_load(static_cast<const float *>(src), g.base, g, xb);
+    else own_load(static_cast<const double *>(src), g.base, g, xb);
     f32x4 acc[NCT][NIT];
-    if (PASS == 0) {
-        if (t.src32) {
-            const float *src = static_cast<const float *>(t.src);
-            stage_bf2(src, g, vec_ok(src, g, 4), Pch, Pcl, nullptr, nullptr);
-        } else {
-            const double *src = static_cast<const double *>(t.src);
-            stage_bf2(src, g, vec_ok(src, g, 2), Pch, Pcl, nullptr, nullptr);
+    for (;;) {
+        own_stage(xb, g, PASS == 0 ? nullptr : Pth, Ptl, PASS == 0 ? P2h : nullptr, P2l);
+        __syncthreads();
+        Geo gn;
+        const bool more = MULTI && geo_of(kind, Mx.n, My.n, Mz.n, 1, 0, Mx.np, Mc.np, p + gridDim.x, gn);
+        if (more) {
+            if (s32) own_load(static_cast<const float *>(src), gn.base, gn, xb);
+            else own_load(static_cast<const double *>(src), gn.base, gn, xb);
         }
-        __syncthreads();
-        bf2_contract(Pch, Pcl, g, Mx.qth, Mx.qtl, false, acc);  // P Qx   (B-form: Qx^T)
-        bf2_store_lds(Pth, Ptl, g, acc, true, nullptr);
-        __syncthreads();
-        bf2_contract(Pth, Ptl, g, My.qth, My.qtl, true, acc);   // Qy^T (.)
-        bf2_store_global(t.work, g, acc);
-    } else if (PASS == 1) {
-        stage_bf2(t.work, g, vec_ok(t.work, g, 4), nullptr, nullptr, Pth, Ptl);
-        __syncthreads();
-        bf2_contract(Pth, Ptl, g, Mz.qth, Mz.qtl, true, acc);   // Qz^T P
-        __syncthreads();
-        bf2_store_lds(Pth, Ptl, g, acc, true, &t);               // scaled
-        __syncthreads();
-        bf2_contract(Pth, Ptl, g, Mz.qh, Mz.ql, true, acc);     // Qz (.)
-        bf2_store_global(t.work, g, acc);
-    } else {
-        stage_bf2(t.work, g, vec_ok(t.work, g, 4), nullptr, nullptr, Pth, Ptl);
-        __syncthreads();
-        bf2_contract(Pth, Ptl, g, My.qh, My.ql, true, acc);     // Qy P
-        bf2_store_lds(Pch, Pcl, g, acc, false, nullptr);
-        __syncthreads();
-        bf2_contract(Pch, Pcl, g, Mx.qh, Mx.ql, false, acc);    // (.) Qx^T  (B-form: Qx)
-        if (t.dst32) bf2_store_global(static_cast<float *>(t.dst), g, acc);
-        else bf2_store_global(static_cast<double *>(t.dst), g, acc);
+        if (PASS == 0) {
+            mm_i_all<NCT, NIT, KS>(P2h, P2l, Mx.qth, Mx.qtl, g, acc);   // X Qx (B form: Qx^T)
+            put_acc(acc, g, Pth, Ptl, nullptr, nullptr);
+            __syncthreads();
+            mm_c_all(ah, al, Pth, Ptl, g, acc);                            // Qy^T (.)
+            bf2_store_global(t.work, g, acc);
+        } else if (PASS == 1) {
+            mm_c_all(ah, al, Pth, Ptl, g, acc);                            // Qz^T X
+            put_scaled(acc, g, t, P2h, P2l);
+            __syncthreads();
+            mm_c_all(bh, bl, P2h, P2l, g, acc);                            // Qz (.)
+            bf2_store_global(t.work, g, acc);
+        } else {
+            mm_c_all(ah, al, Pth, Ptl, g, acc);                            // Qy X
+            put_acc(acc, g, nullptr, nullptr, P2h, P2l);
+            __syncthreads();
+            mm_i_all<NCT, NIT, KS>(P2h, P2l, Mx.qh, Mx.ql, g, acc);     // (.) Qx^T (B form: Qx)
+            if (t.dst32) bf2_store_global(static_cast<float *>(t.dst), g, acc);
+            else bf2_store_global(static_cast<double *>(t.dst), g, acc);
+        }
+        if (!more) break;
+        g = gn;
+        p += gridDim.x;
     }
 }
 
@@ -725,14 +881,19 @@ void init_np()
 {
     static std::once_flag once;
     std::call_once(once, [] {
-        allow_lds(hx_bf2_kernel<0, NPMAX>);
-        allow_lds(hx_bf2_kernel<1, NPMAX>);
+        allow_lds(hx_bf2_kernel<0, NPMAX, false>);
+        allow_lds(hx_bf2_kernel<1, NPMAX, false>);
+        allow_lds(hx_bf2_kernel<0, NPMAX, true>);
+        allow_lds(hx_bf2_kernel<1, NPMAX, true>);
         allow_lds(fdm_f64_kernel<0, NPMAX>);
         allow_lds(fdm_f64_kernel<1, NPMAX>);
         allow_lds(fdm_f64_kernel<2, NPMAX>);
-        allow_lds(fdm_bf2_kernel<0, NPMAX>);
-        allow_lds(fdm_bf2_kernel<1, NPMAX>);
-        allow_lds(fdm_bf2_kernel<2, NPMAX>);
+        allow_lds(fdm_bf2_kernel<0, NPMAX, false>);
+        allow_lds(fdm_bf2_kernel<1, NPMAX, false>);
+        allow_lds(fdm_bf2_kernel<2, NPMAX, false>);
+        allow_lds(fdm_bf2_kernel<0, NPMAX, true>);
+        allow_lds(fdm_bf2_kernel<1, NPMAX, true>);
+        allow_lds(fdm_bf2_kernel<2, NPMAX, true>);
     });
 }
 
@@ -747,13 +908,30 @@ void init_f()
     });
 }
 
+// Planes per workgroup of the split-BF16 kernels (measured at 64^3 with
+// 8 - 160 vectors): H*X runs fastest with one plane per workgroup, the
+// preconditioner with two.  PISCES_PPW_HX / PISCES_PPW_FDM override.
+int ppw_of(const char *env, int def)
+{
+    const char *e = std::getenv(env);
+    const int v = e ? std::atoi(e) : 0;
+    return v > 0 ? v : def;
+}
+
 template <int NPMAX>
 void hx_bf2_launch(const HxJob *d, int n, int pa, int pb, int np, hipStream_t s)
 {
     init_np<NPMAX>();
-    // pass A uses only the i-major copy (first half of the layout)
-    hipLaunchKernelGGL((hx_bf2_kernel<0, NPMAX>), dim3(pa, n), dim3(kPT), bf2_lds(np) / 2, s, d);
-    hipLaunchKernelGGL((hx_bf2_kernel<1, NPMAX>), dim3(pb, n), dim3(kPT), bf2_lds(np), s, d);
+    static const int ppw = ppw_of("PISCES_PPW_HX", 1);
+    const dim3 ga((pa + ppw - 1) / ppw, n), gb((pb + ppw - 1) / ppw, n);
+    // pass A uses only the i-major plane (first half of the layout)
+    if (ppw == 1) {
+        hipLaunchKernelGGL((hx_bf2_kernel<0, NPMAX, false>), ga, dim3(kPT), bf2_lds(np) / 2, s, d);
+        hipLaunchKernelGGL((hx_bf2_kernel<1, NPMAX, false>), gb, dim3(kPT), bf2_lds(np), s, d);
+    } else {
+        hipLaunchKernelGGL((hx_bf2_kernel<0, NPMAX, true>), ga, dim3(kPT), bf2_lds(np) / 2, s, d);
+        hipLaunchKernelGGL((hx_bf2_kernel<1, NPMAX, true>), gb, dim3(kPT), bf2_lds(np), s, d);
+    }
 }
 
 template <int NPMAX>
@@ -767,10 +945,17 @@ void fdm_np_launch(const FdmTask *d, int n, int nplanes, int np, int precision, 
         hipLaunchKernelGGL((fdm_f64_kernel<2, NPMAX>), dim3(nplanes, n), dim3(kPT), lds, s, d);
     } else {
         const size_t lds = bf2_lds(np);
-        hipLaunchKernelGGL((fdm_bf2_kernel<0, NPMAX>), dim3(nplanes, n), dim3(kPT), lds, s, d);
-        // pass 1 uses only the i-major copy (first half of the layout)
-        hipLaunchKernelGGL((fdm_bf2_kernel<1, NPMAX>), dim3(nplanes, n), dim3(kPT), lds / 2, s, d);
-        hipLaunchKernelGGL((fdm_bf2_kernel<2, NPMAX>), dim3(nplanes, n), dim3(kPT), lds, s, d);
+        static const int ppw = ppw_of("PISCES_PPW_FDM", 2);
+        const dim3 grid((nplanes + ppw - 1) / ppw, n);
+        if (ppw == 1) {
+            hipLaunchKernelGGL((fdm_bf2_kernel<0, NPMAX, false>), grid, dim3(kPT), lds, s, d);
+            hipLaunchKernelGGL((fdm_bf2_kernel<1, NPMAX, false>), grid, dim3(kPT), lds, s, d);
+            hipLaunchKernelGGL((fdm_bf2_kernel<2, NPMAX, false>), grid, dim3(kPT), lds, s, d);
+        } else {
+            hipLaunchKernelGGL((fdm_bf2_kernel<0, NPMAX, true>), grid, dim3(kPT), lds, s, d);
+            hipLaunchKernelGGL((fdm_bf2_kernel<1, NPMAX, true>), grid, dim3(kPT), lds, s, d);
+            hipLaunchKernelGGL((fdm_bf2_kernel<2, NPMAX, true>), grid, dim3(kPT), lds, s, d);
+        }
     }
 }
 

@@ -52,7 +52,7 @@ def lib():
         for name in ("pisces_grid", "pisces_set_potential", "pisces_get_potential",
                      "pisces_apply_h", "pisces_diag", "pisces_piab", "pisces_solve",
                      "pisces_build_potential", "pisces_build_cluster_potential",
-                     "pisces_apply_h_device", "pisces_apply_h_ex", "pisces_time_hx",
+                     "pisces_apply_h_device", "pisces_apply_h_ex", "pisces_time_hx", "pisces_time_precond",
                      "pisces_solve_block"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
@@ -236,6 +236,12 @@ class Context:
         ms = ctypes.c_double(0)
         check(lib().pisces_time_hx(self.h, int(nvec), int(precision), int(warmup), int(iters),
                                    ctypes.byref(ms)), "pisces_time_hx")
+        return ms.value
+
+    def time_precond(self, nvec, precision, warmup=5, iters=50):
+        ms = ctypes.c_double(0)
+        check(lib().pisces_time_precond(self.h, int(nvec), int(precision), int(warmup), int(iters),
+                                        ctypes.byref(ms)), "pisces_time_precond")
         return ms.value
 
     def diag(self):
