@@ -5,6 +5,7 @@
 // go through ld_any(); the flag is uniform per task, so the branch is uniform.
 #include "bd.hpp"
 #include "blockops.hpp"
+#include "gmem.hpp"
 
 #include <dlfcn.h>
 #include <cstdio>
@@ -21,18 +22,24 @@ namespace {
 
 constexpr int kT = 256;
 
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 __device__ inline double ld_any(const void *p, long long i, int is32)
 {
-    return is32 ? (double)static_cast<const float *>(p)[i] : static_cast<const double *>(p)[i];
+    return is32 ? (double)glb(static_cast<const float *>(p))[i] : glb(static_cast<const double *>(p))[i];
 }
 __device__ inline void st_any(void *p, long long i, int is32, double v)
 {
-    if (is32) static_cast<float *>(p)[i] = (float)v;
-    else static_cast<double *>(p)[i] = v;
+    if (is32) glb(static_cast<float *>(p))[i] = (float)v;
+    else glb(static_cast<double *>(p))[i] = v;
 }
 
 // ------------------------------------------------------------------ Gram
 // A^T B of two tall column blocks (rows = grid points) on the FP64 matrix cores.
+// The column pointers stay generic (flat loads): with global-address-space
+// loads the compiler hoists every column's loads ahead of the MFMAs, and the
+// 64-row variant drops from three to two waves per SIMD and runs slower.
 // v_mfma_f64_16x16x4f64 takes A^T as a 16 x 4 operand (lane l: row m = l%16,
 // k = l/16) and B as 4 x 16 (k = l/16, column n = l%16) and accumulates a
 // 16 x 16 tile, element r of lane l holding D[l/16 + 4r][l%16].  The k index
@@ -47,17 +54,17 @@ __device__ inline void st_any(void *p, long long i, int is32, double v)
 typedef double gram_f64x4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
-__device__ __forceinline__ void gram_load4(const T *__restrict__ col, long long r, long long N,
-                                           bool vec, double (&v)[4])
+__device__ __forceinline__ void gram_load4(const T *col, long long r, long long N, bool vec,
+                                           double (&v)[4])
 {
     if (vec && r + 4 <= N) {
         if (sizeof(T) == 8) {
-            const double2 a = *reinterpret_cast<const double2 *>(col + r);
-            const double2 b = *reinterpret_cast<const double2 *>(col + r + 2);
-            v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+            const f64x2 a = *reinterpret_cast<const f64x2 *>(col + r);
+            const f64x2 b = *reinterpret_cast<const f64x2 *>(col + r + 2);
+            v[0] = a[0]; v[1] = a[1]; v[2] = b[0]; v[3] = b[1];
         } else {
-            const float4 a = *reinterpret_cast<const float4 *>(col + r);
-            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            const f32x4 a = *reinterpret_cast<const f32x4 *>(col + r);
+            v[0] = a[0]; v[1] = a[1]; v[2] = a[2]; v[3] = a[3];
         }
     } else {
 #pragma unroll
@@ -66,8 +73,8 @@ __device__ __forceinline__ void gram_load4(const T *__restrict__ col, long long 
 }
 
 template <typename TA, typename TB>
-__device__ __forceinline__ void gram_chunk(const TA *__restrict__ A, long long lda, int ma,
-                                           const TB *__restrict__ B, long long ldb, int mb,
+__device__ __forceinline__ void gram_chunk(const TA *A, long long lda, int ma,
+                                           const TB *B, long long ldb, int mb,
                                            long long N, long long r_beg, long long r_end,
                                            bool vec, gram_f64x4 (&acc)[kBdMaxM / 16])
 {
@@ -100,22 +107,22 @@ __device__ __forceinline__ void gram_chunk(const TA *__restrict__ A, long long l
 // loads per column for FP64), fed to 16 instructions; each column is then read
 // in 512-byte runs like the VALU kernel.
 template <typename T>
-__device__ __forceinline__ void gram_load16(const T *__restrict__ col, long long r, long long N,
-                                            bool vec, double (&v)[16])
+__device__ __forceinline__ void gram_load16(const T *col, long long r, long long N, bool vec,
+                                            double (&v)[16])
 {
     if (vec && r + 16 <= N) {
         if (sizeof(T) == 8) {
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
-                const double2 a = *reinterpret_cast<const double2 *>(col + r + 2 * h);
-                v[2 * h] = a.x;
-                v[2 * h + 1] = a.y;
+                const f64x2 a = *reinterpret_cast<const f64x2 *>(col + r + 2 * h);
+                v[2 * h] = a[0];
+                v[2 * h + 1] = a[1];
             }
         } else {
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
-                const float4 a = *reinterpret_cast<const float4 *>(col + r + 4 * h);
-                v[4 * h] = a.x; v[4 * h + 1] = a.y; v[4 * h + 2] = a.z; v[4 * h + 3] = a.w;
+                const f32x4 a = *reinterpret_cast<const f32x4 *>(col + r + 4 * h);
+                v[4 * h] = a[0]; v[4 * h + 1] = a[1]; v[4 * h + 2] = a[2]; v[4 * h + 3] = a[3];
             }
         }
     } else {
@@ -125,8 +132,8 @@ __device__ __forceinline__ void gram_load16(const T *__restrict__ col, long long
 }
 
 template <typename TA, typename TB>
-__device__ __forceinline__ void gram_chunk64(const TA *__restrict__ A, long long lda, int ma,
-                                             const TB *__restrict__ B, long long ldb, int mb,
+__device__ __forceinline__ void gram_chunk64(const TA *A, long long lda, int ma,
+                                             const TB *B, long long ldb, int mb,
                                              long long N, long long r_beg, long long r_end,
                                              bool vec, gram_f64x4 (&acc)[kBdMaxM / 16])
 {
@@ -167,8 +174,8 @@ __device__ __forceinline__ void gram_chunk64(const TA *__restrict__ A, long long
 // the wave (63 shuffles) leaves tile entry `lane` on lane `lane`, and the four
 // waves are summed through LDS into this chunk's partial.
 template <typename TA, typename TB>
-__device__ __forceinline__ void gram_rows(const TA *__restrict__ A, long long lda, int na,
-                                          const TB *__restrict__ B, long long ldb, int nb,
+__device__ __forceinline__ void gram_rows(gptr<const TA> A, long long lda, int na,
+                                          gptr<const TB> B, long long ldb, int nb,
                                           long long r0, long long r1, double (&acc)[kGTile * kGTile])
 {
     if (na == kGTile && nb == kGTile) {
@@ -242,14 +249,16 @@ __global__ void __launch_bounds__(kT) gram_valu_kernel(const GramTask *__restric
     double acc[kGTile * kGTile];
 #pragma unroll
     for (int i = 0; i < kGTile * kGTile; ++i) acc[i] = 0.0;
+    const gptr<const float> Bf = glb(static_cast<const float *>(t.B)) + b0 * t.ldb;
+    const gptr<const double> Bd = glb(static_cast<const double *>(t.B)) + b0 * t.ldb;
     if (t.a32) {
-        const float *A = static_cast<const float *>(t.A) + a0 * t.lda;
-        if (t.b32) gram_rows(A, t.lda, na, static_cast<const float *>(t.B) + b0 * t.ldb, t.ldb, nb, r0, r1, acc);
-        else gram_rows(A, t.lda, na, static_cast<const double *>(t.B) + b0 * t.ldb, t.ldb, nb, r0, r1, acc);
+        const gptr<const float> A = glb(static_cast<const float *>(t.A)) + a0 * t.lda;
+        if (t.b32) gram_rows(A, t.lda, na, Bf, t.ldb, nb, r0, r1, acc);
+        else gram_rows(A, t.lda, na, Bd, t.ldb, nb, r0, r1, acc);
     } else {
-        const double *A = static_cast<const double *>(t.A) + a0 * t.lda;
-        if (t.b32) gram_rows(A, t.lda, na, static_cast<const float *>(t.B) + b0 * t.ldb, t.ldb, nb, r0, r1, acc);
-        else gram_rows(A, t.lda, na, static_cast<const double *>(t.B) + b0 * t.ldb, t.ldb, nb, r0, r1, acc);
+        const gptr<const double> A = glb(static_cast<const double *>(t.A)) + a0 * t.lda;
+        if (t.b32) gram_rows(A, t.lda, na, Bf, t.ldb, nb, r0, r1, acc);
+        else gram_rows(A, t.lda, na, Bd, t.ldb, nb, r0, r1, acc);
     }
     const double w = wave_reduce_scatter64(acc);
     __shared__ double red[kT / 64][64];
@@ -338,7 +347,7 @@ __global__ void __launch_bounds__(kT) gram_reduce_kernel(const GramTask *__restr
 // ------------------------------------------------------------------ combine
 // acc[b] += sum_a A[a*lda + g] w[(a0 + a) + ma*b]: four column loads in flight
 template <typename T>
-__device__ __forceinline__ void comb_cols(const T *__restrict__ A, long long lda, int na,
+__device__ __forceinline__ void comb_cols(gptr<const T> A, long long lda, int na,
                                           const double *w, int ma, int a0, int mb, long long g,
                                           double (&acc)[kBdMaxK])
 {
@@ -368,8 +377,8 @@ __device__ __forceinline__ void comb_source(const void *A, long long lda, int na
                                             double (&acc)[kBdMaxK])
 {
     if (na <= 0) return;
-    if (a32) comb_cols(static_cast<const float *>(A), lda, na, w, ma, a0, mb, g, acc);
-    else comb_cols(static_cast<const double *>(A), lda, na, w, ma, a0, mb, g, acc);
+    if (a32) comb_cols(glb(static_cast<const float *>(A)), lda, na, w, ma, a0, mb, g, acc);
+    else comb_cols(glb(static_cast<const double *>(A)), lda, na, w, ma, a0, mb, g, acc);
 }
 
 __global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ tasks)
@@ -603,7 +612,7 @@ __global__ void __launch_bounds__(64) resid_reduce_kernel(const ResTask *__restr
 // u[b] += sum_a A[a*N + g] y[(a0 + a)][b], hu likewise with HA: two columns
 // (four loads) in flight
 template <int KB, typename T>
-__device__ __forceinline__ void ritz_cols(const T *__restrict__ A, const T *__restrict__ HA,
+__device__ __forceinline__ void ritz_cols(gptr<const T> A, gptr<const T> HA,
                                           long long N, int na, const double *y, int a0,
                                           long long g, double (&u)[KB], double (&hu)[KB])
 {
@@ -653,13 +662,13 @@ __global__ void __launch_bounds__(kT) ritz_kernel(const RitzTask *__restrict__ t
 #pragma unroll
     for (int b = 0; b < KB; ++b) u[b] = hu[b] = 0.0;
     if (row) {
-        ritz_cols(t.X, t.HX, t.N, k, y, 0, g, u, hu);
+        ritz_cols(glb(t.X), glb(t.HX), t.N, k, y, 0, g, u, hu);
         if (t.c32)
-            ritz_cols(static_cast<const float *>(t.C), static_cast<const float *>(t.HC), t.N, mc, y,
-                      k, g, u, hu);
+            ritz_cols(glb(static_cast<const float *>(t.C)), glb(static_cast<const float *>(t.HC)), t.N,
+                      mc, y, k, g, u, hu);
         else
-            ritz_cols(static_cast<const double *>(t.C), static_cast<const double *>(t.HC), t.N, mc,
-                      y, k, g, u, hu);
+            ritz_cols(glb(static_cast<const double *>(t.C)), glb(static_cast<const double *>(t.HC)),
+                      t.N, mc, y, k, g, u, hu);
     }
     double nrm[KB];
 #pragma unroll
@@ -667,8 +676,8 @@ __global__ void __launch_bounds__(kT) ritz_kernel(const RitzTask *__restrict__ t
         nrm[b] = 0.0;
         if (b < k && row) {
             const double r = hu[b] - th[b] * u[b];
-            t.U[b * t.N + g] = u[b];
-            t.HU[b * t.N + g] = hu[b];
+            glb(t.U)[b * t.N + g] = u[b];
+            glb(t.HU)[b * t.N + g] = hu[b];
             st_any(t.R, b * t.N + g, t.r32, r);
             nrm[b] = r * r;
         }

@@ -33,6 +33,7 @@
 // with |lam_x + lam_y + lam_z - shift| floored at `floor`.
 #include "common.hpp"
 #include "fdm.hpp"
+#include "gmem.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -67,12 +68,6 @@ __device__ __forceinline__ bf16x8 ld8(const uint16_t *p)
 {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(p));
 }
-// Global-memory pointers.  Pointers read from job descriptors are generic to
-// the compiler, which then emits flat accesses (counted against both the
-// vector-memory and the LDS counters); casting them to the global address
-// space gives global_load/global_store with plain vector-memory waits.
-template <typename T> using gptr = __attribute__((address_space(1))) T *;
-template <typename T> __device__ __forceinline__ gptr<T> glb(T *p) { return (gptr<T>)p; }
 __device__ __forceinline__ bf16x8 ld8(gptr<const uint16_t> p)
 {
     return __builtin_bit_cast(bf16x8, *reinterpret_cast<gptr<const u32x4>>(p));
@@ -364,6 +359,47 @@ __device__ __forceinline__ void mm_pair(const bf16x8 (&ah)[KS], const bf16x8 (&a
     }
 }
 
+// B-form fragments of a row-major split matrix (leading dimension ld) for all
+// column tiles: rows 16 v + l16, columns 32 s + 8 lq .. + 7.  Held in
+// registers for a workgroup's lifetime where they fit (grids up to 64).
+template <int NIT, int KS>
+__device__ __forceinline__ void bfrag(const uint16_t *Mh, const uint16_t *Ml, int ld, int nti, int ks,
+                                      bf16x8 (&bh)[NIT][KS], bf16x8 (&bl)[NIT][KS])
+{
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int v = 0; v < NIT; ++v)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            if (v < nti && s < ks) {
+                bh[v][s] = ld8(glb(Mh) + (16 * v + l16) * ld + 32 * s + 8 * lq);
+                bl[v][s] = ld8(glb(Ml) + (16 * v + l16) * ld + 32 * s + 8 * lq);
+            } else {
+                bh[v][s] = bf_zero();
+                bl[v][s] = bf_zero();
+            }
+        }
+}
+
+// mm_pair with both operands held in registers.
+template <int KS>
+__device__ __forceinline__ void mm_pair_held(const bf16x8 (&ah)[KS], const bf16x8 (&al)[KS],
+                                             const bf16x8 (&b0h)[KS], const bf16x8 (&b0l)[KS],
+                                             const bf16x8 (&b1h)[KS], const bf16x8 (&b1l)[KS], int ks,
+                                             f32x4 &a0, f32x4 &a1)
+{
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        if (s >= ks) break;
+        a0 = mma_bf(ah[s], b0h[s], a0);
+        a1 = mma_bf(ah[s], b1h[s], a1);
+        a0 = mma_bf(ah[s], b0l[s], a0);
+        a1 = mma_bf(ah[s], b1l[s], a1);
+        a0 = mma_bf(al[s], b0h[s], a0);
+        a1 = mma_bf(al[s], b1h[s], a1);
+    }
+}
+
 // A-form fragments of the wave's own rows of the c-major LDS plane.
 template <int KS>
 __device__ __forceinline__ void afrag_lds(const uint16_t *Pch, const uint16_t *Pcl, int pc, int c0, int ks,
@@ -408,6 +444,11 @@ __global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ j
     const int ntc = g.npc / 16, nti = g.npi / 16, ksc = g.npc / 32, ksi = g.npi / 32;
     bf16x8 ach[NCT][KS], acl[NCT][KS];
     afrag(Mc.th, Mc.tl, g.npc, ntc, ksc, ach, acl);
+    // pass B, grids up to 64: Tx in B form held for the workgroup's lifetime
+    constexpr bool HOLD = PASS == 1 && NPMAX <= 64;
+    constexpr int NB = HOLD ? NIT : 1, KB = HOLD ? KS : 1;
+    bf16x8 bth[NB][KB], btl[NB][KB];
+    if constexpr (HOLD) bfrag(Mi.th, Mi.tl, g.npi, nti, ksi, bth, btl);
     float xb[NCT][NIT][4], wb[NCT][NIT][4];  // X; V (pass A) or Y (pass B)
     own_load(X, g.base, g, xb);
     own_load(PASS == 0 ? V : static_cast<const float *>(J.Y), PASS == 0 ? g.vbase : g.base, g, wb);
@@ -443,7 +484,10 @@ __global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ j
                 }
                 mm_pair(ach[u], acl[u], Pth, Ptl, pt, ksc, 16 * v, a0, a1);
                 // B(k, i) = Tx[k][i] = Tx[i][k] (symmetric)
-                if (PASS == 1) mm_pair(aih, ail, glb(Mi.th), glb(Mi.tl), g.npi, ksi, 16 * v, a0, a1);
+                if constexpr (HOLD)
+                    mm_pair_held(aih, ail, bth[v], btl[v], bth[v + 1], btl[v + 1], ksi, a0, a1);
+                else if (PASS == 1)
+                    mm_pair(aih, ail, glb(Mi.th), glb(Mi.tl), g.npi, ksi, 16 * v, a0, a1);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int c = c0 + 4 * lq + r;
@@ -469,7 +513,161 @@ __global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ j
     }
 }
 
-// ------------------------------------------------------------------ H*X, FP64 / FP32 forms
+// ------------------------------------------------------------------ H*X, FP64 form
+// Owned tiles as above, in the f64 accumulator layout: lane (l16, lq) holds
+// rows lq + 4 r (r < 4) of column l16 in each of its wave's tiles.  The plane
+// is kept in LDS as FP64, c-major (pitch npi + 2 doubles); the along-c matrix
+// (A operand, rows of the wave's tiles) is held in registers, and in pass B
+// the along-i matrix Tx (B operand) is staged once per workgroup into LDS
+// next to the plane.  Each k-step feeds one A value to the MFMAs of all the
+// wave's column tiles (independent accumulator chains).
+template <int NCT, int NIT>
+__device__ __forceinline__ void own_load64(const double *src_, long long base, const Geo &g,
+                                           double (&x)[NCT][NIT][4])
+{
+    const gptr<const double> src = glb(src_) + base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int u = 0; u < NCT; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = 16 * (wave + 4 * u) + lq + 4 * r;
+#pragma unroll
+            for (int v = 0; v < NIT; ++v) {
+                const int i = 16 * v + l16;
+                const bool ok = c < g.nc && i < g.ni;
+                const double y = src[ok ? (long long)c * g.cs + i : 0];
+                x[u][v][r] = ok ? y : 0.0;
+            }
+        }
+}
+
+template <int PASS, int NPMAX, bool MULTI>
+__global__ void __launch_bounds__(kPT) hx_f64_kernel(const HxJob *__restrict__ jobs)
+{
+    constexpr int NCT = Own<NPMAX>::NCT, NIT = Own<NPMAX>::NIT, KF = NPMAX / 4;
+    const HxJob &J = jobs[blockIdx.y];
+    const AxisMats &Mi = J.ax[0];
+    const AxisMats &Mc = J.ax[PASS == 0 ? 1 : 2];
+    int p = blockIdx.x;
+    Geo g;
+    if (!geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, p, g)) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int pitch = g.npi + 2;
+    double *P = reinterpret_cast<double *>(smem);
+    double *Tl = P + g.npc * pitch;  // pass B: Tx, rows k, pitch npi + 2
+    const double *X = static_cast<const double *>(J.X);
+    const double *V = static_cast<const double *>(J.V);
+    const gptr<double> Y = glb(static_cast<double *>(J.Y));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int ntc = g.npc / 16, nti = g.npi / 16, ksc = g.npc / 4, ksi = g.npi / 4;
+    // along-c matrix in A form: ac[u][s] = Tc[16 tc + l16][4 s + lq]
+    double ac[NCT][KF];
+    {
+        const gptr<const double> Tc = glb(Mc.t64);
+#pragma unroll
+        for (int u = 0; u < NCT; ++u)
+#pragma unroll
+            for (int s = 0; s < KF; ++s) {
+                const int tc = wave + 4 * u;
+                const bool ok = tc < ntc && s < ksc;
+                const double y = Tc[ok ? (16 * tc + l16) * g.npc + 4 * s + lq : 0];
+                ac[u][s] = ok ? y : 0.0;
+            }
+    }
+    if (PASS == 1) {
+        const gptr<const double> Ti = glb(Mi.t64);
+        for (int e = threadIdx.x; e < g.npi * g.npi; e += kPT) {
+            const int k = e / g.npi, i = e - k * g.npi;
+            Tl[k * pitch + i] = Ti[e];
+        }
+    }
+    double xb[NCT][NIT][4], wb[NCT][NIT][4];  // X; V (pass A) or Y (pass B)
+    own_load64<NCT, NIT>(X, g.base, g, xb);
+    own_load64<NCT, NIT>(PASS == 0 ? V : static_cast<const double *>(J.Y), PASS == 0 ? g.vbase : g.base, g, wb);
+    for (;;) {
+#pragma unroll
+        for (int u = 0; u < NCT; ++u) {
+            const int tc = wave + 4 * u;
+            if (tc >= ntc) break;
+#pragma unroll
+            for (int v = 0; v < NIT; ++v) {
+                if (v >= nti) break;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    P[(16 * tc + lq + 4 * r) * pitch + 16 * v + l16] = xb[u][v][r];
+                    if (PASS == 0) wb[u][v][r] *= xb[u][v][r];
+                }
+            }
+        }
+        __syncthreads();
+        Geo gn;
+        const bool more = MULTI && geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, p + gridDim.x, gn);
+        if (more) own_load64<NCT, NIT>(X, gn.base, gn, xb);
+#pragma unroll
+        for (int u = 0; u < NCT; ++u) {
+            const int tc = wave + 4 * u;
+            if (tc >= ntc) break;
+            const int c0 = 16 * tc;
+            f64x4 acc[NIT];
+#pragma unroll
+            for (int v = 0; v < NIT; ++v)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[v][r] = PASS == 0 ? wb[u][v][r] : 0.0;
+            // along c: D(c, i) += sum_k Tc[c][k] P[k][i]
+#pragma unroll
+            for (int s = 0; s < KF; ++s) {
+                if (s >= ksc) break;
+                const double *pr = P + (4 * s + lq) * pitch + l16;
+#pragma unroll
+                for (int v = 0; v < NIT; ++v) {
+                    if (v >= nti) break;
+                    acc[v] = mma(ac[u][s], pr[16 * v], acc[v]);
+                }
+            }
+            // along i: D(c, i) += sum_k P[c][k] Tx[k][i]
+            if (PASS == 1) {
+#pragma unroll
+                for (int s = 0; s < KF; ++s) {
+                    if (s >= ksi) break;
+                    const double a = P[(c0 + l16) * pitch + 4 * s + lq];
+                    const double *tr = Tl + (4 * s + lq) * pitch + l16;
+#pragma unroll
+                    for (int v = 0; v < NIT; ++v) {
+                        if (v >= nti) break;
+                        acc[v] = mma(a, tr[16 * v], acc[v]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < NIT; ++v) {
+                if (v >= nti) break;
+                const int i = 16 * v + l16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int c = c0 + lq + 4 * r;
+                    if (c >= g.nc || i >= g.ni) continue;
+                    const long long row = (long long)c * g.cs;
+                    Y[g.base + row + i] = PASS == 0 ? acc[v][r] : wb[u][v][r] + acc[v][r];
+                    if (more) {
+                        const gptr<const double> W =
+                            PASS == 0 ? glb(V) + gn.vbase + row : (gptr<const double>)Y + gn.base + row;
+                        wb[u][v][r] = W[i];
+                    }
+                }
+            }
+        }
+        if (!more) break;
+        __syncthreads();  // every wave is done with this plane's LDS copy
+        g = gn;
+        p += gridDim.x;
+    }
+}
+
+// ------------------------------------------------------------------ H*X, FP32 form
+
 template <typename T> __device__ inline const T *tmat(const AxisMats &m);
 template <> __device__ inline const double *tmat<double>(const AxisMats &m) { return m.t64; }
 template <> __device__ inline const float *tmat<float>(const AxisMats &m) { return m.t32; }
@@ -673,16 +871,19 @@ __global__ void __launch_bounds__(kPT) fdm_f64_kernel(const FdmTask *__restrict_
 }
 
 // ------------------------------------------------------------------ preconditioner, split-BF16
-// Owned tiles scaled by the preconditioner denominators of plane (i = a,
-// c = z index) at fixed j = g.pl, then split into the i-major plane.
+// Owned tiles scaled by scale / (lam_x[i] + ly + lam_z[c]), ly = lam_y[j] -
+// shift of plane (i = a, c = z index) at fixed j, then split into the i-major
+// plane.  lx[v] = lam_x[16 v + l16] and lz[u][r] = lam_z[c] are held per
+// workgroup; the factor is formed in FP32 (the scaled values are FP32 and go
+// on as split BF16).
 template <int NCT, int NIT>
-__device__ __forceinline__ void put_scaled(const f32x4 (&acc)[NCT][NIT], const Geo &g, const FdmTask &t,
-                                           uint16_t *Pth, uint16_t *Ptl)
+__device__ __forceinline__ void put_scaled(const f32x4 (&acc)[NCT][NIT], const Geo &g, float ly,
+                                           const float (&lx)[NIT], const float (&lz)[NCT][4],
+                                           float scale, float floor, uint16_t *Pth, uint16_t *Ptl)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int ntc = g.npc / 16, nti = g.npi / 16, pt = g.npc + 8;
-    const double ly = t.ax[1].lam[g.pl] - t.shift;
 #pragma unroll
     for (int u = 0; u < NCT; ++u) {
         const int tc = wave + 4 * u;
@@ -694,13 +895,9 @@ __device__ __forceinline__ void put_scaled(const f32x4 (&acc)[NCT][NIT], const G
             float x[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int c = cb + r;
-                double den = 1.0;
-                if (i < g.ni && c < g.nc) {
-                    den = t.ax[0].lam[i] + ly + t.ax[2].lam[c];
-                    if (fabs(den) < t.floor) den = den < 0 ? -t.floor : t.floor;
-                }
-                x[r] = (float)((double)acc[u][v][r] * (t.scale / den));
+                float den = lx[v] + ly + lz[u][r];
+                if (fabsf(den) < floor) den = den < 0.f ? -floor : floor;
+                x[r] = acc[u][v][r] * (scale / den);
             }
             put_tile(x, cb, i, pt, 0, Pth, Ptl, nullptr, nullptr);
         }
@@ -769,6 +966,30 @@ __device__ __forceinline__ void mm_c_all(const bf16x8 (&ah)[NCT][KS], const bf16
 }
 
 template <int NCT, int NIT, int KS>
+__device__ __forceinline__ void mm_i_held(const uint16_t *Pch, const uint16_t *Pcl,
+                                          const bf16x8 (&bh)[NIT][KS], const bf16x8 (&bl)[NIT][KS],
+                                          const Geo &g, f32x4 (&acc)[NCT][NIT])
+{
+    const int wave = threadIdx.x >> 6;
+    const int ntc = g.npc / 16, nti = g.npi / 16, ksi = g.npi / 32, pc = g.npi + 8;
+#pragma unroll
+    for (int u = 0; u < NCT; ++u) {
+        const int tc = wave + 4 * u;
+        if (tc >= ntc) break;
+        bf16x8 ah[KS], al[KS];
+        afrag_lds(Pch, Pcl, pc, 16 * tc, ksi, ah, al);
+#pragma unroll
+        for (int v = 0; v < NIT; v += 2) {
+            if (v >= nti) break;
+            f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
+            mm_pair_held(ah, al, bh[v], bl[v], bh[v + 1], bl[v + 1], ksi, a0, a1);
+            acc[u][v] = a0;
+            acc[u][v + 1] = a1;
+        }
+    }
+}
+
+template <int NCT, int NIT, int KS>
 __device__ __forceinline__ void mm_i_all(const uint16_t *Pch, const uint16_t *Pcl, const uint16_t *Mh,
                                          const uint16_t *Ml, const Geo &g, f32x4 (&acc)[NCT][NIT])
 {
@@ -824,6 +1045,36 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict_
         afrag(Mz.qh, Mz.ql, g.npc, ntc, ksc, bh, bl);                    // Qz
     }
     if (PASS == 2) afrag(My.qh, My.ql, g.npc, ntc, ksc, ah, al);         // Qy
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int nti = g.npi / 16, ksi = g.npi / 32;
+    // passes 0 and 2, grids up to 64: Qx in B form held for the workgroup's lifetime
+    constexpr bool HOLD = PASS != 1 && NPMAX <= 64;
+    constexpr int NB = HOLD ? NIT : 1, KB = HOLD ? KS : 1;
+    bf16x8 qbh[NB][KB], qbl[NB][KB];
+    if constexpr (HOLD) {
+        if (PASS == 0) bfrag(Mx.qth, Mx.qtl, g.npi, nti, ksi, qbh, qbl);  // B form of Qx
+        else bfrag(Mx.qh, Mx.ql, g.npi, nti, ksi, qbh, qbl);              // B form of Qx^T
+    }
+    // pass 1: eigenvalues of the owned rows and columns
+    float lx[NIT], lz[NCT][4];
+    if (PASS == 1) {
+        const gptr<const double> Lx = glb(Mx.lam), Lz = glb(Mz.lam);
+#pragma unroll
+        for (int v = 0; v < NIT; ++v) {
+            const int i = 16 * v + l16;
+            const double y = Lx[i < g.ni ? i : 0];
+            lx[v] = i < g.ni ? (float)y : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < NCT; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int c = 16 * (wave + 4 * u) + 4 * lq + r;
+                const double y = Lz[c < g.nc ? c : 0];
+                lz[u][r] = c < g.nc ? (float)y : 0.f;
+            }
+    }
     const void *src = PASS == 0 ? t.src : t.work;
     const bool s32 = PASS != 0 || t.src32;
     float xb[NCT][NIT][4];
@@ -840,14 +1091,17 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict_
             else own_load(static_cast<const double *>(src), gn.base, gn, xb);
         }
         if (PASS == 0) {
-            mm_i_all<NCT, NIT, KS>(P2h, P2l, Mx.qth, Mx.qtl, g, acc);   // X Qx (B form: Qx^T)
+            if constexpr (HOLD) mm_i_held<NCT, NIT, KS>(P2h, P2l, qbh, qbl, g, acc);  // X Qx
+            else mm_i_all<NCT, NIT, KS>(P2h, P2l, Mx.qth, Mx.qtl, g, acc);
             put_acc(acc, g, Pth, Ptl, nullptr, nullptr);
             __syncthreads();
             mm_c_all(ah, al, Pth, Ptl, g, acc);                            // Qy^T (.)
             bf2_store_global(t.work, g, acc);
         } else if (PASS == 1) {
             mm_c_all(ah, al, Pth, Ptl, g, acc);                            // Qz^T X
-            put_scaled(acc, g, t, P2h, P2l);
+            // rows of the scaled plane: i = a, c = z index, at j = g.pl
+            put_scaled(acc, g, (float)(glb(My.lam)[g.pl] - t.shift), lx, lz, (float)t.scale,
+                       (float)t.floor, P2h, P2l);
             __syncthreads();
             mm_c_all(bh, bl, P2h, P2l, g, acc);                            // Qz (.)
             bf2_store_global(t.work, g, acc);
@@ -855,7 +1109,8 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict_
             mm_c_all(ah, al, Pth, Ptl, g, acc);                            // Qy X
             put_acc(acc, g, nullptr, nullptr, P2h, P2l);
             __syncthreads();
-            mm_i_all<NCT, NIT, KS>(P2h, P2l, Mx.qh, Mx.ql, g, acc);     // (.) Qx^T (B form: Qx)
+            if constexpr (HOLD) mm_i_held<NCT, NIT, KS>(P2h, P2l, qbh, qbl, g, acc);  // (.) Qx^T
+            else mm_i_all<NCT, NIT, KS>(P2h, P2l, Mx.qh, Mx.ql, g, acc);
             if (t.dst32) bf2_store_global(static_cast<float *>(t.dst), g, acc);
             else bf2_store_global(static_cast<double *>(t.dst), g, acc);
         }
@@ -908,9 +1163,10 @@ void init_f()
     });
 }
 
-// Planes per workgroup of the split-BF16 kernels (measured at 64^3 with
-// 8 - 160 vectors): H*X runs fastest with one plane per workgroup, the
-// preconditioner with two.  PISCES_PPW_HX / PISCES_PPW_FDM override.
+// Planes per workgroup (measured at 64^3 with 8 - 160 vectors): the H*X
+// kernels run fastest with one plane per workgroup, the split-BF16
+// preconditioner with four.  PISCES_PPW_HX / PISCES_PPW_F64 / PISCES_PPW_FDM
+// override.
 int ppw_of(const char *env, int def)
 {
     const char *e = std::getenv(env);
@@ -935,6 +1191,28 @@ void hx_bf2_launch(const HxJob *d, int n, int pa, int pb, int np, hipStream_t s)
 }
 
 template <int NPMAX>
+void hx_f64_launch(const HxJob *d, int n, int pa, int pb, int np, hipStream_t s)
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        allow_lds(hx_f64_kernel<0, NPMAX, false>);
+        allow_lds(hx_f64_kernel<1, NPMAX, false>);
+        allow_lds(hx_f64_kernel<0, NPMAX, true>);
+        allow_lds(hx_f64_kernel<1, NPMAX, true>);
+    });
+    static const int ppw = ppw_of("PISCES_PPW_F64", 1);
+    const dim3 ga((pa + ppw - 1) / ppw, n), gb((pb + ppw - 1) / ppw, n);
+    const size_t la = (size_t)np * (np + 2) * sizeof(double), lb = 2 * la;
+    if (ppw == 1) {
+        hipLaunchKernelGGL((hx_f64_kernel<0, NPMAX, false>), ga, dim3(kPT), la, s, d);
+        hipLaunchKernelGGL((hx_f64_kernel<1, NPMAX, false>), gb, dim3(kPT), lb, s, d);
+    } else {
+        hipLaunchKernelGGL((hx_f64_kernel<0, NPMAX, true>), ga, dim3(kPT), la, s, d);
+        hipLaunchKernelGGL((hx_f64_kernel<1, NPMAX, true>), gb, dim3(kPT), lb, s, d);
+    }
+}
+
+template <int NPMAX>
 void fdm_np_launch(const FdmTask *d, int n, int nplanes, int np, int precision, hipStream_t s)
 {
     init_np<NPMAX>();
@@ -945,7 +1223,7 @@ void fdm_np_launch(const FdmTask *d, int n, int nplanes, int np, int precision, 
         hipLaunchKernelGGL((fdm_f64_kernel<2, NPMAX>), dim3(nplanes, n), dim3(kPT), lds, s, d);
     } else {
         const size_t lds = bf2_lds(np);
-        static const int ppw = ppw_of("PISCES_PPW_FDM", 2);
+        static const int ppw = ppw_of("PISCES_PPW_FDM", 4);
         const dim3 grid((nplanes + ppw - 1) / ppw, n);
         if (ppw == 1) {
             hipLaunchKernelGGL((fdm_bf2_kernel<0, NPMAX, false>), grid, dim3(kPT), lds, s, d);
@@ -976,7 +1254,12 @@ void hx_mfma_launch(const HxJob *d_jobs, int njobs, int max_planes_a, int max_pl
         const size_t lds = f_lds<float>(np);
         hipLaunchKernelGGL((hx_f_kernel<float, 0>), dim3(max_planes_a, njobs), dim3(kPT), lds, s, d_jobs);
         hipLaunchKernelGGL((hx_f_kernel<float, 1>), dim3(max_planes_b, njobs), dim3(kPT), lds, s, d_jobs);
+    } else if (np <= 32) {
+        hx_f64_launch<32>(d_jobs, njobs, max_planes_a, max_planes_b, np, s);
+    } else if (np <= 64) {
+        hx_f64_launch<64>(d_jobs, njobs, max_planes_a, max_planes_b, np, s);
     } else {
+        // a 128-wide FP64 plane and Tx do not fit in LDS together
         init_f();
         const size_t lds = f_lds<double>(np);
         hipLaunchKernelGGL((hx_f_kernel<double, 0>), dim3(max_planes_a, njobs), dim3(kPT), lds, s, d_jobs);

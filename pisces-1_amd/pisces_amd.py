@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libpisces_hip.so")
+# PISCES_LIB names another build of the library (A/B timing of kernel variants)
+LIB_PATH = os.environ.get("PISCES_LIB") or os.path.join(HERE, "lib", "libpisces_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "pisces_hip.h")
 
 FP64, FP32, BF16X2 = 0, 1, 2
