@@ -574,8 +574,8 @@ __global__ void __launch_bounds__(kT) resid_kernel(const ResTask *__restrict__ t
 #pragma unroll
         for (int i = 0; i < kBdMaxK; ++i) {
             if (i >= t.k) break;
-            const double r = t.HU[i * t.ld + g] - t.theta[i] * t.U[i * t.ld + g];
-            t.R[i * t.ld + g] = r;
+            const double r = glb(t.HU)[i * t.ld + g] - t.theta[i * t.th_ld] * glb(t.U)[i * t.ld + g];
+            st_any(t.R, i * t.ld + g, t.r32, r);
             acc[i] += r * r;
         }
     }
@@ -603,7 +603,7 @@ __global__ void __launch_bounds__(64) resid_reduce_kernel(const ResTask *__restr
     for (int c = 0; c < t.nch; ++c) s += partial[t.part_off + (long long)threadIdx.x * t.nch + c];
     t.out[threadIdx.x] = s;
     if (t.gather) {
-        t.gather[threadIdx.x] = t.theta[threadIdx.x];
+        t.gather[threadIdx.x] = t.theta[threadIdx.x * t.th_ld];
         t.gather[kBdMaxK + threadIdx.x] = s;
     }
 }

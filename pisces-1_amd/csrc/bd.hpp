@@ -70,10 +70,11 @@ struct EigTask {
     int pad0, pad;
 };
 
-// R = HU - theta U for k columns, and partial sums of ||R_i||^2
+// R = HU - theta U for k columns (theta[i * th_ld]; R FP32 if r32), and
+// partial sums of ||R_i||^2
 struct ResTask {
-    const double *U; const double *HU; double *R; long long ld; int k; int pad;
-    const double *theta; long long N; long long part_off; int nch; int pad2;
+    const double *U; const double *HU; void *R; long long ld; int k; int r32;
+    const double *theta; long long N; long long part_off; int nch; int th_ld;
     double *out;     // ||R_i||^2 after reduction
     double *gather;  // host-bound copy: theta[0..16) then ||R_i||^2 [16..32)
 };

@@ -3,7 +3,9 @@
 // A step is two launch phases separated by one device-to-host transfer:
 //   A: restarts (keeping the last search directions, "GD+k"), FP64 refresh of
 //      H X, H C of the new corrections, new columns of G, eigenpairs, Ritz
-//      vectors, residuals and their norms;
+//      vectors, residuals and their norms; a step that starts with a full
+//      correction basis restarts at the end of its phase A instead (see
+//      "late restarts" below);
 //   B: corrections for the unconverged columns (precondition, orthogonalise,
 //      append).
 // All task descriptors of a phase are written to one pinned arena and sent
@@ -45,12 +47,14 @@ constexpr int kSmall = 9856;
 struct BdEngine::Inst {
     BdGrid g;
     int nroots = 0, k = 0, mcmax = 0;
-    dbuf X, HX, U, HU, Up, HUp, R, T, T2, C, HC, small;
+    dbuf X, HX, U, HU, R, T, T2, C, HC, small;
     int mc = 0, nnew = 0, gnew = 0, nt = 0;
     int src[kBdMaxK];
     bool restart = true, init = true, refresh = false, fresh = false, done = false;
     bool have_prev = false;
     bool confirm = false;  // restart requested to confirm convergence on an FP64-fresh HX
+    bool vec_x = false;    // the current Ritz block is X (a late restart), not U
+    bool late = false;     // this step ends in a late restart
     BdResult res;
     double theta[kBdMaxK], rn[kBdMaxK];
     std::shared_ptr<void> keepalive;
@@ -122,7 +126,7 @@ size_t BdEngine::bytes_per_instance(long long N) const
 {
     const size_t e = (prec_ == PISCES_FP64) ? 8 : 4;
     const size_t mcmax = (size_t)(maxsub_ - k_);
-    return (size_t)N * (9 * k_ * 8 + 2 * mcmax * e) + kSmall * 8;
+    return (size_t)N * (7 * k_ * 8 + 2 * mcmax * e) + kSmall * 8;
 }
 
 int BdEngine::live() const
@@ -164,7 +168,7 @@ int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, 
     in->keepalive = std::move(keepalive);
     const long long N = g.N;
     const size_t e = (prec_ == PISCES_FP64) ? 8 : 4;
-    for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->Up, &in->HUp, &in->R, &in->T, &in->T2})
+    for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->R, &in->T, &in->T2})
         *b = take_buf(8 * N * k_);
     in->C = take_buf(e * N * in->mcmax);
     in->HC = take_buf(e * N * in->mcmax);
@@ -201,7 +205,8 @@ const BdResult &BdEngine::result(int slot) const { return inst_.at(slot)->res; }
 void BdEngine::vectors(int slot, double *out)
 {
     Inst *in = inst_.at(slot).get();
-    PC_HIP(hipMemcpyAsync(out, in->U.p, 8 * in->g.N * in->nroots, hipMemcpyDeviceToHost, s_));
+    const dbuf &src = in->vec_x ? in->X : in->U;
+    PC_HIP(hipMemcpyAsync(out, src.p, 8 * in->g.N * in->nroots, hipMemcpyDeviceToHost, s_));
     PC_HIP(hipStreamSynchronize(s_));
 }
 
@@ -230,7 +235,7 @@ void BdEngine::release(int slot)
     // buffers go to the pool (no hipFree: it would stall the device); they are
     // reused by later add() calls, in stream order
     Inst *in = inst_.at(slot).get();
-    for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->Up, &in->HUp, &in->R, &in->T, &in->T2,
+    for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->R, &in->T, &in->T2,
                     &in->C, &in->HC, &in->small})
         if (b->p) pool_.push_back(std::move(*b));
     inst_.at(slot).reset();
@@ -362,9 +367,9 @@ int BdEngine::step()
     const size_t tmax = std::max({sizeof(GramTask), sizeof(CombTask), sizeof(PrecTask),
                                   sizeof(HxJob), sizeof(EigTask), sizeof(RitzTask), sizeof(OrthTask),
                                   sizeof(CholTask), sizeof(FdmTask)});
-    // at most ~30 descriptors per instance and phase, plus alignment slack
-    arena_[0]->reserve((tmax * 32 + 64 * 24) * (size_t)(n + 1), s_);
-    arena_[1]->reserve((tmax * 32 + 64 * 24) * (size_t)(n + 1), s_);
+    // at most ~44 descriptors per instance and phase, plus alignment slack
+    arena_[0]->reserve((tmax * 48 + 64 * 40) * (size_t)(n + 1), s_);
+    arena_[1]->reserve((tmax * 48 + 64 * 40) * (size_t)(n + 1), s_);
     if (dgather_.bytes < sizeof(double) * 2 * kBdMaxK * (size_t)n) {
         PC_HIP(hipStreamSynchronize(s_));
         dgather_.alloc(sizeof(double) * 2 * kBdMaxK * (size_t)std::max(n, 64));
@@ -381,9 +386,17 @@ int BdEngine::step()
     // ============================ phase A: descriptors
     Arena &A = *arena_[0];
     A.used = 0;
-    std::vector<Inst *> rs;
-    for (Inst *in : act)
+    std::vector<Inst *> rs, lt;
+    for (Inst *in : act) {
+        in->late = false;
         if (in->restart) rs.push_back(in);
+        // late restarts: with a full correction basis any unconverged column
+        // overflows it, so this step ends in a restart whatever its residuals
+        else if (in->mc >= in->mcmax) {
+            in->late = true;
+            lt.push_back(in);
+        }
+    }
     const int nr = (int)rs.size();
     // restart: X = orth(U) by Cholesky-QR, HX = HU W.  The Ritz block U is
     // orthonormal up to the rounding of the stored basis (FP32 corrections:
@@ -509,14 +522,12 @@ int BdEngine::step()
         const long long N = in->g.N;
         const int m = k_ + in->mc;
         he[i] = EigTask{in->sm() + oG, kBdMaxM, m, k_, in->sm() + oTh, in->sm() + oY, in->keep(), 0, 0};
-        // the previous Ritz block moves to Up/HUp; the new one is written to U/HU
-        std::swap(in->U, in->Up);
-        std::swap(in->HU, in->HUp);
         const double *Y = in->sm() + oY;
+        const bool late = in->late;
         RitzTask t{in->X.as<double>(), in->HX.as<double>(), in->C.p, in->HC.p, k_, in->mc, e32,
                    r32, Y, in->sm() + oTh, in->U.as<double>(), in->HU.as<double>(), in->R.p, N,
                    res_part, (int)((N + kRitzRows - 1) / kRitzRows), 0, in->sm() + oRn,
-                   dgather_.as<double>() + (size_t)2 * kBdMaxK * i};
+                   late ? nullptr : dgather_.as<double>() + (size_t)2 * kBdMaxK * i};
         hrz[i] = t;
         // X, HX, C, HC read; U, HU, R written
         ritz_bytes += (double)N * (16.0 * k_ + 2.0 * e * in->mc + (16.0 + (r32 ? 4 : 8)) * k_);
@@ -524,10 +535,78 @@ int BdEngine::step()
         res_nch = std::max(res_nch, t.nch);
     }
 
+    // late restarts, after this step's Ritz block U, HU: the GD+k directions
+    // from this step's eigenvectors and the current X, C (before they are
+    // overwritten), X = U W by one Cholesky-QR pass, HX refreshed in FP64 (low
+    // precision forms) or HU W, then G_XX = X^T H X and the fresh residuals
+    // R = HX - X diag(G_XX) with their norms.  In exact arithmetic X spans the
+    // Ritz block of the restarted basis [X | P] (R is orthogonal to P), so the
+    // step goes on to corrections with FP64-fresh residuals instead of
+    // spending a step on re-deriving the same Ritz pairs.
+    const int nl = (int)lt.size();
+    CoefTask *lcf = nullptr, *dlcf = nullptr;
+    lcf = A.take<CoefTask>(nl, &dlcf);
+    CholTask *lch = nullptr, *dlch = nullptr;
+    lch = A.take<CholTask>(nl, &dlch);
+    ResTask *lrs = nullptr, *dlrs = nullptr;
+    lrs = A.take<ResTask>(nl, &dlrs);
+    GramBatch lrg, lgx;
+    lrg.h = A.take<GramTask>(nl, &lrg.d);
+    lgx.h = A.take<GramTask>(nl, &lgx.d);
+    CombBatch lpc, lrc, lpc2;
+    lpc.h = A.take<CombTask>(2 * nl, &lpc.d);
+    lrc.h = A.take<CombTask>(2 * nl, &lrc.d);
+    lpc2.h = A.take<CombTask>(2 * nl, &lpc2.d);
+    HxBatch lhr;
+    lhr.h = A.take<HxJob>(nl, &lhr.d);
+    long long lres_part = res_part;
+    int lres_nch = 0;
+    double lres_bytes = 0;
+    for (int i = 0; i < nl; ++i) {
+        Inst *in = lt[i];
+        const long long N = in->g.N;
+        const int m = k_ + in->mc;
+        double *Q = in->sm() + oQp;
+        lcf[i] = CoefTask{in->sm() + oY, m, k_, Q, in->keep() + k_};
+        lpc.add(CombTask{in->X.p, N, k_, 0, Q, m, in->C.p, N, in->mc, e32, Q + k_, m, in->T.p, N,
+                         0, k_, 0, N});
+        lpc.add(CombTask{in->HX.p, N, k_, 0, Q, m, in->HC.p, N, in->mc, e32, Q + k_, m, in->T2.p,
+                         N, 0, k_, 0, N});
+        lrg.add(in->U.p, k_, 0, in->U.p, k_, 0, N, in->sm() + oGu, k_);
+        lch[i] = CholTask{in->sm() + oGu, k_, 0, in->sm() + oWu, nullptr, in->keep(), 1e-12, nullptr};
+        lrc.add(comb1(in->U.p, N, k_, 0, in->sm() + oWu, k_, in->X.p, 0, k_, 0));
+        if (!lo) lrc.add(comb1(in->HU.p, N, k_, 0, in->sm() + oWu, k_, in->HX.p, 0, k_, 0));
+        lpc2.add(comb1(in->T.p, N, k_, 0, in->sm() + oI, k_, in->C.p, e32, k_, 0));
+        lpc2.add(comb1(in->T2.p, N, k_, 0, in->sm() + oI, k_, in->HC.p, e32, k_, 0));
+        if (lo) {
+            HxJob j{};
+            j.nx = in->g.n[0]; j.ny = in->g.n[1]; j.nz = in->g.n[2];
+            j.ld = N;
+            j.nvec = k_;
+            for (int a = 0; a < 3; ++a) j.ax[a] = in->g.ax[a];
+            j.V = in->g.V64;
+            j.X = in->X.p;
+            j.Y = in->HX.p;
+            lhr.add(j);
+            in->res.n_mv64 += k_;
+        }
+        lgx.add(in->X.p, k_, 0, in->HX.p, k_, 0, N, in->sm() + oG, kBdMaxM);
+        const int nch = (int)((N + kBdResChunk - 1) / kBdResChunk);
+        lrs[i] = ResTask{in->X.as<double>(), in->HX.as<double>(), in->R.p, N, k_, r32,
+                         in->sm() + oG, N, lres_part, nch, kBdMaxM + 1, in->sm() + oRn,
+                         dgather_.as<double>() + (size_t)2 * kBdMaxK * (size_t)(std::find(act.begin(), act.end(), in) - act.begin())};
+        lres_part += (long long)k_ * nch;
+        lres_nch = std::max(lres_nch, nch);
+        lres_bytes += (double)N * k_ * (16.0 + (r32 ? 4 : 8));
+    }
+
     // ============================ phase A: launches
     for (int pass = 0; pass < 2; ++pass) rg[pass].finish();
     gg.finish();
-    const long long part = std::max({rg[0].part, rg[1].part, gg.part, res_part});
+    lrg.finish();
+    lgx.finish();
+    const long long part = std::max({rg[0].part, rg[1].part, gg.part, res_part, lrg.part, lgx.part,
+                                     lres_part});
     grow_part(sizeof(double) * (size_t)std::max(part, 1LL));
     A.flush(s_);
     double *dp = dpart_.as<double>();
@@ -555,13 +634,37 @@ int BdEngine::step()
         ProfScope ps(PK_RESID, s_, ritz_bytes);
         bd_ritz(drz, n, res_nch, k_, dp, s_);
     }
+    if (nl) {
+        bd_coef(dlcf, nl, s_);
+        lpc.launch(s_);
+        lrg.launch(dp, s_);
+        {
+            ProfScope ps(PK_CHOL, s_, 0.0);
+            bd_cholinv(dlch, nl, s_);
+        }
+        lrc.launch(s_);
+        lpc2.launch(s_);
+        lhr.launch(PISCES_FP64, s_);
+        lgx.launch(dp, s_);
+        ProfScope ps(PK_RESID, s_, lres_bytes);
+        bd_resid(dlrs, nl, lres_nch, dp, s_);
+    }
     PC_HIP(hipMemcpyAsync(hsmall_, dgather_.p, sizeof(double) * 2 * kBdMaxK * n,
                           hipMemcpyDeviceToHost, s_));
     PC_HIP(hipStreamSynchronize(s_));
-    for (Inst *in : act) in->have_prev = true;
+    for (Inst *in : act) {
+        in->have_prev = true;
+        in->vec_x = false;
+    }
     for (Inst *in : rs) {
         in->init = false;
         in->restart = false;
+    }
+    for (Inst *in : lt) {
+        in->fresh = true;
+        in->vec_x = true;
+        in->confirm = false;
+        in->res.n_restart++;
     }
 
     // ============================ control
@@ -587,6 +690,11 @@ int BdEngine::step()
         in->nnew = 0;
         in->gnew = 0;
         in->nt = 0;
+        if (in->late) {
+            // the basis is now [X | P]: P's columns of G are formed next step
+            in->mc = k_;
+            in->gnew = k_;
+        }
         if (wanted && was_fresh) {
             in->done = true;
             in->res.status = in->nroots;
@@ -727,7 +835,7 @@ int BdEngine::step()
     for (Inst *in : corr) {
         in->mc += in->nt;
         in->nnew = in->nt;
-        in->gnew = in->nt;
+        in->gnew += in->nt;
     }
     return this->live();
 }
