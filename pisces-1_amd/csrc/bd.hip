@@ -676,8 +676,8 @@ __global__ void __launch_bounds__(kT) ritz_kernel(const RitzTask *__restrict__ t
         nrm[b] = 0.0;
         if (b < k && row) {
             const double r = hu[b] - th[b] * u[b];
-            glb(t.U)[b * t.N + g] = u[b];
-            glb(t.HU)[b * t.N + g] = hu[b];
+            if (t.store & kRitzU) glb(t.U)[b * t.N + g] = u[b];
+            if (t.store & kRitzHU) glb(t.HU)[b * t.N + g] = hu[b];
             st_any(t.R, b * t.N + g, t.r32, r);
             nrm[b] = r * r;
         }

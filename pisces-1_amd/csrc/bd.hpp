@@ -36,14 +36,16 @@ struct CombTask {
 // Ritz vectors and residuals of one instance in one pass over the basis:
 //   U = X Y_X + C Y_C,  HU = HX Y_X + HC Y_C,  R = HU - U diag(theta),
 // plus partial sums of ||R_i||^2 (reduced into out, and into gather as in
-// ResTask).  X, HX, U, HU FP64; C, HC FP32 if c32; R FP32 if r32.
+// ResTask).  X, HX, U, HU FP64; C, HC FP32 if c32; R FP32 if r32.  U and HU
+// are stored only as `store` asks (kRitzU | kRitzHU); R always.
+constexpr int kRitzU = 1, kRitzHU = 2;
 struct RitzTask {
     const double *X, *HX; const void *C, *HC;
     int k, mc, c32, r32;
     const double *Y; const double *theta;
     double *U, *HU; void *R;
     long long N;
-    long long part_off; int nch; int pad;
+    long long part_off; int nch; int store;
     double *out; double *gather;
 };
 

@@ -55,6 +55,8 @@ struct BdEngine::Inst {
     bool confirm = false;  // restart requested to confirm convergence on an FP64-fresh HX
     bool vec_x = false;    // the current Ritz block is X (a late restart), not U
     bool late = false;     // this step ends in a late restart
+    bool u_valid = false;  // U holds the Ritz block of the last step
+    bool late_rf = false;  // the late restart refreshes H X in FP64
     BdResult res;
     double theta[kBdMaxK], rn[kBdMaxK];
     std::shared_ptr<void> keepalive;
@@ -113,6 +115,10 @@ BdEngine::BdEngine(int device, hipStream_t s, int precision, int block, int maxs
     arena_[1] = std::make_unique<Arena>();
     const char *tr = std::getenv("PISCES_TRACE");
     trace_ = tr && tr[0] == '1';
+    // PISCES_REFRESH_BELOW=<r>: late restarts of the low precision forms skip
+    // the FP64 refresh of H X while every wanted residual exceeds r (0: never skip)
+    const char *rb = std::getenv("PISCES_REFRESH_BELOW");
+    if (rb) refresh_below_ = std::atof(rb) > 0 ? std::atof(rb) : 1e300;
     gram_load_config();
 }
 
@@ -395,6 +401,13 @@ int BdEngine::step()
         else if (in->mc >= in->mcmax) {
             in->late = true;
             lt.push_back(in);
+            // low precision forms: refresh H X in FP64 once the wanted
+            // residuals come within refresh_below_ of convergence; above it the
+            // HU W recurrence (split-BF16 products, ~1e-5 relative) is far
+            // more accurate than the residuals need
+            double rmin = 1e300;
+            for (int r = 0; r < in->nroots; ++r) rmin = std::min(rmin, in->rn[r]);
+            in->late_rf = lo && !(rmin > refresh_below_);
         }
     }
     const int nr = (int)rs.size();
@@ -403,6 +416,17 @@ int BdEngine::step()
     // ~1e-7), so one pass leaves ~1e-14; the random start block of a new
     // instance takes a second pass.  HX is not formed where the FP64 refresh
     // below recomputes it.
+    // U = X Y_X + C Y_C of the last step where its Ritz step did not store U
+    CombBatch ur;
+    ur.h = A.take<CombTask>(nr, &ur.d);
+    for (Inst *in : rs)
+        if (!in->init && !in->u_valid) {
+            const long long N = in->g.N;
+            const int m = k_ + in->mc;
+            const double *Y = in->sm() + oY;
+            ur.add(CombTask{in->X.p, N, k_, 0, Y, m, in->C.p, N, in->mc, e32, Y + k_, m, in->U.p, N,
+                            0, k_, 0, N});
+        }
     GramBatch rg[2];
     CholTask *rch[2] = {nullptr, nullptr}, *rcd[2] = {nullptr, nullptr};
     CombBatch rc[2];
@@ -524,13 +548,22 @@ int BdEngine::step()
         he[i] = EigTask{in->sm() + oG, kBdMaxM, m, k_, in->sm() + oTh, in->sm() + oY, in->keep(), 0, 0};
         const double *Y = in->sm() + oY;
         const bool late = in->late;
+        // U is read by a restart (a late one at the end of this step, or a
+        // regular one that may follow) and returned at the end; HU only by the
+        // FP64 form's H X recurrence.  A regular restart without a stored U
+        // rebuilds it from X, C and the eigenvectors.
+        const int store = !lo ? (kRitzU | kRitzHU)
+                              : ((late || in->fresh || in->res.n_iter + 1 >= max_iter_) ? kRitzU : 0) |
+                                    ((late && !in->late_rf) ? kRitzHU : 0);
+        in->u_valid = (store & kRitzU) != 0;
         RitzTask t{in->X.as<double>(), in->HX.as<double>(), in->C.p, in->HC.p, k_, in->mc, e32,
                    r32, Y, in->sm() + oTh, in->U.as<double>(), in->HU.as<double>(), in->R.p, N,
-                   res_part, (int)((N + kRitzRows - 1) / kRitzRows), 0, in->sm() + oRn,
+                   res_part, (int)((N + kRitzRows - 1) / kRitzRows), store, in->sm() + oRn,
                    late ? nullptr : dgather_.as<double>() + (size_t)2 * kBdMaxK * i};
         hrz[i] = t;
-        // X, HX, C, HC read; U, HU, R written
-        ritz_bytes += (double)N * (16.0 * k_ + 2.0 * e * in->mc + (16.0 + (r32 ? 4 : 8)) * k_);
+        // X, HX, C, HC read; R and the stored ones of U, HU written
+        ritz_bytes += (double)N * (16.0 * k_ + 2.0 * e * in->mc + (r32 ? 4 : 8) * k_ +
+                                   (((store & kRitzU) ? 8.0 : 0.0) + ((store & kRitzHU) ? 8.0 : 0.0)) * k_);
         res_part += (long long)k_ * t.nch;
         res_nch = std::max(res_nch, t.nch);
     }
@@ -575,10 +608,10 @@ int BdEngine::step()
         lrg.add(in->U.p, k_, 0, in->U.p, k_, 0, N, in->sm() + oGu, k_);
         lch[i] = CholTask{in->sm() + oGu, k_, 0, in->sm() + oWu, nullptr, in->keep(), 1e-12, nullptr};
         lrc.add(comb1(in->U.p, N, k_, 0, in->sm() + oWu, k_, in->X.p, 0, k_, 0));
-        if (!lo) lrc.add(comb1(in->HU.p, N, k_, 0, in->sm() + oWu, k_, in->HX.p, 0, k_, 0));
+        if (!in->late_rf) lrc.add(comb1(in->HU.p, N, k_, 0, in->sm() + oWu, k_, in->HX.p, 0, k_, 0));
         lpc2.add(comb1(in->T.p, N, k_, 0, in->sm() + oI, k_, in->C.p, e32, k_, 0));
         lpc2.add(comb1(in->T2.p, N, k_, 0, in->sm() + oI, k_, in->HC.p, e32, k_, 0));
-        if (lo) {
+        if (in->late_rf) {
             HxJob j{};
             j.nx = in->g.n[0]; j.ny = in->g.n[1]; j.nz = in->g.n[2];
             j.ld = N;
@@ -610,6 +643,7 @@ int BdEngine::step()
     grow_part(sizeof(double) * (size_t)std::max(part, 1LL));
     A.flush(s_);
     double *dp = dpart_.as<double>();
+    ur.launch(s_);
     if (ngk) {
         bd_coef(dcf, ngk, s_);
         pc.launch(s_);
@@ -661,7 +695,9 @@ int BdEngine::step()
         in->restart = false;
     }
     for (Inst *in : lt) {
-        in->fresh = true;
+        // FP64-fresh (convergence may be declared) unless H X came from the
+        // low precision recurrence
+        in->fresh = !lo || in->late_rf;
         in->vec_x = true;
         in->confirm = false;
         in->res.n_restart++;

@@ -91,6 +91,7 @@ private:
     double *hsmall_ = nullptr;
     size_t hsmall_cap_ = 0;
     bool trace_ = false;  // PISCES_TRACE=1: per-step Ritz values of the first instance
+    double refresh_below_ = 1e-3;  // late restarts refresh H X in FP64 below this residual
     void grow_part(size_t bytes);
     std::vector<dbuf> pool_;  // released instance buffers, reused by add()
     dbuf take_buf(size_t bytes);
