@@ -411,6 +411,7 @@ __global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ t
         if (t.mode == 1) v = ld_any(t.Y, yi, t.y32) - v;
         else if (t.mode == 2) v = ld_any(t.Y, yi, t.y32) + v;
         st_any(t.Y, yi, t.y32, v);
+        if (t.Y2) st_any(t.Y2, yi, t.y2_32, v);
     }
 }
 
@@ -720,20 +721,24 @@ __global__ void __launch_bounds__(64) ritz_reduce_kernel(const RitzTask *__restr
 // ------------------------------------------------------------------ orthogonalisation factor
 // One 64-lane workgroup per task: S = G - h^T h (h may be absent), Cholesky
 // with dropping, W = L^{-T}, F = -h W.  Shared by the Cholesky-QR factor.
-__device__ void orth_factor(const double *__restrict__ hg, const double *__restrict__ G, int m,
-                            int nt, double *__restrict__ Wo, double *__restrict__ Fo,
-                            const int *kin, int *kout, int *kcopy, double tol)
+__device__ void orth_factor(const double *__restrict__ hg, int hld, const double *__restrict__ G,
+                            int gld, int m, int nt, double *__restrict__ Wo,
+                            double *__restrict__ Fo, const int *kin, int *kout, int *kcopy,
+                            double tol)
 {
     constexpr int K = kBdMaxK;
     __shared__ double h[kBdMaxM * kBdMaxK];
     __shared__ double S[K * K], L[K * K], W[K * K];
     __shared__ int keep[K];
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < m * nt; idx += 64) h[idx] = hg[idx];
+    for (int idx = tid; idx < m * nt; idx += 64) {
+        const int a = idx % m, c = idx / m;
+        h[idx] = hg[a + (long long)hld * c];
+    }
     __syncthreads();
     for (int idx = tid; idx < nt * nt; idx += 64) {
         const int i = idx % nt, j = idx / nt;
-        double v = G[i + (long long)nt * j];
+        double v = G[i + (long long)gld * j];
         for (int a = 0; a < m; ++a) v -= h[a + m * i] * h[a + m * j];
         S[i + K * j] = v;
         L[i + K * j] = 0.0;
@@ -743,7 +748,7 @@ __device__ void orth_factor(const double *__restrict__ hg, const double *__restr
         if (tid == 0) {
             const double d = S[j + K * j];
             const int ki = kin ? kin[j] : 1;
-            const int kj = (ki && d > tol * G[j + (long long)nt * j] && d > 0.0) ? 1 : 0;
+            const int kj = (ki && d > tol * G[j + (long long)gld * j] && d > 0.0) ? 1 : 0;
             keep[j] = kj;
             L[j + K * j] = kj ? sqrt(d) : 0.0;
         }
@@ -794,13 +799,14 @@ __device__ void orth_factor(const double *__restrict__ hg, const double *__restr
 __global__ void __launch_bounds__(64) orth_kernel(const OrthTask *__restrict__ tasks)
 {
     const OrthTask &t = tasks[blockIdx.x];
-    orth_factor(t.h, t.G, t.m, t.nt, t.W, t.F, t.keep_in, t.keep_out, t.keep_copy, t.tol);
+    orth_factor(t.h, t.hld, t.G, t.gld, t.m, t.nt, t.W, t.F, t.keep_in, t.keep_out, t.keep_copy,
+                t.tol);
 }
 
 __global__ void __launch_bounds__(64) cholinv_batched_kernel(const CholTask *__restrict__ tasks)
 {
     const CholTask &t = tasks[blockIdx.x];
-    orth_factor(nullptr, t.G, 0, t.k, t.W, nullptr, t.keep_in, t.keep_out, t.keep_copy, t.tol);
+    orth_factor(nullptr, 0, t.G, t.k, 0, t.k, t.W, nullptr, t.keep_in, t.keep_out, t.keep_copy, t.tol);
 }
 
 // ------------------------------------------------------------------ correction
@@ -814,7 +820,7 @@ __global__ void __launch_bounds__(kT) prec_kernel(const PrecTask *__restrict__ t
         const int r = t.src[j];
         double dv = t.theta[r] - dg;
         if (fabs(dv) < 1e-12) dv = 1e-12;
-        t.T[j * t.N + g] = t.R[r * t.ld + g] / dv;
+        st_any(t.T, j * t.N + g, t.t32, t.R[r * t.ld + g] / dv);
     }
 }
 

@@ -11,6 +11,7 @@ namespace pisces {
 
 constexpr int kBdMaxM = 64;   // max subspace dimension (block + corrections)
 constexpr int kBdMaxK = 16;   // max block size
+constexpr int kGramMaxB = 16; // max B columns of one Gram task (A: kBdMaxM)
 
 // partial Gram sums of A^T B over one chunk of rows; out[a + out_ld * b]
 struct GramTask {
@@ -24,13 +25,17 @@ struct GramTask {
 };
 
 // Y (set|-=|+=) A1 W1 + A2 W2 + A3 W3, W in device memory (column-major);
-// the third source is optional (ma3 = 0)
+// the third source is optional (ma3 = 0).  Y2 (optional, mode 0 only, leading
+// dimension ldy) receives a second copy of Y, FP32 if y2_32.  Every row is
+// read in full before it is written, so Y may be one of the sources (the
+// combination is then done in place).
 struct CombTask {
     const void *A1; long long lda1; int ma1; int a1_32; const double *W1; int ldw1;
     const void *A2; long long lda2; int ma2; int a2_32; const double *W2; int ldw2;
     void *Y; long long ldy; int y32; int mb; int mode;
     long long N;
     const void *A3; long long lda3; int ma3; int a3_32; const double *W3; int ldw3;
+    void *Y2; int y2_32; int pad;
 };
 
 // Ritz vectors and residuals of one instance in one pass over the basis:
@@ -50,15 +55,17 @@ struct RitzTask {
 };
 
 // Factor of one orthogonalisation round of a new block T (nt columns) against
-// an orthonormal basis V (m columns), from h = V^T T (m x nt, ld m) and
-// G = T^T T (nt x nt, ld nt): S = G - h^T h = L L^T (columns whose pivot falls
-// below tol * G_jj, or that keep_in marks dead, are dropped), W = L^{-T},
-// F = -h W; then (T - V h) W = T W + V F has orthonormal columns orthogonal to V.
+// an orthonormal basis V (m columns), from h = V^T T (m x nt, leading
+// dimension hld) and G = T^T T (nt x nt, leading dimension gld): S = G - h^T h
+// = L L^T (columns whose pivot falls below tol * G_jj, or that keep_in marks
+// dead, are dropped), W = L^{-T}, F = -h W (m x nt, ld m); then (T - V h) W =
+// T W + V F has orthonormal columns orthogonal to V.
 struct OrthTask {
     const double *h; const double *G; int m; int nt;
     double *W; double *F;
     const int *keep_in; int *keep_out; int *keep_copy;
     double tol;
+    int hld, gld;
 };
 
 // Symmetric eigenproblem of the projected matrix: G (m x m, upper triangle,
@@ -84,7 +91,7 @@ struct ResTask {
 // T[t] = R[src[t]] / (theta[src[t]] - D), |.| >= 1e-12 (Davidson correction)
 struct PrecTask {
     const double *R; long long ld; const double *theta; const double *D;
-    double *T; long long N; int nt; int src[kBdMaxK];
+    void *T; long long N; int nt; int src[kBdMaxK]; int t32;  // T FP32 if t32
 };
 
 // W = L^{-T} of the k x k Gram G (CholQR).  keep_out[c] = 0 marks a column

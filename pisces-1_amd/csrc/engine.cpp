@@ -41,13 +41,18 @@ constexpr int oWu = 7040;    // restart factor (k x k)
 constexpr int oQp = 7552;    // restart search directions in [X|C] coefficients (m x k)
 constexpr int oI = 8576;     // identity (k x k)
 constexpr int oF = 8832;     // orthogonalisation coefficients on [X|C] ((k + mc) x nt)
-constexpr int kSmall = 9856;
+constexpr int oHG = 9856;    // [Xc|C|T]^T T of an orthogonalisation round ((k + mc + nt) x nt)
+constexpr int kSmall = oHG + (kBdMaxM + kBdMaxK) * kBdMaxK;
 }  // namespace
 
 struct BdEngine::Inst {
     BdGrid g;
     int nroots = 0, k = 0, mcmax = 0;
-    dbuf X, HX, U, HU, R, T, T2, C, HC, small;
+    // C holds [Xc | corrections]: Xc = X in the corrections' element type (kept
+    // with X at every restart), so that V = [Xc | C] and a new block T written
+    // behind the corrections are one contiguous column block
+    dbuf X, HX, U, HU, R, T2, C, HC, small;
+    size_t es = 8;  // element size of C, HC
     int mc = 0, nnew = 0, gnew = 0, nt = 0;
     int src[kBdMaxK];
     bool restart = true, init = true, refresh = false, fresh = false, done = false;
@@ -61,6 +66,8 @@ struct BdEngine::Inst {
     double theta[kBdMaxK], rn[kBdMaxK];
     std::shared_ptr<void> keepalive;
     double *sm() const { return small.as<double>(); }
+    void *xc() const { return C.p; }
+    void *corr(int c) const { return C.as<unsigned char>() + es * g.N * (size_t)(k + c); }
     int *keep() const { return reinterpret_cast<int *>(small.as<double>() + oKeep); }
     int *keepT() const { return reinterpret_cast<int *>(small.as<double>() + oKeepT); }
 };
@@ -132,7 +139,7 @@ size_t BdEngine::bytes_per_instance(long long N) const
 {
     const size_t e = (prec_ == PISCES_FP64) ? 8 : 4;
     const size_t mcmax = (size_t)(maxsub_ - k_);
-    return (size_t)N * (7 * k_ * 8 + 2 * mcmax * e) + kSmall * 8;
+    return (size_t)N * (6 * k_ * 8 + (2 * mcmax + k_) * e) + kSmall * 8;
 }
 
 int BdEngine::live() const
@@ -174,9 +181,10 @@ int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, 
     in->keepalive = std::move(keepalive);
     const long long N = g.N;
     const size_t e = (prec_ == PISCES_FP64) ? 8 : 4;
-    for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->R, &in->T, &in->T2})
+    for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->R, &in->T2})
         *b = take_buf(8 * N * k_);
-    in->C = take_buf(e * N * in->mcmax);
+    in->es = e;
+    in->C = take_buf(e * N * (in->mcmax + k_));
     in->HC = take_buf(e * N * in->mcmax);
     in->small = take_buf(8 * kSmall);
     // start vectors in U (the first restart orthonormalises them): the given
@@ -241,7 +249,7 @@ void BdEngine::release(int slot)
     // buffers go to the pool (no hipFree: it would stall the device); they are
     // reused by later add() calls, in stream order
     Inst *in = inst_.at(slot).get();
-    for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->R, &in->T, &in->T2,
+    for (dbuf *b : {&in->X, &in->HX, &in->U, &in->HU, &in->R, &in->T2,
                     &in->C, &in->HC, &in->small})
         if (b->p) pool_.push_back(std::move(*b));
     inst_.at(slot).reset();
@@ -260,6 +268,8 @@ struct GramBatch {
     void add(const void *A, int ma, int a32, const void *B, int mb, int b32, long long N,
              double *out, int out_ld)
     {
+        PC_REQUIRE(ma >= 1 && ma <= kBdMaxM && mb >= 1 && mb <= kGramMaxB, PISCES_ERR_ARG,
+                   "Gram task exceeds the kernel's column limits");
         h[n++] = GramTask{A, N, ma, a32, B, N, mb, b32, N, out, out_ld, 0, 0, 0};
         max_mamb = std::max(max_mamb, ma * mb);
     }
@@ -424,8 +434,8 @@ int BdEngine::step()
             const long long N = in->g.N;
             const int m = k_ + in->mc;
             const double *Y = in->sm() + oY;
-            ur.add(CombTask{in->X.p, N, k_, 0, Y, m, in->C.p, N, in->mc, e32, Y + k_, m, in->U.p, N,
-                            0, k_, 0, N});
+            ur.add(CombTask{in->X.p, N, k_, 0, Y, m, in->corr(0), N, in->mc, e32, Y + k_, m, in->U.p,
+                            N, 0, k_, 0, N});
         }
     GramBatch rg[2];
     CholTask *rch[2] = {nullptr, nullptr}, *rcd[2] = {nullptr, nullptr};
@@ -444,7 +454,10 @@ int BdEngine::step()
             rch[pass][nch2[pass]++] = CholTask{in->sm() + oGu, k_, 0, in->sm() + oWu,
                                                pass ? in->keep() : nullptr, in->keep(), 1e-12,
                                                nullptr};
-            rc[pass].add(comb1(src, N, k_, 0, in->sm() + oWu, k_, in->X.p, 0, k_, 0));
+            CombTask cx = comb1(src, N, k_, 0, in->sm() + oWu, k_, in->X.p, 0, k_, 0);
+            cx.Y2 = in->xc();
+            cx.y2_32 = e32;
+            rc[pass].add(cx);
             const bool refresh = in->init || lo || in->confirm;
             if (!refresh) rc[pass].add(comb1(in->HU.as<double>(), N, k_, 0, in->sm() + oWu, k_,
                                              in->HX.p, 0, k_, 0));
@@ -461,21 +474,19 @@ int BdEngine::step()
     const int ngk = (int)gk.size();
     CoefTask *hcf = nullptr, *dcf = nullptr;
     hcf = A.take<CoefTask>(ngk, &dcf);
-    CombBatch pc, pc2;
+    CombBatch pc;
     pc.h = A.take<CombTask>(2 * ngk, &pc.d);
-    pc2.h = A.take<CombTask>(2 * ngk, &pc2.d);
     for (int i = 0; i < ngk; ++i) {
         Inst *in = gk[i];
         const long long N = in->g.N;
         const int m = k_ + in->mc;  // basis of the last Ritz step
         double *Q = in->sm() + oQp;
         hcf[i] = CoefTask{in->sm() + oY, m, k_, Q, in->keep() + k_};
-        pc.add(CombTask{in->X.p, N, k_, 0, Q, m, in->C.p, N, in->mc, e32, Q + k_, m, in->T.p, N,
-                        0, k_, 0, N});
-        pc.add(CombTask{in->HX.p, N, k_, 0, Q, m, in->HC.p, N, in->mc, e32, Q + k_, m, in->T2.p,
-                        N, 0, k_, 0, N});
-        pc2.add(comb1(in->T.p, N, k_, 0, in->sm() + oI, k_, in->C.p, e32, k_, 0));
-        pc2.add(comb1(in->T2.p, N, k_, 0, in->sm() + oI, k_, in->HC.p, e32, k_, 0));
+        // in place: C[0..k) = [X|C] Q, HC[0..k) = [HX|HC] Q
+        pc.add(CombTask{in->X.p, N, k_, 0, Q, m, in->corr(0), N, in->mc, e32, Q + k_, m,
+                        in->corr(0), N, e32, k_, 0, N});
+        pc.add(CombTask{in->HX.p, N, k_, 0, Q, m, in->HC.p, N, in->mc, e32, Q + k_, m, in->HC.p,
+                        N, e32, k_, 0, N});
     }
     for (Inst *in : rs) {
         in->refresh = in->init || lo || in->confirm;
@@ -512,7 +523,7 @@ int BdEngine::step()
             j.nvec = in->nnew;
             for (int a = 0; a < 3; ++a) j.ax[a] = in->g.ax[a];
             j.V = lo ? (const void *)in->g.V32 : (const void *)in->g.V64;
-            j.X = in->C.as<unsigned char>() + e * in->g.N * (in->mc - in->nnew);
+            j.X = in->corr(in->mc - in->nnew);
             j.Y = in->HC.as<unsigned char>() + e * in->g.N * (in->mc - in->nnew);
             hc.add(j);
             if (lo) in->res.n_mvlo += in->nnew;
@@ -521,17 +532,21 @@ int BdEngine::step()
     }
     // new columns of G
     GramBatch gg;
-    gg.h = A.take<GramTask>(3 * n, &gg.d);
+    int ngg = 0;
+    for (Inst *in : act) ngg += 1 + 2 * ((in->gnew + kGramMaxB - 1) / kGramMaxB);
+    gg.h = A.take<GramTask>(ngg, &gg.d);
     for (Inst *in : act) {
         const long long N = in->g.N;
         if (in->fresh) gg.add(in->X.p, k_, 0, in->HX.p, k_, 0, N, in->sm() + oG, kBdMaxM);
-        if (in->gnew > 0) {
-            const int c0 = in->mc - in->gnew;
-            const void *Cn = in->C.as<unsigned char>() + e * N * c0;
+        // new columns in pieces of at most kGramMaxB (after a late restart they
+        // are the GD+k block and the new corrections together)
+        for (int c0 = in->mc - in->gnew; c0 < in->mc; c0 += kGramMaxB) {
+            const int nb = std::min(kGramMaxB, in->mc - c0);
+            const void *Cn = in->corr(c0);
             const void *HCn = in->HC.as<unsigned char>() + e * N * c0;
             double *col = in->sm() + oG + (size_t)kBdMaxM * (k_ + c0);
-            gg.add(in->HX.p, k_, 0, Cn, in->gnew, e32, N, col, kBdMaxM);
-            gg.add(in->C.p, in->mc, e32, HCn, in->gnew, e32, N, col + k_, kBdMaxM);
+            gg.add(in->HX.p, k_, 0, Cn, nb, e32, N, col, kBdMaxM);
+            gg.add(in->corr(0), in->mc, e32, HCn, nb, e32, N, col + k_, kBdMaxM);
         }
     }
     EigTask *he, *de;
@@ -556,7 +571,7 @@ int BdEngine::step()
                               : ((late || in->fresh || in->res.n_iter + 1 >= max_iter_) ? kRitzU : 0) |
                                     ((late && !in->late_rf) ? kRitzHU : 0);
         in->u_valid = (store & kRitzU) != 0;
-        RitzTask t{in->X.as<double>(), in->HX.as<double>(), in->C.p, in->HC.p, k_, in->mc, e32,
+        RitzTask t{in->X.as<double>(), in->HX.as<double>(), in->corr(0), in->HC.p, k_, in->mc, e32,
                    r32, Y, in->sm() + oTh, in->U.as<double>(), in->HU.as<double>(), in->R.p, N,
                    res_part, (int)((N + kRitzRows - 1) / kRitzRows), store, in->sm() + oRn,
                    late ? nullptr : dgather_.as<double>() + (size_t)2 * kBdMaxK * i};
@@ -586,10 +601,9 @@ int BdEngine::step()
     GramBatch lrg, lgx;
     lrg.h = A.take<GramTask>(nl, &lrg.d);
     lgx.h = A.take<GramTask>(nl, &lgx.d);
-    CombBatch lpc, lrc, lpc2;
+    CombBatch lpc, lrc;
     lpc.h = A.take<CombTask>(2 * nl, &lpc.d);
     lrc.h = A.take<CombTask>(2 * nl, &lrc.d);
-    lpc2.h = A.take<CombTask>(2 * nl, &lpc2.d);
     HxBatch lhr;
     lhr.h = A.take<HxJob>(nl, &lhr.d);
     long long lres_part = res_part;
@@ -601,16 +615,17 @@ int BdEngine::step()
         const int m = k_ + in->mc;
         double *Q = in->sm() + oQp;
         lcf[i] = CoefTask{in->sm() + oY, m, k_, Q, in->keep() + k_};
-        lpc.add(CombTask{in->X.p, N, k_, 0, Q, m, in->C.p, N, in->mc, e32, Q + k_, m, in->T.p, N,
-                         0, k_, 0, N});
-        lpc.add(CombTask{in->HX.p, N, k_, 0, Q, m, in->HC.p, N, in->mc, e32, Q + k_, m, in->T2.p,
-                         N, 0, k_, 0, N});
+        lpc.add(CombTask{in->X.p, N, k_, 0, Q, m, in->corr(0), N, in->mc, e32, Q + k_, m,
+                         in->corr(0), N, e32, k_, 0, N});
+        lpc.add(CombTask{in->HX.p, N, k_, 0, Q, m, in->HC.p, N, in->mc, e32, Q + k_, m, in->HC.p,
+                         N, e32, k_, 0, N});
         lrg.add(in->U.p, k_, 0, in->U.p, k_, 0, N, in->sm() + oGu, k_);
         lch[i] = CholTask{in->sm() + oGu, k_, 0, in->sm() + oWu, nullptr, in->keep(), 1e-12, nullptr};
-        lrc.add(comb1(in->U.p, N, k_, 0, in->sm() + oWu, k_, in->X.p, 0, k_, 0));
+        CombTask cx = comb1(in->U.p, N, k_, 0, in->sm() + oWu, k_, in->X.p, 0, k_, 0);
+        cx.Y2 = in->xc();
+        cx.y2_32 = e32;
+        lrc.add(cx);
         if (!in->late_rf) lrc.add(comb1(in->HU.p, N, k_, 0, in->sm() + oWu, k_, in->HX.p, 0, k_, 0));
-        lpc2.add(comb1(in->T.p, N, k_, 0, in->sm() + oI, k_, in->C.p, e32, k_, 0));
-        lpc2.add(comb1(in->T2.p, N, k_, 0, in->sm() + oI, k_, in->HC.p, e32, k_, 0));
         if (in->late_rf) {
             HxJob j{};
             j.nx = in->g.n[0]; j.ny = in->g.n[1]; j.nz = in->g.n[2];
@@ -656,7 +671,6 @@ int BdEngine::step()
         }
         rc[pass].launch(s_);
     }
-    if (ngk) pc2.launch(s_);
     hr.launch(PISCES_FP64, s_);
     hc.launch(prec_, s_);
     gg.launch(dp, s_);
@@ -677,7 +691,6 @@ int BdEngine::step()
             bd_cholinv(dlch, nl, s_);
         }
         lrc.launch(s_);
-        lpc2.launch(s_);
         lhr.launch(PISCES_FP64, s_);
         lgx.launch(dp, s_);
         ProfScope ps(PK_RESID, s_, lres_bytes);
@@ -768,15 +781,15 @@ int BdEngine::step()
     if (corr.empty()) return this->live();
 
     // ============================ phase B: corrections
-    // precondition the residual columns into T, then two orthogonalisation
-    // rounds against V = [X | C] and within the block, each one Gram launch
-    // ([X|C|T]^T T), one factor kernel and one combination T W + V F; the
-    // second round writes C[mc .. mc + nt)
+    // precondition the residual columns into T = C[mc .. mc + nt), then two
+    // orthogonalisation rounds against V = [Xc | C[0 .. mc)] and within the
+    // block, each one Gram task ([Xc|C|T]^T T: T is read from memory once),
+    // one factor kernel and one in-place combination T = T W + V F
     Arena &B = *arena_[1];
     B.used = 0;
     const int nc = (int)corr.size();
-    const int t32 = r32;
-    const size_t et = t32 ? 4 : 8;
+    const int t32 = e32;
+    const size_t et = e;
     PrecTask *hp = nullptr, *dpc = nullptr;
     FdmTask *hf = nullptr, *dfd = nullptr;
     int nfd = 0, fd_maxn = 0;
@@ -787,7 +800,7 @@ int BdEngine::step()
             Inst *in = corr[i];
             PrecTask t{};
             t.R = in->R.as<double>(); t.ld = in->g.N; t.theta = in->sm() + oTh; t.D = in->g.D;
-            t.T = in->T.as<double>(); t.N = in->g.N; t.nt = in->nt;
+            t.T = in->corr(in->mc); t.N = in->g.N; t.nt = in->nt; t.t32 = t32;
             std::memcpy(t.src, in->src, sizeof(in->src));
             hp[i] = t;
             maxN = std::max(maxN, in->g.N);
@@ -803,10 +816,9 @@ int BdEngine::step()
                 FdmTask f{};
                 for (int a = 0; a < 3; ++a) f.ax[a] = in->g.ax[a];
                 f.src = in->R.as<unsigned char>() + (r32 ? 4 : 8) * in->g.N * in->src[t];
-                f.dst = in->T.as<unsigned char>() + et * in->g.N * t;
+                f.dst = in->corr(in->mc + t);
                 f.src32 = r32;
                 f.dst32 = t32;
-                // T2 is free until the first orthogonalisation round writes it:
                 // FP32 scratch of the split-BF16 form
                 f.work = reinterpret_cast<float *>(in->T2.p) + in->g.N * t;
                 // t = -(T - theta)^{-1} r
@@ -821,26 +833,37 @@ int BdEngine::step()
     CombBatch oc[2];
     OrthTask *hot[2], *dot[2];
     for (int pass = 0; pass < 2; ++pass) {
-        og[pass].h = B.take<GramTask>(3 * nc, &og[pass].d);
+        og[pass].h = B.take<GramTask>(2 * nc, &og[pass].d);
         oc[pass].h = B.take<CombTask>(nc, &oc[pass].d);
         hot[pass] = B.take<OrthTask>(nc, &dot[pass]);
         for (int i = 0; i < nc; ++i) {
             Inst *in = corr[i];
             const long long N = in->g.N;
             const int m = k_ + in->mc, nt = in->nt;
-            double *h = in->sm() + oH, *G = in->sm() + oGt, *W = in->sm() + oWc, *F = in->sm() + oF;
-            const void *src = pass == 0 ? in->T.p : in->T2.p;
-            void *dst = pass == 0 ? in->T2.p : (void *)(in->C.as<unsigned char>() + e * N * in->mc);
-            const int dst32 = pass == 0 ? t32 : e32;
-            og[pass].add(in->X.p, k_, 0, src, nt, t32, N, h, m);
-            if (in->mc > 0) og[pass].add(in->C.p, in->mc, e32, src, nt, t32, N, h + k_, m);
-            og[pass].add(src, nt, t32, src, nt, t32, N, G, nt);
+            double *W = in->sm() + oWc, *F = in->sm() + oF;
+            void *T = in->corr(in->mc);
+            const double *h, *G;
+            int hld, gld;
+            if (m + nt <= kBdMaxM) {
+                double *hg = in->sm() + oHG;
+                og[pass].add(in->xc(), m + nt, e32, T, nt, e32, N, hg, m + nt);
+                h = hg;
+                G = hg + m;
+                hld = gld = m + nt;
+            } else {
+                double *hv = in->sm() + oH, *gt = in->sm() + oGt;
+                og[pass].add(in->xc(), m, e32, T, nt, e32, N, hv, m);
+                og[pass].add(T, nt, e32, T, nt, e32, N, gt, nt);
+                h = hv;
+                G = gt;
+                hld = m;
+                gld = nt;
+            }
             hot[pass][i] = OrthTask{h, G, m, nt, W, F, pass ? in->keepT() : nullptr, in->keepT(),
                                     pass ? in->keep() + k_ + in->mc : nullptr,
-                                    pass ? 1e-10 : 1e-13};
-            CombTask ct{src, N, nt, t32, W, nt, in->X.p, N, k_, 0, F, m, dst, N, dst32, nt, 0, N,
-                        in->C.p, N, in->mc, e32, F + k_, m};
-            oc[pass].add(ct);
+                                    pass ? 1e-10 : 1e-13, hld, gld};
+            oc[pass].add(CombTask{T, N, nt, t32, W, nt, in->xc(), N, m, e32, F, m, T, N, t32, nt, 0,
+                                  N});
         }
     }
     long long partB = 1;

@@ -83,3 +83,6 @@ def test_c3_mixed_within_1e6_of_fp64(pisces):
             print(prec, r["evals"] - r64["evals"], r["n_iter"], r["seconds"], flush=True)
             assert r["nconv"] == 8, r
             np.testing.assert_allclose(r["evals"], r64["evals"], rtol=0, atol=1e-6)
+            # the mixed forms follow the FP64 iteration closely (block 12 exercises
+            # Gram tasks split at 16 columns after late restarts)
+            assert r["n_iter"] <= 1.3 * r64["n_iter"] + 5, (r["n_iter"], r64["n_iter"])
