@@ -286,13 +286,18 @@ struct GramBatch {
             max_blocks = std::max(max_blocks, t.nch * t.cpb);
         }
     }
+    // bytes read once: B is not counted again where its columns lie inside A
     double bytes() const
     {
         double b = 0;
         for (int i = 0; i < n; ++i) {
             const GramTask &t = h[i];
-            b += (double)t.N * t.ma * (t.a32 ? 4 : 8);
-            if (t.B != t.A) b += (double)t.N * t.mb * (t.b32 ? 4 : 8);
+            const size_t ea = t.a32 ? 4 : 8;
+            b += (double)t.N * t.ma * ea;
+            const unsigned char *a0 = static_cast<const unsigned char *>(t.A);
+            const unsigned char *bp = static_cast<const unsigned char *>(t.B);
+            const bool inside = bp >= a0 && bp < a0 + ea * (size_t)t.lda * t.ma;
+            if (!inside) b += (double)t.N * t.mb * (t.b32 ? 4 : 8);
         }
         return b;
     }
