@@ -238,6 +238,7 @@ __global__ void __launch_bounds__(kT) gram_valu_kernel(const GramTask *__restric
                                                        double *__restrict__ partial, int rows)
 {
     const GramTask t = tasks[blockIdx.y];
+    if (t.kind != 0) return;
     const int ntile = t.cpb;  // tiles of this task
     if ((int)blockIdx.x >= t.nch * ntile) return;
     const int chunk = blockIdx.x / ntile, tile = blockIdx.x - chunk * ntile;
@@ -282,7 +283,7 @@ __global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ t
                                                   double *__restrict__ partial, int rows)
 {
     const GramTask &t = tasks[blockIdx.y];
-    if ((int)blockIdx.x >= t.nch) return;
+    if (t.kind != 0 || (int)blockIdx.x >= t.nch) return;
     const long long r_beg = (long long)blockIdx.x * rows;
     const long long r_end = min(r_beg + rows, t.N);
     // 16-byte loads need every column start aligned: uniform per task
@@ -331,6 +332,108 @@ __global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ t
     }
 }
 
+// ------------------------------------------------------------------ Gram, narrow B
+// Tasks with at most kGNarrowB columns of B (a block of corrections against the
+// basis).  On the FP64 matrix cores 3/4 of every 16-wide B tile would be empty,
+// and at the sustained FP64 MFMA rate the padded products cost about as much
+// time as the memory stream; on the VALU the 8 x 4 products of a row are ~1/6
+// of it.  Workgroup = one row chunk x one group of kGNarrowA A columns; each
+// thread streams rows tid, tid + 256, ... (coalesced columns, two rows of loads
+// in flight) and keeps its 32 products in registers; a butterfly
+// reduce-scatter over the wave and a sum over the four waves through LDS give
+// the chunk's partials.
+template <typename TA, typename TB>
+__device__ __forceinline__ void gram_rows_narrow(gptr<const TA> A, long long lda, int na,
+                                                 gptr<const TB> B, long long ldb, int nb,
+                                                 long long r0, long long r1,
+                                                 double (&acc)[kGNarrowA * kGNarrowB])
+{
+    // columns past na / nb read column 0 and are weighted 0
+    double wa[kGNarrowA], wb[kGNarrowB];
+#pragma unroll
+    for (int i = 0; i < kGNarrowA; ++i) wa[i] = i < na ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = 0; j < kGNarrowB; ++j) wb[j] = j < nb ? 1.0 : 0.0;
+#pragma unroll 2
+    for (long long g = r0 + threadIdx.x; g < r1; g += kT) {
+        double a[kGNarrowA], b[kGNarrowB];
+#pragma unroll
+        for (int i = 0; i < kGNarrowA; ++i) a[i] = (double)A[g + (i < na ? i : 0) * lda];
+#pragma unroll
+        for (int j = 0; j < kGNarrowB; ++j) b[j] = (double)B[g + (j < nb ? j : 0) * ldb];
+#pragma unroll
+        for (int i = 0; i < kGNarrowA; ++i)
+#pragma unroll
+            for (int j = 0; j < kGNarrowB; ++j)
+                acc[i * kGNarrowB + j] = fma(a[i] * wa[i], b[j] * wb[j], acc[i * kGNarrowB + j]);
+    }
+}
+
+template <int OFF, int NV>
+__device__ __forceinline__ void rs_step_n(double (&v)[NV], int lane)
+{
+    const bool up = (lane & OFF) != 0;
+#pragma unroll
+    for (int k = 0; k < OFF; ++k) {
+        const double keep = up ? v[k + OFF] : v[k];
+        const double give = up ? v[k] : v[k + OFF];
+        v[k] = keep + __shfl_xor(give, OFF, 64);
+    }
+}
+
+__global__ void __launch_bounds__(kT) gram_narrow_kernel(const GramTask *__restrict__ tasks,
+                                                         double *__restrict__ partial)
+{
+    constexpr int NV = kGNarrowA * kGNarrowB;
+    static_assert(NV == 32, "reduce-scatter of 32 values");
+    const GramTask t = tasks[blockIdx.y];
+    if (t.kind != 1) return;
+    const int ntile = t.cpb;
+    if ((int)blockIdx.x >= t.nch * ntile) return;
+    const int chunk = blockIdx.x / ntile, tile = blockIdx.x - chunk * ntile;
+    const int a0 = tile * kGNarrowA;
+    const int na = min(kGNarrowA, t.ma - a0), nb = t.mb;
+    const long long r0 = (long long)chunk * kGRowsNarrow;
+    const long long r1 = min(r0 + kGRowsNarrow, t.N);
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    const size_t ea = t.a32 ? 4 : 8;
+    const void *Aa = static_cast<const unsigned char *>(t.A) + ea * (size_t)t.lda * a0;
+    const gptr<const float> Bf = glb(static_cast<const float *>(t.B));
+    const gptr<const double> Bd = glb(static_cast<const double *>(t.B));
+    if (t.a32) {
+        const gptr<const float> A = glb(static_cast<const float *>(Aa));
+        if (t.b32) gram_rows_narrow(A, t.lda, na, Bf, t.ldb, nb, r0, r1, acc);
+        else gram_rows_narrow(A, t.lda, na, Bd, t.ldb, nb, r0, r1, acc);
+    } else {
+        const gptr<const double> A = glb(static_cast<const double *>(Aa));
+        if (t.b32) gram_rows_narrow(A, t.lda, na, Bf, t.ldb, nb, r0, r1, acc);
+        else gram_rows_narrow(A, t.lda, na, Bd, t.ldb, nb, r0, r1, acc);
+    }
+    // lane L ends with the sum over its half-wave of value L % 32, then over both halves
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    rs_step_n<16>(acc, lane);
+    rs_step_n<8>(acc, lane);
+    rs_step_n<4>(acc, lane);
+    rs_step_n<2>(acc, lane);
+    rs_step_n<1>(acc, lane);
+    const double w = acc[0] + __shfl_xor(acc[0], 32, 64);
+    __shared__ double red[kT / 64][NV];
+    if (lane < NV) red[wv][lane] = w;
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < kT / 64; ++q) sum += red[q][threadIdx.x];
+        const int i = threadIdx.x / kGNarrowB, j = threadIdx.x - i * kGNarrowB;
+        if (i < na && j < nb) {
+            const long long o = (long long)(a0 + i) + (long long)t.ma * j;
+            partial[t.part_off + o * t.nch + chunk] = sum;
+        }
+    }
+}
+
 // one thread per Gram entry: sum of its chunk partials (fixed order)
 __global__ void __launch_bounds__(kT) gram_reduce_kernel(const GramTask *__restrict__ tasks,
                                                          const double *__restrict__ partial)
@@ -345,13 +448,27 @@ __global__ void __launch_bounds__(kT) gram_reduce_kernel(const GramTask *__restr
 }
 
 // ------------------------------------------------------------------ combine
-// acc[b] += sum_a A[a*lda + g] w[(a0 + a) + ma*b]: four column loads in flight
+// acc[b] += sum_a A[a*lda + g] w[(a0 + a) + ma*b]: eight column loads in flight
 template <typename T>
 __device__ __forceinline__ void comb_cols(gptr<const T> A, long long lda, int na,
                                           const double *w, int ma, int a0, int mb, long long g,
                                           double (&acc)[kBdMaxK])
 {
     int a = 0;
+    for (; a + 8 <= na; a += 8) {
+        double x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = A[(a + q) * lda + g];
+#pragma unroll
+        for (int b = 0; b < kBdMaxK; ++b) {
+            if (b >= mb) break;
+            const double *wb = w + a0 + a + ma * b;
+            double s = acc[b];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s = fma(x[q], wb[q], s);
+            acc[b] = s;
+        }
+    }
     for (; a + 4 <= na; a += 4) {
         const double x0 = A[a * lda + g], x1 = A[(a + 1) * lda + g];
         const double x2 = A[(a + 2) * lda + g], x3 = A[(a + 3) * lda + g];
@@ -610,25 +727,32 @@ __global__ void __launch_bounds__(64) resid_reduce_kernel(const ResTask *__restr
 }
 
 // ------------------------------------------------------------------ Ritz vectors + residuals
-// u[b] += sum_a A[a*N + g] y[(a0 + a)][b], hu likewise with HA: two columns
-// (four loads) in flight
+// u[b] += sum_a A[a*N + g] y[(a0 + a)][b], hu likewise with HA: four columns
+// (eight loads) in flight
 template <int KB, typename T>
 __device__ __forceinline__ void ritz_cols(gptr<const T> A, gptr<const T> HA,
                                           long long N, int na, const double *y, int a0,
                                           long long g, double (&u)[KB], double (&hu)[KB])
 {
     int a = 0;
-    for (; a + 2 <= na; a += 2) {
-        const double x0 = A[a * N + g], h0 = HA[a * N + g];
-        const double x1 = A[(a + 1) * N + g], h1 = HA[(a + 1) * N + g];
-        const double *w0 = y + (a0 + a) * KB, *w1 = w0 + KB;
+    for (; a + 4 <= na; a += 4) {
+        double x[4], h[4];
 #pragma unroll
-        for (int b = 0; b < KB; ++b) {
-            u[b] += x0 * w0[b] + x1 * w1[b];
-            hu[b] += h0 * w0[b] + h1 * w1[b];
+        for (int q = 0; q < 4; ++q) {
+            x[q] = A[(a + q) * N + g];
+            h[q] = HA[(a + q) * N + g];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double *wq = y + (a0 + a + q) * KB;
+#pragma unroll
+            for (int b = 0; b < KB; ++b) {
+                u[b] = fma(x[q], wq[b], u[b]);
+                hu[b] = fma(h[q], wq[b], hu[b]);
+            }
         }
     }
-    if (a < na) {
+    for (; a < na; ++a) {
         const double x0 = A[a * N + g], h0 = HA[a * N + g];
         const double *w0 = y + (a0 + a) * KB;
 #pragma unroll
@@ -898,6 +1022,15 @@ void gram_load_config();
 
 int gram_variant() { return g_gram.variant; }
 
+bool gram_narrow(int mb)
+{
+    static const bool on = [] {
+        const char *e = std::getenv("PISCES_GRAM_NARROW");
+        return !(e && e[0] == '0');
+    }();
+    return on && mb <= kGNarrowB;
+}
+
 int gram_chunks(long long N) { return (int)((N + g_gram.rows - 1) / g_gram.rows); }
 
 int gram_blocks_per_chunk(int ma, int mb)
@@ -906,12 +1039,16 @@ int gram_blocks_per_chunk(int ma, int mb)
     return ((ma + kGTile - 1) / kGTile) * ((mb + kGTile - 1) / kGTile);
 }
 
-void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_mamb, double *d_partial,
-             hipStream_t s)
+void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_blocks_narrow,
+             int max_mamb, double *d_partial, hipStream_t s)
 {
     if (ntasks <= 0) return;
     const int v = g_gram.variant, rows = g_gram.rows;
-    if (v == 0)
+    if (max_blocks_narrow > 0)
+        hipLaunchKernelGGL(gram_narrow_kernel, dim3(max_blocks_narrow, ntasks), dim3(kT), 0, s,
+                           d_tasks, d_partial);
+    if (max_blocks <= 0) {
+    } else if (v == 0)
         hipLaunchKernelGGL(gram_valu_kernel, dim3(max_blocks, ntasks), dim3(kT), 0, s, d_tasks, d_partial, rows);
     else if (v == 1)
         hipLaunchKernelGGL(gram_kernel<0>, dim3(max_blocks, ntasks), dim3(kT), 0, s, d_tasks, d_partial, rows);

@@ -22,6 +22,7 @@ struct GramTask {
     long long part_off;  // this task's region of the partial buffer (doubles)
     int nch;             // row chunks (gram_chunks(N))
     int cpb;             // workgroups per chunk (gram_blocks_per_chunk(ma, mb))
+    int kind;            // 0: the configured variant; 1: narrow-B VALU kernel (gram_narrow)
 };
 
 // Y (set|-=|+=) A1 W1 + A2 W2 + A3 W3, W in device memory (column-major);
@@ -114,8 +115,9 @@ struct CoefTask {
     const double *Y; int m; int k; double *Q; int *keep_out;
 };
 
-void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_mamb, double *d_partial,
-             hipStream_t s);
+// max_blocks / max_blocks_narrow: max over the tasks of kind 0 / 1 of nch * cpb
+void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_blocks_narrow,
+             int max_mamb, double *d_partial, hipStream_t s);
 void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, hipStream_t s);
 void bd_eig(const EigTask *d_tasks, int ntasks, hipStream_t s);
 void bd_resid(const ResTask *d_tasks, int ntasks, int max_nch, double *d_partial, hipStream_t s);
@@ -130,6 +132,11 @@ void bd_coef(const CoefTask *d_tasks, int ntasks, hipStream_t s);
 constexpr int kGRows = 4096;        // rows per Gram workgroup, matrix-core variants
 constexpr int kGTile = 8;           // VALU variant: 8 x 8 tiles of (A, B) columns
 constexpr int kGRowsValu = 256 * 32; // VALU variant: rows per chunk (32 per thread)
+// Narrow-B tasks (mb <= kGNarrowB) run on the VALU with kGNarrowA x kGNarrowB
+// register tiles, kGRowsNarrow rows per workgroup (PISCES_GRAM_NARROW=0 sends
+// them to the configured variant instead).
+constexpr int kGNarrowA = 8, kGNarrowB = 4, kGRowsNarrow = 256 * 32;
+bool gram_narrow(int mb);
 // Gram kernel variant (0 VALU register tiles, 1 matrix cores with 16-row
 // blocks, 2 matrix cores with 64-row runs) and rows per workgroup, and the
 // task sizing they imply.

@@ -261,7 +261,7 @@ namespace {
 
 struct GramBatch {
     GramTask *h = nullptr, *d = nullptr;
-    int n = 0, max_blocks = 0, max_mamb = 0;
+    int n = 0, max_blocks = 0, max_blocks_narrow = 0, max_mamb = 0;
     long long part = 0;
     // tasks are sized (row chunks, workgroups, partial offsets) in finish(),
     // for the Gram configuration in use
@@ -270,20 +270,27 @@ struct GramBatch {
     {
         PC_REQUIRE(ma >= 1 && ma <= kBdMaxM && mb >= 1 && mb <= kGramMaxB, PISCES_ERR_ARG,
                    "Gram task exceeds the kernel's column limits");
-        h[n++] = GramTask{A, N, ma, a32, B, N, mb, b32, N, out, out_ld, 0, 0, 0};
+        h[n++] = GramTask{A, N, ma, a32, B, N, mb, b32, N, out, out_ld, 0, 0, 0, 0};
         max_mamb = std::max(max_mamb, ma * mb);
     }
     void finish()
     {
         part = 0;
-        max_blocks = 0;
+        max_blocks = max_blocks_narrow = 0;
         for (int i = 0; i < n; ++i) {
             GramTask &t = h[i];
             t.part_off = part;
-            t.nch = gram_chunks(t.N);
-            t.cpb = gram_blocks_per_chunk(t.ma, t.mb);
+            t.kind = gram_narrow(t.mb) ? 1 : 0;
+            if (t.kind) {
+                t.nch = (int)((t.N + kGRowsNarrow - 1) / kGRowsNarrow);
+                t.cpb = (t.ma + kGNarrowA - 1) / kGNarrowA;
+                max_blocks_narrow = std::max(max_blocks_narrow, t.nch * t.cpb);
+            } else {
+                t.nch = gram_chunks(t.N);
+                t.cpb = gram_blocks_per_chunk(t.ma, t.mb);
+                max_blocks = std::max(max_blocks, t.nch * t.cpb);
+            }
             part += (long long)t.ma * t.mb * t.nch;
-            max_blocks = std::max(max_blocks, t.nch * t.cpb);
         }
     }
     // bytes read once: B is not counted again where its columns lie inside A
@@ -305,7 +312,7 @@ struct GramBatch {
     {
         if (!n) return;
         ProfScope ps(PK_GRAM, s, prof_enabled() ? bytes() : 0.0);
-        bd_gram(d, n, max_blocks, max_mamb, dp, s);
+        bd_gram(d, n, max_blocks, max_blocks_narrow, max_mamb, dp, s);
     }
 };
 

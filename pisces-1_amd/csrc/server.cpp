@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "blockops.hpp"
+#include <cstdlib>
 #include "context.hpp"
 #include "engine.hpp"
 #include "prof.hpp"
@@ -62,6 +63,7 @@ struct Live {
     long long N;
     int want_vectors;
     std::shared_ptr<InstRes> res;
+    int e;  // engine (block size class)
 };
 
 }  // namespace
@@ -70,8 +72,29 @@ struct pisces_server {
     int device = 0;
     hipStream_t stream = nullptr;
     size_t cap = 0, used = 0;
-    int max_slots = 0, precision = 0, maxsub = 32, ptol = 7, block = 8;
-    std::unique_ptr<BdEngine> eng;
+    int max_slots = 0, precision = 0, maxsub = 32, ptol = 7;
+    // two engines by block size: requests with up to block[0] states (4 unless
+    // PISCES_SERVER_BLOCK says otherwise) and the rest (block 8); the second is
+    // created on first use
+    int block[2] = {4, 8};
+    std::unique_ptr<BdEngine> eng[2];
+    int engine_of(int nroots) const { return nroots <= block[0] ? 0 : 1; }
+    BdEngine &engine(int e)
+    {
+        if (!eng[e]) {
+            const int ms = std::max(maxsub, 3 * block[e]);
+            eng[e] = std::make_unique<BdEngine>(device, stream, precision, block[e], ms, ptol, 600,
+                                                PISCES_PRECOND_KINETIC);
+        }
+        return *eng[e];
+    }
+    size_t engine_pool_bytes() const
+    {
+        size_t b = 0;
+        for (const auto &e : eng)
+            if (e) b += e->pool_bytes();
+        return b;
+    }
     std::map<std::tuple<int, int, int, double, double, double>, std::shared_ptr<GridShape>> shapes;
     std::deque<Pending> queue;
     std::vector<Live> live;
@@ -98,7 +121,8 @@ struct pisces_server {
     ~pisces_server()
     {
         if (stream) static_cast<void>(hipStreamSynchronize(stream));
-        eng.reset();
+        eng[0].reset();
+        eng[1].reset();
         if (h_up) static_cast<void>(hipHostFree(h_up));
         if (stream) static_cast<void>(hipStreamDestroy(stream));
     }
@@ -156,9 +180,9 @@ std::shared_ptr<GridShape> shape_of(pisces_server *sv, const pisces_request &r)
     return g;
 }
 
-size_t footprint(const pisces_server *sv, long long N)
+size_t footprint(pisces_server *sv, long long N, int nroots)
 {
-    return sv->eng->bytes_per_instance(N) + (size_t)N * (8 + 4 + 8);
+    return sv->engine(sv->engine_of(nroots)).bytes_per_instance(N) + (size_t)N * (8 + 4 + 8);
 }
 
 // Host-side tables of one admitted request's potential build.
@@ -301,7 +325,7 @@ void admit(pisces_server *sv)
     int live = (int)sv->live.size();
     for (auto it = sv->queue.begin(); it != sv->queue.end() && live < sv->max_slots; ++it) {
         const long long N = (long long)it->req.n[0] * it->req.n[1] * it->req.n[2];
-        const size_t fp = footprint(sv, N);
+        const size_t fp = footprint(sv, N, it->req.nroots);
         if (live > 0 && used + fp > sv->cap) break;  // wait for memory (first come, first served)
         Prep r;
         r.p = &*it;
@@ -319,7 +343,9 @@ void admit(pisces_server *sv)
     for (Prep &r : batch) {
         Pending &p = *r.p;
         const long long N = (long long)p.req.n[0] * p.req.n[1] * p.req.n[2];
-        const size_t fp = footprint(sv, N);
+        const size_t fp = footprint(sv, N, p.req.nroots);
+        const int e = sv->engine_of(p.req.nroots);
+        BdEngine &eng = sv->engine(e);
         auto res = r.res;
         BdGrid g;
         for (int a = 0; a < 3; ++a) {
@@ -333,14 +359,14 @@ void admit(pisces_server *sv)
         int slot;
         auto w = sv->warm.find(p.req.chain);
         if (p.req.warm && p.req.chain >= 0 && w != sv->warm.end() && w->second.second == N) {
-            slot = sv->eng->add(g, p.req.nroots, w->second.first.as<double>(), p.req.nroots,
-                                (unsigned)p.id, nullptr, true);
+            slot = eng.add(g, p.req.nroots, w->second.first.as<double>(), p.req.nroots,
+                           (unsigned)p.id, nullptr, true);
         } else {
-            slot = sv->eng->add(g, p.req.nroots, res->shape->piab.data(), 1, (unsigned)p.id,
-                                nullptr, false);
+            slot = eng.add(g, p.req.nroots, res->shape->piab.data(), 1, (unsigned)p.id, nullptr,
+                           false);
         }
         sv->live.push_back(Live{p.id, slot, fp, p.t_submit, p.req.chain, p.req.nroots, N,
-                                p.req.want_vectors, res});
+                                p.req.want_vectors, res, e});
         sv->used += fp;
     }
     for (size_t i = 0; i < batch.size(); ++i) sv->queue.pop_front();
@@ -350,11 +376,12 @@ void retire(pisces_server *sv)
 {
     for (size_t i = 0; i < sv->live.size();) {
         Live &L = sv->live[i];
-        if (!sv->eng->finished(L.slot)) {
+        BdEngine &eng = sv->engine(L.e);
+        if (!eng.finished(L.slot)) {
             ++i;
             continue;
         }
-        const BdResult &r = sv->eng->result(L.slot);
+        const BdResult &r = eng.result(L.slot);
         pisces_result out{};
         out.id = L.id;
         out.status = r.status;
@@ -371,14 +398,14 @@ void retire(pisces_server *sv)
             auto &w = sv->warm[L.chain];
             w.first.ensure(sizeof(double) * L.N * L.nroots);
             w.second = L.N;
-            PC_HIP(hipMemcpyAsync(w.first.p, sv->eng->vectors_device(L.slot),
+            PC_HIP(hipMemcpyAsync(w.first.p, eng.vectors_device(L.slot),
                                   sizeof(double) * L.N * L.nroots, hipMemcpyDeviceToDevice,
                                   sv->stream));
         }
         // keep the vectors of the last few finished requests that asked for them
         if (L.want_vectors) {
             std::vector<double> v((size_t)L.N * L.nroots);
-            sv->eng->vectors(L.slot, v.data());
+            eng.vectors(L.slot, v.data());
             sv->vecs[L.id] = std::move(v);
             sv->vec_order.push_back(L.id);
             while (sv->vec_order.size() > 16) {
@@ -386,7 +413,7 @@ void retire(pisces_server *sv)
                 sv->vec_order.pop_front();
             }
         }
-        sv->eng->release(L.slot);
+        eng.release(L.slot);
         for (dbuf *b : {&L.res->V64, &L.res->V32, &L.res->D}) sv->pool.push_back(std::move(*b));
         sv->used -= L.bytes;
         sv->done.push_back(out);
@@ -395,15 +422,19 @@ void retire(pisces_server *sv)
     // pooled buffers of retired instances count against the budget too
     size_t spool = 0;
     for (const dbuf &b : sv->pool) spool += b.bytes;
-    const size_t pool = sv->eng->pool_bytes() + spool;
+    const size_t pool = sv->engine_pool_bytes() + spool;
     if (sv->used + pool > sv->cap) {
         PC_HIP(hipStreamSynchronize(sv->stream));
-        while (!sv->pool.empty() && sv->used + sv->eng->pool_bytes() + spool > sv->cap) {
+        while (!sv->pool.empty() && sv->used + sv->engine_pool_bytes() + spool > sv->cap) {
             spool -= sv->pool.front().bytes;
             sv->pool.erase(sv->pool.begin());
         }
-        const size_t room = sv->cap > sv->used + spool ? sv->cap - sv->used - spool : 0;
-        sv->eng->trim_pool(room);
+        size_t room = sv->cap > sv->used + spool ? sv->cap - sv->used - spool : 0;
+        for (auto &e : sv->eng)
+            if (e) {
+                e->trim_pool(room);
+                room = room > e->pool_bytes() ? room - e->pool_bytes() : 0;
+            }
     }
 }
 
@@ -428,9 +459,14 @@ int pisces_server_create(pisces_server **out, int device, size_t hbm_cap_bytes, 
         sv->precision = precision;
         sv->maxsub = maxsub;
         sv->ptol = ptol;
-        sv->block = 8;
-        sv->eng = std::make_unique<BdEngine>(device, sv->stream, precision, sv->block, maxsub, ptol,
-                                             600, PISCES_PRECOND_KINETIC);
+        // block 4 for up to 4 states (PISCES_SERVER_BLOCK overrides): on (H2O)24
+        // 64^3 4-state solves it takes ~40 steps against ~32 with block 8, but
+        // each step moves about half the bytes -- 284 vs 205 solves/s
+        const char *be = std::getenv("PISCES_SERVER_BLOCK");
+        if (be && std::atoi(be) > 0) sv->block[0] = std::min(8, std::atoi(be));
+        PC_REQUIRE(3 * sv->block[0] <= maxsub, PISCES_ERR_ARG,
+                   "maxsub must be at least 3x the block");
+        sv->engine(0);
         return 0;
     });
     if (rc) {
@@ -451,7 +487,7 @@ long long pisces_server_submit(pisces_server *sv, const pisces_request *req)
     for (int a = 0; a < 3; ++a)
         if (req->n[a] < 4 || req->n[a] > 128 || !(req->L[a] > 0)) return PISCES_ERR_ARG;
     const long long N = (long long)req->n[0] * req->n[1] * req->n[2];
-    if (footprint(sv, N) > sv->cap) return PISCES_ERR_NOMEM;  // can never be admitted
+    if (footprint(sv, N, req->nroots) > sv->cap) return PISCES_ERR_NOMEM;  // can never be admitted
     Pending p;
     p.id = sv->next_id++;
     p.req = *req;
@@ -474,7 +510,10 @@ int pisces_server_step(pisces_server *sv)
     return guarded_call([&] {
         PC_HIP(hipSetDevice(sv->device));
         admit(sv);
-        if (!sv->live.empty()) sv->eng->step();
+        bool any[2] = {false, false};
+        for (const Live &L : sv->live) any[L.e] = true;
+        for (int e = 0; e < 2; ++e)
+            if (any[e]) sv->engine(e).step();
         retire(sv);
         return (int)sv->live.size();
     });
@@ -493,8 +532,9 @@ int pisces_server_poll(pisces_server *sv, pisces_result *out, int max)
 
 size_t pisces_server_footprint(const pisces_server *sv, const pisces_request *req)
 {
-    if (!sv || !req) return 0;
-    return footprint(sv, (long long)req->n[0] * req->n[1] * req->n[2]);
+    if (!sv || !req || req->nroots < 1 || req->nroots > 8) return 0;
+    return footprint(const_cast<pisces_server *>(sv), (long long)req->n[0] * req->n[1] * req->n[2],
+                     req->nroots);
 }
 
 int pisces_server_vectors(pisces_server *sv, long long id, double *v)
