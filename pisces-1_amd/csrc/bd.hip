@@ -541,6 +541,7 @@ __device__ inline int rr_index(int i, int r, int mp)
 }
 
 constexpr int kEigT = 256;
+constexpr int kEigSweeps = 40;
 
 // Cyclic Jacobi with round-robin pairing: every round applies m/2 disjoint
 // rotations J as A <- J^T A J in one pass over 2 x 2 blocks (rows of one
@@ -560,6 +561,7 @@ __global__ void __launch_bounds__(kEigT) eig_kernel(const EigTask *__restrict__ 
     __shared__ int order[kBdMaxM];
     __shared__ int s_stop, s_rot;
     const int m = t.m, tid = threadIdx.x;
+    const int mw = (t.Vfull && t.mwarm > 0 && t.mwarm <= m) ? t.mwarm : 0;
     for (int idx = tid; idx < m * m; idx += kEigT) {
         const int i = idx % m, j = idx / m;
         double g = (i <= j) ? t.G[i + (long long)t.ldg * j] : t.G[j + (long long)t.ldg * i];
@@ -567,13 +569,40 @@ __global__ void __launch_bounds__(kEigT) eig_kernel(const EigTask *__restrict__ 
         const bool dj = t.keep && !t.keep[j];
         if (di || dj) g = (i == j) ? 1e30 : 0.0;
         A[i + LD * j] = g;
-        V[i + LD * j] = (i == j) ? 1.0 : 0.0;
+        V[i + LD * j] = (i < mw && j < mw) ? t.Vfull[i + (long long)kBdMaxM * j] : (i == j ? 1.0 : 0.0);
     }
     __syncthreads();
+    if (mw > 0) {
+        // warm start: V = diag(Vp, I), A <- V^T A V.  Only rows and columns
+        // below mw change: Tm = A[:, :mw] Vp, then A[:mw, :mw] = Vp^T Tm[:mw]
+        // and A[i, j] = A[j, i] = Tm[i, j] for i >= mw > j.  The leading block
+        // comes out diagonal up to rounding (Vp diagonalised it last step), so
+        // only the new rows and columns are left to rotate.
+        __shared__ double Tm[kBdMaxM * LD];
+        for (int idx = tid; idx < m * mw; idx += kEigT) {
+            const int i = idx % m, j = idx / m;
+            double acc = 0.0;
+            for (int l = 0; l < mw; ++l) acc = fma(A[i + LD * l], V[l + LD * j], acc);
+            Tm[i + LD * j] = acc;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < m * mw; idx += kEigT) {
+            const int i = idx % m, j = idx / m;
+            if (i < mw) {
+                double acc = 0.0;
+                for (int l = 0; l < mw; ++l) acc = fma(V[l + LD * i], Tm[l + LD * j], acc);
+                A[i + LD * j] = acc;
+            } else {
+                A[i + LD * j] = Tm[i + LD * j];
+                A[j + LD * i] = Tm[i + LD * j];
+            }
+        }
+        __syncthreads();
+    }
     const int mp = m + (m & 1);
     const int npair = mp / 2;
     const int lane = tid & 63, wave = tid >> 6;
-    for (int sweep = 0; sweep < 40; ++sweep) {
+    for (int sweep = 0; sweep < kEigSweeps; ++sweep) {
         double off = 0.0, tot = 0.0;
         for (int idx = tid; idx < m * m; idx += kEigT) {
             const int i = idx % m, j = idx / m;
@@ -673,6 +702,11 @@ __global__ void __launch_bounds__(kEigT) eig_kernel(const EigTask *__restrict__ 
         t.Y[i + (long long)m * c] = V[i + LD * order[c]];
     }
     if (tid < t.k) t.theta[tid] = A[order[tid] + LD * order[tid]];
+    if (t.Vfull)
+        for (int idx = tid; idx < m * m; idx += kEigT) {
+            const int i = idx % m, j = idx / m;
+            t.Vfull[i + (long long)kBdMaxM * j] = V[i + LD * j];
+        }
 }
 
 // ------------------------------------------------------------------ residual

@@ -77,7 +77,11 @@ struct EigTask {
     const double *G; int ldg; int m; int k;
     double *theta; double *Y;
     const int *keep;  // keep mask of all m columns (may be null)
-    int pad0, pad;
+    int mwarm, pad;
+    // full eigenvector matrix (m x m, ld kBdMaxM, columns unsorted), written
+    // at the end; if mwarm > 0 its leading mwarm x mwarm block (the last solve
+    // on the first mwarm columns of this basis) starts the rotations
+    double *Vfull;
 };
 
 // R = HU - theta U for k columns (theta[i * th_ld]; R FP32 if r32), and

@@ -42,7 +42,8 @@ constexpr int oQp = 7552;    // restart search directions in [X|C] coefficients 
 constexpr int oI = 8576;     // identity (k x k)
 constexpr int oF = 8832;     // orthogonalisation coefficients on [X|C] ((k + mc) x nt)
 constexpr int oHG = 9856;    // [Xc|C|T]^T T of an orthogonalisation round ((k + mc + nt) x nt)
-constexpr int kSmall = oHG + (kBdMaxM + kBdMaxK) * kBdMaxK;
+constexpr int oVf = oHG + (kBdMaxM + kBdMaxK) * kBdMaxK;  // eigenvectors of the last solve (m x m, ld kBdMaxM)
+constexpr int kSmall = oVf + kBdMaxM * kBdMaxM;
 }  // namespace
 
 struct BdEngine::Inst {
@@ -62,6 +63,7 @@ struct BdEngine::Inst {
     bool late = false;     // this step ends in a late restart
     bool u_valid = false;  // U holds the Ritz block of the last step
     bool late_rf = false;  // the late restart refreshes H X in FP64
+    int eig_m = 0;         // columns of the basis the last eigensolve diagonalised (0: basis changed)
     BdResult res;
     double theta[kBdMaxK], rn[kBdMaxK];
     std::shared_ptr<void> keepalive;
@@ -572,7 +574,11 @@ int BdEngine::step()
         Inst *in = act[i];
         const long long N = in->g.N;
         const int m = k_ + in->mc;
-        he[i] = EigTask{in->sm() + oG, kBdMaxM, m, k_, in->sm() + oTh, in->sm() + oY, in->keep(), 0, 0};
+        // warm start from the last solve where the basis has only grown since
+        const int mwarm = in->restart ? 0 : std::min(in->eig_m, m);
+        he[i] = EigTask{in->sm() + oG, kBdMaxM, m, k_, in->sm() + oTh, in->sm() + oY, in->keep(),
+                        mwarm, 0, in->sm() + oVf};
+        in->eig_m = in->late ? 0 : m;
         const double *Y = in->sm() + oY;
         const bool late = in->late;
         // U is read by a restart (a late one at the end of this step, or a
