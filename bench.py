@@ -15,8 +15,9 @@ Reported next to each other on the same snapshots:
     time) and the FP64 form batched through the server;
   * the eigenvalue error of every state of every snapshot against the FP64
     GPU result (which matches the CPU oracle within 1e-8 Eh in the tests);
-  * live HIP-event timing of every batched launch family over the timed
-    region, the roofline of the dominant one;
+  * HIP-event timing of every batched launch family over one more step of
+    the batch on a single engine lane (the headline's two lanes overlap their
+    launches), the roofline of the dominant one;
   * the CPU restatement of the reference path (its Davidson, dense-T H*x, the
     DPP potential) timed on the host cores on a bounded sample.
 Prints one JSON line.
@@ -78,7 +79,12 @@ def run_single(P, geoms, precision):
     return out, lat
 
 
-def run_server(P, geoms, precision, steps, warmup, prof=False):
+def run_server(P, geoms, precision, steps, warmup, prof=False, lanes=None):
+    """`lanes`: the server's engine lanes (None: its default, two)."""
+    if lanes is None:
+        os.environ.pop("PISCES_SERVER_LANES", None)
+    else:
+        os.environ["PISCES_SERVER_LANES"] = str(lanes)
     sv = P.Server(hbm_cap_bytes=200 << 30, max_slots=len(geoms), precision=precision,
                   maxsub=MAXSUB, ptol=PTOL)
     n, L = (N_GRID,) * 3, (L_BOX,) * 3
@@ -108,6 +114,41 @@ def run_server(P, geoms, precision, steps, warmup, prof=False):
     return results, dt, t_steps, fam
 
 
+def roofline_pass(P, geoms, precision):
+    """One step of the served batch on one engine lane with HIP-event timing of
+    every launch family.  The headline runs two lanes whose launches overlap,
+    so there an event pair around a launch also spans the other lane's work;
+    on one lane each family's time is its own."""
+    res, dt, _, fam = run_server(P, geoms, precision, 1, 0, prof=True, lanes=1)
+    return fam, dt
+
+
+def families(fam):
+    out = {}
+    for k, (ms, nl, by) in fam.items():
+        if nl:
+            out[k] = {"ms_total": round(ms, 3), "launches": nl,
+                      "ms_per_launch": round(ms / nl, 5),
+                      "GBs_algorithmic": round(by / (ms * 1e-3) / 1e9, 1) if by else None}
+    return out
+
+
+def roofline_of(fam, dt):
+    """Roofline of the launch family with the most time in the one-lane pass."""
+    fo = families(fam)
+    dom = max(fo, key=lambda k: fo[k]["ms_total"])
+    ms_d, nl_d, by_d = fam[dom]
+    achieved = by_d / (ms_d * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "ms_per_launch": round(ms_d / nl_d, 5),
+            "bytes_per_launch": round(by_d / nl_d),
+            "share_of_timed": round(ms_d * 1e-3 / dt, 3),
+            "measured_on": "one step of the same batch on one engine lane (no overlap)"}
+    roof["traffic"], roof["traffic_over_algorithmic"] = measured_traffic(dom, by_d / nl_d)
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +161,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--served-only", action="store_true",
                     help="time only the served mixed-precision steps (profiling runs)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the one-lane roofline pass after a warm-up step (profiling runs)")
     ap.add_argument("--config", default="c2", choices=["c2", "c4", "c5"],
                     help="c2: the headline workload; c4: 512 uneven snapshots in one batch; "
                          "c5: open-loop Poisson-served MD chains")
@@ -153,6 +196,13 @@ def main():
     log("batch of %d snapshots generated" % B)
 
     nsingle = max(1, min(a.fp64_single, B))
+    if a.roofline_only:
+        if a.warmup:
+            run_server(P, geoms, prec, a.warmup, 0, lanes=1)
+        fam, dt1 = roofline_pass(P, geoms, prec)
+        print(json.dumps({"roofline": roofline_of(fam, dt1), "launch_families": families(fam),
+                          "solves_per_s": round(B / dt1, 3)}), flush=True)
+        return
     if not a.served_only:
         # ---- plain FP64 GPU form, one solve at a time
         run_single(P, geoms[:1], P.FP64)  # warm-up
@@ -169,8 +219,8 @@ def main():
             [(r["status"], r["n_iter"]) for r in res64b[0]]
         log("fp64 batched: %.2f solves/s" % fp64_batched_sps)
 
-    # ---- mixed precision through the server: the headline, timed, with launch timing
-    res, dt, t_steps, fam = run_server(P, geoms, prec, a.steps, a.warmup, prof=True)
+    # ---- mixed precision through the server: the headline, timed
+    res, dt, t_steps, _ = run_server(P, geoms, prec, a.steps, a.warmup)
     sps = B * a.steps / dt
     lat = np.array([r["latency_s"] for step in res for r in step])
     n_ok = sum(r["status"] == NROOTS for step in res for r in step)
@@ -184,22 +234,10 @@ def main():
         de_single = np.abs(evm[:nsingle] - np.array([r["evals"] for r in r64]))
     log("%s batched: %.2f solves/s, max |dE| %.2e" % (a.precision, sps, de.max()))
 
-    # ---- dominant launch family over the timed region and its roofline
-    fam_out = {}
-    for k, (ms, nl, by) in fam.items():
-        if nl:
-            fam_out[k] = {"ms_total": round(ms, 3), "launches": nl,
-                          "ms_per_launch": round(ms / nl, 5),
-                          "GBs_algorithmic": round(by / (ms * 1e-3) / 1e9, 1) if by else None}
-    dom = max(fam_out, key=lambda k: fam_out[k]["ms_total"])
-    ms_d, nl_d, by_d = fam[dom]
-    achieved = by_d / (ms_d * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-            "ms_per_launch": round(ms_d / nl_d, 5),
-            "bytes_per_launch": round(by_d / nl_d),
-            "share_of_timed": round(ms_d * 1e-3 / dt, 3)}
-    roof["traffic"], roof["traffic_over_algorithmic"] = measured_traffic(dom, by_d / nl_d)
+    # ---- launch families and the roofline of the dominant one, on one lane
+    fam, dt1 = roofline_pass(P, geoms, prec)
+    fam_out = families(fam)
+    roof = roofline_of(fam, dt1)
 
     # ---- standalone H*X at 64^3 x 8 vectors (entries uniform in [-1, 1])
     hx = {}
@@ -245,6 +283,7 @@ def main():
         "iters_mean": round(float(np.mean([r["n_iter"] for r in res[-1]])), 1),
         "roofline": roof,
         "launch_families": fam_out,
+        "lanes": 2,
         "autotuned": P.tuning(),
         "hx_standalone_64cube_x8": hx,
         "cpu_baseline": cpu,

@@ -122,6 +122,9 @@ BdEngine::BdEngine(int device, hipStream_t s, int precision, int block, int maxs
                "unknown precision");
     arena_[0] = std::make_unique<Arena>();
     arena_[1] = std::make_unique<Arena>();
+    ones_.assign(kBdMaxM + kBdMaxK, 1);
+    eye_.assign((size_t)k_ * k_, 0.0);
+    for (int i = 0; i < k_; ++i) eye_[(size_t)i * k_ + i] = 1.0;
     const char *tr = std::getenv("PISCES_TRACE");
     trace_ = tr && tr[0] == '1';
     // PISCES_REFRESH_BELOW=<r>: late restarts of the low precision forms skip
@@ -196,15 +199,14 @@ int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, 
                               start_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s_));
     if (nstart < k_)
         fill_uniform(in->U.as<double>() + N * nstart, N * (k_ - nstart), true, seed, s_);
-    std::vector<int> ones(kBdMaxM + kBdMaxK, 1);
     static_assert(kBdMaxM + kBdMaxK <= 128, "keep mask size");
-    PC_HIP(hipMemcpyAsync(in->keep(), ones.data(), sizeof(int) * (kBdMaxM + kBdMaxK),
+    PC_HIP(hipMemcpyAsync(in->keep(), ones_.data(), sizeof(int) * ones_.size(),
                           hipMemcpyHostToDevice, s_));
-    std::vector<double> eye((size_t)k_ * k_, 0.0);
-    for (int i = 0; i < k_; ++i) eye[(size_t)i * k_ + i] = 1.0;
-    PC_HIP(hipMemcpyAsync(in->sm() + oI, eye.data(), sizeof(double) * eye.size(),
+    PC_HIP(hipMemcpyAsync(in->sm() + oI, eye_.data(), sizeof(double) * eye_.size(),
                           hipMemcpyHostToDevice, s_));
-    PC_HIP(hipStreamSynchronize(s_));  // `ones` and the host start vectors go out of scope
+    // the caller's host start vectors may go out of scope after the return
+    // (the mask and the identity are members: they outlive the copies)
+    if (nstart > 0 && !start_on_device) PC_HIP(hipStreamSynchronize(s_));
     in->res.nroots = nroots;
     for (size_t i = 0; i < inst_.size(); ++i)
         if (!inst_[i]) {
@@ -382,12 +384,21 @@ CombTask comb1(const void *A, long long N, int ma, int a32, const double *W, int
 
 int BdEngine::step()
 {
-    std::vector<Inst *> act;
+    step_a();
+    return step_b();
+}
+
+void BdEngine::step_a()
+{
+    mid_ = false;
+    act_.clear();
+    rs_.clear();
+    lt_.clear();
     for (auto &p : inst_)
-        if (p && !p->done) act.push_back(p.get());
-    if (act.empty()) return 0;
+        if (p && !p->done) act_.push_back(p.get());
+    if (act_.empty()) return;
+    const std::vector<Inst *> &act = act_;
     const int n = (int)act.size();
-    const double tol = std::pow(10.0, -ptol_);
     const bool lo = prec_ != PISCES_FP64;
     const int e32 = lo ? 1 : 0;
     const size_t e = lo ? 4 : 8;
@@ -416,7 +427,7 @@ int BdEngine::step()
     // ============================ phase A: descriptors
     Arena &A = *arena_[0];
     A.used = 0;
-    std::vector<Inst *> rs, lt;
+    std::vector<Inst *> &rs = rs_, &lt = lt_;
     for (Inst *in : act) {
         in->late = false;
         if (in->restart) rs.push_back(in);
@@ -716,6 +727,20 @@ int BdEngine::step()
     }
     PC_HIP(hipMemcpyAsync(hsmall_, dgather_.p, sizeof(double) * 2 * kBdMaxK * n,
                           hipMemcpyDeviceToHost, s_));
+    mid_ = true;
+}
+
+int BdEngine::step_b()
+{
+    if (!mid_) return this->live();
+    mid_ = false;
+    const std::vector<Inst *> &act = act_, &rs = rs_, &lt = lt_;
+    const int n = (int)act.size();
+    const double tol = std::pow(10.0, -ptol_);
+    const bool lo = prec_ != PISCES_FP64;
+    const int e32 = lo ? 1 : 0;
+    const size_t e = lo ? 4 : 8;
+    const int r32 = (lo && precond_ == 3) ? 1 : 0;
     PC_HIP(hipStreamSynchronize(s_));
     for (Inst *in : act) {
         in->have_prev = true;
@@ -890,7 +915,7 @@ int BdEngine::step()
         partB = std::max(partB, og[pass].part);
     }
     grow_part(sizeof(double) * (size_t)partB);
-    dp = dpart_.as<double>();
+    double *dp = dpart_.as<double>();
     B.flush(s_);
     {
         // R columns read once, corrections written once (+ diag H for the diagonal form)

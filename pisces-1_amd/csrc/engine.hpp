@@ -64,6 +64,16 @@ public:
             std::shared_ptr<void> keepalive = nullptr, bool start_on_device = false);
     // Advances every live instance by one step; returns the number still live.
     int step();
+    // The two halves of step(): step_a() queues the Ritz part of the step
+    // (projected eigensolve, Ritz vectors, residual norms and their transfer to
+    // the host) and returns without waiting; step_b() waits for the transfer,
+    // decides per instance (converged, restart, corrections) and queues the
+    // corrections.  Engines on different streams overlap their device work
+    // and their host decisions when every step_a() is called before the
+    // step_b()s.  step_b() must follow step_a() before the engine is used again.
+    void step_a();
+    int step_b();
+    hipStream_t stream() const { return s_; }
     bool finished(int slot) const;
     const BdResult &result(int slot) const;
     // Ritz vectors of a finished instance (nroots x N doubles, host).
@@ -94,6 +104,10 @@ private:
     double refresh_below_ = 1e-3;  // late restarts refresh H X in FP64 below this residual
     void grow_part(size_t bytes);
     std::vector<dbuf> pool_;  // released instance buffers, reused by add()
+    std::vector<Inst *> act_, rs_, lt_;  // the step between step_a() and step_b()
+    bool mid_ = false;                   // step_a() queued, step_b() not yet run
+    std::vector<int> ones_;              // host sources of add()'s copies
+    std::vector<double> eye_;
     dbuf take_buf(size_t bytes);
 };
 

@@ -6,7 +6,11 @@
 // potentials on the GPU (rigid-water projection on the host, FP64 induced
 // dipoles and the grid potential on the device), adds them to the batched
 // block-Davidson engine, advances every live instance by one Davidson step and
-// retires the finished ones.  Requests of a chain warm-start from the chain's
+// retires the finished ones.  Instances are spread over lanes (two by
+// default), each an engine on its own stream: the server queues the first half of a step on
+// every lane before it waits on any, so one lane's latency-bound work (the
+// projected eigensolves, small factor kernels, the host's per-instance
+// decisions) runs beside the other lane's memory-bound streams.  Requests of a chain warm-start from the chain's
 // previous converged Ritz vectors (kept on the device).  Grids of the same
 // shape share their kinetic matrices and eigenbases.
 #include <algorithm>
@@ -39,6 +43,7 @@ struct GridShape {
     dbuf dx[3], dT64[3];
     AxisStore ax[3];
     std::vector<double> piab;
+    dbuf dpiab;  // the start vector on the device
 };
 
 struct InstRes {
@@ -64,6 +69,17 @@ struct Live {
     int want_vectors;
     std::shared_ptr<InstRes> res;
     int e;  // engine (block size class)
+    int l;  // lane
+};
+
+// Converged vectors of a chain's last request.  `ev` is recorded after every
+// copy into or out of them, on the stream of that copy, and every such copy
+// first waits for it: the lanes that write and read them have their own
+// streams.
+struct WarmVec {
+    dbuf v;
+    long long N = 0;
+    hipEvent_t ev = nullptr;
 };
 
 }  // namespace
@@ -73,33 +89,38 @@ struct pisces_server {
     hipStream_t stream = nullptr;
     size_t cap = 0, used = 0;
     int max_slots = 0, precision = 0, maxsub = 32, ptol = 7;
-    // two engines by block size: requests with up to block[0] states (4 unless
-    // PISCES_SERVER_BLOCK says otherwise) and the rest (block 8); the second is
-    // created on first use
+    // engines by block size: requests with up to block[0] states (4 unless
+    // PISCES_SERVER_BLOCK says otherwise) and the rest (block 8), each class
+    // on `lanes` lanes (2 unless PISCES_SERVER_LANES says 1 .. kMaxLanes);
+    // engines are created on first use
+    static constexpr int kMaxLanes = 3;
     int block[2] = {4, 8};
-    std::unique_ptr<BdEngine> eng[2];
+    int lanes = 2;
+    hipStream_t lane_s[kMaxLanes] = {};
+    std::unique_ptr<BdEngine> eng[2][kMaxLanes];  // [block class][lane]
     int engine_of(int nroots) const { return nroots <= block[0] ? 0 : 1; }
-    BdEngine &engine(int e)
+    BdEngine &engine(int e, int l)
     {
-        if (!eng[e]) {
+        if (!eng[e][l]) {
             const int ms = std::max(maxsub, 3 * block[e]);
-            eng[e] = std::make_unique<BdEngine>(device, stream, precision, block[e], ms, ptol, 600,
-                                                PISCES_PRECOND_KINETIC);
+            eng[e][l] = std::make_unique<BdEngine>(device, lane_s[l], precision, block[e], ms, ptol,
+                                                   600, PISCES_PRECOND_KINETIC);
         }
-        return *eng[e];
+        return *eng[e][l];
     }
     size_t engine_pool_bytes() const
     {
         size_t b = 0;
-        for (const auto &e : eng)
-            if (e) b += e->pool_bytes();
+        for (const auto &c : eng)
+            for (const auto &e : c)
+                if (e) b += e->pool_bytes();
         return b;
     }
     std::map<std::tuple<int, int, int, double, double, double>, std::shared_ptr<GridShape>> shapes;
     std::deque<Pending> queue;
     std::vector<Live> live;
     std::deque<pisces_result> done;
-    std::map<long long, std::pair<dbuf, long long>> warm;  // chain -> vectors, N
+    std::map<long long, WarmVec> warm;  // chain -> converged vectors
     std::map<long long, std::vector<double>> vecs;        // finished request -> vectors
     std::deque<long long> vec_order;
     long long next_id = 0;
@@ -121,9 +142,13 @@ struct pisces_server {
     ~pisces_server()
     {
         if (stream) static_cast<void>(hipStreamSynchronize(stream));
-        eng[0].reset();
-        eng[1].reset();
+        for (auto &c : eng)
+            for (auto &e : c) e.reset();
+        for (auto &kv : warm)
+            if (kv.second.ev) static_cast<void>(hipEventDestroy(kv.second.ev));
         if (h_up) static_cast<void>(hipHostFree(h_up));
+        for (hipStream_t ls : lane_s)
+            if (ls) static_cast<void>(hipStreamDestroy(ls));
         if (stream) static_cast<void>(hipStreamDestroy(stream));
     }
 };
@@ -176,13 +201,15 @@ std::shared_ptr<GridShape> shape_of(pisces_server *sv, const pisces_request &r)
                 }
                 g->piab[i + (long long)g->n[0] * (j + (long long)g->n[1] * k)] = w;
             }
+    // read by the lanes' streams after build_potentials() has waited for this one
+    upload(g->dpiab, g->piab.data(), sizeof(double) * g->N, sv->stream);
     sv->shapes[key] = g;
     return g;
 }
 
 size_t footprint(pisces_server *sv, long long N, int nroots)
 {
-    return sv->engine(sv->engine_of(nroots)).bytes_per_instance(N) + (size_t)N * (8 + 4 + 8);
+    return sv->engine(sv->engine_of(nroots), 0).bytes_per_instance(N) + (size_t)N * (8 + 4 + 8);
 }
 
 // Host-side tables of one admitted request's potential build.
@@ -345,7 +372,14 @@ void admit(pisces_server *sv)
         const long long N = (long long)p.req.n[0] * p.req.n[1] * p.req.n[2];
         const size_t fp = footprint(sv, N, p.req.nroots);
         const int e = sv->engine_of(p.req.nroots);
-        BdEngine &eng = sv->engine(e);
+        // the lane of this class with the fewest live instances
+        int cnt[pisces_server::kMaxLanes] = {};
+        for (const Live &L : sv->live)
+            if (L.e == e) ++cnt[L.l];
+        int l = 0;
+        for (int q = 1; q < sv->lanes; ++q)
+            if (cnt[q] < cnt[l]) l = q;
+        BdEngine &eng = sv->engine(e, l);
         auto res = r.res;
         BdGrid g;
         for (int a = 0; a < 3; ++a) {
@@ -358,15 +392,18 @@ void admit(pisces_server *sv)
         g.D = res->D.as<double>();
         int slot;
         auto w = sv->warm.find(p.req.chain);
-        if (p.req.warm && p.req.chain >= 0 && w != sv->warm.end() && w->second.second == N) {
-            slot = eng.add(g, p.req.nroots, w->second.first.as<double>(), p.req.nroots,
-                           (unsigned)p.id, nullptr, true);
+        if (p.req.warm && p.req.chain >= 0 && w != sv->warm.end() && w->second.N == N) {
+            WarmVec &wv = w->second;
+            PC_HIP(hipStreamWaitEvent(eng.stream(), wv.ev, 0));
+            slot = eng.add(g, p.req.nroots, wv.v.as<double>(), p.req.nroots, (unsigned)p.id,
+                           nullptr, true);
+            PC_HIP(hipEventRecord(wv.ev, eng.stream()));
         } else {
-            slot = eng.add(g, p.req.nroots, res->shape->piab.data(), 1, (unsigned)p.id, nullptr,
-                           false);
+            slot = eng.add(g, p.req.nroots, res->shape->dpiab.as<double>(), 1, (unsigned)p.id,
+                           nullptr, true);
         }
         sv->live.push_back(Live{p.id, slot, fp, p.t_submit, p.req.chain, p.req.nroots, N,
-                                p.req.want_vectors, res, e});
+                                p.req.want_vectors, res, e, l});
         sv->used += fp;
     }
     for (size_t i = 0; i < batch.size(); ++i) sv->queue.pop_front();
@@ -376,7 +413,7 @@ void retire(pisces_server *sv)
 {
     for (size_t i = 0; i < sv->live.size();) {
         Live &L = sv->live[i];
-        BdEngine &eng = sv->engine(L.e);
+        BdEngine &eng = sv->engine(L.e, L.l);
         if (!eng.finished(L.slot)) {
             ++i;
             continue;
@@ -395,12 +432,20 @@ void retire(pisces_server *sv)
         out.latency_s = std::chrono::duration<double>(clk::now() - L.t_submit).count();
         // warm-start snapshot of the chain, kept on the device
         if (L.chain >= 0 && r.status == L.nroots) {
-            auto &w = sv->warm[L.chain];
-            w.first.ensure(sizeof(double) * L.N * L.nroots);
-            w.second = L.N;
-            PC_HIP(hipMemcpyAsync(w.first.p, eng.vectors_device(L.slot),
-                                  sizeof(double) * L.N * L.nroots, hipMemcpyDeviceToDevice,
-                                  sv->stream));
+            WarmVec &w = sv->warm[L.chain];
+            const size_t need = sizeof(double) * L.N * L.nroots;
+            if (!w.ev) {
+                PC_HIP(hipEventCreateWithFlags(&w.ev, hipEventDisableTiming));
+            } else if (need > w.v.bytes) {
+                PC_HIP(hipEventSynchronize(w.ev));  // the old buffer is freed
+            } else {
+                PC_HIP(hipStreamWaitEvent(eng.stream(), w.ev, 0));
+            }
+            w.v.ensure(need);
+            w.N = L.N;
+            PC_HIP(hipMemcpyAsync(w.v.p, eng.vectors_device(L.slot), need,
+                                  hipMemcpyDeviceToDevice, eng.stream()));
+            PC_HIP(hipEventRecord(w.ev, eng.stream()));
         }
         // keep the vectors of the last few finished requests that asked for them
         if (L.want_vectors) {
@@ -430,11 +475,12 @@ void retire(pisces_server *sv)
             sv->pool.erase(sv->pool.begin());
         }
         size_t room = sv->cap > sv->used + spool ? sv->cap - sv->used - spool : 0;
-        for (auto &e : sv->eng)
-            if (e) {
-                e->trim_pool(room);
-                room = room > e->pool_bytes() ? room - e->pool_bytes() : 0;
-            }
+        for (auto &c : sv->eng)
+            for (auto &e : c)
+                if (e) {
+                    e->trim_pool(room);
+                    room = room > e->pool_bytes() ? room - e->pool_bytes() : 0;
+                }
     }
 }
 
@@ -454,6 +500,10 @@ int pisces_server_create(pisces_server **out, int device, size_t hbm_cap_bytes, 
         sv->device = device;
         PC_HIP(hipSetDevice(device));
         PC_HIP(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
+        const char *ln = std::getenv("PISCES_SERVER_LANES");
+        if (ln && std::atoi(ln) >= 1) sv->lanes = std::min(std::atoi(ln), pisces_server::kMaxLanes);
+        for (int l = 0; l < sv->lanes; ++l)
+            PC_HIP(hipStreamCreateWithFlags(&sv->lane_s[l], hipStreamNonBlocking));
         sv->cap = hbm_cap_bytes;
         sv->max_slots = max_slots;
         sv->precision = precision;
@@ -466,7 +516,7 @@ int pisces_server_create(pisces_server **out, int device, size_t hbm_cap_bytes, 
         if (be && std::atoi(be) > 0) sv->block[0] = std::min(8, std::atoi(be));
         PC_REQUIRE(3 * sv->block[0] <= maxsub, PISCES_ERR_ARG,
                    "maxsub must be at least 3x the block");
-        sv->engine(0);
+        sv->engine(0, 0);
         return 0;
     });
     if (rc) {
@@ -510,10 +560,14 @@ int pisces_server_step(pisces_server *sv)
     return guarded_call([&] {
         PC_HIP(hipSetDevice(sv->device));
         admit(sv);
-        bool any[2] = {false, false};
-        for (const Live &L : sv->live) any[L.e] = true;
+        bool any[2][pisces_server::kMaxLanes] = {};
+        for (const Live &L : sv->live) any[L.e][L.l] = true;
         for (int e = 0; e < 2; ++e)
-            if (any[e]) sv->engine(e).step();
+            for (int l = 0; l < sv->lanes; ++l)
+                if (any[e][l]) sv->engine(e, l).step_a();
+        for (int e = 0; e < 2; ++e)
+            for (int l = 0; l < sv->lanes; ++l)
+                if (any[e][l]) sv->engine(e, l).step_b();
         retire(sv);
         return (int)sv->live.size();
     });
