@@ -448,87 +448,99 @@ __global__ void __launch_bounds__(kT) gram_reduce_kernel(const GramTask *__restr
 }
 
 // ------------------------------------------------------------------ combine
-// acc[b] += sum_a A[a*lda + g] w[(a0 + a) + ma*b]: eight column loads in flight
-template <typename T>
-__device__ __forceinline__ void comb_cols(gptr<const T> A, long long lda, int na,
-                                          const double *w, int ma, int a0, int mb, long long g,
-                                          double (&acc)[kBdMaxK])
+// Workgroup = kT * RPT rows (thread: rows g0 + kT r, r < RPT, each column load
+// coalesced over the wave), MB >= mb columns of Y accumulated in FP64
+// registers.  The coefficients are wave-uniform loads from W through the
+// constant address space (scalar loads; W is written only by earlier launches):
+// no LDS staging and no barrier before the first column load, and each
+// coefficient serves RPT rows.  Rows past N load row g0 (in range) and are
+// not stored.
+// one group of G consecutive columns from column a: G loads per row in flight
+// before the products.  Columns of Y past mb get coefficient 0 from a clamped
+// (in range) index, with no branch, so a group's loads overlap the products of
+// the group before.
+template <typename T, int MB, int RPT, int G>
+__device__ __forceinline__ void comb_group(gptr<const T> A, long long lda, int a, cptr<double> W,
+                                           int ldw, int mb, const long long (&gi)[RPT],
+                                           double (&acc)[RPT][MB])
 {
-    int a = 0;
-    for (; a + 8 <= na; a += 8) {
-        double x[8];
+    T x[RPT][G];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) x[q] = A[(a + q) * lda + g];
+    for (int q = 0; q < G; ++q)
 #pragma unroll
-        for (int b = 0; b < kBdMaxK; ++b) {
-            if (b >= mb) break;
-            const double *wb = w + a0 + a + ma * b;
-            double s = acc[b];
+        for (int r = 0; r < RPT; ++r) x[r][q] = A[(long long)(a + q) * lda + gi[r]];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) s = fma(x[q], wb[q], s);
-            acc[b] = s;
-        }
-    }
-    for (; a + 4 <= na; a += 4) {
-        const double x0 = A[a * lda + g], x1 = A[(a + 1) * lda + g];
-        const double x2 = A[(a + 2) * lda + g], x3 = A[(a + 3) * lda + g];
+    for (int b = 0; b < MB; ++b) {
+        const int bc = b < mb ? b : mb - 1;
 #pragma unroll
-        for (int b = 0; b < kBdMaxK; ++b) {
-            if (b >= mb) break;
-            const double *wb = w + a0 + a + ma * b;
-            acc[b] += x0 * wb[0] + x1 * wb[1] + x2 * wb[2] + x3 * wb[3];
-        }
-    }
-    for (; a < na; ++a) {
-        const double x0 = A[a * lda + g];
+        for (int q = 0; q < G; ++q) {
+            const double wl = W[a + q + (long long)ldw * bc];
+            const double w = b < mb ? wl : 0.0;
 #pragma unroll
-        for (int b = 0; b < kBdMaxK; ++b) {
-            if (b >= mb) break;
-            acc[b] += x0 * w[a0 + a + ma * b];
+            for (int r = 0; r < RPT; ++r) acc[r][b] = fma((double)x[r][q], w, acc[r][b]);
         }
     }
 }
 
+// columns in groups of 8 (FP32) or 4 (FP64) -- 32 bytes per row in flight --
+// then one half group, then single columns
+template <typename T, int MB, int RPT>
+__device__ __forceinline__ void comb_cols(gptr<const T> A, long long lda, int na, cptr<double> W,
+                                          int ldw, int mb, const long long (&gi)[RPT],
+                                          double (&acc)[RPT][MB])
+{
+    constexpr int CW = sizeof(T) == 4 ? 8 : 4;
+    int a = 0;
+    for (; a + CW <= na; a += CW) comb_group<T, MB, RPT, CW>(A, lda, a, W, ldw, mb, gi, acc);
+    if (a + CW / 2 <= na) {
+        comb_group<T, MB, RPT, CW / 2>(A, lda, a, W, ldw, mb, gi, acc);
+        a += CW / 2;
+    }
+    for (; a < na; ++a) comb_group<T, MB, RPT, 1>(A, lda, a, W, ldw, mb, gi, acc);
+}
+
+template <int MB, int RPT>
 __device__ __forceinline__ void comb_source(const void *A, long long lda, int na, int a32,
-                                            const double *w, int ma, int a0, int mb, long long g,
-                                            double (&acc)[kBdMaxK])
+                                            const double *W, int ldw, int mb,
+                                            const long long (&gi)[RPT], double (&acc)[RPT][MB])
 {
     if (na <= 0) return;
-    if (a32) comb_cols(glb(static_cast<const float *>(A)), lda, na, w, ma, a0, mb, g, acc);
-    else comb_cols(glb(static_cast<const double *>(A)), lda, na, w, ma, a0, mb, g, acc);
+    if (a32) comb_cols<float, MB, RPT>(glb(static_cast<const float *>(A)), lda, na, cst(W), ldw, mb, gi, acc);
+    else comb_cols<double, MB, RPT>(glb(static_cast<const double *>(A)), lda, na, cst(W), ldw, mb, gi, acc);
 }
 
+template <int MB, int RPT>
 __global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ tasks)
 {
     const CombTask t = tasks[blockIdx.y];
-    __shared__ double w[(kBdMaxM + kBdMaxK) * kBdMaxK];
-    const int ma = t.ma1 + t.ma2 + t.ma3;
-    for (int idx = threadIdx.x; idx < ma * t.mb; idx += kT) {
-        const int a = idx % ma, b = idx / ma;
-        double v;
-        if (a < t.ma1) v = t.W1[a + (long long)t.ldw1 * b];
-        else if (a < t.ma1 + t.ma2) v = t.W2[(a - t.ma1) + (long long)t.ldw2 * b];
-        else v = t.W3[(a - t.ma1 - t.ma2) + (long long)t.ldw3 * b];
-        w[idx] = v;
-    }
-    __syncthreads();
-    const long long g = (long long)blockIdx.x * kT + threadIdx.x;
-    if (g >= t.N) return;
-    double acc[kBdMaxK];
+    const long long g0 = (long long)blockIdx.x * (kT * RPT) + threadIdx.x;
+    if (g0 >= t.N) return;
+    long long gi[RPT];
 #pragma unroll
-    for (int b = 0; b < kBdMaxK; ++b) acc[b] = 0.0;
-    comb_source(t.A1, t.lda1, t.ma1, t.a1_32, w, ma, 0, t.mb, g, acc);
-    comb_source(t.A2, t.lda2, t.ma2, t.a2_32, w, ma, t.ma1, t.mb, g, acc);
-    comb_source(t.A3, t.lda3, t.ma3, t.a3_32, w, ma, t.ma1 + t.ma2, t.mb, g, acc);
+    for (int r = 0; r < RPT; ++r) gi[r] = (g0 + kT * r < t.N) ? g0 + kT * r : g0;
+    double acc[RPT][MB];
 #pragma unroll
-    for (int b = 0; b < kBdMaxK; ++b) {
-        if (b >= t.mb) break;
-        const long long yi = b * t.ldy + g;
-        double v = acc[b];
-        if (t.mode == 1) v = ld_any(t.Y, yi, t.y32) - v;
-        else if (t.mode == 2) v = ld_any(t.Y, yi, t.y32) + v;
-        st_any(t.Y, yi, t.y32, v);
-        if (t.Y2) st_any(t.Y2, yi, t.y2_32, v);
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int b = 0; b < MB; ++b) acc[r][b] = 0.0;
+    comb_source<MB, RPT>(t.A1, t.lda1, t.ma1, t.a1_32, t.W1, t.ldw1, t.mb, gi, acc);
+    comb_source<MB, RPT>(t.A2, t.lda2, t.ma2, t.a2_32, t.W2, t.ldw2, t.mb, gi, acc);
+    comb_source<MB, RPT>(t.A3, t.lda3, t.ma3, t.a3_32, t.W3, t.ldw3, t.mb, gi, acc);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const long long g = g0 + kT * r;
+        if (g >= t.N) break;
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            if (b < t.mb) {
+                const long long yi = b * t.ldy + g;
+                double v = acc[r][b];
+                if (t.mode == 1) v = ld_any(t.Y, yi, t.y32) - v;
+                else if (t.mode == 2) v = ld_any(t.Y, yi, t.y32) + v;
+                st_any(t.Y, yi, t.y32, v);
+                if (t.Y2) st_any(t.Y2, yi, t.y2_32, v);
+            }
+        }
     }
 }
 
@@ -1144,10 +1156,15 @@ int gram_config(int *rows)
     return g_gram.variant;
 }
 
-void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, hipStream_t s)
+// rows per thread x columns: 4 x 4, 2 x 8, 1 x 16 (the same accumulator count)
+void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, int max_mb, hipStream_t s)
 {
     if (ntasks <= 0) return;
-    hipLaunchKernelGGL(comb_kernel, dim3(nblk(max_N), ntasks), dim3(kT), 0, s, d_tasks);
+    PC_REQUIRE(max_mb >= 1 && max_mb <= kBdMaxK, PISCES_ERR_ARG, "combination width 1..16");
+    auto grid = [&](int rpt) { return dim3((unsigned)((max_N + kT * rpt - 1) / (kT * rpt)), ntasks); };
+    if (max_mb <= 4) hipLaunchKernelGGL((comb_kernel<4, 4>), grid(4), dim3(kT), 0, s, d_tasks);
+    else if (max_mb <= 8) hipLaunchKernelGGL((comb_kernel<8, 2>), grid(2), dim3(kT), 0, s, d_tasks);
+    else hipLaunchKernelGGL((comb_kernel<16, 1>), grid(1), dim3(kT), 0, s, d_tasks);
     PC_HIP(hipGetLastError());
 }
 

@@ -122,7 +122,8 @@ struct CoefTask {
 // max_blocks / max_blocks_narrow: max over the tasks of kind 0 / 1 of nch * cpb
 void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_blocks_narrow,
              int max_mamb, double *d_partial, hipStream_t s);
-void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, hipStream_t s);
+// max_mb: max over the tasks of mb
+void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, int max_mb, hipStream_t s);
 void bd_eig(const EigTask *d_tasks, int ntasks, hipStream_t s);
 void bd_resid(const ResTask *d_tasks, int ntasks, int max_nch, double *d_partial, hipStream_t s);
 void bd_precond(const PrecTask *d_tasks, int ntasks, long long max_N, hipStream_t s);

@@ -122,9 +122,17 @@ BdEngine::BdEngine(int device, hipStream_t s, int precision, int block, int maxs
                "unknown precision");
     arena_[0] = std::make_unique<Arena>();
     arena_[1] = std::make_unique<Arena>();
-    ones_.assign(kBdMaxM + kBdMaxK, 1);
-    eye_.assign((size_t)k_ * k_, 0.0);
-    for (int i = 0; i < k_; ++i) eye_[(size_t)i * k_ + i] = 1.0;
+    {
+        // device copy of every new instance's keep mask and identity
+        static_assert(sizeof(int) * (kBdMaxM + kBdMaxK) <= kInitEye, "keep mask size");
+        std::vector<unsigned char> h(kInitEye + sizeof(double) * kBdMaxK * kBdMaxK, 0);
+        int *ones = reinterpret_cast<int *>(h.data());
+        for (int i = 0; i < kBdMaxM + kBdMaxK; ++i) ones[i] = 1;
+        double *eye = reinterpret_cast<double *>(h.data() + kInitEye);
+        for (int i = 0; i < k_; ++i) eye[(size_t)i * k_ + i] = 1.0;
+        dinit_.alloc(h.size());
+        PC_HIP(hipMemcpy(dinit_.p, h.data(), h.size(), hipMemcpyHostToDevice));
+    }
     const char *tr = std::getenv("PISCES_TRACE");
     trace_ = tr && tr[0] == '1';
     // PISCES_REFRESH_BELOW=<r>: late restarts of the low precision forms skip
@@ -199,13 +207,12 @@ int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, 
                               start_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s_));
     if (nstart < k_)
         fill_uniform(in->U.as<double>() + N * nstart, N * (k_ - nstart), true, seed, s_);
-    static_assert(kBdMaxM + kBdMaxK <= 128, "keep mask size");
-    PC_HIP(hipMemcpyAsync(in->keep(), ones_.data(), sizeof(int) * ones_.size(),
-                          hipMemcpyHostToDevice, s_));
-    PC_HIP(hipMemcpyAsync(in->sm() + oI, eye_.data(), sizeof(double) * eye_.size(),
-                          hipMemcpyHostToDevice, s_));
+    // keep mask (all ones) and the k x k identity from their device copy
+    PC_HIP(hipMemcpyAsync(in->keep(), dinit_.p, sizeof(int) * (kBdMaxM + kBdMaxK),
+                          hipMemcpyDeviceToDevice, s_));
+    PC_HIP(hipMemcpyAsync(in->sm() + oI, dinit_.as<unsigned char>() + kInitEye,
+                          sizeof(double) * k_ * k_, hipMemcpyDeviceToDevice, s_));
     // the caller's host start vectors may go out of scope after the return
-    // (the mask and the identity are members: they outlive the copies)
     if (nstart > 0 && !start_on_device) PC_HIP(hipStreamSynchronize(s_));
     in->res.nroots = nroots;
     for (size_t i = 0; i < inst_.size(); ++i)
@@ -322,12 +329,13 @@ struct GramBatch {
 
 struct CombBatch {
     CombTask *h = nullptr, *d = nullptr;
-    int n = 0;
+    int n = 0, maxmb = 0;
     long long maxN = 0;
     void add(const CombTask &t)
     {
         h[n++] = t;
         maxN = std::max(maxN, t.N);
+        maxmb = std::max(maxmb, t.mb);
     }
     double bytes() const
     {
@@ -344,7 +352,7 @@ struct CombBatch {
     {
         if (!n) return;
         ProfScope ps(PK_COMB, s, prof_enabled() ? bytes() : 0.0);
-        bd_combine(d, n, maxN, s);
+        bd_combine(d, n, maxN, maxmb, s);
     }
 };
 

@@ -106,8 +106,8 @@ private:
     std::vector<dbuf> pool_;  // released instance buffers, reused by add()
     std::vector<Inst *> act_, rs_, lt_;  // the step between step_a() and step_b()
     bool mid_ = false;                   // step_a() queued, step_b() not yet run
-    std::vector<int> ones_;              // host sources of add()'s copies
-    std::vector<double> eye_;
+    dbuf dinit_;  // keep mask (ints, all ones) and, at byte kInitEye, the k x k identity
+    static constexpr size_t kInitEye = 512;
     dbuf take_buf(size_t bytes);
 };
 

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -98,6 +99,12 @@ struct pisces_server {
     int lanes = 2;
     hipStream_t lane_s[kMaxLanes] = {};
     std::unique_ptr<BdEngine> eng[2][kMaxLanes];  // [block class][lane]
+    // PISCES_HOST_TIMING=1: host wall time of the parts of pisces_server_step()
+    // (admission, of it the potential build, the two step halves, retirement),
+    // printed to stderr when the server is destroyed
+    bool htime = false;
+    double ht[5] = {0, 0, 0, 0, 0};
+    long long hsteps = 0;
     int engine_of(int nroots) const { return nroots <= block[0] ? 0 : 1; }
     BdEngine &engine(int e, int l)
     {
@@ -141,6 +148,11 @@ struct pisces_server {
     size_t h_up_cap = 0;
     ~pisces_server()
     {
+        if (htime)
+            std::fprintf(stderr,
+                         "[server] host ms over %lld steps: admit %.1f (potentials %.1f), "
+                         "step_a %.1f, step_b %.1f, retire %.1f\n",
+                         hsteps, ht[0], ht[1], ht[2], ht[3], ht[4]);
         if (stream) static_cast<void>(hipStreamSynchronize(stream));
         for (auto &c : eng)
             for (auto &e : c) e.reset();
@@ -366,7 +378,9 @@ void admit(pisces_server *sv)
         ++live;
     }
     if (batch.empty()) return;
+    const auto tp = clk::now();
     build_potentials(sv, batch);
+    if (sv->htime) sv->ht[1] += std::chrono::duration<double, std::milli>(clk::now() - tp).count();
     for (Prep &r : batch) {
         Pending &p = *r.p;
         const long long N = (long long)p.req.n[0] * p.req.n[1] * p.req.n[2];
@@ -500,6 +514,8 @@ int pisces_server_create(pisces_server **out, int device, size_t hbm_cap_bytes, 
         sv->device = device;
         PC_HIP(hipSetDevice(device));
         PC_HIP(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
+        const char *htv = std::getenv("PISCES_HOST_TIMING");
+        sv->htime = htv && htv[0] == '1';
         const char *ln = std::getenv("PISCES_SERVER_LANES");
         if (ln && std::atoi(ln) >= 1) sv->lanes = std::min(std::atoi(ln), pisces_server::kMaxLanes);
         for (int l = 0; l < sv->lanes; ++l)
@@ -559,16 +575,28 @@ int pisces_server_step(pisces_server *sv)
     if (!sv) return PISCES_ERR_ARG;
     return guarded_call([&] {
         PC_HIP(hipSetDevice(sv->device));
+        const auto t0 = clk::now();
         admit(sv);
+        const auto t1 = clk::now();
         bool any[2][pisces_server::kMaxLanes] = {};
         for (const Live &L : sv->live) any[L.e][L.l] = true;
         for (int e = 0; e < 2; ++e)
             for (int l = 0; l < sv->lanes; ++l)
                 if (any[e][l]) sv->engine(e, l).step_a();
+        const auto t2 = clk::now();
         for (int e = 0; e < 2; ++e)
             for (int l = 0; l < sv->lanes; ++l)
                 if (any[e][l]) sv->engine(e, l).step_b();
+        const auto t3 = clk::now();
         retire(sv);
+        if (sv->htime) {
+            using ms = std::chrono::duration<double, std::milli>;
+            sv->ht[0] += ms(t1 - t0).count();
+            sv->ht[2] += ms(t2 - t1).count();
+            sv->ht[3] += ms(t3 - t2).count();
+            sv->ht[4] += ms(clk::now() - t3).count();
+            ++sv->hsteps;
+        }
         return (int)sv->live.size();
     });
 }
