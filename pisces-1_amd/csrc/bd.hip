@@ -338,34 +338,57 @@ __global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ t
 // and at the sustained FP64 MFMA rate the padded products cost about as much
 // time as the memory stream; on the VALU the 8 x 4 products of a row are ~1/6
 // of it.  Workgroup = one row chunk x one group of kGNarrowA A columns; each
-// thread streams rows tid, tid + 256, ... (coalesced columns, two rows of loads
-// in flight) and keeps its 32 products in registers; a butterfly
+// thread streams rows in groups of kGNarrowRows (rows g + kT r: coalesced
+// columns, all 12 columns of every row of the group loaded before the
+// products) and keeps its 32 products in registers; a butterfly
 // reduce-scatter over the wave and a sum over the four waves through LDS give
 // the chunk's partials.
+constexpr int kGNarrowRows = 4;
 template <typename TA, typename TB>
 __device__ __forceinline__ void gram_rows_narrow(gptr<const TA> A, long long lda, int na,
                                                  gptr<const TB> B, long long ldb, int nb,
                                                  long long r0, long long r1,
                                                  double (&acc)[kGNarrowA * kGNarrowB])
 {
-    // columns past na / nb read column 0 and are weighted 0
+    constexpr int R = kGNarrowRows;
+    // columns past na / nb read column 0 and are weighted 0; rows past r1
+    // read row g and get weight 0 on B
     double wa[kGNarrowA], wb[kGNarrowB];
 #pragma unroll
     for (int i = 0; i < kGNarrowA; ++i) wa[i] = i < na ? 1.0 : 0.0;
 #pragma unroll
     for (int j = 0; j < kGNarrowB; ++j) wb[j] = j < nb ? 1.0 : 0.0;
-#pragma unroll 2
-    for (long long g = r0 + threadIdx.x; g < r1; g += kT) {
-        double a[kGNarrowA], b[kGNarrowB];
+    for (long long g = r0 + threadIdx.x; g < r1; g += kT * R) {
+        long long gi[R];
+        double wr[R];
 #pragma unroll
-        for (int i = 0; i < kGNarrowA; ++i) a[i] = (double)A[g + (i < na ? i : 0) * lda];
+        for (int r = 0; r < R; ++r) {
+            const bool in = g + kT * r < r1;
+            gi[r] = in ? g + kT * r : g;
+            wr[r] = in ? 1.0 : 0.0;
+        }
+        TA a[R][kGNarrowA];
+        TB b[R][kGNarrowB];
 #pragma unroll
-        for (int j = 0; j < kGNarrowB; ++j) b[j] = (double)B[g + (j < nb ? j : 0) * ldb];
+        for (int r = 0; r < R; ++r) {
 #pragma unroll
-        for (int i = 0; i < kGNarrowA; ++i)
+            for (int i = 0; i < kGNarrowA; ++i) a[r][i] = A[gi[r] + (i < na ? i : 0) * lda];
 #pragma unroll
-            for (int j = 0; j < kGNarrowB; ++j)
-                acc[i * kGNarrowB + j] = fma(a[i] * wa[i], b[j] * wb[j], acc[i * kGNarrowB + j]);
+            for (int j = 0; j < kGNarrowB; ++j) b[r][j] = B[gi[r] + (j < nb ? j : 0) * ldb];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            double bv[kGNarrowB];
+#pragma unroll
+            for (int j = 0; j < kGNarrowB; ++j) bv[j] = (double)b[r][j] * (wb[j] * wr[r]);
+#pragma unroll
+            for (int i = 0; i < kGNarrowA; ++i) {
+                const double av = (double)a[r][i] * wa[i];
+#pragma unroll
+                for (int j = 0; j < kGNarrowB; ++j)
+                    acc[i * kGNarrowB + j] = fma(av, bv[j], acc[i * kGNarrowB + j]);
+            }
+        }
     }
 }
 
@@ -773,84 +796,105 @@ __global__ void __launch_bounds__(64) resid_reduce_kernel(const ResTask *__restr
 }
 
 // ------------------------------------------------------------------ Ritz vectors + residuals
-// u[b] += sum_a A[a*N + g] y[(a0 + a)][b], hu likewise with HA: four columns
-// (eight loads) in flight
-template <int KB, typename T>
-__device__ __forceinline__ void ritz_cols(gptr<const T> A, gptr<const T> HA,
-                                          long long N, int na, const double *y, int a0,
-                                          long long g, double (&u)[KB], double (&hu)[KB])
+// G consecutive basis columns from column a (coefficient rows a0 + a ..) for
+// RPT rows per thread: the 2 G RPT loads are issued before the products;
+// coefficient columns past k are 0 from a clamped index (no branch).
+template <typename T, int KB, int RPT, int G>
+__device__ __forceinline__ void ritz_group(gptr<const T> A, gptr<const T> HA, long long N, int a,
+                                           cptr<double> Y, int m, int a0, int k,
+                                           const long long (&gi)[RPT], double (&u)[RPT][KB],
+                                           double (&hu)[RPT][KB])
 {
-    int a = 0;
-    for (; a + 4 <= na; a += 4) {
-        double x[4], h[4];
+    T x[RPT][G], h[RPT][G];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            x[q] = A[(a + q) * N + g];
-            h[q] = HA[(a + q) * N + g];
+    for (int q = 0; q < G; ++q)
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            x[r][q] = A[(long long)(a + q) * N + gi[r]];
+            h[r][q] = HA[(long long)(a + q) * N + gi[r]];
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double *wq = y + (a0 + a + q) * KB;
+    for (int b = 0; b < KB; ++b) {
+        const int bc = b < k ? b : k - 1;
 #pragma unroll
-            for (int b = 0; b < KB; ++b) {
-                u[b] = fma(x[q], wq[b], u[b]);
-                hu[b] = fma(h[q], wq[b], hu[b]);
+        for (int q = 0; q < G; ++q) {
+            const double wl = Y[(a0 + a + q) + (long long)m * bc];
+            const double w = b < k ? wl : 0.0;
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                u[r][b] = fma((double)x[r][q], w, u[r][b]);
+                hu[r][b] = fma((double)h[r][q], w, hu[r][b]);
             }
-        }
-    }
-    for (; a < na; ++a) {
-        const double x0 = A[a * N + g], h0 = HA[a * N + g];
-        const double *w0 = y + (a0 + a) * KB;
-#pragma unroll
-        for (int b = 0; b < KB; ++b) {
-            u[b] += x0 * w0[b];
-            hu[b] += h0 * w0[b];
         }
     }
 }
 
-// KB >= k: block size rounded up; coefficient columns k .. KB-1 are zero.
-template <int KB>
+// groups of 4 (FP32) or 2 (FP64) columns (half that for 16-column blocks),
+// then single columns
+template <typename T, int KB, int RPT>
+__device__ __forceinline__ void ritz_cols(gptr<const T> A, gptr<const T> HA, long long N, int na,
+                                          cptr<double> Y, int m, int a0, int k,
+                                          const long long (&gi)[RPT], double (&u)[RPT][KB],
+                                          double (&hu)[RPT][KB])
+{
+    constexpr int G = (sizeof(T) == 4 ? 4 : 2) / (KB > 8 ? 2 : 1);
+    int a = 0;
+    for (; a + G <= na; a += G) ritz_group<T, KB, RPT, G>(A, HA, N, a, Y, m, a0, k, gi, u, hu);
+    for (; a < na; ++a) ritz_group<T, KB, RPT, 1>(A, HA, N, a, Y, m, a0, k, gi, u, hu);
+}
+
+// KB >= k: block size rounded up.  Workgroup = kT * RPT rows (thread: rows
+// g0 + kT r); the Ritz coefficients and theta are scalar loads through the
+// constant address space.  Rows past N read row 0 and are neither stored nor
+// counted.
+template <int KB, int RPT>
 __global__ void __launch_bounds__(kT) ritz_kernel(const RitzTask *__restrict__ tasks,
                                                   double *__restrict__ partial)
 {
     const RitzTask &t = tasks[blockIdx.y];
     if ((int)blockIdx.x >= t.nch) return;
     const int k = t.k, mc = t.mc, m = k + mc;
-    __shared__ double y[kBdMaxM * KB];  // y[a][b], b fastest
-    __shared__ double th[KB];
     __shared__ double red[KB][kT / 64];
-    for (int idx = threadIdx.x; idx < m * KB; idx += kT) {
-        const int a = idx / KB, b = idx - a * KB;
-        y[idx] = b < k ? t.Y[a + (long long)m * b] : 0.0;
-    }
-    if (threadIdx.x < KB) th[threadIdx.x] = threadIdx.x < k ? t.theta[threadIdx.x] : 0.0;
-    __syncthreads();
-    // one row per thread (rows of a block are consecutive: coalesced columns)
-    const long long g = (long long)blockIdx.x * kRitzRows + threadIdx.x;
-    const bool row = g < t.N;
-    double u[KB], hu[KB];
+    const cptr<double> Y = cst(t.Y);
+    const long long g0 = (long long)blockIdx.x * (kT * RPT) + threadIdx.x;
+    long long gi[RPT];
+    bool ok[RPT];
 #pragma unroll
-    for (int b = 0; b < KB; ++b) u[b] = hu[b] = 0.0;
-    if (row) {
-        ritz_cols(glb(t.X), glb(t.HX), t.N, k, y, 0, g, u, hu);
-        if (t.c32)
-            ritz_cols(glb(static_cast<const float *>(t.C)), glb(static_cast<const float *>(t.HC)), t.N,
-                      mc, y, k, g, u, hu);
-        else
-            ritz_cols(glb(static_cast<const double *>(t.C)), glb(static_cast<const double *>(t.HC)),
-                      t.N, mc, y, k, g, u, hu);
+    for (int r = 0; r < RPT; ++r) {
+        ok[r] = g0 + kT * r < t.N;
+        gi[r] = ok[r] ? g0 + kT * r : 0;
     }
+    double u[RPT][KB], hu[RPT][KB];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+#pragma unroll
+        for (int b = 0; b < KB; ++b) u[r][b] = hu[r][b] = 0.0;
+    ritz_cols<double, KB, RPT>(glb(t.X), glb(t.HX), t.N, k, Y, m, 0, k, gi, u, hu);
+    if (t.c32)
+        ritz_cols<float, KB, RPT>(glb(static_cast<const float *>(t.C)),
+                                  glb(static_cast<const float *>(t.HC)), t.N, mc, Y, m, k, k, gi, u, hu);
+    else
+        ritz_cols<double, KB, RPT>(glb(static_cast<const double *>(t.C)),
+                                   glb(static_cast<const double *>(t.HC)), t.N, mc, Y, m, k, k, gi, u,
+                                   hu);
+    const cptr<double> th = cst(t.theta);
     double nrm[KB];
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
         nrm[b] = 0.0;
-        if (b < k && row) {
-            const double r = hu[b] - th[b] * u[b];
-            if (t.store & kRitzU) glb(t.U)[b * t.N + g] = u[b];
-            if (t.store & kRitzHU) glb(t.HU)[b * t.N + g] = hu[b];
-            st_any(t.R, b * t.N + g, t.r32, r);
-            nrm[b] = r * r;
+        if (b < k) {
+            const double thb = th[b];
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                if (ok[r]) {
+                    const long long gr = gi[r];
+                    const double res = hu[r][b] - thb * u[r][b];
+                    if (t.store & kRitzU) glb(t.U)[b * t.N + gr] = u[r][b];
+                    if (t.store & kRitzHU) glb(t.HU)[b * t.N + gr] = hu[r][b];
+                    st_any(t.R, b * t.N + gr, t.r32, res);
+                    nrm[b] += res * res;
+                }
+            }
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1175,16 +1219,20 @@ void bd_coef(const CoefTask *d_tasks, int ntasks, hipStream_t s)
     PC_HIP(hipGetLastError());
 }
 
+int ritz_rows(int k) { return kT * (k <= 4 ? 2 : 1); }
+
 void bd_ritz(const RitzTask *d_tasks, int ntasks, int max_nch, int k, double *d_partial,
              hipStream_t s)
 {
     if (ntasks <= 0) return;
     PC_REQUIRE(k >= 1 && k <= kBdMaxK, PISCES_ERR_ARG, "block size must be 1..16");
+    PC_REQUIRE(max_nch >= 1, PISCES_ERR_ARG, "no rows");
     const dim3 grid(max_nch, ntasks);
-    if (k <= 2) hipLaunchKernelGGL(ritz_kernel<2>, grid, dim3(kT), 0, s, d_tasks, d_partial);
-    else if (k <= 4) hipLaunchKernelGGL(ritz_kernel<4>, grid, dim3(kT), 0, s, d_tasks, d_partial);
-    else if (k <= 8) hipLaunchKernelGGL(ritz_kernel<8>, grid, dim3(kT), 0, s, d_tasks, d_partial);
-    else hipLaunchKernelGGL(ritz_kernel<16>, grid, dim3(kT), 0, s, d_tasks, d_partial);
+    // rows per workgroup = kT * RPT = ritz_rows(k)
+    if (k <= 2) hipLaunchKernelGGL((ritz_kernel<2, 2>), grid, dim3(kT), 0, s, d_tasks, d_partial);
+    else if (k <= 4) hipLaunchKernelGGL((ritz_kernel<4, 2>), grid, dim3(kT), 0, s, d_tasks, d_partial);
+    else if (k <= 8) hipLaunchKernelGGL((ritz_kernel<8, 1>), grid, dim3(kT), 0, s, d_tasks, d_partial);
+    else hipLaunchKernelGGL((ritz_kernel<16, 1>), grid, dim3(kT), 0, s, d_tasks, d_partial);
     hipLaunchKernelGGL(ritz_reduce_kernel, dim3(ntasks), dim3(64), 0, s, d_tasks, d_partial);
     PC_HIP(hipGetLastError());
 }

@@ -153,6 +153,7 @@ int gram_variant();
 int gram_chunks(long long N);
 int gram_blocks_per_chunk(int ma, int mb);
 constexpr int kBdResChunk = 1024; // rows per residual chunk
-constexpr int kRitzRows = 256;    // rows per Ritz/residual workgroup (one per thread)
+// rows per Ritz workgroup for block size k (RitzTask::nch = ceil(N / ritz_rows(k)))
+int ritz_rows(int k);
 
 }  // namespace pisces

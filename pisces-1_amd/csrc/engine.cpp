@@ -610,7 +610,7 @@ void BdEngine::step_a()
         in->u_valid = (store & kRitzU) != 0;
         RitzTask t{in->X.as<double>(), in->HX.as<double>(), in->corr(0), in->HC.p, k_, in->mc, e32,
                    r32, Y, in->sm() + oTh, in->U.as<double>(), in->HU.as<double>(), in->R.p, N,
-                   res_part, (int)((N + kRitzRows - 1) / kRitzRows), store, in->sm() + oRn,
+                   res_part, (int)((N + ritz_rows(k_) - 1) / ritz_rows(k_)), store, in->sm() + oRn,
                    late ? nullptr : dgather_.as<double>() + (size_t)2 * kBdMaxK * i};
         hrz[i] = t;
         // X, HX, C, HC read; R and the stored ones of U, HU written
