@@ -347,7 +347,7 @@ constexpr int kGNarrowRows = 4;
 template <typename TA, typename TB>
 __device__ __forceinline__ void gram_rows_narrow(gptr<const TA> A, long long lda, int na,
                                                  gptr<const TB> B, long long ldb, int nb,
-                                                 long long r0, long long r1,
+                                                 long long r0, long long r1, long long S,
                                                  double (&acc)[kGNarrowA * kGNarrowB])
 {
     constexpr int R = kGNarrowRows;
@@ -358,13 +358,13 @@ __device__ __forceinline__ void gram_rows_narrow(gptr<const TA> A, long long lda
     for (int i = 0; i < kGNarrowA; ++i) wa[i] = i < na ? 1.0 : 0.0;
 #pragma unroll
     for (int j = 0; j < kGNarrowB; ++j) wb[j] = j < nb ? 1.0 : 0.0;
-    for (long long g = r0 + threadIdx.x; g < r1; g += kT * R) {
+    for (long long g = r0; g < r1; g += S * R) {
         long long gi[R];
         double wr[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const bool in = g + kT * r < r1;
-            gi[r] = in ? g + kT * r : g;
+            const bool in = g + S * r < r1;
+            gi[r] = in ? g + S * r : g;
             wr[r] = in ? 1.0 : 0.0;
         }
         TA a[R][kGNarrowA];
@@ -427,12 +427,12 @@ __global__ void __launch_bounds__(kT) gram_narrow_kernel(const GramTask *__restr
     const gptr<const double> Bd = glb(static_cast<const double *>(t.B));
     if (t.a32) {
         const gptr<const float> A = glb(static_cast<const float *>(Aa));
-        if (t.b32) gram_rows_narrow(A, t.lda, na, Bf, t.ldb, nb, r0, r1, acc);
-        else gram_rows_narrow(A, t.lda, na, Bd, t.ldb, nb, r0, r1, acc);
+        if (t.b32) gram_rows_narrow(A, t.lda, na, Bf, t.ldb, nb, r0 + threadIdx.x, r1, kT, acc);
+        else gram_rows_narrow(A, t.lda, na, Bd, t.ldb, nb, r0 + threadIdx.x, r1, kT, acc);
     } else {
         const gptr<const double> A = glb(static_cast<const double *>(Aa));
-        if (t.b32) gram_rows_narrow(A, t.lda, na, Bf, t.ldb, nb, r0, r1, acc);
-        else gram_rows_narrow(A, t.lda, na, Bd, t.ldb, nb, r0, r1, acc);
+        if (t.b32) gram_rows_narrow(A, t.lda, na, Bf, t.ldb, nb, r0 + threadIdx.x, r1, kT, acc);
+        else gram_rows_narrow(A, t.lda, na, Bd, t.ldb, nb, r0 + threadIdx.x, r1, kT, acc);
     }
     // lane L ends with the sum over its half-wave of value L % 32, then over both halves
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -576,6 +576,24 @@ __device__ inline int rr_index(int i, int r, int mp)
 }
 
 constexpr int kEigT = 256;
+
+// 1/x and 1/sqrt(x) for the rotation parameters: the hardware estimates
+// refined by two Newton-Raphson steps each (r <- r + r (1 - x r);
+// y <- y (3 - x y^2) / 2), each step squaring the relative error.
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+__device__ __forceinline__ double rsq64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x, y * y, 1.5);
+    y = y * fma(-0.5 * x, y * y, 1.5);
+    return y;
+}
 constexpr int kEigSweeps = 40;
 
 // Cyclic Jacobi with round-robin pairing: every round applies m/2 disjoint
@@ -674,11 +692,17 @@ __global__ void __launch_bounds__(kEigT) eig_kernel(const EigTask *__restrict__ 
                     const double apq = A[p + LD * q];
                     const double app = A[p + LD * p], aqq = A[q + LD * q];
                     if (fabs(apq) > 1e-17 * (fabs(app) + fabs(aqq)) + 1e-300) {
-                        const double th = (aqq - app) / (2.0 * apq);
+                        // t = tan of the annihilating angle to ~1 ulp; any t
+                        // gives an exact rotation (c^2 + s^2 = 1 to rounding)
+                        const double th = (aqq - app) * rcp64(2.0 * apq);
                         double tt;
-                        if (fabs(th) > 1e150) tt = 0.5 / th;
-                        else tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-                        c = 1.0 / sqrt(tt * tt + 1.0);
+                        if (fabs(th) > 1e150) {
+                            tt = 0.5 * rcp64(th);
+                        } else {
+                            const double h = fma(th, th, 1.0);
+                            tt = copysign(rcp64(fabs(th) + h * rsq64(h)), th);
+                        }
+                        c = rsq64(fma(tt, tt, 1.0));
                         s = tt * c;
                         s_rot = 1;  // every writer stores 1
                     }
