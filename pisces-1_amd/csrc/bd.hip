@@ -413,7 +413,10 @@ __global__ void __launch_bounds__(kT) gram_narrow_kernel(const GramTask *__restr
     if (t.kind != 1) return;
     const int ntile = t.cpb;
     if ((int)blockIdx.x >= t.nch * ntile) return;
-    const int chunk = blockIdx.x / ntile, tile = blockIdx.x - chunk * ntile;
+    // tile-major numbering: workgroups are dealt to the 8 XCDs round-robin, so
+    // with nch a multiple of 8 every tile of a chunk runs on the same XCD,
+    // whose L2 then serves the chunk's B columns to all of them
+    const int tile = blockIdx.x / t.nch, chunk = blockIdx.x - tile * t.nch;
     const int a0 = tile * kGNarrowA;
     const int na = min(kGNarrowA, t.ma - a0), nb = t.mb;
     const long long r0 = (long long)chunk * kGRowsNarrow;
