@@ -344,7 +344,8 @@ __global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ t
 // reduce-scatter over the wave and a sum over the four waves through LDS give
 // the chunk's partials.
 constexpr int kGNarrowRows = 4;
-template <typename TA, typename TB>
+// NA: A columns loaded per row (8, or 4 for a tile with at most 4 columns)
+template <int NA, typename TA, typename TB>
 __device__ __forceinline__ void gram_rows_narrow(gptr<const TA> A, long long lda, int na,
                                                  gptr<const TB> B, long long ldb, int nb,
                                                  long long r0, long long r1, long long S,
@@ -353,9 +354,9 @@ __device__ __forceinline__ void gram_rows_narrow(gptr<const TA> A, long long lda
     constexpr int R = kGNarrowRows;
     // columns past na / nb read column 0 and are weighted 0; rows past r1
     // read row g and get weight 0 on B
-    double wa[kGNarrowA], wb[kGNarrowB];
+    double wa[NA], wb[kGNarrowB];
 #pragma unroll
-    for (int i = 0; i < kGNarrowA; ++i) wa[i] = i < na ? 1.0 : 0.0;
+    for (int i = 0; i < NA; ++i) wa[i] = i < na ? 1.0 : 0.0;
 #pragma unroll
     for (int j = 0; j < kGNarrowB; ++j) wb[j] = j < nb ? 1.0 : 0.0;
     for (long long g = r0; g < r1; g += S * R) {
@@ -367,12 +368,12 @@ __device__ __forceinline__ void gram_rows_narrow(gptr<const TA> A, long long lda
             gi[r] = in ? g + S * r : g;
             wr[r] = in ? 1.0 : 0.0;
         }
-        TA a[R][kGNarrowA];
+        TA a[R][NA];
         TB b[R][kGNarrowB];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
 #pragma unroll
-            for (int i = 0; i < kGNarrowA; ++i) a[r][i] = A[gi[r] + (i < na ? i : 0) * lda];
+            for (int i = 0; i < NA; ++i) a[r][i] = A[gi[r] + (i < na ? i : 0) * lda];
 #pragma unroll
             for (int j = 0; j < kGNarrowB; ++j) b[r][j] = B[gi[r] + (j < nb ? j : 0) * ldb];
         }
@@ -382,7 +383,7 @@ __device__ __forceinline__ void gram_rows_narrow(gptr<const TA> A, long long lda
 #pragma unroll
             for (int j = 0; j < kGNarrowB; ++j) bv[j] = (double)b[r][j] * (wb[j] * wr[r]);
 #pragma unroll
-            for (int i = 0; i < kGNarrowA; ++i) {
+            for (int i = 0; i < NA; ++i) {
                 const double av = (double)a[r][i] * wa[i];
 #pragma unroll
                 for (int j = 0; j < kGNarrowB; ++j)
@@ -428,14 +429,23 @@ __global__ void __launch_bounds__(kT) gram_narrow_kernel(const GramTask *__restr
     const void *Aa = static_cast<const unsigned char *>(t.A) + ea * (size_t)t.lda * a0;
     const gptr<const float> Bf = glb(static_cast<const float *>(t.B));
     const gptr<const double> Bd = glb(static_cast<const double *>(t.B));
+    const long long g0 = r0 + threadIdx.x;
     if (t.a32) {
         const gptr<const float> A = glb(static_cast<const float *>(Aa));
-        if (t.b32) gram_rows_narrow(A, t.lda, na, Bf, t.ldb, nb, r0 + threadIdx.x, r1, kT, acc);
-        else gram_rows_narrow(A, t.lda, na, Bd, t.ldb, nb, r0 + threadIdx.x, r1, kT, acc);
+        if (t.b32) {
+            if (na <= 4) gram_rows_narrow<4>(A, t.lda, na, Bf, t.ldb, nb, g0, r1, kT, acc);
+            else gram_rows_narrow<8>(A, t.lda, na, Bf, t.ldb, nb, g0, r1, kT, acc);
+        } else {
+            gram_rows_narrow<8>(A, t.lda, na, Bd, t.ldb, nb, g0, r1, kT, acc);
+        }
     } else {
         const gptr<const double> A = glb(static_cast<const double *>(Aa));
-        if (t.b32) gram_rows_narrow(A, t.lda, na, Bf, t.ldb, nb, r0 + threadIdx.x, r1, kT, acc);
-        else gram_rows_narrow(A, t.lda, na, Bd, t.ldb, nb, r0 + threadIdx.x, r1, kT, acc);
+        if (t.b32) {
+            if (na <= 4) gram_rows_narrow<4>(A, t.lda, na, Bf, t.ldb, nb, g0, r1, kT, acc);
+            else gram_rows_narrow<8>(A, t.lda, na, Bf, t.ldb, nb, g0, r1, kT, acc);
+        } else {
+            gram_rows_narrow<8>(A, t.lda, na, Bd, t.ldb, nb, g0, r1, kT, acc);
+        }
     }
     // lane L ends with the sum over its half-wave of value L % 32, then over both halves
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
