@@ -21,6 +21,12 @@ namespace pisces {
 namespace {
 
 constexpr int kT = 256;
+// combinations and Ritz passes walk their grids in reverse (the Gram passes
+// they follow walk forward)
+#ifndef PISCES_COMB_REVERSE
+#define PISCES_COMB_REVERSE 1
+#endif
+constexpr int kCombReverse = PISCES_COMB_REVERSE;
 
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -545,11 +551,18 @@ __device__ __forceinline__ void comb_source(const void *A, long long lda, int na
     else comb_cols<double, MB, RPT>(glb(static_cast<const double *>(A)), lda, na, cst(W), ldw, mb, gi, acc);
 }
 
+// Launches that read the columns the previous launch read walk the grid in
+// reverse order (rev = 1): workgroups are dispatched in order of their linear
+// index, so such a launch starts on the rows whose lines the previous one
+// left last in the L2 and memory-side caches.
+__device__ __forceinline__ int grid_x(int rev) { return rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x; }
+__device__ __forceinline__ int grid_y(int rev) { return rev ? gridDim.y - 1 - blockIdx.y : blockIdx.y; }
+
 template <int MB, int RPT>
-__global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ tasks)
+__global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ tasks, int rev)
 {
-    const CombTask t = tasks[blockIdx.y];
-    const long long g0 = (long long)blockIdx.x * (kT * RPT) + threadIdx.x;
+    const CombTask t = tasks[grid_y(rev)];
+    const long long g0 = (long long)grid_x(rev) * (kT * RPT) + threadIdx.x;
     if (g0 >= t.N) return;
     long long gi[RPT];
 #pragma unroll
@@ -886,14 +899,15 @@ __device__ __forceinline__ void ritz_cols(gptr<const T> A, gptr<const T> HA, lon
 // counted.
 template <int KB, int RPT>
 __global__ void __launch_bounds__(kT) ritz_kernel(const RitzTask *__restrict__ tasks,
-                                                  double *__restrict__ partial)
+                                                  double *__restrict__ partial, int rev)
 {
-    const RitzTask &t = tasks[blockIdx.y];
-    if ((int)blockIdx.x >= t.nch) return;
+    const RitzTask &t = tasks[grid_y(rev)];
+    const int bx = grid_x(rev);
+    if (bx >= t.nch) return;
     const int k = t.k, mc = t.mc, m = k + mc;
     __shared__ double red[KB][kT / 64];
     const cptr<double> Y = cst(t.Y);
-    const long long g0 = (long long)blockIdx.x * (kT * RPT) + threadIdx.x;
+    const long long g0 = (long long)bx * (kT * RPT) + threadIdx.x;
     long long gi[RPT];
     bool ok[RPT];
 #pragma unroll
@@ -945,7 +959,7 @@ __global__ void __launch_bounds__(kT) ritz_kernel(const RitzTask *__restrict__ t
     if (threadIdx.x < k) {
         double sum = 0;
         for (int w = 0; w < kT / 64; ++w) sum += red[threadIdx.x][w];
-        partial[t.part_off + (long long)threadIdx.x * t.nch + blockIdx.x] = sum;
+        partial[t.part_off + (long long)threadIdx.x * t.nch + bx] = sum;
     }
 }
 
@@ -1243,9 +1257,10 @@ void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, int max_mb
     if (ntasks <= 0) return;
     PC_REQUIRE(max_mb >= 1 && max_mb <= kBdMaxK, PISCES_ERR_ARG, "combination width 1..16");
     auto grid = [&](int rpt) { return dim3((unsigned)((max_N + kT * rpt - 1) / (kT * rpt)), ntasks); };
-    if (max_mb <= 4) hipLaunchKernelGGL((comb_kernel<4, 4>), grid(4), dim3(kT), 0, s, d_tasks);
-    else if (max_mb <= 8) hipLaunchKernelGGL((comb_kernel<8, 2>), grid(2), dim3(kT), 0, s, d_tasks);
-    else hipLaunchKernelGGL((comb_kernel<16, 1>), grid(1), dim3(kT), 0, s, d_tasks);
+    const int rev = kCombReverse;
+    if (max_mb <= 4) hipLaunchKernelGGL((comb_kernel<4, 4>), grid(4), dim3(kT), 0, s, d_tasks, rev);
+    else if (max_mb <= 8) hipLaunchKernelGGL((comb_kernel<8, 2>), grid(2), dim3(kT), 0, s, d_tasks, rev);
+    else hipLaunchKernelGGL((comb_kernel<16, 1>), grid(1), dim3(kT), 0, s, d_tasks, rev);
     PC_HIP(hipGetLastError());
 }
 
@@ -1266,10 +1281,11 @@ void bd_ritz(const RitzTask *d_tasks, int ntasks, int max_nch, int k, double *d_
     PC_REQUIRE(max_nch >= 1, PISCES_ERR_ARG, "no rows");
     const dim3 grid(max_nch, ntasks);
     // rows per workgroup = kT * RPT = ritz_rows(k)
-    if (k <= 2) hipLaunchKernelGGL((ritz_kernel<2, 2>), grid, dim3(kT), 0, s, d_tasks, d_partial);
-    else if (k <= 4) hipLaunchKernelGGL((ritz_kernel<4, 2>), grid, dim3(kT), 0, s, d_tasks, d_partial);
-    else if (k <= 8) hipLaunchKernelGGL((ritz_kernel<8, 1>), grid, dim3(kT), 0, s, d_tasks, d_partial);
-    else hipLaunchKernelGGL((ritz_kernel<16, 1>), grid, dim3(kT), 0, s, d_tasks, d_partial);
+    const int rev = kCombReverse;
+    if (k <= 2) hipLaunchKernelGGL((ritz_kernel<2, 2>), grid, dim3(kT), 0, s, d_tasks, d_partial, rev);
+    else if (k <= 4) hipLaunchKernelGGL((ritz_kernel<4, 2>), grid, dim3(kT), 0, s, d_tasks, d_partial, rev);
+    else if (k <= 8) hipLaunchKernelGGL((ritz_kernel<8, 1>), grid, dim3(kT), 0, s, d_tasks, d_partial, rev);
+    else hipLaunchKernelGGL((ritz_kernel<16, 1>), grid, dim3(kT), 0, s, d_tasks, d_partial, rev);
     hipLaunchKernelGGL(ritz_reduce_kernel, dim3(ntasks), dim3(64), 0, s, d_tasks, d_partial);
     PC_HIP(hipGetLastError());
 }
