@@ -603,23 +603,6 @@ __device__ inline int rr_index(int i, int r, int mp)
 
 constexpr int kEigT = 256;
 
-// 1/x and 1/sqrt(x) for the rotation parameters: the hardware estimates
-// refined by two Newton-Raphson steps each (r <- r + r (1 - x r);
-// y <- y (3 - x y^2) / 2), each step squaring the relative error.
-__device__ __forceinline__ double rcp64(double x)
-{
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
-    r = fma(r, fma(-x, r, 1.0), r);
-    return r;
-}
-__device__ __forceinline__ double rsq64(double x)
-{
-    double y = __builtin_amdgcn_rsq(x);
-    y = y * fma(-0.5 * x, y * y, 1.5);
-    y = y * fma(-0.5 * x, y * y, 1.5);
-    return y;
-}
 constexpr int kEigSweeps = 40;
 
 // Cyclic Jacobi with round-robin pairing: every round applies m/2 disjoint

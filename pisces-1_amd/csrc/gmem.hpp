@@ -1,4 +1,4 @@
-// gmem.hpp -- global-address-space pointers for device code.
+// gmem.hpp -- address-space pointers and FP64 reciprocals for device code.
 //
 // Pointers that kernels read out of task descriptors are generic to the
 // compiler, which then emits flat memory instructions: they count against both
@@ -22,5 +22,23 @@ template <typename T> __device__ __forceinline__ gptr<T> glb(T *p) { return (gpt
 template <typename T> using cptr = __attribute__((address_space(4))) const T *;
 
 template <typename T> __device__ __forceinline__ cptr<T> cst(const T *p) { return (cptr<T>)p; }
+
+// 1/x and 1/sqrt(x) in FP64 to about one ulp: the hardware estimates refined
+// by two Newton-Raphson steps each (r <- r + r (1 - x r);
+// y <- y (3 - x y^2) / 2), each step squaring the relative error.
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+__device__ __forceinline__ double rsq64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x, y * y, 1.5);
+    y = y * fma(-0.5 * x, y * y, 1.5);
+    return y;
+}
 
 }  // namespace pisces

@@ -9,9 +9,13 @@
 // them.  The per-site distance table of the reference (Rx, Ry, Rz, R2, R,
 // Rminus3; src/Potential.cpp:136-146) is recomputed per use instead of stored,
 // with the same expressions, so the FP64 result agrees with the reference to
-// rounding.  The work is N_grid x N_site FP64 arithmetic with one sqrt per site
-// and one exp per repulsive core: compute-bound on the FP64 VALU.
+// rounding.  The work is N_grid x N_site FP64 arithmetic with one reciprocal
+// square root per site (1/R to ~1 ulp; R, R^-3, R^-4 follow by products, exact
+// divisions only inside the damping radii) and one exp per repulsive core:
+// compute-bound on the FP64 VALU.  The per-cluster tables are read as scalar
+// loads through the constant address space.
 #include "common.hpp"
+#include "gmem.hpp"
 
 namespace pisces {
 
@@ -34,6 +38,9 @@ __device__ inline double reff(double r, double r0)
     return r;
 }
 
+// |r| from r^2 and 1/|r| (0 at r = 0, where 1/|r| is infinite)
+__device__ __forceinline__ double norm_of(double r2, double ri) { return r2 > 0.0 ? r2 * ri : 0.0; }
+
 __device__ void potential_point(const PotParams &P, const PotDev &D, const SiteLds *__restrict__ S,
                                 int nx, int ny, int nz, const double *__restrict__ gx,
                                 const double *__restrict__ gy, const double *__restrict__ gz,
@@ -43,34 +50,38 @@ __device__ void potential_point(const PotParams &P, const PotDev &D, const SiteL
     const int j = (int)((g / nx) % ny);
     const int k = (int)(g / ((long long)nx * ny));
     const double r0 = gx[i], r1 = gy[j], r2 = gz[k];
+    const cptr<double> q = cst(D.q), dm = cst(D.d), rp = cst(D.rep), ionx = cst(D.ion_xyz),
+                       ionq = cst(D.ion_q);
+    const cptr<int> iq = cst(D.iq), id = cst(D.id), irep = cst(D.irep), ipps = cst(D.ipps);
 
     // point charges, src/Potential.cpp:1168-1210
     double vpc = 0.0;
     for (int c = 0; c < D.nq; ++c) {
-        const SiteLds s = S[D.iq[c]];
+        const SiteLds s = S[iq[c]];
         const double rx = r0 - s.x, ry = r1 - s.y, rz = r2 - s.z;
-        const double R = sqrt(rx * rx + ry * ry + rz * rz);
+        const double R2 = rx * rx + ry * ry + rz * rz;
+        const double ri = rsq64(R2);
         if (P.damp_type == 1) {
-            const double R2 = rx * rx + ry * ry + rz * rz;
-            vpc += -D.q[c] / R * (1.0 - exp(-P.qdamp * R2));
+            vpc += -q[c] * ri * (1.0 - exp(-P.qdamp * R2));
         } else {
-            vpc += -D.q[c] / reff(R, P.qdamp);
+            const double R = norm_of(R2, ri);
+            vpc += (R < P.qdamp) ? -q[c] / reff(R, P.qdamp) : -q[c] * ri;
         }
     }
     // point dipoles, src/Potential.cpp:1222-1270
     double vind = 0.0;
     for (int c = 0; c < D.nd; ++c) {
-        const SiteLds s = S[D.id[c]];
+        const SiteLds s = S[id[c]];
         const double rx = r0 - s.x, ry = r1 - s.y, rz = r2 - s.z;
         const double R2 = rx * rx + ry * ry + rz * rz;
-        const double R = sqrt(R2);
-        const double m0 = D.d[3 * c], m1 = D.d[3 * c + 1], m2 = D.d[3 * c + 2];
-        const double dot = rx * m0 + ry * m1 + rz * m2;
-        double rm3 = 1.0 / (R2 * R);
+        const double ri = rsq64(R2);
+        const double dot = rx * dm[3 * c] + ry * dm[3 * c + 1] + rz * dm[3 * c + 2];
+        double rm3 = ri * ri * ri;
         if (P.damp_type == 1) {
-            const double dm = 1.0 - exp(-P.ddamp * R2);
-            vind += -dot * rm3 * dm * dm;
+            const double dd = 1.0 - exp(-P.ddamp * R2);
+            vind += -dot * rm3 * dd * dd;
         } else {
+            const double R = norm_of(R2, ri);
             if (R < P.ddamp) {
                 const double re = reff(R, P.ddamp);
                 rm3 = 1.0 / (re * re * re);
@@ -82,11 +93,11 @@ __device__ void potential_point(const PotParams &P, const PotDev &D, const SiteL
     double vrep = 0.0;
     if (P.rep_type > 0) {
         for (int c = 0; c < D.nrep; ++c) {
-            const SiteLds s = S[D.irep[c]];
+            const SiteLds s = S[irep[c]];
             const double rx = r0 - s.x, ry = r1 - s.y, rz = r2 - s.z;
             const double R2 = rx * rx + ry * ry + rz * rz;
-            if (P.rep_type == 1) vrep += D.rep[2 * c + 1] * exp(-D.rep[2 * c] * R2);
-            else vrep += D.rep[2 * c + 1] * exp(-D.rep[2 * c] * sqrt(R2));
+            if (P.rep_type == 1) vrep += rp[2 * c + 1] * exp(-rp[2 * c] * R2);
+            else vrep += rp[2 * c + 1] * exp(-rp[2 * c] * norm_of(R2, rsq64(R2)));
         }
     }
     // polarization of non-interacting sites, src/Potential.cpp:1331-1374, :875-887
@@ -94,20 +105,21 @@ __device__ void potential_point(const PotParams &P, const PotDev &D, const SiteL
     if (P.pol_type == 1) {
         double sp = 0.0;
         for (int c = 0; c < D.npps; ++c) {
-            const SiteLds s = S[D.ipps[c]];
+            const SiteLds s = S[ipps[c]];
             const double rx = r0 - s.x, ry = r1 - s.y, rz = r2 - s.z;
             const double rsq = rx * rx + ry * ry + rz * rz;
+            const double ri = rsq64(rsq), ri2 = ri * ri;
             if (P.damp_type == 1) {
-                const double dm = 1.0 - exp(-P.pdamp * rsq);
-                sp += P.alpha * dm * dm / (rsq * rsq);
+                const double dd = 1.0 - exp(-P.pdamp * rsq);
+                sp += P.alpha * dd * dd * (ri2 * ri2);
             } else {
-                const double R = sqrt(rsq);
+                const double R = norm_of(rsq, ri);
                 if (R < P.pdamp) {
                     const double re = reff(R, P.pdamp);
                     const double re2 = re * re;
                     sp += P.alpha * rsq / (re2 * re2 * re2);
                 } else {
-                    sp += P.alpha / (rsq * rsq);
+                    sp += P.alpha * (ri2 * ri2);
                 }
             }
         }
@@ -116,10 +128,10 @@ __device__ void potential_point(const PotParams &P, const PotDev &D, const SiteL
     // ions: erf-damped charges (Bloomfield form, src/NaCl.cpp:300-307)
     double vion = 0.0;
     for (int c = 0; c < D.nion; ++c) {
-        const double rx = r0 - D.ion_xyz[3 * c], ry = r1 - D.ion_xyz[3 * c + 1],
-                     rz = r2 - D.ion_xyz[3 * c + 2];
-        const double R = sqrt(rx * rx + ry * ry + rz * rz);
-        vion += -D.ion_q[c] * erf(R / P.ion_r) / R;
+        const double rx = r0 - ionx[3 * c], ry = r1 - ionx[3 * c + 1], rz = r2 - ionx[3 * c + 2];
+        const double R2 = rx * rx + ry * ry + rz * rz;
+        const double ri = rsq64(R2);
+        vion += -ionq[c] * erf(norm_of(R2, ri) / P.ion_r) * ri;
     }
     const double vt = vpc + vind + vrep + vpol + vion;
     if (v) v[g] = vt;

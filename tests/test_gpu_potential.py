@@ -26,6 +26,29 @@ def test_build_potential_lists(pisces, oracle_mod, flagset):
     assert np.abs(V - Vo).max() <= 1e-13 * np.abs(Vo).max()
 
 
+@pytest.mark.parametrize("which", ["iq", "id"])
+def test_site_on_a_grid_point(pisces, oracle_mod, which):
+    """A charge site (which = iq) or a dipole site (id) exactly on a grid point
+    (zero distance): the damped forms stay finite and match the oracle."""
+    f = golden("potential_c1.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    n, L = (33, 33, 33), (36.0, 36.0, 36.0)
+    g = oracle_mod.Grid(n, L)
+    s0 = int(np.asarray(getattr(cl, which))[0])
+    mid = np.array([g.x[a][n[a] // 2] for a in range(3)])
+    sites = np.asarray(cl.sites, dtype=np.float64).reshape(-1, 3)
+    sites = (sites - sites[s0]) + mid
+    assert np.array_equal(sites[s0], mid)
+    cl.sites = sites.reshape(np.shape(cl.sites))
+    flags, pp = f["flags_default"], f["potpara_default"]
+    Vo = cl.potential_grid(g, flags, pp)
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential(flags, pp, cl.sites, cl.q, cl.iq, cl.d, cl.id)
+        V = ctx.get_potential()
+    assert np.all(np.isfinite(V)) and np.all(np.isfinite(Vo))
+    assert np.abs(V - Vo).max() <= 1e-13 * np.abs(Vo).max()
+
+
 @pytest.mark.parametrize("nw,rad,seed,n,L", [(6, 3.5, 1000, (32, 32, 32), 36.0),
                                              (24, 6.0, 2000, (64, 64, 64), 40.0),
                                              (45, 8.0, 3000, (48, 48, 48), 48.0)])
