@@ -562,6 +562,7 @@ template <int MB, int RPT>
 __global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ tasks, int rev)
 {
     const CombTask t = tasks[grid_y(rev)];
+    if (t.skip && *cst(t.skip)) return;
     const long long g0 = (long long)grid_x(rev) * (kT * RPT) + threadIdx.x;
     if (g0 >= t.N) return;
     long long gi[RPT];
@@ -972,7 +973,7 @@ __global__ void __launch_bounds__(64) ritz_reduce_kernel(const RitzTask *__restr
 __device__ void orth_factor(const double *__restrict__ hg, int hld, const double *__restrict__ G,
                             int gld, int m, int nt, double *__restrict__ Wo,
                             double *__restrict__ Fo, const int *kin, int *kout, int *kcopy,
-                            double tol)
+                            double tol, int *skip = nullptr, double skip_tol = 0.0)
 {
     constexpr int K = kBdMaxK;
     __shared__ double h[kBdMaxM * kBdMaxK];
@@ -1031,16 +1032,31 @@ __device__ void orth_factor(const double *__restrict__ hg, int hld, const double
         const int i = idx % nt, c = idx / nt;
         Wo[i + (long long)nt * c] = W[i + K * c];
     }
+    double dev = 0.0;  // max |W - I|, |F| over this thread's entries
+    for (int idx = tid; idx < nt * nt; idx += 64) {
+        const int i = idx % nt, c = idx / nt;
+        dev = fmax(dev, fabs(W[i + K * c] - (i == c ? 1.0 : 0.0)));
+    }
     if (Fo)
         for (int idx = tid; idx < m * nt; idx += 64) {
             const int a = idx % m, c = idx / m;
             double f = 0.0;
             for (int i = 0; i < nt; ++i) f -= h[a + m * i] * W[i + K * c];
             Fo[a + (long long)m * c] = f;
+            dev = fmax(dev, fabs(f));
         }
     if (tid < nt) {
         kout[tid] = keep[tid];
         if (kcopy) kcopy[tid] = keep[tid];
+    }
+    if (skip) {
+        int dropped = (tid < nt && !keep[tid]) ? 1 : 0;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            dev = fmax(dev, __shfl_xor(dev, off, 64));
+            dropped |= __shfl_xor(dropped, off, 64);
+        }
+        if (tid == 0) *skip = (!dropped && dev < skip_tol) ? 1 : 0;
     }
 }
 
@@ -1048,7 +1064,7 @@ __global__ void __launch_bounds__(64) orth_kernel(const OrthTask *__restrict__ t
 {
     const OrthTask &t = tasks[blockIdx.x];
     orth_factor(t.h, t.hld, t.G, t.gld, t.m, t.nt, t.W, t.F, t.keep_in, t.keep_out, t.keep_copy,
-                t.tol);
+                t.tol, t.skip, t.skip_tol);
 }
 
 __global__ void __launch_bounds__(64) cholinv_batched_kernel(const CholTask *__restrict__ tasks)

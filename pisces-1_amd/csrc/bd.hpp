@@ -29,7 +29,8 @@ struct GramTask {
 // the third source is optional (ma3 = 0).  Y2 (optional, mode 0 only, leading
 // dimension ldy) receives a second copy of Y, FP32 if y2_32.  Every row is
 // read in full before it is written, so Y may be one of the sources (the
-// combination is then done in place).
+// combination is then done in place).  `skip` (optional) is a flag written by
+// an earlier launch (orth_kernel) that turns the task into a no-op.
 struct CombTask {
     const void *A1; long long lda1; int ma1; int a1_32; const double *W1; int ldw1;
     const void *A2; long long lda2; int ma2; int a2_32; const double *W2; int ldw2;
@@ -37,6 +38,7 @@ struct CombTask {
     long long N;
     const void *A3; long long lda3; int ma3; int a3_32; const double *W3; int ldw3;
     void *Y2; int y2_32; int pad;
+    const int *skip;  // optional: the task does nothing when *skip != 0 (set by an earlier launch)
 };
 
 // Ritz vectors and residuals of one instance in one pass over the basis:
@@ -61,12 +63,16 @@ struct RitzTask {
 // = L L^T (columns whose pivot falls below tol * G_jj, or that keep_in marks
 // dead, are dropped), W = L^{-T}, F = -h W (m x nt, ld m); then (T - V h) W =
 // T W + V F has orthonormal columns orthogonal to V.
+// skip (optional): set to 1 when every column is kept and |W - I|, |F| stay
+// below skip_tol everywhere (T W + V F then equals T to well below the
+// rounding of FP32 columns, so the combination may be left out), else 0.
 struct OrthTask {
     const double *h; const double *G; int m; int nt;
     double *W; double *F;
     const int *keep_in; int *keep_out; int *keep_copy;
     double tol;
     int hld, gld;
+    int *skip; double skip_tol;
 };
 
 // Symmetric eigenproblem of the projected matrix: G (m x m, upper triangle,

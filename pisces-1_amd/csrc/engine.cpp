@@ -44,6 +44,11 @@ constexpr int oF = 8832;     // orthogonalisation coefficients on [X|C] ((k + mc
 constexpr int oHG = 9856;    // [Xc|C|T]^T T of an orthogonalisation round ((k + mc + nt) x nt)
 constexpr int oVf = oHG + (kBdMaxM + kBdMaxK) * kBdMaxK;  // eigenvectors of the last solve (m x m, ld kBdMaxM)
 constexpr int kSmall = oVf + kBdMaxM * kBdMaxM;
+static_assert(oKeepT * 8 + (2 * kBdMaxK + 1) * 4 <= oGu * 8, "new-block mask and skip flag fit before oGu");
+// |W - I|, |F| bound under which the second orthogonalisation round's
+// combination of FP32 columns is left out: it would move an element by about
+// (m + nt) * 1e-9 of the column scale, well below FP32 rounding (6e-8)
+constexpr double kSkipTol = 1e-9;
 }  // namespace
 
 struct BdEngine::Inst {
@@ -72,6 +77,8 @@ struct BdEngine::Inst {
     void *corr(int c) const { return C.as<unsigned char>() + es * g.N * (size_t)(k + c); }
     int *keep() const { return reinterpret_cast<int *>(small.as<double>() + oKeep); }
     int *keepT() const { return reinterpret_cast<int *>(small.as<double>() + oKeepT); }
+    // flag of the second orthogonalisation round (after the new block's mask)
+    int *skip2() const { return keepT() + 2 * kBdMaxK; }
 };
 
 // ------------------------------------------------------------------ arena
@@ -910,11 +917,15 @@ int BdEngine::step_b()
                 hld = m;
                 gld = nt;
             }
+            // FP32 columns: the second round's combination is left out where it
+            // would change T by far less than the columns' rounding
+            int *skip = (pass == 1 && t32) ? in->skip2() : nullptr;
             hot[pass][i] = OrthTask{h, G, m, nt, W, F, pass ? in->keepT() : nullptr, in->keepT(),
                                     pass ? in->keep() + k_ + in->mc : nullptr,
-                                    pass ? 1e-10 : 1e-13, hld, gld};
-            oc[pass].add(CombTask{T, N, nt, t32, W, nt, in->xc(), N, m, e32, F, m, T, N, t32, nt, 0,
-                                  N});
+                                    pass ? 1e-10 : 1e-13, hld, gld, skip, kSkipTol};
+            CombTask ct{T, N, nt, t32, W, nt, in->xc(), N, m, e32, F, m, T, N, t32, nt, 0, N};
+            ct.skip = skip;
+            oc[pass].add(ct);
         }
     }
     long long partB = 1;
