@@ -216,6 +216,13 @@ template <typename T> __host__ __device__ inline int fpitch(int np) { return siz
 // move HBM -> registers -> LDS without a layout change.  A workgroup walks the
 // planes p = blockIdx.x, blockIdx.x + gridDim.x, ... of one job; the loads of
 // the next plane are issued before the current one is contracted.
+// The middle pass of H*X (pass B) and of the preconditioner (pass 1) reads
+// what the pass before wrote: it walks its grid in reverse (workgroups are
+// dispatched in order of their linear index), starting on the data the
+// previous pass left last in the L2 and the memory-side cache.
+__device__ __forceinline__ int plane_x(bool rev) { return rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x; }
+__device__ __forceinline__ int plane_y(bool rev) { return rev ? gridDim.y - 1 - blockIdx.y : blockIdx.y; }
+
 template <int NPMAX> struct Own {
     static constexpr int NIT = NPMAX / 16;            // column tiles
     static constexpr int NCT = (NPMAX / 16 + 3) / 4;  // row tiles per wave
@@ -424,10 +431,10 @@ template <int PASS, int NPMAX, bool MULTI>
 __global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ jobs)
 {
     constexpr int NCT = Own<NPMAX>::NCT, NIT = Own<NPMAX>::NIT, KS = Own<NPMAX>::KS;
-    const HxJob &J = jobs[blockIdx.y];
+    const HxJob &J = jobs[plane_y(PASS == 1)];
     const AxisMats &Mi = J.ax[0];
     const AxisMats &Mc = J.ax[PASS == 0 ? 1 : 2];
-    int p = blockIdx.x;
+    int p = plane_x(PASS == 1);
     Geo g;
     if (!geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, p, g)) return;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -547,10 +554,10 @@ template <int PASS, int NPMAX, bool MULTI>
 __global__ void __launch_bounds__(kPT) hx_f64_kernel(const HxJob *__restrict__ jobs)
 {
     constexpr int NCT = Own<NPMAX>::NCT, NIT = Own<NPMAX>::NIT, KF = NPMAX / 4;
-    const HxJob &J = jobs[blockIdx.y];
+    const HxJob &J = jobs[plane_y(PASS == 1)];
     const AxisMats &Mi = J.ax[0];
     const AxisMats &Mc = J.ax[PASS == 0 ? 1 : 2];
-    int p = blockIdx.x;
+    int p = plane_x(PASS == 1);
     Geo g;
     if (!geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, p, g)) return;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1024,11 +1031,11 @@ template <int PASS, int NPMAX, bool MULTI>
 __global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict__ tasks)
 {
     constexpr int NCT = Own<NPMAX>::NCT, NIT = Own<NPMAX>::NIT, KS = Own<NPMAX>::KS;
-    const FdmTask &t = tasks[blockIdx.y];
+    const FdmTask &t = tasks[plane_y(PASS == 1)];
     const int kind = PASS == 1 ? 1 : 0;
     const AxisMats &Mx = t.ax[0], &My = t.ax[1], &Mz = t.ax[2];
     const AxisMats &Mc = PASS == 1 ? Mz : My;
-    int p = blockIdx.x;
+    int p = plane_x(PASS == 1);
     Geo g;
     if (!geo_of(kind, Mx.n, My.n, Mz.n, 1, 0, Mx.np, Mc.np, p, g)) return;
     extern __shared__ __align__(16) unsigned char smem[];
