@@ -597,9 +597,12 @@ __global__ void __launch_bounds__(kT) comb_kernel(const CombTask *__restrict__ t
 // ------------------------------------------------------------------ eigensolver
 // Parallel Jacobi (round-robin pairing, m/2 disjoint rotations per round) on
 // the symmetrised projected matrix in LDS; one workgroup per task.
+// 0 <= i < mp, 0 <= r < mp - 1: (i - 1 + r) mod (mp - 1) by one subtraction
 __device__ inline int rr_index(int i, int r, int mp)
 {
-    return (i == 0) ? 0 : ((i - 1 + r) % (mp - 1)) + 1;
+    if (i == 0) return 0;
+    const int x = i - 1 + r;
+    return (x >= mp - 1 ? x - (mp - 1) : x) + 1;
 }
 
 constexpr int kEigT = 256;
@@ -665,6 +668,22 @@ __global__ void __launch_bounds__(kEigT) eig_kernel(const EigTask *__restrict__ 
     const int mp = m + (m & 1);
     const int npair = mp / 2;
     const int lane = tid & 63, wave = tid >> 6;
+    // this thread's 2 x 2 blocks (b = tid + kEigT q) and V items, fixed for
+    // every round: their pair indices are divided out once here
+    constexpr int kNB = (kBdMaxM / 2) * (kBdMaxM / 2) / kEigT, kNV = kBdMaxM * (kBdMaxM / 2) / kEigT;
+    int bP1[kNB], bP2[kNB], vK[kNV], vP[kNV];
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) {
+        const int b = tid + kEigT * q;
+        bP1[q] = b % npair;
+        bP2[q] = b / npair;
+    }
+#pragma unroll
+    for (int q = 0; q < kNV; ++q) {
+        const int item = tid + kEigT * q;
+        vK[q] = item % m;
+        vP[q] = item / m;
+    }
     for (int sweep = 0; sweep < kEigSweeps; ++sweep) {
         double off = 0.0, tot = 0.0;
         for (int idx = tid; idx < m * m; idx += kEigT) {
@@ -723,8 +742,10 @@ __global__ void __launch_bounds__(kEigT) eig_kernel(const EigTask *__restrict__ 
                 sn[tid] = s;
             }
             __syncthreads();
-            for (int b = tid; b < npair * npair; b += kEigT) {
-                const int P1 = b % npair, P2 = b / npair;
+#pragma unroll
+            for (int qb = 0; qb < kNB; ++qb) {
+                if (tid + kEigT * qb >= npair * npair) break;
+                const int P1 = bP1[qb], P2 = bP2[qb];
                 const double s1 = sn[P1], s2 = sn[P2];
                 if (s1 == 0.0 && s2 == 0.0) continue;
                 const double c1 = cs[P1], c2 = cs[P2];
@@ -741,8 +762,10 @@ __global__ void __launch_bounds__(kEigT) eig_kernel(const EigTask *__restrict__ 
                 if (v1) A[q1 + LD * p2] = s1 * x00 + c1 * x10;
                 if (v1 && v2) A[q1 + LD * q2] = s1 * x01 + c1 * x11;
             }
-            for (int item = tid; item < m * npair; item += kEigT) {
-                const int k = item % m, pr = item / m;
+#pragma unroll
+            for (int qv = 0; qv < kNV; ++qv) {
+                if (tid + kEigT * qv >= m * npair) break;
+                const int k = vK[qv], pr = vP[qv];
                 const double s = sn[pr];
                 if (s == 0.0) continue;
                 const int p = pp[pr], q = qq[pr];
