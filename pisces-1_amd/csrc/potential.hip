@@ -177,6 +177,7 @@ potential_grid_batched_kernel(PotParams P, const PotTask *__restrict__ tasks)
 // T_ij (src/DPP.cpp:2377-2410) do not change during the iteration and are
 // computed once into d_tensor (6 doubles per ordered pair).
 constexpr int kScfThreads = 256;
+constexpr int kScfSharedSites = kScfThreads / 2;  // clusters solved with the dipoles in LDS
 constexpr double kA2B = 1.0 / 0.52917720859;
 
 __device__ inline void site_params(int s, double &q, double &sig, double &alpha)
@@ -263,8 +264,74 @@ __device__ void water_scf_body(int nw, const double *__restrict__ sites, double 
     __syncthreads();
     const int maxit = 5 * (ns + 10);
     int it = 0, conv = 0;
-    // mu_out holds the current dipoles; a private next value per owned site is
-    // kept in registers (at most ceil(ns/256) sites per thread)
+    if (ns <= kScfSharedSites) {
+        // ns <= 128: tps = floor(256 / ns) >= 2 threads per site, each summing
+        // the partners j = part, part + tps, ...; the dipoles live in LDS and
+        // the tps partial fields of a site are added there
+        __shared__ double smu[3 * kScfSharedSites];
+        __shared__ double pe[kScfThreads][3];
+        const int tps = kScfThreads / ns;
+        const int i = threadIdx.x / tps, part = threadIdx.x - i * tps;
+        const bool own = i < ns;
+        for (int idx = threadIdx.x; idx < 3 * ns; idx += blockDim.x) smu[idx] = mu_out[idx];
+        double q, sg, ai = 0.0;
+        if (own) site_params(i % 3, q, sg, ai);
+        __syncthreads();
+        for (it = 0; it < maxit; ++it) {
+            double e0 = 0, e1 = 0, e2 = 0;
+            if (own) {
+                const double *t = tens + 6LL * i * ns;
+                for (int j = part; j < ns; j += tps) {
+                    if (j == i) continue;
+                    const double m0 = smu[3 * j], m1 = smu[3 * j + 1], m2 = smu[3 * j + 2];
+                    const double *tj = t + 6 * j;
+                    e0 += tj[0] * m0 + tj[3] * m1 + tj[4] * m2;
+                    e1 += tj[3] * m0 + tj[1] * m1 + tj[5] * m2;
+                    e2 += tj[4] * m0 + tj[5] * m1 + tj[2] * m2;
+                }
+            }
+            pe[threadIdx.x][0] = e0;
+            pe[threadIdx.x][1] = e1;
+            pe[threadIdx.x][2] = e2;
+            __syncthreads();  // partial fields written, old dipoles read
+            double part_d = 0.0, n0 = 0, n1 = 0, n2 = 0;
+            if (own && part == 0) {
+                for (int p2 = 1; p2 < tps; ++p2) {
+                    e0 += pe[threadIdx.x + p2][0];
+                    e1 += pe[threadIdx.x + p2][1];
+                    e2 += pe[threadIdx.x + p2][2];
+                }
+                const double o0 = smu[3 * i], o1 = smu[3 * i + 1], o2 = smu[3 * i + 2];
+                n0 = 0.5 * o0 + 0.5 * ai * (epc_out[3 * i] + e0);
+                n1 = 0.5 * o1 + 0.5 * ai * (epc_out[3 * i + 1] + e1);
+                n2 = 0.5 * o2 + 0.5 * ai * (epc_out[3 * i + 2] + e2);
+                const double d0 = o0 - n0, d1 = o1 - n1, d2 = o2 - n2;
+                part_d = d0 * d0 + d1 * d1 + d2 * d2;
+            }
+            red[threadIdx.x] = part_d;
+            __syncthreads();  // every owner has read its old dipole
+            if (own && part == 0) {
+                smu[3 * i] = n0;
+                smu[3 * i + 1] = n1;
+                smu[3 * i + 2] = n2;
+            }
+            for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+                __syncthreads();
+                if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            }
+            if (threadIdx.x == 0) s_done = (sqrt(red[0]) / 3 / ns < 1e-12) ? 1 : 0;
+            __syncthreads();
+            if (s_done) { conv = 1; break; }
+        }
+        for (int idx = threadIdx.x; idx < 3 * ns; idx += blockDim.x) mu_out[idx] = smu[idx];
+        if (threadIdx.x == 0) {
+            info[0] = conv;
+            info[1] = conv ? it + 1 : it;
+        }
+        return;
+    }
+    // larger clusters: mu_out holds the current dipoles; a private next value
+    // per owned site is kept in registers (at most ceil(ns/256) sites per thread)
     constexpr int kPer = 4;
     for (it = 0; it < maxit; ++it) {
         double nxt[kPer][3];
