@@ -993,24 +993,42 @@ __global__ void __launch_bounds__(64) ritz_reduce_kernel(const RitzTask *__restr
 // ------------------------------------------------------------------ orthogonalisation factor
 // One 64-lane workgroup per task: S = G - h^T h (h may be absent), Cholesky
 // with dropping, W = L^{-T}, F = -h W.  Shared by the Cholesky-QR factor.
-__device__ void orth_factor(const double *__restrict__ hg, int hld, const double *__restrict__ G,
+__device__ void orth_factor(const double *__restrict__ hg, int hld, const double *__restrict__ Gin,
                             int gld, int m, int nt, double *__restrict__ Wo,
                             double *__restrict__ Fo, const int *kin, int *kout, int *kcopy,
-                            double tol, int *skip = nullptr, double skip_tol = 0.0)
+                            double tol, int *skip = nullptr, double skip_tol = 0.0,
+                            const double *part = nullptr, long long part_off = 0, int nch = 0)
 {
     constexpr int K = kBdMaxK;
     __shared__ double h[kBdMaxM * kBdMaxK];
-    __shared__ double S[K * K], L[K * K], W[K * K];
+    __shared__ double S[K * K], L[K * K], W[K * K], Gs[K * K];
     __shared__ int keep[K];
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < m * nt; idx += 64) {
-        const int a = idx % m, c = idx / m;
-        h[idx] = hg[a + (long long)hld * c];
+    if (part) {
+        // [V | T]^T T from its chunk partials (fixed order per entry)
+        const int ma = m + nt;
+        for (int idx = tid; idx < ma * nt; idx += 64) {
+            const int a = idx % ma, c = idx / ma;
+            const double *p = part + part_off + (long long)(a + ma * c) * nch;
+            double sum = 0.0;
+            for (int q = 0; q < nch; ++q) sum += p[q];
+            if (a < m) h[a + m * c] = sum;
+            else Gs[(a - m) + K * c] = sum;
+        }
+    } else {
+        for (int idx = tid; idx < m * nt; idx += 64) {
+            const int a = idx % m, c = idx / m;
+            h[idx] = hg[a + (long long)hld * c];
+        }
+        for (int idx = tid; idx < nt * nt; idx += 64) {
+            const int i = idx % nt, j = idx / nt;
+            Gs[i + K * j] = Gin[i + (long long)gld * j];
+        }
     }
     __syncthreads();
     for (int idx = tid; idx < nt * nt; idx += 64) {
         const int i = idx % nt, j = idx / nt;
-        double v = G[i + (long long)gld * j];
+        double v = Gs[i + K * j];
         for (int a = 0; a < m; ++a) v -= h[a + m * i] * h[a + m * j];
         S[i + K * j] = v;
         L[i + K * j] = 0.0;
@@ -1020,7 +1038,7 @@ __device__ void orth_factor(const double *__restrict__ hg, int hld, const double
         if (tid == 0) {
             const double d = S[j + K * j];
             const int ki = kin ? kin[j] : 1;
-            const int kj = (ki && d > tol * G[j + (long long)gld * j] && d > 0.0) ? 1 : 0;
+            const int kj = (ki && d > tol * Gs[j + K * j] && d > 0.0) ? 1 : 0;
             keep[j] = kj;
             L[j + K * j] = kj ? sqrt(d) : 0.0;
         }
@@ -1087,7 +1105,7 @@ __global__ void __launch_bounds__(64) orth_kernel(const OrthTask *__restrict__ t
 {
     const OrthTask &t = tasks[blockIdx.x];
     orth_factor(t.h, t.hld, t.G, t.gld, t.m, t.nt, t.W, t.F, t.keep_in, t.keep_out, t.keep_copy,
-                t.tol, t.skip, t.skip_tol);
+                t.tol, t.skip, t.skip_tol, t.part, t.part_off, t.nch);
 }
 
 __global__ void __launch_bounds__(64) cholinv_batched_kernel(const CholTask *__restrict__ tasks)
@@ -1203,7 +1221,7 @@ int gram_blocks_per_chunk(int ma, int mb)
 }
 
 void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_blocks_narrow,
-             int max_mamb, double *d_partial, hipStream_t s)
+             int max_mamb, double *d_partial, hipStream_t s, bool reduce)
 {
     if (ntasks <= 0) return;
     const int v = g_gram.variant, rows = g_gram.rows;
@@ -1217,8 +1235,9 @@ void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_blocks
         hipLaunchKernelGGL(gram_kernel<0>, dim3(max_blocks, ntasks), dim3(kT), 0, s, d_tasks, d_partial, rows);
     else
         hipLaunchKernelGGL(gram_kernel<1>, dim3(max_blocks, ntasks), dim3(kT), 0, s, d_tasks, d_partial, rows);
-    hipLaunchKernelGGL(gram_reduce_kernel, dim3((max_mamb + kT - 1) / kT, ntasks), dim3(kT), 0, s,
-                       d_tasks, d_partial);
+    if (reduce)
+        hipLaunchKernelGGL(gram_reduce_kernel, dim3((max_mamb + kT - 1) / kT, ntasks), dim3(kT), 0,
+                           s, d_tasks, d_partial);
     PC_HIP(hipGetLastError());
 }
 

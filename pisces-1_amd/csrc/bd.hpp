@@ -73,6 +73,10 @@ struct OrthTask {
     double tol;
     int hld, gld;
     int *skip; double skip_tol;
+    // optional: h and G are the row-chunk partials of the Gram [V | T]^T T
+    // (GramTask layout, (m + nt) x nt entries) and are summed here; the Gram
+    // launch then needs no reduction of its own
+    const double *part; long long part_off; int nch; int pad;
 };
 
 // Symmetric eigenproblem of the projected matrix: G (m x m, upper triangle,
@@ -125,9 +129,10 @@ struct CoefTask {
     const double *Y; int m; int k; double *Q; int *keep_out;
 };
 
-// max_blocks / max_blocks_narrow: max over the tasks of kind 0 / 1 of nch * cpb
+// max_blocks / max_blocks_narrow: max over the tasks of kind 0 / 1 of nch * cpb;
+// reduce = false leaves the partials to the consumer (OrthTask::part)
 void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_blocks_narrow,
-             int max_mamb, double *d_partial, hipStream_t s);
+             int max_mamb, double *d_partial, hipStream_t s, bool reduce = true);
 // max_mb: max over the tasks of mb
 void bd_combine(const CombTask *d_tasks, int ntasks, long long max_N, int max_mb, hipStream_t s);
 void bd_eig(const EigTask *d_tasks, int ntasks, hipStream_t s);

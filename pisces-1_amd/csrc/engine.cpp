@@ -281,6 +281,7 @@ struct GramBatch {
     GramTask *h = nullptr, *d = nullptr;
     int n = 0, max_blocks = 0, max_blocks_narrow = 0, max_mamb = 0;
     long long part = 0;
+    bool reduce = true;  // false: every task's consumer sums the partials itself
     // tasks are sized (row chunks, workgroups, partial offsets) in finish(),
     // for the Gram configuration in use
     void add(const void *A, int ma, int a32, const void *B, int mb, int b32, long long N,
@@ -330,7 +331,7 @@ struct GramBatch {
     {
         if (!n) return;
         ProfScope ps(PK_GRAM, s, prof_enabled() ? bytes() : 0.0);
-        bd_gram(d, n, max_blocks, max_blocks_narrow, max_mamb, dp, s);
+        bd_gram(d, n, max_blocks, max_blocks_narrow, max_mamb, dp, s, reduce);
     }
 };
 
@@ -890,7 +891,12 @@ int BdEngine::step_b()
     GramBatch og[2];
     CombBatch oc[2];
     OrthTask *hot[2], *dot[2];
+    // the orth kernel sums the partials of a one-task Gram [Xc|C|T]^T T itself;
+    // a pass reduces on its own only where an instance needs two Gram tasks
+    std::vector<int> og_idx[2];
+    bool og_red[2] = {false, false};
     for (int pass = 0; pass < 2; ++pass) {
+        og_idx[pass].assign(nc, -1);
         og[pass].h = B.take<GramTask>(2 * nc, &og[pass].d);
         oc[pass].h = B.take<CombTask>(nc, &oc[pass].d);
         hot[pass] = B.take<OrthTask>(nc, &dot[pass]);
@@ -904,12 +910,14 @@ int BdEngine::step_b()
             int hld, gld;
             if (m + nt <= kBdMaxM) {
                 double *hg = in->sm() + oHG;
+                og_idx[pass][i] = og[pass].n;
                 og[pass].add(in->xc(), m + nt, e32, T, nt, e32, N, hg, m + nt);
                 h = hg;
                 G = hg + m;
                 hld = gld = m + nt;
             } else {
                 double *hv = in->sm() + oH, *gt = in->sm() + oGt;
+                og_red[pass] = true;
                 og[pass].add(in->xc(), m, e32, T, nt, e32, N, hv, m);
                 og[pass].add(T, nt, e32, T, nt, e32, N, gt, nt);
                 h = hv;
@@ -935,6 +943,16 @@ int BdEngine::step_b()
     }
     grow_part(sizeof(double) * (size_t)partB);
     double *dp = dpart_.as<double>();
+    for (int pass = 0; pass < 2; ++pass) {
+        og[pass].reduce = og_red[pass];
+        for (int i = 0; i < nc; ++i) {
+            const int q = og_idx[pass][i];
+            if (q < 0) continue;
+            hot[pass][i].part = dp;
+            hot[pass][i].part_off = og[pass].h[q].part_off;
+            hot[pass][i].nch = og[pass].h[q].nch;
+        }
+    }
     B.flush(s_);
     {
         // R columns read once, corrections written once (+ diag H for the diagonal form)
