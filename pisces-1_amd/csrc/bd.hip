@@ -245,6 +245,7 @@ __global__ void __launch_bounds__(kT) gram_valu_kernel(const GramTask *__restric
 {
     const GramTask t = tasks[blockIdx.y];
     if (t.kind != 0) return;
+    if (t.skip && *cst(t.skip)) return;
     const int ntile = t.cpb;  // tiles of this task
     if ((int)blockIdx.x >= t.nch * ntile) return;
     const int chunk = blockIdx.x / ntile, tile = blockIdx.x - chunk * ntile;
@@ -290,6 +291,7 @@ __global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ t
 {
     const GramTask &t = tasks[blockIdx.y];
     if (t.kind != 0 || (int)blockIdx.x >= t.nch) return;
+    if (t.skip && *cst(t.skip)) return;
     const long long r_beg = (long long)blockIdx.x * rows;
     const long long r_end = min(r_beg + rows, t.N);
     // 16-byte loads need every column start aligned: uniform per task
@@ -418,6 +420,7 @@ __global__ void __launch_bounds__(kT) gram_narrow_kernel(const GramTask *__restr
     static_assert(NV == 32, "reduce-scatter of 32 values");
     const GramTask t = tasks[blockIdx.y];
     if (t.kind != 1) return;
+    if (t.skip && *cst(t.skip)) return;
     const int ntile = t.cpb;
     if ((int)blockIdx.x >= t.nch * ntile) return;
     // tile-major numbering: workgroups are dealt to the 8 XCDs round-robin, so
@@ -481,6 +484,7 @@ __global__ void __launch_bounds__(kT) gram_reduce_kernel(const GramTask *__restr
                                                          const double *__restrict__ partial)
 {
     const GramTask t = tasks[blockIdx.y];
+    if (t.skip && *cst(t.skip)) return;
     const int o = blockIdx.x * kT + threadIdx.x;
     if (o >= t.ma * t.mb) return;
     const double *p = partial + t.part_off + (long long)o * t.nch;
@@ -1101,9 +1105,229 @@ __device__ void orth_factor(const double *__restrict__ hg, int hld, const double
     }
 }
 
+// Cholesky with dropping of the nt x nt matrix A (LDS, ld kBdMaxK; its lower
+// triangle is overwritten), then W = L^{-T} (zero columns for dropped ones).
+// Column j is kept if kin[j] (null: all) and its pivot exceeds tol * dref[j].
+// All 64 threads take part.
+__device__ void chol_winv(double *A, const double *dref, int nt, const int *kin, double tol,
+                          double *L, double *W, int *keep)
+{
+    constexpr int K = kBdMaxK;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < nt * nt; idx += 64) L[(idx % nt) + K * (idx / nt)] = 0.0;
+    __syncthreads();
+    for (int j = 0; j < nt; ++j) {
+        if (tid == 0) {
+            const double d = A[j + K * j];
+            const int ki = kin ? kin[j] : 1;
+            const int kj = (ki && d > tol * dref[j] && d > 0.0) ? 1 : 0;
+            keep[j] = kj;
+            L[j + K * j] = kj ? sqrt(d) : 0.0;
+        }
+        __syncthreads();
+        if (keep[j]) {
+            const double ljj = L[j + K * j];
+            for (int i = j + 1 + tid; i < nt; i += 64) L[i + K * j] = A[i + K * j] / ljj;
+            __syncthreads();
+            for (int idx = tid; idx < nt * nt; idx += 64) {
+                const int i = idx % nt, l = idx / nt;
+                if (l > j && i >= l) A[i + K * l] -= L[i + K * j] * L[l + K * j];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < nt) {
+        const int c = tid;
+        for (int i = nt - 1; i >= 0; --i) {
+            if (!keep[i] || !keep[c]) {
+                W[i + K * c] = 0.0;
+                continue;
+            }
+            double v = (i == c) ? 1.0 : 0.0;
+            for (int p = i + 1; p < nt; ++p) v -= L[p + K * i] * W[p + K * c];
+            W[i + K * c] = v / L[i + K * i];
+        }
+    }
+    __syncthreads();
+}
+
+// S is written one column at a time (rows 0 .. of the basis it then had), so
+// only its upper triangle is current: S(a, b) = S[min + lds * max]
+__device__ __forceinline__ double sym_at(cptr<double> S, int lds, int a, int b)
+{
+    return a <= b ? S[a + (long long)lds * b] : S[b + (long long)lds * a];
+}
+
+__global__ void __launch_bounds__(64) orth2_kernel(const Orth2Task *__restrict__ tasks)
+{
+    constexpr int K = kBdMaxK, M = kBdMaxM;
+    const Orth2Task &t = tasks[blockIdx.x];
+    const int m = t.m, nt = t.nt, ma = m + nt, tid = threadIdx.x;
+    __shared__ double h0[M * K], F1[M * K], h1[M * K];
+    __shared__ double G0[K * K], A[K * K], L[K * K], W1[K * K], W2[K * K], G1[K * K], dref[K];
+    __shared__ int keep1[K], keep2[K];
+    // h0, G0 from the chunk partials (fixed order per entry)
+    for (int idx = tid; idx < ma * nt; idx += 64) {
+        const int a = idx % ma, c = idx / ma;
+        const double *p = t.part + t.part_off + (long long)(a + ma * c) * t.nch;
+        double sum = 0.0;
+        for (int q = 0; q < t.nch; ++q) sum += p[q];
+        if (a < m) h0[a + M * c] = sum;
+        else G0[(a - m) + K * c] = sum;
+    }
+    __syncthreads();
+    // round 1
+    for (int idx = tid; idx < nt * nt; idx += 64) {
+        const int i = idx % nt, j = idx / nt;
+        double v = G0[i + K * j];
+        for (int a = 0; a < m; ++a) v -= h0[a + M * i] * h0[a + M * j];
+        A[i + K * j] = v;
+    }
+    if (tid < nt) dref[tid] = G0[tid + K * tid];
+    __syncthreads();
+    // amplification of round 1: |T_j|^2 / |T_j - V h0_j|^2 over the kept columns
+    __shared__ int s_alg;
+    if (tid == 0) {
+        int ok = 1;
+        for (int j = 0; j < nt; ++j) {
+            const double d = A[j + K * j];
+            if (d > t.tol1 * dref[j] && d > 0.0 && dref[j] > t.amp2_max * d) ok = 0;
+        }
+        s_alg = ok;
+    }
+    chol_winv(A, dref, nt, nullptr, t.tol1, L, W1, keep1);
+    const bool alg = s_alg != 0;  // written before chol_winv's barriers
+    // F1 = -h0 W1; P = h0 W1 (kept in h1 for now)
+    for (int idx = tid; idx < m * nt; idx += 64) {
+        const int a = idx % m, c = idx / m;
+        double pv = 0.0;
+        for (int i = 0; i < nt; ++i) pv += h0[a + M * i] * W1[i + K * c];
+        h1[a + M * c] = pv;
+        F1[a + M * c] = -pv;
+    }
+    __syncthreads();
+    if (!alg) {
+        // round 1 alone; the second round runs on the data
+        for (int idx = tid; idx < nt * nt; idx += 64) {
+            const int i = idx % nt, c = idx / nt;
+            t.W[i + (long long)nt * c] = W1[i + K * c];
+        }
+        for (int idx = tid; idx < m * nt; idx += 64) {
+            const int a = idx % m, c = idx / m;
+            t.F[a + (long long)m * c] = F1[a + M * c];
+        }
+        if (tid < nt) t.keep_out[tid] = keep1[tid];
+        if (tid == 0) {
+            *t.alg = 0;
+            *t.skip2 = 0;
+        }
+        return;
+    }
+    // Q = S F1, then h1 = P + Q and
+    // G1 = W1^T G0 W1 + P^T F1 + F1^T P + F1^T Q (P = h0 W1, held in h1)
+    const cptr<double> Sv = cst(t.Sv);
+    __shared__ double Q[M * K];
+    for (int idx = tid; idx < m * nt; idx += 64) {
+        const int a = idx % m, c = idx / m;
+        double v = 0.0;
+        for (int b = 0; b < m; ++b) v += sym_at(Sv, t.lds, a, b) * F1[b + M * c];
+        Q[a + M * c] = v;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nt * nt; idx += 64) {
+        const int i = idx % nt, j = idx / nt;
+        if (i < j) continue;
+        double v = 0.0;
+        for (int p = 0; p < nt; ++p) {
+            double gw = 0.0;
+            for (int q = 0; q < nt; ++q) gw += G0[p + K * q] * W1[q + K * j];
+            v += W1[p + K * i] * gw;
+        }
+        for (int a = 0; a < m; ++a)
+            v += h1[a + M * i] * F1[a + M * j] + F1[a + M * i] * h1[a + M * j] + F1[a + M * i] * Q[a + M * j];
+        G1[i + K * j] = v;
+        G1[j + K * i] = v;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < m * nt; idx += 64) {
+        const int a = idx % m, c = idx / m;
+        h1[a + M * c] += Q[a + M * c];
+    }
+    __syncthreads();
+    // round 2
+    for (int idx = tid; idx < nt * nt; idx += 64) {
+        const int i = idx % nt, j = idx / nt;
+        double v = G1[i + K * j];
+        for (int a = 0; a < m; ++a) v -= h1[a + M * i] * h1[a + M * j];
+        A[i + K * j] = v;
+    }
+    if (tid < nt) dref[tid] = G1[tid + K * tid];
+    __syncthreads();
+    chol_winv(A, dref, nt, keep1, t.tol2, L, W2, keep2);
+    // W = W1 W2 (into W1's place: Wm), F = (F1 - h1) W2 (into F1's place)
+    __shared__ double Wm[K * K];
+    for (int idx = tid; idx < nt * nt; idx += 64) {
+        const int i = idx % nt, c = idx / nt;
+        double v = 0.0;
+        for (int p = 0; p < nt; ++p) v += W1[i + K * p] * W2[p + K * c];
+        Wm[i + K * c] = v;
+        t.W[i + (long long)nt * c] = v;
+    }
+    for (int idx = tid; idx < m * nt; idx += 64) {
+        const int a = idx % m, c = idx / m;
+        double v = 0.0;
+        for (int p = 0; p < nt; ++p) v += (F1[a + M * p] - h1[a + M * p]) * W2[p + K * c];
+        Q[a + M * c] = v;  // F
+        t.F[a + (long long)m * c] = v;
+    }
+    __syncthreads();
+    // overlaps of the new block T W + V F (basis columns m .. m + nt) with the
+    // basis and among themselves: P' = h0 W, Q' = S F,
+    // V^T T' = P' + Q',  T'^T T' = W^T G0 W + P'^T F + F^T P' + F^T Q'
+    // (the rounding of the stored block, ~1e-10 here, is not captured; every
+    // restart rebuilds S from the data)
+    for (int idx = tid; idx < m * nt; idx += 64) {
+        const int a = idx % m, c = idx / m;
+        double pv = 0.0, qv = 0.0;
+        for (int p = 0; p < nt; ++p) pv += h0[a + M * p] * Wm[p + K * c];
+        for (int b = 0; b < m; ++b) qv += sym_at(Sv, t.lds, a, b) * Q[b + M * c];
+        F1[a + M * c] = pv;  // P'
+        h1[a + M * c] = qv;  // Q'
+    }
+    __syncthreads();
+    double *So = t.Sout;
+    for (int idx = tid; idx < m * nt; idx += 64) {
+        const int a = idx % m, c = idx / m;
+        So[a + (long long)t.lds * (m + c)] = F1[a + M * c] + h1[a + M * c];
+    }
+    for (int idx = tid; idx < nt * nt; idx += 64) {
+        const int i = idx % nt, j = idx / nt;
+        if (i > j) continue;
+        double v = 0.0;
+        for (int p = 0; p < nt; ++p) {
+            double gw = 0.0;
+            for (int q = 0; q < nt; ++q) gw += G0[p + K * q] * Wm[q + K * j];
+            v += Wm[p + K * i] * gw;
+        }
+        for (int a = 0; a < m; ++a)
+            v += F1[a + M * i] * Q[a + M * j] + Q[a + M * i] * F1[a + M * j] + Q[a + M * i] * h1[a + M * j];
+        So[(m + i) + (long long)t.lds * (m + j)] = v;
+    }
+    if (tid < nt) {
+        t.keep_out[tid] = keep2[tid];
+        if (t.keep_copy) t.keep_copy[tid] = keep2[tid];
+    }
+    if (tid == 0) {
+        *t.alg = 1;
+        *t.skip2 = 1;
+    }
+}
+
 __global__ void __launch_bounds__(64) orth_kernel(const OrthTask *__restrict__ tasks)
 {
     const OrthTask &t = tasks[blockIdx.x];
+    if (t.noop && *cst(t.noop)) return;
+    if (t.clear && threadIdx.x == 0) *t.clear = 0;
     orth_factor(t.h, t.hld, t.G, t.gld, t.m, t.nt, t.W, t.F, t.keep_in, t.keep_out, t.keep_copy,
                 t.tol, t.skip, t.skip_tol, t.part, t.part_off, t.nch);
 }
@@ -1328,6 +1552,13 @@ void bd_ritz(const RitzTask *d_tasks, int ntasks, int max_nch, int k, double *d_
     else if (k <= 8) hipLaunchKernelGGL((ritz_kernel<8, 1>), grid, dim3(kT), 0, s, d_tasks, d_partial, rev);
     else hipLaunchKernelGGL((ritz_kernel<16, 1>), grid, dim3(kT), 0, s, d_tasks, d_partial, rev);
     hipLaunchKernelGGL(ritz_reduce_kernel, dim3(ntasks), dim3(64), 0, s, d_tasks, d_partial);
+    PC_HIP(hipGetLastError());
+}
+
+void bd_orth2(const Orth2Task *d_tasks, int ntasks, hipStream_t s)
+{
+    if (ntasks <= 0) return;
+    hipLaunchKernelGGL(orth2_kernel, dim3(ntasks), dim3(64), 0, s, d_tasks);
     PC_HIP(hipGetLastError());
 }
 

@@ -23,6 +23,7 @@ struct GramTask {
     int nch;             // row chunks (gram_chunks(N))
     int cpb;             // workgroups per chunk (gram_blocks_per_chunk(ma, mb))
     int kind;            // 0: the configured variant; 1: narrow-B VALU kernel (gram_narrow)
+    const int *skip;     // optional: the task does nothing when *skip != 0 (set by an earlier launch)
 };
 
 // Y (set|-=|+=) A1 W1 + A2 W2 + A3 W3, W in device memory (column-major);
@@ -73,11 +74,44 @@ struct OrthTask {
     double tol;
     int hld, gld;
     int *skip; double skip_tol;
+    const int *noop;  // optional: the task does nothing when *noop != 0 (set by an earlier launch)
+    int *clear;       // optional: set to 0 (a flag of orth2 that this task's path replaces)
     // optional: h and G are the row-chunk partials of the Gram [V | T]^T T
     // (GramTask layout, (m + nt) x nt entries) and are summed here; the Gram
     // launch then needs no reduction of its own
     const double *part; long long part_off; int nch; int pad;
 };
+
+// Both orthogonalisation rounds of a new block T (nt columns) against the
+// stored basis V (m columns) from one Gram: h0 = V^T T and G0 = T^T T come as
+// the row-chunk partials of [V | T]^T T (GramTask layout), S = V^T V (the
+// exact overlaps of the stored columns, leading dimension lds; only its upper
+// triangle is read) gives the
+// second round without a pass over the data:
+//   round 1: G0 - h0^T h0 = L1 L1^T (dropping, tol1), W1 = L1^{-T}, F1 = -h0 W1;
+//   T1 = T W1 + V F1 has V^T T1 = h1 = h0 W1 + S F1 and
+//   T1^T T1 = G1 = W1^T G0 W1 + P^T F1 + F1^T P + F1^T S F1 (P = h0 W1);
+//   round 2: G1 - h1^T h1 = L2 L2^T (dropping, tol2, after round 1's), W2 = L2^{-T};
+// out: W = W1 W2 (nt x nt, ld nt) and F = (F1 - h1) W2 (m x nt, ld m), so that
+// one combination T W + V F gives the block of both rounds.
+// Where a kept column's |T|^2 / |T - V h0|^2 exceeds amp2_max (round 1 then
+// cancels digits that only a second pass over the data restores), the task
+// writes round 1 alone (W1, F1, its mask) and *alg = 0; otherwise the merged
+// factors, the final mask and *alg = 1, *skip2 = 1 (the caller's second-round
+// launches read them and do nothing).
+// With *alg = 1 the task also writes S's columns m .. m + nt - 1 (rows
+// 0 .. m + nt - 1, upper triangle) for the new block from the same algebra, so
+// the caller need not form them from the data.
+struct Orth2Task {
+    const double *part; long long part_off; int nch; int m;
+    const double *Sv; int lds; int nt;
+    double *W; double *F;
+    int *keep_out; int *keep_copy;
+    double tol1, tol2, amp2_max;
+    int *alg; int *skip2;
+    double *Sout;  // S itself (its new columns)
+};
+void bd_orth2(const Orth2Task *d_tasks, int ntasks, hipStream_t s);
 
 // Symmetric eigenproblem of the projected matrix: G (m x m, upper triangle,
 // leading dimension ldg) -> lowest k eigenvalues theta and eigenvectors Y (m x k,

@@ -43,12 +43,16 @@ constexpr int oI = 8576;     // identity (k x k)
 constexpr int oF = 8832;     // orthogonalisation coefficients on [X|C] ((k + mc) x nt)
 constexpr int oHG = 9856;    // [Xc|C|T]^T T of an orthogonalisation round ((k + mc + nt) x nt)
 constexpr int oVf = oHG + (kBdMaxM + kBdMaxK) * kBdMaxK;  // eigenvectors of the last solve (m x m, ld kBdMaxM)
-constexpr int kSmall = oVf + kBdMaxM * kBdMaxM;
-static_assert(oKeepT * 8 + (2 * kBdMaxK + 1) * 4 <= oGu * 8, "new-block mask and skip flag fit before oGu");
+constexpr int oS = oVf + kBdMaxM * kBdMaxM;  // S = V^T V of the stored basis [Xc | C] (ld kBdMaxM)
+constexpr int kSmall = oS + kBdMaxM * kBdMaxM;
+static_assert(oKeepT * 8 + (2 * kBdMaxK + 2) * 4 <= oGu * 8, "new-block mask and flags fit before oGu");
 // |W - I|, |F| bound under which the second orthogonalisation round's
 // combination of FP32 columns is left out: it would move an element by about
 // (m + nt) * 1e-9 of the column scale, well below FP32 rounding (6e-8)
 constexpr double kSkipTol = 1e-9;
+// |T|^2 / |T - V h0|^2 above which the first round's cancellation needs the
+// second round on the data (both rounds from S keep about 1e-16 times it)
+constexpr double kAmp2Max = 1e6;
 }  // namespace
 
 struct BdEngine::Inst {
@@ -77,8 +81,11 @@ struct BdEngine::Inst {
     void *corr(int c) const { return C.as<unsigned char>() + es * g.N * (size_t)(k + c); }
     int *keep() const { return reinterpret_cast<int *>(small.as<double>() + oKeep); }
     int *keepT() const { return reinterpret_cast<int *>(small.as<double>() + oKeepT); }
-    // flag of the second orthogonalisation round (after the new block's mask)
+    // flags of the second orthogonalisation round (after the new block's mask):
+    // its combination is a no-op / both rounds came from one Gram and S
     int *skip2() const { return keepT() + 2 * kBdMaxK; }
+    int *alg() const { return keepT() + 2 * kBdMaxK + 1; }
+    double *Sv() const { return small.as<double>() + oS; }
 };
 
 // ------------------------------------------------------------------ arena
@@ -318,6 +325,7 @@ struct GramBatch {
         double b = 0;
         for (int i = 0; i < n; ++i) {
             const GramTask &t = h[i];
+            if (t.skip) continue;  // conservative: usually skipped on the device
             const size_t ea = t.a32 ? 4 : 8;
             b += (double)t.N * t.ma * ea;
             const unsigned char *a0 = static_cast<const unsigned char *>(t.A);
@@ -327,10 +335,14 @@ struct GramBatch {
         }
         return b;
     }
+    // nexp: tasks the device is expected to skip (second-round Grams after
+    // orth2); a launch of only those is timed with the small factor kernels
+    int nexp = 0;
+    bool all_skip() const { return nexp == n; }
     void launch(double *dp, hipStream_t s) const
     {
         if (!n) return;
-        ProfScope ps(PK_GRAM, s, prof_enabled() ? bytes() : 0.0);
+        ProfScope ps(all_skip() ? PK_CHOL : PK_GRAM, s, prof_enabled() ? bytes() : 0.0);
         bd_gram(d, n, max_blocks, max_blocks_narrow, max_mamb, dp, s, reduce);
     }
 };
@@ -350,16 +362,19 @@ struct CombBatch {
         double b = 0;
         for (int i = 0; i < n; ++i) {
             const CombTask &t = h[i];
+            if (t.skip) continue;  // conservative: usually skipped on the device
             b += (double)t.N * (t.ma1 * (t.a1_32 ? 4 : 8) + t.ma2 * (t.a2_32 ? 4 : 8) +
                                 t.ma3 * (t.a3_32 ? 4 : 8));
             b += (double)t.N * t.mb * (t.y32 ? 4 : 8) * (t.mode ? 2 : 1);
         }
         return b;
     }
+    int nexp = 0;  // as GramBatch::nexp (second-round combinations after orth2)
+    bool all_skip() const { return nexp == n; }
     void launch(hipStream_t s) const
     {
         if (!n) return;
-        ProfScope ps(PK_COMB, s, prof_enabled() ? bytes() : 0.0);
+        ProfScope ps(all_skip() ? PK_CHOL : PK_COMB, s, prof_enabled() ? bytes() : 0.0);
         bd_combine(d, n, maxN, maxmb, s);
     }
 };
@@ -423,10 +438,10 @@ void BdEngine::step_a()
     const int r32 = (lo && precond_ == 3) ? 1 : 0;
     const size_t tmax = std::max({sizeof(GramTask), sizeof(CombTask), sizeof(PrecTask),
                                   sizeof(HxJob), sizeof(EigTask), sizeof(RitzTask), sizeof(OrthTask),
-                                  sizeof(CholTask), sizeof(FdmTask)});
+                                  sizeof(CholTask), sizeof(FdmTask), sizeof(Orth2Task)});
     // at most ~44 descriptors per instance and phase, plus alignment slack
-    arena_[0]->reserve((tmax * 48 + 64 * 40) * (size_t)(n + 1), s_);
-    arena_[1]->reserve((tmax * 48 + 64 * 40) * (size_t)(n + 1), s_);
+    arena_[0]->reserve((tmax * 56 + 64 * 48) * (size_t)(n + 1), s_);
+    arena_[1]->reserve((tmax * 56 + 64 * 48) * (size_t)(n + 1), s_);
     if (dgather_.bytes < sizeof(double) * 2 * kBdMaxK * (size_t)n) {
         PC_HIP(hipStreamSynchronize(s_));
         dgather_.alloc(sizeof(double) * 2 * kBdMaxK * (size_t)std::max(n, 64));
@@ -574,7 +589,9 @@ void BdEngine::step_a()
     // new columns of G
     GramBatch gg;
     int ngg = 0;
-    for (Inst *in : act) ngg += 1 + 2 * ((in->gnew + kGramMaxB - 1) / kGramMaxB);
+    for (Inst *in : act)
+        ngg += 1 + 3 * ((in->gnew + kGramMaxB - 1) / kGramMaxB) +
+               (k_ + in->mc + kGramMaxB - 1) / kGramMaxB;
     gg.h = A.take<GramTask>(ngg, &gg.d);
     for (Inst *in : act) {
         const long long N = in->g.N;
@@ -588,6 +605,19 @@ void BdEngine::step_a()
             double *col = in->sm() + oG + (size_t)kBdMaxM * (k_ + c0);
             gg.add(in->HX.p, k_, 0, Cn, nb, e32, N, col, kBdMaxM);
             gg.add(in->corr(0), in->mc, e32, HCn, nb, e32, N, col + k_, kBdMaxM);
+        }
+        // low precision forms: S = V^T V of the stored basis V = [Xc | C], in
+        // full after a restart, else its new columns -- which the last step's
+        // orth2 already wrote where it ran (*alg = 1: the tasks do nothing)
+        if (lo) {
+            const int mv = k_ + in->mc;
+            const int b0 = in->fresh ? 0 : k_ + in->mc - in->gnew;
+            for (int c0 = b0; c0 < mv; c0 += kGramMaxB) {
+                const int nb = std::min(kGramMaxB, mv - c0);
+                const void *Vc = static_cast<const unsigned char *>(in->xc()) + e * N * (size_t)c0;
+                gg.add(in->xc(), mv, e32, Vc, nb, e32, N, in->Sv() + (size_t)kBdMaxM * c0, kBdMaxM);
+                if (!in->fresh) gg.h[gg.n - 1].skip = in->alg();
+            }
         }
     }
     EigTask *he, *de;
@@ -645,7 +675,7 @@ void BdEngine::step_a()
     lrs = A.take<ResTask>(nl, &dlrs);
     GramBatch lrg, lgx;
     lrg.h = A.take<GramTask>(nl, &lrg.d);
-    lgx.h = A.take<GramTask>(nl, &lgx.d);
+    lgx.h = A.take<GramTask>(nl * (1 + (2 * k_ + kGramMaxB - 1) / kGramMaxB), &lgx.d);
     CombBatch lpc, lrc;
     lpc.h = A.take<CombTask>(2 * nl, &lpc.d);
     lrc.h = A.take<CombTask>(2 * nl, &lrc.d);
@@ -684,6 +714,13 @@ void BdEngine::step_a()
             in->res.n_mv64 += k_;
         }
         lgx.add(in->X.p, k_, 0, in->HX.p, k_, 0, N, in->sm() + oG, kBdMaxM);
+        if (lo)  // S of the restarted basis [Xc | P]
+            for (int c0 = 0; c0 < 2 * k_; c0 += kGramMaxB) {
+                const int nb = std::min(kGramMaxB, 2 * k_ - c0);
+                const void *Vc = static_cast<const unsigned char *>(in->xc()) + e * N * (size_t)c0;
+                lgx.add(in->xc(), 2 * k_, e32, Vc, nb, e32, N, in->Sv() + (size_t)kBdMaxM * c0,
+                        kBdMaxM);
+            }
         const int nch = (int)((N + kBdResChunk - 1) / kBdResChunk);
         lrs[i] = ResTask{in->X.as<double>(), in->HX.as<double>(), in->R.p, N, k_, r32,
                          in->sm() + oG, N, lres_part, nch, kBdMaxM + 1, in->sm() + oRn,
@@ -895,6 +932,13 @@ int BdEngine::step_b()
     // a pass reduces on its own only where an instance needs two Gram tasks
     std::vector<int> og_idx[2];
     bool og_red[2] = {false, false};
+    // FP32 columns with a one-task Gram: both rounds from the first Gram and S
+    // (orth2), the second round's launches skip them on the device
+    Orth2Task *h2t = nullptr, *d2t = nullptr;
+    h2t = B.take<Orth2Task>(nc, &d2t);
+    int n2 = 0;
+    std::vector<int> o2_inst;
+    const int *one = dinit_.as<int>();  // a device 1 (the all-ones keep mask)
     for (int pass = 0; pass < 2; ++pass) {
         og_idx[pass].assign(nc, -1);
         og[pass].h = B.take<GramTask>(2 * nc, &og[pass].d);
@@ -934,6 +978,23 @@ int BdEngine::step_b()
             CombTask ct{T, N, nt, t32, W, nt, in->xc(), N, m, e32, F, m, T, N, t32, nt, 0, N};
             ct.skip = skip;
             oc[pass].add(ct);
+            if (t32 && m + nt <= kBdMaxM) {
+                if (pass == 0) {
+                    hot[0][i].noop = one;
+                    h2t[n2++] = Orth2Task{nullptr, 0, 0, m, in->Sv(), kBdMaxM, nt, W, F, in->keepT(),
+                                          in->keep() + k_ + in->mc, 1e-13, 1e-10, kAmp2Max, in->alg(),
+                                          in->skip2(), in->Sv()};
+                    o2_inst.push_back(i);
+                } else {
+                    og[1].h[og[1].n - 1].skip = in->alg();
+                    hot[1][i].noop = in->alg();
+                    ++og[1].nexp;
+                    ++oc[1].nexp;
+                }
+            } else if (lo && pass == 0) {
+                // the data path: next step's Gram forms S's new columns
+                hot[0][i].clear = in->alg();
+            }
         }
     }
     long long partB = 1;
@@ -953,6 +1014,12 @@ int BdEngine::step_b()
             hot[pass][i].nch = og[pass].h[q].nch;
         }
     }
+    for (int q = 0; q < n2; ++q) {
+        const GramTask &gt = og[0].h[og_idx[0][o2_inst[q]]];
+        h2t[q].part = dp;
+        h2t[q].part_off = gt.part_off;
+        h2t[q].nch = gt.nch;
+    }
     B.flush(s_);
     {
         // R columns read once, corrections written once (+ diag H for the diagonal form)
@@ -968,6 +1035,7 @@ int BdEngine::step_b()
         {
             ProfScope ps(PK_CHOL, s_, 0.0);
             bd_orth(dot[pass], nc, s_);
+            if (pass == 0) bd_orth2(d2t, n2, s_);
         }
         oc[pass].launch(s_);
     }
