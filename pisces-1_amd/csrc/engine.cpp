@@ -50,9 +50,7 @@ static_assert(oKeepT * 8 + (2 * kBdMaxK + 2) * 4 <= oGu * 8, "new-block mask and
 // combination of FP32 columns is left out: it would move an element by about
 // (m + nt) * 1e-9 of the column scale, well below FP32 rounding (6e-8)
 constexpr double kSkipTol = 1e-9;
-// |T|^2 / |T - V h0|^2 above which the first round's cancellation needs the
-// second round on the data (both rounds from S keep about 1e-16 times it)
-constexpr double kAmp2Max = 1e6;
+
 }  // namespace
 
 struct BdEngine::Inst {
@@ -153,6 +151,10 @@ BdEngine::BdEngine(int device, hipStream_t s, int precision, int block, int maxs
     // the FP64 refresh of H X while every wanted residual exceeds r (0: never skip)
     const char *rb = std::getenv("PISCES_REFRESH_BELOW");
     if (rb) refresh_below_ = std::atof(rb) > 0 ? std::atof(rb) : 1e300;
+    // PISCES_AMP2_MAX=<a>: threshold of the first round's squared amplification
+    // above which the second orthogonalisation round runs on the data (1: always)
+    const char *am = std::getenv("PISCES_AMP2_MAX");
+    if (am && std::atof(am) > 0) amp2_max_ = std::atof(am);
     gram_load_config();
 }
 
@@ -982,7 +984,7 @@ int BdEngine::step_b()
                 if (pass == 0) {
                     hot[0][i].noop = one;
                     h2t[n2++] = Orth2Task{nullptr, 0, 0, m, in->Sv(), kBdMaxM, nt, W, F, in->keepT(),
-                                          in->keep() + k_ + in->mc, 1e-13, 1e-10, kAmp2Max, in->alg(),
+                                          in->keep() + k_ + in->mc, 1e-13, 1e-10, amp2_max_, in->alg(),
                                           in->skip2(), in->Sv()};
                     o2_inst.push_back(i);
                 } else {

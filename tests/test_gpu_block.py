@@ -39,6 +39,27 @@ def test_block_solve_matches_reference(pisces, oracle_mod, tag, block, maxsub, p
     assert np.all(res < 1e-9 * 1.01), res
 
 
+@pytest.mark.parametrize("amp2_max", ["1", "1.5"])
+def test_block_solve_second_round_on_the_data(pisces, oracle_mod, amp2_max, monkeypatch):
+    """The FP32 form with the second orthogonalisation round forced onto the
+    data (amp2_max 1: every step) or mixed step by step with the round from S
+    (1.5: the first round's squared amplification is 1-4 here)."""
+    monkeypatch.setenv("PISCES_AMP2_MAX", amp2_max)
+    f, n, L, V = _golden_case(oracle_mod, "g12_r4")
+    nroots = int(f["params"][0])
+    with pisces.Context(n, L) as ctx:
+        ctx.set_potential(V)
+        r = ctx.solve_block(nroots, f["start"], block=4, maxsub=16, max_iter=3000, ptol=9,
+                            precision=pisces.FP32)
+        HU = ctx.apply_h(r["vecs"])
+    assert r["nconv"] == nroots, r
+    np.testing.assert_allclose(r["evals"], f["evals"], rtol=0, atol=1e-6)
+    for i in range(nroots):
+        assert abs(r["vecs"][i] @ f["vecs"][i]) >= 1 - 1e-8
+    res = np.linalg.norm(HU - r["evals"][:, None] * r["vecs"], axis=1)
+    assert np.all(res < 1e-9 * 1.01), res
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x2"])
 def test_block_solve_c2(pisces, precision):
     """Benchmark size: (H2O)24, 64^3, 4 states, block 8, maxsub 32, pTol 7."""
