@@ -151,10 +151,18 @@ BdEngine::BdEngine(int device, hipStream_t s, int precision, int block, int maxs
     // the FP64 refresh of H X while every wanted residual exceeds r (0: never skip)
     const char *rb = std::getenv("PISCES_REFRESH_BELOW");
     if (rb) refresh_below_ = std::atof(rb) > 0 ? std::atof(rb) : 1e300;
-    // PISCES_AMP2_MAX=<a>: threshold of the first round's squared amplification
-    // above which the second orthogonalisation round runs on the data (1: always)
+    // PISCES_AMP2_MAX=<a>: keep the second orthogonalisation round's launches on
+    // the data, taken where the first round's squared amplification exceeds a
+    // (1: always).  By default both rounds come from S whatever the
+    // amplification: its error is ~1e-16 times it (1.1-3.8 on the bench), and
+    // convergence is only ever declared on a re-orthonormalised block with an
+    // FP64-fresh H X, so a less orthogonal block could cost iterations, not
+    // accuracy.
     const char *am = std::getenv("PISCES_AMP2_MAX");
-    if (am && std::atof(am) > 0) amp2_max_ = std::atof(am);
+    if (am && std::atof(am) > 0) {
+        data_round2_ = true;
+        amp2_max_ = std::atof(am);
+    }
     gram_load_config();
 }
 
@@ -948,6 +956,12 @@ int BdEngine::step_b()
         hot[pass] = B.take<OrthTask>(nc, &dot[pass]);
         for (int i = 0; i < nc; ++i) {
             Inst *in = corr[i];
+            if (pass == 1 && !data_round2_ && t32 && k_ + in->mc + in->nt <= kBdMaxM) {
+                // both rounds from S: nothing of the second round is launched
+                hot[1][i] = OrthTask{};
+                hot[1][i].noop = one;
+                continue;
+            }
             const long long N = in->g.N;
             const int m = k_ + in->mc, nt = in->nt;
             double *W = in->sm() + oWc, *F = in->sm() + oF;
@@ -1036,7 +1050,8 @@ int BdEngine::step_b()
         og[pass].launch(dp, s_);
         {
             ProfScope ps(PK_CHOL, s_, 0.0);
-            bd_orth(dot[pass], nc, s_);
+            // the classic factor tasks of orth2 instances are no-ops
+            if (n2 < nc || (pass == 1 && data_round2_)) bd_orth(dot[pass], nc, s_);
             if (pass == 0) bd_orth2(d2t, n2, s_);
         }
         oc[pass].launch(s_);
