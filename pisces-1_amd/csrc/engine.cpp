@@ -71,6 +71,7 @@ struct BdEngine::Inst {
     bool u_valid = false;  // U holds the Ritz block of the last step
     bool late_rf = false;  // the late restart refreshes H X in FP64
     int eig_m = 0;         // columns of the basis the last eigensolve diagonalised (0: basis changed)
+    bool s_alg = false;    // the last corrections' columns of S came from orth2 (no data path)
     BdResult res;
     double theta[kBdMaxK], rn[kBdMaxK];
     std::shared_ptr<void> keepalive;
@@ -619,7 +620,7 @@ void BdEngine::step_a()
         // low precision forms: S = V^T V of the stored basis V = [Xc | C], in
         // full after a restart, else its new columns -- which the last step's
         // orth2 already wrote where it ran (*alg = 1: the tasks do nothing)
-        if (lo) {
+        if (lo && (in->fresh || !in->s_alg)) {
             const int mv = k_ + in->mc;
             const int b0 = in->fresh ? 0 : k_ + in->mc - in->gnew;
             for (int c0 = b0; c0 < mv; c0 += kGramMaxB) {
@@ -956,6 +957,7 @@ int BdEngine::step_b()
         hot[pass] = B.take<OrthTask>(nc, &dot[pass]);
         for (int i = 0; i < nc; ++i) {
             Inst *in = corr[i];
+            if (pass == 0) in->s_alg = !data_round2_ && t32 && k_ + in->mc + in->nt <= kBdMaxM;
             if (pass == 1 && !data_round2_ && t32 && k_ + in->mc + in->nt <= kBdMaxM) {
                 // both rounds from S: nothing of the second round is launched
                 hot[1][i] = OrthTask{};
