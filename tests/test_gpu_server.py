@@ -125,3 +125,28 @@ def test_warm_chains_cross_lanes(pisces):
             assert w["status"] == nr
             np.testing.assert_allclose(w["evals"], r["evals"], rtol=0, atol=1e-6)
         assert sum(w["n_iter"] for w in warm[c][1:]) < sum(r["n_iter"] for r in cold[c][1:])
+
+
+def test_block_classes_and_lanes_in_one_mix(pisces):
+    """Requests of 2, 6 and 4 states together: the 6-state one runs on a block-8
+    engine, the others on block-4 engines, spread over two lanes each; every
+    result equals the same request solved alone."""
+    cases = [((32, 32, 32), 30.0, 6, 3.5, 4200, 2), ((40, 40, 40), 34.0, 10, 4.0, 4201, 6),
+             ((32, 32, 32), 30.0, 8, 3.5, 4202, 4), ((36, 36, 36), 32.0, 7, 3.5, 4203, 6)]
+    alone = []
+    for n, L, nw, rad, seed, nr in cases:
+        sv = pisces.Server(hbm_cap_bytes=64 << 30, max_slots=1, precision=pisces.BF16X2)
+        sv.submit(n, (L, L, L), _xyz(nw, rad, seed), nr)
+        (r,) = sv.run_until_done()
+        sv.close()
+        assert r["status"] == nr, r
+        alone.append(r)
+    sv = pisces.Server(hbm_cap_bytes=64 << 30, max_slots=8, precision=pisces.BF16X2)
+    ids = [sv.submit(n, (L, L, L), _xyz(nw, rad, seed), nr) for n, L, nw, rad, seed, nr in cases]
+    mix = {r["id"]: r for r in sv.run_until_done()}
+    sv.close()
+    for i, rid in enumerate(ids):
+        r = mix[rid]
+        assert r["status"] == cases[i][5] and len(r["evals"]) == cases[i][5]
+        np.testing.assert_allclose(r["evals"], alone[i]["evals"], rtol=0, atol=1e-6)
+        assert np.all(r["resid"] < 1e-7)
