@@ -105,12 +105,33 @@ int pisces_get_potential(const pisces_ctx *ctx, double *v_diag);
  *   q/iq      charges and their site indices; d/id dipoles and site indices
  *   ion_*     optional erf-damped point charges (Bloomfield form,
  *             src/NaCl.cpp:285-318); nion may be 0.
- * Supported: Potential 1-4, DampType 1-2, Polarization 0-1 (and adiabatic). */
+ * Supported: Potential 1-4, DampType 1-2, Polarization 0-1 (and adiabatic);
+ * Polarization 3 goes through pisces_build_potential_scp. */
 int pisces_build_potential(pisces_ctx *ctx, const int flags[6], const double *potpara,
                            int nsites, const double *sites, int nq, const double *q,
                            const int *iq, int nd, const double *d, const int *id,
                            int nion, const double *ion_xyz, const double *ion_q,
                            double ion_r);
+
+/* The same with self-consistent polarizable sites (Polarization flag 3,
+ * flags[3] == 3): Potential::Setup with its polarizable-site arguments
+ * (src/Potential.h:38-42: np, alpha, ip, Epc; SetupDPPGTOP case 3,
+ * src/Potential.cpp:387-432, InvA from ComputeInvA, src/MolPolAux.cpp:39-136)
+ * and VpolWaterWater (src/Potential.h:85, set at src/ClusterAnion.cpp:173).
+ * The water model's point dipoles are not taken: PolType 3 ignores them
+ * (src/Potential.cpp:327-335).  Vpol at a grid point is
+ * SelfConsistentPolarizability (src/Potential.cpp:1495-1592):
+ * -E^T InvA E / 2 - vpol_ww, E the electron's damped field at the sites plus
+ * epc.  Where a grid point coincides with a site, that site's electron field
+ * is taken as its limit 0 (the reference: 0 for DampType 2, 0 * inf for 1).
+ *   np sites ip[] (indices into sites), alpha[np], epc[3*np]; 3*np <= 576.
+ * Errors: PISCES_ERR_ARG (flags[3] != 3, bad lists), PISCES_ERR_NUMERIC
+ * (singular polarization matrix), PISCES_ERR_UNSUPPORTED (too many sites). */
+int pisces_build_potential_scp(pisces_ctx *ctx, const int flags[6], const double *potpara,
+                               int nsites, const double *sites, int nq, const double *q,
+                               const int *iq, int np, const double *alpha, const int *ip,
+                               const double *epc, double vpol_ww, int nion,
+                               const double *ion_xyz, const double *ion_q, double ion_r);
 
 /* Water model on the GPU: rigid DPP monomers from O,H1,H2 Cartesian input in
  * Angstrom (Water::Reposition + UpdatePositions, src/Water.cpp:695-943), the
@@ -118,6 +139,9 @@ int pisces_build_potential(pisces_ctx *ctx, const int flags[6], const double *po
  * src/DPP.cpp:760-1065), and the lists above (GetLists :402-498,
  * ReportInducedDipoles :505-529), then the grid potential.
  * Outputs (optional, may be NULL): sites[12*nw], mu[9*nw].
+ * flags[3] == 3 builds the self-consistent polarization of the O,H1,H2 sites
+ * (ReportPolSitesAndField, src/DPP.cpp:540-642) with VpolWaterWater = the
+ * water-water polarization energy (src/ClusterAnion.cpp:173).
  * Returns > 0 if the dipole SCF did not converge (the potential is still built,
  * as in the reference, which only warns). */
 int pisces_build_cluster_potential(pisces_ctx *ctx, int nw, const double *xyz_ang,

@@ -215,6 +215,56 @@ class Cluster:
         return v
 
 
+    # PolType 3: polarizable sites O,H1,H2 of every water, their permanent-
+    # charge field and VpolWaterWater = the water model's polarization energy
+    def pol3_lists(self):
+        nw = self.sites.shape[0] // 4
+        a3 = (1.0 / 0.52917720859) ** 3
+        alpha = np.tile([1.22 * a3, 0.28 * a3, 0.28 * a3], nw)
+        ip = np.array([4 * w + s for w in range(nw) for s in range(3)], dtype=np.int32)
+        return alpha, ip, np.ascontiguousarray(self.epc), float(self.e_pol)
+
+    def potential_points_pol3(self, pts, flags, potpara=None, pol=None):
+        f, pp = self._args(flags, default_potpara() if potpara is None else potpara)
+        alpha, ip, epc, vww = self.pol3_lists() if pol is None else pol
+        pts = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 3))
+        v = np.zeros(pts.shape[0])
+        parts = np.zeros((pts.shape[0], 5))
+        rc = lib().or_potential_points_pol3(
+            _pi(f), _p(pp), self.sites.shape[0], _p(self.sites), len(self.q), _p(self.q),
+            _pi(self.iq), len(ip), _p(np.ascontiguousarray(alpha, dtype=np.float64)),
+            _pi(np.ascontiguousarray(ip, dtype=np.int32)), _p(np.ascontiguousarray(epc)),
+            ctypes.c_double(vww), pts.shape[0], _p(pts), _p(v), _p(parts))
+        if rc != 0:
+            raise ValueError("unsupported potential flags (rc=%d)" % rc)
+        return v, parts
+
+    def potential_grid_pol3(self, grid, flags, potpara=None):
+        f, pp = self._args(flags, default_potpara() if potpara is None else potpara)
+        alpha, ip, epc, vww = self.pol3_lists()
+        v = np.zeros(grid.ngp)
+        nx, ny, nz = grid.n
+        rc = lib().or_potential_grid_pol3(
+            _pi(f), _p(pp), self.sites.shape[0], _p(self.sites), len(self.q), _p(self.q),
+            _pi(self.iq), len(ip), _p(alpha), _pi(ip), _p(epc), ctypes.c_double(vww),
+            nx, ny, nz, _p(grid.x[0]), _p(grid.x[1]), _p(grid.x[2]), _p(v))
+        if rc != 0:
+            raise ValueError("unsupported potential flags (rc=%d)" % rc)
+        return v
+
+
+def pol3_inva(rpol, alpha):
+    """InvA = (alpha^-1 - T)^-1 of the polarizable sites (row-major 3np x 3np)."""
+    rp = np.ascontiguousarray(np.asarray(rpol, dtype=np.float64).reshape(-1))
+    np_ = rp.size // 3
+    out = np.zeros((3 * np_, 3 * np_))
+    rc = lib().or_pol3_inva(np_, _p(rp), _p(np.ascontiguousarray(alpha, dtype=np.float64)),
+                            _p(out))
+    if rc != 0:
+        raise ValueError("singular polarization matrix")
+    return out
+
+
 # ---------------------------------------------------------------- reference runs
 def write_records(path, recs):
     with open(path, "wb") as f:

@@ -464,6 +464,13 @@ typedef struct {
     int *ipps;
     double *rep;  /* exponent, coefficient */
     int *irep;
+    /* PolType 3: np polarizable sites ip[], their charge field epc[3np],
+     * InvA (3np x 3np, both triangles from the lower one) and VpolWaterWater */
+    int np;
+    const int *ip;
+    const double *epc;
+    double *inva;
+    double vpolww;
 } or_pot;
 
 static double reff(double r, double r0)
@@ -484,7 +491,7 @@ static int pot_setup(or_pot *P, const int *flags, const double *pp, int nsites)
     P->damp_type = flags[1];
     P->pol_type = flags[3];
     P->adiabatic = flags[4];
-    if (P->pol_type != 0 && P->pol_type != 1) return -2;
+    if (P->pol_type != 0 && P->pol_type != 1 && P->pol_type != 3) return -2;
     P->alpha = pp[0];
     double vrep = pp[2], vrepO = (flags[2] == 1) ? pp[5] : pp[2];
     if (P->damp_type == 1) {
@@ -547,13 +554,14 @@ static int pot_setup(or_pot *P, const int *flags, const double *pp, int nsites)
 
 static void pot_free(or_pot *P)
 {
-    free(P->ipps); free(P->rep); free(P->irep);
+    free(P->ipps); free(P->rep); free(P->irep); free(P->inva);
 }
 
 /* EvaluateDPPGTOP at one point with the per-site distance table of
  * Potential::Evaluate (src/Potential.cpp:136-146). */
 static double pot_eval(const or_pot *P, const double *r, double *R, double *R2,
-                       double *Rx, double *Ry, double *Rz, double *Rm3, double *parts)
+                       double *Rx, double *Ry, double *Rz, double *Rm3, double *parts,
+                       double *work)
 {
     for (int i = 0; i < P->ns; ++i) {
         Rx[i] = r[0] - P->sites[3 * i];
@@ -615,6 +623,35 @@ static double pot_eval(const or_pot *P, const double *r, double *R, double *R2,
         }
         if (P->adiabatic) vpol = P->eps - sqrt(P->eps * (P->eps + sp));
         else vpol = -0.5 * sp;
+    } else if (P->pol_type == 3) {
+        /* SelfConsistentPolarizability, src/Potential.cpp:1495-1592: the
+         * electron's field at the polarizable sites (damped R^-3), plus the
+         * permanent-charge field, mu = InvA E (dsymv on the lower triangle),
+         * Vpol = -E.mu / 2 - VpolWaterWater */
+        const int n3 = 3 * P->np;
+        double *E = work, *mu = work + n3;
+        for (int i = 0; i < P->np; ++i) {
+            int s = P->ip[i];
+            double g = Rm3[s];
+            if (P->damp_type == 1) {
+                g *= 1.0 - exp(-P->pdamp * R[s] * R2[s]);
+            } else if (R[s] < P->pdamp) {
+                double re = reff(R[s], P->pdamp);
+                g = 1.0 / (re * re * re);
+            }
+            E[3 * i] = g * Rx[s] + P->epc[3 * i];
+            E[3 * i + 1] = g * Ry[s] + P->epc[3 * i + 1];
+            E[3 * i + 2] = g * Rz[s] + P->epc[3 * i + 2];
+        }
+        for (int a = 0; a < n3; ++a) {
+            const double *row = P->inva + (long)n3 * a;
+            double acc = 0;
+            for (int b = 0; b < n3; ++b) acc += row[b] * E[b];
+            mu[a] = acc;
+        }
+        double em = 0;
+        for (int a = 0; a < n3; ++a) em += E[a] * mu[a];
+        vpol = -0.5 * em - P->vpolww;
     }
     /* ions (Bloomfield erf-damped charge, src/NaCl.cpp:300-307) */
     double vion = 0;
@@ -646,6 +683,42 @@ static int pot_init(or_pot *P, const int *flags, const double *pp, int nsites,
     return 0;
 }
 
+static void pot_points(const or_pot *P, int npts, const double *pts, double *v, double *parts)
+{
+    const int ns = P->ns, n3 = 3 * P->np;
+#pragma omp parallel num_threads(nthreads())
+    {
+        double *buf = malloc(sizeof(double) * (6 * (ns + 1) + 2 * n3 + 2));
+        double *R = buf, *R2 = buf + ns, *Rx = buf + 2 * ns, *Ry = buf + 3 * ns,
+               *Rz = buf + 4 * ns, *Rm3 = buf + 5 * ns, *work = buf + 6 * (ns + 1);
+#pragma omp for schedule(static)
+        for (int p = 0; p < npts; ++p)
+            v[p] = pot_eval(P, pts + 3 * p, R, R2, Rx, Ry, Rz, Rm3, parts ? parts + 5 * p : NULL,
+                            work);
+        free(buf);
+    }
+}
+
+static void pot_grid(const or_pot *P, int nx, int ny, int nz, const double *xg,
+                     const double *yg, const double *zg, double *v)
+{
+    const int ns = P->ns, n3 = 3 * P->np;
+    const long ng = (long)nx * ny * nz;
+#pragma omp parallel num_threads(nthreads())
+    {
+        double *buf = malloc(sizeof(double) * (6 * (ns + 1) + 2 * n3 + 2));
+        double *R = buf, *R2 = buf + ns, *Rx = buf + 2 * ns, *Ry = buf + 3 * ns,
+               *Rz = buf + 4 * ns, *Rm3 = buf + 5 * ns, *work = buf + 6 * (ns + 1);
+#pragma omp for schedule(static)
+        for (long g = 0; g < ng; ++g) {
+            int i = (int)(g % nx), j = (int)((g / nx) % ny), k = (int)(g / ((long)nx * ny));
+            double r[3] = {xg[i], yg[j], zg[k]};
+            v[g] = pot_eval(P, r, R, R2, Rx, Ry, Rz, Rm3, NULL, work);
+        }
+        free(buf);
+    }
+}
+
 int or_potential_points(const int *flags, const double *potpara, int nsites,
                         const double *sites, int nq, const double *q, const int *iq,
                         int nd, const double *d, const int *id, int nion,
@@ -655,17 +728,9 @@ int or_potential_points(const int *flags, const double *potpara, int nsites,
     or_pot P;
     int rc = pot_init(&P, flags, potpara, nsites, sites, nq, q, iq, nd, d, id, nion,
                       ion_xyz, ion_q, ion_r);
+    if (!rc && P.pol_type == 3) rc = -2;  /* needs the polarizable-site lists */
     if (rc) { pot_free(&P); return rc; }
-#pragma omp parallel num_threads(nthreads())
-    {
-        double *buf = malloc(sizeof(double) * 6 * (nsites + 1));
-        double *R = buf, *R2 = buf + nsites, *Rx = buf + 2 * nsites, *Ry = buf + 3 * nsites,
-               *Rz = buf + 4 * nsites, *Rm3 = buf + 5 * nsites;
-#pragma omp for schedule(static)
-        for (int p = 0; p < npts; ++p)
-            v[p] = pot_eval(&P, pts + 3 * p, R, R2, Rx, Ry, Rz, Rm3, parts ? parts + 5 * p : NULL);
-        free(buf);
-    }
+    pot_points(&P, npts, pts, v, parts);
     pot_free(&P);
     return 0;
 }
@@ -680,21 +745,132 @@ int or_potential_grid(const int *flags, const double *potpara, int nsites,
     or_pot P;
     int rc = pot_init(&P, flags, potpara, nsites, sites, nq, q, iq, nd, d, id, nion,
                       ion_xyz, ion_q, ion_r);
+    if (!rc && P.pol_type == 3) rc = -2;
     if (rc) { pot_free(&P); return rc; }
-    const long ng = (long)nx * ny * nz;
-#pragma omp parallel num_threads(nthreads())
-    {
-        double *buf = malloc(sizeof(double) * 6 * (nsites + 1));
-        double *R = buf, *R2 = buf + nsites, *Rx = buf + 2 * nsites, *Ry = buf + 3 * nsites,
-               *Rz = buf + 4 * nsites, *Rm3 = buf + 5 * nsites;
-#pragma omp for schedule(static)
-        for (long g = 0; g < ng; ++g) {
-            int i = (int)(g % nx), j = (int)((g / nx) % ny), k = (int)(g / ((long)nx * ny));
-            double r[3] = {xg[i], yg[j], zg[k]};
-            v[g] = pot_eval(&P, r, R, R2, Rx, Ry, Rz, Rm3, NULL);
+    pot_grid(&P, nx, ny, nz, xg, yg, zg, v);
+    pot_free(&P);
+    return 0;
+}
+
+/* ------------------------------------------------------------ PolType 3 */
+/* Thole-damped dipole-dipole tensor of a site pair, CalcDDTensor
+ * (src/MolPolAux.cpp): T = 3 f5 R R^T / r^5 - f3 I / r^3 with
+ * x = a r^3 / sqrt(alpha_i alpha_j), f3 = 1 - e^-x, f5 = 1 - (1 + x) e^-x. */
+static void dd_tensor(const double *rij, double ai, double aj, double a, double T[9])
+{
+    double r2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+    double r = sqrt(r2), r3 = r * r2, r5 = r2 * r3;
+    double x = a * r3 / sqrt(ai * aj), e = exp(-x);
+    double f3 = 1 - e, f5 = 1 - (1 + x) * e;
+    for (int u = 0; u < 3; ++u)
+        for (int w = 0; w < 3; ++w)
+            T[3 * u + w] = 3.0 * f5 * rij[u] * rij[w] / r5 - (u == w ? f3 / r3 : 0.0);
+}
+
+int or_pol3_inva(int np, const double *rpol, const double *alpha, double *inva)
+{
+    /* BuildPolarizationMatrix (src/MolPolAux.cpp:100-136): alpha^-1 on the
+     * diagonal 3x3 blocks, -T_ij off the diagonal for every pair (Thole
+     * a = 0.3, src/Potential.cpp:425); then the inverse (InvertMatrix uses
+     * dsytrf/dsytri; here Gauss-Jordan elimination with partial pivoting). */
+    const int n = 3 * np;
+    double *A = malloc(sizeof(double) * (size_t)n * n);
+    memset(A, 0, sizeof(double) * (size_t)n * n);
+    for (int i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) A[(size_t)n * (3 * i + c) + 3 * i + c] = 1.0 / alpha[i];
+    for (int i = 0; i < np; ++i)
+        for (int j = i + 1; j < np; ++j) {
+            double rij[3] = {rpol[3 * i] - rpol[3 * j], rpol[3 * i + 1] - rpol[3 * j + 1],
+                             rpol[3 * i + 2] - rpol[3 * j + 2]};
+            double T[9];
+            dd_tensor(rij, alpha[i], alpha[j], 0.3, T);
+            for (int u = 0; u < 3; ++u)
+                for (int w = 0; w < 3; ++w) {
+                    A[(size_t)n * (3 * i + u) + 3 * j + w] = -T[3 * u + w];
+                    A[(size_t)n * (3 * j + u) + 3 * i + w] = -T[3 * w + u];
+                }
         }
-        free(buf);
+    for (long k = 0; k < (long)n * n; ++k) inva[k] = (k % (n + 1) == 0) ? 1.0 : 0.0;
+    int rc = 0;
+    for (int c = 0; c < n && !rc; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (fabs(A[(size_t)n * r + c]) > fabs(A[(size_t)n * piv + c])) piv = r;
+        if (A[(size_t)n * piv + c] == 0.0) { rc = -1; break; }
+        if (piv != c)
+            for (int k = 0; k < n; ++k) {
+                double t = A[(size_t)n * c + k]; A[(size_t)n * c + k] = A[(size_t)n * piv + k]; A[(size_t)n * piv + k] = t;
+                t = inva[(size_t)n * c + k]; inva[(size_t)n * c + k] = inva[(size_t)n * piv + k]; inva[(size_t)n * piv + k] = t;
+            }
+        double f = 1.0 / A[(size_t)n * c + c];
+        for (int k = 0; k < n; ++k) { A[(size_t)n * c + k] *= f; inva[(size_t)n * c + k] *= f; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            double m = A[(size_t)n * r + c];
+            if (m == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                A[(size_t)n * r + k] -= m * A[(size_t)n * c + k];
+                inva[(size_t)n * r + k] -= m * inva[(size_t)n * c + k];
+            }
+        }
     }
+    free(A);
+    return rc;
+}
+
+static int pol3_init(or_pot *P, int np, const double *alpha, const int *ip, const double *epc,
+                     double vpolww)
+{
+    /* Potential::SetupDPPGTOP case 3 (src/Potential.cpp:387-432): the
+     * polarizable sites, their charge field and InvA; the point dipoles of the
+     * water model are ignored (:327-335) */
+    if (P->pol_type != 3 || np <= 0) return -2;
+    for (int i = 0; i < np; ++i)
+        if (ip[i] < 0 || ip[i] >= P->ns) return -3;
+    const int n = 3 * np;
+    double *rp = malloc(sizeof(double) * n);
+    for (int i = 0; i < np; ++i)
+        for (int c = 0; c < 3; ++c) rp[3 * i + c] = P->sites[3 * ip[i] + c];
+    P->inva = malloc(sizeof(double) * (size_t)n * n);
+    int rc = or_pol3_inva(np, rp, alpha, P->inva);
+    free(rp);
+    if (rc) return rc;
+    /* dsymv("L") reads the lower triangle only */
+    for (int a = 0; a < n; ++a)
+        for (int b = a + 1; b < n; ++b) P->inva[(size_t)n * a + b] = P->inva[(size_t)n * b + a];
+    P->np = np; P->ip = ip; P->epc = epc; P->vpolww = vpolww;
+    P->nd = 0;
+    return 0;
+}
+
+int or_potential_points_pol3(const int *flags, const double *potpara, int nsites,
+                             const double *sites, int nq, const double *q, const int *iq,
+                             int np, const double *alpha, const int *ip, const double *epc,
+                             double vpolww, int npts, const double *pts, double *v,
+                             double *parts)
+{
+    or_pot P;
+    int rc = pot_init(&P, flags, potpara, nsites, sites, nq, q, iq, 0, NULL, NULL, 0, NULL,
+                      NULL, 1.0);
+    if (!rc) rc = pol3_init(&P, np, alpha, ip, epc, vpolww);
+    if (rc) { pot_free(&P); return rc; }
+    pot_points(&P, npts, pts, v, parts);
+    pot_free(&P);
+    return 0;
+}
+
+int or_potential_grid_pol3(const int *flags, const double *potpara, int nsites,
+                           const double *sites, int nq, const double *q, const int *iq,
+                           int np, const double *alpha, const int *ip, const double *epc,
+                           double vpolww, int nx, int ny, int nz, const double *xg,
+                           const double *yg, const double *zg, double *v)
+{
+    or_pot P;
+    int rc = pot_init(&P, flags, potpara, nsites, sites, nq, q, iq, 0, NULL, NULL, 0, NULL,
+                      NULL, 1.0);
+    if (!rc) rc = pol3_init(&P, np, alpha, ip, epc, vpolww);
+    if (rc) { pot_free(&P); return rc; }
+    pot_grid(&P, nx, ny, nz, xg, yg, zg, v);
     pot_free(&P);
     return 0;
 }

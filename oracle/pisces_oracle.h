@@ -112,6 +112,36 @@ int or_potential_grid(const int *flags, const double *potpara, int nsites,
                       int nx, int ny, int nz, const double *xg, const double *yg,
                       const double *zg, double *v);
 
+/* PolType 3 (fully interacting polarizable sites).
+ * or_pol3_inva: InvA = (alpha^-1 - T)^-1 (3np x 3np, row-major) of np sites
+ * at rpol (bohr), Thole-damped T with a = 0.3 between every pair
+ * (ComputeInvA / BuildPolarizationMatrix / CalcDDTensor, src/MolPolAux.cpp:39-136;
+ * InvertMatrix :264-293 uses dsytrf/dsytri, here Gauss-Jordan with partial
+ * pivoting).  Returns 0, or -1 for a singular matrix. */
+int or_pol3_inva(int np, const double *rpol, const double *alpha, double *inva);
+/* The potential with flags[3] == 3: Vpc + Vrep as above, the water model's
+ * point dipoles ignored (src/Potential.cpp:327-335), and
+ * Vpol = -E^T InvA E / 2 - VpolWaterWater (SelfConsistentPolarizability,
+ * src/Potential.cpp:1495-1592) with E the electron's damped field at the np
+ * polarizable sites ip[] (site indices; alpha[np]) plus their permanent-charge
+ * field epc[3np] (ReportPolSitesAndField, src/DPP.cpp:540-642), and
+ * vpolww = VpolWaterWater (src/ClusterAnion.cpp:173). */
+int or_potential_points_pol3(const int *flags, const double *potpara, int nsites,
+                             const double *sites, int nq, const double *q, const int *iq,
+                             int np, const double *alpha, const int *ip, const double *epc,
+                             double vpolww, int npts, const double *pts, double *v,
+                             double *parts);
+int or_potential_grid_pol3(const int *flags, const double *potpara, int nsites,
+                           const double *sites, int nq, const double *q, const int *iq,
+                           int np, const double *alpha, const int *ip, const double *epc,
+                           double vpolww, int nx, int ny, int nz, const double *xg,
+                           const double *yg, const double *zg, double *v);
+/* Pinned: InvA E_pc equals the water model's self-consistent dipoles
+ * (or_water_induced solves the same fixed point with the same tensors by
+ * iteration); with vpolww = e_pol the field-free limit is 0; one site reduces
+ * to PolType 1 (DampType 2, same Reff damping); InvA (alpha^-1 - T) = I.
+ * No reference run: the reference's PolType 3 needs LAPACK, absent here. */
+
 /* Lists exactly as ClusterAnion::EnergyFromConfiguration builds them
  * (GetLists, src/DPP.cpp:402-498; ReportInducedDipoles :505-529):
  *   q[3*nw], iq[3*nw]  charges on H1,H2,M;  d[9*nw], id[3*nw] induced dipoles

@@ -184,19 +184,16 @@ int pisces_get_potential(const pisces_ctx *c, double *v)
     });
 }
 
-int pisces_build_potential(pisces_ctx *c, const int flags[6], const double *potpara, int nsites,
-                           const double *sites, int nq, const double *q, const int *iq, int nd,
-                           const double *d, const int *id, int nion, const double *ion_xyz,
-                           const double *ion_q, double ion_r)
+namespace {
+
+// Potential::Setup + DVR::ComputePotential: the site terms on the grid, then
+// (np > 0, PolType 3) the self-consistent polarization added on top
+int build_potential_impl(pisces_ctx *c, const int flags[6], const double *potpara, int nsites,
+                         const double *sites, int nq, const double *q, const int *iq, int nd,
+                         const double *d, const int *id, int np, const double *alpha,
+                         const int *ip, const double *epc, double vpol_ww, int nion,
+                         const double *ion_xyz, const double *ion_q, double ion_r)
 {
-    if (!c || !flags || !potpara || nsites <= 0 || !sites || nq < 0 || nd < 0 || nion < 0)
-        return PISCES_ERR_ARG;
-    if ((nq && (!q || !iq)) || (nd && (!d || !id)) || (nion && (!ion_xyz || !ion_q)))
-        return PISCES_ERR_ARG;
-    for (int i = 0; i < nq; ++i)
-        if (iq[i] < 0 || iq[i] >= nsites) return PISCES_ERR_ARG;
-    for (int i = 0; i < nd; ++i)
-        if (id[i] < 0 || id[i] >= nsites) return PISCES_ERR_ARG;
     return guarded([&] {
         PC_HIP(hipSetDevice(c->device));
         PotParams P{};
@@ -204,8 +201,10 @@ int pisces_build_potential(pisces_ctx *c, const int flags[6], const double *potp
         std::vector<int> irep, ipps;
         PC_REQUIRE(potential_setup(flags, potpara, nsites, P, rep, irep, ipps),
                    PISCES_ERR_UNSUPPORTED, "unsupported potential flags");
+        PC_REQUIRE((P.pol_type == 3) == (np > 0), PISCES_ERR_ARG,
+                   "PolType 3 takes the polarizable-site lists (pisces_build_potential_scp)");
         P.ion_r = ion_r;
-        dbuf ds, dq, diq, dd, did, drep, direp, dipps, dix, diq2;
+        dbuf ds, dq, diq, dd, did, drep, direp, dipps, dix, diq2, drp, depc, dinv;
         upload(ds, sites, sizeof(double) * 3 * nsites, c->stream);
         if (nq) { upload(dq, q, sizeof(double) * nq, c->stream); upload(diq, iq, sizeof(int) * nq, c->stream); }
         if (nd) { upload(dd, d, sizeof(double) * 3 * nd, c->stream); upload(did, id, sizeof(int) * nd, c->stream); }
@@ -225,10 +224,73 @@ int pisces_build_potential(pisces_ctx *c, const int flags[6], const double *potp
         potential_grid_launch(P, D, c->n[0], c->n[1], c->n[2], c->dx[0].as<double>(),
                               c->dx[1].as<double>(), c->dx[2].as<double>(), c->dV64.as<double>(),
                               nullptr, c->stream);
+        if (np > 0) {
+            // SetupDPPGTOP case 3 (src/Potential.cpp:387-432): InvA of the
+            // polarizable sites on the host, the grid sum on the matrix cores
+            const int n3p = pol3_padded(np);
+            PC_REQUIRE(n3p <= kPol3MaxN, PISCES_ERR_UNSUPPORTED, "too many polarizable sites");
+            std::vector<double> rp((size_t)3 * np), ep((size_t)n3p, 0.0), inv((size_t)n3p * n3p);
+            for (int i = 0; i < np; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    rp[3 * i + k] = sites[3 * ip[i] + k];
+                    ep[3 * i + k] = epc[3 * i + k];
+                }
+            PC_REQUIRE(pol3_inva_host(np, rp.data(), alpha, n3p, inv.data()), PISCES_ERR_NUMERIC,
+                       "singular polarization matrix");
+            upload(drp, rp.data(), sizeof(double) * rp.size(), c->stream);
+            upload(depc, ep.data(), sizeof(double) * ep.size(), c->stream);
+            upload(dinv, inv.data(), sizeof(double) * inv.size(), c->stream);
+            Pol3Dev Q{np, n3p, P.damp_type, 0, P.pdamp, vpol_ww, drp.as<double>(),
+                      depc.as<double>(), dinv.as<double>()};
+            pol3_grid_launch(Q, c->n[0], c->n[1], c->n[2], c->dx[0].as<double>(),
+                             c->dx[1].as<double>(), c->dx[2].as<double>(), c->dV64.as<double>(),
+                             c->stream);
+        }
         set_potential_device(c);
         c->sync();
         return 0;
     });
+}
+
+bool lists_ok(int n, const int *idx, int nsites)
+{
+    for (int i = 0; i < n; ++i)
+        if (idx[i] < 0 || idx[i] >= nsites) return false;
+    return true;
+}
+
+}  // namespace
+
+int pisces_build_potential(pisces_ctx *c, const int flags[6], const double *potpara, int nsites,
+                           const double *sites, int nq, const double *q, const int *iq, int nd,
+                           const double *d, const int *id, int nion, const double *ion_xyz,
+                           const double *ion_q, double ion_r)
+{
+    if (!c || !flags || !potpara || nsites <= 0 || !sites || nq < 0 || nd < 0 || nion < 0)
+        return PISCES_ERR_ARG;
+    if ((nq && (!q || !iq)) || (nd && (!d || !id)) || (nion && (!ion_xyz || !ion_q)))
+        return PISCES_ERR_ARG;
+    if (!lists_ok(nq, iq, nsites) || !lists_ok(nd, id, nsites)) return PISCES_ERR_ARG;
+    return build_potential_impl(c, flags, potpara, nsites, sites, nq, q, iq, nd, d, id, 0,
+                                nullptr, nullptr, nullptr, 0.0, nion, ion_xyz, ion_q, ion_r);
+}
+
+int pisces_build_potential_scp(pisces_ctx *c, const int flags[6], const double *potpara,
+                               int nsites, const double *sites, int nq, const double *q,
+                               const int *iq, int np, const double *alpha, const int *ip,
+                               const double *epc, double vpol_ww, int nion,
+                               const double *ion_xyz, const double *ion_q, double ion_r)
+{
+    if (!c || !flags || !potpara || nsites <= 0 || !sites || nq < 0 || np <= 0 || nion < 0)
+        return PISCES_ERR_ARG;
+    if ((nq && (!q || !iq)) || !alpha || !ip || !epc || (nion && (!ion_xyz || !ion_q)))
+        return PISCES_ERR_ARG;
+    if (flags[3] != 3) return PISCES_ERR_ARG;
+    if (!lists_ok(nq, iq, nsites) || !lists_ok(np, ip, nsites)) return PISCES_ERR_ARG;
+    // the water model's point dipoles are not passed: PolType 3 ignores them
+    // (src/Potential.cpp:327-335)
+    return build_potential_impl(c, flags, potpara, nsites, sites, nq, q, iq, 0, nullptr, nullptr,
+                                np, alpha, ip, epc, vpol_ww, nion, ion_xyz, ion_q, ion_r);
 }
 
 int pisces_build_cluster_potential(pisces_ctx *c, int nw, const double *xyz_ang,
@@ -262,9 +324,33 @@ int pisces_build_cluster_potential(pisces_ctx *c, int nw, const double *xyz_ang,
         std::vector<double> q;
         std::vector<int> iq, id;
         water_lists_host(nw, q, iq, id);
-        const int rc = pisces_build_potential(c, flags, potpara, 4 * nw, sites.data(), (int)q.size(),
-                                              q.data(), iq.data(), (int)id.size(), mu.data(),
-                                              id.data(), nion, ion_xyz_bohr, ion_q, ion_r);
+        int rc;
+        if (flags[3] == 3) {
+            // ReportPolSitesAndField (src/DPP.cpp:540-642): the sites O,H1,H2
+            // with their polarizabilities and permanent-charge field; and
+            // VpolWaterWater = the water-water polarization energy
+            // (src/ClusterAnion.cpp:173; WaterWaterPolarizationEnergy,
+            // src/DPP.cpp:1935-2024, summed over all pairs: -mu.Epc / 2)
+            std::vector<double> epc((size_t)3 * ns);
+            PC_HIP(hipMemcpy(epc.data(), depc.p, sizeof(double) * epc.size(), hipMemcpyDeviceToHost));
+            const double a3 = 1.0 / (0.52917720859 * 0.52917720859 * 0.52917720859);
+            std::vector<double> alpha((size_t)ns);
+            std::vector<int> ip((size_t)ns);
+            double eww = 0.0;
+            for (int i = 0; i < ns; ++i) {
+                alpha[i] = (i % 3 == 0 ? 1.22 : 0.28) * a3;  // src/ttm_data.h:51-52
+                ip[i] = 4 * (i / 3) + i % 3;
+            }
+            for (int k = 0; k < 3 * ns; ++k) eww += mu[k] * epc[k];
+            rc = pisces_build_potential_scp(c, flags, potpara, 4 * nw, sites.data(), (int)q.size(),
+                                            q.data(), iq.data(), ns, alpha.data(), ip.data(),
+                                            epc.data(), -0.5 * eww, nion, ion_xyz_bohr, ion_q,
+                                            ion_r);
+        } else {
+            rc = pisces_build_potential(c, flags, potpara, 4 * nw, sites.data(), (int)q.size(),
+                                        q.data(), iq.data(), (int)id.size(), mu.data(), id.data(),
+                                        nion, ion_xyz_bohr, ion_q, ion_r);
+        }
         if (rc < 0) return rc;
         return info[0] ? 0 : 1;
     });

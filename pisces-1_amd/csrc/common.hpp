@@ -154,6 +154,27 @@ void potential_grid_batched_launch(const PotParams &P, const PotTask *d_tasks, i
                                    long long max_ng, hipStream_t s);
 int potential_max_sites();
 
+// ------------------------------------- self-consistent polarization (PolType 3)
+// The polarizable sites of Potential::SetupDPPGTOP case 3 on the device.
+// n3p = 3 np rounded up to a multiple of 16; epc and inva are zero past 3 np.
+struct Pol3Dev {
+    int np, n3p, damp_type, pad;
+    double pdamp, vpolww;
+    const double *rpol;  // 3*np site positions
+    const double *epc;   // n3p: permanent-charge field at the sites
+    const double *inva;  // n3p x n3p, row-major
+};
+constexpr int kPol3MaxN = 576;  // largest n3p (192 polarizable sites, 64 waters)
+inline int pol3_padded(int np) { return (3 * np + 15) / 16 * 16; }
+// Host: InvA = (alpha^-1 - T)^-1 (Thole a = 0.3 between every pair), both
+// triangles taken from the lower one, zero-padded to n3p x n3p.  false if the
+// matrix is singular.
+bool pol3_inva_host(int np, const double *rpol, const double *alpha, int n3p, double *inva);
+// Device: d_v[g] += -E^T InvA E / 2 - vpolww at every grid point, E the
+// electron's damped field at the sites plus epc.
+void pol3_grid_launch(const Pol3Dev &D, int nx, int ny, int nz, const double *d_x,
+                      const double *d_y, const double *d_z, double *d_v, hipStream_t s);
+
 // --------------------------------------------------------------- water model
 // Host: rigid-monomer projection of O,H1,H2 (Angstrom) to 4 sites (bohr).
 bool water_sites_host(int nw, const double *xyz_ang, double *sites);

@@ -1,6 +1,7 @@
 // host.cpp -- host-side FP64 pieces of the PISCES MI355X library: sine-DVR
 // kinetic matrices, the rigid-water projection, the potential parameter tables
 // and the small symmetric eigensolver of the projected Davidson matrix.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -149,6 +150,70 @@ void water_lists_host(int nw, std::vector<double> &q, std::vector<int> &iq, std:
     }
 }
 
+// ------------------------------------------------------------------ PolType 3
+// ComputeInvA (src/MolPolAux.cpp:39-136): A = alpha^-1 on the diagonal 3x3
+// blocks and -T_ij off the diagonal, T_ij the Thole-damped dipole tensor of
+// CalcDDTensor (T = 3 f5 R R^T / r^5 - f3 I / r^3, x = a r^3 / sqrt(alpha_i
+// alpha_j), f3 = 1 - e^-x, f5 = 1 - (1 + x) e^-x, a = 0.3) for every pair; then
+// its inverse (the reference: dsytrf/dsytri; here Gauss-Jordan elimination with
+// partial pivoting on [A | I]).
+bool pol3_inva_host(int np, const double *rpol, const double *alpha, int n3p, double *inva)
+{
+    const int n = 3 * np;
+    if (np <= 0 || n3p < n) return false;
+    std::vector<double> A((size_t)n * n, 0.0), X((size_t)n * n, 0.0);
+    for (int i = 0; i < np; ++i) {
+        if (!(alpha[i] > 0.0)) return false;
+        for (int c = 0; c < 3; ++c) A[(size_t)n * (3 * i + c) + 3 * i + c] = 1.0 / alpha[i];
+    }
+    for (int i = 0; i < np; ++i)
+        for (int j = i + 1; j < np; ++j) {
+            const double r[3] = {rpol[3 * i] - rpol[3 * j], rpol[3 * i + 1] - rpol[3 * j + 1],
+                                 rpol[3 * i + 2] - rpol[3 * j + 2]};
+            const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            const double rr = std::sqrt(r2), r3 = rr * r2, r5 = r3 * r2;
+            const double x = 0.3 * r3 / std::sqrt(alpha[i] * alpha[j]), ex = std::exp(-x);
+            const double f3 = 1.0 - ex, f5 = 1.0 - (1.0 + x) * ex;
+            for (int u = 0; u < 3; ++u)
+                for (int w = 0; w < 3; ++w) {
+                    const double t = 3.0 * f5 * r[u] * r[w] / r5 - (u == w ? f3 / r3 : 0.0);
+                    A[(size_t)n * (3 * i + u) + 3 * j + w] = -t;
+                    A[(size_t)n * (3 * j + w) + 3 * i + u] = -t;
+                }
+        }
+    for (int k = 0; k < n; ++k) X[(size_t)n * k + k] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(A[(size_t)n * r + c]) > std::fabs(A[(size_t)n * piv + c])) piv = r;
+        if (A[(size_t)n * piv + c] == 0.0) return false;
+        if (piv != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(A[(size_t)n * c + k], A[(size_t)n * piv + k]);
+                std::swap(X[(size_t)n * c + k], X[(size_t)n * piv + k]);
+            }
+        const double f = 1.0 / A[(size_t)n * c + c];
+        double *ac = &A[(size_t)n * c], *xc = &X[(size_t)n * c];
+        for (int k = 0; k < n; ++k) { ac[k] *= f; xc[k] *= f; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double m = A[(size_t)n * r + c];
+            if (m == 0.0) continue;
+            double *ar = &A[(size_t)n * r], *xr = &X[(size_t)n * r];
+            for (int k = c; k < n; ++k) ar[k] -= m * ac[k];
+            for (int k = 0; k < n; ++k) xr[k] -= m * xc[k];
+        }
+    }
+    // dsymv("L") in SelfConsistentPolarizability reads the lower triangle only
+    std::fill(inva, inva + (size_t)n3p * n3p, 0.0);
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b <= a; ++b)
+            inva[(size_t)n3p * a + b] = inva[(size_t)n3p * b + a] = X[(size_t)n * a + b];
+    for (size_t k = 0; k < (size_t)n3p * n3p; ++k)
+        if (!std::isfinite(inva[k])) return false;
+    return true;
+}
+
 // ------------------------------------------------------------------ potential
 // Potential::SetupDPPGTOP (src/Potential.cpp:234-593) with the repulsive cores
 // Set6GTORepCore / Set12GTORepCore / Set4STORepCore (:598-728).  The numeric
@@ -162,7 +227,7 @@ bool potential_setup(const int *flags, const double *pp, int nsites, PotParams &
     P.damp_type = flags[1];
     P.pol_type = flags[3];
     P.adiabatic = flags[4];
-    if (P.pol_type != 0 && P.pol_type != 1) return false;
+    if (P.pol_type != 0 && P.pol_type != 1 && P.pol_type != 3) return false;
     P.alpha = pp[0];
     const double vrep = pp[2], vrepO = (flags[2] == 1) ? pp[5] : pp[2];
     if (P.damp_type == 1) {

@@ -1,0 +1,162 @@
+// pol3.hip -- the self-consistent polarization potential (PolType 3) on the grid.
+//
+// Potential::SelfConsistentPolarizability (src/Potential.cpp:1495-1592) at a
+// grid point r: E_i = g(R_i) R_i + Epc_i at the polarizable sites i (R_i =
+// r - site i, g the damped R^-3), mu = InvA E, Vpol = -E.mu / 2 -
+// VpolWaterWater.  Over a tile of grid points that is the product E InvA (points
+// x 3np times 3np x 3np) followed by a row-wise dot with E: FP64 matrix-core
+// work of 2 (3np)^2 flops per point.  A workgroup takes 32 points: it builds
+// their E rows in LDS (odd pitch: the 16 rows that one matrix-core operand
+// reads fall on distinct banks), then each of its four waves takes 16-column
+// blocks of InvA, streams them from L2 (InvA is shared by all workgroups: at
+// most 2.6 MB) as the B operand of v_mfma_f64_16x16x4 against both 16-point row
+// blocks of E, and folds each 16 x 16 result into the per-point dot at once.
+#include "common.hpp"
+#include "gmem.hpp"
+
+#include <mutex>
+
+namespace pisces {
+
+namespace {
+
+constexpr int kP3Threads = 256;
+constexpr int kP3Pts = 32;  // grid points per workgroup: two 16-row blocks
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline double reff3(double r, double r0)
+{
+    // effective-distance damping, src/Potential.cpp:1191-1196
+    if (r < r0) {
+        const double t = r / r0;
+        return r0 * (0.5 + t * t * t * (1.0 - 0.5 * t));
+    }
+    return r;
+}
+
+__global__ void __launch_bounds__(kP3Threads)
+pol3_grid_kernel(Pol3Dev D, int nx, int ny, int nz, const double *__restrict__ gx,
+                 const double *__restrict__ gy, const double *__restrict__ gz,
+                 double *__restrict__ v)
+{
+    extern __shared__ double Es[];  // kP3Pts rows of n3p (+1) values
+    __shared__ double red[kP3Threads / 64][kP3Pts];
+    const int n3p = D.n3p, ldE = n3p + 1, np = D.np, n3 = 3 * np;
+    const long long ng = (long long)nx * ny * nz;
+    const long long g0 = (long long)blockIdx.x * kP3Pts;
+    const gptr<const double> rp = glb(D.rpol), ep = glb(D.epc);
+
+    // E rows: the electron's field at the sites (src/Potential.cpp:1531-1562,
+    // DampType 1: R^-3 (1 - exp(-p R^3)); 2: Reff^-3 inside the damping radius)
+    // plus the permanent-charge field; rows past the grid stay 0
+    for (int idx = threadIdx.x; idx < kP3Pts * np; idx += kP3Threads) {
+        const int p = idx / np, i = idx - p * np;
+        const long long g = g0 + p;
+        double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+        if (g < ng) {
+            const int ix = (int)(g % nx), iy = (int)((g / nx) % ny);
+            const int iz = (int)(g / ((long long)nx * ny));
+            const double rx = gx[ix] - rp[3 * i], ry = gy[iy] - rp[3 * i + 1],
+                         rz = gz[iz] - rp[3 * i + 2];
+            const double R2 = rx * rx + ry * ry + rz * rz;
+            double gij = 0.0;  // R = 0: the field's limit, 0
+            if (R2 > 0.0) {
+                const double ri = rsq64(R2), R = R2 * ri;
+                if (D.damp_type == 1) {
+                    gij = ri * ri * ri * (1.0 - exp(-D.pdamp * R * R2));
+                } else if (R < D.pdamp) {
+                    const double re = reff3(R, D.pdamp);
+                    gij = 1.0 / (re * re * re);
+                } else {
+                    gij = ri * ri * ri;
+                }
+            }
+            e0 = gij * rx + ep[3 * i];
+            e1 = gij * ry + ep[3 * i + 1];
+            e2 = gij * rz + ep[3 * i + 2];
+        }
+        double *row = Es + p * ldE + 3 * i;
+        row[0] = e0;
+        row[1] = e1;
+        row[2] = e2;
+    }
+    const int padw = n3p - n3;
+    for (int idx = threadIdx.x; idx < kP3Pts * padw; idx += kP3Threads) {
+        const int p = idx / padw;
+        Es[p * ldE + n3 + (idx - p * padw)] = 0.0;
+    }
+    __syncthreads();
+
+    // (E InvA) one 16-column block at a time, dotted with E at once
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const gptr<const double> A = glb(D.inva);
+    double s0[4] = {0.0, 0.0, 0.0, 0.0}, s1[4] = {0.0, 0.0, 0.0, 0.0};
+    const double *e0p = Es + l16 * ldE + lq, *e1p = Es + (16 + l16) * ldE + lq;
+    for (int cb = wave; cb < n3p / 16; cb += kP3Threads / 64) {
+        f64x4 a0 = {0.0, 0.0, 0.0, 0.0}, a1 = {0.0, 0.0, 0.0, 0.0};
+        const gptr<const double> bc = A + (long long)lq * n3p + 16 * cb + l16;
+#pragma unroll 8
+        for (int k0 = 0; k0 < n3p; k0 += 4) {
+            const double b = bc[(long long)k0 * n3p];  // InvA[k0 + lq][16 cb + l16]
+            a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e0p[k0], b, a0, 0, 0, 0);
+            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e1p[k0], b, a1, 0, 0, 0);
+        }
+        // a0[r] = (E InvA)[point lq + 4 r][column 16 cb + l16]
+        const int c = 16 * cb + l16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            s0[r] += a0[r] * Es[(lq + 4 * r) * ldE + c];
+            s1[r] += a1[r] * Es[(16 + lq + 4 * r) * ldE + c];
+        }
+    }
+    // sum over the 16 columns a lane group holds, then over the waves
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            s0[r] += __shfl_xor(s0[r], off, 64);
+            s1[r] += __shfl_xor(s1[r], off, 64);
+        }
+    if (l16 == 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            red[wave][lq + 4 * r] = s0[r];
+            red[wave][16 + lq + 4 * r] = s1[r];
+        }
+    __syncthreads();
+    if (threadIdx.x < kP3Pts) {
+        const long long g = g0 + threadIdx.x;
+        if (g < ng) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < kP3Threads / 64; ++w) s += red[w][threadIdx.x];
+            v[g] += -0.5 * s - D.vpolww;
+        }
+    }
+}
+
+}  // namespace
+
+void pol3_grid_launch(const Pol3Dev &D, int nx, int ny, int nz, const double *d_x,
+                      const double *d_y, const double *d_z, double *d_v, hipStream_t s)
+{
+    PC_REQUIRE(D.np > 0 && D.n3p == pol3_padded(D.np) && D.n3p <= kPol3MaxN,
+               PISCES_ERR_UNSUPPORTED, "too many polarizable sites");
+    PC_REQUIRE(D.damp_type == 1 || D.damp_type == 2, PISCES_ERR_ARG, "damping type");
+    static std::once_flag once;
+    std::call_once(once, [] {
+        static_cast<void>(hipFuncSetAttribute(reinterpret_cast<const void *>(pol3_grid_kernel),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              kP3Pts * (kPol3MaxN + 1) * (int)sizeof(double)));
+    });
+    const long long ng = (long long)nx * ny * nz;
+    const unsigned blocks = (unsigned)((ng + kP3Pts - 1) / kP3Pts);
+    const size_t lds = (size_t)kP3Pts * (D.n3p + 1) * sizeof(double);
+    hipLaunchKernelGGL(pol3_grid_kernel, dim3(blocks), dim3(kP3Threads), lds, s, D, nx, ny, nz,
+                       d_x, d_y, d_z, d_v);
+    PC_HIP(hipGetLastError());
+}
+
+}  // namespace pisces

@@ -53,6 +53,7 @@ def lib():
         for name in ("pisces_grid", "pisces_set_potential", "pisces_get_potential",
                      "pisces_apply_h", "pisces_diag", "pisces_piab", "pisces_solve",
                      "pisces_build_potential", "pisces_build_cluster_potential",
+                     "pisces_build_potential_scp",
                      "pisces_apply_h_device", "pisces_apply_h_ex", "pisces_time_hx", "pisces_time_precond",
                      "pisces_solve_block"):
             getattr(L, name).restype = ctypes.c_int
@@ -156,6 +157,28 @@ class Context:
         check(lib().pisces_build_potential(
             self.h, _pi(f), _p(pp), s.size // 3, _p(s), q.size, _p(q), _pi(iq), idd.size, _p(d),
             _pi(idd), nion, _p(ix), _p(iqq), ctypes.c_double(ion_r)), "pisces_build_potential")
+
+    def build_potential_scp(self, flags, potpara, sites, q, iq, alpha, ip, epc, vpol_ww,
+                            ions=None, ion_r=2.9):
+        """Potential with self-consistent polarizable sites (flags[3] == 3)."""
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        pp = np.ascontiguousarray(potpara, dtype=np.float64)
+        s = np.ascontiguousarray(sites, dtype=np.float64).reshape(-1)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        iq = np.ascontiguousarray(iq, dtype=np.int32)
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        ip = np.ascontiguousarray(ip, dtype=np.int32)
+        epc = np.ascontiguousarray(epc, dtype=np.float64).reshape(-1)
+        if ions is None:
+            ix, iqq, nion = np.zeros(3), np.zeros(1), 0
+        else:
+            ix = np.ascontiguousarray(ions[0], dtype=np.float64).reshape(-1)
+            iqq = np.ascontiguousarray(ions[1], dtype=np.float64).reshape(-1)
+            nion = iqq.size
+        check(lib().pisces_build_potential_scp(
+            self.h, _pi(f), _p(pp), s.size // 3, _p(s), q.size, _p(q), _pi(iq), ip.size,
+            _p(alpha), _pi(ip), _p(epc), ctypes.c_double(vpol_ww), nion, _p(ix), _p(iqq),
+            ctypes.c_double(ion_r)), "pisces_build_potential_scp")
 
     def build_cluster_potential(self, xyz_ang, flags=(3, 2, 0, 1, 0, 0), potpara=None,
                                 ions=None, ion_r=2.9):

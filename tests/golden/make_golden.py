@@ -94,6 +94,43 @@ def golden_potential(tag, nw, radius, seed, box):
     np.savez_compressed(os.path.join(HERE, "potential_%s.npz" % tag), **out)
 
 
+# PolType 3 (self-consistent polarizable sites, InvA): point dipoles ignored,
+# the polarizable sites O,H1,H2 with their permanent-charge field and
+# VpolWaterWater = the water model's polarization energy
+POL3_FLAG_SETS = {
+    "scp": ((3, 2, 0, 3, 0, 0), None),
+    "scp_gaussdamp": ((3, 1, 0, 3, 0, 0), None),
+    "scp_gto12": ((2, 2, 0, 3, 0, 0), None),
+}
+
+
+def golden_potential_pol3(tag, nw, radius, seed, box):
+    xyz, _ = workload.water_cluster(nw, radius, seed)
+    cl = oracle.Cluster(xyz)
+    rng = np.random.default_rng(seed + 17)
+    pts = sample_points(cl.sites, rng, box)
+    # far points: the field-free limit of Vpol
+    far = rng.normal(size=(8, 3))
+    far *= (1.0e4 / np.linalg.norm(far, axis=1))[:, None]
+    pts = np.concatenate([pts, far])
+    alphas = np.tile([1.22 * ANGS2BOHR ** 3, 0.28 * ANGS2BOHR ** 3, 0.28 * ANGS2BOHR ** 3], nw)
+    ips = np.array([4 * w + s for w in range(nw) for s in range(3)], dtype=np.float64)
+    out = dict(xyz=xyz, pts=pts, sites=cl.sites, q=cl.q, iq=cl.iq, alphas=alphas, ips=ips,
+               epc=cl.epc, e_pol=cl.e_pol, mu=cl.mu)
+    for name, (flags, over) in POL3_FLAG_SETS.items():
+        pp = oracle.default_potpara()
+        if over:
+            for k, v in over.items():
+                pp[k] = v
+        recs = [flags, pp, cl.sites, cl.q, cl.iq, cl.d, cl.id, alphas, ips, cl.epc,
+                [cl.e_pol], pts]
+        (e,) = run_ref("potential", recs)
+        out["flags_" + name] = np.array(flags)
+        out["potpara_" + name] = pp
+        out["E_" + name] = e.reshape(-1, 5)
+    np.savez_compressed(os.path.join(HERE, "potential_pol3_%s.npz" % tag), **out)
+
+
 def golden_davidson(tag, n, L, nw, radius, seed, params, store_vecs=True):
     xyz, _ = workload.water_cluster(nw, radius, seed)
     cl = oracle.Cluster(xyz)
@@ -120,6 +157,8 @@ def main():
     golden_tkin()
     golden_potential("c1", 6, 3.5, 1000, 36.0)
     golden_potential("c2", 24, 6.0, 2000, 40.0)
+    golden_potential_pol3("c1", 6, 3.5, 1000, 36.0)
+    golden_potential_pol3("c2", 24, 6.0, 2000, 40.0)
     golden_davidson("g10_r1", (10, 10, 10), (20.0, 20.0, 20.0), 6, 3.5, 1000, (1, 20, 100, 9, 1))
     golden_davidson("g12_r4", (12, 11, 10), (22.0, 21.0, 20.0), 6, 3.5, 1001, (4, 16, 100, 8, 1))
     golden_davidson("g10_r3_small", (10, 10, 10), (20.0, 20.0, 20.0), 6, 3.5, 1002, (3, 6, 200, 8, 1))

@@ -1,0 +1,129 @@
+"""Self-consistent polarizable sites (PolType 3) on the GPU against the oracle,
+whose PolType 3 restatement is pinned to the reference's Potential::Evaluate by
+tests/test_oracle_pol3.py."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+POL3 = ["scp", "scp_gaussdamp", "scp_gto12"]
+
+
+@pytest.mark.parametrize("flagset", POL3)
+def test_build_potential_scp_c1(pisces, oracle_mod, flagset):
+    """(H2O)6 on a 32^3 grid, the whole grid."""
+    f = golden("potential_pol3_c1.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    n, L = (32, 32, 32), (36.0, 36.0, 36.0)
+    g = oracle_mod.Grid(n, L)
+    flags, pp = f["flags_" + flagset], f["potpara_" + flagset]
+    Vo = cl.potential_grid_pol3(g, flags, pp)
+    alpha, ip, epc, e_pol = cl.pol3_lists()
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential_scp(flags, pp, cl.sites, cl.q, cl.iq, alpha, ip, epc, e_pol)
+        V = ctx.get_potential()
+    assert np.all(np.isfinite(V))
+    assert np.abs(V - Vo).max() <= 1e-12 * np.abs(Vo).max()
+
+
+def test_build_potential_scp_c2(pisces, oracle_mod):
+    """(H2O)24 on a 64^3 grid (216 polarizable dimensions), against the oracle
+    at 4096 grid points."""
+    f = golden("potential_pol3_c2.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    n, L = (64, 64, 64), (40.0, 40.0, 40.0)
+    g = oracle_mod.Grid(n, L)
+    flags, pp = f["flags_scp"], f["potpara_scp"]
+    alpha, ip, epc, e_pol = cl.pol3_lists()
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential_scp(flags, pp, cl.sites, cl.q, cl.iq, alpha, ip, epc, e_pol)
+        V = ctx.get_potential()
+    rng = np.random.default_rng(3)
+    idx = rng.choice(g.ngp, size=4096, replace=False)
+    i, j, k = idx % 64, (idx // 64) % 64, idx // 4096
+    pts = np.stack([g.x[0][i], g.x[1][j], g.x[2][k]], axis=1)
+    Vo, _ = cl.potential_points_pol3(pts, flags, pp)
+    assert np.abs(V[idx] - Vo).max() <= 1e-12 * np.abs(Vo).max()
+
+
+def test_cluster_potential_poltype3(pisces, oracle_mod):
+    """Water model on the GPU (dipole SCF, Epc, VpolWaterWater) + PolType 3
+    potential, against the oracle's water model and potential."""
+    f = golden("potential_pol3_c1.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    n, L = (32, 32, 32), (36.0, 36.0, 36.0)
+    g = oracle_mod.Grid(n, L)
+    flags = (3, 2, 0, 3, 0, 0)
+    Vo = cl.potential_grid_pol3(g, flags)
+    with pisces.Context(n, L) as ctx:
+        sites, mu, rc = ctx.build_cluster_potential(f["xyz"], flags=flags)
+        V = ctx.get_potential()
+    assert rc == 0
+    np.testing.assert_allclose(sites, cl.sites, rtol=0, atol=1e-12)
+    # two converged SCFs (stop at a mean change < 1e-12) feed Epc and VpolWW
+    assert np.abs(V - Vo).max() <= 1e-9 * np.abs(Vo).max()
+
+
+def test_polarizable_site_on_a_grid_point(pisces, oracle_mod):
+    """A polarizable site exactly on a grid point: DampType 2 gives the field
+    0 there (Reff is finite), as in the reference."""
+    f = golden("potential_pol3_c1.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    n, L = (33, 33, 33), (36.0, 36.0, 36.0)
+    g = oracle_mod.Grid(n, L)
+    alpha, ip, epc, e_pol = cl.pol3_lists()
+    mid = np.array([g.x[a][n[a] // 2] for a in range(3)])
+    sites = np.asarray(cl.sites, dtype=np.float64).reshape(-1, 3)
+    sites = (sites - sites[ip[0]]) + mid
+    assert np.abs(sites[ip[0]] - mid).max() == 0.0
+    flags, pp = f["flags_scp"], f["potpara_scp"]
+    pts = np.stack(np.meshgrid(g.x[0], g.x[1], g.x[2], indexing="ij"), axis=-1)
+    pts = pts.transpose(2, 1, 0, 3).reshape(-1, 3)
+    c2 = oracle_mod.Cluster(f["xyz"])
+    c2.sites = sites
+    Vo, _ = c2.potential_points_pol3(pts, flags, pp, pol=(alpha, ip, epc, e_pol))
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential_scp(flags, pp, sites, cl.q, cl.iq, alpha, ip, epc, e_pol)
+        V = ctx.get_potential()
+    assert np.all(np.isfinite(V))
+    assert np.abs(V - Vo).max() <= 1e-12 * np.abs(Vo).max()
+
+
+def test_scp_argument_errors(pisces, oracle_mod):
+    f = golden("potential_pol3_c1.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    alpha, ip, epc, e_pol = cl.pol3_lists()
+    pp = oracle_mod.default_potpara()
+    with pisces.Context((16, 16, 16), (36.0, 36.0, 36.0)) as ctx:
+        with pytest.raises(pisces.PiscesError):  # flags[3] must be 3
+            ctx.build_potential_scp((3, 2, 0, 1, 0, 0), pp, cl.sites, cl.q, cl.iq, alpha, ip,
+                                    epc, e_pol)
+        with pytest.raises(pisces.PiscesError):  # PolType 3 without the site lists
+            ctx.build_potential((3, 2, 0, 3, 0, 0), pp, cl.sites, cl.q, cl.iq, cl.d, cl.id)
+        bad = np.array(ip)
+        bad[0] = 10 ** 6
+        with pytest.raises(pisces.PiscesError):
+            ctx.build_potential_scp((3, 2, 0, 3, 0, 0), pp, cl.sites, cl.q, cl.iq, alpha, bad,
+                                    epc, e_pol)
+
+
+def test_solve_on_poltype3_potential(pisces, oracle_mod):
+    """FP64 eigensolve on the GPU-built PolType 3 potential against the
+    oracle's Davidson on the oracle's potential (same grid, same start)."""
+    f = golden("potential_pol3_c1.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    n, L = (24, 24, 24), (36.0, 36.0, 36.0)
+    g = oracle_mod.Grid(n, L)
+    flags = (3, 2, 0, 3, 0, 0)
+    Vo = cl.potential_grid_pol3(g, flags)
+    start = g.piab().reshape(1, -1)
+    ro = g.solve(Vo, 1, 20, 100, 9, 1, start)
+    alpha, ip, epc, e_pol = cl.pol3_lists()
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential_scp(flags, oracle_mod.default_potpara(), cl.sites, cl.q, cl.iq,
+                                alpha, ip, epc, e_pol)
+        r = ctx.solve(1, start, maxsub=20, maxmacro=100, ptol=9, corrflag=1)
+    assert ro["nconv"] == 1 and r["nconv"] == 1
+    assert abs(r["evals"][0] - ro["evals"][0]) <= 1e-8
