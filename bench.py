@@ -41,6 +41,7 @@ SEED = 2000
 NROOTS = 4
 BLOCK = 4        # block engine: block and subspace of the served solver's 4-state class
 MAXSUB = 28
+FP64_MATRIX_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix
 REF_MAXSUB = 32  # the reference algorithm's subspace (cpu_baseline's matvec count)
 PTOL = 7
 DE_TOL = 1e-6
@@ -249,6 +250,9 @@ def main():
             hx[name] = {"ms": round(ms, 5), "GBs_algorithmic": round(by / (ms * 1e-3) / 1e9, 1),
                         "TFLOPs": round(fl / (ms * 1e-3) / 1e12, 2)}
 
+    # ---- potential with self-consistent polarizable sites (PolType 3)
+    pol3 = None if a.served_only else pol3_measure(P, workload)
+
     cpu = None
     if not a.no_cpu and not a.served_only:
         cpu = cpu_baseline(P, geoms[0], a.cpu_seconds)
@@ -286,6 +290,7 @@ def main():
         "lanes": 2,
         "autotuned": P.tuning(),
         "hx_standalone_64cube_x8": hx,
+        "pol3_potential": pol3,
         "cpu_baseline": cpu,
     }
     if a.served_only:
@@ -472,6 +477,41 @@ def measured_traffic(family, algorithmic_bytes_per_launch):
     except (OSError, KeyError, ValueError):
         return None, None
     return round(ratio * algorithmic_bytes_per_launch), round(ratio, 3)
+
+
+def pol3_measure(P, workload):
+    """Grid build of the potential with self-consistent polarizable sites
+    (PolType 3) for (H2O)24 on 64^3 (216 polarizable dimensions) and (H2O)45 on
+    96^3 (405): the FP64 matrix-core kernel's time from HIP events around its
+    launch, the flops it executes (the upper block triangle of the symmetric
+    InvA, padded to 16) and the fraction of the FP64 matrix peak."""
+    out = {}
+    flags = (3, 2, 0, 3, 0, 0)
+    for tag, nw, radius, seed, n, L in (("h2o24_64cube", 24, 6.0, 2000, 64, 40.0),
+                                        ("h2o45_96cube", 45, 8.0, 3000, 96, 48.0)):
+        xyz, _ = workload.water_cluster(nw, radius, seed)
+        with P.Context((n,) * 3, (L,) * 3) as ctx:
+            ctx.build_cluster_potential(xyz, flags=flags)  # warm-up
+            P.prof_enable(True)
+            P.prof_reset()
+            for _ in range(5):
+                ctx.build_cluster_potential(xyz, flags=flags)
+            fam = P.prof_read()
+            P.prof_enable(False)
+        ms, nl, _ = fam["pol_scp"]
+        t = ms / nl * 1e-3
+        n3 = 9 * nw
+        ncb = (n3 + 15) // 16
+        ng = n ** 3
+        wgs = (ng + 31) // 32
+        fl_exec = 16384.0 * ncb * (ncb + 1) / 2 * wgs  # MFMA flops of the launch
+        fl_ref = 2.0 * n3 * n3 * ng                     # the reference's dsymv per point
+        out[tag] = {"n_pol_dims": n3, "grid": n, "ms_kernel": round(1e3 * t, 4),
+                    "TFLOPs_executed": round(fl_exec / t / 1e12, 2),
+                    "TFLOPs_dsymv_equivalent": round(fl_ref / t / 1e12, 2),
+                    "peak_TFLOPs": FP64_MATRIX_PEAK_TFLOPS,
+                    "frac_executed": round(fl_exec / t / 1e12 / FP64_MATRIX_PEAK_TFLOPS, 4)}
+    return out
 
 
 def cpu_baseline(P, xyz, seconds):

@@ -9,6 +9,7 @@
 
 #include "context.hpp"
 #include "fdm.hpp"
+#include "prof.hpp"
 
 using namespace pisces;
 
@@ -242,6 +243,7 @@ int build_potential_impl(pisces_ctx *c, const int flags[6], const double *potpar
             upload(dinv, inv.data(), sizeof(double) * inv.size(), c->stream);
             Pol3Dev Q{np, n3p, P.damp_type, 0, P.pdamp, vpol_ww, drp.as<double>(),
                       depc.as<double>(), dinv.as<double>()};
+            ProfScope ps(PK_POL3, c->stream, 0.0);
             pol3_grid_launch(Q, c->n[0], c->n[1], c->n[2], c->dx[0].as<double>(),
                              c->dx[1].as<double>(), c->dx[2].as<double>(), c->dV64.as<double>(),
                              c->stream);

@@ -11,6 +11,8 @@
 // blocks of InvA, streams them from L2 (InvA is shared by all workgroups: at
 // most 2.6 MB) as the B operand of v_mfma_f64_16x16x4 against both 16-point row
 // blocks of E, and folds each 16 x 16 result into the per-point dot at once.
+// InvA is symmetric, so only its upper block triangle is multiplied: (3np)^2
+// (1 + 16 / 3np) flops per point.
 #include "common.hpp"
 #include "gmem.hpp"
 
@@ -88,27 +90,41 @@ pol3_grid_kernel(Pol3Dev D, int nx, int ny, int nz, const double *__restrict__ g
     }
     __syncthreads();
 
-    // (E InvA) one 16-column block at a time, dotted with E at once
+    // E^T InvA E over the upper block triangle of the symmetric InvA: column
+    // block cb takes rows k < 16 cb twice and its diagonal 16 x 16 tile once.
+    // Block cb costs cb + 1 row blocks, so the waves take the blocks of each
+    // group of four in alternating order to even out their work.
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const gptr<const double> A = glb(D.inva);
     double s0[4] = {0.0, 0.0, 0.0, 0.0}, s1[4] = {0.0, 0.0, 0.0, 0.0};
     const double *e0p = Es + l16 * ldE + lq, *e1p = Es + (16 + l16) * ldE + lq;
-    for (int cb = wave; cb < n3p / 16; cb += kP3Threads / 64) {
-        f64x4 a0 = {0.0, 0.0, 0.0, 0.0}, a1 = {0.0, 0.0, 0.0, 0.0};
+    const int ncb = n3p / 16;
+    for (int j = 0; 4 * j < ncb; ++j) {
+        const int cb = (j & 1) ? 4 * j + 3 - wave : 4 * j + wave;
+        if (cb >= ncb) continue;
+        f64x4 o0 = {0.0, 0.0, 0.0, 0.0}, o1 = {0.0, 0.0, 0.0, 0.0};
+        f64x4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = {0.0, 0.0, 0.0, 0.0};
         const gptr<const double> bc = A + (long long)lq * n3p + 16 * cb + l16;
-#pragma unroll 8
-        for (int k0 = 0; k0 < n3p; k0 += 4) {
+        const int kd = 16 * cb;
+#pragma unroll 4
+        for (int k0 = 0; k0 < kd; k0 += 4) {
             const double b = bc[(long long)k0 * n3p];  // InvA[k0 + lq][16 cb + l16]
-            a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e0p[k0], b, a0, 0, 0, 0);
-            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e1p[k0], b, a1, 0, 0, 0);
+            o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e0p[k0], b, o0, 0, 0, 0);
+            o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e1p[k0], b, o1, 0, 0, 0);
         }
-        // a0[r] = (E InvA)[point lq + 4 r][column 16 cb + l16]
+#pragma unroll
+        for (int k0 = kd; k0 < kd + 16; k0 += 4) {
+            const double b = bc[(long long)k0 * n3p];
+            d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e0p[k0], b, d0, 0, 0, 0);
+            d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e1p[k0], b, d1, 0, 0, 0);
+        }
+        // o0[r], d0[r]: point lq + 4 r, column 16 cb + l16
         const int c = 16 * cb + l16;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            s0[r] += a0[r] * Es[(lq + 4 * r) * ldE + c];
-            s1[r] += a1[r] * Es[(16 + lq + 4 * r) * ldE + c];
+            s0[r] += (2.0 * o0[r] + d0[r]) * Es[(lq + 4 * r) * ldE + c];
+            s1[r] += (2.0 * o1[r] + d1[r]) * Es[(16 + lq + 4 * r) * ldE + c];
         }
     }
     // sum over the 16 columns a lane group holds, then over the waves

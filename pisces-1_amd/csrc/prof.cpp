@@ -56,7 +56,7 @@ void resolve()
 }
 
 const char *kNames[PK_COUNT] = {"hx_fp64", "hx_lo", "gram", "combine", "eig", "ritz_resid", "precond",
-                                "factors", "potential", "dipole_scf"};
+                                "factors", "potential", "dipole_scf", "pol_scp"};
 
 }  // namespace
 

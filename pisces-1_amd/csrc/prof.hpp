@@ -22,6 +22,7 @@ enum ProfKind {
     PK_CHOL,      // Cholesky / orthogonalisation factors (small)
     PK_POT,       // electron-water potential on the grid (+ diag H)
     PK_SCF,       // induced-dipole solve of the water cluster
+    PK_POL3,      // self-consistent polarization on the grid (PolType 3)
     PK_COUNT
 };
 

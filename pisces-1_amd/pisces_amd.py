@@ -336,7 +336,7 @@ def start_vectors(ctx, nroots, seed=0):
 
 
 # ---------------------------------------------------------------- launch timing
-PROF_KINDS = 10
+PROF_KINDS = 11
 
 
 def prof_enable(on=True):
