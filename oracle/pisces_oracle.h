@@ -78,8 +78,10 @@ int or_water_sites(int nw, const double *xyz_ang, double *sites, double *conf);
 int or_water_induced(int nw, const double *sites, double *mu, double *epc,
                      double *e_pol, int *iters);
 /* Pinned: the fixed point solves the dense linear system (1 - alpha T) mu =
- * alpha E_pc.  Parity unpinned: the damping forms and constants themselves
- * (restated from src/DPP.cpp:1798-1812, 2377-2410) and e_pol. */
+ * alpha E_pc; mu equals the reference's InvA (ComputeInvA, through the PolType
+ * 3 fixtures) times E_pc, which pins the Thole dipole tensor; e_pol equals
+ * -E_pc.InvA.E_pc / 2.  Parity unpinned: the charge-dipole damping of E_pc
+ * (restated from src/DPP.cpp:1798-1804). */
 
 /* ---------------------------------------------------------------- a5, a6 */
 /* The electron-water potential at npts points, for the site/charge/dipole/
