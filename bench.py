@@ -164,6 +164,8 @@ def main():
                     help="time only the served mixed-precision steps (profiling runs)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the one-lane roofline pass after a warm-up step (profiling runs)")
+    ap.add_argument("--pol3-only", action="store_true",
+                    help="only the PolType 3 potential builds (profiling runs)")
     ap.add_argument("--config", default="c2", choices=["c2", "c4", "c5"],
                     help="c2: the headline workload; c4: 512 uneven snapshots in one batch; "
                          "c5: open-loop Poisson-served MD chains")
@@ -186,6 +188,9 @@ def main():
     import workload
 
     prec = P.PRECISIONS[a.precision]
+    if a.pol3_only:
+        print(json.dumps({"pol3_potential": pol3_measure(P, workload)}), flush=True)
+        return
     if a.config == "c4":
         print(json.dumps(bench_c4(P, workload, prec, a)), flush=True)
         return
