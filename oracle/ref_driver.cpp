@@ -18,6 +18,12 @@
 //   ref_driver davidson  <in> <out>   in: [nx,ny,nz,Lx,Ly,Lz], V, [nroots,maxsub,
 //                                         maxmacro,ptol,corrflag], start vectors
 //                                     out: [nconv,n_mtx], evals, vectors
+//   ref_driver input     <file> <out> the reference's TSIN + GetInputParameters +
+//                                     ReadWatersGroup on an input file
+//                                     out: [runtype, nWater, DVRType, Sampling,
+//                                     DiagMethod, nStates, maxSub, maxIter, ptol,
+//                                     istartvec], PotFlag[6], PotPara[11],
+//                                     ngrid[3], gpara[3], water coordinates
 #include <mpi.h>
 
 #include <cmath>
@@ -31,6 +37,9 @@
 #include "Potential.h"
 #include "sine_dvr.h"
 #include "Davidson.h"
+#include "tsin.h"
+#include "Parameters.h"
+#include "GetInput.h"
 
 typedef std::vector<double> rec;
 
@@ -179,6 +188,27 @@ static int cmd_davidson(const char *in, const char *out)
     return 0;
 }
 
+static int cmd_input(const char *in, const char *out)
+{
+    TSIN Input;
+    Input.ReadFromFile(in, 0);
+    Parameters P;
+    GetInputParameters(P, Input);
+    dVec xyz;
+    int nw = ReadWatersGroup(Input, xyz);
+    if (P.nWater == 0) P.nWater = nw;
+    rec a = {(double)P.runtype, (double)P.nWater, (double)P.DVRType, (double)P.Sampling,
+             (double)P.DiagMethod, (double)P.nStates, (double)P.maxSub, (double)P.maxIter,
+             (double)P.ptol, (double)P.istartvec};
+    rec f(6), pp(11), ng(3), gp(3);
+    for (int k = 0; k < 6; ++k) f[k] = P.PotFlag[k];
+    for (int k = 0; k < 11; ++k) pp[k] = P.PotPara[k];
+    for (int k = 0; k < 3; ++k) { ng[k] = P.ngrid[k]; gp[k] = P.gpara[k]; }
+    rec w(xyz.begin(), xyz.begin() + 9 * nw);
+    write_records(out, {a, f, pp, ng, gp, w});
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     MPI_Init(&argc, &argv);
@@ -187,6 +217,7 @@ int main(int argc, char **argv)
         if (!strcmp(argv[1], "tkin")) rc = cmd_tkin(argv[2], argv[3]);
         else if (!strcmp(argv[1], "potential")) rc = cmd_potential(argv[2], argv[3]);
         else if (!strcmp(argv[1], "davidson")) rc = cmd_davidson(argv[2], argv[3]);
+        else if (!strcmp(argv[1], "input")) rc = cmd_input(argv[2], argv[3]);
     } else {
         fprintf(stderr, "usage: ref_driver {tkin|potential|davidson} <in> <out>\n");
     }

@@ -1,0 +1,351 @@
+// pisces_amd_run -- the reference's input file in, electron binding energies out.
+//
+//   pisces_amd_run <input> [--parse-only] [--precision fp64|fp32|bf16x2] [--device N]
+//
+// The drop-in for `pisces <input>` with runtype = 1 (src/pisces.cpp:112-130,
+// ClusterAnion::SinglePoint, src/ClusterAnion.cpp:92-94 and :296-320): read the
+// input groups, build the water model and the electron's potential on the GPU,
+// solve for the lowest nStates states and print the binding energies in the
+// reference's format.
+//
+// Input reading restates TSIN (src/tsin.cpp): lines beginning with '#' and
+// empty lines are dropped; a group is the lines after the first line that
+// contains "Begin" + the group name (cut to 15 characters) up to the first line
+// that contains "End" + that name; a key is found by substring search inside
+// its group and read as "<token> = value" (arrays: "key = a, b, c,").  Missing
+// scalar keys take the defaults of GetInputParameters (src/GetInput.cpp:23-186);
+// a missing array key is an error, as in the reference.  The Waters group holds
+// O, H, H lines in Angstrom (ReadWatersGroup, src/GetInput.cpp:256-320).
+//
+// Differences, stated where they occur: the second random start vectors come
+// from a seeded generator (the reference calls the unseeded rand()); the neutral
+// cluster energy E0 (water-water electrostatics, polarization and dispersion of
+// the water model) is not part of this library, so the energy block reports the
+// binding energy alone.
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "pisces_hip.h"
+
+namespace {
+
+constexpr int kSigSigns = 15;   // significant characters of a group name (src/tsin.h)
+constexpr int kLineLength = 256;
+constexpr double kAu2Mev = 27211.38386;  // src/constants.h
+
+struct InputError {
+    std::string msg;
+};
+
+class InputFile {
+public:
+    explicit InputFile(const char *path)
+    {
+        FILE *f = std::fopen(path, "r");
+        if (!f) throw InputError{std::string("cannot open ") + path};
+        char buf[kLineLength];
+        while (std::fgets(buf, kLineLength, f)) {
+            const size_t l = std::strlen(buf);
+            if (buf[0] == '#' || l == 1) continue;
+            lines_.emplace_back(buf);
+        }
+        std::fclose(f);
+    }
+
+    // lines strictly between Begin<group> and End<group>; -1 if no Begin line
+    int find_group(const char *group, int *end) const
+    {
+        const std::string b = "Begin" + std::string(group).substr(0, kSigSigns);
+        const std::string e = "End" + std::string(group).substr(0, kSigSigns);
+        int i = 0;
+        while (i < (int)lines_.size() && lines_[i].find(b) == std::string::npos) ++i;
+        if (i == (int)lines_.size()) return -1;
+        int j = i + 1;
+        while (j < (int)lines_.size() && lines_[j].find(e) == std::string::npos) ++j;
+        *end = j;
+        return i + 1;
+    }
+
+    std::vector<std::string> group(const char *group) const
+    {
+        int end = 0;
+        const int b = find_group(group, &end);
+        if (b < 0) throw InputError{std::string("group ") + group + " not found"};
+        return std::vector<std::string>(lines_.begin() + b, lines_.begin() + end);
+    }
+
+    // the text from the key's first occurrence inside the group, or nullptr
+    const char *key_text(const char *group, const char *key, int *line) const
+    {
+        int end = 0;
+        const int b = find_group(group, &end);
+        if (b < 0) return nullptr;
+        for (int i = b; i < end; ++i) {
+            const size_t p = lines_[i].find(key);
+            if (p != std::string::npos) {
+                *line = i;
+                return lines_[i].c_str() + p;
+            }
+        }
+        return nullptr;
+    }
+
+    int get_int(const char *group, const char *key, int def) const
+    {
+        int li = 0;
+        const char *s = key_text(group, key, &li);
+        if (!s) return def;
+        int v = 0;
+        if (std::sscanf(s, "%*s = %i", &v) != 1) throw bad_value(group, key, li);
+        return v;
+    }
+
+    double get_double(const char *group, const char *key, double def) const
+    {
+        int li = 0;
+        const char *s = key_text(group, key, &li);
+        if (!s) return def;
+        double v = 0;
+        if (std::sscanf(s, "%*s = %lf", &v) != 1) throw bad_value(group, key, li);
+        return v;
+    }
+
+    // "key = v0, v1, ..., " (src/tsin.cpp:400-546)
+    template <typename T>
+    void get_array(const char *group, const char *key, T *fill, int n) const
+    {
+        int li = 0;
+        const char *s = key_text(group, key, &li);
+        if (!s) throw InputError{std::string("key ") + key + " not found in group " + group};
+        const char *fmt0 = std::is_same<T, int>::value ? "%*s = %i," : "%*s = %lf,";
+        const char *fmtk = std::is_same<T, int>::value ? ", %i," : ", %lf,";
+        if (std::sscanf(s, fmt0, &fill[0]) != 1) throw bad_value(group, key, li);
+        for (int k = 1; k < n; ++k) {
+            ++s;
+            while (*s != ',') {
+                ++s;
+                if (*s == '\0') throw bad_value(group, key, li);
+            }
+            if (std::sscanf(s, fmtk, &fill[k]) != 1) throw bad_value(group, key, li);
+        }
+    }
+
+private:
+    InputError bad_value(const char *group, const char *key, int li) const
+    {
+        return InputError{std::string("cannot read ") + key + " in group " + group + " from: " +
+                          lines_[li]};
+    }
+    std::vector<std::string> lines_;
+};
+
+// GetInputParameters (src/GetInput.cpp:23-186): the keys of a single point
+struct Params {
+    int runtype = 0, nElectron = 1, nWater = 0, KTFlag = 0, CenterFlag = 0;
+    int PotFlag[6] = {0, 0, 0, 0, 0, 0};
+    double PotPara[11] = {0};
+    int DVRType = 0, Sampling = 1, ngrid[3] = {0, 0, 0};
+    double gpara[3] = {0, 0, 0};
+    int DiagMethod = 2, nStates = 1, maxSub = 20, maxIter = 100, ptol = 5, istartvec = 1;
+};
+
+Params read_params(const InputFile &in)
+{
+    Params P;
+    P.runtype = in.get_int("Job", "runtype", 0);
+    P.nElectron = in.get_int("Job", "nElectron", 1);
+    P.nWater = in.get_int("WaterModel", "NoOfWaters", 0);
+    P.KTFlag = in.get_int("WaterModel", "KTCharges", 0);
+    P.CenterFlag = in.get_int("WaterModel", "CoMOrigin", 0);
+    if (P.nElectron > 0) {
+        const char *g = "ElectronPotential";
+        P.PotFlag[0] = in.get_int(g, "Potential", 3);
+        P.PotFlag[1] = in.get_int(g, "DampType", 2);
+        P.PotFlag[2] = in.get_int(g, "SigmaOFlag", 0);
+        P.PotFlag[3] = in.get_int(g, "Polarization", 1);
+        P.PotFlag[4] = in.get_int(g, "AdiabaticPolPot", 0);
+        P.PotFlag[5] = in.get_int(g, "InternalParam", 0);
+        if (P.PotFlag[0] < 1 || P.PotFlag[0] > 4)
+            throw InputError{"Potential " + std::to_string(P.PotFlag[0]) +
+                             ": only the DPP water potentials 1-4 are supported"};
+        P.PotPara[0] = in.get_double(g, "Alpha", 9.745);
+        P.PotPara[1] = in.get_double(g, "PolDamping", 2.536);
+        P.PotPara[2] = in.get_double(g, "RepCoreScale", 4.4837);
+        P.PotPara[3] = in.get_double(g, "CoulombDamping", 1.7724);
+        P.PotPara[4] = in.get_double(g, "DipoleDamping", 1.7724);
+        P.PotPara[5] = in.get_double(g, "ORepCoreScale", 4.4837);
+        P.PotPara[6] = in.get_double(g, "EpsDrude", 0.5 / std::sqrt(P.PotPara[0]));
+        P.PotPara[10] = in.get_double(g, "Rtol", 20.0);
+        if (P.PotFlag[5] == 1) {
+            // the internal parameter sets per polarization model (src/GetInput.cpp:86-116)
+            P.PotFlag[0] = 3;
+            P.PotFlag[1] = 2;
+            static const double pol[4] = {5.011, 5.1543, 5.005, 4.84};
+            static const double rep[4] = {6.6396, 6.6717, 6.6382, 6.602};
+            const int m = P.PotFlag[3];
+            if (m >= 1 && m <= 4) {
+                P.PotPara[1] = pol[m - 1];
+                P.PotPara[4] = P.PotPara[1];
+                P.PotPara[2] = rep[m - 1];
+            }
+        }
+    }
+    P.DVRType = in.get_int("GridDef", "DVRType", 0);
+    P.Sampling = in.get_int("GridDef", "Sampling", 1);
+    in.get_array("GridDef", "NoOfGridPoints", P.ngrid, 3);
+    in.get_array("GridDef", P.DVRType == 1 ? "Omega0" : "Length", P.gpara, 3);
+    P.DiagMethod = in.get_int("Diag", "Method", 2);
+    P.nStates = in.get_int("Diag", "nStates", 1);
+    P.maxSub = in.get_int("Diag", "maxSubspace", 20);
+    P.maxIter = in.get_int("Diag", "maxIter", 100);
+    P.ptol = in.get_int("Diag", "pTol", 5);
+    P.istartvec = in.get_int("Diag", "StartVector", 1);
+    return P;
+}
+
+// ReadWatersGroup (src/GetInput.cpp:256-320): O, H, H per water, Angstrom
+std::vector<double> read_waters(const InputFile &in)
+{
+    const std::vector<std::string> L = in.group("Waters");
+    if (L.size() % 3 != 0) throw InputError{"Waters: the number of lines is not a multiple of 3"};
+    std::vector<double> xyz;
+    for (size_t i = 0; i < L.size(); ++i) {
+        char sym[64] = {0};
+        double x, y, z;
+        if (std::sscanf(L[i].c_str(), " %63s %lf %lf %lf", sym, &x, &y, &z) != 4)
+            throw InputError{"Waters: cannot read line " + std::to_string(i) + ": " + L[i]};
+        const bool oxygen = i % 3 == 0;
+        if (oxygen ? (sym[0] != '8' && sym[0] != 'O') : (sym[0] != '1' && sym[0] != 'H'))
+            throw InputError{"Waters line " + std::to_string(i) + (oxygen ? ": O or 8" : ": H or 1") +
+                             " is expected"};
+        xyz.push_back(x);
+        xyz.push_back(y);
+        xyz.push_back(z);
+    }
+    return xyz;
+}
+
+void print_json(const Params &P, const std::vector<double> &xyz)
+{
+    std::printf("{\"runtype\": %d, \"nWater\": %d, \"KTCharges\": %d, \"CoMOrigin\": %d, ", P.runtype,
+                P.nWater, P.KTFlag, P.CenterFlag);
+    std::printf("\"PotFlag\": [%d, %d, %d, %d, %d, %d], \"PotPara\": [", P.PotFlag[0], P.PotFlag[1],
+                P.PotFlag[2], P.PotFlag[3], P.PotFlag[4], P.PotFlag[5]);
+    for (int k = 0; k < 11; ++k) std::printf("%s%.17g", k ? ", " : "", P.PotPara[k]);
+    std::printf("], \"DVRType\": %d, \"Sampling\": %d, \"ngrid\": [%d, %d, %d], ", P.DVRType,
+                P.Sampling, P.ngrid[0], P.ngrid[1], P.ngrid[2]);
+    std::printf("\"gpara\": [%.17g, %.17g, %.17g], ", P.gpara[0], P.gpara[1], P.gpara[2]);
+    std::printf("\"DiagMethod\": %d, \"nStates\": %d, \"maxSub\": %d, \"maxIter\": %d, \"ptol\": %d, "
+                "\"StartVector\": %d, \"waters\": [",
+                P.DiagMethod, P.nStates, P.maxSub, P.maxIter, P.ptol, P.istartvec);
+    for (size_t k = 0; k < xyz.size(); ++k) std::printf("%s%.17g", k ? ", " : "", xyz[k]);
+    std::printf("]}\n");
+}
+
+int fail(const std::string &msg)
+{
+    std::fprintf(stderr, "pisces_amd_run: %s\n", msg.c_str());
+    return 1;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    const char *path = nullptr;
+    bool parse_only = false;
+    int precision = PISCES_FP64, device = 0;
+    for (int a = 1; a < argc; ++a) {
+        if (!std::strcmp(argv[a], "--parse-only")) parse_only = true;
+        else if (!std::strcmp(argv[a], "--device") && a + 1 < argc) device = std::atoi(argv[++a]);
+        else if (!std::strcmp(argv[a], "--precision") && a + 1 < argc) {
+            const char *p = argv[++a];
+            precision = !std::strcmp(p, "fp32") ? PISCES_FP32
+                        : !std::strcmp(p, "bf16x2") ? PISCES_BF16X2 : PISCES_FP64;
+        } else path = argv[a];
+    }
+    if (!path) {
+        std::printf("usage: pisces_amd_run input-file [--parse-only] [--precision fp64|fp32|bf16x2]\n");
+        return 1;
+    }
+    Params P;
+    std::vector<double> xyz;
+    try {
+        InputFile in(path);
+        P = read_params(in);
+        xyz = read_waters(in);
+        const int nw = (int)(xyz.size() / 9);
+        if (P.nWater == 0) P.nWater = nw;
+        else if (nw != P.nWater)
+            throw InputError{"inconsistent no of waters in Waters and WaterModel groups"};
+    } catch (const InputError &e) {
+        return fail(e.msg);
+    }
+    if (parse_only) {
+        print_json(P, xyz);
+        return 0;
+    }
+    if (P.runtype != 1) return fail("runtype " + std::to_string(P.runtype) + ": only single points (1) run here");
+    if (P.nElectron != 1) return fail("nElectron must be 1");
+    if (P.DVRType != 0 || P.Sampling != 1) return fail("only the sine DVR (DVRType 0) with Sampling 1");
+    if (P.KTFlag != 0 || P.CenterFlag != 0) return fail("KTCharges and CoMOrigin must be 0");
+    // DVR::Diagonalize (src/DVR.cpp:462-471): Method 2 Davidson, 3 Jacobi-Davidson, 4/5 none
+    int corrflag;
+    switch (P.DiagMethod) {
+    case 2: corrflag = 1; break;
+    case 3: corrflag = 2; break;
+    case 4:
+    case 5: corrflag = 0; break;
+    default: return fail("Diag Method " + std::to_string(P.DiagMethod) + " is not supported (2-5 are)");
+    }
+    if (P.istartvec != 1) return fail("StartVector 1 (particle in a box) is the one supported here");
+
+    pisces_ctx *ctx = nullptr;
+    const double L[3] = {P.gpara[0], P.gpara[1], P.gpara[2]};
+    int rc = pisces_create(&ctx, P.ngrid, L, device);
+    if (rc) return fail(std::string("pisces_create: ") + pisces_last_error());
+    rc = pisces_build_cluster_potential(ctx, P.nWater, xyz.data(), P.PotFlag, P.PotPara, 0, nullptr,
+                                        nullptr, 2.9, nullptr, nullptr);
+    if (rc < 0) {
+        pisces_destroy(ctx);
+        return fail(std::string("potential: ") + pisces_last_error());
+    }
+    if (rc > 0) std::printf("Warning: the induced dipoles of the water model did not converge\n");
+    const long long ngp = (long long)P.ngrid[0] * P.ngrid[1] * P.ngrid[2];
+    std::vector<double> vecs((size_t)ngp * P.nStates), ev(P.nStates);
+    // start: the half-box particle-in-a-box function, then uniform [-1, 1) vectors
+    // (src/DVR.cpp:369-513)
+    pisces_piab(ctx, vecs.data());
+    std::mt19937_64 gen(12345);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    for (size_t k = (size_t)ngp; k < vecs.size(); ++k) vecs[k] = u(gen);
+    pisces_stats st{};
+    rc = pisces_solve(ctx, P.nStates, P.maxSub, P.maxIter, P.ptol, corrflag, precision, vecs.data(),
+                      ev.data(), &st);
+    pisces_destroy(ctx);
+    if (rc < 0) return fail(std::string("solve: ") + pisces_last_error());
+    const int nconv = rc;
+    std::printf("\nElectron binding energies:\n");
+    for (int s = 0; s < P.nStates; ++s)
+        std::printf("State %-3i  %12.8f Hartree =  %10.2f meV\n", s, ev[s], ev[s] * kAu2Mev);
+    if (nconv < P.nStates) {
+        std::printf("single-point:: Only %d States of %d converged.\n", nconv, P.nStates);
+        return 1;
+    }
+    std::printf("\n  ------------------------------------------  \n");
+    std::printf("  ***  Energies For The Current Geometry ***  \n");
+    std::printf("  ------------------------------------------  \n");
+    std::printf("       EBE      = %.10g meV\n", ev[0] * kAu2Mev);
+    std::printf("       (Eneutral and Etotal: the water model's own energy is not computed here)\n");
+    std::printf("  ------------------------------------------  \n");
+    std::printf("  matvecs %d, iterations %d, restarts %d, %.3f s\n", st.n_matvec, st.n_iter,
+                st.n_macro, st.seconds);
+    return 0;
+}

@@ -131,6 +131,136 @@ def golden_potential_pol3(tag, nw, radius, seed, box):
     np.savez_compressed(os.path.join(HERE, "potential_pol3_%s.npz" % tag), **out)
 
 
+def _waters_block(xyz):
+    lines = []
+    for w in np.asarray(xyz).reshape(-1, 9):
+        for a, sym in enumerate(("O", "H", "H")):
+            lines.append("%s  %.17g  %.17g  %.17g" % ((sym,) + tuple(w[3 * a:3 * a + 3])))
+    return "\n".join(lines)
+
+
+def input_files():
+    """Input files in the reference's format (src/tsin.h, src/GetInput.cpp),
+    written to tests/golden/inputs/."""
+    c1 = np.load(os.path.join(HERE, "davidson_c1.npz"))
+    xyz6 = c1["xyz"]
+    xyz3 = xyz6[:3]
+    files = {}
+    files["c1"] = """# (H2O)6 anion, 32^3 grid, one state (the reference Davidson fixture davidson_c1)
+BeginJob
+  runtype = 1,
+EndJob
+BeginWaterModel
+  NoOfWaters = 6,
+EndWaterModel
+BeginElectronPotential
+  Potential = 3,
+  Polarization = 1,
+EndElectronPotential
+BeginGridDef
+  NoOfGridPoints = 32, 32, 32,
+  Length = 36.0, 36.0, 36.0,
+EndGridDef
+BeginDiag
+  Method = 2,
+  nStates = 1,
+  maxSubspace = 20,
+  maxIter = 100,
+  pTol = 9,
+  StartVector = 1,
+EndDiag
+BeginWaters
+%s
+EndWaters
+""" % _waters_block(xyz6)
+    files["defaults"] = """BeginJob
+  runtype = 1,
+EndJob
+BeginGridDef
+  NoOfGridPoints = 24, 24, 24,
+  Length = 30.0, 30.0, 30.0,
+EndGridDef
+BeginWaters
+%s
+EndWaters
+""" % _waters_block(xyz3)
+    files["internal"] = """BeginJob
+  runtype = 1,
+EndJob
+BeginElectronPotential
+  Potential = 2,
+  DampType = 1,
+  Polarization = 3,
+  InternalParam = 1,
+  Alpha = 10.5,
+  CoulombDamping = 1.5,
+EndElectronPotential
+BeginGridDef
+  NoOfGridPoints = 20, 22, 24,
+  Length = 30.0, 31.5, 33.0,
+EndGridDef
+BeginDiag
+  Method = 3,
+  nStates = 2,
+  maxSubspace = 16,
+  pTol = 7,
+EndDiag
+BeginWaters
+%s
+EndWaters
+""" % _waters_block(xyz3)
+    blk = _waters_block(xyz3).replace("O  ", "8\t").replace("H  ", "1 ")
+    files["spacing"] = """# odd spacing, numeric atom symbols, a comment inside a group
+BeginJob
+      runtype   =  1 ,
+EndJob
+BeginWaterModel
+  Verbose = 3,
+EndWaterModel
+BeginElectronPotential
+  Verbose = 2,
+  SigmaOFlag = 1,
+  ORepCoreScale = 3.9,
+  AdiabaticPolPot = 1,
+EndElectronPotential
+BeginGridDef
+# comment
+  NoOfGridPoints = 16,18, 20
+  Length = 28.0,29.0,  30.25,
+EndGridDef
+BeginDiag
+  Method = 4,
+  maxIter = 50,
+EndDiag
+BeginWaters
+%s
+EndWaters
+""" % blk
+    return files
+
+
+def golden_inputs():
+    d = os.path.join(HERE, "inputs")
+    os.makedirs(d, exist_ok=True)
+    out = {}
+    for name, text in input_files().items():
+        path = os.path.join(d, name + ".inp")
+        with open(path, "w") as f:
+            f.write(text)
+        with tempfile.TemporaryDirectory() as td:
+            fout = os.path.join(td, "out.bin")
+            subprocess.run([oracle.REF_DRIVER, "input", path, fout], check=True,
+                           stdout=subprocess.DEVNULL, env=dict(os.environ, OMP_NUM_THREADS="1"))
+            a, f_, pp, ng, gp, w = oracle.read_records(fout)
+        out[name + "_scalars"] = a
+        out[name + "_potflag"] = f_
+        out[name + "_potpara"] = pp
+        out[name + "_ngrid"] = ng
+        out[name + "_gpara"] = gp
+        out[name + "_waters"] = w
+    np.savez_compressed(os.path.join(HERE, "input_parse.npz"), **out)
+
+
 def golden_davidson(tag, n, L, nw, radius, seed, params, store_vecs=True):
     xyz, _ = workload.water_cluster(nw, radius, seed)
     cl = oracle.Cluster(xyz)
@@ -159,6 +289,7 @@ def main():
     golden_potential("c2", 24, 6.0, 2000, 40.0)
     golden_potential_pol3("c1", 6, 3.5, 1000, 36.0)
     golden_potential_pol3("c2", 24, 6.0, 2000, 40.0)
+    golden_inputs()
     golden_davidson("g10_r1", (10, 10, 10), (20.0, 20.0, 20.0), 6, 3.5, 1000, (1, 20, 100, 9, 1))
     golden_davidson("g12_r4", (12, 11, 10), (22.0, 21.0, 20.0), 6, 3.5, 1001, (4, 16, 100, 8, 1))
     golden_davidson("g10_r3_small", (10, 10, 10), (20.0, 20.0, 20.0), 6, 3.5, 1002, (3, 6, 200, 8, 1))
