@@ -81,6 +81,21 @@ int pisces_tuning_gram(int *variant, int *rows);
  * Replaces DVR::SetupDVR for DVRType 0 (src/DVR.cpp:64-161, :524-651).
  * device: HIP device ordinal. */
 int pisces_create(pisces_ctx **ctx, const int n[3], const double L[3], int device);
+/* The same with the DVR type of DVR::SetupDVR (src/DVR.h:56-91, `type`):
+ * 0 the sine DVR (as pisces_create); 3 the FFT kinetic energy of DVRType 3
+ * (Tdiag, src/KE_diag.cpp; applied through FFTW by VectorFFT::apply,
+ * src/VectorFFT.hpp:32, src/VectorFFTW.cpp), on the same grid points.  The
+ * kinetic operator of type 3 is diagonal in the discrete Fourier basis of each
+ * axis; here it is applied as the equivalent real circulant matrix per axis
+ * through the same matrix-core axis contractions.  For non-cubic grids this
+ * follows the per-axis definition (the reference's 3-D index of KE_diag is
+ * only right for cubic grids, src/VectorFFTW.cpp:83).  diag(H) of type 3 takes
+ * the circulant's diagonal, the mean of the axis' Fourier kinetic energies.
+ * Type 3 needs even grid sizes: for odd n, Tdiag's wave numbers (a < n/2: a,
+ * else a - n) are not symmetric about 0 and the multiplier is not real on real
+ * vectors.  Other types and odd sizes: PISCES_ERR_UNSUPPORTED. */
+int pisces_create_ex(pisces_ctx **ctx, const int n[3], const double L[3], int device,
+                     int dvr_type);
 void pisces_destroy(pisces_ctx *ctx);
 
 /* Grid points (x[n0], y[n1], z[n2]) and kinetic matrices (n_a*n_a each,

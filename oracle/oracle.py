@@ -54,15 +54,31 @@ def sine_dvr(n, L, mass=1.0, packed=False):
     return x, T
 
 
-class Grid:
-    """Sine-DVR product grid (x fastest), as DVR::SetupDVR builds it."""
+def fft_kinetic(n, L, mass=1.0):
+    """DVRType 3 (Tdiag): grid, the circulant kinetic matrix and the Fourier
+    kinetic energies in FFT order."""
+    x = np.zeros(n)
+    T = np.zeros((n, n))
+    ek = np.zeros(n)
+    lib().or_fft_kinetic(n, ctypes.c_double(mass), ctypes.c_double(-0.5 * L),
+                         ctypes.c_double(0.5 * L), _p(x), _p(T), _p(ek))
+    return x, T, ek
 
-    def __init__(self, n, L):
+
+class Grid:
+    """Product grid (x fastest), as DVR::SetupDVR builds it: the sine DVR
+    (dvr_type 0) or the FFT kinetic energy on the same points (3)."""
+
+    def __init__(self, n, L, dvr_type=0):
         self.n = tuple(int(v) for v in n)
         self.L = tuple(float(v) for v in L)
+        self.dvr_type = dvr_type
         self.x, self.T = [], []
         for a in range(3):
-            x, T = sine_dvr(self.n[a], self.L[a])
+            if dvr_type == 3:
+                x, T, _ = fft_kinetic(self.n[a], self.L[a])
+            else:
+                x, T = sine_dvr(self.n[a], self.L[a])
             self.x.append(x)
             self.T.append(np.ascontiguousarray(T))
         self.ngp = self.n[0] * self.n[1] * self.n[2]

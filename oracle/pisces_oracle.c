@@ -453,6 +453,31 @@ int or_water_lists(int nw, const double *sites, const double *mu, double *q,
     return nq;
 }
 
+void or_fft_kinetic(int n, double mass, double x0, double xnp1, double *x, double *T,
+                    double *ek)
+{
+    /* Tdiag, src/KE_diag.cpp */
+    const double pi = 3.14159265358979323846;
+    double dx = (xnp1 - x0) / (double)(n + 1);
+    for (int j = 0; j < n; ++j) x[j] = x0 + dx * (double)(j + 1);
+    double dk = 2.0 * pi / ((double)n * dx);
+    double *e = malloc(sizeof(double) * n);
+    int *m = malloc(sizeof(int) * n);
+    for (int a = 0; a < n; ++a) {
+        m[a] = (a >= n / 2) ? a - n : a;
+        e[a] = (dk * m[a]) * (dk * m[a]) / (2.0 * mass);
+        if (ek) ek[a] = e[a];
+    }
+    for (int j = 0; j < n; ++j)
+        for (int l = 0; l < n; ++l) {
+            double t = 0;
+            for (int a = 0; a < n; ++a) t += e[a] * cos(2.0 * pi * (double)m[a] * (double)(j - l) / (double)n);
+            T[(long)j * n + l] = t / (double)n;
+        }
+    free(e);
+    free(m);
+}
+
 /* ==================================================================== a5 a6 */
 typedef struct {
     int damp_type, rep_type, pol_type, adiabatic;

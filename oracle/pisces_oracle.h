@@ -34,6 +34,18 @@ void or_sine_dvr(int n, double mass, double x0, double xnp1, double *x, double *
 void or_sine_dvr_packed(int n, double mass, double x0, double xnp1, double *x, double *Tp);
 /* Pinned: reference output (tkin fixture, packed layout). */
 
+/* DVRType 3: the FFT kinetic energy on the same grid points.  Restates Tdiag
+ * (src/KE_diag.cpp): dx = (xN+1 - x0)/(n+1), k_a = 2 pi m_a / (n dx), m_a = a
+ * (a < n/2) or a - n, e_a = k_a^2 / (2 mass), which VectorFFT::apply
+ * (src/VectorFFTW.cpp:56-95) multiplies in Fourier space; T here is the same
+ * operator on the grid: T_jl = (1/n) sum_a e_a cos(2 pi m_a (j - l) / n),
+ * full, row-major n*n.  ek (optional, n): the e_a in FFT order. */
+void or_fft_kinetic(int n, double mass, double x0, double xnp1, double *x, double *T,
+                    double *ek);
+/* Pinned: the spectrum of T is {e_a}; T cos(k x) = e_k cos(k x); with
+ * or_apply_h it equals numpy's FFT form V x + ifftn(E fftn(x)).  No reference
+ * run: VectorFFTW.cpp needs FFTW, absent here. */
+
 /* ---------------------------------------------------------------- a3, a4 */
 /* Y[:,v] = V o X[:,v] + (Tx (x) I (x) I + I (x) Ty (x) I + I (x) I (x) Tz) X[:,v]
  * for v < nvec; vectors are columns of length nx*ny*nz, x fastest.

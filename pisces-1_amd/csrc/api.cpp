@@ -103,7 +103,21 @@ const char *pisces_last_error(void) { return pisces::last_error(); }
 
 int pisces_create(pisces_ctx **out, const int n[3], const double L[3], int device)
 {
+    return pisces_create_ex(out, n, L, device, 0);
+}
+
+int pisces_create_ex(pisces_ctx **out, const int n[3], const double L[3], int device, int dvr_type)
+{
     if (!out || !n || !L) return PISCES_ERR_ARG;
+    if (dvr_type != 0 && dvr_type != 3) {
+        set_last_error("DVR type must be 0 (sine DVR) or 3 (FFT kinetic energy)");
+        return PISCES_ERR_UNSUPPORTED;
+    }
+    // Tdiag's wave numbers m = a (a < n/2), a - n are symmetric only for even n
+    if (dvr_type == 3 && ((n[0] | n[1] | n[2]) & 1)) {
+        set_last_error("DVR type 3 needs even grid sizes");
+        return PISCES_ERR_UNSUPPORTED;
+    }
     *out = nullptr;
     for (int a = 0; a < 3; ++a)
         if (n[a] < 2 || n[a] > 128 || !(L[a] > 0)) {
@@ -114,6 +128,7 @@ int pisces_create(pisces_ctx **out, const int n[3], const double L[3], int devic
     if (!c) return PISCES_ERR_NOMEM;
     int rc = guarded([&] {
         c->device = device;
+        c->dvr_type = dvr_type;
         PC_HIP(hipSetDevice(device));
         PC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->ngp = 1;
@@ -124,8 +139,12 @@ int pisces_create(pisces_ctx **out, const int n[3], const double L[3], int devic
             c->maxn = std::max(c->maxn, n[a]);
             c->x[a].resize(n[a]);
             c->T[a].resize((size_t)n[a] * n[a]);
-            // DVR::ComputeGridPointsAndKineticEnergy: sine_dvr on [-L/2, L/2], mass 1
-            sine_dvr_full(n[a], 1.0, -0.5 * L[a], 0.5 * L[a], c->x[a].data(), c->T[a].data());
+            // DVR::ComputeGridPointsAndKineticEnergy (src/DVR.cpp:583-606) on
+            // [-L/2, L/2], mass 1: sine_dvr, or Tdiag for DVRType 3
+            if (dvr_type == 3)
+                fft_kinetic_full(n[a], 1.0, -0.5 * L[a], 0.5 * L[a], c->x[a].data(), c->T[a].data());
+            else
+                sine_dvr_full(n[a], 1.0, -0.5 * L[a], 0.5 * L[a], c->x[a].data(), c->T[a].data());
             upload(c->dx[a], c->x[a].data(), sizeof(double) * n[a], c->stream);
             upload(c->dT64[a], c->T[a].data(), sizeof(double) * n[a] * n[a], c->stream);
             c->ax[a].build(n[a], c->T[a].data(), c->stream);

@@ -195,5 +195,7 @@ void water_lists_host(int nw, std::vector<double> &q, std::vector<int> &iq,
 
 // --------------------------------------------------------------- host helpers
 void sine_dvr_full(int n, double mass, double x0, double xnp1, double *x, double *T);
+// DVRType 3: the FFT kinetic energy as the real circulant n x n matrix
+void fft_kinetic_full(int n, double mass, double x0, double xnp1, double *x, double *T);
 
 }  // namespace pisces

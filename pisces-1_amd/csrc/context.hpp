@@ -14,6 +14,7 @@
 // Re-entrant: everything lives here, nothing in globals.
 struct pisces_ctx {
     int device = 0;
+    int dvr_type = 0;  // 0 sine DVR, 3 FFT kinetic energy
     hipStream_t stream = nullptr;
     int n[3] = {0, 0, 0};
     double L[3] = {0, 0, 0};

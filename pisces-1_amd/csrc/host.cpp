@@ -47,6 +47,37 @@ void sine_dvr_full(int n, double mass, double x0, double xnp1, double *x, double
         }
 }
 
+// DVRType 3 (src/KE_diag.cpp Tdiag, applied by VectorFFT::apply through FFTW,
+// src/VectorFFTW.cpp): grid points as for the sine DVR, x_j = x0 + (j + 1) dx,
+// dx = L / (n + 1); the kinetic energy is diagonal in the discrete Fourier basis
+// of the n points, e_a = k_a^2 / (2 mass) with k_a = 2 pi m_a / (n dx),
+// m_a = a for a < n/2 and a - n above.  In the grid basis that is the real
+// symmetric circulant T_jl = (1/n) sum_a e_a cos(2 pi m_a (j - l) / n), so the
+// FFT kinetic of a 3-D grid is the same axis sum T_x + T_y + T_z as the sine DVR
+// with these matrices (the reference's 3-D KE_diag is e_x + e_y + e_z).
+void fft_kinetic_full(int n, double mass, double x0, double xnp1, double *x, double *T)
+{
+    const double dx = (xnp1 - x0) / (double)(n + 1);
+    for (int j = 0; j < n; ++j) x[j] = x0 + dx * (double)(j + 1);
+    const double dk = 2.0 * kPi / ((double)n * dx);
+    std::vector<double> e(n);
+    std::vector<int> m(n);
+    for (int a = 0; a < n; ++a) {
+        m[a] = a >= n / 2 ? a - n : a;
+        e[a] = (dk * m[a]) * (dk * m[a]) / (2.0 * mass);
+    }
+    for (int d = 0; d < n; ++d) {
+        // entry for j - l = d (mod n); the phase index m d is reduced mod n first
+        double t = 0.0;
+        for (int a = 0; a < n; ++a) {
+            const long long r = (((long long)m[a] * d) % n + n) % n;
+            t += e[a] * std::cos(2.0 * kPi * (double)r / (double)n);
+        }
+        t /= (double)n;
+        for (int j = 0; j < n; ++j) T[(size_t)j * n + (j + n - d) % n] = t;  // T[j][j - d]
+    }
+}
+
 // ------------------------------------------------------------------ water
 // Standard orientation of the DPP monomer relative to its centre of mass, bohr:
 // Water::SetupCharge case 2 (src/Water.cpp:216-236) with src/ttm_data.h:29-44.

@@ -294,7 +294,8 @@ int main(int argc, char **argv)
     }
     if (P.runtype != 1) return fail("runtype " + std::to_string(P.runtype) + ": only single points (1) run here");
     if (P.nElectron != 1) return fail("nElectron must be 1");
-    if (P.DVRType != 0 || P.Sampling != 1) return fail("only the sine DVR (DVRType 0) with Sampling 1");
+    if ((P.DVRType != 0 && P.DVRType != 3) || P.Sampling != 1)
+        return fail("only the sine DVR (DVRType 0) or the FFT kinetic energy (3), with Sampling 1");
     if (P.KTFlag != 0 || P.CenterFlag != 0) return fail("KTCharges and CoMOrigin must be 0");
     // DVR::Diagonalize (src/DVR.cpp:462-471): Method 2 Davidson, 3 Jacobi-Davidson, 4/5 none
     int corrflag;
@@ -309,8 +310,8 @@ int main(int argc, char **argv)
 
     pisces_ctx *ctx = nullptr;
     const double L[3] = {P.gpara[0], P.gpara[1], P.gpara[2]};
-    int rc = pisces_create(&ctx, P.ngrid, L, device);
-    if (rc) return fail(std::string("pisces_create: ") + pisces_last_error());
+    int rc = pisces_create_ex(&ctx, P.ngrid, L, device, P.DVRType);
+    if (rc) return fail(std::string("pisces_create_ex: ") + pisces_last_error());
     rc = pisces_build_cluster_potential(ctx, P.nWater, xyz.data(), P.PotFlag, P.PotPara, 0, nullptr,
                                         nullptr, 2.9, nullptr, nullptr);
     if (rc < 0) {

@@ -55,7 +55,7 @@ def lib():
                      "pisces_build_potential", "pisces_build_cluster_potential",
                      "pisces_build_potential_scp",
                      "pisces_apply_h_device", "pisces_apply_h_ex", "pisces_time_hx", "pisces_time_precond",
-                     "pisces_solve_block"):
+                     "pisces_solve_block", "pisces_create_ex"):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -89,16 +89,22 @@ def set_hx_variant(v):
 
 
 class Context:
-    """One sine-DVR grid on one GPU (pisces_create)."""
+    """One grid on one GPU (pisces_create; pisces_create_ex for the FFT
+    kinetic energy, dvr_type 3)."""
 
-    def __init__(self, n, L, device=0):
+    def __init__(self, n, L, device=0, dvr_type=0):
         self.n = tuple(int(v) for v in n)
         self.L = tuple(float(v) for v in L)
         self.ngp = self.n[0] * self.n[1] * self.n[2]
         h = ctypes.c_void_p()
         na = np.array(self.n, dtype=np.int32)
         La = np.array(self.L, dtype=np.float64)
-        check(lib().pisces_create(ctypes.byref(h), _pi(na), _p(La), int(device)), "pisces_create")
+        if dvr_type == 0:
+            check(lib().pisces_create(ctypes.byref(h), _pi(na), _p(La), int(device)),
+                  "pisces_create")
+        else:
+            check(lib().pisces_create_ex(ctypes.byref(h), _pi(na), _p(La), int(device),
+                                         int(dvr_type)), "pisces_create_ex")
         self.h = h
 
     def close(self):
