@@ -508,8 +508,9 @@ def pol3_measure(P, workload):
         n3 = 9 * nw
         ncb = (n3 + 15) // 16
         ng = n ** 3
-        wgs = (ng + 31) // 32
-        fl_exec = 16384.0 * ncb * (ncb + 1) / 2 * wgs  # MFMA flops of the launch
+        # MFMA flops of the launch: per 16 points, (cb + 1) 16x16x16 tiles of
+        # 2 * 16^3 flops for column block cb of the upper block triangle
+        fl_exec = 256.0 * ncb * (ncb + 1) * ng
         fl_ref = 2.0 * n3 * n3 * ng                     # the reference's dsymv per point
         out[tag] = {"n_pol_dims": n3, "grid": n, "ms_kernel": round(1e3 * t, 4),
                     "TFLOPs_executed": round(fl_exec / t / 1e12, 2),

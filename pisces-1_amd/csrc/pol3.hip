@@ -5,17 +5,19 @@
 // r - site i, g the damped R^-3), mu = InvA E, Vpol = -E.mu / 2 -
 // VpolWaterWater.  Over a tile of grid points that is the product E InvA (points
 // x 3np times 3np x 3np) followed by a row-wise dot with E: FP64 matrix-core
-// work of 2 (3np)^2 flops per point.  A workgroup takes 32 points: it builds
-// their E rows in LDS (odd pitch: the 16 rows that one matrix-core operand
-// reads fall on distinct banks), then each of its four waves takes 16-column
-// blocks of InvA, streams them from L2 (InvA is shared by all workgroups: at
-// most 2.6 MB) as the B operand of v_mfma_f64_16x16x4 against both 16-point row
-// blocks of E, and folds each 16 x 16 result into the per-point dot at once.
+// work of 2 (3np)^2 flops per point.  A workgroup takes 16 RB points (RB = 1
+// by default): it builds their E rows in LDS (odd pitch: the 16 rows that one
+// matrix-core operand reads fall on distinct banks), then each of its four waves
+// takes 16-column blocks of InvA, streams them from L2 (InvA is shared by all
+// workgroups: at most 2.6 MB) as the B operand of v_mfma_f64_16x16x4 against
+// the 16-point row blocks of E, and folds each 16 x 16 result into the
+// per-point dot at once.
 // InvA is symmetric, so only its upper block triangle is multiplied: (3np)^2
 // (1 + 16 / 3np) flops per point.
 #include "common.hpp"
 #include "gmem.hpp"
 
+#include <cstdlib>
 #include <mutex>
 
 namespace pisces {
@@ -23,7 +25,7 @@ namespace pisces {
 namespace {
 
 constexpr int kP3Threads = 256;
-constexpr int kP3Pts = 32;  // grid points per workgroup: two 16-row blocks
+constexpr int kLdsMax = 160 * 1024 - (kP3Threads / 64) * 64 * (int)sizeof(double);
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -37,11 +39,14 @@ __device__ inline double reff3(double r, double r0)
     return r;
 }
 
+// RB 16-point row blocks per workgroup (kP3Pts = 16 RB grid points)
+template <int RB>
 __global__ void __launch_bounds__(kP3Threads)
 pol3_grid_kernel(Pol3Dev D, int nx, int ny, int nz, const double *__restrict__ gx,
                  const double *__restrict__ gy, const double *__restrict__ gz,
                  double *__restrict__ v)
 {
+    constexpr int kP3Pts = 16 * RB;
     extern __shared__ double Es[];  // kP3Pts rows of n3p (+1) values
     __shared__ double red[kP3Threads / 64][kP3Pts];
     const int n3p = D.n3p, ldE = n3p + 1, np = D.np, n3 = 3 * np;
@@ -97,50 +102,58 @@ pol3_grid_kernel(Pol3Dev D, int nx, int ny, int nz, const double *__restrict__ g
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const gptr<const double> A = glb(D.inva);
-    double s0[4] = {0.0, 0.0, 0.0, 0.0}, s1[4] = {0.0, 0.0, 0.0, 0.0};
-    const double *e0p = Es + l16 * ldE + lq, *e1p = Es + (16 + l16) * ldE + lq;
+    double sr[RB][4];
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sr[b][r] = 0.0;
+    const double *ea = Es + l16 * ldE + lq;  // row block b: + 16 b ldE
     const int ncb = n3p / 16;
     for (int j = 0; 4 * j < ncb; ++j) {
         const int cb = (j & 1) ? 4 * j + 3 - wave : 4 * j + wave;
         if (cb >= ncb) continue;
-        f64x4 o0 = {0.0, 0.0, 0.0, 0.0}, o1 = {0.0, 0.0, 0.0, 0.0};
-        f64x4 d0 = {0.0, 0.0, 0.0, 0.0}, d1 = {0.0, 0.0, 0.0, 0.0};
+        f64x4 o[RB], d[RB];
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            o[b] = f64x4{0.0, 0.0, 0.0, 0.0};
+            d[b] = f64x4{0.0, 0.0, 0.0, 0.0};
+        }
         const gptr<const double> bc = A + (long long)lq * n3p + 16 * cb + l16;
         const int kd = 16 * cb;
 #pragma unroll 4
         for (int k0 = 0; k0 < kd; k0 += 4) {
-            const double b = bc[(long long)k0 * n3p];  // InvA[k0 + lq][16 cb + l16]
-            o0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e0p[k0], b, o0, 0, 0, 0);
-            o1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e1p[k0], b, o1, 0, 0, 0);
+            const double bv = bc[(long long)k0 * n3p];  // InvA[k0 + lq][16 cb + l16]
+#pragma unroll
+            for (int b = 0; b < RB; ++b)
+                o[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ea[16 * b * ldE + k0], bv, o[b], 0, 0, 0);
         }
 #pragma unroll
         for (int k0 = kd; k0 < kd + 16; k0 += 4) {
-            const double b = bc[(long long)k0 * n3p];
-            d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(e0p[k0], b, d0, 0, 0, 0);
-            d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(e1p[k0], b, d1, 0, 0, 0);
+            const double bv = bc[(long long)k0 * n3p];
+#pragma unroll
+            for (int b = 0; b < RB; ++b)
+                d[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ea[16 * b * ldE + k0], bv, d[b], 0, 0, 0);
         }
-        // o0[r], d0[r]: point lq + 4 r, column 16 cb + l16
+        // o[b][r], d[b][r]: point 16 b + lq + 4 r, column 16 cb + l16
         const int c = 16 * cb + l16;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            s0[r] += (2.0 * o0[r] + d0[r]) * Es[(lq + 4 * r) * ldE + c];
-            s1[r] += (2.0 * o1[r] + d1[r]) * Es[(16 + lq + 4 * r) * ldE + c];
-        }
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                sr[b][r] += (2.0 * o[b][r] + d[b][r]) * Es[(16 * b + lq + 4 * r) * ldE + c];
     }
     // sum over the 16 columns a lane group holds, then over the waves
 #pragma unroll
     for (int off = 1; off < 16; off <<= 1)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            s0[r] += __shfl_xor(s0[r], off, 64);
-            s1[r] += __shfl_xor(s1[r], off, 64);
-        }
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sr[b][r] += __shfl_xor(sr[b][r], off, 64);
     if (l16 == 0)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            red[wave][lq + 4 * r] = s0[r];
-            red[wave][16 + lq + 4 * r] = s1[r];
-        }
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[wave][16 * b + lq + 4 * r] = sr[b][r];
     __syncthreads();
     if (threadIdx.x < kP3Pts) {
         const long long g = g0 + threadIdx.x;
@@ -163,15 +176,38 @@ void pol3_grid_launch(const Pol3Dev &D, int nx, int ny, int nz, const double *d_
     PC_REQUIRE(D.damp_type == 1 || D.damp_type == 2, PISCES_ERR_ARG, "damping type");
     static std::once_flag once;
     std::call_once(once, [] {
-        static_cast<void>(hipFuncSetAttribute(reinterpret_cast<const void *>(pol3_grid_kernel),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              kP3Pts * (kPol3MaxN + 1) * (int)sizeof(double)));
+        static_cast<void>(hipFuncSetAttribute(reinterpret_cast<const void *>(pol3_grid_kernel<2>),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
+        static_cast<void>(hipFuncSetAttribute(reinterpret_cast<const void *>(pol3_grid_kernel<4>),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
+        static_cast<void>(hipFuncSetAttribute(reinterpret_cast<const void *>(pol3_grid_kernel<1>),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
     });
+    // RB 16-point row blocks per workgroup (PISCES_POL3_RB = 1, 2 or 4).  The E
+    // rows in LDS bound how many workgroups a CU holds, and that decides the
+    // speed: 16-point tiles (1) measured 1.27x / 1.16x faster than 32-point
+    // ones at 216 / 405 polarizable dimensions, 64-point tiles slower still,
+    // although each InvA fragment then feeds fewer matrix-core products.
+    static const int rb_env = [] {
+        const char *e = std::getenv("PISCES_POL3_RB");
+        return e ? std::atoi(e) : 0;
+    }();
+    const size_t row = (size_t)(D.n3p + 1) * sizeof(double);
+    int rb = (rb_env == 1 || rb_env == 2 || rb_env == 4) ? rb_env : 1;
+    while (rb > 1 && 16 * rb * row > (size_t)kLdsMax) rb /= 2;
     const long long ng = (long long)nx * ny * nz;
-    const unsigned blocks = (unsigned)((ng + kP3Pts - 1) / kP3Pts);
-    const size_t lds = (size_t)kP3Pts * (D.n3p + 1) * sizeof(double);
-    hipLaunchKernelGGL(pol3_grid_kernel, dim3(blocks), dim3(kP3Threads), lds, s, D, nx, ny, nz,
-                       d_x, d_y, d_z, d_v);
+    const int pts = 16 * rb;
+    const unsigned blocks = (unsigned)((ng + pts - 1) / pts);
+    const size_t lds = (size_t)pts * row;
+    if (rb == 4)
+        hipLaunchKernelGGL(pol3_grid_kernel<4>, dim3(blocks), dim3(kP3Threads), lds, s, D, nx, ny,
+                           nz, d_x, d_y, d_z, d_v);
+    else if (rb == 2)
+        hipLaunchKernelGGL(pol3_grid_kernel<2>, dim3(blocks), dim3(kP3Threads), lds, s, D, nx, ny,
+                           nz, d_x, d_y, d_z, d_v);
+    else
+        hipLaunchKernelGGL(pol3_grid_kernel<1>, dim3(blocks), dim3(kP3Threads), lds, s, D, nx, ny,
+                           nz, d_x, d_y, d_z, d_v);
     PC_HIP(hipGetLastError());
 }
 
