@@ -82,7 +82,10 @@ int pisces_tuning_gram(int *variant, int *rows);
  * device: HIP device ordinal. */
 int pisces_create(pisces_ctx **ctx, const int n[3], const double L[3], int device);
 /* The same with the DVR type of DVR::SetupDVR (src/DVR.h:56-91, `type`):
- * 0 the sine DVR (as pisces_create); 3 the FFT kinetic energy of DVRType 3
+ * 0 the sine DVR (as pisces_create); 1 the harmonic-oscillator DVR (ho_dvr,
+ * src/ho_dvr.cpp), with L[a] the oscillator frequency Omega0 of axis a
+ * (src/GetInput.cpp:133-140), mass 1; 20 the Colbert-Miller DVR (cm_ham,
+ * src/cm_dvr.cpp) on [-L/2, L/2]; 3 the FFT kinetic energy of DVRType 3
  * (Tdiag, src/KE_diag.cpp; applied through FFTW by VectorFFT::apply,
  * src/VectorFFT.hpp:32, src/VectorFFTW.cpp), on the same grid points.  The
  * kinetic operator of type 3 is diagonal in the discrete Fourier basis of each

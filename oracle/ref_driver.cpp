@@ -18,6 +18,9 @@
 //   ref_driver davidson  <in> <out>   in: [nx,ny,nz,Lx,Ly,Lz], V, [nroots,maxsub,
 //                                         maxmacro,ptol,corrflag], start vectors
 //                                     out: [nconv,n_mtx], evals, vectors
+//   ref_driver kin       <in> <out>   in: [type, n, para] (type 1: ho_dvr with
+//                                     frequency para; 20: cm_ham on [-para/2, para/2])
+//                                     out: [x(n)], [T full n*n]
 //   ref_driver input     <file> <out> the reference's TSIN + GetInputParameters +
 //                                     ReadWatersGroup on an input file
 //                                     out: [runtype, nWater, DVRType, Sampling,
@@ -38,6 +41,8 @@
 #include "sine_dvr.h"
 #include "Davidson.h"
 #include "tsin.h"
+#include "ho_dvr.h"
+#include "cm_dvr.h"
 #include "Parameters.h"
 #include "GetInput.h"
 
@@ -188,6 +193,18 @@ static int cmd_davidson(const char *in, const char *out)
     return 0;
 }
 
+static int cmd_kin(const char *in, const char *out)
+{
+    std::vector<rec> r = read_records(in);
+    const int type = (int)r[0][0], n = (int)r[0][1];
+    const double para = r[0][2];
+    rec x(n), t((size_t)n * n), z((size_t)n * n);
+    if (type == 1) ho_dvr(n, para, x.data(), t.data(), n, z.data(), 1, 0);
+    else cm_ham(n, 1.0, -0.5 * para, 0.5 * para, t.data(), x.data(), n, 0);
+    write_records(out, {x, t});
+    return 0;
+}
+
 static int cmd_input(const char *in, const char *out)
 {
     TSIN Input;
@@ -218,6 +235,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[1], "potential")) rc = cmd_potential(argv[2], argv[3]);
         else if (!strcmp(argv[1], "davidson")) rc = cmd_davidson(argv[2], argv[3]);
         else if (!strcmp(argv[1], "input")) rc = cmd_input(argv[2], argv[3]);
+        else if (!strcmp(argv[1], "kin")) rc = cmd_kin(argv[2], argv[3]);
     } else {
         fprintf(stderr, "usage: ref_driver {tkin|potential|davidson} <in> <out>\n");
     }

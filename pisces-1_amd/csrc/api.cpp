@@ -109,8 +109,9 @@ int pisces_create(pisces_ctx **out, const int n[3], const double L[3], int devic
 int pisces_create_ex(pisces_ctx **out, const int n[3], const double L[3], int device, int dvr_type)
 {
     if (!out || !n || !L) return PISCES_ERR_ARG;
-    if (dvr_type != 0 && dvr_type != 3) {
-        set_last_error("DVR type must be 0 (sine DVR) or 3 (FFT kinetic energy)");
+    if (dvr_type != 0 && dvr_type != 1 && dvr_type != 3 && dvr_type != 20) {
+        set_last_error("DVR type must be 0 (sine), 1 (harmonic oscillator), 3 (FFT) or 20 "
+                       "(Colbert-Miller)");
         return PISCES_ERR_UNSUPPORTED;
     }
     // Tdiag's wave numbers m = a (a < n/2), a - n are symmetric only for even n
@@ -143,6 +144,10 @@ int pisces_create_ex(pisces_ctx **out, const int n[3], const double L[3], int de
             // [-L/2, L/2], mass 1: sine_dvr, or Tdiag for DVRType 3
             if (dvr_type == 3)
                 fft_kinetic_full(n[a], 1.0, -0.5 * L[a], 0.5 * L[a], c->x[a].data(), c->T[a].data());
+            else if (dvr_type == 1)  // L[a] is the oscillator frequency (Omega0)
+                ho_dvr_full(n[a], L[a], c->x[a].data(), c->T[a].data());
+            else if (dvr_type == 20)
+                cm_dvr_full(n[a], 1.0, -0.5 * L[a], 0.5 * L[a], c->x[a].data(), c->T[a].data());
             else
                 sine_dvr_full(n[a], 1.0, -0.5 * L[a], 0.5 * L[a], c->x[a].data(), c->T[a].data());
             upload(c->dx[a], c->x[a].data(), sizeof(double) * n[a], c->stream);

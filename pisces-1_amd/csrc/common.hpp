@@ -195,6 +195,9 @@ void water_lists_host(int nw, std::vector<double> &q, std::vector<int> &iq,
 
 // --------------------------------------------------------------- host helpers
 void sine_dvr_full(int n, double mass, double x0, double xnp1, double *x, double *T);
+// DVRType 1: harmonic-oscillator DVR of frequency w; DVRType 20: Colbert-Miller
+void ho_dvr_full(int n, double w, double *x, double *T);
+void cm_dvr_full(int n, double mass, double rl, double ru, double *x, double *T);
 // DVRType 3: the FFT kinetic energy as the real circulant n x n matrix
 void fft_kinetic_full(int n, double mass, double x0, double xnp1, double *x, double *T);
 

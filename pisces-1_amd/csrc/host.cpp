@@ -78,6 +78,67 @@ void fft_kinetic_full(int n, double mass, double x0, double xnp1, double *x, dou
     }
 }
 
+// DVRType 1 (ho_dvr, src/ho_dvr.cpp): the harmonic-oscillator DVR of frequency w
+// (mass 1).  The grid points are the eigenvalues of the position operator in
+// the oscillator basis (tridiagonal, off-diagonal sqrt(i + 1)) scaled by
+// 1/sqrt(2 w); with Z its eigenvectors (each signed so that its first component
+// is positive), T = Z^T T_FBR Z, T_FBR the pentadiagonal kinetic matrix of the
+// basis: diagonal w (i + 1/2) / 2, (i, i + 2) entries -w sqrt((i + 1)(i + 2)) / 4.
+void ho_dvr_full(int n, double w, double *x, double *T)
+{
+    std::vector<double> X((size_t)n * n, 0.0), Z((size_t)n * n), ev(n);
+    for (int i = 0; i + 1 < n; ++i)
+        X[(size_t)i * n + i + 1] = X[(size_t)(i + 1) * n + i] = std::sqrt((double)(i + 1));
+    syev_host(n, X.data(), n, ev.data(), Z.data(), n);  // Z(r, c) = Z[r + n c]
+    const double s = 1.0 / std::sqrt(2.0 * w);
+    for (int i = 0; i < n; ++i) x[i] = ev[i] * s;
+    for (int c = 0; c < n; ++c)
+        if (Z[(size_t)n * c] < 0)
+            for (int r = 0; r < n; ++r) Z[r + (size_t)n * c] = -Z[r + (size_t)n * c];
+    std::vector<double> F((size_t)n * n, 0.0), FZ((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        F[(size_t)i * n + i] = 0.5 * w * (i + 0.5);
+        if (i + 2 < n)
+            F[(size_t)i * n + i + 2] = F[(size_t)(i + 2) * n + i] =
+                -0.25 * w * std::sqrt((double)(i + 1) * (double)(i + 2));
+    }
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < n; ++c) {
+            double a = 0.0;
+            for (int k = 0; k < n; ++k) a += F[(size_t)i * n + k] * Z[k + (size_t)n * c];
+            FZ[(size_t)i * n + c] = a;
+        }
+    for (int a = 0; a < n; ++a)
+        for (int b = 0; b <= a; ++b) {
+            double t = 0.0;
+            for (int k = 0; k < n; ++k) t += Z[k + (size_t)n * a] * FZ[(size_t)k * n + b];
+            T[(size_t)a * n + b] = T[(size_t)b * n + a] = t;
+        }
+}
+
+// DVRType 20 (cm_ham / cm_dvr, src/cm_dvr.cpp): the Colbert-Miller kinetic
+// matrix of n points over the length ru - rl, with grid points rl + i (ru - rl)/n
+// as the reference places them.
+void cm_dvr_full(int n, double mass, double rl, double ru, double *x, double *T)
+{
+    const double len = ru - rl, dr = len / (double)n;
+    for (int i = 0; i < n; ++i) x[i] = rl + dr * (double)i;
+    const int m = n + 1;
+    const double f1 = kPi * kPi / (4.0 * mass * len * len);
+    const double f2 = kPi / (double)(2 * m);
+    const double f3 = (2.0 * m * m + 1.0) / 3.0;
+    for (int i = 0; i < n; ++i) {
+        const double si = std::sin(kPi * (double)(i + 1) / (double)m);
+        T[(size_t)i * n + i] = f1 * (f3 - 1.0 / (si * si));
+        for (int j = 0; j < i; ++j) {
+            const double a = std::sin(f2 * (j - i)), b = std::sin(f2 * (j + i + 2));
+            double t = f1 * (1.0 / (a * a) - 1.0 / (b * b));
+            if ((i - j) % 2 == 1) t = -t;
+            T[(size_t)i * n + j] = T[(size_t)j * n + i] = t;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ water
 // Standard orientation of the DPP monomer relative to its centre of mass, bohr:
 // Water::SetupCharge case 2 (src/Water.cpp:216-236) with src/ttm_data.h:29-44.

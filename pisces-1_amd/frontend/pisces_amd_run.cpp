@@ -294,8 +294,8 @@ int main(int argc, char **argv)
     }
     if (P.runtype != 1) return fail("runtype " + std::to_string(P.runtype) + ": only single points (1) run here");
     if (P.nElectron != 1) return fail("nElectron must be 1");
-    if ((P.DVRType != 0 && P.DVRType != 3) || P.Sampling != 1)
-        return fail("only the sine DVR (DVRType 0) or the FFT kinetic energy (3), with Sampling 1");
+    if ((P.DVRType != 0 && P.DVRType != 1 && P.DVRType != 3 && P.DVRType != 20) || P.Sampling != 1)
+        return fail("DVRType 0 (sine), 1 (oscillator), 3 (FFT) or 20 (Colbert-Miller), with Sampling 1");
     if (P.KTFlag != 0 || P.CenterFlag != 0) return fail("KTCharges and CoMOrigin must be 0");
     // DVR::Diagonalize (src/DVR.cpp:462-471): Method 2 Davidson, 3 Jacobi-Davidson, 4/5 none
     int corrflag;

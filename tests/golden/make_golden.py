@@ -48,6 +48,18 @@ def golden_tkin():
     np.savez_compressed(os.path.join(HERE, "tkin.npz"), ncases=len(cases), **out)
 
 
+def golden_kin_types():
+    """The harmonic-oscillator (DVRType 1) and Colbert-Miller (20) DVRs."""
+    cases = [(1, 8, 0.05), (1, 17, 0.3), (1, 32, 0.02), (20, 8, 20.0), (20, 33, 36.0)]
+    out = {}
+    for i, (t, n, p) in enumerate(cases):
+        x, T = run_ref("kin", [[t, n, p]])
+        out["case%d" % i] = np.array([t, n, p])
+        out["x%d" % i] = x
+        out["T%d" % i] = T.reshape(n, n)
+    np.savez_compressed(os.path.join(HERE, "tkin_types.npz"), ncases=len(cases), **out)
+
+
 def sample_points(sites, rng, box, nbox=400, nnear=40):
     pts = [rng.uniform(-box / 2, box / 2, size=(nbox, 3))]
     for s in sites:  # points near every site, 0.05 .. 4 bohr away (damping branches)
@@ -285,6 +297,7 @@ def main():
         sys.exit("oracle/_ref/ref_driver is not built")
     oracle.set_threads(1)
     golden_tkin()
+    golden_kin_types()
     golden_potential("c1", 6, 3.5, 1000, 36.0)
     golden_potential("c2", 24, 6.0, 2000, 40.0)
     golden_potential_pol3("c1", 6, 3.5, 1000, 36.0)
