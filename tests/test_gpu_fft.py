@@ -79,4 +79,4 @@ def test_odd_sizes_are_refused(pisces):
     with pytest.raises(pisces.PiscesError):
         pisces.Context((15, 16, 16), (20.0, 20.0, 20.0), dvr_type=3)
     with pytest.raises(pisces.PiscesError):
-        pisces.Context((16, 16, 16), (20.0, 20.0, 20.0), dvr_type=1)
+        pisces.Context((16, 16, 16), (20.0, 20.0, 20.0), dvr_type=2)
