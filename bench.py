@@ -287,6 +287,9 @@ def main():
         "speedup_vs_fp64_single": round(sps / fp64_single_sps, 2),
         "speedup_vs_fp64_batched": round(sps / fp64_batched_sps, 2),
         "max_abs_dE_vs_fp64": float("%.3e" % max(de.max(), de_single.max())),
+        # per state: the largest |E_mixed - E_fp64| over the batch's snapshots
+        "max_abs_dE_vs_fp64_per_state": [float("%.3e" % v) for v in
+                                         np.max(np.atleast_2d(de), axis=0)],
         "dE_ok": bool(max(de.max(), de_single.max()) <= DE_TOL and n_ok == B * a.steps),
         "evals_snapshot0": [round(float(x), 10) for x in evm[0]],
         "iters_mean": round(float(np.mean([r["n_iter"] for r in res[-1]])), 1),
@@ -300,7 +303,7 @@ def main():
     }
     if a.served_only:
         for key in ("fp64_gpu", "speedup_vs_fp64_single", "speedup_vs_fp64_batched",
-                    "max_abs_dE_vs_fp64", "dE_ok"):
+                    "max_abs_dE_vs_fp64", "max_abs_dE_vs_fp64_per_state", "dE_ok"):
             out[key] = None
     print(json.dumps(out), flush=True)
 
