@@ -153,6 +153,15 @@ def water_induced(sites):
     return dict(mu=mu, epc=epc, e_pol=ep.value, iters=it.value, converged=(rc == 0))
 
 
+def damp(r, ai, aj, a):
+    """Charge-dipole and dipole-dipole damping factors of the water model."""
+    r, ai, aj, a = (np.ascontiguousarray(v, dtype=np.float64) for v in (r, ai, aj, a))
+    dc = np.zeros(r.size)
+    dd = np.zeros(r.size)
+    lib().or_damp(r.size, _p(r), _p(ai), _p(aj), _p(a), _p(dc), _p(dd))
+    return dc, dd
+
+
 def water_lists(sites, mu):
     nw = sites.shape[0] // 4
     q = np.zeros(3 * nw)

@@ -308,6 +308,15 @@ static double damp_dd(double r, double ai, double aj, double a)
     return 1.0 - (1 + x) * exp(-x);
 }
 
+void or_damp(int n, const double *r, const double *ai, const double *aj, const double *a,
+             double *dc, double *dd)
+{
+    for (int i = 0; i < n; ++i) {
+        dc[i] = damp_cd(r[i], ai[i], aj[i], a[i]);
+        dd[i] = damp_dd(r[i], ai[i], aj[i], a[i]);
+    }
+}
+
 int or_water_induced(int nw, const double *sites, double *mu, double *epc,
                      double *e_pol, int *iters)
 {

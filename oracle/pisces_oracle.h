@@ -78,6 +78,13 @@ int or_water_sites(int nw, const double *xyz_ang, double *sites, double *conf);
  * reproduced, M on the bisector 0.25 A from O, centre of mass with masses 16/1/1.
  * Parity unpinned: the Euler-angle conventions for non-rigid input. */
 
+/* Charge-dipole and dipole-dipole damping factors of the water model
+ * (ElectrostaticDCDamp / ElectrostaticDDDamp, src/DPP.cpp:1798-1812):
+ * 1 - e^-x and 1 - (1 + x) e^-x with x = a (r / (alpha_i alpha_j)^(1/6))^3.
+ * Pinned: the reference's own functions (fixture damp.npz). */
+void or_damp(int n, const double *r, const double *ai, const double *aj, const double *a,
+             double *dc, double *dd);
+
 /* Self-consistent induced dipoles on the polarizable sites (O,H1,H2 of each
  * water): WaterCluster::CalcInducedDipoles, src/DPP.cpp:760-1065.
  * sites: 12*nw bohr (from or_water_sites).  Outputs:
@@ -92,8 +99,9 @@ int or_water_induced(int nw, const double *sites, double *mu, double *epc,
 /* Pinned: the fixed point solves the dense linear system (1 - alpha T) mu =
  * alpha E_pc; mu equals the reference's InvA (ComputeInvA, through the PolType
  * 3 fixtures) times E_pc, which pins the Thole dipole tensor; e_pol equals
- * -E_pc.InvA.E_pc / 2.  Parity unpinned: the charge-dipole damping of E_pc
- * (restated from src/DPP.cpp:1798-1804). */
+ * -E_pc.InvA.E_pc / 2; the damping factors equal the reference's own
+ * (or_damp).  The SCF's stop test (mean change < 1e-12) is restated from
+ * src/DPP.cpp:953-1006. */
 
 /* ---------------------------------------------------------------- a5, a6 */
 /* The electron-water potential at npts points, for the site/charge/dipole/

@@ -21,6 +21,9 @@
 //   ref_driver kin       <in> <out>   in: [type, n, para] (type 1: ho_dvr with
 //                                     frequency para; 20: cm_ham on [-para/2, para/2])
 //                                     out: [x(n)], [T full n*n]
+//   ref_driver damp      <in> <out>   in: [r], [alpha_i], [alpha_j], [a]
+//                                     out: ElectrostaticDCDamp, ElectrostaticDDDamp
+//                                     (src/DPP.cpp:1798-1812) per entry
 //   ref_driver input     <file> <out> the reference's TSIN + GetInputParameters +
 //                                     ReadWatersGroup on an input file
 //                                     out: [runtype, nWater, DVRType, Sampling,
@@ -193,6 +196,23 @@ static int cmd_davidson(const char *in, const char *out)
     return 0;
 }
 
+// defined in src/DPP.cpp (declared there, not in a header)
+double ElectrostaticDCDamp(double distance, double alphai, double alphaj, double a);
+double ElectrostaticDDDamp(double distance, double alphai, double alphaj, double a);
+
+static int cmd_damp(const char *in, const char *out)
+{
+    std::vector<rec> r = read_records(in);
+    const size_t n = r[0].size();
+    rec dc(n), dd(n);
+    for (size_t i = 0; i < n; ++i) {
+        dc[i] = ElectrostaticDCDamp(r[0][i], r[1][i], r[2][i], r[3][i]);
+        dd[i] = ElectrostaticDDDamp(r[0][i], r[1][i], r[2][i], r[3][i]);
+    }
+    write_records(out, {dc, dd});
+    return 0;
+}
+
 static int cmd_kin(const char *in, const char *out)
 {
     std::vector<rec> r = read_records(in);
@@ -236,6 +256,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[1], "davidson")) rc = cmd_davidson(argv[2], argv[3]);
         else if (!strcmp(argv[1], "input")) rc = cmd_input(argv[2], argv[3]);
         else if (!strcmp(argv[1], "kin")) rc = cmd_kin(argv[2], argv[3]);
+        else if (!strcmp(argv[1], "damp")) rc = cmd_damp(argv[2], argv[3]);
     } else {
         fprintf(stderr, "usage: ref_driver {tkin|potential|davidson} <in> <out>\n");
     }

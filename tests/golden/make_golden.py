@@ -48,6 +48,22 @@ def golden_tkin():
     np.savez_compressed(os.path.join(HERE, "tkin.npz"), ncases=len(cases), **out)
 
 
+def golden_damp():
+    """The water model's damping factors (src/DPP.cpp:1798-1812) from the
+    reference's own functions, over the distances and polarizabilities the
+    model meets."""
+    rng = np.random.default_rng(12)
+    a3 = ANGS2BOHR ** 3
+    al = np.array([1.22 * a3, 0.28 * a3])
+    n = 400
+    r = np.concatenate([rng.uniform(0.1, 12.0, n - 4), [1e-3, 0.5, 3.0, 30.0]])
+    ai = al[rng.integers(0, 2, n)]
+    aj = al[rng.integers(0, 2, n)]
+    a = rng.choice([0.21, 0.3], n)
+    dc, dd = run_ref("damp", [r, ai, aj, a])
+    np.savez_compressed(os.path.join(HERE, "damp.npz"), r=r, ai=ai, aj=aj, a=a, dc=dc, dd=dd)
+
+
 def golden_kin_types():
     """The harmonic-oscillator (DVRType 1) and Colbert-Miller (20) DVRs."""
     cases = [(1, 8, 0.05), (1, 17, 0.3), (1, 32, 0.02), (20, 8, 20.0), (20, 33, 36.0)]
@@ -298,6 +314,7 @@ def main():
     oracle.set_threads(1)
     golden_tkin()
     golden_kin_types()
+    golden_damp()
     golden_potential("c1", 6, 3.5, 1000, 36.0)
     golden_potential("c2", 24, 6.0, 2000, 40.0)
     golden_potential_pol3("c1", 6, 3.5, 1000, 36.0)

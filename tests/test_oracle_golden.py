@@ -48,3 +48,13 @@ def test_davidson_matches_reference(oracle_mod, tag):
     for i in range(p[0]):
         ov = abs(np.dot(r["vecs"][i], f["vecs"][i]))
         assert ov > 1 - 1e-8, (i, ov)
+
+
+def test_water_damping_matches_reference(oracle_mod):
+    """The water model's charge-dipole and dipole-dipole damping
+    (ElectrostaticDCDamp / ElectrostaticDDDamp, src/DPP.cpp:1798-1812),
+    evaluated by the reference's own functions."""
+    f = golden("damp.npz")
+    dc, dd = oracle_mod.damp(f["r"], f["ai"], f["aj"], f["a"])
+    np.testing.assert_allclose(dc, f["dc"], rtol=1e-14, atol=1e-300)
+    np.testing.assert_allclose(dd, f["dd"], rtol=1e-13, atol=1e-300)
