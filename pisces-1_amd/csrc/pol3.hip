@@ -188,10 +188,8 @@ void pol3_grid_launch(const Pol3Dev &D, int nx, int ny, int nz, const double *d_
     // speed: 16-point tiles (1) measured 1.27x / 1.16x faster than 32-point
     // ones at 216 / 405 polarizable dimensions, 64-point tiles slower still,
     // although each InvA fragment then feeds fewer matrix-core products.
-    static const int rb_env = [] {
-        const char *e = std::getenv("PISCES_POL3_RB");
-        return e ? std::atoi(e) : 0;
-    }();
+    const char *rb_e = std::getenv("PISCES_POL3_RB");  // read per build: not a hot path
+    const int rb_env = rb_e ? std::atoi(rb_e) : 0;
     const size_t row = (size_t)(D.n3p + 1) * sizeof(double);
     int rb = (rb_env == 1 || rb_env == 2 || rb_env == 4) ? rb_env : 1;
     while (rb > 1 && 16 * rb * row > (size_t)kLdsMax) rb /= 2;

@@ -127,3 +127,20 @@ def test_solve_on_poltype3_potential(pisces, oracle_mod):
         r = ctx.solve(1, start, maxsub=20, maxmacro=100, ptol=9, corrflag=1)
     assert ro["nconv"] == 1 and r["nconv"] == 1
     assert abs(r["evals"][0] - ro["evals"][0]) <= 1e-8
+
+
+@pytest.mark.parametrize("rb", ["1", "2", "4"])
+def test_tile_sizes(pisces, oracle_mod, rb, monkeypatch):
+    """16-, 32- and 64-point tiles (PISCES_POL3_RB) give the same potential."""
+    monkeypatch.setenv("PISCES_POL3_RB", rb)
+    f = golden("potential_pol3_c1.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    n, L = (32, 32, 32), (36.0, 36.0, 36.0)
+    g = oracle_mod.Grid(n, L)
+    flags, pp = f["flags_scp"], f["potpara_scp"]
+    Vo = cl.potential_grid_pol3(g, flags, pp)
+    alpha, ip, epc, e_pol = cl.pol3_lists()
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential_scp(flags, pp, cl.sites, cl.q, cl.iq, alpha, ip, epc, e_pol)
+        V = ctx.get_potential()
+    assert np.abs(V - Vo).max() <= 1e-12 * np.abs(Vo).max()
