@@ -7,8 +7,8 @@
 // x 3np times 3np x 3np) followed by a row-wise dot with E: FP64 matrix-core
 // work of 2 (3np)^2 flops per point.  A workgroup takes 16 RB points (RB = 1
 // by default): it builds their E rows in LDS (odd pitch: the 16 rows that one
-// matrix-core operand reads fall on distinct banks), then each of its four waves
-// takes 16-column blocks of InvA, streams them from L2 (InvA is shared by all
+// matrix-core operand reads fall on distinct banks), then each of its 8 or 16
+// waves takes 16-column blocks of InvA, streams them from L2 (InvA is shared by all
 // workgroups: at most 2.6 MB) as the B operand of v_mfma_f64_16x16x4 against
 // the 16-point row blocks of E, and folds each 16 x 16 result into the
 // per-point dot at once.
@@ -24,8 +24,7 @@ namespace pisces {
 
 namespace {
 
-constexpr int kP3Threads = 256;
-constexpr int kLdsMax = 160 * 1024 - (kP3Threads / 64) * 64 * (int)sizeof(double);
+constexpr int kLdsMax = 160 * 1024 - 16 * 64 * (int)sizeof(double);  // E rows + red (16 waves)
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -39,14 +38,14 @@ __device__ inline double reff3(double r, double r0)
     return r;
 }
 
-// RB 16-point row blocks per workgroup (kP3Pts = 16 RB grid points)
-template <int RB>
-__global__ void __launch_bounds__(kP3Threads)
+// RB 16-point row blocks per workgroup (kP3Pts = 16 RB grid points), NT threads
+template <int RB, int NT>
+__global__ void __launch_bounds__(NT)
 pol3_grid_kernel(Pol3Dev D, int nx, int ny, int nz, const double *__restrict__ gx,
                  const double *__restrict__ gy, const double *__restrict__ gz,
                  double *__restrict__ v)
 {
-    constexpr int kP3Pts = 16 * RB;
+    constexpr int kP3Pts = 16 * RB, kP3Threads = NT, NW = NT / 64;
     extern __shared__ double Es[];  // kP3Pts rows of n3p (+1) values
     __shared__ double red[kP3Threads / 64][kP3Pts];
     const int n3p = D.n3p, ldE = n3p + 1, np = D.np, n3 = 3 * np;
@@ -109,8 +108,8 @@ pol3_grid_kernel(Pol3Dev D, int nx, int ny, int nz, const double *__restrict__ g
         for (int r = 0; r < 4; ++r) sr[b][r] = 0.0;
     const double *ea = Es + l16 * ldE + lq;  // row block b: + 16 b ldE
     const int ncb = n3p / 16;
-    for (int j = 0; 4 * j < ncb; ++j) {
-        const int cb = (j & 1) ? 4 * j + 3 - wave : 4 * j + wave;
+    for (int j = 0; NW * j < ncb; ++j) {
+        const int cb = (j & 1) ? NW * j + NW - 1 - wave : NW * j + wave;
         if (cb >= ncb) continue;
         f64x4 o[RB], d[RB];
 #pragma unroll
@@ -176,12 +175,14 @@ void pol3_grid_launch(const Pol3Dev &D, int nx, int ny, int nz, const double *d_
     PC_REQUIRE(D.damp_type == 1 || D.damp_type == 2, PISCES_ERR_ARG, "damping type");
     static std::once_flag once;
     std::call_once(once, [] {
-        static_cast<void>(hipFuncSetAttribute(reinterpret_cast<const void *>(pol3_grid_kernel<2>),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
-        static_cast<void>(hipFuncSetAttribute(reinterpret_cast<const void *>(pol3_grid_kernel<4>),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
-        static_cast<void>(hipFuncSetAttribute(reinterpret_cast<const void *>(pol3_grid_kernel<1>),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
+        const void *ks[] = {reinterpret_cast<const void *>(pol3_grid_kernel<1, 256>),
+                            reinterpret_cast<const void *>(pol3_grid_kernel<2, 256>),
+                            reinterpret_cast<const void *>(pol3_grid_kernel<4, 256>),
+                            reinterpret_cast<const void *>(pol3_grid_kernel<1, 512>),
+                            reinterpret_cast<const void *>(pol3_grid_kernel<1, 1024>)};
+        for (const void *k : ks)
+            static_cast<void>(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  kLdsMax));
     });
     // RB 16-point row blocks per workgroup (PISCES_POL3_RB = 1, 2 or 4).  The E
     // rows in LDS bound how many workgroups a CU holds, and that decides the
@@ -190,6 +191,14 @@ void pol3_grid_launch(const Pol3Dev &D, int nx, int ny, int nz, const double *d_
     // although each InvA fragment then feeds fewer matrix-core products.
     const char *rb_e = std::getenv("PISCES_POL3_RB");  // read per build: not a hot path
     const int rb_env = rb_e ? std::atoi(rb_e) : 0;
+    // threads per workgroup (PISCES_POL3_NT = 256, 512 or 1024): the waves of a
+    // workgroup share its column blocks, and more resident waves hide the L2
+    // latency of the InvA operand.  Measured at 16-point tiles: 256 / 512 /
+    // 1024 threads 0.578 / 0.419 / 0.487 ms at 216 dimensions (14 blocks),
+    // 8.74 / 4.66 / 4.44 ms at 405 (26 blocks): 1024 from 24 blocks on.
+    const char *nt_e = std::getenv("PISCES_POL3_NT");
+    const int nt_v = nt_e ? std::atoi(nt_e) : (D.n3p / 16 >= 24 ? 1024 : 512);
+    const int nt = (nt_v == 256 || nt_v == 1024) ? nt_v : 512;
     const size_t row = (size_t)(D.n3p + 1) * sizeof(double);
     int rb = (rb_env == 1 || rb_env == 2 || rb_env == 4) ? rb_env : 1;
     while (rb > 1 && 16 * rb * row > (size_t)kLdsMax) rb /= 2;
@@ -197,14 +206,20 @@ void pol3_grid_launch(const Pol3Dev &D, int nx, int ny, int nz, const double *d_
     const int pts = 16 * rb;
     const unsigned blocks = (unsigned)((ng + pts - 1) / pts);
     const size_t lds = (size_t)pts * row;
-    if (rb == 4)
-        hipLaunchKernelGGL(pol3_grid_kernel<4>, dim3(blocks), dim3(kP3Threads), lds, s, D, nx, ny,
+    if (rb == 1 && nt == 1024)
+        hipLaunchKernelGGL((pol3_grid_kernel<1, 1024>), dim3(blocks), dim3(1024), lds, s, D, nx, ny,
+                           nz, d_x, d_y, d_z, d_v);
+    else if (rb == 1 && nt == 512)
+        hipLaunchKernelGGL((pol3_grid_kernel<1, 512>), dim3(blocks), dim3(512), lds, s, D, nx, ny,
+                           nz, d_x, d_y, d_z, d_v);
+    else if (rb == 4)
+        hipLaunchKernelGGL((pol3_grid_kernel<4, 256>), dim3(blocks), dim3(256), lds, s, D, nx, ny,
                            nz, d_x, d_y, d_z, d_v);
     else if (rb == 2)
-        hipLaunchKernelGGL(pol3_grid_kernel<2>, dim3(blocks), dim3(kP3Threads), lds, s, D, nx, ny,
+        hipLaunchKernelGGL((pol3_grid_kernel<2, 256>), dim3(blocks), dim3(256), lds, s, D, nx, ny,
                            nz, d_x, d_y, d_z, d_v);
     else
-        hipLaunchKernelGGL(pol3_grid_kernel<1>, dim3(blocks), dim3(kP3Threads), lds, s, D, nx, ny,
+        hipLaunchKernelGGL((pol3_grid_kernel<1, 256>), dim3(blocks), dim3(256), lds, s, D, nx, ny,
                            nz, d_x, d_y, d_z, d_v);
     PC_HIP(hipGetLastError());
 }

@@ -144,3 +144,25 @@ def test_tile_sizes(pisces, oracle_mod, rb, monkeypatch):
         ctx.build_potential_scp(flags, pp, cl.sites, cl.q, cl.iq, alpha, ip, epc, e_pol)
         V = ctx.get_potential()
     assert np.abs(V - Vo).max() <= 1e-12 * np.abs(Vo).max()
+
+
+@pytest.mark.parametrize("nt", ["256", "512", "1024"])
+def test_thread_counts(pisces, oracle_mod, nt, monkeypatch):
+    """Workgroups of 256, 512 and 1024 threads (PISCES_POL3_NT) on the 64^3
+    (H2O)24 grid (14 column blocks: 1024 threads leave waves without one)."""
+    monkeypatch.setenv("PISCES_POL3_NT", nt)
+    f = golden("potential_pol3_c2.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    n, L = (64, 64, 64), (40.0, 40.0, 40.0)
+    g = oracle_mod.Grid(n, L)
+    flags, pp = f["flags_scp"], f["potpara_scp"]
+    alpha, ip, epc, e_pol = cl.pol3_lists()
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential_scp(flags, pp, cl.sites, cl.q, cl.iq, alpha, ip, epc, e_pol)
+        V = ctx.get_potential()
+    rng = np.random.default_rng(5)
+    idx = rng.choice(g.ngp, size=2048, replace=False)
+    i, j, k = idx % 64, (idx // 64) % 64, idx // 4096
+    pts = np.stack([g.x[0][i], g.x[1][j], g.x[2][k]], axis=1)
+    Vo, _ = cl.potential_points_pol3(pts, flags, pp)
+    assert np.abs(V[idx] - Vo).max() <= 1e-12 * np.abs(Vo).max()
