@@ -609,7 +609,7 @@ __device__ inline int rr_index(int i, int r, int mp)
     return (x >= mp - 1 ? x - (mp - 1) : x) + 1;
 }
 
-constexpr int kEigT = 512;  // more lanes per round: measured 10% faster than 256 at m = 28
+constexpr int kEigT = 512;  // measured at m = 28: 10% faster than 256 lanes, as fast as 1024
 
 constexpr int kEigSweeps = 40;
 
