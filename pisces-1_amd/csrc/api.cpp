@@ -313,6 +313,11 @@ int pisces_build_potential_scp(pisces_ctx *c, const int flags[6], const double *
         return PISCES_ERR_ARG;
     if (flags[3] != 3) return PISCES_ERR_ARG;
     if (!lists_ok(nq, iq, nsites) || !lists_ok(np, ip, nsites)) return PISCES_ERR_ARG;
+    for (int i = 0; i < np; ++i)
+        if (!(alpha[i] > 0.0)) {
+            set_last_error("polarizabilities must be positive");
+            return PISCES_ERR_ARG;
+        }
     // the water model's point dipoles are not passed: PolType 3 ignores them
     // (src/Potential.cpp:327-335)
     return build_potential_impl(c, flags, potpara, nsites, sites, nq, q, iq, 0, nullptr, nullptr,

@@ -102,6 +102,11 @@ def test_scp_argument_errors(pisces, oracle_mod):
                                     epc, e_pol)
         with pytest.raises(pisces.PiscesError):  # PolType 3 without the site lists
             ctx.build_potential((3, 2, 0, 3, 0, 0), pp, cl.sites, cl.q, cl.iq, cl.d, cl.id)
+        zero = np.array(alpha)
+        zero[1] = 0.0
+        with pytest.raises(pisces.PiscesError, match="positive"):
+            ctx.build_potential_scp((3, 2, 0, 3, 0, 0), pp, cl.sites, cl.q, cl.iq, zero, ip,
+                                    epc, e_pol)
         bad = np.array(ip)
         bad[0] = 10 ** 6
         with pytest.raises(pisces.PiscesError):
