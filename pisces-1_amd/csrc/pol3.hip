@@ -119,7 +119,7 @@ pol3_grid_kernel(Pol3Dev D, int nx, int ny, int nz, const double *__restrict__ g
         }
         const gptr<const double> bc = A + (long long)lq * n3p + 16 * cb + l16;
         const int kd = 16 * cb;
-#pragma unroll 4
+#pragma unroll 8
         for (int k0 = 0; k0 < kd; k0 += 4) {
             const double bv = bc[(long long)k0 * n3p];  // InvA[k0 + lq][16 cb + l16]
 #pragma unroll
