@@ -51,3 +51,31 @@ def test_single_point_fft_kinetic(tmp_path, oracle_mod):
     ro = g.solve(cl.potential_grid(g), 1, 20, 200, 9, 1, g.piab().reshape(1, -1))
     assert ro["nconv"] == 1
     assert abs(ebe - float(ro["evals"][0])) <= 1e-8
+
+
+def test_single_point_poltype3(tmp_path, oracle_mod):
+    """Polarization = 3 in the input file: self-consistent polarizable sites
+    through the front-end against the oracle's PolType 3 potential and Davidson."""
+    ref = golden("davidson_c1.npz")
+    xyz = ref["xyz"]
+    lines = ["BeginJob", "  runtype = 1,", "EndJob", "BeginElectronPotential",
+             "  Polarization = 3,", "EndElectronPotential", "BeginGridDef",
+             "  NoOfGridPoints = 24, 24, 24,", "  Length = 36.0, 36.0, 36.0,", "EndGridDef",
+             "BeginDiag", "  nStates = 1,", "  maxSubspace = 20,", "  maxIter = 200,",
+             "  pTol = 9,", "EndDiag", "BeginWaters"]
+    for w in xyz.reshape(-1, 9):
+        for a, sym in enumerate(("O", "H", "H")):
+            lines.append("%s %.17g %.17g %.17g" % ((sym,) + tuple(w[3 * a:3 * a + 3])))
+    lines.append("EndWaters")
+    path = tmp_path / "pol3.inp"
+    path.write_text("\n".join(lines) + "\n")
+    r = subprocess.run([BIN, str(path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ebe = float(re.search(r"EBE\s+=\s+(\S+) meV", r.stdout).group(1)) / AU2MEV
+    cl = oracle_mod.Cluster(xyz)
+    g = oracle_mod.Grid((24, 24, 24), (36.0, 36.0, 36.0))
+    V = cl.potential_grid_pol3(g, (3, 2, 0, 3, 0, 0))
+    ro = g.solve(V, 1, 20, 200, 9, 1, g.piab().reshape(1, -1))
+    assert ro["nconv"] == 1
+    # the GPU water model's SCF and the oracle's stop at a mean change < 1e-12
+    assert abs(ebe - float(ro["evals"][0])) <= 1e-8
