@@ -247,8 +247,19 @@ void water_lists_host(int nw, std::vector<double> &q, std::vector<int> &iq, std:
 // blocks and -T_ij off the diagonal, T_ij the Thole-damped dipole tensor of
 // CalcDDTensor (T = 3 f5 R R^T / r^5 - f3 I / r^3, x = a r^3 / sqrt(alpha_i
 // alpha_j), f3 = 1 - e^-x, f5 = 1 - (1 + x) e^-x, a = 0.3) for every pair; then
-// its inverse (the reference: dsytrf/dsytri; here Gauss-Jordan elimination with
-// partial pivoting on [A | I]).
+// its inverse (the reference: dsytrf/dsytri; here by Cholesky where A is
+// positive definite, else Gauss-Jordan elimination with partial pivoting).
+// sum_{k < n} a[k] b[k] in eight interleaved partial sums (independent chains)
+static double dot8(const double *a, const double *b, int n)
+{
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int k = 0;
+    for (; k + 8 <= n; k += 8)
+        for (int u = 0; u < 8; ++u) s[u] += a[k + u] * b[k + u];
+    for (; k < n; ++k) s[0] += a[k] * b[k];
+    return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
 bool pol3_inva_host(int np, const double *rpol, const double *alpha, int n3p, double *inva)
 {
     const int n = 3 * np;
@@ -273,6 +284,53 @@ bool pol3_inva_host(int np, const double *rpol, const double *alpha, int n3p, do
                     A[(size_t)n * (3 * j + w) + 3 * i + u] = -t;
                 }
         }
+    // symmetric positive definite (any physical set of sites): Cholesky A = L L^T,
+    // M = L^-1, InvA = M^T M -- about n^3 / 2 multiply-adds, every inner loop a
+    // dot product of contiguous rows
+    {
+        std::vector<double> L((size_t)n * n, 0.0), Mt((size_t)n * n, 0.0);
+        bool spd = true;
+        for (int j = 0; j < n && spd; ++j) {
+            const double *lj = &L[(size_t)n * j];
+            const double d = A[(size_t)n * j + j] - dot8(lj, lj, j);
+            if (!(d > 0.0)) {
+                spd = false;
+                break;
+            }
+            const double ljj = std::sqrt(d), inv = 1.0 / ljj;
+            L[(size_t)n * j + j] = ljj;
+            for (int i = j + 1; i < n; ++i) {
+                const double *li = &L[(size_t)n * i];
+                L[(size_t)n * i + j] = (A[(size_t)n * i + j] - dot8(li, lj, j)) * inv;
+            }
+        }
+        if (spd) {
+            // Mt = (L^-1)^T, upper triangular: column j of L^-1 solves L x = e_j
+            std::vector<double> x(n);
+            for (int j = 0; j < n; ++j) {
+                for (int i = 0; i < j; ++i) x[i] = 0.0;
+                for (int i = j; i < n; ++i) {
+                    const double *li = &L[(size_t)n * i];
+                    x[i] = ((i == j ? 1.0 : 0.0) - dot8(li + j, &x[j], i - j)) / li[i];
+                }
+                for (int i = j; i < n; ++i) Mt[(size_t)n * j + i] = x[i];
+            }
+            // InvA[a][b] = sum_{k >= max(a, b)} M[k][a] M[k][b] = Mt[a][.] . Mt[b][.]
+            std::fill(inva, inva + (size_t)n3p * n3p, 0.0);
+            for (int a = 0; a < n; ++a) {
+                const double *ra = &Mt[(size_t)n * a];
+                for (int b = 0; b <= a; ++b) {
+                    const double *rb = &Mt[(size_t)n * b];
+                    const double t = dot8(ra + a, rb + a, n - a);
+                    inva[(size_t)n3p * a + b] = inva[(size_t)n3p * b + a] = t;
+                }
+            }
+            for (size_t k = 0; k < (size_t)n3p * n3p; ++k)
+                if (!std::isfinite(inva[k])) return false;
+            return true;
+        }
+    }
+    // otherwise Gauss-Jordan elimination with partial pivoting on [A | I]
     for (int k = 0; k < n; ++k) X[(size_t)n * k + k] = 1.0;
     for (int c = 0; c < n; ++c) {
         int piv = c;
