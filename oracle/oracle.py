@@ -153,6 +153,15 @@ def water_induced(sites):
     return dict(mu=mu, epc=epc, e_pol=ep.value, iters=it.value, converged=(rc == 0))
 
 
+def water_energy(sites, mu):
+    """Neutral-cluster energy: (E0, elec, pol, disp) in hartree."""
+    nw = sites.shape[0] // 4
+    out = np.zeros(4)
+    lib().or_water_energy(nw, _p(np.ascontiguousarray(sites, dtype=np.float64)),
+                          _p(np.ascontiguousarray(mu, dtype=np.float64)), _p(out))
+    return out
+
+
 def damp(r, ai, aj, a):
     """Charge-dipole and dipole-dipole damping factors of the water model."""
     r, ai, aj, a = (np.ascontiguousarray(v, dtype=np.float64) for v in (r, ai, aj, a))
@@ -198,6 +207,7 @@ class Cluster:
         ind = water_induced(self.sites)
         self.mu, self.epc, self.e_pol = ind["mu"], ind["epc"], ind["e_pol"]
         self.scf_iters = ind["iters"]
+        self.e0, self.e_elec, _, self.e_disp = water_energy(self.sites, self.mu)
         self.q, self.iq, self.d, self.id = water_lists(self.sites, self.mu)
         if ions is None:
             self.ion_xyz = np.zeros((0, 3))

@@ -437,6 +437,117 @@ int or_water_induced(int nw, const double *sites, double *mu, double *epc,
     return conv ? 0 : 1;
 }
 
+/* Pair terms of the neutral water cluster, src/DPP.cpp:1820-2024, 2256-2303.
+ * WaterWaterElectrostaticEnergy: charge-charge, undamped. */
+static double ww_elec(const double *s1, const double *s2)
+{
+    double e = 0;
+    for (int i = 0; i < 4; ++i) {
+        double q1, g1, a1;
+        site_params(i, &q1, &g1, &a1);
+        if (q1 == 0) continue;
+        for (int j = 0; j < 4; ++j) {
+            double q2, g2, a2;
+            site_params(j, &q2, &g2, &a2);
+            if (q2 == 0) continue;
+            double dx = s1[3 * i] - s2[3 * j], dy = s1[3 * i + 1] - s2[3 * j + 1],
+                   dz = s1[3 * i + 2] - s2[3 * j + 2];
+            e += q1 * q2 / sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    }
+    return e;
+}
+
+/* WaterWaterPolarizationEnergy: induced dipole (O,H1,H2) in the damped field
+ * of the other water's charges, halved; m1, m2 the dipoles of the two waters */
+static double ww_pol(const double *s1, const double *s2, const double *m1, const double *m2)
+{
+    double e = 0;
+    for (int k = 0; k < 3; ++k) {
+        double q, g, a;
+        site_params(k, &q, &g, &a);
+        const double *x1 = s1 + 3 * k, *x2 = s2 + 3 * k, *d1 = m1 + 3 * k, *d2 = m2 + 3 * k;
+        for (int j = 0; j < 4; ++j) { /* W2 polarizing W1 */
+            double q2, g2, a2;
+            site_params(j, &q2, &g2, &a2);
+            if (q2 == 0) continue;
+            double rx = x1[0] - s2[3 * j], ry = x1[1] - s2[3 * j + 1], rz = x1[2] - s2[3 * j + 2];
+            double dist = sqrt(rx * rx + ry * ry + rz * rz);
+            double dmp = damp_cd(dist, a, g2, OR_DAMP_CD);
+            e -= q2 * (d1[0] * rx + d1[1] * ry + d1[2] * rz) * dmp *
+                 pow(rx * rx + ry * ry + rz * rz, -1.5) / 2.0;
+        }
+        for (int j = 0; j < 4; ++j) { /* W1 polarizing W2 */
+            double q1, g1, a1;
+            site_params(j, &q1, &g1, &a1);
+            if (q1 == 0) continue;
+            double rx = x2[0] - s1[3 * j], ry = x2[1] - s1[3 * j + 1], rz = x2[2] - s1[3 * j + 2];
+            double dist = sqrt(rx * rx + ry * ry + rz * rz);
+            double dmp = damp_cd(dist, a, g1, OR_DAMP_CD);
+            e -= q1 * (d2[0] * rx + d2[1] * ry + d2[2] * rz) * dmp *
+                 pow(rx * rx + ry * ry + rz * rz, -1.5) / 2.0;
+        }
+    }
+    return e;
+}
+
+/* WaterWaterDispersionEnergy: O-O exponential repulsion plus Tang-Toennies
+ * damped C6, H-H and O-H exponential repulsion (DPP parameters of
+ * src/ttm_data.h:76-119, Water::SetupCharge case 2, src/Water.cpp:225-235) */
+#define OR_AU2KCAL (6.02214179e23 * 4.35974394e-18 / 4.184 / 1000)
+static double ww_disp(const double *s1, const double *s2)
+{
+    const double a2b = OR_ANGS2BOHR;
+    const double delta = 2.23 / a2b, c6 = -1300.0000000 / OR_AU2KCAL * pow(a2b, 6);
+    const double A1 = 115271.77066806905 / OR_AU2KCAL, b1 = 4.167868773196562 / a2b;
+    const double A2 = 119085.35806729361 / OR_AU2KCAL, b2 = 5.581698710292979 / a2b;
+    const double A3 = 1803.662375255084 / OR_AU2KCAL, b3 = 3.5354602029164868 / a2b;
+    double e = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double dx = s1[3 * i] - s2[3 * j], dy = s1[3 * i + 1] - s2[3 * j + 1],
+                   dz = s1[3 * i + 2] - s2[3 * j + 2];
+            double R2 = dx * dx + dy * dy + dz * dz;
+            double R = sqrt(R2);
+            if (i == 0 && j == 0) {
+                double R6 = R2 * R2 * R2, dR = delta * R;
+                double damp = 1 - (1.0 + dR + pow(dR, 2) / 2.0 + pow(dR, 3) / 6.0 +
+                                   pow(dR, 4) / 24.0 + pow(dR, 5) / 120.0 + pow(dR, 6) / 720.0) *
+                                      exp(-dR);
+                e = e + A1 * exp(-b1 * R) + damp * c6 / R6;
+            } else if (i != 0 && j != 0) {
+                e = e + A2 * exp(-b2 * R);
+            } else {
+                e = e + A3 * exp(-b3 * R);
+            }
+        }
+    return e;
+}
+
+int or_water_energy(int nw, const double *sites, const double *mu, double *out)
+{
+    /* WaterCluster::CalcIntermolecularPotential, src/DPP.cpp:1648-1700: the
+     * pair loop i < j, E0 = sum(elec + pol) + sum(disp) */
+    double nre = 0, lj = 0, el = 0, pol = 0, dis = 0;
+    for (int i = 0; i < nw - 1; ++i)
+        for (int j = i + 1; j < nw; ++j) {
+            const double *s1 = sites + 12 * i, *s2 = sites + 12 * j;
+            double e = ww_elec(s1, s2);
+            double p = ww_pol(s1, s2, mu + 9 * i, mu + 9 * j);
+            double d = ww_disp(s1, s2);
+            el += e;
+            nre += e + p;
+            lj += d;
+            pol += p;
+            dis += d;
+        }
+    out[0] = nre + lj;
+    out[1] = el;
+    out[2] = pol;
+    out[3] = dis;
+    return 0;
+}
+
 int or_water_lists(int nw, const double *sites, const double *mu, double *q,
                    int *iq, double *d, int *id)
 {

@@ -168,6 +168,14 @@ int or_potential_grid_pol3(const int *flags, const double *potpara, int nsites,
  * (GetLists, src/DPP.cpp:402-498; ReportInducedDipoles :505-529):
  *   q[3*nw], iq[3*nw]  charges on H1,H2,M;  d[9*nw], id[3*nw] induced dipoles
  *   on O,H1,H2.  Returns the number of charges (3*nw). */
+/* Energy of the neutral cluster, WaterCluster::CalcIntermolecularPotential
+ * (src/DPP.cpp:1648-1700) with the pair terms WaterWaterElectrostaticEnergy,
+ * WaterWaterPolarizationEnergy and WaterWaterDispersionEnergy
+ * (src/DPP.cpp:1820-2024, 2256-2303).  sites 12*nw bohr, mu 9*nw (the
+ * induced dipoles of O,H1,H2 from or_water_induced).
+ * out[4]: E0 (= elec + pol + disp), elec, pol, disp (hartree). */
+int or_water_energy(int nw, const double *sites, const double *mu, double *out);
+
 int or_water_lists(int nw, const double *sites, const double *mu, double *q,
                    int *iq, double *d, int *id);
 /* Parity unpinned beyond charge neutrality of each water: the fixtures pass

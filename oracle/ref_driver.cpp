@@ -30,6 +30,17 @@
 //                                     DiagMethod, nStates, maxSub, maxIter, ptol,
 //                                     istartvec], PotFlag[6], PotPara[11],
 //                                     ngrid[3], gpara[3], water coordinates
+//   ref_driver water     <in> <out>   in: [nW, CenterFlag, KTFlag], [O,H1,H2 xyz in
+//                                     Angstrom, 9*nW]
+//                                     the water model exactly as ClusterAnion runs it
+//                                     (src/ClusterAnion.cpp:59-66, 165-197):
+//                                     SetStructure -> GetConfiguration ->
+//                                     SetConfiguration -> CalcEnergy -> GetLists ->
+//                                     ReportInducedDipoles -> ReportPolSitesAndField
+//                                     out: conf[6nW], sites[12nW], q, iq, d (one per
+//                                     water), id, all dipoles[9nW], their sites,
+//                                     alpha[3nW], ipp[3nW], Epc[9nW],
+//                                     [E0, PolarizationEnergy]
 #include <mpi.h>
 
 #include <cmath>
@@ -37,6 +48,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iostream>
 #include <vector>
 
 #include "vecdefs.h"
@@ -48,6 +60,11 @@
 #include "cm_dvr.h"
 #include "Parameters.h"
 #include "GetInput.h"
+#include "GTO.h"
+#include "MO.h"
+#include "AtomCenter.h"
+#include "Water.h"
+#include "DPP.h"
 
 typedef std::vector<double> rec;
 
@@ -246,6 +263,38 @@ static int cmd_input(const char *in, const char *out)
     return 0;
 }
 
+static rec as_rec(const int *v, int n) { return rec(v, v + n); }
+
+static int cmd_water(const char *in, const char *out)
+{
+    std::vector<rec> r = read_records(in);
+    const int nw = (int)r[0][0], center = (int)r[0][1], kt = (int)r[0][2];
+    const rec &xyz = r[1];
+    WaterCluster W;
+    W.SetStructure(nw, xyz.data(), 1, center, kt, 0);
+    rec conf(6 * nw);
+    W.GetConfiguration(nw, conf.data(), 1.0, 0);
+    W.SetConfiguration(nw, conf.data(), 1, 0);
+    const double e0 = W.CalcEnergy(0);
+    const double epol = W.ReportPolarization();
+    int ns = 0, nq = 0, nd = 0;
+    W.ReportNoOfSites(ns, nq, nd);
+    const int np = W.ReportNoOfPolSites();
+    rec sites(3 * ns), q(nq), d(3 * nd), dmu(3 * ns * 3 * nd + 1, 0.0);
+    std::vector<int> iq(nq), id(nd);
+    W.GetLists(ns, sites.data(), nq, q.data(), iq.data(), nd, d.data(), id.data(), dmu.data(), 0);
+    rec dall(9 * nd);
+    std::vector<int> idall(3 * nd);
+    W.ReportInducedDipoles(3 * nd, dall.data(), idall.data(), 0);
+    rec alpha(np), epc(3 * np);
+    std::vector<int> ipp(np);
+    W.ReportPolSitesAndField(np, alpha.data(), ipp.data(), epc.data(), 0);
+    write_records(out, {conf, sites, q, as_rec(iq.data(), nq), d, as_rec(id.data(), nd), dall,
+                        as_rec(idall.data(), 3 * nd), alpha, as_rec(ipp.data(), np), epc,
+                        rec{e0, epol}});
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     MPI_Init(&argc, &argv);
@@ -257,6 +306,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[1], "input")) rc = cmd_input(argv[2], argv[3]);
         else if (!strcmp(argv[1], "kin")) rc = cmd_kin(argv[2], argv[3]);
         else if (!strcmp(argv[1], "damp")) rc = cmd_damp(argv[2], argv[3]);
+        else if (!strcmp(argv[1], "water")) rc = cmd_water(argv[2], argv[3]);
     } else {
         fprintf(stderr, "usage: ref_driver {tkin|potential|davidson} <in> <out>\n");
     }

@@ -8,7 +8,8 @@ outputs as small .npz files.  Run in the build container:
 
 The fixtures are data only (inputs and the reference's outputs).  The site,
 charge and dipole lists fed to the reference Potential come from the oracle's
-water model (the reference's Water.cpp does not compile in this image).
+water model, which water_*.npz pins to the reference's own WaterCluster
+(src/DPP.cpp, src/Water.cpp).
 """
 import os
 import subprocess
@@ -157,6 +158,33 @@ def golden_potential_pol3(tag, nw, radius, seed, box):
         out["potpara_" + name] = pp
         out["E_" + name] = e.reshape(-1, 5)
     np.savez_compressed(os.path.join(HERE, "potential_pol3_%s.npz" % tag), **out)
+
+
+def golden_water(tag, xyz):
+    """The reference's water model (WaterCluster, src/DPP.cpp; Water,
+    src/Water.cpp) on one geometry, in the order ClusterAnion runs it:
+    projection to rigid monomers, the induced-dipole SCF, the neutral-cluster
+    energy and the lists handed to the electron's Potential."""
+    xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 9)
+    nw = xyz.shape[0]
+    (conf, sites, q, iq, d, idd, dall, idall, alpha, ipp, epc, en) = run_ref(
+        "water", [[nw, 0, 0], xyz.reshape(-1)])
+    np.savez_compressed(
+        os.path.join(HERE, "water_%s.npz" % tag), xyz=xyz, conf=conf.reshape(nw, 6),
+        sites=sites.reshape(-1, 3), q=q, iq=iq.astype(np.int32), d_water=d.reshape(nw, 3),
+        id_water=idd.astype(np.int32), mu=dall, id_mu=idall.astype(np.int32), alpha=alpha,
+        ipp=ipp.astype(np.int32), epc=epc, e0=en[0], e_pol=en[1])
+
+
+def golden_waters():
+    for tag, (nw, radius, seed) in (("c1", (6, 3.5, 1000)), ("c2", (24, 6.0, 2000)),
+                                    ("c3", (45, 8.0, 3000))):
+        golden_water(tag, workload.water_cluster(nw, radius, seed)[0])
+    # non-rigid input: every atom of the C1 geometry moved by 0.03 A (Gaussian),
+    # so the projection onto rigid monomers (Water::Reposition) does real work
+    xyz, _ = workload.water_cluster(6, 3.5, 1000)
+    rng = np.random.default_rng(4242)
+    golden_water("nonrigid", xyz + rng.normal(scale=0.03, size=xyz.shape))
 
 
 def _waters_block(xyz):
@@ -315,6 +343,7 @@ def main():
     golden_tkin()
     golden_kin_types()
     golden_damp()
+    golden_waters()
     golden_potential("c1", 6, 3.5, 1000, 36.0)
     golden_potential("c2", 24, 6.0, 2000, 40.0)
     golden_potential_pol3("c1", 6, 3.5, 1000, 36.0)
