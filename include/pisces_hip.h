@@ -167,6 +167,16 @@ int pisces_build_cluster_potential(pisces_ctx *ctx, int nw, const double *xyz_an
                                    const double *ion_xyz_bohr, const double *ion_q,
                                    double ion_r, double *sites, double *mu);
 
+/* Energy of the neutral water cluster, E0 of the reference's energy block
+ * (WaterCluster::CalcEnergy -> CalcIntermolecularPotential, src/DPP.cpp:
+ * 251-262, 1648-1700; printed as Eneutral, src/ClusterAnion.cpp:306-311):
+ * the water-water electrostatic, polarization and DPP repulsion/dispersion
+ * terms summed over pairs.  sites: 12*nw (bohr, O,H1,H2,M per water) and mu:
+ * 9*nw (induced dipoles of O,H1,H2), as pisces_build_cluster_potential returns
+ * them.  energies[4]: E0, electrostatic, polarization, dispersion (hartree).
+ * Host computation; no GPU is used. */
+int pisces_water_energy(int nw, const double *sites, const double *mu, double *energies);
+
 /* Y[:,v] = H X[:,v] for v < nvec, host buffers (ngp*nvec doubles each).
  * Replaces DVR::MatrixTimesVector (src/mtx.cpp:22-108) called once per vector
  * by davdriver (src/davdriver.cpp:67-78); the context's potential is used. */

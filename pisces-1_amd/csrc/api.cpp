@@ -387,6 +387,13 @@ int pisces_build_cluster_potential(pisces_ctx *c, int nw, const double *xyz_ang,
     });
 }
 
+int pisces_water_energy(int nw, const double *sites, const double *mu, double *energies)
+{
+    if (nw < 0 || (nw > 0 && (!sites || !mu)) || !energies) return PISCES_ERR_ARG;
+    water_energy_host(nw, sites, mu, energies);
+    return PISCES_OK;
+}
+
 int pisces_apply_h(pisces_ctx *c, const double *X, double *Y, int nvec)
 {
     if (!c || !X || !Y || nvec <= 0) return PISCES_ERR_ARG;

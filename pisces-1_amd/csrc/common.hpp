@@ -189,6 +189,9 @@ struct ScfTask {
 };
 void water_scf_batched_launch(const ScfTask *d_tasks, int ntasks, hipStream_t s);
 int water_scf_max_waters();
+// Host: energy of the neutral cluster (E0, electrostatic, polarization,
+// dispersion) from the sites (12*nw, bohr) and the induced dipoles (9*nw).
+void water_energy_host(int nw, const double *sites, const double *mu, double out[4]);
 // Charges of the DPP monomer (H1, H2, M) and polarizabilities (O, H1, H2).
 void water_lists_host(int nw, std::vector<double> &q, std::vector<int> &iq,
                       std::vector<int> &id);

@@ -230,6 +230,85 @@ bool water_sites_host(int nw, const double *xyz_ang, double *sites)
 
 // GetLists / ReportInducedDipoles ordering (src/DPP.cpp:402-529): charges on
 // H1, H2, M of each water; induced dipoles on O, H1, H2.
+// Energy of the neutral water cluster, WaterCluster::CalcIntermolecularPotential
+// (src/DPP.cpp:1648-1700): over every pair of waters the charge-charge
+// electrostatics (WaterWaterElectrostaticEnergy, :1820-1870), the induced
+// dipoles of O,H1,H2 in the other water's damped charge field, halved
+// (WaterWaterPolarizationEnergy, :1935-2024, damping a = 0.21), and the DPP
+// exponential repulsion with the Tang-Toennies damped O-O C6 dispersion
+// (WaterWaterDispersionEnergy, :2256-2303; src/ttm_data.h:76-119,
+// src/Water.cpp:225-235).  out: E0, electrostatic, polarization, dispersion.
+void water_energy_host(int nw, const double *sites, const double *mu, double out[4])
+{
+    const double a2b = 1.0 / kBohr2Angs, a3 = a2b * a2b * a2b;
+    const double kcal = 6.02214179e23 * 4.35974394e-18 / 4.184 / 1000;  // hartree in kcal/mol
+    const double qh = 0.5742185854046506;
+    // per site O, H1, H2, M: charge, damping polarizability, polarizability
+    const double q[4] = {0.0, qh, qh, -2.0 * qh};
+    const double sg[4] = {0.0, 0.28 * a3, 0.28 * a3, 1.22 * a3};
+    const double al[3] = {1.22 * a3, 0.28 * a3, 0.28 * a3};
+    const double delta = 2.23 / a2b, c6 = -1300.0 / kcal * std::pow(a2b, 6);
+    // repulsion (A, b) of the O-O, H-H and O-H pairs
+    const double rep[3][2] = {{115271.77066806905 / kcal, 4.167868773196562 / a2b},
+                              {119085.35806729361 / kcal, 5.581698710292979 / a2b},
+                              {1803.662375255084 / kcal, 3.5354602029164868 / a2b}};
+    auto dist = [](const double *a, const double *b, double d[3]) {
+        d[0] = a[0] - b[0];
+        d[1] = a[1] - b[1];
+        d[2] = a[2] - b[2];
+        return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    };
+    auto damp_cd = [](double r, double ai, double aj) {
+        const double A = std::pow(ai * aj, 1.0 / 6.0);
+        return 1.0 - std::exp(-0.21 * std::pow(r / A, 3));
+    };
+    double e_es = 0, e_pol = 0, e_disp = 0, nre = 0, lj = 0;
+    for (int i = 0; i + 1 < nw; ++i)
+        for (int j = i + 1; j < nw; ++j) {
+            const double *s1 = sites + 12 * i, *s2 = sites + 12 * j;
+            double d[3], es = 0, pol = 0, disp = 0;
+            for (int a = 1; a < 4; ++a)
+                for (int b = 1; b < 4; ++b) es += q[a] * q[b] / std::sqrt(dist(s1 + 3 * a, s2 + 3 * b, d));
+            for (int k = 0; k < 3; ++k) {
+                const double *m1 = mu + 9 * i + 3 * k, *m2 = mu + 9 * j + 3 * k;
+                for (int b = 1; b < 4; ++b) {  // charges of water j at dipole k of water i
+                    const double r2 = dist(s1 + 3 * k, s2 + 3 * b, d), r = std::sqrt(r2);
+                    pol -= q[b] * (m1[0] * d[0] + m1[1] * d[1] + m1[2] * d[2]) *
+                           damp_cd(r, al[k], sg[b]) * std::pow(r2, -1.5) / 2.0;
+                }
+                for (int b = 1; b < 4; ++b) {  // charges of water i at dipole k of water j
+                    const double r2 = dist(s2 + 3 * k, s1 + 3 * b, d), r = std::sqrt(r2);
+                    pol -= q[b] * (m2[0] * d[0] + m2[1] * d[1] + m2[2] * d[2]) *
+                           damp_cd(r, al[k], sg[b]) * std::pow(r2, -1.5) / 2.0;
+                }
+            }
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double r2 = dist(s1 + 3 * a, s2 + 3 * b, d), r = std::sqrt(r2);
+                    if (a == 0 && b == 0) {
+                        const double x = delta * r;
+                        const double tt = 1.0 + x + std::pow(x, 2) / 2.0 + std::pow(x, 3) / 6.0 +
+                                          std::pow(x, 4) / 24.0 + std::pow(x, 5) / 120.0 +
+                                          std::pow(x, 6) / 720.0;
+                        disp += rep[0][0] * std::exp(-rep[0][1] * r) +
+                                (1 - tt * std::exp(-x)) * c6 / (r2 * r2 * r2);
+                    } else {
+                        const int t = (a != 0 && b != 0) ? 1 : 2;
+                        disp += rep[t][0] * std::exp(-rep[t][1] * r);
+                    }
+                }
+            e_es += es;
+            e_pol += pol;
+            e_disp += disp;
+            nre += es + pol;
+            lj += disp;
+        }
+    out[0] = nre + lj;
+    out[1] = e_es;
+    out[2] = e_pol;
+    out[3] = e_disp;
+}
+
 void water_lists_host(int nw, std::vector<double> &q, std::vector<int> &iq, std::vector<int> &id)
 {
     const double qh = 0.5742185854046506;  // src/ttm_data.h:14

@@ -17,11 +17,14 @@
 // a missing array key is an error, as in the reference.  The Waters group holds
 // O, H, H lines in Angstrom (ReadWatersGroup, src/GetInput.cpp:256-320).
 //
-// Differences, stated where they occur: the second random start vectors come
-// from a seeded generator (the reference calls the unseeded rand()); the neutral
-// cluster energy E0 (water-water electrostatics, polarization and dispersion of
-// the water model) is not part of this library, so the energy block reports the
-// binding energy alone.
+// Output follows ClusterAnion::EnergyFromConfiguration and SinglePoint
+// (src/ClusterAnion.cpp:165-320, 70-99): the neutral cluster's energy E0
+// (WaterCluster::CalcEnergy), the energy block (Eneutral, EBE, Etotal in meV),
+// the binding energies per state and DVR::ExpectationValues
+// (src/DVR.cpp:2329-2449).  One difference, stated where it occurs: the second
+// and later start vectors come from a seeded generator (the reference calls the
+// unseeded rand()).
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -51,8 +54,17 @@ public:
         FILE *f = std::fopen(path, "r");
         if (!f) throw InputError{std::string("cannot open ") + path};
         char buf[kLineLength];
+        int lineno = 0;
         while (std::fgets(buf, kLineLength, f)) {
             const size_t l = std::strlen(buf);
+            ++lineno;
+            // a line that does not fit the buffer would be split into two lines
+            if (l == kLineLength - 1 && buf[l - 1] != '\n' && !std::feof(f)) {
+                std::fclose(f);
+                throw InputError{std::string(path) + ": line " + std::to_string(lineno) +
+                                 " is longer than " + std::to_string(kLineLength - 2) +
+                                 " characters"};
+            }
             if (buf[0] == '#' || l == 1) continue;
             lines_.emplace_back(buf);
         }
@@ -128,11 +140,9 @@ public:
         const char *fmtk = std::is_same<T, int>::value ? ", %i," : ", %lf,";
         if (std::sscanf(s, fmt0, &fill[0]) != 1) throw bad_value(group, key, li);
         for (int k = 1; k < n; ++k) {
-            ++s;
-            while (*s != ',') {
-                ++s;
-                if (*s == '\0') throw bad_value(group, key, li);
-            }
+            ++s;  // s sits on the previous comma (or the key), never on the terminator
+            while (*s && *s != ',') ++s;
+            if (!*s) throw bad_value(group, key, li);
             if (std::sscanf(s, fmtk, &fill[k]) != 1) throw bad_value(group, key, li);
         }
     }
@@ -249,6 +259,54 @@ void print_json(const Params &P, const std::vector<double> &xyz)
     std::printf("]}\n");
 }
 
+// DVR::ExpectationValues (src/DVR.cpp:2329-2449) for the converged states:
+// normalisation, <|r|> = |<r>|, sqrt(<r^2>) and the spread sqrt(<r^2> - <|r|>^2)
+// in Angstrom, and the transition dipoles <n|r|0> in atomic units.
+void expectation_values(const int n[3], const double *grid, const double *wf, int nconv)
+{
+    const double *xg = grid, *yg = grid + n[0], *zg = grid + n[0] + n[1];
+    const long long ngp = (long long)n[0] * n[1] * n[2];
+    std::vector<double> norm(nconv, 0.0), xyz(3 * nconv, 0.0), r2(nconv, 0.0),
+        tdip(3 * (size_t)std::max(nconv - 1, 0), 0.0);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const long long g = i + (long long)n[0] * (j + (long long)n[1] * k);
+                const double q[3] = {xg[i], yg[j], zg[k]};
+                for (int s = 0; s < nconv; ++s) {
+                    const double w = wf[s * ngp + g], rho = w * w;
+                    norm[s] += rho;
+                    for (int a = 0; a < 3; ++a) xyz[3 * s + a] += rho * q[a];
+                    r2[s] += rho * (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+                    if (s > 0)
+                        for (int a = 0; a < 3; ++a) tdip[3 * (s - 1) + a] += w * q[a] * wf[g];
+                }
+            }
+    const double b2a = 0.52917720859;
+    std::printf("\nState     <|r|>     sqrt(<r^2>)   variance  (all in Angstrom)\n");
+    for (int s = 0; s < nconv; ++s) {
+        if (std::fabs(norm[s] - 1.0) > 1e-8)
+            std::printf("Warning: normaliziation integral of state %d is not 1.0, but %g\n"
+                        "This chould not happen.\n", s, norm[s]);
+        std::printf("normalization of state%d is %g\n", s, norm[s]);
+        double r = 0;
+        for (int a = 0; a < 3; ++a) r += xyz[3 * s + a] * xyz[3 * s + a];
+        r = std::sqrt(r);
+        std::printf(" %3i   %10.5f  %10.5f  %10.5f\n", s, b2a * r, b2a * std::sqrt(r2[s]),
+                    b2a * std::sqrt(r2[s] - r * r));
+    }
+    std::printf("\n");
+    if (nconv > 1) {
+        std::printf("Transition dipoles d^2 and d=(<n|x|0>, <n|y|0>, <n|z|0>) (all in au)\n");
+        for (int s = 1; s < nconv; ++s) {
+            const double *d = &tdip[3 * (s - 1)];
+            std::printf(" %3i      %10.5f     (%10.5f,  %10.5f,  %10.5f)\n", s,
+                        d[0] * d[0] + d[1] * d[1] + d[2] * d[2], d[0], d[1], d[2]);
+        }
+        std::printf("\n");
+    }
+}
+
 int fail(const std::string &msg)
 {
     std::fprintf(stderr, "pisces_amd_run: %s\n", msg.c_str());
@@ -312,13 +370,20 @@ int main(int argc, char **argv)
     const double L[3] = {P.gpara[0], P.gpara[1], P.gpara[2]};
     int rc = pisces_create_ex(&ctx, P.ngrid, L, device, P.DVRType);
     if (rc) return fail(std::string("pisces_create_ex: ") + pisces_last_error());
+    std::vector<double> sites((size_t)12 * P.nWater), mu((size_t)9 * P.nWater);
     rc = pisces_build_cluster_potential(ctx, P.nWater, xyz.data(), P.PotFlag, P.PotPara, 0, nullptr,
-                                        nullptr, 2.9, nullptr, nullptr);
+                                        nullptr, 2.9, sites.data(), mu.data());
     if (rc < 0) {
         pisces_destroy(ctx);
         return fail(std::string("potential: ") + pisces_last_error());
     }
     if (rc > 0) std::printf("Warning: the induced dipoles of the water model did not converge\n");
+    // WaterCluster::CalcEnergy (src/ClusterAnion.cpp:169-175)
+    double ew[4];
+    pisces_water_energy(P.nWater, sites.data(), mu.data(), ew);
+    const double E0 = ew[0];
+    std::printf("ElectrostaticEnergy = %g\n", ew[1]);
+    std::printf("Energy of the neutral cluster E0 = %g\n", E0);
     const long long ngp = (long long)P.ngrid[0] * P.ngrid[1] * P.ngrid[2];
     std::vector<double> vecs((size_t)ngp * P.nStates), ev(P.nStates);
     // start: the half-box particle-in-a-box function, then uniform [-1, 1) vectors
@@ -330,9 +395,27 @@ int main(int argc, char **argv)
     pisces_stats st{};
     rc = pisces_solve(ctx, P.nStates, P.maxSub, P.maxIter, P.ptol, corrflag, precision, vecs.data(),
                       ev.data(), &st);
+    std::vector<double> grid((size_t)P.ngrid[0] + P.ngrid[1] + P.ngrid[2]);
+    pisces_grid(ctx, grid.data(), nullptr);
     pisces_destroy(ctx);
     if (rc < 0) return fail(std::string("solve: ") + pisces_last_error());
     const int nconv = rc;
+    if (nconv < 1) {
+        std::printf("GetClusterEnergy:: Convergence failure.\n");
+        return 1;
+    }
+    // the energy block (src/ClusterAnion.cpp:303-315)
+    const double EBE = ev[0];
+    std::printf("                                              \n");
+    std::printf("  ------------------------------------------  \n");
+    std::printf("  ***  Energies For The Current Geometry ***  \n");
+    std::printf("  ------------------------------------------  \n");
+    std::printf("       Eneutral = %g meV\n", E0 * kAu2Mev);
+    std::printf("       EBE      = %g meV\n", EBE * kAu2Mev);
+    std::printf("       Etotal   = %g meV\n", (EBE + E0) * kAu2Mev);
+    std::printf("  ------------------------------------------  \n");
+    std::printf("  **************  END ENERGY ***************\n");
+    std::printf("                                              \n");
     std::printf("\nElectron binding energies:\n");
     for (int s = 0; s < P.nStates; ++s)
         std::printf("State %-3i  %12.8f Hartree =  %10.2f meV\n", s, ev[s], ev[s] * kAu2Mev);
@@ -340,13 +423,8 @@ int main(int argc, char **argv)
         std::printf("single-point:: Only %d States of %d converged.\n", nconv, P.nStates);
         return 1;
     }
-    std::printf("\n  ------------------------------------------  \n");
-    std::printf("  ***  Energies For The Current Geometry ***  \n");
-    std::printf("  ------------------------------------------  \n");
-    std::printf("       EBE      = %.10g meV\n", ev[0] * kAu2Mev);
-    std::printf("       (Eneutral and Etotal: the water model's own energy is not computed here)\n");
-    std::printf("  ------------------------------------------  \n");
-    std::printf("  matvecs %d, iterations %d, restarts %d, %.3f s\n", st.n_matvec, st.n_iter,
-                st.n_macro, st.seconds);
+    expectation_values(P.ngrid, grid.data(), vecs.data(), nconv);
+    std::printf("  matvecs %d, iterations %d, restarts %d, %.3f s (EBE %.12f Eh, E0 %.12f Eh)\n",
+                st.n_matvec, st.n_iter, st.n_macro, st.seconds, EBE, E0);
     return 0;
 }

@@ -1,6 +1,8 @@
 """The front-end's single point (runtype 1) on the GPU against the reference:
 the input file c1.inp holds the geometry, grid and Davidson settings of the
-reference Davidson fixture davidson_c1 (32^3, (H2O)6, one state, pTol 9)."""
+reference Davidson fixture davidson_c1 (32^3, (H2O)6, one state, pTol 9); the
+same geometry's neutral-cluster energy E0 comes from the reference's own
+WaterCluster::CalcEnergy (water_c1.npz)."""
 import os
 import re
 import subprocess
@@ -24,8 +26,25 @@ def test_single_point_matches_reference(precision, tol):
                        timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert re.search(r"State 0\s+-?\d+\.\d{8} Hartree =\s+-?\d+\.\d{2} meV", r.stdout)
-    ebe = float(re.search(r"EBE\s+=\s+(\S+) meV", r.stdout).group(1)) / AU2MEV
+    ebe = float(re.search(r"EBE (\S+) Eh", r.stdout).group(1))
     assert abs(ebe - float(ref["evals"][0])) <= tol
+    # the energy block of src/ClusterAnion.cpp:306-311, six significant digits
+    # as the reference's cout prints them
+    e0_ref = float(golden("water_c1.npz")["e0"])
+    e0 = float(re.search(r"E0 (\S+) Eh", r.stdout).group(1))
+    assert abs(e0 - e0_ref) <= 1e-12 * abs(e0_ref)
+    blk = {k: float(re.search(r"%s\s+=\s+(\S+) meV" % k, r.stdout).group(1))
+           for k in ("Eneutral", "EBE", "Etotal")}
+    assert abs(blk["Eneutral"] - e0_ref * AU2MEV) <= 1e-5 * abs(e0_ref * AU2MEV)
+    assert abs(blk["EBE"] - ebe * AU2MEV) <= 1e-5 * abs(ebe * AU2MEV)
+    assert abs(blk["Etotal"] - (ebe + e0_ref) * AU2MEV) <= 1e-5 * abs((ebe + e0_ref) * AU2MEV)
+    assert float(re.search(r"Energy of the neutral cluster E0 = (\S+)", r.stdout).group(1)) \
+        == pytest.approx(e0_ref, rel=1e-5)
+    # DVR::ExpectationValues: the state's norm and its size
+    assert re.search(r"normalization of state0 is 1\b", r.stdout), r.stdout
+    m = re.search(r"\n\s+0\s+(\S+)\s+(\S+)\s+(\S+)\n", r.stdout)
+    r_mean, r_rms, spread = (float(v) for v in m.groups())
+    assert 0 <= r_mean <= r_rms and spread > 0
 
 
 def test_single_point_fft_kinetic(tmp_path, oracle_mod):
@@ -45,7 +64,7 @@ def test_single_point_fft_kinetic(tmp_path, oracle_mod):
     path.write_text("\n".join(lines) + "\n")
     r = subprocess.run([BIN, str(path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    ebe = float(re.search(r"EBE\s+=\s+(\S+) meV", r.stdout).group(1)) / AU2MEV
+    ebe = float(re.search(r"EBE (\S+) Eh", r.stdout).group(1))
     cl = oracle_mod.Cluster(xyz)
     g = oracle_mod.Grid((24, 24, 24), (36.0, 36.0, 36.0), dvr_type=3)
     ro = g.solve(cl.potential_grid(g), 1, 20, 200, 9, 1, g.piab().reshape(1, -1))
@@ -71,7 +90,7 @@ def test_single_point_poltype3(tmp_path, oracle_mod):
     path.write_text("\n".join(lines) + "\n")
     r = subprocess.run([BIN, str(path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    ebe = float(re.search(r"EBE\s+=\s+(\S+) meV", r.stdout).group(1)) / AU2MEV
+    ebe = float(re.search(r"EBE (\S+) Eh", r.stdout).group(1))
     cl = oracle_mod.Cluster(xyz)
     g = oracle_mod.Grid((24, 24, 24), (36.0, 36.0, 36.0))
     V = cl.potential_grid_pol3(g, (3, 2, 0, 3, 0, 0))
