@@ -208,6 +208,24 @@ int pisces_time_hx(pisces_ctx *ctx, int nvec, int precision, int warmup, int ite
 int pisces_kinetic_precond(pisces_ctx *ctx, int nvec, const double *r, double shift,
                            double scale, double floor, int precision, double *t);
 
+/* Allocation-free form of pisces_kinetic_precond.  It has no counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code
+ * under stream capture, serving loops that forbid hipMalloc).  d_r and d_t are
+ * distinct device arrays of nvec*ngp doubles.  The per-vector task
+ * descriptors and, in the split-BF16 form, the FP32 intermediate between the
+ * three plane passes live in a fixed array of pisces_noalloc_scratch_bytes()
+ * bytes (32 MiB) that each GPU holds until the process ends, so the split-BF16
+ * form takes at most about 8.3 million grid points x vectors (64^3 x 31
+ * vectors); larger requests return PISCES_ERR_ARG before any launch.  All
+ * calls on one GPU share that array: calls that overlap in time on one GPU
+ * (other threads, other contexts) race.  The array's address on a device is
+ * looked up when the first context is created there.  Results equal
+ * pisces_kinetic_precond's bit for bit (same kernels); synchronises the
+ * context's stream before returning. */
+size_t pisces_noalloc_scratch_bytes(void);
+int pisces_kinetic_precond_noalloc(pisces_ctx *ctx, int nvec, const double *d_r, double shift,
+                                   double scale, double floor, int precision, double *d_t);
+
 /* Times `iters` applications of the kinetic preconditioner (all three passes)
  * to `nvec` device-resident vectors, as pisces_time_hx does for H*X: FP64
  * vectors for PISCES_FP64, FP32 vectors with the split-BF16 form for

@@ -131,6 +131,7 @@ int pisces_create_ex(pisces_ctx **out, const int n[3], const double L[3], int de
         c->device = device;
         c->dvr_type = dvr_type;
         PC_HIP(hipSetDevice(device));
+        c->scratch = noalloc_scratch(device);
         PC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->ngp = 1;
         for (int a = 0; a < 3; ++a) {

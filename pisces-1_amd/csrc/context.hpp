@@ -42,6 +42,10 @@ struct pisces_ctx {
     // that the state's definition stays in its own translation unit
     std::shared_ptr<void> rc;
 
+    // device address of the library's fixed scratch array on this device
+    // (pisces_kinetic_precond_noalloc), looked up at creation
+    void *scratch = nullptr;
+
     pisces_ctx() = default;
     ~pisces_ctx();
     pisces_ctx(const pisces_ctx &) = delete;
@@ -54,6 +58,9 @@ struct pisces_ctx {
 };
 
 namespace pisces {
+// Address of the allocation-free entry point's scratch array on `device` (the
+// current device); looked up once per device.
+void *noalloc_scratch(int device);
 // Runs f, converting library exceptions into C-ABI return codes.
 template <typename F>
 int guarded_call(F &&f)

@@ -84,6 +84,67 @@ def exported_symbols():
     return sorted(set(re.findall(r"\b(pisces_[a-z0-9_]+)\s*\(", txt)))
 
 
+class DeviceArray:
+    """Device buffer of float64 values allocated through the HIP runtime the
+    library itself links (libamdhip64.so.7), for the device-pointer entry
+    points.  torch's wheel bundles a different HIP runtime, so its tensors are
+    not used here."""
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            lib()  # loads the library and with it its HIP runtime
+            h = ctypes.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+            h.hipFree.argtypes = [ctypes.c_void_p]
+            h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            h.hipSetDevice.argtypes = [ctypes.c_int]
+            for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipSetDevice"):
+                getattr(h, f).restype = ctypes.c_int
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, shape, device=0):
+        self.shape = tuple(shape)
+        self.nbytes = 8 * int(np.prod(self.shape))
+        self.ptr = ctypes.c_void_p()
+        h = self.hip()
+        if h.hipSetDevice(int(device)) or h.hipMalloc(ctypes.byref(self.ptr), self.nbytes):
+            raise PiscesError("hipMalloc of %d bytes failed" % self.nbytes)
+
+    @classmethod
+    def from_host(cls, a, device=0):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.shape, device)
+        if cls.hip().hipMemcpy(d.ptr, a.ctypes.data, d.nbytes, 1):
+            raise PiscesError("hipMemcpy host to device failed")
+        return d
+
+    def to_host(self):
+        a = np.empty(self.shape)
+        if self.hip().hipMemcpy(a.ctypes.data, self.ptr, self.nbytes, 2):
+            raise PiscesError("hipMemcpy device to host failed")
+        return a
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def noalloc_scratch_bytes():
+    L = lib()
+    L.pisces_noalloc_scratch_bytes.restype = ctypes.c_size_t
+    return int(L.pisces_noalloc_scratch_bytes())
+
+
 def set_hx_variant(v):
     check(lib().pisces_debug_set_hx_variant(int(v)), "set_hx_variant")
 
@@ -233,6 +294,19 @@ class Context:
                                        float(floor), int(precision), _p(t)),
               "pisces_kinetic_precond")
         return t
+
+    def kinetic_precond_noalloc(self, d_r, d_t, nvec, shift, scale=-1.0, floor=1e-3,
+                                precision=FP64):
+        """The allocation-free preconditioner (pisces_kinetic_precond_noalloc) on
+        device arrays d_r, d_t (DeviceArray of nvec x ngp); returns the code."""
+        L = lib()
+        L.pisces_kinetic_precond_noalloc.argtypes = [
+            ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+            ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+        L.pisces_kinetic_precond_noalloc.restype = ctypes.c_int
+        return L.pisces_kinetic_precond_noalloc(self.h, int(nvec), d_r.ptr, float(shift),
+                                                float(scale), float(floor), int(precision),
+                                                d_t.ptr)
 
     def davidson_rc(self, start, diag, matvec, maxsub=20, maxmacro=100, ptol=7, corrflag=1):
         """Reverse-communication Davidson (pisces_davidson): the caller's
