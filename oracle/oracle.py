@@ -288,6 +288,34 @@ class Cluster:
         return v
 
 
+def gradient_grid(cl, grid, psi, flags=(3, 2, 0, 3, 0, 0), potpara=None):
+    """Electron-side gradient reductions (gel, tmu, mcm) of cluster cl on grid
+    for the wavefunction psi (the reference's ComputeGradient grid loop)."""
+    f = np.ascontiguousarray(np.asarray(flags, dtype=np.int32))
+    pp = np.ascontiguousarray(default_potpara() if potpara is None else potpara, dtype=np.float64)
+    alpha, ip, epc, _ = cl.pol3_lists()
+    ns = cl.sites.shape[0]
+    n3 = 3 * ip.size
+    gel, tmu, mcm = np.zeros(3 * ns), np.zeros(n3), np.zeros(n3 * n3)
+    nx, ny, nz = grid.n
+    rc = lib().or_gradient_grid(
+        _pi(f), _p(pp), ns, _p(cl.sites), len(cl.q), _p(cl.q), _pi(cl.iq), ip.size,
+        _p(np.ascontiguousarray(alpha)), _pi(np.ascontiguousarray(ip, dtype=np.int32)),
+        _p(np.ascontiguousarray(epc)), nx, ny, nz, _p(grid.x[0]), _p(grid.x[1]), _p(grid.x[2]),
+        _p(np.ascontiguousarray(psi, dtype=np.float64)), _p(gel), _p(tmu), _p(mcm))
+    if rc != 0:
+        raise ValueError("unsupported flags for the gradient (rc=%d)" % rc)
+    return gel, tmu, mcm.reshape(n3, n3)
+
+
+def gradient_final(nsites, np_, gel, tmu, mcm, dTx, dTy, dTz, dEfield, gww, polgrad):
+    out = np.zeros(3 * nsites)
+    a = [np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+         for v in (gel, tmu, mcm, dTx, dTy, dTz, dEfield, gww, polgrad)]
+    lib().or_gradient_final(nsites, np_, *[_p(v) for v in a], _p(out))
+    return out
+
+
 def pol3_inva(rpol, alpha):
     """InvA = (alpha^-1 - T)^-1 of the polarizable sites (row-major 3np x 3np)."""
     rp = np.ascontiguousarray(np.asarray(rpol, dtype=np.float64).reshape(-1))

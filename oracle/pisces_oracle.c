@@ -1299,3 +1299,152 @@ int or_solve_grid(int nx, int ny, int nz, const double *Tx, const double *Ty,
     free(D);
     return rc;
 }
+
+/* ======================================================================= f2 */
+/* Potential::EvaluateDPPGTOPGradient (src/Gradients.cpp:79-176) at one point,
+ * the terms weighted by w = psi^2 and summed as DVR::ComputeGradient does
+ * (src/DVR.cpp:1099-1176): the point-charge term with the effective distance,
+ * the Slater repulsive cores, and the electron's field derivative at the
+ * polarizable sites dotted with the induced dipoles mu = InvA E
+ * (DerivElecField, :179-225); MuCrossMu (:227-238) accumulates psi^2 mu and
+ * psi^2 mu mu^T.  R = r - site throughout. */
+static void grad_point(const or_pot *P, const double *r, double w, double psi, double *R,
+                       double *Rx, double *Ry, double *Rz, double *Rm3, double *E,
+                       double *mu, double *gel, double *tmu, double *mcm)
+{
+    for (int i = 0; i < P->ns; ++i) {
+        Rx[i] = r[0] - P->sites[3 * i];
+        Ry[i] = r[1] - P->sites[3 * i + 1];
+        Rz[i] = r[2] - P->sites[3 * i + 2];
+        double r2 = Rx[i] * Rx[i] + Ry[i] * Ry[i] + Rz[i] * Rz[i];
+        R[i] = sqrt(r2);
+        Rm3[i] = 1.0 / (r2 * R[i]);
+    }
+    for (int i = 0; i < P->nq; ++i) {
+        int s = P->iq[i];
+        double re = R[s], g;
+        if (re < P->qdamp) {
+            double t = re / P->qdamp;
+            re = P->qdamp * (0.5 + t * t * t * (1.0 - 0.5 * t));
+            g = P->q[i] * (3.0 * t * t - 2.0 * t * t * t) / (re * re);
+        } else {
+            g = P->q[i] / (R[s] * R[s]);
+        }
+        gel[3 * s] += w * (-g * Rx[s] / R[s]);
+        gel[3 * s + 1] += w * (-g * Ry[s] / R[s]);
+        gel[3 * s + 2] += w * (-g * Rz[s] / R[s]);
+    }
+    for (int i = 0; i < P->nrep; ++i) {
+        int s = P->irep[i];
+        double de = P->rep[2 * i + 1] * (-P->rep[2 * i]) * exp(-P->rep[2 * i] * R[s]);
+        gel[3 * s] += w * (-de * Rx[s] / R[s]);
+        gel[3 * s + 1] += w * (-de * Ry[s] / R[s]);
+        gel[3 * s + 2] += w * (-de * Rz[s] / R[s]);
+    }
+    const int n = 3 * P->np;
+    for (int i = 0; i < P->np; ++i) {
+        int s = P->ip[i];
+        double g = Rm3[s], re = R[s];
+        if (re < P->pdamp) {
+            double t = re / P->pdamp;
+            re = P->pdamp * (0.5 + t * t * t * (1.0 - 0.5 * t));
+            g = 1.0 / (re * re * re);
+        }
+        E[3 * i] = g * Rx[s] + P->epc[3 * i];
+        E[3 * i + 1] = g * Ry[s] + P->epc[3 * i + 1];
+        E[3 * i + 2] = g * Rz[s] + P->epc[3 * i + 2];
+    }
+    for (int a = 0; a < n; ++a) {
+        double v = 0;
+        for (int b = 0; b < n; ++b) v += P->inva[(size_t)n * b + a] * E[b];
+        mu[a] = v;
+    }
+    for (int i = 0; i < P->np; ++i) {
+        int s = P->ip[i];
+        double g = Rm3[s], re = R[s], dg = -3.0 / (re * re * re * re);
+        if (re < P->pdamp) {
+            double t = re / P->pdamp;
+            re = P->pdamp * (0.5 + t * t * t * (1.0 - 0.5 * t));
+            dg = (3.0 * t * t - 0.5 * 4.0 * t * t * t) * (-3.0 / (re * re * re * re));
+            g = 1.0 / (re * re * re);
+        }
+        const double Rv[3] = {Rx[s], Ry[s], Rz[s]};
+        const double *m = mu + 3 * i;
+        for (int a = 0; a < 3; ++a) {
+            /* dE/dR_site along a, dotted with mu (src/Gradients.cpp:206-220) */
+            double dE0 = -dg * Rv[a] / R[s] * Rv[0] - (a == 0 ? g : 0.0);
+            double dE1 = -dg * Rv[a] / R[s] * Rv[1] - (a == 1 ? g : 0.0);
+            double dE2 = -dg * Rv[a] / R[s] * Rv[2] - (a == 2 ? g : 0.0);
+            gel[3 * s + a] += w * (-(dE0 * m[0] + dE1 * m[1] + dE2 * m[2]));
+        }
+    }
+    for (int a = 0; a < n; ++a) {
+        double t0 = psi * psi * mu[a];
+        tmu[a] += t0;
+        for (int b = 0; b < n; ++b) mcm[(size_t)n * a + b] += t0 * mu[b];
+    }
+}
+
+int or_gradient_grid(const int *flags, const double *potpara, int nsites, const double *sites,
+                     int nq, const double *q, const int *iq, int np, const double *alpha,
+                     const int *ip, const double *epc, int nx, int ny, int nz,
+                     const double *xg, const double *yg, const double *zg, const double *psi,
+                     double *gel, double *tmu, double *mcm)
+{
+    or_pot P;
+    int rc = pot_init(&P, flags, potpara, nsites, sites, nq, q, iq, 0, NULL, NULL, 0, NULL,
+                      NULL, 1.0);
+    if (!rc) rc = pol3_init(&P, np, alpha, ip, epc, 0.0);
+    if (!rc && (flags[0] != 3 || P.damp_type != 2)) rc = -2;
+    if (rc) { pot_free(&P); return rc; }
+    const int n = 3 * np;
+    memset(gel, 0, sizeof(double) * 3 * nsites);
+    memset(tmu, 0, sizeof(double) * n);
+    memset(mcm, 0, sizeof(double) * (size_t)n * n);
+    double *R = malloc(sizeof(double) * 5 * nsites), *E = malloc(sizeof(double) * 2 * n);
+    double *Rx = R + nsites, *Ry = R + 2 * nsites, *Rz = R + 3 * nsites, *Rm3 = R + 4 * nsites;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                long g = i + (long)nx * (j + (long)ny * k);
+                const double r[3] = {xg[i], yg[j], zg[k]};
+                grad_point(&P, r, psi[g] * psi[g], psi[g], R, Rx, Ry, Rz, Rm3, E, E + n, gel,
+                           tmu, mcm);
+            }
+    free(R); free(E);
+    pot_free(&P);
+    return 0;
+}
+
+void or_gradient_final(int nsites, int np, const double *gel, const double *tmu,
+                       const double *mcm, const double *dTx, const double *dTy,
+                       const double *dTz, const double *dEfield, const double *gww,
+                       const double *polgrad, double *gfinal)
+{
+    /* Potential::FinalGradient (src/Gradients.cpp:240-268), the sum into the
+     * water-water gradient and SubtractWWGradient (:270-284), in the order of
+     * DVR::ComputeGradient (src/DVR.cpp:1160-1176) */
+    const int n3 = 3 * np, n9 = 9 * np;
+    double *loc = malloc(sizeof(double) * 3 * nsites);
+    memcpy(loc, gel, sizeof(double) * 3 * nsites);
+    for (int j = 0; j < nsites; ++j) {
+        for (int d = 0; d < 3; ++d) {
+            const double *de = dEfield + (size_t)n3 * (3 * j + d);
+            double s = 0;
+            for (int k = 0; k < n3; ++k) s += de[k] * tmu[k];
+            loc[3 * j + d] += -0.5 * 2.0 * s;
+        }
+        int aj = 3 * (j / 4) + j % 4;
+        if ((j + 1) % 4 != 0) {
+            const double *T[3] = {dTx, dTy, dTz};
+            for (int d = 0; d < 3; ++d) {
+                double s = 0;
+                for (int k = 0; k < n9; ++k)
+                    s += T[d][(size_t)aj * n9 + k] * mcm[(size_t)aj * n9 + k];
+                loc[3 * j + d] += -0.5 * 2.0 * s;
+            }
+        }
+    }
+    for (int c = 0; c < 3 * nsites; ++c) gfinal[c] = gww[c] + loc[c] - polgrad[c];
+    free(loc);
+}

@@ -212,6 +212,29 @@ int or_solve_grid(int nx, int ny, int nz, const double *Tx, const double *Ty,
 int or_syev(int n, const double *A, int lda, double *W, double *Vec, int ldv);
 /* Pinned: LAPACK (numpy eigvalsh) and A V = V diag(W). */
 
+/* Electron side of the analytic gradient: DVR::ComputeGradient's
+ * grid loop (src/DVR.cpp:1099-1176) over Potential::EvaluateDPPGTOPGradient
+ * (src/Gradients.cpp:79-238) for PotFlag 3, DampType 2, Polarization 3, on the
+ * lists of pisces_build_potential_scp: gel[3 nsites] = sum psi^2 dV/dR_site
+ * (charges, repulsive cores, the electron's field derivative at the
+ * polarizable sites dotted with mu), tmu[3np] = sum psi^2 mu and
+ * mcm[(3np)^2] = sum psi^2 mu mu^T (row-major), mu = InvA E at each point.
+ * Returns -2 for other flag sets. */
+int or_gradient_grid(const int *flags, const double *potpara, int nsites, const double *sites,
+                     int nq, const double *q, const int *iq, int np, const double *alpha,
+                     const int *ip, const double *epc, int nx, int ny, int nz,
+                     const double *xg, const double *yg, const double *zg, const double *psi,
+                     double *gel, double *tmu, double *mcm);
+/* FinalGradient + SubtractWWGradient (src/Gradients.cpp:240-284) with the
+ * water side's dT_x/y/z (BuildSuperdT), dEfield (Calc_dInducedDipoles), the
+ * water-water gradient gww and PolGrad (CalcGradient): gfinal[3 nsites]. */
+void or_gradient_final(int nsites, int np, const double *gel, const double *tmu,
+                       const double *mcm, const double *dTx, const double *dTy,
+                       const double *dTz, const double *dEfield, const double *gww,
+                       const double *polgrad, double *gfinal);
+/* Pinned: the reference's own EvaluateGradient / FinalGradient /
+ * SubtractWWGradient through WaterCluster (grad_*.npz). */
+
 /* Threads used by the OpenMP loops (0 = library default). */
 void or_set_threads(int n);
 

@@ -41,6 +41,17 @@
 //                                     water), id, all dipoles[9nW], their sites,
 //                                     alpha[3nW], ipp[3nW], Epc[9nW],
 //                                     [E0, PolarizationEnergy]
+//   ref_driver grad      <in> <out>   in: [nW], O,H1,H2 xyz (Angstrom), [nx,ny,nz,Lx,Ly,Lz],
+//                                     psi (ngp, x fastest), flags[6], potpara[11]
+//                                     the analytic gradient of ClusterAnion::GetAnalGrad
+//                                     (src/ClusterAnion.cpp:322-430) after the setup of
+//                                     EnergyFromConfiguration (:165-246), with the grid
+//                                     loop of DVR::ComputeGradient (src/DVR.cpp:1099-1176)
+//                                     over the reference's Potential::EvaluateGradient,
+//                                     FinalGradient and SubtractWWGradient (src/Gradients.cpp)
+//                                     out: Gel = sum psi^2 tGrad [3 nSites], tmu [3 nPol],
+//                                     mu_cross_mu [(3 nPol)^2], dT_x, dT_y, dT_z, dEfield,
+//                                     PolGrad, Gww (CalcGradient), Gfinal, analgrad [6 nW], [E0]
 #include <mpi.h>
 
 #include <cmath>
@@ -295,6 +306,86 @@ static int cmd_water(const char *in, const char *out)
     return 0;
 }
 
+void BuildSuperdT(int nPolSites, const double *R, const double *alpha, double aThole,
+                  double *dT_x, double *dT_y, double *dT_z);
+
+static int cmd_grad(const char *in, const char *out)
+{
+    std::vector<rec> r = read_records(in);
+    const int nw = (int)r[0][0];
+    const rec &xyz = r[1];
+    const int n[3] = {(int)r[2][0], (int)r[2][1], (int)r[2][2]};
+    const double L[3] = {r[2][3], r[2][4], r[2][5]};
+    const rec &psi = r[3];
+    std::vector<int> flags = as_int(r[4]);
+    rec pp(r[5]);
+    pp.resize(32, 0.0);
+    // EnergyFromConfiguration: the water model and the electron's Potential
+    WaterCluster W;
+    W.SetStructure(nw, xyz.data(), 1, 0, 0, 0);
+    rec conf(6 * nw);
+    W.GetConfiguration(nw, conf.data(), 1.0, 0);
+    W.SetConfiguration(nw, conf.data(), 1, 0);
+    const double e0 = W.CalcEnergy(0);
+    Potential V;
+    V.VpolWaterWater = W.ReportPolarization();
+    int ns = 0, nq = 0, nd = 0;
+    W.ReportNoOfSites(ns, nq, nd);
+    const int np = W.ReportNoOfPolSites();
+    rec sites(3 * ns), q(nq), d(9 * nd), dmu(3 * ns * 3 * nd + 1, 0.0);
+    std::vector<int> iq(nq), id(3 * nd);
+    W.GetLists(ns, sites.data(), nq, q.data(), iq.data(), nd, d.data(), id.data(), dmu.data(), 0);
+    W.ReportInducedDipoles(3 * nd, d.data(), id.data(), 0);
+    rec alpha(np), epc(3 * np);
+    std::vector<int> ipp(np);
+    W.ReportPolSitesAndField(np, alpha.data(), ipp.data(), epc.data(), 0);
+    iVec pf(flags.begin(), flags.end());
+    V.Setup(pf, ns, sites.data(), nq, q.data(), iq.data(), 3 * nd, d.data(), id.data(), np,
+            alpha.data(), ipp.data(), epc.data(), dmu.data(), pp.data());
+    // GetAnalGrad
+    rec dEfield((size_t)3 * ns * nq * 3), dTensor((size_t)3 * ns * 3 * nd * 3);
+    rec dTx((size_t)9 * np * np), dTy((size_t)9 * np * np), dTz((size_t)9 * np * np);
+    rec grad(3 * ns, 0.0), polgrad(3 * ns, 0.0);
+    W.SetConfiguration(nw, conf.data(), 1, 0);
+    W.Calc_dInducedDipoles(0, dmu.data(), dEfield.data(), dTensor.data());
+    W.CalcGradient(3 * ns, grad.data(), polgrad.data(), 0);
+    const rec gww(grad);
+    W.ReportPolSitesAndField(np, alpha.data(), ipp.data(), epc.data(), 0);
+    rec rpps(3 * np);
+    for (int i = 0; i < np; ++i)
+        for (int k = 0; k < 3; ++k) rpps[3 * i + k] = sites[3 * ipp[i] + k];
+    BuildSuperdT(np, rpps.data(), alpha.data(), 0.3, dTx.data(), dTy.data(), dTz.data());
+    // DVR::ComputeGradient: the grid loop (x fastest) over EvaluateGradient
+    rec xg[3], Tm[3];
+    for (int a = 0; a < 3; ++a) {
+        xg[a].resize(n[a]);
+        Tm[a].resize((size_t)n[a] * n[a]);
+        sine_dvr(n[a], 1.0, -0.5 * L[a], 0.5 * L[a], xg[a].data(), Tm[a].data(), n[a], 0);
+    }
+    const int n3 = 3 * np;
+    dVec tGrad(3 * ns), tmu(n3), mcm((size_t)n3 * n3, 0.0);
+    rec gel(3 * ns, 0.0);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const long g = i + (long)n[0] * (j + (long)n[1] * k);
+                const double pt[3] = {xg[0][i], xg[1][j], xg[2][k]};
+                std::fill(tGrad.begin(), tGrad.end(), 0.0);
+                V.EvaluateGradient(pt, tGrad, tmu, mcm, psi[g], W);
+                for (int c = 0; c < 3 * ns; ++c) gel[c] += psi[g] * psi[g] * tGrad[c];
+            }
+    rec loc(gel);
+    V.FinalGradient(np, loc.data(), &tmu[0], &mcm[0], dTx.data(), dTy.data(), dTz.data(),
+                    dEfield.data());
+    for (int c = 0; c < 3 * ns; ++c) grad[c] += loc[c];
+    V.SubtractWWGradient(ns, polgrad.data(), grad.data());
+    rec ana(6 * nw);
+    W.ConvertF(grad.data(), ana.data());
+    write_records(out, {gel, rec(tmu.begin(), tmu.end()), rec(mcm.begin(), mcm.end()), dTx, dTy,
+                        dTz, dEfield, polgrad, gww, grad, ana, rec{e0}});
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     MPI_Init(&argc, &argv);
@@ -307,6 +398,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[1], "kin")) rc = cmd_kin(argv[2], argv[3]);
         else if (!strcmp(argv[1], "damp")) rc = cmd_damp(argv[2], argv[3]);
         else if (!strcmp(argv[1], "water")) rc = cmd_water(argv[2], argv[3]);
+        else if (!strcmp(argv[1], "grad")) rc = cmd_grad(argv[2], argv[3]);
     } else {
         fprintf(stderr, "usage: ref_driver {tkin|potential|davidson} <in> <out>\n");
     }

@@ -187,6 +187,30 @@ def golden_waters():
     golden_water("nonrigid", xyz + rng.normal(scale=0.03, size=xyz.shape))
 
 
+def golden_gradient(tag, nw, radius, seed, n, L):
+    """The analytic gradient of the reference (ClusterAnion::GetAnalGrad with
+    DVR::ComputeGradient's grid loop over Potential::EvaluateGradient) for the
+    ground state of the PolType 3 potential (Potential 3, DampType 2,
+    Polarization 3), the wavefunction from the oracle's Davidson."""
+    xyz, _ = workload.water_cluster(nw, radius, seed)
+    flags = (3, 2, 0, 3, 0, 0)
+    pp = oracle.default_potpara()
+    g = oracle.Grid(n, L)
+    cl = oracle.Cluster(xyz)
+    V = cl.potential_grid_pol3(g, flags)
+    r = g.solve(V, 1, 20, 300, 9, 1, g.piab().reshape(1, -1))
+    assert r["nconv"] == 1
+    psi = r["vecs"][0]
+    names = ("gel", "tmu", "mcm", "dTx", "dTy", "dTz", "dEfield", "polgrad", "gww", "gfinal",
+             "analgrad", "e0")
+    out = run_ref("grad", [[nw], xyz.reshape(-1), list(n) + list(L), psi, list(flags), pp])
+    d = dict(zip(names, out))
+    d["e0"] = d["e0"][0]
+    np.savez_compressed(os.path.join(HERE, "grad_%s.npz" % tag), xyz=xyz, n=np.array(n),
+                        L=np.array(L), psi=psi, flags=np.array(flags), potpara=pp,
+                        eval=r["evals"][0], **d)
+
+
 def _waters_block(xyz):
     lines = []
     for w in np.asarray(xyz).reshape(-1, 9):
@@ -344,6 +368,8 @@ def main():
     golden_kin_types()
     golden_damp()
     golden_waters()
+    golden_gradient("c1", 6, 3.5, 1000, (16, 16, 16), (24.0, 24.0, 24.0))
+    golden_gradient("c2", 24, 6.0, 2000, (20, 20, 20), (32.0, 32.0, 32.0))
     golden_potential("c1", 6, 3.5, 1000, 36.0)
     golden_potential("c2", 24, 6.0, 2000, 40.0)
     golden_potential_pol3("c1", 6, 3.5, 1000, 36.0)
