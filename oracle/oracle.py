@@ -316,6 +316,28 @@ def gradient_final(nsites, np_, gel, tmu, mcm, dTx, dTy, dTz, dEfield, gww, polg
     return out
 
 
+def dual_coarse(n, L):
+    nc = ctypes.c_int(0)
+    Lc = ctypes.c_double(0)
+    lib().or_dual_coarse(int(n), ctypes.c_double(L), ctypes.byref(nc), ctypes.byref(Lc))
+    return nc.value, Lc.value
+
+
+def dual_weights(kind, rx, ry, rz):
+    w = np.zeros(8)
+    lib().or_dual_weights(int(kind), int(rx), int(ry), int(rz), _p(w))
+    return w
+
+
+def dual_extend(n, nc, kind, coarse):
+    fine = np.zeros(n ** 3)
+    rc = lib().or_dual_extend(int(n), int(nc), int(kind),
+                              _p(np.ascontiguousarray(coarse, dtype=np.float64)), _p(fine))
+    if rc != 0:
+        raise ValueError("coarse grid does not match")
+    return fine
+
+
 def pol3_inva(rpol, alpha):
     """InvA = (alpha^-1 - T)^-1 of the polarizable sites (row-major 3np x 3np)."""
     rp = np.ascontiguousarray(np.asarray(rpol, dtype=np.float64).reshape(-1))

@@ -1448,3 +1448,131 @@ void or_gradient_final(int nsites, int np, const double *gel, const double *tmu,
     for (int c = 0; c < 3 * nsites; ++c) gfinal[c] = gww[c] + loc[c] - polgrad[c];
     free(loc);
 }
+
+/* ======================================================================= f3 */
+/* Dual-grid coarse -> fine transfer, cubic grids (ClusterAnion's orchestration,
+ * src/ClusterAnion.cpp:207-299).  Coarse grid: odd n -> (n + 1) / 2 points at
+ * twice the spacing; even n -> (n / 2 + 1) / 3 * 2 points at three times it
+ * (:209-232).  Values on the fine grid: a coarse point where the fine point
+ * coincides with one, else a weighted sum of the 2, 4 or 8 coarse corners
+ * around it: midpoint averages for double spacing (InterpolVdouble,
+ * ExtendWfDouble, src/DVR.cpp:1380-1421, 1646-1722), inverse-distance weights
+ * with the reference's six-digit constants for triple spacing (InterpolVtriple
+ * for potentials, ExtendWfThree for wavefunctions, src/DVR.cpp:1240-1378,
+ * 1446-1644), which differ in three weights (below). */
+void or_dual_coarse(int n, double L, int *nc, double *Lc)
+{
+    double h = L / (n + 1);
+    if (n % 2 == 1) {
+        *nc = (n + 1) / 2;
+        *Lc = *nc * 2 * h + 2 * h;
+    } else {
+        *nc = (n / 2 + 1) / 3 * 2;
+        *Lc = *nc * 3 * h + 3 * h;
+    }
+}
+
+/* weights of the corners (a, b, c) in {0,1}^3, index a + 2 b + 4 c, for a fine
+ * point with residues (rx, ry, rz) in {0,1,2}^3 of triple spacing; kind 0 the
+ * wavefunction (ExtendWfThree), 1 the potential (InterpolVtriple) */
+static void tri_weights(int kind, int rx, int ry, int rz, double *w)
+{
+    const double w1 = 0.707107 / 1.955087, w2 = 0.447214 / 1.955087, w3 = 0.353553 / 1.955087;
+    const double g1 = 0.57735 / 3.09077, g2 = 0.408248 / 3.09077, g3 = 0.33333 / 3.09077,
+                 g4 = 0.288675 / 3.09077;
+    for (int k = 0; k < 8; ++k) w[k] = 0.0;
+    int r[3] = {rx, ry, rz}, nz = (rx != 0) + (ry != 0) + (rz != 0);
+    if (nz == 0) { w[0] = 1.0; return; }
+    if (nz == 1) {
+        int ax = rx ? 0 : ry ? 1 : 2, s = 1 << ax;
+        w[0] = r[ax] == 1 ? 2.0 / 3.0 : 1.0 / 3.0;
+        w[s] = r[ax] == 1 ? 1.0 / 3.0 : 2.0 / 3.0;
+        return;
+    }
+    if (nz == 2) {
+        /* the two moving axes u < v; corners (0,0), (u), (v), (u,v) */
+        int u = rx ? 0 : 1, v = rz ? 2 : 1;
+        int su = 1 << u, sv = 1 << v, ru = r[u], rv = r[v];
+        double c00, cu, cv, cuv;
+        if (ru == 1 && rv == 1) { c00 = w1; cu = w2; cv = w2; cuv = w3; }
+        else if (ru == 1 && rv == 2) { c00 = w2; cu = w3; cv = w1; cuv = w2; }
+        else if (ru == 2 && rv == 1) {
+            c00 = w2; cu = w1; cv = w3; cuv = w2;
+            /* ExtendWfThree, plane (x, z): w1 on (0, c+1) (src/DVR.cpp:1570) */
+            if (kind == 0 && u == 0 && v == 2) cv = w1;
+        } else { c00 = w3; cu = w2; cv = w2; cuv = w1; }
+        w[0] = c00; w[su] = cu; w[sv] = cv; w[su + sv] = cuv;
+        return;
+    }
+    /* nz == 3: corners 0, x, y, z, xy, yz, xz, xyz */
+    static const double *tab = 0;
+    (void)tab;
+    double t[8];
+    int key = 100 * rx + 10 * ry + rz;
+    switch (key) {
+    case 111: { double v[8] = {g1, g2, g2, g2, g3, g3, g3, g4}; memcpy(t, v, sizeof(t)); } break;
+    case 211: { double v[8] = {g2, g1, g3, g3, g2, g4, g2, g3}; memcpy(t, v, sizeof(t)); } break;
+    case 121: { double v[8] = {g2, g3, g1, g3, g2, g2, g4, g3}; memcpy(t, v, sizeof(t)); } break;
+    case 112: { double v[8] = {g2, g3, g3, g1, g4, g2, g2, g3}; memcpy(t, v, sizeof(t)); } break;
+    case 221: { double v[8] = {g3, g2, g2, g4, g1, g3, g3, g2}; memcpy(t, v, sizeof(t)); } break;
+    case 212:
+        if (kind == 0) { /* ExtendWfThree (src/DVR.cpp:1615-1620): x and xy swapped */
+            double v[8] = {g3, g3, g4, g2, g2, g3, g1, g2}; memcpy(t, v, sizeof(t));
+        } else {
+            double v[8] = {g3, g2, g4, g2, g3, g3, g1, g2}; memcpy(t, v, sizeof(t));
+        }
+        break;
+    case 122: { double v[8] = {g3, g4, g2, g2, g3, g1, g3, g2}; memcpy(t, v, sizeof(t)); } break;
+    default: { double v[8] = {g4, g3, g3, g3, g2, g2, g2, g1}; memcpy(t, v, sizeof(t)); } break;
+    }
+    /* t in the reference's order 0, x, y, z, xy, yz, xz, xyz -> corner index */
+    static const int idx[8] = {0, 1, 2, 4, 3, 6, 5, 7};
+    for (int k = 0; k < 8; ++k) w[idx[k]] = t[k];
+}
+
+void or_dual_weights(int kind, int rx, int ry, int rz, double *w) { tri_weights(kind, rx, ry, rz, w); }
+
+/* fine[n^3] from coarse[nc^3] (both x fastest); kind 0 wavefunction, 1 potential */
+int or_dual_extend(int n, int nc, int kind, const double *coarse, double *fine)
+{
+    if (n % 2 == 1) {
+        if (nc != (n + 1) / 2) return -1;
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) {
+                    int p[3] = {i / 2, j / 2, k / 2}, odd[3] = {i & 1, j & 1, k & 1};
+                    int m = odd[0] + odd[1] + odd[2];
+                    double s = 0.0;
+                    for (int c = 0; c < 8; ++c) {
+                        int a = c & 1, b = (c >> 1) & 1, e = (c >> 2) & 1;
+                        if ((a && !odd[0]) || (b && !odd[1]) || (e && !odd[2])) continue;
+                        s += coarse[(p[0] + a) + (long)nc * ((p[1] + b) + (long)nc * (p[2] + e))];
+                    }
+                    fine[i + (long)n * (j + (long)n * k)] = s / (double)(1 << m);
+                }
+        return 0;
+    }
+    if (nc != (n / 2 + 1) / 3 * 2) return -1;
+    int check = n - (nc * 3 / 2 - 1) * 2, sh = check / 2;
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                int q[3] = {i - sh, j - sh, k - sh};
+                double v = 0.0;
+                int out = 0;
+                for (int d = 0; d < 3; ++d)
+                    if (q[d] < 0 || q[d] >= n - check) out = 1;
+                if (!out) {
+                    double w[8];
+                    tri_weights(kind, q[0] % 3, q[1] % 3, q[2] % 3, w);
+                    int p[3] = {q[0] / 3, q[1] / 3, q[2] / 3};
+                    for (int c = 0; c < 8; ++c) {
+                        if (w[c] == 0.0) continue;
+                        int a = c & 1, b = (c >> 1) & 1, e = (c >> 2) & 1;
+                        v += coarse[(p[0] + a) + (long)nc * ((p[1] + b) + (long)nc * (p[2] + e))] * w[c];
+                    }
+                }
+                fine[i + (long)n * (j + (long)n * k)] = v;
+            }
+    return 0;
+}

@@ -11,9 +11,12 @@
  * Parity: each function's comment ends with what pins it -- the reference's own
  * outputs on the same inputs (tests/golden/, written by tests/golden/make_golden.py
  * from oracle/_ref/ref_driver built out of the reference's sine_dvr.cpp,
- * Davidson.cpp, Potential.cpp, MolPolAux.cpp and DerivDDTensor.cpp), the
- * mathematics (tests/test_oracle_math.py), or "parity unpinned" where neither
- * reaches.  The table in oracle/README.md collects them.
+ * Davidson.cpp, Potential.cpp, MolPolAux.cpp, DerivDDTensor.cpp, Gradients.cpp,
+ * DPP.cpp and Water.cpp), the mathematics (tests/test_oracle_*.py), or "parity
+ * unpinned" where neither reaches.  The table in oracle/README.md collects them.
+ * Parity unpinned: or_piab_start (the half-box start vector; DVR.cpp, where it
+ * lives, needs FFTW headers and does not build here; it changes only the
+ * iteration path, and the reference fixtures were run from the same start).
  */
 #ifndef PISCES_ORACLE_H
 #define PISCES_ORACLE_H
@@ -234,6 +237,15 @@ void or_gradient_final(int nsites, int np, const double *gel, const double *tmu,
                        const double *polgrad, double *gfinal);
 /* Pinned: the reference's own EvaluateGradient / FinalGradient /
  * SubtractWWGradient through WaterCluster (grad_*.npz). */
+
+/* Dual-grid transfer, cubic grids: the coarse grid of n points on
+ * length L (src/ClusterAnion.cpp:209-232), the corner weights of triple
+ * spacing for residues (rx, ry, rz) (w[a + 2b + 4c]; kind 0 ExtendWfThree,
+ * 1 InterpolVtriple), and fine[n^3] from coarse[nc^3] (src/DVR.cpp:1240-1722).
+ * Triple spacing leaves the points outside the coarse grid's hull at 0. */
+void or_dual_coarse(int n, double L, int *nc, double *Lc);
+void or_dual_weights(int kind, int rx, int ry, int rz, double *w);
+int or_dual_extend(int n, int nc, int kind, const double *coarse, double *fine);
 
 /* Threads used by the OpenMP loops (0 = library default). */
 void or_set_threads(int n);
