@@ -134,6 +134,59 @@ def families(fam):
     return out
 
 
+# matrix-core peaks of MI355X (dense): FP64 78.6, FP32 157.3, BF16 2500 TFLOP/s
+PEAK_TFS = {"fp64": 78.6, "fp32": 157.3, "bf16x2": 2500.0}
+
+
+def hx_roofline(hx, fam):
+    """The fused H*X against both roofline terms: the standalone 64^3 x 8
+    application per precision (bytes (2 + 1/k) N s, flops (6n + 4) N k, the
+    split-BF16 form executing three BF16 products per contraction) as
+    max(bytes / 8 TB/s, flops / peak) / measured time, and the served batch's
+    split-BF16 launches (HBM-bound: bytes only)."""
+    out = {}
+    for name, e in (("fp64", 8), ("fp32", 4), ("bf16x2", 4)):
+        by, fl = hx_algorithmic(N_GRID, 8, e)
+        fl_exec = fl * (3 if name == "bf16x2" else 1)
+        t = hx[name]["ms"] * 1e-3
+        t_bytes, t_flops = by / (HBM_PEAK_GBS * 1e9), fl_exec / (PEAK_TFS[name] * 1e12)
+        out[name + "_64cube_x8"] = {
+            "bytes": by, "flops": fl, "flops_executed": fl_exec, "ms": hx[name]["ms"],
+            "t_bytes_us": round(t_bytes * 1e6, 3), "t_flops_us": round(t_flops * 1e6, 3),
+            "bound": "hbm" if t_bytes >= t_flops else "mfma",
+            "frac": round(max(t_bytes, t_flops) / t, 4),
+            "hbm_frac": round(t_bytes / t, 4)}
+    if fam and "hx_lo" in fam and fam["hx_lo"][1]:
+        ms, nl, by = fam["hx_lo"]
+        gbs = by / (ms * 1e-3) / 1e9
+        out["served_bf16x2"] = {"launches": nl, "ms_per_launch": round(ms / nl, 5),
+                                "bytes_per_launch": round(by / nl), "GBs_algorithmic": round(gbs, 1),
+                                "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def potential_roofline(fam, batch):
+    """The grid build of the potential (FP64 VALU-bound: rsqrt and exp per site
+    term) in the served batch: executed FP64 flops per launch from the
+    per-point count that the SQ_INSTS_VALU_*_F64 counters give for this
+    workload (profiles/potential_pmc.json), over the launch time, against the
+    FP64 vector peak."""
+    if not fam or "potential" not in fam or not fam["potential"][1]:
+        return None
+    ms, nl, _ = fam["potential"]
+    path = os.path.join(ROOT, "profiles", "potential_pmc.json")
+    try:
+        with open(path) as f:
+            fpp = json.load(f)["flops_per_point"]
+    except (OSError, KeyError, ValueError):
+        return {"ms_per_launch": round(ms / nl, 4), "TFLOPs": None}
+    fl = fpp * N_GRID ** 3 * batch / nl
+    tf = fl / (ms / nl * 1e-3) / 1e12
+    return {"ms_per_launch": round(ms / nl, 4), "flops_per_point": fpp,
+            "flops_per_launch": fl, "TFLOPs": round(tf, 2), "peak_TFLOPs": FP64_PEAK_TFS,
+            "frac": round(tf / FP64_PEAK_TFS, 4), "bound": "fp64 valu"}
+
+
 def roofline_of(fam, dt):
     """Roofline of the launch family with the most time in the one-lane pass."""
     fo = families(fam)
@@ -174,6 +227,10 @@ def main():
                     help="autotune the Gram kernel configuration on the served workload and "
                          "write pisces-1_amd/tuning.txt")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only a plain device-to-device copy of BYTES (the calibration of the "
+                         "FETCH_SIZE / WRITE_SIZE byte counters: one launch moves BYTES in and "
+                         "BYTES out)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: one GPU only; this build has no multi-GPU path (--gpus %d)" % a.gpus,
@@ -187,6 +244,9 @@ def main():
     import pisces_amd as P
     import workload
 
+    if a.copy_probe:
+        print(json.dumps(copy_probe(P, a.copy_probe, max(1, a.steps))), flush=True)
+        return
     prec = P.PRECISIONS[a.precision]
     if a.pol3_only:
         print(json.dumps({"pol3_potential": pol3_measure(P, workload)}), flush=True)
@@ -254,6 +314,7 @@ def main():
             by, fl = hx_algorithmic(N_GRID, 8, e)
             hx[name] = {"ms": round(ms, 5), "GBs_algorithmic": round(by / (ms * 1e-3) / 1e9, 1),
                         "TFLOPs": round(fl / (ms * 1e-3) / 1e12, 2)}
+    hx_roof = hx_roofline(hx, fam)
 
     # ---- potential with self-consistent polarizable sites (PolType 3)
     pol3 = None if a.served_only else pol3_measure(P, workload)
@@ -280,7 +341,10 @@ def main():
         "config": {"workload": "(H2O)24 anion snapshots, sine-DVR 64^3 on 40 bohr, 4 states, "
                                "block %d, subspace %d, residual tol 1e-%d, batch %d per step, "
                                "H*X %s" % (BLOCK, MAXSUB, PTOL, B, a.precision),
-                   "batch": B, "parallelism": "single GPU"},
+                   "batch": B, "parallelism": "single GPU",
+                   "note": "the C2 configuration states block 8 and subspace 32; the served "
+                           "4-state class runs block 4, subspace 28 (fewer bytes per step; "
+                           "the FP64 comparator uses the same parameters)"},
         "fp64_gpu": {"single_solves_per_s": round(fp64_single_sps, 3),
                      "single_p50_ms": round(1000 * float(np.percentile(lat64, 50)), 3),
                      "batched_solves_per_s": round(fp64_batched_sps, 3)},
@@ -298,6 +362,8 @@ def main():
         "lanes": 2,
         "autotuned": P.tuning(),
         "hx_standalone_64cube_x8": hx,
+        "hx_roofline": hx_roof,
+        "potential_build": potential_roofline(fam, B),
         "pol3_potential": pol3,
         "cpu_baseline": cpu,
     }
@@ -309,34 +375,16 @@ def main():
 
 
 # ---------------------------------------------------------------- batched / served configs
-# (N, L) by water count (<= 12, <= 22, more), moved up while the box is
-# shorter than the cluster radius + 20 bohr
-C4_GRIDS = [(32, 32.0), (48, 38.0), (64, 44.0)]
-C5_BIG = (96, 56.0)
 SEED_C4 = 4000
 SEED_C5 = 5000
-
-
-def c4_snapshot(workload, i, rng):
-    """Snapshot i of the batched mix: water count uniform in [4, 32], placement as
-    for the other configs (sphere radius growing with the count), 1-4 states."""
-    nw = int(rng.integers(4, 33))
-    radius = max(3.0, 2.1 * nw ** (1.0 / 3.0))
-    xyz, _ = workload.water_cluster(nw, radius, SEED_C4 + i)
-    rc = workload.cluster_radius_bohr(xyz)
-    g = 0 if nw <= 12 else (1 if nw <= 22 else 2)
-    while g < len(C4_GRIDS) - 1 and C4_GRIDS[g][1] < rc + 20.0:
-        g += 1
-    n, L = C4_GRIDS[g]
-    return xyz, n, L, int(rng.integers(1, 5))
 
 
 def bench_c4(P, workload, prec, a, count=512):
     """512 independent snapshots of uneven size submitted together; solves/s of
     the whole batch and per-request latency."""
     rng = np.random.default_rng(SEED_C4)
-    snaps = [c4_snapshot(workload, i, rng) for i in range(count)]
-    log("c4: %d snapshots, grids %s" % (count, {n: sum(s[1] == n for s in snaps) for n, _ in C4_GRIDS}))
+    snaps = [workload.mixed_snapshot(i, rng, SEED_C4) for i in range(count)]
+    log("c4: %d snapshots, grids %s" % (count, {n: sum(s[1] == n for s in snaps) for n, _ in workload.MIX_GRIDS}))
     sv = P.Server(hbm_cap_bytes=200 << 30, max_slots=count, precision=prec, maxsub=MAXSUB,
                   ptol=PTOL)
     for xyz, n, L, nr in snaps[:32]:  # warm-up: shapes, pools, kernels
@@ -349,8 +397,16 @@ def bench_c4(P, workload, prec, a, count=512):
     sv.close()
     ok = [r for r in res if r["status"] == r["evals"].size]
     lat = np.array([r["latency_s"] for r in res])
+    # eigenvalue error against the plain FP64 form on a seeded subset of 32
+    byid = {r["id"]: r for r in res}
+    sub = np.random.default_rng(SEED_C4 + 1).choice(count, 32, replace=False)
+    ev64 = fp64_reference(P, [snaps[i] for i in sub])
+    de = max(float(np.abs(byid[ids[i]]["evals"] - e).max()) for i, e in zip(sub, ev64))
+    log("c4: %.1f solves/s, max |dE| vs FP64 on 32 snapshots %.2e" % (count / dt, de))
     return {"metric": "solves/s, 512 uneven snapshots in one batch (4-32 waters, 32^3/48^3/64^3, "
                       "1-4 states)",
+            "max_abs_dE_vs_fp64_subset32": float("%.3e" % de),
+            "dE_ok": bool(de <= DE_TOL and len(ok) == count),
             "value": round(count / dt, 3), "unit": "solves/s", "n_gpus": 1,
             "higher_is_better": True, "seconds": round(dt, 3),
             "p50_ms": round(1000 * float(np.percentile(lat, 50)), 2),
@@ -362,7 +418,19 @@ def bench_c4(P, workload, prec, a, count=512):
             "config": {"workload": "C4: 512 snapshots, water count uniform in [4, 32], grid "
                                    "32^3/32, 48^3/38 or 64^3/44 bohr by water count (box >= "
                                    "radius + 20), 1-4 states, block 4, subspace 28, tol 1e-7",
-                       "grids": {str(n): sum(s[1] == n for s in snaps) for n, _ in C4_GRIDS}}}
+                       "grids": {str(n): sum(s[1] == n for s in snaps) for n, _ in workload.MIX_GRIDS}}}
+
+
+def fp64_reference(P, items):
+    """Eigenvalues of the plain FP64 form (cold starts, the same server code)
+    for (xyz, n, L, nroots) items."""
+    sv = P.Server(hbm_cap_bytes=200 << 30, max_slots=len(items), precision=P.FP64,
+                  maxsub=MAXSUB, ptol=PTOL)
+    ids = [sv.submit((n,) * 3, (L,) * 3, xyz, nr) for xyz, n, L, nr in items]
+    res = {r["id"]: r for r in sv.run_until_done()}
+    sv.close()
+    assert all(res[i]["status"] == res[i]["evals"].size for i in ids)
+    return [res[i]["evals"] for i in ids]
 
 
 def bench_c5(P, workload, prec, a, nchains=64):
@@ -374,9 +442,9 @@ def bench_c5(P, workload, prec, a, nchains=64):
     rng = np.random.default_rng(SEED_C5)
     chains = []
     for c in range(nchains):
-        xyz, n, L, nr = c4_snapshot(workload, 10000 + c, rng)
+        xyz, n, L, nr = workload.mixed_snapshot(10000 + c, rng, SEED_C4)
         if rng.uniform() < 0.2:
-            n, L = C5_BIG
+            n, L = workload.MIX_BIG
         chains.append({"xyz": xyz, "n": n, "L": L, "nr": nr})
     sv = P.Server(hbm_cap_bytes=200 << 30, max_slots=512, precision=prec, maxsub=MAXSUB,
                   ptol=PTOL)
@@ -388,6 +456,7 @@ def bench_c5(P, workload, prec, a, nchains=64):
     cap = nchains / (time.perf_counter() - t0)
     log("c5: closed-batch capacity %.1f solves/s" % cap)
     out_rates = []
+    submitted, done = {}, {}
     for frac in (0.25, 0.5, 0.75, 0.95):
         rate = frac * cap
         t_start = time.perf_counter()
@@ -401,7 +470,9 @@ def bench_c5(P, workload, prec, a, nchains=64):
                 c = k % nchains
                 ch = chains[c]
                 ch["xyz"] = workload.perturb(ch["xyz"], rng)
-                sv.submit((ch["n"],) * 3, (ch["L"],) * 3, ch["xyz"], ch["nr"], chain=c, warm=True)
+                rid = sv.submit((ch["n"],) * 3, (ch["L"],) * 3, ch["xyz"], ch["nr"], chain=c,
+                                warm=True)
+                submitted[rid] = (ch["xyz"], ch["n"], ch["L"], ch["nr"])
                 k += 1
                 next_t += rng.exponential(1.0 / rate)
             live = sv.step()
@@ -411,6 +482,7 @@ def bench_c5(P, workload, prec, a, nchains=64):
             if live == 0 and sv.pending() == 0:
                 time.sleep(min(0.0005, max(0.0, next_t - now)))
         elapsed = time.perf_counter() - t_start
+        done.update({r["id"]: r for r in res})
         lat = np.array([r["latency_s"] for r in res]) if res else np.zeros(1)
         out_rates.append({"offered_per_s": round(rate, 2), "submitted": k, "completed": len(res),
                           "solves_per_s": round(len(res) / elapsed, 2),
@@ -422,12 +494,26 @@ def bench_c5(P, workload, prec, a, nchains=64):
             % (rate, out_rates[-1]["solves_per_s"], out_rates[-1]["p50_ms"],
                out_rates[-1]["p99_ms"]))
     sv.close()
+    # warm-started results against cold plain-FP64 solves of the same snapshots:
+    # a seeded sample of 24 completed requests, the 96^3 ones among them
+    big = [i for i in done if submitted[i][1] == workload.MIX_BIG[0]]
+    small = [i for i in done if submitted[i][1] != workload.MIX_BIG[0]]
+    pick = np.random.default_rng(SEED_C5 + 1)
+    sample = list(pick.choice(big, min(8, len(big)), replace=False)) + \
+        list(pick.choice(small, min(16, len(small)), replace=False))
+    ev64 = fp64_reference(P, [submitted[i] for i in sample])
+    de = max(float(np.abs(done[i]["evals"] - e).max()) for i, e in zip(sample, ev64))
+    nconv = sum(r["status"] == r["evals"].size for r in done.values())
+    log("c5: max |dE| of %d warm results vs cold FP64 %.2e" % (len(sample), de))
     best = max(out_rates, key=lambda r: r["solves_per_s"])
     return {"metric": "served solves/s and p50/p99 latency under open-loop Poisson load "
                       "(warm-started MD chains, 20% at 96^3)",
             "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
             "higher_is_better": True, "closed_batch_capacity_per_s": round(cap, 2),
             "rates": out_rates,
+            "max_abs_dE_vs_fp64_cold_sample": float("%.3e" % de), "dE_sample": len(sample),
+            "converged": nconv, "completed": len(done),
+            "dE_ok": bool(de <= DE_TOL and nconv == len(done)),
             "dtype": {"fp32": "f32+f64", "bf16x2": "bf16x2+f64"}[a.precision],
             "data": "synthetic (seeded water clusters, rigid-move chains)",
             "config": {"workload": "C5: %d chains, sizes as C4 plus 20%% at 96^3/56 bohr, "
@@ -470,6 +556,20 @@ def tune_gram(a):
             f.write("# %s: %.1f GB/s\n" % (c, res[c]["gram_GBs"]))
         f.write("gram %s %s\n" % (v, r))
     return {"tuned": "gram", "best": best, "candidates": res, "written": path}
+
+
+def copy_probe(P, nbytes, reps):
+    """A plain streaming copy of nbytes, reps timed launches after one warm-up
+    (pisces_copy_probe): under rocprofv3 --pmc it calibrates the byte counters,
+    alone it gives the achievable copy bandwidth."""
+    import ctypes
+    L = P.lib()
+    L.pisces_copy_probe.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_double)]
+    ms = ctypes.c_double()
+    P.check(L.pisces_copy_probe(0, nbytes, reps, ctypes.byref(ms)), "pisces_copy_probe")
+    return {"bytes": nbytes, "launches": reps + 1, "ms_per_launch": ms.value,
+            "GBs_read_plus_write": 2 * nbytes / (ms.value * 1e-3) / 1e9}
 
 
 def measured_traffic(family, algorithmic_bytes_per_launch):
@@ -565,13 +665,51 @@ def cpu_baseline(P, xyz, seconds):
         macro *= 2
     per_mv = t_dav / max(1, rr["n_mtx"])
     t_solve = t_pot + per_mv * rref["n_matvec"]
+    # one core: the same per-matvec time on a single thread (two macro iterations)
+    oracle.set_threads(1)
+    t1 = time.perf_counter()
+    r1 = g.solve(V, NROOTS, REF_MAXSUB, 2, PTOL, 1, start)
+    per_mv1 = (time.perf_counter() - t1) / max(1, r1["n_mtx"])
+    t1p = time.perf_counter()
+    cl.potential_grid(g)
+    t_pot1 = time.perf_counter() - t1p
+    oracle.set_threads(cores)
+    # a completed solve at the CPU-runnable configuration: (H2O)6, 32^3 on 36
+    # bohr, one state, residual 1e-7, the reference algorithm to the end
+    xyz1, _ = __import__("workload").water_cluster(6, 3.5, 1000)
+    t2 = time.perf_counter()
+    cl1 = oracle.Cluster(xyz1)
+    g1 = oracle.Grid((32,) * 3, (36.0,) * 3)
+    V1 = cl1.potential_grid(g1)
+    r1c = g1.solve(V1, 1, 20, 200, PTOL, 1, g1.piab().reshape(1, -1))
+    t_c1 = time.perf_counter() - t2
     return {"value": round(1.0 / t_solve, 6), "unit": "solves/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": "oracle potential build (%.2f s) + its Davidson limited to %d macro "
                       "iterations (%d matvecs, %.1f s); extrapolated to the reference "
                       "algorithm's %d matvecs on this snapshot (it converges %d of %d states "
                       "within 200 macro iterations)"
                       % (t_pot, macro, rr["n_mtx"], t_dav, rref["n_matvec"], rref["nconv"],
-                         NROOTS)}
+                         NROOTS),
+            "one_core": {"solves_per_s": round(1.0 / (t_pot1 + per_mv1 * rref["n_matvec"]), 7),
+                         "s_per_matvec": round(per_mv1, 4), "potential_s": round(t_pot1, 3),
+                         "sample": "the same extrapolation from 2 macro iterations on 1 thread"},
+            "c1_completed": {"solves_per_s": round(1.0 / t_c1, 4), "seconds": round(t_c1, 3),
+                             "cores": cores, "nconv": r1c["nconv"], "matvecs": r1c["n_mtx"],
+                             "eval": round(float(r1c["evals"][0]), 10),
+                             "workload": "(H2O)6, 32^3 on 36 bohr, 1 state, tol 1e-7, potential "
+                                         "build + the reference Davidson to convergence"}}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":

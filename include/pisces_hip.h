@@ -53,9 +53,27 @@ const char *pisces_version(void);
 const char *pisces_strerror(int code);
 /* Message of the last failing call on this thread (empty if none). */
 const char *pisces_last_error(void);
-/* Selects the H*X kernel family: 0 LDS-tiled VALU, 1 matrix cores (default).
- * For cross-checks; no reference counterpart. */
-int pisces_debug_set_hx_variant(int variant);
+/* Selects the H*X kernels of one context: 1 matrix cores (default: the passes
+ * in separate launches unless PISCES_HX_FUSED=1 / PISCES_FDM_FUSED=1), 2 matrix
+ * cores in separate launches always, 3 matrix cores with the split-BF16 H*X
+ * and kinetic preconditioner each in one XCD-ordered launch (slower than 1 on
+ * the served batch; PISCES_ERR_UNSUPPORTED on a device without eight XCDs),
+ * 0 LDS-tiled VALU (FP64 and FP32; the split-BF16 form as for 2).  For
+ * cross-checks; no reference counterpart. */
+int pisces_debug_set_hx_variant(pisces_ctx *ctx, int variant);
+/* Number of times a wait inside the one-launch split-BF16 H*X reached its spin
+ * limit on the current device since the library was loaded (0 unless the
+ * ordering it relies on broke; the launch then goes on).  No reference
+ * counterpart. */
+unsigned pisces_debug_hx_spin_timeouts(void);
+/* Inverse of a symmetric n x n matrix (row-major) as the PolType 3 set-up
+ * inverts the polarization matrix (ComputeInvA, src/MolPolAux.cpp:39-136, with
+ * dsytrf/dsytri there): method 0 Cholesky where positive definite, else
+ * Gauss-Jordan with partial pivoting (what pisces_build_potential_scp uses);
+ * 1 Cholesky only; 2 Gauss-Jordan only.  Host arithmetic, no device needed.
+ * Errors: PISCES_ERR_ARG, PISCES_ERR_NUMERIC (singular; for method 1 also not
+ * positive definite). */
+int pisces_debug_sym_inverse(int n, const double *a, int method, double *inv);
 
 /* Launch timing: when enabled, the solver brackets each of its batched launch
  * families with HIP events on the stream they run on and accumulates time,
@@ -176,6 +194,52 @@ int pisces_build_cluster_potential(pisces_ctx *ctx, int nw, const double *xyz_an
  * them.  energies[4]: E0, electrostatic, polarization, dispersion (hartree).
  * Host computation; no GPU is used. */
 int pisces_water_energy(int nw, const double *sites, const double *mu, double *energies);
+
+/* Electron side of the analytic gradient (DVR::ComputeGradient's grid loop,
+ * src/DVR.cpp:1099-1176, over Potential::EvaluateDPPGTOPGradient,
+ * src/Gradients.cpp:79-238) for the context's last potential, which must be a
+ * PolType 3 one with Potential 3 and DampType 2 (the flag set the reference's
+ * gradient is written for) and no ions: for the wavefunction psi (host, ngp
+ * doubles, x fastest; DVR::wavefn of the state),
+ *   gel[3 nsites] = sum_g psi_g^2 dV(r_g)/dR_site (charges, Slater cores and
+ *                   the electron's field derivative at the polarizable sites
+ *                   dotted with mu = InvA (E + Epc)),
+ *   tmu[3 np]     = sum_g psi_g^2 mu(r_g),
+ *   mcm[(3 np)^2] = sum_g psi_g^2 mu(r_g) mu(r_g)^T (row-major),
+ * with np and nsites those of the potential.  PISCES_ERR_STATE without such a
+ * potential. */
+int pisces_gradient_terms(pisces_ctx *ctx, const double *psi, double *gel, double *tmu,
+                          double *mcm);
+
+/* Replaces DVR::ComputeGradient (src/DVR.cpp:1099-1176) with its arguments
+ * (src/ClusterAnion.cpp:407): gradient[3 nsites] holds the water-water
+ * gradient from WaterCluster::CalcGradient on entry and the total on return:
+ * += gel, += FinalGradient's terms (src/Gradients.cpp:240-268) from dEfield
+ * (WaterCluster::Calc_dInducedDipoles, 3 nsites x 3 np) and dT_x/y/z
+ * (BuildSuperdT, (3 np)^2 each), -= polgrad (SubtractWWGradient).  nsites: 4
+ * per water, the potential's sites. */
+int pisces_compute_gradient(pisces_ctx *ctx, const double *psi, int nsites, double *gradient,
+                            const double *dT_x, const double *dT_y, const double *dT_z,
+                            const double *polgrad, const double *dEfield);
+
+/* Dual-grid method (ClusterAnion with Polarization 6, src/ClusterAnion.cpp:
+ * 207-299), cubic grids.  pisces_dual_grid_coarse: the coarse grid of an n^3
+ * grid on length L -- (n + 1) / 2 points at twice the spacing for odd n,
+ * (n / 2 + 1) / 3 * 2 points at three times it for even n (:209-232).
+ * pisces_dual_grid_extend: a coarse vector (nc^3 doubles, host) carried to the
+ * fine grid of ctx (ngp doubles, host); kind 0 a wavefunction (ExtendWfDouble /
+ * ExtendWfThree, src/DVR.cpp:1446-1722: the start vector of SVFlag 3), 1 a
+ * potential (InterpolVdouble / InterpolVtriple, :1240-1421).
+ * pisces_dual_grid_potential: the potential of Polarization 6
+ * (DVR::ComputePotential, src/DVR.cpp:776-830): on ctx's grid, whose exact
+ * potential must have been built, every point on the coarse lattice or farther
+ * than rtol (PotPara[10], default 20 bohr) from every site of that build takes
+ * the coarse potential carried over; the others keep their exact value. */
+int pisces_dual_grid_coarse(const int n[3], const double L[3], int nc[3], double Lc[3]);
+int pisces_dual_grid_extend(pisces_ctx *ctx, const int nc[3], const double *coarse, int kind,
+                            double *fine);
+int pisces_dual_grid_potential(pisces_ctx *ctx, const int nc[3], const double *coarse_v,
+                               double rtol);
 
 /* Y[:,v] = H X[:,v] for v < nvec, host buffers (ngp*nvec doubles each).
  * Replaces DVR::MatrixTimesVector (src/mtx.cpp:22-108) called once per vector
@@ -358,8 +422,9 @@ int pisces_server_step(pisces_server *srv);
 int pisces_server_poll(pisces_server *srv, pisces_result *out, int max);
 /* Device bytes an instance of this request would hold (the admission cost). */
 size_t pisces_server_footprint(const pisces_server *srv, const pisces_request *req);
-/* Fetch the Ritz vectors of a finished request (nroots*ngp doubles); the
- * vectors of the 16 most recently finished requests are kept. */
+/* Fetch the Ritz vectors of a finished request that asked for them
+ * (want_vectors; nroots*ngp doubles); those of the 64 most recently finished
+ * such requests are kept on the host. */
 int pisces_server_vectors(pisces_server *srv, long long id, double *vecs);
 /* Requests queued and not yet admitted; device bytes held by live instances. */
 int pisces_server_pending(const pisces_server *srv);

@@ -51,7 +51,18 @@ void pisces_ctx::hx(const void *X, void *Y, long long ld, int nvec, int precisio
     j.V = f64 ? dV64.p : dV32.p;
     j.X = X; j.Y = Y; j.ld = ld;
     const HxJob *d = push_job(j);
-    hx_launch(d, 1, n[2] * nvec, n[1] * nvec, maxn, precision, stream);
+    if (precision == PISCES_BF16X2 && (hx_variant == 3 || (hx_variant == 1 && hx_fused_enabled()))) {
+        // the one-launch form: its plan (with zeroed counters) is uploaded from
+        // pageable memory, which the copy has consumed when it returns
+        std::vector<int> plan(hx_fused_plan_ints(&j, 1));
+        hx_fused_plan(&j, 1, plan.data());
+        dplan.ensure(sizeof(int) * plan.size());
+        PC_HIP(hipMemcpyAsync(dplan.p, plan.data(), sizeof(int) * plan.size(),
+                              hipMemcpyHostToDevice, stream));
+        hx_fused_launch(&j, d, 1, plan.data(), dplan.as<int>(), maxn, stream);
+        return;
+    }
+    hx_launch(d, 1, n[2] * nvec, n[1] * nvec, maxn, precision, stream, hx_variant);
 }
 
 void pisces_ctx::sync() { PC_HIP(hipStreamSynchronize(stream)); }
@@ -69,6 +80,24 @@ void upload(dbuf &b, const void *h, size_t bytes, hipStream_t s)
 {
     b.ensure(bytes);
     PC_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, s));
+}
+
+// The kinetic preconditioner on device tasks (host copies in h): the split-
+// BF16 form in one XCD-ordered launch where the context's H*X variant is the
+// default, else in three launches.
+void precond_launch(pisces_ctx *c, const std::vector<FdmTask> &h, const FdmTask *d, int precision)
+{
+    const int n = (int)h.size();
+    if (precision == PISCES_BF16X2 && (c->hx_variant == 3 || (c->hx_variant == 1 && fdm_fused_enabled()))) {
+        std::vector<int> plan(fdm_fused_plan_ints(n));
+        fdm_fused_plan(h.data(), n, plan.data());
+        c->dplan.ensure(sizeof(int) * plan.size());
+        PC_HIP(hipMemcpyAsync(c->dplan.p, plan.data(), sizeof(int) * plan.size(),
+                              hipMemcpyHostToDevice, c->stream));
+        fdm_fused_launch(d, c->dplan.as<int>(), plan.data(), c->maxn, c->stream);
+        return;
+    }
+    fdm_launch(d, n, c->maxn, precision, c->stream);
 }
 
 void set_potential_device(pisces_ctx *c)
@@ -230,6 +259,7 @@ int build_potential_impl(pisces_ctx *c, const int flags[6], const double *potpar
         PC_REQUIRE((P.pol_type == 3) == (np > 0), PISCES_ERR_ARG,
                    "PolType 3 takes the polarizable-site lists (pisces_build_potential_scp)");
         P.ion_r = ion_r;
+        c->last_sites.assign(sites, sites + 3 * (size_t)nsites);
         dbuf ds, dq, diq, dd, did, drep, direp, dipps, dix, diq2, drp, depc, dinv;
         upload(ds, sites, sizeof(double) * 3 * nsites, c->stream);
         if (nq) { upload(dq, q, sizeof(double) * nq, c->stream); upload(diq, iq, sizeof(int) * nq, c->stream); }
@@ -268,10 +298,49 @@ int build_potential_impl(pisces_ctx *c, const int flags[6], const double *potpar
             upload(dinv, inv.data(), sizeof(double) * inv.size(), c->stream);
             Pol3Dev Q{np, n3p, P.damp_type, 0, P.pdamp, vpol_ww, drp.as<double>(),
                       depc.as<double>(), dinv.as<double>()};
-            ProfScope ps(PK_POL3, c->stream, 0.0);
-            pol3_grid_launch(Q, c->n[0], c->n[1], c->n[2], c->dx[0].as<double>(),
-                             c->dx[1].as<double>(), c->dx[2].as<double>(), c->dV64.as<double>(),
-                             c->stream);
+            {
+                ProfScope ps(PK_POL3, c->stream, 0.0);
+                pol3_grid_launch(Q, c->n[0], c->n[1], c->n[2], c->dx[0].as<double>(),
+                                 c->dx[1].as<double>(), c->dx[2].as<double>(),
+                                 c->dV64.as<double>(), c->stream);
+            }
+            // what the analytic gradient evaluates (EvaluateDPPGTOPGradient is
+            // written for Potential 3 with DampType 2): per site its charge and
+            // Slater cores, the polarizable sites, their field and InvA
+            std::vector<double> s4((size_t)4 * nsites, 0.0), sr((size_t)4 * nsites, 0.0);
+            bool ok = flags[0] == 3 && P.damp_type == 2 && nion == 0;
+            for (int s = 0; s < nsites; ++s)
+                for (int k = 0; k < 3; ++k) s4[4 * s + k] = sites[3 * s + k];
+            for (int i = 0; i < nq; ++i) {
+                ok = ok && s4[4 * iq[i] + 3] == 0.0;  // one charge per site
+                s4[4 * iq[i] + 3] = q[i];
+            }
+            std::vector<int> nr((size_t)nsites, 0);
+            for (size_t i = 0; i < irep.size(); ++i) {
+                const int s = irep[i];
+                if (nr[s] >= 2) { ok = false; break; }
+                sr[4 * s + 2 * nr[s]] = rep[2 * i];
+                sr[4 * s + 2 * nr[s] + 1] = rep[2 * i + 1];
+                ++nr[s];
+            }
+            c->gs.valid = false;
+            if (ok) {
+                GradState &G = c->gs;
+                G.nsites = nsites;
+                G.np = np;
+                G.n3p = n3p;
+                G.qdamp = P.qdamp;
+                G.pdamp = P.pdamp;
+                upload(G.site4, s4.data(), sizeof(double) * s4.size(), c->stream);
+                upload(G.srep, sr.data(), sizeof(double) * sr.size(), c->stream);
+                G.rpol = std::move(drp);
+                G.epc = std::move(depc);
+                G.inva = std::move(dinv);
+                G.ip.assign(ip, ip + np);
+                G.valid = true;
+            }
+        } else {
+            c->gs.valid = false;
         }
         set_potential_device(c);
         c->sync();
@@ -388,6 +457,116 @@ int pisces_build_cluster_potential(pisces_ctx *c, int nw, const double *xyz_ang,
     });
 }
 
+int pisces_gradient_terms(pisces_ctx *c, const double *psi, double *gel, double *tmu, double *mcm)
+{
+    if (!c || !psi || !gel || !tmu || !mcm) return PISCES_ERR_ARG;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        gradient_terms(c, psi, gel, tmu, mcm);
+        return 0;
+    });
+}
+
+int pisces_compute_gradient(pisces_ctx *c, const double *psi, int nsites, double *gradient,
+                            const double *dT_x, const double *dT_y, const double *dT_z,
+                            const double *polgrad, const double *dEfield)
+{
+    if (!c || !psi || !gradient || !dT_x || !dT_y || !dT_z || !polgrad || !dEfield)
+        return PISCES_ERR_ARG;
+    if (!c->gs.valid) {
+        set_last_error("the gradient needs a PolType 3 potential built on this context");
+        return PISCES_ERR_STATE;
+    }
+    if (nsites != c->gs.nsites || c->gs.np * 4 != nsites * 3) {
+        set_last_error("nsites must be the potential's site count, 4 per water, 3 polarizable");
+        return PISCES_ERR_ARG;
+    }
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        const int np = c->gs.np, n3 = 3 * np, n9 = 9 * np;
+        std::vector<double> loc((size_t)3 * nsites), tmu((size_t)n3), mcm((size_t)n3 * n3);
+        gradient_terms(c, psi, loc.data(), tmu.data(), mcm.data());
+        // Potential::FinalGradient (src/Gradients.cpp:240-268): the derivative
+        // of the permanent-charge field dotted with sum psi^2 mu, and for the
+        // polarizable sites (not M) the dipole-tensor derivative against
+        // sum psi^2 mu mu^T over the rows of the site's atom
+        for (int j = 0; j < nsites; ++j) {
+            for (int d = 0; d < 3; ++d) {
+                const double *de = dEfield + (size_t)n3 * (3 * j + d);
+                double s = 0.0;
+                for (int k = 0; k < n3; ++k) s += de[k] * tmu[k];
+                loc[3 * j + d] -= s;
+            }
+            if ((j + 1) % 4 != 0) {
+                const size_t off = (size_t)(3 * (j / 4) + j % 4) * n9;
+                const double *T[3] = {dT_x, dT_y, dT_z};
+                for (int d = 0; d < 3; ++d) {
+                    double s = 0.0;
+                    for (int k = 0; k < n9; ++k) s += T[d][off + k] * mcm[off + k];
+                    loc[3 * j + d] -= s;
+                }
+            }
+        }
+        // the caller's water-water gradient += electron part, then
+        // SubtractWWGradient (src/Gradients.cpp:270-284; src/DVR.cpp:1165-1175)
+        for (int k = 0; k < 3 * nsites; ++k) gradient[k] = gradient[k] + loc[k] - polgrad[k];
+        return 0;
+    });
+}
+
+int pisces_dual_grid_coarse(const int n[3], const double L[3], int nc[3], double Lc[3])
+{
+    if (!n || !L || !nc || !Lc) return PISCES_ERR_ARG;
+    if (n[1] != n[0] || n[2] != n[0] || L[1] != L[0] || L[2] != L[0]) {
+        set_last_error("the dual grid needs a cubic grid");
+        return PISCES_ERR_UNSUPPORTED;
+    }
+    int m;
+    double Lm;
+    if (!dual_coarse_grid(n[0], L[0], m, Lm)) return PISCES_ERR_ARG;
+    for (int a = 0; a < 3; ++a) {
+        nc[a] = m;
+        Lc[a] = Lm;
+    }
+    return PISCES_OK;
+}
+
+int pisces_dual_grid_extend(pisces_ctx *c, const int nc[3], const double *coarse, int kind,
+                            double *fine)
+{
+    if (!c || !nc || !coarse || !fine || (kind != 0 && kind != 1)) return PISCES_ERR_ARG;
+    if (nc[1] != nc[0] || nc[2] != nc[0]) return PISCES_ERR_ARG;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        const long long ncg = (long long)nc[0] * nc[0] * nc[0];
+        dbuf dc(sizeof(double) * ncg), df(sizeof(double) * c->ngp);
+        PC_HIP(hipMemcpyAsync(dc.p, coarse, sizeof(double) * ncg, hipMemcpyHostToDevice, c->stream));
+        dual_transfer(c, nc[0], dc.as<double>(), kind, nullptr, 0, 0.0, df.as<double>());
+        PC_HIP(hipMemcpy(fine, df.p, sizeof(double) * c->ngp, hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+
+int pisces_dual_grid_potential(pisces_ctx *c, const int nc[3], const double *coarse_v, double rtol)
+{
+    if (!c || !nc || !coarse_v || nc[1] != nc[0] || nc[2] != nc[0]) return PISCES_ERR_ARG;
+    if (!c->has_v || c->last_sites.empty()) {
+        set_last_error("build the fine grid's potential first");
+        return PISCES_ERR_STATE;
+    }
+    return guarded([&] {
+        PC_HIP(hipSetDevice(c->device));
+        const long long ncg = (long long)nc[0] * nc[0] * nc[0];
+        dbuf dc(sizeof(double) * ncg);
+        PC_HIP(hipMemcpyAsync(dc.p, coarse_v, sizeof(double) * ncg, hipMemcpyHostToDevice, c->stream));
+        dual_transfer(c, nc[0], dc.as<double>(), 1, c->last_sites.data(),
+                      (int)(c->last_sites.size() / 3), rtol, c->dV64.as<double>());
+        set_potential_device(c);
+        c->sync();
+        return 0;
+    });
+}
+
 int pisces_water_energy(int nw, const double *sites, const double *mu, double *energies)
 {
     if (nw < 0 || (nw > 0 && (!sites || !mu)) || !energies) return PISCES_ERR_ARG;
@@ -501,12 +680,12 @@ int pisces_time_precond(pisces_ctx *c, int nvec, int precision, int warmup, int 
         }
         PC_HIP(hipMemcpyAsync(tasks.p, h.data(), sizeof(FdmTask) * nvec, hipMemcpyHostToDevice,
                               c->stream));
-        for (int w = 0; w < warmup; ++w) fdm_launch(tasks.as<FdmTask>(), nvec, c->maxn, precision, c->stream);
+        for (int w = 0; w < warmup; ++w) precond_launch(c, h, tasks.as<FdmTask>(), precision);
         hipEvent_t e0, e1;
         PC_HIP(hipEventCreate(&e0));
         PC_HIP(hipEventCreate(&e1));
         PC_HIP(hipEventRecord(e0, c->stream));
-        for (int w = 0; w < iters; ++w) fdm_launch(tasks.as<FdmTask>(), nvec, c->maxn, precision, c->stream);
+        for (int w = 0; w < iters; ++w) precond_launch(c, h, tasks.as<FdmTask>(), precision);
         PC_HIP(hipEventRecord(e1, c->stream));
         PC_HIP(hipEventSynchronize(e1));
         float t = 0;
@@ -571,7 +750,7 @@ int pisces_kinetic_precond(pisces_ctx *c, int nvec, const double *r, double shif
         PC_HIP(hipMemcpyAsync(R.p, r, bytes, hipMemcpyHostToDevice, c->stream));
         PC_HIP(hipMemcpyAsync(tasks.p, h.data(), sizeof(FdmTask) * nvec, hipMemcpyHostToDevice,
                               c->stream));
-        fdm_launch(tasks.as<FdmTask>(), nvec, c->maxn, precision, c->stream);
+        precond_launch(c, h, tasks.as<FdmTask>(), precision);
         PC_HIP(hipMemcpyAsync(t, Tb.p, bytes, hipMemcpyDeviceToHost, c->stream));
         c->sync();
         return 0;
@@ -633,12 +812,43 @@ int pisces_solve(pisces_ctx *c, int nroots, int maxsub, int maxmacro, int ptol, 
     });
 }
 
-// internal test hook: select the H*X kernel family (0 VALU, 1 MFMA)
-int pisces_debug_set_hx_variant(int v)
+unsigned pisces_debug_hx_spin_timeouts(void)
 {
-    if (v < 0 || v > 1) return PISCES_ERR_ARG;
-    hx_set_variant(v);
-    return PISCES_OK;
+    unsigned v = 0;
+    guarded([&] {
+        v = hx_fused_spin_timeouts();
+        return 0;
+    });
+    return v;
+}
+
+// test hook: the polarization-matrix inverse behind PolType 3 on the host,
+// with its method chosen (no device work)
+int pisces_debug_sym_inverse(int n, const double *a, int method, double *inv)
+{
+    return guarded([&] {
+        PC_REQUIRE(n > 0 && a && inv && method >= 0 && method <= 2, PISCES_ERR_ARG,
+                   "pisces_debug_sym_inverse: bad arguments");
+        std::vector<double> A(a, a + (size_t)n * n);
+        PC_REQUIRE(sym_inverse(n, A, n, inv, method), PISCES_ERR_NUMERIC,
+                   "matrix singular (or not positive definite for method 1)");
+        return PISCES_OK;
+    });
+}
+
+// test hook: the H*X kernels of one context (0 VALU, 1 matrix cores, 2 matrix
+// cores with every form in its multi-launch shape, 3 matrix cores with the
+// split-BF16 H*X and preconditioner in their one-launch XCD-ordered form)
+int pisces_debug_set_hx_variant(pisces_ctx *c, int v)
+{
+    return guarded([&] {
+        PC_REQUIRE(c && v >= 0 && v <= 3, PISCES_ERR_ARG, "pisces_debug_set_hx_variant: variant 0-3");
+        PC_HIP(hipSetDevice(c->device));
+        PC_REQUIRE(v != 3 || one_launch_available(), PISCES_ERR_UNSUPPORTED,
+                   "the one-launch form needs a device with eight XCDs");
+        c->hx_variant = v;
+        return PISCES_OK;
+    });
 }
 
 }  // extern "C"

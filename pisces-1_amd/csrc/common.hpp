@@ -105,14 +105,27 @@ struct HxJob {
 };
 
 // precision codes: PISCES_FP64 / PISCES_FP32 / PISCES_BF16X2
+// variant 0 selects the LDS-tiled VALU kernels (FP64 and FP32; a cross-check
+// of the matrix-core kernels, which every other value selects)
 void hx_launch(const HxJob *d_jobs, int njobs, int max_planes_x, int max_planes_z,
-               int max_n, int precision, hipStream_t s);
+               int max_n, int precision, hipStream_t s, int variant = 1);
+// Split-BF16 H*X with both passes in one launch, ordered per XCD so that
+// pass B reads X and Y from the L2 that pass A left them in.  The plan (ints,
+// hx_fused_plan_ints of them) is built on the host from the host copy of the
+// jobs and must reach the device unchanged (it carries zeroed counters):
+// every launch takes a freshly uploaded plan.
+int hx_fused_plan_ints(const HxJob *h_jobs, int njobs);
+void hx_fused_plan(const HxJob *h_jobs, int njobs, int *plan);
+void hx_fused_launch(const HxJob *h_jobs, const HxJob *d_jobs, int njobs, const int *h_plan,
+                     int *d_plan, int max_n, hipStream_t s);
+// waits of the fused launch that hit their spin limit (0 unless something is broken)
+unsigned hx_fused_spin_timeouts();
+// whether split-BF16 H*X takes the one-launch form (default; PISCES_HX_FUSED=0: two launches)
+bool hx_fused_enabled();
+bool one_launch_available();
 // matrix-core plane kernels (both passes)
 void hx_mfma_launch(const HxJob *d_jobs, int njobs, int max_planes_a, int max_planes_b,
                     int max_n, int precision, hipStream_t s);
-// Kernel variant selector (0 = VALU LDS-tiled, 1 = MFMA); default 1 where available.
-void hx_set_variant(int v);
-int hx_get_variant();
 
 // --------------------------------------------------------------- potential
 struct PotParams {
@@ -170,6 +183,7 @@ inline int pol3_padded(int np) { return (3 * np + 15) / 16 * 16; }
 // triangles taken from the lower one, zero-padded to n3p x n3p.  false if the
 // matrix is singular.
 bool pol3_inva_host(int np, const double *rpol, const double *alpha, int n3p, double *inva);
+bool sym_inverse(int n, std::vector<double> &A, int n3p, double *inva, int method);
 // Device: d_v[g] += -E^T InvA E / 2 - vpolww at every grid point, E the
 // electron's damped field at the sites plus epc.
 void pol3_grid_launch(const Pol3Dev &D, int nx, int ny, int nz, const double *d_x,

@@ -10,6 +10,16 @@
 #include "blockops.hpp"
 #include "common.hpp"
 
+// Device tables of the last PolType 3 potential built on a context (Potential
+// 3, DampType 2, Polarization 3): what the analytic gradient evaluates.
+struct GradState {
+    bool valid = false;
+    int nsites = 0, np = 0, n3p = 0;
+    double qdamp = 0, pdamp = 0;
+    pisces::dbuf site4, srep, rpol, epc, inva;
+    std::vector<int> ip;
+};
+
 // One sine-DVR grid on one device, with its potential and solver workspace.
 // Re-entrant: everything lives here, nothing in globals.
 struct pisces_ctx {
@@ -25,10 +35,11 @@ struct pisces_ctx {
     pisces::AxisStore ax[3];  // padded operand matrices of the plane kernels
     pisces::dbuf dV64, dV32;
     bool has_v = false;
+    int hx_variant = 1;  // H*X kernels: 1 matrix cores (default), 2 multi-launch only, 3 one-launch, 0 VALU
 
     // job ring for H*X launches
     static constexpr int kJobRing = 256;
-    pisces::dbuf djobs;
+    pisces::dbuf djobs, dplan;
     pisces::HxJob *hjobs = nullptr;  // pinned
     int job_slot = 0;
 
@@ -41,6 +52,9 @@ struct pisces_ctx {
     // reverse-communication Davidson state (pisces_davidson); type-erased so
     // that the state's definition stays in its own translation unit
     std::shared_ptr<void> rc;
+
+    GradState gs;
+    std::vector<double> last_sites;  // sites of the last potential build (3 per site)
 
     // device address of the library's fixed scratch array on this device
     // (pisces_kinetic_precond_noalloc), looked up at creation
@@ -61,6 +75,17 @@ namespace pisces {
 // Address of the allocation-free entry point's scratch array on `device` (the
 // current device); looked up once per device.
 void *noalloc_scratch(int device);
+// The electron side of the analytic gradient for the context's PolType 3
+// potential and the wavefunction psi (host, ngp): gel[3 nsites], tmu[3 np],
+// mcm[(3 np)^2] row-major.
+void gradient_terms(pisces_ctx *c, const double *psi_host, double *gel, double *tmu, double *mcm);
+// Dual grid: the coarse grid of a cubic fine grid (n points on length L), and
+// the coarse -> fine transfer of a device vector (kind 0 wavefunction, 1
+// potential; with sites, only the points farther than rtol from every site or
+// on the coarse lattice are written).
+bool dual_coarse_grid(int n, double L, int &nc, double &Lc);
+void dual_transfer(pisces_ctx *c, int nc, const double *d_coarse, int kind, const double *sites,
+                   int n_sites, double rtol, double *d_fine);
 // Runs f, converting library exceptions into C-ABI return codes.
 template <typename F>
 int guarded_call(F &&f)

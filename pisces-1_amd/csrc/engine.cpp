@@ -152,18 +152,18 @@ BdEngine::BdEngine(int device, hipStream_t s, int precision, int block, int maxs
     // the FP64 refresh of H X while every wanted residual exceeds r (0: never skip)
     const char *rb = std::getenv("PISCES_REFRESH_BELOW");
     if (rb) refresh_below_ = std::atof(rb) > 0 ? std::atof(rb) : 1e300;
-    // PISCES_AMP2_MAX=<a>: keep the second orthogonalisation round's launches on
-    // the data, taken where the first round's squared amplification exceeds a
-    // (1: always).  By default both rounds come from S whatever the
-    // amplification: its error is ~1e-16 times it (1.1-3.8 on the bench), and
-    // convergence is only ever declared on a re-orthonormalised block with an
-    // FP64-fresh H X, so a less orthogonal block could cost iterations, not
-    // accuracy.
+    // The second orthogonalisation round of the FP32-stored forms comes from S
+    // (no pass over the data) unless the first round's squared amplification
+    // |T|^2 / |T - V h0|^2 of a kept column exceeds amp2_max: the algebraic
+    // round's error is ~1e-16 times that amplification (1.1-3.8 on the bench),
+    // so at the default 1e6 it stays below 1e-10, and an instance above it runs
+    // the round on its data (the launches are there and skip the others on the
+    // device).  PISCES_AMP2_MAX=<a> sets the threshold (1: always on the data;
+    // 0: never, whatever the amplification).
     const char *am = std::getenv("PISCES_AMP2_MAX");
-    if (am && std::atof(am) > 0) {
-        data_round2_ = true;
-        amp2_max_ = std::atof(am);
-    }
+    const double a2 = am ? std::atof(am) : 1e6;
+    data_round2_ = a2 > 0;
+    amp2_max_ = a2 > 0 ? a2 : 1e300;
     gram_load_config();
 }
 
@@ -392,6 +392,7 @@ struct CombBatch {
 
 struct HxBatch {
     HxJob *h = nullptr, *d = nullptr;
+    int *plan_h = nullptr, *plan_d = nullptr;  // one-launch split-BF16 form
     int n = 0, pa = 0, pb = 0, mx = 0;
     void add(const HxJob &j)
     {
@@ -407,12 +408,21 @@ struct HxBatch {
         for (int i = 0; i < n; ++i) b += (double)(2 * h[i].nvec + 1) * h[i].nx * h[i].ny * h[i].nz * e;
         return b;
     }
+    // The split-BF16 form runs both passes in one XCD-ordered launch; its plan
+    // goes into the arena with the jobs (before the arena's upload).
+    template <typename Ar> void plan(Ar &A, int precision)
+    {
+        if (!n || precision != PISCES_BF16X2 || !hx_fused_enabled()) return;
+        plan_h = A.template take<int>(hx_fused_plan_ints(h, n), &plan_d);
+        hx_fused_plan(h, n, plan_h);
+    }
     void launch(int precision, hipStream_t s) const
     {
         if (!n) return;
         const size_t e = precision == PISCES_FP64 ? 8 : 4;
         ProfScope ps(precision == PISCES_FP64 ? PK_HX64 : PK_HXLO, s, prof_enabled() ? bytes(e) : 0.0);
-        hx_launch(d, n, pa, pb, mx, precision, s);
+        if (plan_d) hx_fused_launch(h, d, n, plan_h, plan_d, mx, s);
+        else hx_launch(d, n, pa, pb, mx, precision, s);
     }
 };
 
@@ -450,9 +460,11 @@ void BdEngine::step_a()
     const size_t tmax = std::max({sizeof(GramTask), sizeof(CombTask), sizeof(PrecTask),
                                   sizeof(HxJob), sizeof(EigTask), sizeof(RitzTask), sizeof(OrthTask),
                                   sizeof(CholTask), sizeof(FdmTask), sizeof(Orth2Task)});
-    // at most ~44 descriptors per instance and phase, plus alignment slack
-    arena_[0]->reserve((tmax * 56 + 64 * 48) * (size_t)(n + 1), s_);
-    arena_[1]->reserve((tmax * 56 + 64 * 48) * (size_t)(n + 1), s_);
+    // at most ~44 descriptors per instance and phase, plus alignment slack and
+    // the plan of a one-launch H*X (13 ints per vector) or preconditioner (20)
+    const size_t per = tmax * 56 + 64 * 48 + sizeof(int) * 21 * kBdMaxK;
+    arena_[0]->reserve(per * (size_t)(n + 1), s_);
+    arena_[1]->reserve(per * (size_t)(n + 1), s_);
     if (dgather_.bytes < sizeof(double) * 2 * kBdMaxK * (size_t)n) {
         PC_HIP(hipStreamSynchronize(s_));
         dgather_.alloc(sizeof(double) * 2 * kBdMaxK * (size_t)std::max(n, 64));
@@ -597,6 +609,7 @@ void BdEngine::step_a()
             else in->res.n_mv64 += in->nnew;
         }
     }
+    hc.plan(A, prec_);
     // new columns of G
     GramBatch gg;
     int ngg = 0;
@@ -936,6 +949,12 @@ int BdEngine::step_b()
             }
         }
     }
+    // the split-BF16 preconditioner's three passes in one XCD-ordered launch
+    int *fplan_h = nullptr, *fplan_d = nullptr;
+    if (precond_ != 1 && lo && nfd && fdm_fused_enabled()) {
+        fplan_h = B.take<int>(fdm_fused_plan_ints(nfd), &fplan_d);
+        fdm_fused_plan(hf, nfd, fplan_h);
+    }
     GramBatch og[2];
     CombBatch oc[2];
     OrthTask *hot[2], *dot[2];
@@ -1046,6 +1065,7 @@ int BdEngine::step_b()
             pb += (double)in->g.N * in->nt * ((r32 ? 4 : 8) + et) + (precond_ == 1 ? 8.0 * in->g.N : 0.0);
         ProfScope ps(PK_PREC, s_, pb);
         if (precond_ == 1) bd_precond(dpc, nc, maxN, s_);
+        else if (fplan_d) fdm_fused_launch(dfd, fplan_d, fplan_h, fd_maxn, s_);
         else fdm_launch(dfd, nfd, fd_maxn, lo ? PISCES_BF16X2 : PISCES_FP64, s_);
     }
     for (int pass = 0; pass < 2; ++pass) {

@@ -102,11 +102,10 @@ private:
     size_t hsmall_cap_ = 0;
     bool trace_ = false;  // PISCES_TRACE=1: per-step Ritz values of the first instance
     double refresh_below_ = 1e-3;  // late restarts refresh H X in FP64 below this residual
-    // PISCES_AMP2_MAX set: the second orthogonalisation round stays launched on
-    // the data and runs where the first round's squared amplification exceeds
-    // amp2_max_; unset: both rounds always come from S (no second-round launches)
-    bool data_round2_ = false;
-    double amp2_max_ = 1e300;
+    // second orthogonalisation round on the data where the first round's
+    // squared amplification exceeds amp2_max_ (default 1e6; PISCES_AMP2_MAX)
+    bool data_round2_ = true;
+    double amp2_max_ = 1e6;
     void grow_part(size_t bytes);
     std::vector<dbuf> pool_;  // released instance buffers, reused by add()
     std::vector<Inst *> act_, rs_, lt_;  // the step between step_a() and step_b()

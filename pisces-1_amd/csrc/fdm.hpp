@@ -22,4 +22,16 @@ struct FdmTask {
 // precision: PISCES_FP64 (FP64 matrix cores) or PISCES_BF16X2 (split BF16)
 void fdm_launch(const FdmTask *d_tasks, int ntasks, int max_n, int precision, hipStream_t s);
 
+// The split-BF16 form with its three passes in one XCD-ordered launch (each
+// pass reads the intermediate the previous one left in the L2).  The plan
+// (fdm_fused_plan_ints ints, built on the host from the host copy of the
+// tasks) must reach the device unchanged with every launch: it carries zeroed
+// counters.
+int fdm_fused_plan_ints(int ntasks);
+void fdm_fused_plan(const FdmTask *h_tasks, int ntasks, int *plan);
+void fdm_fused_launch(const FdmTask *d_tasks, int *d_plan, const int *h_plan, int max_n,
+                      hipStream_t s);
+// default on (PISCES_FDM_FUSED=0: three launches)
+bool fdm_fused_enabled();
+
 }  // namespace pisces

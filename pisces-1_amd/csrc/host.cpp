@@ -343,7 +343,7 @@ bool pol3_inva_host(int np, const double *rpol, const double *alpha, int n3p, do
 {
     const int n = 3 * np;
     if (np <= 0 || n3p < n) return false;
-    std::vector<double> A((size_t)n * n, 0.0), X((size_t)n * n, 0.0);
+    std::vector<double> A((size_t)n * n, 0.0);
     for (int i = 0; i < np; ++i) {
         if (!(alpha[i] > 0.0)) return false;
         for (int c = 0; c < 3; ++c) A[(size_t)n * (3 * i + c) + 3 * i + c] = 1.0 / alpha[i];
@@ -363,10 +363,20 @@ bool pol3_inva_host(int np, const double *rpol, const double *alpha, int n3p, do
                     A[(size_t)n * (3 * j + w) + 3 * i + u] = -t;
                 }
         }
+    return sym_inverse(n, A, n3p, inva, 0);
+}
+
+// Inverse of the symmetric n x n matrix A (row-major, overwritten) into the
+// leading n x n block of inva (pitch n3p; the rest zeroed).  method 0: Cholesky
+// where A is positive definite, else Gauss-Jordan; 1: Cholesky only (false if
+// A is not positive definite); 2: Gauss-Jordan only.  False on a singular or
+// non-finite result.
+bool sym_inverse(int n, std::vector<double> &A, int n3p, double *inva, int method)
+{
     // symmetric positive definite (any physical set of sites): Cholesky A = L L^T,
     // M = L^-1, InvA = M^T M -- about n^3 / 2 multiply-adds, every inner loop a
     // dot product of contiguous rows
-    {
+    if (method != 2) {
         std::vector<double> L((size_t)n * n, 0.0), Mt((size_t)n * n, 0.0);
         bool spd = true;
         for (int j = 0; j < n && spd; ++j) {
@@ -408,8 +418,10 @@ bool pol3_inva_host(int np, const double *rpol, const double *alpha, int n3p, do
                 if (!std::isfinite(inva[k])) return false;
             return true;
         }
+        if (method == 1) return false;
     }
     // otherwise Gauss-Jordan elimination with partial pivoting on [A | I]
+    std::vector<double> X((size_t)n * n, 0.0);
     for (int k = 0; k < n; ++k) X[(size_t)n * k + k] = 1.0;
     for (int c = 0; c < n; ++c) {
         int piv = c;

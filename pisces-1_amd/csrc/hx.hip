@@ -16,10 +16,6 @@
 
 namespace pisces {
 
-static int g_variant = 1;
-void hx_set_variant(int v) { g_variant = v; }
-int hx_get_variant() { return g_variant; }
-
 namespace {
 
 constexpr int kThreads = 256;
@@ -139,11 +135,11 @@ __global__ void __launch_bounds__(kThreads) hx_plane_valu(const HxJob *__restric
 }  // namespace
 
 void hx_launch(const HxJob *d_jobs, int njobs, int max_planes_a, int max_planes_b,
-               int max_n, int precision, hipStream_t s)
+               int max_n, int precision, hipStream_t s, int variant)
 {
     if (njobs <= 0) return;
     PC_REQUIRE(max_n > 0 && max_n <= 128, PISCES_ERR_UNSUPPORTED, "grid dimension above 128");
-    if (g_variant != 0 || precision == PISCES_BF16X2) {
+    if (variant != 0 || precision == PISCES_BF16X2) {
         hx_mfma_launch(d_jobs, njobs, max_planes_a, max_planes_b, max_n, precision, s);
         return;
     }

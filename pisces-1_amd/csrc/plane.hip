@@ -427,17 +427,16 @@ __device__ __forceinline__ void afrag_lds(const uint16_t *Pch, const uint16_t *P
 // along i; the wave's Y values of the next plane are loaded as soon as those
 // of the current plane have been used.  LDS: the i-major plane (pass A), plus
 // the c-major plane (pass B).
+// Planes p, p + pstride, ... (MULTI) or plane p alone of job J; returns at
+// once where p is past the job's planes.
 template <int PASS, int NPMAX, bool MULTI>
-__global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ jobs)
+__device__ __forceinline__ void hx_bf2_planes(const HxJob &J, int p, int pstride, unsigned char *smem)
 {
     constexpr int NCT = Own<NPMAX>::NCT, NIT = Own<NPMAX>::NIT, KS = Own<NPMAX>::KS;
-    const HxJob &J = jobs[plane_y(PASS == 1)];
     const AxisMats &Mi = J.ax[0];
     const AxisMats &Mc = J.ax[PASS == 0 ? 1 : 2];
-    int p = plane_x(PASS == 1);
     Geo g;
     if (!geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, p, g)) return;
-    extern __shared__ __align__(16) unsigned char smem[];
     const int pt = g.npc + 8, pc = g.npi + 8;
     uint16_t *Pth = reinterpret_cast<uint16_t *>(smem);
     uint16_t *Ptl = Pth + g.npi * pt;
@@ -471,7 +470,7 @@ __global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ j
         }
         __syncthreads();
         Geo gn;
-        const bool more = MULTI && geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, p + gridDim.x, gn);
+        const bool more = MULTI && geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, p + pstride, gn);
         if (more) own_load(X, gn.base, gn, xb);
 #pragma unroll
         for (int u = 0; u < NCT; ++u) {
@@ -516,7 +515,103 @@ __global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ j
         if (!more) break;
         __syncthreads();  // every wave is done with this plane's LDS copy
         g = gn;
-        p += gridDim.x;
+        p += pstride;
+    }
+}
+
+template <int PASS, int NPMAX, bool MULTI>
+__global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ jobs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    hx_bf2_planes<PASS, NPMAX, MULTI>(jobs[plane_y(PASS == 1)], plane_x(PASS == 1), gridDim.x, smem);
+}
+
+// ------------------------------------------------------------------ H*X, split-BF16, one launch
+// Both passes in one launch, ordered per XCD so that pass B of a vector reads
+// the X and Y that pass A left in that XCD's L2 instead of HBM.  The vectors
+// are split into eight contiguous runs of about equal work, one per XCD; the
+// workgroups of an XCD (read from the hardware's XCC_ID) take tickets from
+// that XCD's counter in order, and the ticket sequence of run s0, s1, ... is
+//   A(s0) | A(s1) B(s0) | A(s2) B(s1) | ... | B(s_last)
+// (A(s): the n_z (i,j) planes of pass A of vector s; B(s): its n_y (i,k)
+// planes).  A workgroup that draws a plane of B(s) waits until every plane
+// of A(s) is stored: each A plane, once its stores have completed (all
+// memory counters drained), adds 1 to the vector's completion count, and the
+// wait polls that count at device scope.  Tickets are drawn in order, so
+// every plane of A(s) was drawn earlier by a resident workgroup that waits on
+// nothing: the wait always ends.  It is bounded all the same (a spin limit;
+// a workgroup past it counts the event in hx_spin_timeouts and goes on).
+// X is not written by the launch and Y of a vector only by its own A and B
+// planes, so no stale copy of either can sit in a CU's L1: the L1 is written
+// through and holds only lines that the CU read after their last writer
+// finished.
+// Plan layout (ints): [0, 8) ticket counters, [8, 16) tickets per XCD,
+// [16, 25) first segment of each XCD, [25] number of vectors, then the
+// vectors' completion counts, then the segments (kSegInts ints each:
+// job, vector, pass, first ticket, planes, vector slot).  The counters arrive
+// zeroed with the plan.
+constexpr int kSegInts = 6;
+constexpr int kXcds = 8;
+__device__ unsigned int hx_spin_timeouts;
+
+template <int NPMAX>
+__global__ void __launch_bounds__(kPT) hx_bf2_fused_kernel(const HxJob *__restrict__ jobs,
+                                                           int *__restrict__ plan)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_task[4];  // job, vector, pass, plane; -1 job: no tickets left
+    const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcds - 1);
+    const int nvecs = plan[25];
+    int *done = plan + 26;
+    const int *segs = plan + 26 + nvecs;
+    const int seg_end = plan[16 + xcc + 1];
+    int cursor = plan[16 + xcc];
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int t = (int)__hip_atomic_fetch_add(reinterpret_cast<unsigned *>(plan) + xcc, 1u,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int job = -1;
+            if (t < plan[8 + xcc]) {
+                while (cursor < seg_end && t >= segs[kSegInts * cursor + 3] + segs[kSegInts * cursor + 4])
+                    ++cursor;
+                const int *sg = segs + kSegInts * cursor;
+                job = sg[0];
+                s_task[1] = sg[1];
+                s_task[2] = sg[2];
+                s_task[3] = t - sg[3];
+                if (sg[2] == 1) {
+                    // pass B: every (i,j) plane of this vector's pass A must be stored
+                    const int need = jobs[sg[0]].nz;
+                    unsigned spins = 0;
+                    while (__hip_atomic_load(done + sg[5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                        __builtin_amdgcn_s_sleep(2);
+                        if (++spins > (1u << 24)) {
+                            atomicAdd(&hx_spin_timeouts, 1u);
+                            break;
+                        }
+                    }
+                }
+            }
+            s_task[0] = job;
+        }
+        __syncthreads();
+        const int job = s_task[0];
+        if (job < 0) break;
+        const HxJob &J = jobs[job];
+        const int pass = s_task[2];
+        // plane index within the job: vector-major, as the two-launch kernels count
+        if (pass == 0) hx_bf2_planes<0, NPMAX, false>(J, s_task[1] * J.nz + s_task[3], 1, smem);
+        else hx_bf2_planes<1, NPMAX, false>(J, s_task[1] * J.ny + s_task[3], 1, smem);
+        if (pass == 0) {
+            // this plane's Y stores have reached the L2 before it is counted
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const int *sg = segs + kSegInts * cursor;
+                __hip_atomic_fetch_add(done + sg[5], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();  // s_task and the LDS planes are reused by the next ticket
     }
 }
 
@@ -1027,18 +1122,16 @@ __device__ __forceinline__ void mm_i_all(const uint16_t *Pch, const uint16_t *Pc
 //   2, plane (i,j) at k: work -> i-major plane; Qy (.) along c -> c-major
 //      plane; (.) Qx^T along i -> t
 // The c-major plane is only read by the wave that wrote its rows.
+// Planes p, p + pstride, ... (MULTI) or plane p alone of one pass of task t.
 template <int PASS, int NPMAX, bool MULTI>
-__global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict__ tasks)
+__device__ __forceinline__ void fdm_bf2_planes(const FdmTask &t, int p, int pstride, unsigned char *smem)
 {
     constexpr int NCT = Own<NPMAX>::NCT, NIT = Own<NPMAX>::NIT, KS = Own<NPMAX>::KS;
-    const FdmTask &t = tasks[plane_y(PASS == 1)];
     const int kind = PASS == 1 ? 1 : 0;
     const AxisMats &Mx = t.ax[0], &My = t.ax[1], &Mz = t.ax[2];
     const AxisMats &Mc = PASS == 1 ? Mz : My;
-    int p = plane_x(PASS == 1);
     Geo g;
     if (!geo_of(kind, Mx.n, My.n, Mz.n, 1, 0, Mx.np, Mc.np, p, g)) return;
-    extern __shared__ __align__(16) unsigned char smem[];
     const int pt = g.npc + 8, pc = g.npi + 8;
     uint16_t *Pth = reinterpret_cast<uint16_t *>(smem);
     uint16_t *Ptl = Pth + g.npi * pt;
@@ -1092,7 +1185,7 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict_
         own_stage(xb, g, PASS == 0 ? nullptr : Pth, Ptl, PASS == 0 ? P2h : nullptr, P2l);
         __syncthreads();
         Geo gn;
-        const bool more = MULTI && geo_of(kind, Mx.n, My.n, Mz.n, 1, 0, Mx.np, Mc.np, p + gridDim.x, gn);
+        const bool more = MULTI && geo_of(kind, Mx.n, My.n, Mz.n, 1, 0, Mx.np, Mc.np, p + pstride, gn);
         if (more) {
             if (s32) own_load(static_cast<const float *>(src), gn.base, gn, xb);
             else own_load(static_cast<const double *>(src), gn.base, gn, xb);
@@ -1123,16 +1216,103 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict_
         }
         if (!more) break;
         g = gn;
-        p += gridDim.x;
+        p += pstride;
+    }
+}
+
+template <int PASS, int NPMAX, bool MULTI>
+__global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict__ tasks)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    fdm_bf2_planes<PASS, NPMAX, MULTI>(tasks[plane_y(PASS == 1)], plane_x(PASS == 1), gridDim.x, smem);
+}
+
+// ------------------------------------------------------------------ preconditioner, one launch
+// The three passes of the split-BF16 preconditioner in one launch, ordered per
+// XCD as the one-launch H*X is: the vectors split into eight contiguous runs,
+// one per XCD, whose ticket sequence is
+//   P0(s0) | P0(s1) P1(s0) | P0(s2) P1(s1) P2(s0) | ... | P2(s_last)
+// so that pass 1 reads the intermediate pass 0 left in the XCD's L2 and pass 2
+// the one pass 1 left there.  A plane of P1(s) waits until every plane of
+// P0(s) is stored, a plane of P2(s) until every plane of P1(s) is; the counts
+// are kept per vector as for H*X, and every awaited plane was drawn earlier by
+// a resident workgroup that is not waiting on a later one.
+// Plan layout (ints): [0, 8) ticket counters, [8, 16) tickets per XCD,
+// [16, 25) first segment of each XCD, [25] vectors, then per vector the
+// completion counts of pass 0 and of pass 1, then the segments (task, 0,
+// pass, first ticket, planes, vector slot).
+template <int NPMAX>
+__global__ void __launch_bounds__(kPT) fdm_bf2_fused_kernel(const FdmTask *__restrict__ tasks,
+                                                            int *__restrict__ plan)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_task[3];  // task, pass, plane; -1 task: no tickets left
+    const unsigned xcc = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcds - 1);
+    const int nvecs = plan[25];
+    int *done = plan + 26;  // done[2 v + pass]
+    const int *segs = plan + 26 + 2 * nvecs;
+    const int seg_end = plan[16 + xcc + 1];
+    int cursor = plan[16 + xcc];
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int t = (int)__hip_atomic_fetch_add(reinterpret_cast<unsigned *>(plan) + xcc, 1u,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int task = -1;
+            if (t < plan[8 + xcc]) {
+                while (cursor < seg_end && t >= segs[kSegInts * cursor + 3] + segs[kSegInts * cursor + 4])
+                    ++cursor;
+                const int *sg = segs + kSegInts * cursor;
+                task = sg[0];
+                s_task[1] = sg[2];
+                s_task[2] = t - sg[3];
+                if (sg[2] > 0) {
+                    // every plane of the previous pass of this vector must be stored
+                    const FdmTask &T = tasks[sg[0]];
+                    const int need = sg[2] == 1 ? T.ax[2].n : T.ax[1].n;
+                    unsigned spins = 0;
+                    while (__hip_atomic_load(done + 2 * sg[5] + sg[2] - 1, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) < need) {
+                        __builtin_amdgcn_s_sleep(2);
+                        if (++spins > (1u << 24)) {
+                            atomicAdd(&hx_spin_timeouts, 1u);
+                            break;
+                        }
+                    }
+                }
+            }
+            s_task[0] = task;
+        }
+        __syncthreads();
+        const int task = s_task[0];
+        if (task < 0) break;
+        const FdmTask &T = tasks[task];
+        const int pass = s_task[1];
+        if (pass == 0) fdm_bf2_planes<0, NPMAX, false>(T, s_task[2], 1, smem);
+        else if (pass == 1) fdm_bf2_planes<1, NPMAX, false>(T, s_task[2], 1, smem);
+        else fdm_bf2_planes<2, NPMAX, false>(T, s_task[2], 1, smem);
+        if (pass < 2) {
+            // this plane's stores have reached the L2 before it is counted
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const int *sg = segs + kSegInts * cursor;
+                __hip_atomic_fetch_add(done + 2 * sg[5] + pass, 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();  // s_task and the LDS planes are reused by the next ticket
     }
 }
 
 // ------------------------------------------------------------------ launch helpers
+// Raises a kernel's dynamic LDS limit to `bytes` (the 160 KB of a CU less the
+// kernel's static LDS); a failure is reported, not left behind as the thread's
+// last HIP error.
 template <typename K>
-void allow_lds(K kernel)
+void allow_lds(K kernel, size_t bytes = 160 * 1024)
 {
-    static_cast<void>(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    PC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
 size_t bf2_lds(int np) { return (size_t)2 * 2 * np * (np + 8) * sizeof(uint16_t); }
@@ -1273,6 +1453,215 @@ void hx_mfma_launch(const HxJob *d_jobs, int njobs, int max_planes_a, int max_pl
         hipLaunchKernelGGL((hx_f_kernel<double, 1>), dim3(max_planes_b, njobs), dim3(kPT), lds, s, d_jobs);
     }
     PC_HIP(hipGetLastError());
+}
+
+int hx_fused_plan_ints(const HxJob *h_jobs, int njobs)
+{
+    int nv = 0;
+    for (int j = 0; j < njobs; ++j) nv += h_jobs[j].nvec;
+    return 26 + nv + kSegInts * 2 * nv;
+}
+
+void hx_fused_plan(const HxJob *h_jobs, int njobs, int *plan)
+{
+    // every vector (job-major) with its plane work; eight contiguous runs of
+    // about equal work, one per XCD, so the vectors of a job mostly share an
+    // XCD and its L2 copy of the job's potential
+    struct Vec { int job, vec; double w; };
+    std::vector<Vec> vs;
+    double total = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const HxJob &J = h_jobs[j];
+        const double w = (double)(J.nz + J.ny) * axis_pad(J.nx) * axis_pad(std::max(J.ny, J.nz));
+        for (int v = 0; v < J.nvec; ++v) {
+            vs.push_back({j, v, w});
+            total += w;
+        }
+    }
+    const int nv = (int)vs.size();
+    std::fill(plan, plan + 26 + nv, 0);
+    plan[25] = nv;
+    int *segs = plan + 26 + nv;
+    int nseg = 0, k = 0;
+    double acc = 0;
+    for (int x = 0; x < kXcds; ++x) {
+        plan[16 + x] = nseg;
+        std::vector<int> run;  // vector indices of this XCD, in order
+        const double upto = total * (x + 1) / kXcds;
+        while (k < nv && (x == kXcds - 1 || acc + 0.5 * vs[k].w <= upto)) {
+            acc += vs[k].w;
+            run.push_back(k++);
+        }
+        int t = 0;
+        auto add = [&](int vi, int pass) {
+            const HxJob &J = h_jobs[vs[vi].job];
+            int *sg = segs + kSegInts * nseg++;
+            sg[0] = vs[vi].job;
+            sg[1] = vs[vi].vec;
+            sg[2] = pass;
+            sg[3] = t;
+            sg[4] = pass == 0 ? J.nz : J.ny;
+            sg[5] = vi;
+            t += sg[4];
+        };
+        // A(s0) | A(s1) B(s0) | ... | B(s_last)
+        for (size_t r = 0; r < run.size(); ++r) {
+            add(run[r], 0);
+            if (r > 0) add(run[r - 1], 1);
+        }
+        if (!run.empty()) add(run.back(), 1);
+        plan[8 + x] = t;
+    }
+    plan[16 + kXcds] = nseg;
+}
+
+namespace {
+template <int NPMAX>
+void hx_fused_np_launch(const HxJob *d_jobs, int *d_plan, int np, int max_tickets, hipStream_t s)
+{
+    static std::once_flag once;
+    std::call_once(once, [] { allow_lds(hx_bf2_fused_kernel<NPMAX>, bf2_lds(NPMAX)); });
+    // workgroups per XCD: two per CU (32 CUs) unless an XCD has fewer planes
+    static const int wpx = ppw_of("PISCES_HXF_WG", 64);
+    const int w = std::max(1, std::min(wpx, max_tickets));
+    hipLaunchKernelGGL((hx_bf2_fused_kernel<NPMAX>), dim3(kXcds * w), dim3(kPT), bf2_lds(np), s,
+                       d_jobs, d_plan);
+}
+}  // namespace
+
+void hx_fused_launch(const HxJob *h_jobs, const HxJob *d_jobs, int njobs, const int *h_plan,
+                     int *d_plan, int max_n, hipStream_t s)
+{
+    if (njobs <= 0) return;
+    PC_REQUIRE(max_n > 0 && max_n <= 128, PISCES_ERR_UNSUPPORTED, "grid dimension above 128");
+    int mt = 0;
+    for (int x = 0; x < kXcds; ++x) mt = std::max(mt, h_plan[8 + x]);
+    const int np = axis_pad(max_n);
+    if (np <= 32) hx_fused_np_launch<32>(d_jobs, d_plan, np, mt, s);
+    else if (np <= 64) hx_fused_np_launch<64>(d_jobs, d_plan, np, mt, s);
+    else hx_fused_np_launch<128>(d_jobs, d_plan, np, mt, s);
+    PC_HIP(hipGetLastError());
+}
+
+bool one_launch_available()
+{
+    // the ordering needs the eight XCDs of one device (SPX partitioning):
+    // with fewer compute units the device is a partition of one XCD
+    static const bool on = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return false;
+        return cus >= 8 * 16;
+    }();
+    return on;
+}
+
+// The one-launch forms are opt-in (PISCES_HX_FUSED=1, PISCES_FDM_FUSED=1): on
+// the served 64^3 batch they take 0.62 ms (H*X) and 0.80 ms (preconditioner)
+// per launch against 0.12 and 0.16 ms for the multi-launch forms -- every
+// ticket costs a device-scope atomic and a store drain in series with its
+// plane, where the multi-launch grids keep thousands of planes in flight.
+bool hx_fused_enabled()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("PISCES_HX_FUSED");
+        return e && e[0] == '1' && one_launch_available();
+    }();
+    return on;
+}
+
+unsigned hx_fused_spin_timeouts()
+{
+    unsigned v = 0;
+    PC_HIP(hipMemcpyFromSymbol(&v, HIP_SYMBOL(hx_spin_timeouts), sizeof(v), 0, hipMemcpyDeviceToHost));
+    return v;
+}
+
+int fdm_fused_plan_ints(int ntasks) { return 26 + 2 * ntasks + kSegInts * 3 * ntasks; }
+
+void fdm_fused_plan(const FdmTask *h_tasks, int ntasks, int *plan)
+{
+    // runs of about equal work per XCD (plane work ~ planes x plane size)
+    std::vector<double> w(ntasks);
+    double total = 0;
+    for (int i = 0; i < ntasks; ++i) {
+        const FdmTask &T = h_tasks[i];
+        w[i] = (double)(2 * T.ax[2].n + T.ax[1].n) * axis_pad(T.ax[0].n) *
+               axis_pad(std::max(T.ax[1].n, T.ax[2].n));
+        total += w[i];
+    }
+    std::fill(plan, plan + 26 + 2 * ntasks, 0);
+    plan[25] = ntasks;
+    int *segs = plan + 26 + 2 * ntasks;
+    int nseg = 0, k = 0;
+    double acc = 0;
+    for (int x = 0; x < kXcds; ++x) {
+        plan[16 + x] = nseg;
+        std::vector<int> run;
+        const double upto = total * (x + 1) / kXcds;
+        while (k < ntasks && (x == kXcds - 1 || acc + 0.5 * w[k] <= upto)) {
+            acc += w[k];
+            run.push_back(k++);
+        }
+        int t = 0;
+        auto add = [&](int ti, int pass) {
+            const FdmTask &T = h_tasks[ti];
+            int *sg = segs + kSegInts * nseg++;
+            sg[0] = ti;
+            sg[1] = 0;
+            sg[2] = pass;
+            sg[3] = t;
+            sg[4] = pass == 1 ? T.ax[1].n : T.ax[2].n;
+            sg[5] = ti;
+            t += sg[4];
+        };
+        // P0(s) | P0(s+1) P1(s) | P0(s+2) P1(s+1) P2(s) | ...
+        const int S = (int)run.size();
+        for (int st = 0; st < S + 2; ++st) {
+            if (st < S) add(run[st], 0);
+            if (st >= 1 && st - 1 < S) add(run[st - 1], 1);
+            if (st >= 2 && st - 2 < S) add(run[st - 2], 2);
+        }
+        plan[8 + x] = t;
+    }
+    plan[16 + kXcds] = nseg;
+}
+
+namespace {
+template <int NPMAX>
+void fdm_fused_np_launch(const FdmTask *d_tasks, int *d_plan, int np, int max_tickets, hipStream_t s)
+{
+    static std::once_flag once;
+    std::call_once(once, [] { allow_lds(fdm_bf2_fused_kernel<NPMAX>, bf2_lds(NPMAX)); });
+    static const int wpx = ppw_of("PISCES_FDMF_WG", 64);
+    const int w = std::max(1, std::min(wpx, max_tickets));
+    hipLaunchKernelGGL((fdm_bf2_fused_kernel<NPMAX>), dim3(kXcds * w), dim3(kPT), bf2_lds(np), s,
+                       d_tasks, d_plan);
+}
+}  // namespace
+
+void fdm_fused_launch(const FdmTask *d_tasks, int *d_plan, const int *h_plan, int max_n,
+                      hipStream_t s)
+{
+    PC_REQUIRE(max_n > 0 && max_n <= 128, PISCES_ERR_UNSUPPORTED, "grid dimension above 128");
+    int mt = 0;
+    for (int x = 0; x < kXcds; ++x) mt = std::max(mt, h_plan[8 + x]);
+    if (mt == 0) return;
+    const int np = axis_pad(max_n);
+    if (np <= 32) fdm_fused_np_launch<32>(d_tasks, d_plan, np, mt, s);
+    else if (np <= 64) fdm_fused_np_launch<64>(d_tasks, d_plan, np, mt, s);
+    else fdm_fused_np_launch<128>(d_tasks, d_plan, np, mt, s);
+    PC_HIP(hipGetLastError());
+}
+
+bool fdm_fused_enabled()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("PISCES_FDM_FUSED");
+        return e && e[0] == '1' && one_launch_available();
+    }();
+    return on;
 }
 
 void fdm_launch(const FdmTask *d_tasks, int ntasks, int max_n, int precision, hipStream_t s)

@@ -181,8 +181,7 @@ void pol3_grid_launch(const Pol3Dev &D, int nx, int ny, int nz, const double *d_
                             reinterpret_cast<const void *>(pol3_grid_kernel<1, 512>),
                             reinterpret_cast<const void *>(pol3_grid_kernel<1, 1024>)};
         for (const void *k : ks)
-            static_cast<void>(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                  kLdsMax));
+            PC_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
     });
     // RB 16-point row blocks per workgroup (PISCES_POL3_RB = 1, 2 or 4).  The E
     // rows in LDS bound how many workgroups a CU holds, and that decides the

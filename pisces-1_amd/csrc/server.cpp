@@ -467,7 +467,7 @@ void retire(pisces_server *sv)
             eng.vectors(L.slot, v.data());
             sv->vecs[L.id] = std::move(v);
             sv->vec_order.push_back(L.id);
-            while (sv->vec_order.size() > 16) {
+            while (sv->vec_order.size() > 64) {
                 sv->vecs.erase(sv->vec_order.front());
                 sv->vec_order.pop_front();
             }

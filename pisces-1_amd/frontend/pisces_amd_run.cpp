@@ -366,13 +366,54 @@ int main(int argc, char **argv)
     }
     if (P.istartvec != 1) return fail("StartVector 1 (particle in a box) is the one supported here");
 
-    pisces_ctx *ctx = nullptr;
+    // Polarization 6, the dual-grid method (src/ClusterAnion.cpp:207-299): a
+    // solve on the coarse grid with self-consistent polarization, then the
+    // fine grid's potential with the coarse one carried over far from the
+    // sites, started from the coarse ground state
+    const bool dual = P.PotFlag[3] == 6;
+    int flags[6];
+    std::copy(P.PotFlag, P.PotFlag + 6, flags);
+    if (dual) flags[3] = 3;
     const double L[3] = {P.gpara[0], P.gpara[1], P.gpara[2]};
-    int rc = pisces_create_ex(&ctx, P.ngrid, L, device, P.DVRType);
+    int rc;
+    int nc[3] = {0, 0, 0};
+    std::vector<double> coarse_v, coarse_wf;
+    if (dual) {
+        if (P.DVRType != 0) return fail("the dual-grid method (Polarization 6) runs on the sine DVR");
+        double Lc[3];
+        if (pisces_dual_grid_coarse(P.ngrid, L, nc, Lc))
+            return fail(std::string("dual grid: ") + pisces_last_error());
+        pisces_ctx *cc = nullptr;
+        if (pisces_create(&cc, nc, Lc, device)) return fail(std::string("coarse grid: ") + pisces_last_error());
+        rc = pisces_build_cluster_potential(cc, P.nWater, xyz.data(), flags, P.PotPara, 0, nullptr,
+                                            nullptr, 2.9, nullptr, nullptr);
+        const long long ncg = (long long)nc[0] * nc[1] * nc[2];
+        std::vector<double> cv((size_t)ncg * P.nStates), cev(P.nStates);
+        pisces_piab(cc, cv.data());
+        std::mt19937_64 cgen(54321);
+        std::uniform_real_distribution<double> cu(-1.0, 1.0);
+        for (size_t k = (size_t)ncg; k < cv.size(); ++k) cv[k] = cu(cgen);
+        pisces_stats cst{};
+        if (rc >= 0)
+            rc = pisces_solve(cc, P.nStates, P.maxSub, P.maxIter, P.ptol, corrflag, precision,
+                              cv.data(), cev.data(), &cst);
+        coarse_v.resize((size_t)ncg);
+        if (rc >= 0) pisces_get_potential(cc, coarse_v.data());
+        pisces_destroy(cc);
+        if (rc < 0) return fail(std::string("coarse grid: ") + pisces_last_error());
+        std::printf("  ***  Energies For pre-calculations    ***  \n");
+        std::printf("       EBE      = %g meV\n", cev[0] * kAu2Mev);
+        std::printf("  coarse grid %d^3 on %.6g bohr: %d matvecs\n", nc[0], Lc[0], cst.n_matvec);
+        coarse_wf.assign(cv.begin(), cv.begin() + ncg);
+    }
+    pisces_ctx *ctx = nullptr;
+    rc = pisces_create_ex(&ctx, P.ngrid, L, device, P.DVRType);
     if (rc) return fail(std::string("pisces_create_ex: ") + pisces_last_error());
     std::vector<double> sites((size_t)12 * P.nWater), mu((size_t)9 * P.nWater);
-    rc = pisces_build_cluster_potential(ctx, P.nWater, xyz.data(), P.PotFlag, P.PotPara, 0, nullptr,
+    rc = pisces_build_cluster_potential(ctx, P.nWater, xyz.data(), flags, P.PotPara, 0, nullptr,
                                         nullptr, 2.9, sites.data(), mu.data());
+    if (rc >= 0 && dual && pisces_dual_grid_potential(ctx, nc, coarse_v.data(), P.PotPara[10]) < 0)
+        rc = -1;
     if (rc < 0) {
         pisces_destroy(ctx);
         return fail(std::string("potential: ") + pisces_last_error());
@@ -386,9 +427,11 @@ int main(int argc, char **argv)
     std::printf("Energy of the neutral cluster E0 = %g\n", E0);
     const long long ngp = (long long)P.ngrid[0] * P.ngrid[1] * P.ngrid[2];
     std::vector<double> vecs((size_t)ngp * P.nStates), ev(P.nStates);
-    // start: the half-box particle-in-a-box function, then uniform [-1, 1) vectors
-    // (src/DVR.cpp:369-513)
-    pisces_piab(ctx, vecs.data());
+    // start: the half-box particle-in-a-box function (or, for the dual grid,
+    // the coarse ground state carried over: SVFlag 3), then uniform [-1, 1)
+    // vectors (src/DVR.cpp:369-513)
+    if (dual) pisces_dual_grid_extend(ctx, nc, coarse_wf.data(), 0, vecs.data());
+    else pisces_piab(ctx, vecs.data());
     std::mt19937_64 gen(12345);
     std::uniform_real_distribution<double> u(-1.0, 1.0);
     for (size_t k = (size_t)ngp; k < vecs.size(); ++k) vecs[k] = u(gen);
@@ -424,7 +467,7 @@ int main(int argc, char **argv)
         return 1;
     }
     expectation_values(P.ngrid, grid.data(), vecs.data(), nconv);
-    std::printf("  matvecs %d, iterations %d, restarts %d, %.3f s (EBE %.12f Eh, E0 %.12f Eh)\n",
+    std::printf("  matvecs %d, iterations %d, restarts %d, %.3f s (EBE %.12f Eh, E0 %.17g Eh)\n",
                 st.n_matvec, st.n_iter, st.n_macro, st.seconds, EBE, E0);
     return 0;
 }

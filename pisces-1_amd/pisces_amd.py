@@ -139,14 +139,83 @@ class DeviceArray:
             pass
 
 
+def dual_coarse(n, L):
+    """The coarse grid of the dual-grid method: ((nc,)*3, (Lc,)*3)."""
+    na = np.ascontiguousarray(n, dtype=np.int32)
+    La = np.ascontiguousarray(L, dtype=np.float64)
+    nc = np.zeros(3, dtype=np.int32)
+    Lc = np.zeros(3)
+    L_ = lib()
+    L_.pisces_dual_grid_coarse.argtypes = [_I, _D, _I, _D]
+    L_.pisces_dual_grid_coarse.restype = ctypes.c_int
+    check(L_.pisces_dual_grid_coarse(_pi(na), _p(La), _pi(nc), _p(Lc)), "pisces_dual_grid_coarse")
+    return tuple(int(v) for v in nc), tuple(float(v) for v in Lc)
+
+
+def hx_spin_timeouts():
+    L = lib()
+    L.pisces_debug_hx_spin_timeouts.restype = ctypes.c_uint
+    return int(L.pisces_debug_hx_spin_timeouts())
+
+
+def sym_inverse(a, method=0):
+    """Inverse of a symmetric matrix by the PolType 3 set-up's host routine
+    (pisces_debug_sym_inverse): method 0 Cholesky else Gauss-Jordan, 1 Cholesky
+    only, 2 Gauss-Jordan only.  Raises PiscesError where the routine fails."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 2 or a.shape[0] != a.shape[1]:
+        raise ValueError("a square matrix expected")
+    out = np.empty_like(a)
+    check(lib().pisces_debug_sym_inverse(a.shape[0], _p(a), int(method), _p(out)),
+          "pisces_debug_sym_inverse")
+    return out
+
+
 def noalloc_scratch_bytes():
     L = lib()
     L.pisces_noalloc_scratch_bytes.restype = ctypes.c_size_t
     return int(L.pisces_noalloc_scratch_bytes())
 
 
-def set_hx_variant(v):
-    check(lib().pisces_debug_set_hx_variant(int(v)), "set_hx_variant")
+def scp_arrays(flags, potpara, sites, q, iq, alpha, ip, epc, ions=None):
+    """Contiguous, typed and size-checked arrays of pisces_build_potential_scp:
+    sites (ns, 3); q and iq (charges and their site indices) of one length;
+    alpha one polarizability per polarizable site ip; epc 3 per polarizable
+    site; every index inside [0, ns).  A mismatch raises ValueError before the
+    C side could read past a host array."""
+    f = np.ascontiguousarray(flags, dtype=np.int32)
+    pp = np.ascontiguousarray(potpara, dtype=np.float64)
+    s = np.ascontiguousarray(sites, dtype=np.float64).reshape(-1)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+    iq = np.ascontiguousarray(iq, dtype=np.int32).reshape(-1)
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1)
+    ip = np.ascontiguousarray(ip, dtype=np.int32).reshape(-1)
+    epc = np.ascontiguousarray(epc, dtype=np.float64).reshape(-1)
+    if f.size < 6:
+        raise ValueError("flags: 6 values expected, got %d" % f.size)
+    if s.size % 3:
+        raise ValueError("sites: %d values, not a multiple of 3" % s.size)
+    ns = s.size // 3
+    if iq.size != q.size:
+        raise ValueError("iq: %d indices for %d charges" % (iq.size, q.size))
+    if alpha.size != ip.size:
+        raise ValueError("alpha: %d polarizabilities for %d polarizable sites"
+                         % (alpha.size, ip.size))
+    if epc.size != 3 * ip.size:
+        raise ValueError("epc: %d values for %d polarizable sites (3 each)"
+                         % (epc.size, ip.size))
+    for name, idx in (("iq", iq), ("ip", ip)):
+        if idx.size and (idx.min() < 0 or idx.max() >= ns):
+            raise ValueError("%s: site index outside [0, %d)" % (name, ns))
+    if ions is None:
+        ix, iqq, nion = np.zeros(3), np.zeros(1), 0
+    else:
+        ix = np.ascontiguousarray(ions[0], dtype=np.float64).reshape(-1)
+        iqq = np.ascontiguousarray(ions[1], dtype=np.float64).reshape(-1)
+        nion = iqq.size
+        if ix.size != 3 * nion:
+            raise ValueError("ions: %d coordinates for %d charges" % (ix.size, nion))
+    return f, pp, s, q, iq, alpha, ip, epc, ix, iqq, nion
 
 
 class Context:
@@ -184,6 +253,14 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_hx_variant(self, v):
+        """H*X kernels of this context: 1 matrix cores (default), 2 matrix cores
+        in separate launches always, 3 split-BF16 H*X and preconditioner in one
+        XCD-ordered launch each, 0 VALU."""
+        L = lib()
+        L.pisces_debug_set_hx_variant.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        check(L.pisces_debug_set_hx_variant(self.h, int(v)), "pisces_debug_set_hx_variant")
 
     def grid(self):
         xyz = np.zeros(sum(self.n))
@@ -228,20 +305,8 @@ class Context:
     def build_potential_scp(self, flags, potpara, sites, q, iq, alpha, ip, epc, vpol_ww,
                             ions=None, ion_r=2.9):
         """Potential with self-consistent polarizable sites (flags[3] == 3)."""
-        f = np.ascontiguousarray(flags, dtype=np.int32)
-        pp = np.ascontiguousarray(potpara, dtype=np.float64)
-        s = np.ascontiguousarray(sites, dtype=np.float64).reshape(-1)
-        q = np.ascontiguousarray(q, dtype=np.float64)
-        iq = np.ascontiguousarray(iq, dtype=np.int32)
-        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
-        ip = np.ascontiguousarray(ip, dtype=np.int32)
-        epc = np.ascontiguousarray(epc, dtype=np.float64).reshape(-1)
-        if ions is None:
-            ix, iqq, nion = np.zeros(3), np.zeros(1), 0
-        else:
-            ix = np.ascontiguousarray(ions[0], dtype=np.float64).reshape(-1)
-            iqq = np.ascontiguousarray(ions[1], dtype=np.float64).reshape(-1)
-            nion = iqq.size
+        f, pp, s, q, iq, alpha, ip, epc, ix, iqq, nion = scp_arrays(
+            flags, potpara, sites, q, iq, alpha, ip, epc, ions)
         check(lib().pisces_build_potential_scp(
             self.h, _pi(f), _p(pp), s.size // 3, _p(s), q.size, _p(q), _pi(iq), ip.size,
             _p(alpha), _pi(ip), _p(epc), ctypes.c_double(vpol_ww), nion, _p(ix), _p(iqq),
@@ -294,6 +359,57 @@ class Context:
                                        float(floor), int(precision), _p(t)),
               "pisces_kinetic_precond")
         return t
+
+    def gradient_terms(self, psi, nsites, npol):
+        """Electron side of the analytic gradient (pisces_gradient_terms) for
+        the context's PolType 3 potential: (gel[3 nsites], tmu[3 npol],
+        mcm[3 npol, 3 npol])."""
+        psi = np.ascontiguousarray(psi, dtype=np.float64).reshape(-1)
+        assert psi.size == self.ngp
+        gel, tmu, mcm = np.zeros(3 * nsites), np.zeros(3 * npol), np.zeros(9 * npol * npol)
+        L = lib()
+        L.pisces_gradient_terms.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
+        L.pisces_gradient_terms.restype = ctypes.c_int
+        check(L.pisces_gradient_terms(self.h, _p(psi), _p(gel), _p(tmu), _p(mcm)),
+              "pisces_gradient_terms")
+        return gel, tmu, mcm.reshape(3 * npol, 3 * npol)
+
+    def compute_gradient(self, psi, gradient, dTx, dTy, dTz, polgrad, dEfield):
+        """DVR::ComputeGradient (pisces_compute_gradient): returns gradient +
+        the electron part - polgrad."""
+        psi = np.ascontiguousarray(psi, dtype=np.float64).reshape(-1)
+        g = np.array(gradient, dtype=np.float64).reshape(-1)
+        a = [np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+             for v in (dTx, dTy, dTz, polgrad, dEfield)]
+        L = lib()
+        L.pisces_compute_gradient.argtypes = [ctypes.c_void_p, _D, ctypes.c_int, _D, _D, _D, _D,
+                                              _D, _D]
+        L.pisces_compute_gradient.restype = ctypes.c_int
+        check(L.pisces_compute_gradient(self.h, _p(psi), g.size // 3, _p(g), *[_p(v) for v in a]),
+              "pisces_compute_gradient")
+        return g
+
+    def dual_extend(self, nc, coarse, kind=0):
+        """Coarse (nc^3) -> this grid: kind 0 wavefunction, 1 potential."""
+        nca = np.ascontiguousarray(nc, dtype=np.int32)
+        c = np.ascontiguousarray(coarse, dtype=np.float64).reshape(-1)
+        fine = np.zeros(self.ngp)
+        L = lib()
+        L.pisces_dual_grid_extend.argtypes = [ctypes.c_void_p, _I, _D, ctypes.c_int, _D]
+        L.pisces_dual_grid_extend.restype = ctypes.c_int
+        check(L.pisces_dual_grid_extend(self.h, _pi(nca), _p(c), int(kind), _p(fine)),
+              "pisces_dual_grid_extend")
+        return fine
+
+    def dual_potential(self, nc, coarse_v, rtol=20.0):
+        """Polarization 6: the coarse potential carried over far from the sites."""
+        nca = np.ascontiguousarray(nc, dtype=np.int32)
+        c = np.ascontiguousarray(coarse_v, dtype=np.float64).reshape(-1)
+        L = lib()
+        L.pisces_dual_grid_potential.argtypes = [ctypes.c_void_p, _I, _D, ctypes.c_double]
+        L.pisces_dual_grid_potential.restype = ctypes.c_int
+        check(L.pisces_dual_grid_potential(self.h, _pi(nca), _p(c), float(rtol)),
+              "pisces_dual_grid_potential")
 
     def kinetic_precond_noalloc(self, d_r, d_t, nvec, shift, scale=-1.0, floor=1e-3,
                                 precision=FP64):

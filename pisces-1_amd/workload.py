@@ -88,3 +88,25 @@ def perturb(xyz, rng, sigma_t=0.05, sigma_deg=2.0):
 def cluster_radius_bohr(xyz):
     p = np.asarray(xyz).reshape(-1, 3)
     return float(np.max(np.linalg.norm(p, axis=1))) * ANGS2BOHR
+
+
+# Batched and served mixes.  Grids (N, L bohr) by water count (<= 12, <= 22,
+# more), moved up while the box is shorter than the cluster radius + 20 bohr.
+MIX_GRIDS = [(32, 32.0), (48, 38.0), (64, 44.0)]
+MIX_BIG = (96, 56.0)
+SEED_MIX = 4000
+
+
+def mixed_snapshot(i, rng, seed0=SEED_MIX):
+    """Snapshot i of the uneven batched mix: water count uniform in [4, 32],
+    placement as for the other workloads (sphere radius growing with the
+    count), 1-4 states.  Returns (xyz, n, L, nroots)."""
+    nw = int(rng.integers(4, 33))
+    radius = max(3.0, 2.1 * nw ** (1.0 / 3.0))
+    xyz, _ = water_cluster(nw, radius, seed0 + i)
+    rc = cluster_radius_bohr(xyz)
+    g = 0 if nw <= 12 else (1 if nw <= 22 else 2)
+    while g < len(MIX_GRIDS) - 1 and MIX_GRIDS[g][1] < rc + 20.0:
+        g += 1
+    n, L = MIX_GRIDS[g]
+    return xyz, n, L, int(rng.integers(1, 5))

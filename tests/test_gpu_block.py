@@ -107,3 +107,38 @@ def test_c3_mixed_within_1e6_of_fp64(pisces):
             # the mixed forms follow the FP64 iteration closely (block 12 exercises
             # Gram tasks split at 16 columns after late restarts)
             assert r["n_iter"] <= 1.3 * r64["n_iter"] + 5, (r["n_iter"], r64["n_iter"])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x2"])
+def test_block_solve_degenerate_levels_default_path(pisces, precision):
+    """Exactly degenerate levels under the default orthogonalisation (second
+    round from S unless a column's squared amplification exceeds 1e6).  The
+    potential f(x) + f(y) + f(z) - 1 (f = 1e-4 x^4) on a cubic grid makes H
+    separable and symmetric under axis permutations: its eigenvalues are the
+    sums e_a + e_b + e_c - 1 of the 1-D eigenvalues of T + diag(f), so the
+    levels after the lowest come in exact triples, and the seven wanted roots
+    end on a complete triple.  The shift puts the levels below 0, as bound
+    states are: the kinetic preconditioner (T - theta)^-1 is then positive
+    definite (with theta inside the kinetic spectrum it is indefinite and the
+    corrections stall; a constant potential would make it exact, and its
+    correction would equal the Ritz vector)."""
+    n, L = (24, 24, 24), (30.0, 30.0, 30.0)
+    with pisces.Context(n, L) as ctx:
+        xs, Ts = ctx.grid()
+        f = 1e-4 * xs[0] ** 4
+        e1 = np.linalg.eigvalsh(Ts[0] + np.diag(f))
+        sums = np.sort((e1[:6, None, None] + e1[None, :6, None] + e1[None, None, :6]).ravel())
+        exact = sums[:7] - 1.0
+        assert exact[7 - 1] < sums[7] - 1.0 - 1e-4  # the seventh root closes a level
+        V = (f[None, None, :] + f[None, :, None] + f[:, None, None]).reshape(-1) - 1.0
+        ctx.set_potential(V)
+        start = np.random.default_rng(11).uniform(-1, 1, size=(7, ctx.ngp))
+        r = ctx.solve_block(7, start, block=8, maxsub=40, max_iter=2000, ptol=8,
+                            precision=pisces.PRECISIONS[precision])
+        print(precision, r["nconv"], r["n_iter"], r["evals"] - exact, flush=True)
+        HU = ctx.apply_h(r["vecs"])
+    assert r["nconv"] == 7, r
+    np.testing.assert_allclose(r["evals"], exact, rtol=0, atol=1e-9)
+    res = np.linalg.norm(HU - r["evals"][:, None] * r["vecs"], axis=1)
+    assert np.all(res < 1e-8 * 1.01), res
+    np.testing.assert_allclose(r["vecs"] @ r["vecs"].T, np.eye(7), atol=1e-10)

@@ -98,3 +98,38 @@ def test_single_point_poltype3(tmp_path, oracle_mod):
     assert ro["nconv"] == 1
     # the GPU water model's SCF and the oracle's stop at a mean change < 1e-12
     assert abs(ebe - float(ro["evals"][0])) <= 1e-8
+
+
+def test_single_point_dual_grid(tmp_path):
+    """Polarization = 6 in the input file: the dual-grid single point (a coarse
+    solve, the potential carried over beyond Rtol, the fine solve started from
+    the coarse ground state) agrees with the direct PolType 3 single point on
+    the same grid when Rtol is large enough that every point is exact (then
+    only the start vector differs), in fewer matvecs."""
+    ref = golden("davidson_c1.npz")
+    xyz = ref["xyz"]
+
+    def run(pol, rtol):
+        lines = ["BeginJob", "  runtype = 1,", "EndJob", "BeginElectronPotential",
+                 "  Polarization = %d," % pol, "  Rtol = %g," % rtol, "EndElectronPotential",
+                 "BeginGridDef", "  NoOfGridPoints = 33, 33, 33,",
+                 "  Length = 36.0, 36.0, 36.0,", "EndGridDef", "BeginDiag", "  nStates = 1,",
+                 "  maxSubspace = 20,", "  maxIter = 200,", "  pTol = 9,", "EndDiag",
+                 "BeginWaters"]
+        for w in xyz.reshape(-1, 9):
+            for a, sym in enumerate(("O", "H", "H")):
+                lines.append("%s %.17g %.17g %.17g" % ((sym,) + tuple(w[3 * a:3 * a + 3])))
+        lines.append("EndWaters")
+        path = tmp_path / ("p%d.inp" % pol)
+        path.write_text("\n".join(lines) + "\n")
+        r = subprocess.run([BIN, str(path)], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        ebe = float(re.search(r"EBE (\S+) Eh", r.stdout).group(1))
+        mv = int(re.search(r"matvecs (\d+), iterations", r.stdout).group(1))
+        return ebe, mv, r.stdout
+
+    e3, mv3, _ = run(3, 20.0)
+    e6, mv6, out6 = run(6, 60.0)   # every fine point within 60 bohr of a site: all exact
+    assert "Energies For pre-calculations" in out6
+    assert abs(e6 - e3) <= 1e-8
+    assert mv6 < mv3

@@ -1,0 +1,84 @@
+"""The electron side of the analytic gradient on the GPU (pisces_gradient_terms,
+pisces_compute_gradient) against the reference's own gradient code
+(grad_*.npz: ClusterAnion::GetAnalGrad with DVR::ComputeGradient's grid loop
+over Potential::EvaluateGradient, FinalGradient and SubtractWWGradient, and its
+WaterCluster for the water side), and against central differences of the GPU
+potential itself."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+FLAGS = (3, 2, 0, 3, 0, 0)
+
+
+def _ctx(pisces, f):
+    n, L = tuple(int(v) for v in f["n"]), tuple(float(v) for v in f["L"])
+    ctx = pisces.Context(n, L)
+    sites, mu, _ = ctx.build_cluster_potential(f["xyz"], flags=FLAGS)
+    return ctx, sites
+
+
+@pytest.mark.parametrize("tag", ["c1", "c2"])
+def test_gradient_matches_reference(pisces, tag):
+    print("gradient", tag, flush=True)
+    f = golden("grad_%s.npz" % tag)
+    ctx, sites = _ctx(pisces, f)
+    with ctx:
+        nsites, npol = sites.shape[0], f["tmu"].size // 3
+        gel, tmu, mcm = ctx.gradient_terms(f["psi"], nsites, npol)
+        # the water model's SCF stops at a mean change of 1e-12 on both sides
+        assert np.abs(gel - f["gel"]).max() <= 1e-10 * np.abs(f["gel"]).max()
+        assert np.abs(tmu - f["tmu"]).max() <= 1e-10 * np.abs(f["tmu"]).max()
+        assert np.abs(mcm.reshape(-1) - f["mcm"]).max() <= 1e-10 * np.abs(f["mcm"]).max()
+        g = ctx.compute_gradient(f["psi"], f["gww"], f["dTx"], f["dTy"], f["dTz"], f["polgrad"],
+                                 f["dEfield"])
+    assert np.abs(g - f["gfinal"]).max() <= 1e-10 * np.abs(f["gfinal"]).max()
+
+
+def test_gradient_is_the_derivative_of_the_gpu_potential(pisces, oracle_mod):
+    """Central differences of sum psi^2 V over the GPU's own PolType 3 grid
+    potential, one site displaced at a time (the permanent-charge field and
+    VpolWaterWater held, InvA rebuilt from the moved sites), against the GPU
+    gradient terms plus the dipole-tensor term of the reference's BuildSuperdT."""
+    f = golden("grad_c1.npz")
+    n, L = tuple(int(v) for v in f["n"]), tuple(float(v) for v in f["L"])
+    cl = oracle_mod.Cluster(f["xyz"])
+    alpha, ip, epc, vww = cl.pol3_lists()
+    pp = oracle_mod.default_potpara()
+    psi = f["psi"]
+    w = psi * psi
+    npol = ip.size
+    mcm_ref = f["mcm"].reshape(3 * npol, 3 * npol)
+    dT = [f[k].reshape(3 * npol, 3 * npol) for k in ("dTx", "dTy", "dTz")]
+    h = 1e-4
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential_scp(FLAGS, pp, cl.sites, cl.q, cl.iq, alpha, ip, epc, vww)
+        gel, _, mcm = ctx.gradient_terms(psi, cl.sites.shape[0], npol)
+        scale = np.abs(gel).max()
+        for s in (0, 1, 2, 3, 5, 10, 15, 23):
+            for a in range(3):
+                F = []
+                for sign in (1, -1):
+                    moved = cl.sites.copy()
+                    moved[s, a] += sign * h
+                    ctx.build_potential_scp(FLAGS, pp, moved, cl.q, cl.iq, alpha, ip, epc, vww)
+                    F.append(w @ ctx.get_potential())
+                fd = (F[0] - F[1]) / (2 * h)
+                an = gel[3 * s + a]
+                if s % 4 != 3:
+                    j = 3 * (s // 4) + s % 4
+                    an -= np.sum(dT[a][3 * j:3 * j + 3, :] * mcm[3 * j:3 * j + 3, :])
+                assert abs(fd - an) <= 1e-6 * scale, (s, a, fd, an)
+    assert np.abs(mcm - mcm_ref).max() <= 1e-10 * np.abs(mcm_ref).max()
+
+
+def test_gradient_needs_a_poltype3_potential(pisces):
+    f = golden("grad_c1.npz")
+    n, L = tuple(int(v) for v in f["n"]), tuple(float(v) for v in f["L"])
+    with pisces.Context(n, L) as ctx:
+        ctx.build_cluster_potential(f["xyz"])  # Polarization 1
+        with pytest.raises(pisces.PiscesError, match="PolType 3"):
+            ctx.gradient_terms(f["psi"], 24, 18)

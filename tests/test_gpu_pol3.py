@@ -1,6 +1,8 @@
 """Self-consistent polarizable sites (PolType 3) on the GPU against the oracle,
 whose PolType 3 restatement is pinned to the reference's Potential::Evaluate by
 tests/test_oracle_pol3.py."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -109,9 +111,19 @@ def test_scp_argument_errors(pisces, oracle_mod):
                                     epc, e_pol)
         bad = np.array(ip)
         bad[0] = 10 ** 6
-        with pytest.raises(pisces.PiscesError):
+        with pytest.raises(ValueError):  # the binding's own check
             ctx.build_potential_scp((3, 2, 0, 3, 0, 0), pp, cl.sites, cl.q, cl.iq, alpha, bad,
                                     epc, e_pol)
+        # the library's check of the same list, called past the binding
+        f, pp_, s, q, iq, al, ip_, ep, ix, iqq, nion = pisces.scp_arrays(
+            (3, 2, 0, 3, 0, 0), pp, cl.sites, cl.q, cl.iq, alpha, ip, epc)
+        ip_[0] = 10 ** 6
+        rc = pisces.lib().pisces_build_potential_scp(
+            ctx.h, pisces._pi(f), pisces._p(pp_), s.size // 3, pisces._p(s), q.size,
+            pisces._p(q), pisces._pi(iq), ip_.size, pisces._p(al), pisces._pi(ip_),
+            pisces._p(ep), ctypes.c_double(e_pol), 0, pisces._p(ix), pisces._p(iqq),
+            ctypes.c_double(2.9))
+        assert rc == -1
 
 
 def test_solve_on_poltype3_potential(pisces, oracle_mod):
@@ -171,3 +183,42 @@ def test_thread_counts(pisces, oracle_mod, nt, monkeypatch):
     pts = np.stack([g.x[0][i], g.x[1][j], g.x[2][k]], axis=1)
     Vo, _ = cl.potential_points_pol3(pts, flags, pp)
     assert np.abs(V[idx] - Vo).max() <= 1e-12 * np.abs(Vo).max()
+
+
+@pytest.mark.parametrize("nw, radius, seed", [(45, 8.0, 3000), (64, 9.0, 21)])
+def test_large_polarizable_sets(pisces, oracle_mod, nw, radius, seed):
+    """(H2O)45 (405 polarizable dimensions, the largest cluster a served
+    configuration names) and (H2O)64 (576, the library's limit) on a 24^3 grid:
+    the default 1024-thread workgroups and dynamic LDS above 64 KB, against the
+    oracle at 1024 sampled points."""
+    import workload
+    xyz, _ = workload.water_cluster(nw, radius, seed)
+    cl = oracle_mod.Cluster(xyz)
+    n, L = (24, 24, 24), (48.0, 48.0, 48.0)
+    g = oracle_mod.Grid(n, L)
+    f = golden("potential_pol3_c1.npz")
+    flags, pp = f["flags_scp"], f["potpara_scp"]
+    alpha, ip, epc, e_pol = cl.pol3_lists()
+    assert 3 * len(ip) == 9 * nw
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential_scp(flags, pp, cl.sites, cl.q, cl.iq, alpha, ip, epc, e_pol)
+        V = ctx.get_potential()
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(g.ngp, size=1024, replace=False)
+    i, j, k = idx % 24, (idx // 24) % 24, idx // 576
+    pts = np.stack([g.x[0][i], g.x[1][j], g.x[2][k]], axis=1)
+    Vo, _ = cl.potential_points_pol3(pts, flags, pp)
+    assert np.abs(V[idx] - Vo).max() <= 1e-12 * np.abs(Vo).max()
+
+
+def test_too_many_polarizable_sites(pisces, oracle_mod):
+    """3 np above 576 is refused before any launch."""
+    import workload
+    xyz, _ = workload.water_cluster(65, 9.5, 22)
+    cl = oracle_mod.Cluster(xyz)
+    f = golden("potential_pol3_c1.npz")
+    alpha, ip, epc, e_pol = cl.pol3_lists()
+    with pisces.Context((16, 16, 16), (48.0, 48.0, 48.0)) as ctx:
+        with pytest.raises(pisces.PiscesError, match="polarizable"):
+            ctx.build_potential_scp(f["flags_scp"], f["potpara_scp"], cl.sites, cl.q, cl.iq,
+                                    alpha, ip, epc, e_pol)

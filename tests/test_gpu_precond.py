@@ -43,3 +43,24 @@ def test_kinetic_precond(pisces, oracle_mod, n, L, precision, tol):
         # (T - shift) t = scale r, shift below the spectrum (no flooring)
         lhs = g.apply_h(np.zeros(g.ngp), t_lo) + 0.05 * t_lo
         np.testing.assert_allclose(lhs, -r, rtol=0, atol=1e-10 * np.abs(r).max())
+
+
+@pytest.mark.parametrize("n,L,nvec", [((64, 64, 64), (40.0, 40.0, 40.0), 9),
+                                      ((40, 36, 33), (40.0, 38.0, 36.0), 4),
+                                      ((20, 18, 16), (30.0, 28.0, 26.0), 1)])
+def test_precond_bf16x2_one_launch_equals_three(pisces, n, L, nvec):
+    """The split-BF16 preconditioner with its three passes in one XCD-ordered
+    launch (pass 1 waiting on pass 0's planes of the vector, pass 2 on pass
+    1's) does the same arithmetic per plane as three launches: equal bit for
+    bit, on every one of several repetitions."""
+    print("precond fused", n, nvec, flush=True)
+    r = np.random.default_rng(sum(n) + nvec).uniform(-1, 1, size=(nvec, int(np.prod(n))))
+    with pisces.Context(n, L) as ctx:
+        ctx.set_hx_variant(2)
+        t3 = ctx.kinetic_precond(r, shift=-0.05, scale=-1.0, floor=1e-3, precision=pisces.BF16X2)
+        ctx.set_hx_variant(3)
+        for _ in range(4):
+            t1 = ctx.kinetic_precond(r, shift=-0.05, scale=-1.0, floor=1e-3,
+                                     precision=pisces.BF16X2)
+            np.testing.assert_array_equal(t1, t3)
+    assert pisces.hx_spin_timeouts() == 0
