@@ -231,7 +231,18 @@ def main():
                     help="only a plain device-to-device copy of BYTES (the calibration of the "
                          "FETCH_SIZE / WRITE_SIZE byte counters: one launch moves BYTES in and "
                          "BYTES out)")
+    ap.add_argument("--pmc-summary", nargs=5, default=None,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV",
+                             "ROOF_JSON"),
+                    help="only print the HBM traffic summary per launch family (the "
+                         "profiles/traffic.json format) from rocprofv3 --pmc FETCH_SIZE / "
+                         "WRITE_SIZE counter CSVs of a --roofline-only run and of a "
+                         "--copy-probe run, with the algorithmic bytes of the --roofline-only "
+                         "JSON line; no GPU")
     a = ap.parse_args()
+    if a.pmc_summary:
+        print(json.dumps(pmc_summary(*a.pmc_summary), indent=1))
+        return
     if a.gpus != 1:
         print("bench.py: one GPU only; this build has no multi-GPU path (--gpus %d)" % a.gpus,
               file=sys.stderr)
@@ -570,6 +581,92 @@ def copy_probe(P, nbytes, reps):
     P.check(L.pisces_copy_probe(0, nbytes, reps, ctypes.byref(ms)), "pisces_copy_probe")
     return {"bytes": nbytes, "launches": reps + 1, "ms_per_launch": ms.value,
             "GBs_read_plus_write": 2 * nbytes / (ms.value * 1e-3) / 1e9}
+
+
+# kernel name fragments -> launch family of the solver's HIP-event brackets
+# (first match wins)
+KERNEL_FAMILIES = (("hx_f64_kernel", "hx_fp64"), ("hx_f_kernel<double", "hx_fp64"),
+                   ("hx_bf2", "hx_lo"), ("hx_f_kernel<float", "hx_lo"), ("gram", "gram"),
+                   ("comb_kernel", "combine"), ("eig_kernel", "eig"), ("ritz", "ritz_resid"),
+                   ("resid", "ritz_resid"), ("fdm_", "precond"), ("precond_kernel", "precond"),
+                   ("cholinv", "factors"), ("orth", "factors"), ("coef_kernel", "factors"),
+                   ("potential_grid", "potential"), ("water_scf", "dipole_scf"))
+
+
+def pmc_summary(fetch_csv, write_csv, copyf_csv, copyw_csv, roof_json):
+    """HBM bytes per launch of each launch family: FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch) summed over the family's kernels, each scaled so that
+    the plain copy kernel of --copy-probe reads and writes its byte count, over
+    the launch count and algorithmic bytes of the --roofline-only line."""
+    import csv
+
+    def per_kernel(path):
+        out = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                k = r["Kernel_Name"]
+                out.setdefault(k, []).append(float(r["Counter_Value"]))
+        return out
+
+    def copy_kib(path):
+        v = [x for k, xs in per_kernel(path).items() if "copy_probe_kernel" in k for x in xs]
+        return sum(v) / len(v)
+
+    with open(roof_json) as f:
+        line = json.loads(f.read().strip().splitlines()[-1])
+    fam = line["launch_families"]
+    # the copy probe moves 1 GiB in and 1 GiB out per launch
+    probe = 1 << 30
+    cf, cw = copy_kib(copyf_csv), copy_kib(copyw_csv)
+    fs, ws = probe / 1024.0 / cf, probe / 1024.0 / cw
+    tot = {}
+    for which, path, scale in (("fetch", fetch_csv, fs), ("write", write_csv, ws)):
+        for k, xs in per_kernel(path).items():
+            name = next((f for frag, f in KERNEL_FAMILIES if frag in k), None)
+            if name is None:
+                continue
+            t = tot.setdefault(name, {"fetch": 0.0, "write": 0.0})
+            t[which] += sum(xs) * 1024.0 * scale
+    out = {"note": "HBM bytes per launch of each solver launch family in a one-lane roofline "
+                   "pass (bench.py --roofline-only --warmup 0), from FETCH_SIZE and WRITE_SIZE "
+                   "(KiB per dispatch, summed over the family's kernels), each scaled by what "
+                   "the counter reads for a plain copy of 1 GiB; ratio = measured / algorithmic "
+                   "bytes of the same pass",
+           "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+           "copy_fetch_kib": cf, "copy_write_kib": cw,
+           "commands": {
+               "fetch": "cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d "
+                        "$R/gpurun_out/pmc_fetch -o f -- python3 $R/bench.py --roofline-only "
+                        "--warmup 0",
+               "write": "cd /tmp && rocprofv3 --pmc WRITE_SIZE --output-format csv -d "
+                        "$R/gpurun_out/pmc_write -o w -- python3 $R/bench.py --roofline-only "
+                        "--warmup 0",
+               "copy": "cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d "
+                       "$R/gpurun_out/pmc_copyf -o c -- python3 $R/bench.py --copy-probe "
+                       "1073741824 --steps 2 (and WRITE_SIZE into pmc_copyw)",
+               "algorithmic": "cd /tmp && rocprofv3 --kernel-trace --stats --output-format csv "
+                              "-d $R/gpurun_out/prof_roof -o bench -- python3 $R/bench.py "
+                              "--roofline-only --warmup 1 > $R/gpurun_out/roof.json",
+               "summary": "python3 bench.py --pmc-summary gpurun_out/pmc_fetch/f_counter_collection.csv "
+                          "gpurun_out/pmc_write/w_counter_collection.csv "
+                          "gpurun_out/pmc_copyf/c_counter_collection.csv "
+                          "gpurun_out/pmc_copyw/c_counter_collection.csv gpurun_out/roof.json"},
+           "families": {}}
+    for name, t in tot.items():
+        f = fam.get(name)
+        if not f or not f["launches"]:
+            continue
+        nl = f["launches"]
+        e = {"launches": nl,
+             "hbm_bytes_per_launch": round((t["fetch"] + t["write"]) / nl),
+             "fetch_bytes_per_launch": round(t["fetch"] / nl),
+             "write_bytes_per_launch": round(t["write"] / nl)}
+        if f.get("GBs_algorithmic"):
+            alg = f["GBs_algorithmic"] * 1e9 * f["ms_per_launch"] * 1e-3
+            e["algorithmic_bytes_per_launch"] = round(alg)
+            e["ratio"] = round((t["fetch"] + t["write"]) / nl / alg, 4)
+        out["families"][name] = e
+    return out
 
 
 def measured_traffic(family, algorithmic_bytes_per_launch):
