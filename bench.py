@@ -219,7 +219,7 @@ def main():
                     help="only the one-lane roofline pass after a warm-up step (profiling runs)")
     ap.add_argument("--pol3-only", action="store_true",
                     help="only the PolType 3 potential builds (profiling runs)")
-    ap.add_argument("--config", default="c2", choices=["c2", "c4", "c5"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: the headline workload; c4: 512 uneven snapshots in one batch; "
                          "c5: open-loop Poisson-served MD chains")
     ap.add_argument("--duration", type=float, default=8.0, help="c5: seconds per offered rate")
@@ -261,6 +261,9 @@ def main():
     prec = P.PRECISIONS[a.precision]
     if a.pol3_only:
         print(json.dumps({"pol3_potential": pol3_measure(P, workload)}), flush=True)
+        return
+    if a.config == "c3":
+        print(json.dumps(bench_c3(P, workload, prec)), flush=True)
         return
     if a.config == "c4":
         print(json.dumps(bench_c4(P, workload, prec, a)), flush=True)
@@ -386,7 +389,66 @@ def main():
 
 
 # ---------------------------------------------------------------- batched / served configs
+SEED_C3 = 3000
 SEED_C4 = 4000
+
+
+def bench_c3(P, workload, prec, nroots=8, block=8, maxsub=40):
+    """(H2O)45 + Na+ on 96^3 (48 bohr), 8 states, in the mixed form: a cold
+    solve from the particle-in-a-box start against the dual-grid start of
+    ClusterAnion (src/ClusterAnion.cpp:207-299) -- the same cluster solved on
+    the coarse grid (every third point, 32^3), its states carried to the fine
+    grid by ExtendWfThree's weights -- with the fine solve's matvecs, steps and
+    time for both, the coarse grid's cost, and the largest eigenvalue
+    difference between the two fine results."""
+    xyz, ion = workload.water_cluster(45, 8.0, SEED_C3, ion_min=2.3)
+    ions = (np.asarray(ion) * workload.ANGS2BOHR, [1.0])
+    n, L = 96, 48.0
+    (nc, _, _), (Lc, _, _) = P.dual_coarse((n,) * 3, (L,) * 3)
+
+    def solve(ctx, start):
+        t0 = time.perf_counter()
+        r = ctx.solve_block(nroots, start, block=block, maxsub=maxsub, max_iter=2000,
+                            ptol=PTOL, precision=prec)
+        return r, time.perf_counter() - t0
+
+    with P.Context((nc,) * 3, (Lc,) * 3) as cc:
+        t0 = time.perf_counter()
+        cc.build_cluster_potential(xyz, ions=ions, ion_r=2.9)
+        t_build = time.perf_counter() - t0
+        solve(cc, P.start_vectors(cc, nroots, seed=3))  # warm-up of the coarse shapes
+        rc_, tc = solve(cc, P.start_vectors(cc, nroots, seed=3))
+        t_coarse = t_build + tc
+    assert rc_["nconv"] == nroots, rc_["nconv"]
+    with P.Context((n,) * 3, (L,) * 3) as ctx:
+        ctx.build_cluster_potential(xyz, ions=ions, ion_r=2.9)
+        solve(ctx, P.start_vectors(ctx, nroots, seed=3))  # warm-up
+        cold, t_cold = solve(ctx, P.start_vectors(ctx, nroots, seed=3))
+        t0 = time.perf_counter()
+        start = np.stack([ctx.dual_extend((nc,) * 3, v, 0) for v in rc_["vecs"]])
+        t_ext = time.perf_counter() - t0
+        warm, t_warm = solve(ctx, start)
+    assert cold["nconv"] == nroots and warm["nconv"] == nroots, (cold["nconv"], warm["nconv"])
+    de = float(np.abs(warm["evals"] - cold["evals"]).max())
+    mv = lambda r: r["n_matvec"] + r["n_matvec_lo"]
+    log("c3: cold %d steps %d matvecs %.3f s; dual-grid start %d steps %d matvecs %.3f s "
+        "(+ coarse %.3f s, transfer %.3f s); max |dE| %.2e"
+        % (cold["n_iter"], mv(cold), t_cold, warm["n_iter"], mv(warm), t_warm, t_coarse, t_ext, de))
+    return {"config": {"workload": "(H2O)45 + Na+ anion, sine-DVR 96^3 on 48 bohr, 8 states, "
+                                   "block %d, subspace %d, residual tol 1e-%d" % (block, maxsub, PTOL),
+                       "coarse_grid": "%d^3 on %.3f bohr" % (nc, Lc)},
+            "dtype": "bf16x2+f64" if prec == P.BF16X2 else "f32+f64",
+            "cold": {"steps": cold["n_iter"], "matvecs_fp64": cold["n_matvec"],
+                     "matvecs_low": cold["n_matvec_lo"], "seconds": round(t_cold, 4)},
+            "dual_grid": {"steps": warm["n_iter"], "matvecs_fp64": warm["n_matvec"],
+                          "matvecs_low": warm["n_matvec_lo"], "seconds": round(t_warm, 4),
+                          "transfer_s": round(t_ext, 4),
+                          "coarse_build_and_solve_s": round(t_coarse, 4),
+                          "coarse_steps": rc_["n_iter"]},
+            "note": "seconds of the fine solve alone (potential built beforehand); the dual-grid "
+                    "total adds the coarse build and solve and the host-buffer transfer",
+            "evals_cold": [round(float(e), 10) for e in cold["evals"]],
+            "max_abs_dE_dual_vs_cold": de, "dE_ok": de <= 1e-6}
 SEED_C5 = 5000
 
 

@@ -266,7 +266,9 @@ int pisces_time_hx(pisces_ctx *ctx, int nvec, int precision, int warmup, int ite
 /* Kinetic preconditioner t = scale * (Tx (+) Ty (+) Tz - shift)^{-1} r for nvec
  * host vectors (nvec*ngp doubles each way), applied through the eigenbases of
  * the 1-D kinetic matrices with |lam - shift| floored at `floor`; precision
- * PISCES_FP64 or PISCES_BF16X2.  The block solver's correction step; exposed
+ * PISCES_FP64 or PISCES_BF16X2 (split-BF16 products, BF16 intermediates between
+ * the three plane passes: about 3e-3 relative, enough for a Davidson
+ * correction, which only steers the search).  The block solver's correction step; exposed
  * for testing.  No reference counterpart (the reference corrects with the
  * diagonal, src/Davidson.cpp:557-566). */
 int pisces_kinetic_precond(pisces_ctx *ctx, int nvec, const double *r, double shift,
@@ -276,10 +278,10 @@ int pisces_kinetic_precond(pisces_ctx *ctx, int nvec, const double *r, double sh
  * the reference; it is for callers that cannot allocate device memory (code
  * under stream capture, serving loops that forbid hipMalloc).  d_r and d_t are
  * distinct device arrays of nvec*ngp doubles.  The per-vector task
- * descriptors and, in the split-BF16 form, the FP32 intermediate between the
+ * descriptors and, in the split-BF16 form, the BF16 intermediate between the
  * three plane passes live in a fixed array of pisces_noalloc_scratch_bytes()
  * bytes (32 MiB) that each GPU holds until the process ends, so the split-BF16
- * form takes at most about 8.3 million grid points x vectors (64^3 x 31
+ * form takes at most about 16.7 million grid points x vectors (64^3 x 63
  * vectors); larger requests return PISCES_ERR_ARG before any launch.  All
  * calls on one GPU share that array: calls that overlap in time on one GPU
  * (other threads, other contexts) race.  The array's address on a device is

@@ -664,7 +664,7 @@ int pisces_time_precond(pisces_ctx *c, int nvec, int precision, int warmup, int 
         const bool f64 = precision == PISCES_FP64;
         const long long n = c->ngp * (long long)nvec;
         const size_t es = f64 ? sizeof(double) : sizeof(float);
-        dbuf R(es * n), Tb(es * n), W(sizeof(float) * n), tasks(sizeof(FdmTask) * nvec);
+        dbuf R(es * n), Tb(es * n), W(sizeof(uint16_t) * n), tasks(sizeof(FdmTask) * nvec);
         fill_uniform(R.p, n, f64, 4321u, c->stream);
         std::vector<FdmTask> h(nvec);
         for (int v = 0; v < nvec; ++v) {
@@ -673,7 +673,7 @@ int pisces_time_precond(pisces_ctx *c, int nvec, int precision, int warmup, int 
             f.src = R.as<unsigned char>() + es * c->ngp * v;
             f.dst = Tb.as<unsigned char>() + es * c->ngp * v;
             f.src32 = f.dst32 = f64 ? 0 : 1;
-            f.work = W.as<float>() + c->ngp * v;
+            f.work = W.as<uint16_t>() + c->ngp * v;
             f.shift = -0.05;
             f.scale = -1.0;
             f.floor = 1e-3;
@@ -732,7 +732,7 @@ int pisces_kinetic_precond(pisces_ctx *c, int nvec, const double *r, double shif
     return guarded([&] {
         PC_HIP(hipSetDevice(c->device));
         const size_t bytes = sizeof(double) * c->ngp * (size_t)nvec;
-        dbuf R(bytes), Tb(bytes), W(sizeof(float) * c->ngp * (size_t)nvec);
+        dbuf R(bytes), Tb(bytes), W(sizeof(uint16_t) * c->ngp * (size_t)nvec);
         dbuf tasks(sizeof(FdmTask) * nvec);
         std::vector<FdmTask> h(nvec);
         for (int v = 0; v < nvec; ++v) {
@@ -742,7 +742,7 @@ int pisces_kinetic_precond(pisces_ctx *c, int nvec, const double *r, double shif
             f.src32 = 0;
             f.dst32 = 0;
             f.dst = Tb.as<double>() + c->ngp * v;
-            f.work = W.as<float>() + c->ngp * v;
+            f.work = W.as<uint16_t>() + c->ngp * v;
             f.shift = shift;
             f.scale = scale;
             f.floor = floor;

@@ -939,8 +939,8 @@ int BdEngine::step_b()
                 f.dst = in->corr(in->mc + t);
                 f.src32 = r32;
                 f.dst32 = t32;
-                // FP32 scratch of the split-BF16 form
-                f.work = reinterpret_cast<float *>(in->T2.p) + in->g.N * t;
+                // BF16 intermediate of the split-BF16 form
+                f.work = reinterpret_cast<uint16_t *>(in->T2.p) + in->g.N * t;
                 // t = -(T - theta)^{-1} r
                 f.shift = in->theta[in->src[t]];
                 f.scale = -1.0;

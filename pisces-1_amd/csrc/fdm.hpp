@@ -14,7 +14,7 @@ struct FdmTask {
     AxisMats ax[3];
     const void *src;  // FP64, or FP32 if src32 (split-BF16 form only)
     void *dst;        // FP64, or FP32 if dst32 (split-BF16 form only)
-    float *work;
+    uint16_t *work;   // BF16 intermediate between the passes (split-BF16 form only)
     double shift, scale, floor;
     int src32, dst32;
 };

@@ -31,7 +31,7 @@ void *g_scratch_ptr[kMaxDevices];
 // One descriptor per vector: the same fields pisces_kinetic_precond fills on
 // the host, written on the device so that no host-to-device copy is needed.
 __global__ void fdm_tasks_kernel(FdmTask *tasks, int nvec, AxisMats ax0, AxisMats ax1,
-                                 AxisMats ax2, const double *r, double *t, float *work,
+                                 AxisMats ax2, const double *r, double *t, uint16_t *work,
                                  long long ngp, double shift, double scale, double floor)
 {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -94,7 +94,7 @@ int pisces_kinetic_precond_noalloc(pisces_ctx *c, int nvec, const double *d_r, d
     }
     const size_t task_bytes = round256(sizeof(FdmTask) * (size_t)nvec);
     const size_t work_bytes =
-        precision == PISCES_BF16X2 ? sizeof(float) * (size_t)c->ngp * (size_t)nvec : 0;
+        precision == PISCES_BF16X2 ? sizeof(uint16_t) * (size_t)c->ngp * (size_t)nvec : 0;
     if (task_bytes + work_bytes > kScratchBytes) {
         set_last_error("pisces_kinetic_precond_noalloc: " + std::to_string(nvec) +
                        " vectors of " + std::to_string(c->ngp) + " points need " +
@@ -106,7 +106,7 @@ int pisces_kinetic_precond_noalloc(pisces_ctx *c, int nvec, const double *d_r, d
         PC_HIP(hipSetDevice(c->device));
         unsigned char *base = static_cast<unsigned char *>(c->scratch);
         FdmTask *tasks = reinterpret_cast<FdmTask *>(base);
-        float *work = work_bytes ? reinterpret_cast<float *>(base + task_bytes) : nullptr;
+        uint16_t *work = work_bytes ? reinterpret_cast<uint16_t *>(base + task_bytes) : nullptr;
         hipLaunchKernelGGL(fdm_tasks_kernel, dim3((nvec + 63) / 64), dim3(64), 0, c->stream,
                            tasks, nvec, c->ax[0].m, c->ax[1].m, c->ax[2].m, d_r, d_t, work,
                            c->ngp, shift, scale, floor);

@@ -229,9 +229,15 @@ template <int NPMAX> struct Own {
     static constexpr int KS = NPMAX / 32;             // k-steps of the bf16 MFMA
 };
 
-// Owned plane values as FP32 (zero outside the n_c x n_i plane).  Every lane
-// loads unconditionally -- out-of-plane elements from the plane's (0, 0) --
-// so the loads issue back to back, and the value is selected afterwards.
+// Owned plane values as FP32 (zero outside the n_c x n_i plane) from FP64,
+// FP32 or BF16 (uint16_t bit patterns) storage.  Every lane loads
+// unconditionally -- out-of-plane elements from the plane's (0, 0) -- so the
+// loads issue back to back, and the value is selected afterwards.
+template <typename S> __device__ __forceinline__ float as_f32(S v) { return (float)v; }
+template <> __device__ __forceinline__ float as_f32<uint16_t>(uint16_t v)
+{
+    return __uint_as_float((unsigned int)v << 16);
+}
 template <typename S, int NCT, int NIT>
 __device__ __forceinline__ void own_load(const S *src_, long long base, const Geo &g,
                                          float (&x)[NCT][NIT][4])
@@ -248,7 +254,7 @@ __device__ __forceinline__ void own_load(const S *src_, long long base, const Ge
             for (int v = 0; v < NIT; ++v) {
                 const int i = 16 * v + l16;
                 const bool ok = c < g.nc && i < g.ni;
-                const float y = (float)src[ok ? (long long)c * g.cs + i : 0];
+                const float y = as_f32(src[ok ? (long long)c * g.cs + i : 0]);
                 x[u][v][r] = ok ? y : 0.f;
             }
         }
@@ -1020,6 +1026,11 @@ __device__ __forceinline__ void put_acc(const f32x4 (&acc)[NCT][NIT], const Geo 
     own_stage(x, g, Pth, Ptl, Pch, Pcl);
 }
 
+template <typename D> __device__ __forceinline__ D of_f32(float v) { return (D)v; }
+template <> __device__ __forceinline__ uint16_t of_f32<uint16_t>(float v) { return bf16_rne(v); }
+
+// Stores the owned tiles to FP64, FP32 or BF16 (uint16_t bit patterns, rounded
+// to nearest even) storage.
 template <typename D, int NCT, int NIT>
 __device__ __forceinline__ void bf2_store_global(D *dst_, const Geo &g, const f32x4 (&acc)[NCT][NIT])
 {
@@ -1037,7 +1048,7 @@ __device__ __forceinline__ void bf2_store_global(D *dst_, const Geo &g, const f3
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int c = 16 * tc + 4 * lq + r, i = 16 * v + l16;
-                if (c < g.nc && i < g.ni) dst[g.base + (long long)c * g.cs + i] = (D)acc[u][v][r];
+                if (c < g.nc && i < g.ni) dst[g.base + (long long)c * g.cs + i] = of_f32<D>(acc[u][v][r]);
             }
         }
     }
@@ -1175,21 +1186,21 @@ __device__ __forceinline__ void fdm_bf2_planes(const FdmTask &t, int p, int pstr
                 lz[u][r] = c < g.nc ? (float)y : 0.f;
             }
     }
-    const void *src = PASS == 0 ? t.src : t.work;
-    const bool s32 = PASS != 0 || t.src32;
+    // pass 0 reads r (FP64 or FP32); passes 1 and 2 the BF16 intermediate
     float xb[NCT][NIT][4];
-    if (s32) own_load(static_cast<const float *>(src), g.base, g, xb);
-    else own_load(static_cast<const double *>(src), g.base, g, xb);
+    auto load = [&](const Geo &gg) {
+        if constexpr (PASS != 0) own_load(static_cast<const uint16_t *>(t.work), gg.base, gg, xb);
+        else if (t.src32) own_load(static_cast<const float *>(t.src), gg.base, gg, xb);
+        else own_load(static_cast<const double *>(t.src), gg.base, gg, xb);
+    };
+    load(g);
     f32x4 acc[NCT][NIT];
     for (;;) {
         own_stage(xb, g, PASS == 0 ? nullptr : Pth, Ptl, PASS == 0 ? P2h : nullptr, P2l);
         __syncthreads();
         Geo gn;
         const bool more = MULTI && geo_of(kind, Mx.n, My.n, Mz.n, 1, 0, Mx.np, Mc.np, p + pstride, gn);
-        if (more) {
-            if (s32) own_load(static_cast<const float *>(src), gn.base, gn, xb);
-            else own_load(static_cast<const double *>(src), gn.base, gn, xb);
-        }
+        if (more) load(gn);
         if (PASS == 0) {
             if constexpr (HOLD) mm_i_held<NCT, NIT, KS>(P2h, P2l, qbh, qbl, g, acc);  // X Qx
             else mm_i_all<NCT, NIT, KS>(P2h, P2l, Mx.qth, Mx.qtl, g, acc);

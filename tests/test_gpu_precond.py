@@ -23,7 +23,10 @@ def _reference(g, r, shift, scale, floor):
 
 
 @pytest.mark.parametrize("n,L", SHAPES)
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("bf16x2", 2e-3)])
+# bf16x2: the two intermediates between the plane passes are stored in BF16
+# (rounding 2^-9); carried through the transforms in numpy that alone gives
+# 1.8e-3 to 3.0e-3 relative on these shapes
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("bf16x2", 5e-3)])
 def test_kinetic_precond(pisces, oracle_mod, n, L, precision, tol):
     print("precond", n, precision, flush=True)
     g = oracle_mod.Grid(n, L)
@@ -36,6 +39,7 @@ def test_kinetic_precond(pisces, oracle_mod, n, L, precision, tol):
                                    precision=pisces.PRECISIONS[precision])
     for v in range(2):
         ref = _reference(g, r[v], -0.05, -1.0, 1e-3)
+        print("relative error", np.linalg.norm(t_lo[v] - ref) / np.linalg.norm(ref), flush=True)
         assert np.linalg.norm(t_lo[v] - ref) <= tol * np.linalg.norm(ref)
         ref = _reference(g, r[v], 0.3, 2.0, 1e-3)
         assert np.linalg.norm(t_in[v] - ref) <= tol * np.linalg.norm(ref)

@@ -43,7 +43,7 @@ def test_noalloc_rejects_oversized_request(pisces):
     n, L = (64, 64, 64), (40.0, 40.0, 40.0)
     cap = pisces.noalloc_scratch_bytes()
     with pisces.Context(n, L) as ctx:
-        nvec = cap // (4 * ctx.ngp)  # the FP32 intermediate alone fills the array
+        nvec = cap // (2 * ctx.ngp)  # the BF16 intermediate alone fills the array
         print("noalloc oversized", nvec, flush=True)
         d_r = pisces.DeviceArray((nvec, ctx.ngp))
         d_t = pisces.DeviceArray((nvec, ctx.ngp))
