@@ -217,6 +217,9 @@ def main():
                     help="time only the served mixed-precision steps (profiling runs)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the one-lane roofline pass after a warm-up step (profiling runs)")
+    ap.add_argument("--gradient-only", action="store_true",
+                    help="only the electron side of the analytic gradient: (H2O)24 on 64^3 with a "
+                         "PolType 3 potential, time per pisces_gradient_terms call")
     ap.add_argument("--pol3-only", action="store_true",
                     help="only the PolType 3 potential builds (profiling runs)")
     ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
@@ -261,6 +264,9 @@ def main():
     prec = P.PRECISIONS[a.precision]
     if a.pol3_only:
         print(json.dumps({"pol3_potential": pol3_measure(P, workload)}), flush=True)
+        return
+    if a.gradient_only:
+        print(json.dumps({"gradient": gradient_measure(P, workload, max(1, a.steps))}), flush=True)
         return
     if a.config == "c3":
         print(json.dumps(bench_c3(P, workload, prec)), flush=True)
@@ -744,6 +750,38 @@ def measured_traffic(family, algorithmic_bytes_per_launch):
     except (OSError, KeyError, ValueError):
         return None, None
     return round(ratio * algorithmic_bytes_per_launch), round(ratio, 3)
+
+
+def gradient_measure(P, workload, reps):
+    """Electron side of the analytic gradient (DVR::ComputeGradient's grid loop)
+    for the ground state of (H2O)24 on 64^3 with a PolType 3 potential (216
+    polarizable dimensions, padded to 224): wall time per pisces_gradient_terms
+    call (the wavefunction crosses PCIe once per call) after a warm-up, and the
+    matrix work per grid point -- mu = (E + Epc) InvA, 2 n3p^2 flops, and the
+    upper 16 x 16 blocks of sum psi^2 (E + Epc)(E + Epc)^T, ncb (ncb + 1) / 2 x
+    512 flops -- against the FP64 peak."""
+    flags = (3, 2, 0, 3, 0, 0)
+    xyz, _ = workload.water_cluster(N_WATER, RADIUS, SEED)
+    n, L = N_GRID, 40.0
+    with P.Context((n,) * 3, (L,) * 3) as ctx:
+        sites, _, _ = ctx.build_cluster_potential(xyz, flags=flags)
+        r = ctx.solve(1, P.start_vectors(ctx, 1), maxsub=20, maxmacro=200, ptol=8, corrflag=1)
+        psi = r["vecs"][0]
+        ns, npol = len(sites), 3 * N_WATER
+        ctx.gradient_terms(psi, ns, npol)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.gradient_terms(psi, ns, npol)
+        ms = (time.perf_counter() - t0) / reps * 1e3
+    n3p = (3 * npol + 15) // 16 * 16
+    ncb = n3p // 16
+    fpp = 2.0 * n3p * n3p + ncb * (ncb + 1) / 2 * 512
+    fl = fpp * n ** 3
+    return {"workload": "(H2O)24 anion ground state, 64^3 on 40 bohr, PolType 3 (216 dims)",
+            "ms_per_call_wall": round(ms, 4), "flops_per_point": fpp, "flops_per_call": fl,
+            "TFLOPs_wall": round(fl / (ms * 1e-3) / 1e12, 2), "peak_TFLOPs": FP64_PEAK_TFS,
+            "note": "wall time includes the 2 MB wavefunction upload and the host's final sums; "
+                    "the kernels' own time is in the rocprofv3 kernel trace"}
 
 
 def pol3_measure(P, workload):
