@@ -16,10 +16,11 @@
 //     E rows in LDS, mu = E InvA on the FP64 matrix cores, InvA streamed from
 //     L2 -- with the site terms and the dipole terms summed per workgroup over
 //     its tiles and written as one row of partial sums;
-//   grad_syrk_kernel: S = sum w (E + Epc)(E + Epc)^T over 16 x 16 blocks of the
-//     upper triangle, E generated per lane on the fly, split over point ranges;
+//   grad_s_kernel: S = sum w (E + Epc)(E + Epc)^T over 16 x 16 blocks of the
+//     upper triangle on the matrix cores, E rows built once per tile in LDS,
+//     split over tile ranges and block groups;
 //   colsum_kernel: the partial rows summed in a fixed order;
-//   sandwich_kernel: mcm = InvA S InvA (two small products).
+//   matmul_kernel: mcm = InvA S InvA (two small products).
 #include "common.hpp"
 #include "context.hpp"
 #include "gmem.hpp"
@@ -233,62 +234,117 @@ grad_points_kernel(GradDev D, int nx, int ny, int nz, const double *__restrict__
     }
 }
 
-// S block (bi, bj), bi <= bj, numbered row by row over the upper triangle;
-// blockIdx.y: the range of points.  Partial: part[y][blk][16 x 16] row-major.
-__global__ void __launch_bounds__(256)
-grad_syrk_kernel(GradDev D, int nx, int ny, int nz, const double *__restrict__ gx,
-                 const double *__restrict__ gy, const double *__restrict__ gz,
-                 const double *__restrict__ psi, long long chunk, double *__restrict__ part)
+// S = sum_p psi_p^2 (E_p + Epc)(E_p + Epc)^T over its upper 16 x 16 blocks
+// (bi <= bj, numbered row by row over the upper triangle) on the FP64 matrix
+// cores.  blockIdx.x: a range of 16-point tiles; blockIdx.y: a group of
+// NW * kGradBPW blocks, block q of the group taken by wave q % NW.  Per tile the
+// workgroup builds the E rows once in LDS (as grad_points_kernel does), and each
+// wave adds, for each of its blocks, A = E[points][16 bi ..] (16 x 4 per
+// k-step) times B = psi^2 E[points][16 bj ..] into a register accumulator:
+// four v_mfma_f64_16x16x4f64 per block and tile.  Partial: part[x][blk][16 x
+// 16] row-major.
+constexpr int kGradBPW = 8;  // blocks per wave (4 accumulator registers each)
+
+__global__ void __launch_bounds__(kGradNT)
+grad_s_kernel(GradDev D, int nx, int ny, int nz, const double *__restrict__ gx,
+              const double *__restrict__ gy, const double *__restrict__ gz,
+              const double *__restrict__ psi, long long tiles_per_wg, double *__restrict__ part)
 {
-    const int ncb = D.n3p / 16, n3 = 3 * D.np;
-    int blk = blockIdx.x, bi = 0;
-    while (blk >= ncb - bi) {
-        blk -= ncb - bi;
-        ++bi;
-    }
-    const int bj = bi + blk;
+    constexpr int NW = kGradNT / 64;
+    extern __shared__ double Es[];  // 16 rows of n3p (+1)
+    __shared__ double pt[16][4];    // point x, y, z and its weight psi^2
+    const int n3p = D.n3p, ldE = n3p + 1, np = D.np, n3 = 3 * np;
+    const int ncb = n3p / 16, nblk = ncb * (ncb + 1) / 2;
     const long long ng = (long long)nx * ny * nz;
-    const long long p0 = (long long)blockIdx.y * chunk, p1 = p0 + chunk < ng ? p0 + chunk : ng;
+    const long long ntiles = (ng + 15) / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const gptr<const double> rp = glb(D.rpol), ep = glb(D.epc);
-    const int ca = 16 * bi + l16, cbb = 16 * bj + l16;
-    const bool oka = ca < n3, okb = cbb < n3;
-    const int ia = oka ? ca / 3 : 0, ib = okb ? cbb / 3 : 0;
-    const int ka = ca - 3 * ia, kb = cbb - 3 * ib;
-    const double ax = rp[3 * ia], ay = rp[3 * ia + 1], az = rp[3 * ia + 2], ea0 = oka ? ep[ca] : 0.0;
-    const double bx = rp[3 * ib], by = rp[3 * ib + 1], bz = rp[3 * ib + 2], eb0 = okb ? ep[cbb] : 0.0;
-    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (long long q0 = p0 + 4 * wave; q0 < p1; q0 += 16) {
-        const long long p = q0 + lq;
-        double fa = 0.0, fb = 0.0;
-        if (p < p1) {
-            double r[3];
-            point_of(p, nx, ny, gx, gy, gz, r);
-            const double s = psi[p];
-            double g, dg;
-            const double rax = r[0] - ax, ray = r[1] - ay, raz = r[2] - az;
-            const double Ra2 = rax * rax + ray * ray + raz * raz;
-            g = 0.0;
-            if (Ra2 > 0.0) field_g(sqrt(Ra2), D.pdamp, g, dg);
-            fa = oka ? s * (g * (ka == 0 ? rax : ka == 1 ? ray : raz) + ea0) : 0.0;
-            const double rbx = r[0] - bx, rby = r[1] - by, rbz = r[2] - bz;
-            const double Rb2 = rbx * rbx + rby * rby + rbz * rbz;
-            g = 0.0;
-            if (Rb2 > 0.0) field_g(sqrt(Rb2), D.pdamp, g, dg);
-            fb = okb ? s * (g * (kb == 0 ? rbx : kb == 1 ? rby : rbz) + eb0) : 0.0;
-        }
-        // A[c1][k] = F(point k, column 16 bi + c1), B[k][c2] = F(point k, 16 bj + c2)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa, fb, acc, 0, 0, 0);
-    }
-    __shared__ double red[4][256];
+    // this wave's blocks: their A and B column offsets in an E row
+    int oa[kGradBPW], ob[kGradBPW];
+    int nown = 0;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) red[wave][(lq + 4 * r) * 16 + l16] = acc[r];
-    __syncthreads();
-    const int nblk = ncb * (ncb + 1) / 2;
-    double *out = part + ((size_t)blockIdx.y * nblk + blockIdx.x) * 256;
-    out[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
-                       red[3][threadIdx.x];
+    for (int u = 0; u < kGradBPW; ++u) {
+        int blk = blockIdx.y * NW * kGradBPW + wave + NW * u;
+        oa[u] = ob[u] = 0;
+        if (blk < nblk) {
+            int bi = 0;
+            while (blk >= ncb - bi) {
+                blk -= ncb - bi;
+                ++bi;
+            }
+            oa[u] = 16 * bi;
+            ob[u] = 16 * (bi + blk);
+            nown = u + 1;
+        }
+    }
+    f64x4 acc[kGradBPW];
+#pragma unroll
+    for (int u = 0; u < kGradBPW; ++u) acc[u] = f64x4{0.0, 0.0, 0.0, 0.0};
+    const long long t0 = (long long)blockIdx.x * tiles_per_wg;
+    const long long t1 = t0 + tiles_per_wg < ntiles ? t0 + tiles_per_wg : ntiles;
+    for (long long t = t0; t < t1; ++t) {
+        if (threadIdx.x < 16) {
+            const long long g = 16 * t + threadIdx.x;
+            double r[3] = {0.0, 0.0, 0.0};
+            double w = 0.0;
+            if (g < ng) {
+                point_of(g, nx, ny, gx, gy, gz, r);
+                w = psi[g] * psi[g];
+            }
+            pt[threadIdx.x][0] = r[0];
+            pt[threadIdx.x][1] = r[1];
+            pt[threadIdx.x][2] = r[2];
+            pt[threadIdx.x][3] = w;
+        }
+        __syncthreads();
+        // E rows (electron field + permanent-charge field), zero past 3 np;
+        // rows past the grid carry weight 0
+        for (int idx = threadIdx.x; idx < 16 * np; idx += kGradNT) {
+            const int p = idx / np, i = idx - p * np;
+            const double rx = pt[p][0] - rp[3 * i], ry = pt[p][1] - rp[3 * i + 1],
+                         rz = pt[p][2] - rp[3 * i + 2];
+            const double R2 = rx * rx + ry * ry + rz * rz;
+            double g = 0.0, dg;
+            if (R2 > 0.0) field_g(sqrt(R2), D.pdamp, g, dg);
+            double *row = Es + p * ldE + 3 * i;
+            row[0] = g * rx + ep[3 * i];
+            row[1] = g * ry + ep[3 * i + 1];
+            row[2] = g * rz + ep[3 * i + 2];
+        }
+        for (int idx = threadIdx.x; idx < 16 * (n3p - n3); idx += kGradNT) {
+            const int p = idx / (n3p - n3);
+            Es[p * ldE + n3 + (idx - p * (n3p - n3))] = 0.0;
+        }
+        __syncthreads();
+        // k-step s: points 4 s + lq; A[c1][k] = E[4 s + k][16 bi + c1],
+        // B[k][c2] = w_{4 s + k} E[4 s + k][16 bj + c2]
+        double w4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w4[q] = pt[4 * q + lq][3];
+        const double *er = Es + lq * ldE + l16;
+#pragma unroll
+        for (int u = 0; u < kGradBPW; ++u) {
+            if (u < nown) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double *e = er + 4 * q * ldE;
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(e[oa[u]], w4[q] * e[ob[u]],
+                                                                 acc[u], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();  // Es and pt are rewritten by the next tile
+    }
+#pragma unroll
+    for (int u = 0; u < kGradBPW; ++u) {
+        const int blk = blockIdx.y * NW * kGradBPW + wave + NW * u;
+        if (u < nown) {
+            double *out = part + ((size_t)blockIdx.x * nblk + blk) * 256;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) out[(lq + 4 * r) * 16 + l16] = acc[u][r];
+        }
+    }
 }
 
 // out[c] = sum over rows of part[row][c], rows in order
@@ -345,6 +401,8 @@ void gradient_terms(pisces_ctx *c, const double *psi_host, double *gel, double *
     std::call_once(once, [] {
         PC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(grad_points_kernel),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
+        PC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(grad_s_kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax));
     });
     const size_t lds = (size_t)16 * (n3p + 1) * sizeof(double);
     PC_REQUIRE(lds <= (size_t)kLdsMax && n3p <= 8 * kGradCBW * 16, PISCES_ERR_UNSUPPORTED,
@@ -356,8 +414,11 @@ void gradient_terms(pisces_ctx *c, const double *psi_host, double *gel, double *
     const int wg = (int)std::min<long long>(ntiles, 1024);
     const long long cols = 3LL * ns + 4LL * n3p;
     const int nblk = ncb * (ncb + 1) / 2;
-    const int nk = std::max(1, std::min<int>((int)((ng + 4095) / 4096), (2048 + nblk - 1) / nblk));
-    const long long chunk = (ng + nk - 1) / nk;
+    // S: groups of 8 x kGradBPW blocks; about 256 workgroups (one per CU at
+    // its register use) over the tiles, so that the partials stay small
+    const int ngrp = (nblk + 8 * kGradBPW - 1) / (8 * kGradBPW);
+    const long long tpw = std::max<long long>(1, (ntiles * ngrp + 255) / 256);
+    const int nk = (int)((ntiles + tpw - 1) / tpw);
     dbuf dpsi(sizeof(double) * ng), dpart(sizeof(double) * (size_t)wg * cols),
         dsp(sizeof(double) * (size_t)nk * nblk * 256), dtot(sizeof(double) * cols),
         dblk(sizeof(double) * (size_t)nblk * 256), dS(sizeof(double) * (size_t)n3p * n3p),
@@ -366,8 +427,8 @@ void gradient_terms(pisces_ctx *c, const double *psi_host, double *gel, double *
     const double *gxp = c->dx[0].as<double>(), *gyp = c->dx[1].as<double>(), *gzp = c->dx[2].as<double>();
     hipLaunchKernelGGL(grad_points_kernel, dim3(wg), dim3(kGradNT), lds, c->stream, D, c->n[0],
                        c->n[1], c->n[2], gxp, gyp, gzp, dpsi.as<double>(), dpart.as<double>());
-    hipLaunchKernelGGL(grad_syrk_kernel, dim3(nblk, nk), dim3(256), 0, c->stream, D, c->n[0],
-                       c->n[1], c->n[2], gxp, gyp, gzp, dpsi.as<double>(), chunk, dsp.as<double>());
+    hipLaunchKernelGGL(grad_s_kernel, dim3(nk, ngrp), dim3(kGradNT), lds, c->stream, D, c->n[0],
+                       c->n[1], c->n[2], gxp, gyp, gzp, dpsi.as<double>(), tpw, dsp.as<double>());
     hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, c->stream,
                        dpart.as<double>(), (long long)wg, cols, dtot.as<double>());
     const long long sc = (long long)nblk * 256;

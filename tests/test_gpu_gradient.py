@@ -82,3 +82,29 @@ def test_gradient_needs_a_poltype3_potential(pisces):
         ctx.build_cluster_potential(f["xyz"])  # Polarization 1
         with pytest.raises(pisces.PiscesError, match="PolType 3"):
             ctx.gradient_terms(f["psi"], 24, 18)
+
+
+@pytest.mark.parametrize("nw, radius, seed", [(45, 8.0, 3000), (64, 9.0, 21)])
+def test_gradient_terms_large_polarizable_sets(pisces, oracle_mod, nw, radius, seed):
+    """(H2O)45 (405 polarizable dimensions, 26 column blocks: 351 blocks of
+    S in several groups of the matrix-core kernel) and (H2O)64 (576, the
+    library's limit) on a 16^3 grid with a smooth normalised psi, against the
+    oracle's grid loop."""
+    import workload
+    print("gradient large", nw, flush=True)
+    xyz, _ = workload.water_cluster(nw, radius, seed)
+    cl = oracle_mod.Cluster(xyz)
+    n, L = (16, 16, 16), (48.0, 48.0, 48.0)
+    g = oracle_mod.Grid(n, L)
+    X, Y, Z = np.meshgrid(g.x[0], g.x[1], g.x[2], indexing="ij")
+    psi = np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 60.0).transpose(2, 1, 0).reshape(-1)
+    psi /= np.linalg.norm(psi)
+    gel_o, tmu_o, mcm_o = oracle_mod.gradient_grid(cl, g, psi)
+    alpha, ip, epc, vww = cl.pol3_lists()
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential_scp(FLAGS, oracle_mod.default_potpara(), cl.sites, cl.q, cl.iq, alpha,
+                                ip, epc, vww)
+        gel, tmu, mcm = ctx.gradient_terms(psi, cl.sites.shape[0], ip.size)
+    assert np.abs(gel - gel_o).max() <= 1e-10 * np.abs(gel_o).max()
+    assert np.abs(tmu - tmu_o).max() <= 1e-10 * np.abs(tmu_o).max()
+    assert np.abs(mcm - mcm_o).max() <= 1e-10 * np.abs(mcm_o).max()
