@@ -941,8 +941,13 @@ int BdEngine::step_b()
                 f.dst32 = t32;
                 // BF16 intermediate of the split-BF16 form
                 f.work = reinterpret_cast<uint16_t *>(in->T2.p) + in->g.N * t;
-                // t = -(T - theta)^{-1} r
+                // t = -|T - theta|^{-1} r: (T - theta)^{-1} for bound states
+                // (theta below the kinetic spectrum); where theta lies inside
+                // it, T - theta is indefinite and its corrections stall, and
+                // for a constant potential it is exact, so that its correction
+                // would be the Ritz vector itself
                 f.shift = in->theta[in->src[t]];
+                f.absden = 1;
                 f.scale = -1.0;
                 f.floor = 1e-3;
                 hf[nfd++] = f;

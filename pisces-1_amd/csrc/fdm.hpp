@@ -8,15 +8,18 @@ namespace pisces {
 // dst = scale * (Tx (+) Ty (+) Tz - shift)^{-1} src for one grid vector, with
 // |lam_x + lam_y + lam_z - shift| floored at `floor`, applied through the
 // eigenbases of the 1-D kinetic matrices (ax[a].q64 / qt64 / split BF16 and
-// ax[a].lam).  work: ngp floats of scratch (split-BF16 form only).  The FP64
-// form works in place in dst.
+// ax[a].lam).  absden: divide by |lam_x + lam_y + lam_z - shift| instead (the
+// absolute value of T - shift: positive definite wherever shift lies, the
+// same operator when shift is below the kinetic spectrum).  work: ngp BF16
+// values of scratch (split-BF16 form only).  The FP64 form works in place in
+// dst.
 struct FdmTask {
     AxisMats ax[3];
     const void *src;  // FP64, or FP32 if src32 (split-BF16 form only)
     void *dst;        // FP64, or FP32 if dst32 (split-BF16 form only)
     uint16_t *work;   // BF16 intermediate between the passes (split-BF16 form only)
     double shift, scale, floor;
-    int src32, dst32;
+    int src32, dst32, absden;
 };
 
 // precision: PISCES_FP64 (FP64 matrix cores) or PISCES_BF16X2 (split BF16)

@@ -48,6 +48,7 @@ __global__ void fdm_tasks_kernel(FdmTask *tasks, int nvec, AxisMats ax0, AxisMat
     f.floor = floor;
     f.src32 = 0;
     f.dst32 = 0;
+    f.absden = 0;
     tasks[v] = f;
 }
 

@@ -916,6 +916,7 @@ __device__ __forceinline__ void f64_store_lds(double *P, int pitch, const Geo &g
                     double den = 1.0;
                     if (i < g.ni && c < g.nc) {
                         den = t->ax[0].lam[i] + t->ax[1].lam[g.pl] + t->ax[2].lam[c] - t->shift;
+                        if (t->absden) den = fabs(den);
                         if (fabs(den) < t->floor) den = den < 0 ? -t->floor : t->floor;
                     }
                     x *= t->scale / den;
@@ -987,7 +988,8 @@ __global__ void __launch_bounds__(kPT) fdm_f64_kernel(const FdmTask *__restrict_
 template <int NCT, int NIT>
 __device__ __forceinline__ void put_scaled(const f32x4 (&acc)[NCT][NIT], const Geo &g, float ly,
                                            const float (&lx)[NIT], const float (&lz)[NCT][4],
-                                           float scale, float floor, uint16_t *Pth, uint16_t *Ptl)
+                                           float scale, float floor, bool absden, uint16_t *Pth,
+                                           uint16_t *Ptl)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
@@ -1004,6 +1006,7 @@ __device__ __forceinline__ void put_scaled(const f32x4 (&acc)[NCT][NIT], const G
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 float den = lx[v] + ly + lz[u][r];
+                if (absden) den = fabsf(den);
                 if (fabsf(den) < floor) den = den < 0.f ? -floor : floor;
                 x[r] = acc[u][v][r] * (scale / den);
             }
@@ -1212,7 +1215,7 @@ __device__ __forceinline__ void fdm_bf2_planes(const FdmTask &t, int p, int pstr
             mm_c_all(ah, al, Pth, Ptl, g, acc);                            // Qz^T X
             // rows of the scaled plane: i = a, c = z index, at j = g.pl
             put_scaled(acc, g, (float)(glb(My.lam)[g.pl] - t.shift), lx, lz, (float)t.scale,
-                       (float)t.floor, P2h, P2l);
+                       (float)t.floor, t.absden != 0, P2h, P2l);
             __syncthreads();
             mm_c_all(bh, bl, P2h, P2l, g, acc);                            // Qz (.)
             bf2_store_global(t.work, g, acc);

@@ -109,36 +109,56 @@ def test_c3_mixed_within_1e6_of_fp64(pisces):
             assert r["n_iter"] <= 1.3 * r64["n_iter"] + 5, (r["n_iter"], r64["n_iter"])
 
 
+@pytest.mark.parametrize("shift", [-1.0, 0.0])
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x2"])
-def test_block_solve_degenerate_levels_default_path(pisces, precision):
+def test_block_solve_degenerate_levels_default_path(pisces, precision, shift):
     """Exactly degenerate levels under the default orthogonalisation (second
     round from S unless a column's squared amplification exceeds 1e6).  The
-    potential f(x) + f(y) + f(z) - 1 (f = 1e-4 x^4) on a cubic grid makes H
+    potential f(x) + f(y) + f(z) + shift (f = 1e-4 x^4) on a cubic grid makes H
     separable and symmetric under axis permutations: its eigenvalues are the
-    sums e_a + e_b + e_c - 1 of the 1-D eigenvalues of T + diag(f), so the
+    sums e_a + e_b + e_c + shift of the 1-D eigenvalues of T + diag(f), so the
     levels after the lowest come in exact triples, and the seven wanted roots
-    end on a complete triple.  The shift puts the levels below 0, as bound
-    states are: the kinetic preconditioner (T - theta)^-1 is then positive
-    definite (with theta inside the kinetic spectrum it is indefinite and the
-    corrections stall; a constant potential would make it exact, and its
-    correction would equal the Ritz vector)."""
+    end on a complete triple.  shift -1 puts the levels below 0, as bound
+    states are; with shift 0 they lie inside the kinetic spectrum, where
+    T - theta is indefinite and the correction uses |T - theta|."""
     n, L = (24, 24, 24), (30.0, 30.0, 30.0)
     with pisces.Context(n, L) as ctx:
         xs, Ts = ctx.grid()
         f = 1e-4 * xs[0] ** 4
         e1 = np.linalg.eigvalsh(Ts[0] + np.diag(f))
         sums = np.sort((e1[:6, None, None] + e1[None, :6, None] + e1[None, None, :6]).ravel())
-        exact = sums[:7] - 1.0
-        assert exact[7 - 1] < sums[7] - 1.0 - 1e-4  # the seventh root closes a level
-        V = (f[None, None, :] + f[None, :, None] + f[:, None, None]).reshape(-1) - 1.0
+        exact = sums[:7] + shift
+        assert exact[7 - 1] < sums[7] + shift - 1e-4  # the seventh root closes a level
+        V = (f[None, None, :] + f[None, :, None] + f[:, None, None]).reshape(-1) + shift
         ctx.set_potential(V)
         start = np.random.default_rng(11).uniform(-1, 1, size=(7, ctx.ngp))
         r = ctx.solve_block(7, start, block=8, maxsub=40, max_iter=2000, ptol=8,
                             precision=pisces.PRECISIONS[precision])
-        print(precision, r["nconv"], r["n_iter"], r["evals"] - exact, flush=True)
+        print(precision, shift, r["nconv"], r["n_iter"], r["evals"] - exact, flush=True)
         HU = ctx.apply_h(r["vecs"])
     assert r["nconv"] == 7, r
     np.testing.assert_allclose(r["evals"], exact, rtol=0, atol=1e-9)
     res = np.linalg.norm(HU - r["evals"][:, None] * r["vecs"], axis=1)
     assert np.all(res < 1e-8 * 1.01), res
     np.testing.assert_allclose(r["vecs"] @ r["vecs"].T, np.eye(7), atol=1e-10)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x2"])
+def test_block_solve_constant_potential(pisces, precision):
+    """V = 0: the kinetic preconditioner is the exact inverse of H - theta, so
+    (H - theta)^{-1} r would give back the Ritz vector; with |T - theta| the
+    correction keeps the components on either side of theta apart and the
+    solve converges.  The sine DVR's levels are exact particle-in-a-box
+    energies pi^2 / (2 L^2) (a^2 + b^2 + c^2): 1, then the (2,1,1) and (2,2,1)
+    triples."""
+    n, L = (20, 20, 20), (30.0, 30.0, 30.0)
+    e = lambda a, b, c: np.pi ** 2 / (2 * 30.0 ** 2) * (a * a + b * b + c * c)
+    exact = np.array([e(1, 1, 1)] + [e(2, 1, 1)] * 3 + [e(2, 2, 1)] * 3)
+    with pisces.Context(n, L) as ctx:
+        ctx.set_potential(np.zeros(ctx.ngp))
+        start = np.random.default_rng(5).uniform(-1, 1, size=(7, ctx.ngp))
+        r = ctx.solve_block(7, start, block=8, maxsub=40, max_iter=1000, ptol=8,
+                            precision=pisces.PRECISIONS[precision])
+        print(precision, r["nconv"], r["n_iter"], r["evals"] - exact, flush=True)
+    assert r["nconv"] == 7, r
+    np.testing.assert_allclose(r["evals"], exact, rtol=0, atol=1e-9)
