@@ -93,3 +93,19 @@ def test_scp_arrays_rejects_mismatched_lists(pisces, bad):
         ions = (np.zeros(4), np.ones(1))
     with pytest.raises(ValueError):
         pisces.scp_arrays(**a, ions=ions)
+
+
+def test_dual_coarse_grid_matches_oracle(pisces, oracle_mod):
+    """pisces_dual_grid_coarse (host arithmetic) against the oracle's coarse
+    grid for odd and even n, and its refusals: non-cubic grids and grids too
+    small to have a coarse one."""
+    for n in range(5, 130, 7):
+        for L in (30.0, 47.5):
+            nc, Lc = pisces.dual_coarse((n,) * 3, (L,) * 3)
+            nco, Lco = oracle_mod.dual_coarse(n, L)
+            assert nc == (nco,) * 3
+            assert Lc == (Lco,) * 3
+    with pytest.raises(pisces.PiscesError):
+        pisces.dual_coarse((32, 32, 33), (40.0,) * 3)
+    with pytest.raises(pisces.PiscesError):
+        pisces.dual_coarse((3, 3, 3), (40.0,) * 3)
