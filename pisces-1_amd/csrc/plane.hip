@@ -456,8 +456,10 @@ __device__ __forceinline__ void hx_bf2_planes(const HxJob &J, int p, int pstride
     const int ntc = g.npc / 16, nti = g.npi / 16, ksc = g.npc / 32, ksi = g.npi / 32;
     bf16x8 ach[NCT][KS], acl[NCT][KS];
     afrag(Mc.th, Mc.tl, g.npc, ntc, ksc, ach, acl);
-    // pass B, grids up to 64: Tx in B form held for the workgroup's lifetime
-    constexpr bool HOLD = PASS == 1 && NPMAX <= 64;
+    // pass B, grids up to 64, several planes per workgroup: Tx in B form held
+    // for the workgroup's lifetime (with one plane it is loaded where used:
+    // the same loads, 64 fewer live registers)
+    constexpr bool HOLD = PASS == 1 && NPMAX <= 64 && MULTI;
     constexpr int NB = HOLD ? NIT : 1, KB = HOLD ? KS : 1;
     bf16x8 bth[NB][KB], btl[NB][KB];
     if constexpr (HOLD) bfrag(Mi.th, Mi.tl, g.npi, nti, ksi, bth, btl);
@@ -543,7 +545,7 @@ __global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ j
 // planes).  A workgroup that draws a plane of B(s) waits until every plane
 // of A(s) is stored: each A plane, once its stores have completed (all
 // memory counters drained), adds 1 to the vector's completion count, and the
-// wait polls that count at device scope.  Tickets are drawn in order, so
+// wait polls that count in the XCD's L2.  Tickets are drawn in order, so
 // every plane of A(s) was drawn earlier by a resident workgroup that waits on
 // nothing: the wait always ends.  It is bounded all the same (a spin limit;
 // a workgroup past it counts the event in hx_spin_timeouts and goes on).
@@ -560,6 +562,24 @@ constexpr int kSegInts = 6;
 constexpr int kXcds = 8;
 __device__ unsigned int hx_spin_timeouts;
 
+// The ticket and completion counters of an XCD are touched only by that XCD's
+// workgroups, and read-modify-writes are performed in the L2 (the vector L1
+// performs no atomics), which every CU of the XCD shares.  A count is read by
+// a compare-and-swap that never matches (an atomic performed in the L2), not
+// by a load: a workgroup-scope load may be served by the L1, and an atomic add
+// of 0 is compiled as such a load.
+__device__ __forceinline__ int xcd_add(int *p, int v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int xcd_read(int *p)
+{
+    int seen = -1;  // counts are never negative: the swap never happens
+    __hip_atomic_compare_exchange_strong(p, &seen, -1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    return seen;
+}
+
 template <int NPMAX>
 __global__ void __launch_bounds__(kPT) hx_bf2_fused_kernel(const HxJob *__restrict__ jobs,
                                                            int *__restrict__ plan)
@@ -574,8 +594,7 @@ __global__ void __launch_bounds__(kPT) hx_bf2_fused_kernel(const HxJob *__restri
     int cursor = plan[16 + xcc];
     for (;;) {
         if (threadIdx.x == 0) {
-            const int t = (int)__hip_atomic_fetch_add(reinterpret_cast<unsigned *>(plan) + xcc, 1u,
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int t = xcd_add(plan + xcc, 1);
             int job = -1;
             if (t < plan[8 + xcc]) {
                 while (cursor < seg_end && t >= segs[kSegInts * cursor + 3] + segs[kSegInts * cursor + 4])
@@ -589,7 +608,7 @@ __global__ void __launch_bounds__(kPT) hx_bf2_fused_kernel(const HxJob *__restri
                     // pass B: every (i,j) plane of this vector's pass A must be stored
                     const int need = jobs[sg[0]].nz;
                     unsigned spins = 0;
-                    while (__hip_atomic_load(done + sg[5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    while (xcd_read(done + sg[5]) < need) {
                         __builtin_amdgcn_s_sleep(2);
                         if (++spins > (1u << 24)) {
                             atomicAdd(&hx_spin_timeouts, 1u);
@@ -614,7 +633,7 @@ __global__ void __launch_bounds__(kPT) hx_bf2_fused_kernel(const HxJob *__restri
             __syncthreads();
             if (threadIdx.x == 0) {
                 const int *sg = segs + kSegInts * cursor;
-                __hip_atomic_fetch_add(done + sg[5], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                xcd_add(done + sg[5], 1);
             }
         }
         __syncthreads();  // s_task and the LDS planes are reused by the next ticket
@@ -1162,8 +1181,9 @@ __device__ __forceinline__ void fdm_bf2_planes(const FdmTask &t, int p, int pstr
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int nti = g.npi / 16, ksi = g.npi / 32;
-    // passes 0 and 2, grids up to 64: Qx in B form held for the workgroup's lifetime
-    constexpr bool HOLD = PASS != 1 && NPMAX <= 64;
+    // passes 0 and 2, grids up to 64, several planes per workgroup: Qx in B form
+    // held for the workgroup's lifetime (with one plane it is loaded where used)
+    constexpr bool HOLD = PASS != 1 && NPMAX <= 64 && MULTI;
     constexpr int NB = HOLD ? NIT : 1, KB = HOLD ? KS : 1;
     bf16x8 qbh[NB][KB], qbl[NB][KB];
     if constexpr (HOLD) {
@@ -1269,8 +1289,7 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_fused_kernel(const FdmTask *__res
     int cursor = plan[16 + xcc];
     for (;;) {
         if (threadIdx.x == 0) {
-            const int t = (int)__hip_atomic_fetch_add(reinterpret_cast<unsigned *>(plan) + xcc, 1u,
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int t = xcd_add(plan + xcc, 1);
             int task = -1;
             if (t < plan[8 + xcc]) {
                 while (cursor < seg_end && t >= segs[kSegInts * cursor + 3] + segs[kSegInts * cursor + 4])
@@ -1284,8 +1303,7 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_fused_kernel(const FdmTask *__res
                     const FdmTask &T = tasks[sg[0]];
                     const int need = sg[2] == 1 ? T.ax[2].n : T.ax[1].n;
                     unsigned spins = 0;
-                    while (__hip_atomic_load(done + 2 * sg[5] + sg[2] - 1, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    while (xcd_read(done + 2 * sg[5] + sg[2] - 1) < need) {
                         __builtin_amdgcn_s_sleep(2);
                         if (++spins > (1u << 24)) {
                             atomicAdd(&hx_spin_timeouts, 1u);
@@ -1310,8 +1328,7 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_fused_kernel(const FdmTask *__res
             __syncthreads();
             if (threadIdx.x == 0) {
                 const int *sg = segs + kSegInts * cursor;
-                __hip_atomic_fetch_add(done + 2 * sg[5] + pass, 1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                xcd_add(done + 2 * sg[5] + pass, 1);
             }
         }
         __syncthreads();  // s_task and the LDS planes are reused by the next ticket
@@ -1572,10 +1589,10 @@ bool one_launch_available()
 }
 
 // The one-launch forms are opt-in (PISCES_HX_FUSED=1, PISCES_FDM_FUSED=1): on
-// the served 64^3 batch they take 0.62 ms (H*X) and 0.80 ms (preconditioner)
-// per launch against 0.12 and 0.16 ms for the multi-launch forms -- every
-// ticket costs a device-scope atomic and a store drain in series with its
-// plane, where the multi-launch grids keep thousands of planes in flight.
+// the served 64^3 batch they took 0.62 ms (H*X) and 0.80 ms (preconditioner)
+// per launch against 0.12 and 0.16 ms for the multi-launch forms (0.53 ms for
+// H*X at 4 waves per SIMD): each ticket puts a counter round trip, the plane's
+// load latency and a store drain in series.
 bool hx_fused_enabled()
 {
     static const bool on = [] {
