@@ -699,26 +699,26 @@ def pmc_summary(fetch_csv, write_csv, copyf_csv, copyw_csv, roof_json):
                    "pass (bench.py --roofline-only --warmup 0), from FETCH_SIZE and WRITE_SIZE "
                    "(KiB per dispatch, summed over the family's kernels), each scaled by what "
                    "the counter reads for a plain copy of 1 GiB; ratio = measured / algorithmic "
-                   "bytes of the same pass",
+                   "bytes of the same pass; $R: the repository root, $OUT: the output folder",
            "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
            "copy_fetch_kib": cf, "copy_write_kib": cw,
            "commands": {
                "fetch": "cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d "
-                        "$R/gpurun_out/pmc_fetch -o f -- python3 $R/bench.py --roofline-only "
+                        "$OUT/pmc_fetch -o f -- python3 $R/bench.py --roofline-only "
                         "--warmup 0",
                "write": "cd /tmp && rocprofv3 --pmc WRITE_SIZE --output-format csv -d "
-                        "$R/gpurun_out/pmc_write -o w -- python3 $R/bench.py --roofline-only "
+                        "$OUT/pmc_write -o w -- python3 $R/bench.py --roofline-only "
                         "--warmup 0",
                "copy": "cd /tmp && rocprofv3 --pmc FETCH_SIZE --output-format csv -d "
-                       "$R/gpurun_out/pmc_copyf -o c -- python3 $R/bench.py --copy-probe "
+                       "$OUT/pmc_copyf -o c -- python3 $R/bench.py --copy-probe "
                        "1073741824 --steps 2 (and WRITE_SIZE into pmc_copyw)",
                "algorithmic": "cd /tmp && rocprofv3 --kernel-trace --stats --output-format csv "
-                              "-d $R/gpurun_out/prof_roof -o bench -- python3 $R/bench.py "
-                              "--roofline-only --warmup 1 > $R/gpurun_out/roof.json",
-               "summary": "python3 bench.py --pmc-summary gpurun_out/pmc_fetch/f_counter_collection.csv "
-                          "gpurun_out/pmc_write/w_counter_collection.csv "
-                          "gpurun_out/pmc_copyf/c_counter_collection.csv "
-                          "gpurun_out/pmc_copyw/c_counter_collection.csv gpurun_out/roof.json"},
+                              "-d $OUT/prof_roof -o bench -- python3 $R/bench.py "
+                              "--roofline-only --warmup 1 > $OUT/roof.json",
+               "summary": "python3 bench.py --pmc-summary $OUT/pmc_fetch/f_counter_collection.csv "
+                          "$OUT/pmc_write/w_counter_collection.csv "
+                          "$OUT/pmc_copyf/c_counter_collection.csv "
+                          "$OUT/pmc_copyw/c_counter_collection.csv $OUT/roof.json"},
            "families": {}}
     for name, t in tot.items():
         f = fam.get(name)

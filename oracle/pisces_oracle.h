@@ -215,7 +215,7 @@ int or_solve_grid(int nx, int ny, int nz, const double *Tx, const double *Ty,
 int or_syev(int n, const double *A, int lda, double *W, double *Vec, int ldv);
 /* Pinned: LAPACK (numpy eigvalsh) and A V = V diag(W). */
 
-/* Electron side of the analytic gradient (SURVEY f2): DVR::ComputeGradient's
+/* Electron side of the analytic gradient: DVR::ComputeGradient's
  * grid loop (src/DVR.cpp:1099-1176) over Potential::EvaluateDPPGTOPGradient
  * (src/Gradients.cpp:79-238) for PotFlag 3, DampType 2, Polarization 3, on the
  * lists of pisces_build_potential_scp: gel[3 nsites] = sum psi^2 dV/dR_site
@@ -238,7 +238,7 @@ void or_gradient_final(int nsites, int np, const double *gel, const double *tmu,
 /* Pinned: the reference's own EvaluateGradient / FinalGradient /
  * SubtractWWGradient through WaterCluster (grad_*.npz). */
 
-/* Dual-grid transfer (SURVEY f3), cubic grids: the coarse grid of n points on
+/* Dual-grid transfer, cubic grids: the coarse grid of n points on
  * length L (src/ClusterAnion.cpp:209-232), the corner weights of triple
  * spacing for residues (rx, ry, rz) (w[a + 2b + 4c]; kind 0 ExtendWfThree,
  * 1 InterpolVtriple), and fine[n^3] from coarse[nc^3] (src/DVR.cpp:1240-1722).
