@@ -2,8 +2,8 @@
 pisces_compute_gradient) against the reference's own gradient code
 (grad_*.npz: ClusterAnion::GetAnalGrad with DVR::ComputeGradient's grid loop
 over Potential::EvaluateGradient, FinalGradient and SubtractWWGradient, and its
-WaterCluster for the water side), and against central differences of the GPU
-potential itself."""
+WaterCluster for the water side), against central differences of the GPU
+potential itself, and against the closed-form gradient of one damped charge."""
 import numpy as np
 import pytest
 
@@ -108,3 +108,39 @@ def test_gradient_terms_large_polarizable_sets(pisces, oracle_mod, nw, radius, s
     assert np.abs(gel - gel_o).max() <= 1e-10 * np.abs(gel_o).max()
     assert np.abs(tmu - tmu_o).max() <= 1e-10 * np.abs(tmu_o).max()
     assert np.abs(mcm - mcm_o).max() <= 1e-10 * np.abs(mcm_o).max()
+
+
+def test_single_charge_gradient_closed_form(pisces):
+    """One charge q on the M site of a quartet whose other sites sit 800 bohr
+    away (the polarizable O site there, tiny alpha): the charge site's entry of
+    sum psi^2 dV/dR is the derivative of the damped Coulomb term -q / Reff(d),
+    d = |r - R|, Reff(d) = r0 (1/2 + rho^3 (1 - rho/2)) for rho = d / r0 < 1 and
+    d beyond r0 (src/Potential.cpp:1191-1196), so
+        dV/dR = -q Reff'(d) (r - R) / (d Reff(d)^2),  Reff' = 3 rho^2 - 2 rho^3 inside,
+    summed with the weights psi^2 in numpy."""
+    n, L = (24, 24, 24), (30.0, 30.0, 30.0)
+    pp = pisces.default_potpara()
+    r0, q = abs(pp[3]), 0.7
+    R = np.array([0.37, -0.52, 0.21])
+    far = np.array([800.0, 800.0, 800.0])
+    sites = np.array([far, far + [1.0, 0, 0], far + [0, 1.0, 0], R])
+    flags = (3, 2, 0, 3, 0, 0)
+    with pisces.Context(n, L) as ctx:
+        xs, _ = ctx.grid()
+        X, Y, Z = np.meshgrid(xs[0], xs[1], xs[2], indexing="ij")
+        psi = np.exp(-((X - 1.0) ** 2 + Y ** 2 + (Z + 0.5) ** 2) / 8.0)
+        psi = (psi / np.linalg.norm(psi)).transpose(2, 1, 0).reshape(-1)
+        ctx.build_potential_scp(flags, pp, sites, [q], [3], [1e-3], [0], np.zeros(3), 0.0)
+        gel, _, _ = ctx.gradient_terms(psi, 4, 1)
+    pts = np.stack([X.transpose(2, 1, 0).reshape(-1), Y.transpose(2, 1, 0).reshape(-1),
+                    Z.transpose(2, 1, 0).reshape(-1)], axis=1)
+    dr = pts - R
+    d = np.linalg.norm(dr, axis=1)
+    rho = d / r0
+    inside = d < r0
+    reff = np.where(inside, r0 * (0.5 + rho ** 3 * (1.0 - 0.5 * rho)), d)
+    dreff = np.where(inside, 3.0 * rho ** 2 - 2.0 * rho ** 3, 1.0)
+    dv = -q * (dreff / (d * reff ** 2))[:, None] * dr
+    expect = (psi ** 2) @ dv
+    assert inside.any()
+    np.testing.assert_allclose(gel[9:12], expect, rtol=1e-11, atol=1e-14 * np.abs(expect).max())
