@@ -233,7 +233,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only a plain device-to-device copy of BYTES (the calibration of the "
                          "FETCH_SIZE / WRITE_SIZE byte counters: one launch moves BYTES in and "
-                         "BYTES out)")
+                         "BYTES out), then read-only streams of BYTES with the default and the "
+                         "nontemporal load policy")
     ap.add_argument("--pmc-summary", nargs=5, default=None,
                     metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV",
                              "ROOF_JSON"),
@@ -640,15 +641,24 @@ def tune_gram(a):
 def copy_probe(P, nbytes, reps):
     """A plain streaming copy of nbytes, reps timed launches after one warm-up
     (pisces_copy_probe): under rocprofv3 --pmc it calibrates the byte counters,
-    alone it gives the achievable copy bandwidth."""
+    alone it gives the achievable copy bandwidth; then read-only streams of the
+    same size with the default and the nontemporal load policy."""
     import ctypes
     L = P.lib()
     L.pisces_copy_probe.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_double)]
     ms = ctypes.c_double()
     P.check(L.pisces_copy_probe(0, nbytes, reps, ctypes.byref(ms)), "pisces_copy_probe")
-    return {"bytes": nbytes, "launches": reps + 1, "ms_per_launch": ms.value,
-            "GBs_read_plus_write": 2 * nbytes / (ms.value * 1e-3) / 1e9}
+    out = {"bytes": nbytes, "launches": reps + 1, "ms_per_launch": ms.value,
+           "GBs_read_plus_write": 2 * nbytes / (ms.value * 1e-3) / 1e9}
+    # read-only streams over the same size (pisces_stream_probe), default and
+    # nontemporal load policy
+    L.pisces_stream_probe.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double)]
+    for nt, key in ((0, "GBs_read_default"), (1, "GBs_read_nontemporal")):
+        P.check(L.pisces_stream_probe(0, nbytes, reps, nt, ctypes.byref(ms)), "pisces_stream_probe")
+        out[key] = nbytes / (ms.value * 1e-3) / 1e9
+    return out
 
 
 # kernel name fragments -> launch family of the solver's HIP-event brackets

@@ -305,6 +305,13 @@ int pisces_time_precond(pisces_ctx *ctx, int nvec, int precision, int warmup, in
  * Measurement hook; no reference counterpart. */
 int pisces_copy_probe(int device, long long bytes, int reps, double *ms);
 
+/* Read-only stream over `bytes` (multiple of 16) of device memory, with the
+ * nontemporal load hint if `nontemporal` != 0; `reps` timed launches after
+ * one warm-up, *ms = mean launch time.  The read bandwidth a plain stream
+ * reaches under each cache policy.  Measurement hook; no reference
+ * counterpart. */
+int pisces_stream_probe(int device, long long bytes, int reps, int nontemporal, double *ms);
+
 /* diag(H) on the host, ngp doubles.  Replaces DVR::ComputeDiagonal
  * (src/DVR.cpp:1808-1881). */
 int pisces_diag(pisces_ctx *ctx, double *diag);

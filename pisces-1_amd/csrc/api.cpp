@@ -724,6 +724,34 @@ int pisces_copy_probe(int device, long long bytes, int reps, double *ms)
     });
 }
 
+int pisces_stream_probe(int device, long long bytes, int reps, int nontemporal, double *ms)
+{
+    if (bytes <= 0 || bytes % 16 || reps <= 0 || !ms) return PISCES_ERR_ARG;
+    return guarded([&] {
+        PC_HIP(hipSetDevice(device));
+        hipStream_t s;
+        PC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        dbuf a(bytes), sink(sizeof(unsigned));
+        fill_uniform(a.p, bytes / 8, true, 7u, s);
+        unsigned *k = static_cast<unsigned *>(sink.p);
+        stream_probe(a.p, bytes, nontemporal != 0, k, s);  // warm-up
+        hipEvent_t e0, e1;
+        PC_HIP(hipEventCreate(&e0));
+        PC_HIP(hipEventCreate(&e1));
+        PC_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) stream_probe(a.p, bytes, nontemporal != 0, k, s);
+        PC_HIP(hipEventRecord(e1, s));
+        PC_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        PC_HIP(hipEventElapsedTime(&t, e0, e1));
+        static_cast<void>(hipEventDestroy(e0));
+        static_cast<void>(hipEventDestroy(e1));
+        PC_HIP(hipStreamDestroy(s));
+        *ms = (double)t / reps;
+        return 0;
+    });
+}
+
 int pisces_kinetic_precond(pisces_ctx *c, int nvec, const double *r, double shift, double scale,
                            double floor, int precision, double *t)
 {

@@ -59,17 +59,25 @@ __device__ inline void st_any(void *p, long long i, int is32, double v)
 // into the chunk's partial.
 typedef double gram_f64x4 __attribute__((ext_vector_type(4)));
 
-template <typename T>
+// A columns are read by one workgroup each (nontemporal); the B tile is read
+// by every workgroup of its rows (default policy, served from the L2)
+template <bool NT, typename V> __device__ __forceinline__ V gram_ld(const V *p)
+{
+    if constexpr (NT) return ldnt(p);
+    else return *p;
+}
+
+template <bool NT, typename T>
 __device__ __forceinline__ void gram_load4(const T *col, long long r, long long N, bool vec,
                                            double (&v)[4])
 {
     if (vec && r + 4 <= N) {
         if (sizeof(T) == 8) {
-            const f64x2 a = *reinterpret_cast<const f64x2 *>(col + r);
-            const f64x2 b = *reinterpret_cast<const f64x2 *>(col + r + 2);
+            const f64x2 a = gram_ld<NT>(reinterpret_cast<const f64x2 *>(col + r));
+            const f64x2 b = gram_ld<NT>(reinterpret_cast<const f64x2 *>(col + r + 2));
             v[0] = a[0]; v[1] = a[1]; v[2] = b[0]; v[3] = b[1];
         } else {
-            const f32x4 a = *reinterpret_cast<const f32x4 *>(col + r);
+            const f32x4 a = gram_ld<NT>(reinterpret_cast<const f32x4 *>(col + r));
             v[0] = a[0]; v[1] = a[1]; v[2] = a[2]; v[3] = a[3];
         }
     } else {
@@ -92,14 +100,14 @@ __device__ __forceinline__ void gram_chunk(const TA *A, long long lda, int ma,
     for (long long r0 = r_beg + 16 * wave; r0 < r_end; r0 += 64) {
         const long long r = r0 + 4 * q;
         double bv[4];
-        gram_load4(bcol, r, N, vec, bv);
+        gram_load4<false>(bcol, r, N, vec, bv);
         if (!bval) bv[0] = bv[1] = bv[2] = bv[3] = 0.0;
 #pragma unroll
         for (int t = 0; t < kBdMaxM / 16; ++t) {
             if (t >= nta) break;
             const int a = 16 * t + n;
             double av[4];
-            gram_load4(A + (long long)(a < ma ? a : 0) * lda, r, N, vec, av);
+            gram_load4<true>(A + (long long)(a < ma ? a : 0) * lda, r, N, vec, av);
             if (a >= ma) av[0] = av[1] = av[2] = av[3] = 0.0;
 #pragma unroll
             for (int j = 0; j < 4; ++j)
@@ -112,7 +120,7 @@ __device__ __forceinline__ void gram_chunk(const TA *A, long long lda, int ma,
 // lane (n, q) holding rows r0 + 16q .. r0 + 16q + 15 of column n (four 16-byte
 // loads per column for FP64), fed to 16 instructions; each column is then read
 // in 512-byte runs like the VALU kernel.
-template <typename T>
+template <bool NT, typename T>
 __device__ __forceinline__ void gram_load16(const T *col, long long r, long long N, bool vec,
                                             double (&v)[16])
 {
@@ -120,14 +128,14 @@ __device__ __forceinline__ void gram_load16(const T *col, long long r, long long
         if (sizeof(T) == 8) {
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
-                const f64x2 a = *reinterpret_cast<const f64x2 *>(col + r + 2 * h);
+                const f64x2 a = gram_ld<NT>(reinterpret_cast<const f64x2 *>(col + r + 2 * h));
                 v[2 * h] = a[0];
                 v[2 * h + 1] = a[1];
             }
         } else {
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
-                const f32x4 a = *reinterpret_cast<const f32x4 *>(col + r + 4 * h);
+                const f32x4 a = gram_ld<NT>(reinterpret_cast<const f32x4 *>(col + r + 4 * h));
                 v[4 * h] = a[0]; v[4 * h + 1] = a[1]; v[4 * h + 2] = a[2]; v[4 * h + 3] = a[3];
             }
         }
@@ -151,7 +159,7 @@ __device__ __forceinline__ void gram_chunk64(const TA *A, long long lda, int ma,
     for (long long r0 = r_beg + 64 * wave; r0 < r_end; r0 += 256) {
         const long long r = r0 + 16 * q;
         double bv[16];
-        gram_load16(bcol, r, N, vec, bv);
+        gram_load16<false>(bcol, r, N, vec, bv);
         if (!bval)
 #pragma unroll
             for (int j = 0; j < 16; ++j) bv[j] = 0.0;
@@ -160,7 +168,7 @@ __device__ __forceinline__ void gram_chunk64(const TA *A, long long lda, int ma,
             if (t >= nta) break;
             const int a = 16 * t + n;
             double av[16];
-            gram_load16(A + (long long)(a < ma ? a : 0) * lda, r, N, vec, av);
+            gram_load16<true>(A + (long long)(a < ma ? a : 0) * lda, r, N, vec, av);
             if (a >= ma)
 #pragma unroll
                 for (int j = 0; j < 16; ++j) av[j] = 0.0;
@@ -350,7 +358,8 @@ __global__ void __launch_bounds__(kT) gram_kernel(const GramTask *__restrict__ t
 // columns, all 12 columns of every row of the group loaded before the
 // products) and keeps its 32 products in registers; a butterfly
 // reduce-scatter over the wave and a sum over the four waves through LDS give
-// the chunk's partials.
+// the chunk's partials.  A columns are loaded nontemporal; the B columns keep
+// the default policy, since every tile of the chunk reads them (from the L2).
 constexpr int kGNarrowRows = 4;
 // NA: A columns loaded per row (8, or 4 for a tile with at most 4 columns)
 template <int NA, typename TA, typename TB>
@@ -381,7 +390,7 @@ __device__ __forceinline__ void gram_rows_narrow(gptr<const TA> A, long long lda
 #pragma unroll
         for (int r = 0; r < R; ++r) {
 #pragma unroll
-            for (int i = 0; i < NA; ++i) a[r][i] = A[gi[r] + (i < na ? i : 0) * lda];
+            for (int i = 0; i < NA; ++i) a[r][i] = ldnt(A + (gi[r] + (i < na ? i : 0) * lda));
 #pragma unroll
             for (int j = 0; j < kGNarrowB; ++j) b[r][j] = B[gi[r] + (j < nb ? j : 0) * ldb];
         }
@@ -479,18 +488,51 @@ __global__ void __launch_bounds__(kT) gram_narrow_kernel(const GramTask *__restr
     }
 }
 
-// one thread per Gram entry: sum of its chunk partials (fixed order)
+// Partial-sum reductions.  Their loads are what the step waits on between a
+// streaming pass and the small factor kernels that read its sums, so every
+// load of an entry is issued before the first add: a lane sums a strided
+// subset of the partials with its loads in flight together, and a fixed
+// butterfly over the lanes of the entry gives the total (the same order on
+// every run).
+// Sum over c < n of p[c] by the L lanes (a power of two, lanes j of an aligned
+// group of L) of the calling group: lane j takes c = j, j + L, ... in runs of
+// eight loads.
+template <int L>
+__device__ __forceinline__ double group_sum(const double *__restrict__ p, int n, int j)
+{
+    double s[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u] = 0.0;
+    for (int c0 = j; c0 < n; c0 += 8 * L) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int c = c0 + L * u;
+            x[u] = p[c < n ? c : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s[u] += (c0 + L * u < n) ? x[u] : 0.0;
+    }
+    double v = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+#pragma unroll
+    for (int off = L / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Gram entries: kGramRedLanes lanes per entry, kT / kGramRedLanes entries per
+// workgroup; the entry's chunk partials are summed as above
+constexpr int kGramRedLanes = 8;
 __global__ void __launch_bounds__(kT) gram_reduce_kernel(const GramTask *__restrict__ tasks,
                                                          const double *__restrict__ partial)
 {
     const GramTask t = tasks[blockIdx.y];
     if (t.skip && *cst(t.skip)) return;
-    const int o = blockIdx.x * kT + threadIdx.x;
+    const int o = blockIdx.x * (kT / kGramRedLanes) + threadIdx.x / kGramRedLanes;
+    const int j = threadIdx.x & (kGramRedLanes - 1);
+    // whole lane groups leave together, so the group's shuffles see only live lanes
     if (o >= t.ma * t.mb) return;
-    const double *p = partial + t.part_off + (long long)o * t.nch;
-    double s = 0.0;
-    for (int c = 0; c < t.nch; ++c) s += p[c];
-    t.out[(o % t.ma) + (long long)t.out_ld * (o / t.ma)] = s;
+    const double s = group_sum<kGramRedLanes>(partial + t.part_off + (long long)o * t.nch, t.nch, j);
+    if (j == 0) t.out[(o % t.ma) + (long long)t.out_ld * (o / t.ma)] = s;
 }
 
 // ------------------------------------------------------------------ combine
@@ -514,7 +556,7 @@ __device__ __forceinline__ void comb_group(gptr<const T> A, long long lda, int a
 #pragma unroll
     for (int q = 0; q < G; ++q)
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) x[r][q] = A[(long long)(a + q) * lda + gi[r]];
+        for (int r = 0; r < RPT; ++r) x[r][q] = ldnt(A + ((long long)(a + q) * lda + gi[r]));
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
         const int bc = b < mb ? b : mb - 1;
@@ -822,7 +864,8 @@ __global__ void __launch_bounds__(kT) resid_kernel(const ResTask *__restrict__ t
 #pragma unroll
         for (int i = 0; i < kBdMaxK; ++i) {
             if (i >= t.k) break;
-            const double r = glb(t.HU)[i * t.ld + g] - t.theta[i * t.th_ld] * glb(t.U)[i * t.ld + g];
+            const double r = ldnt(glb(t.HU) + (i * t.ld + g)) -
+                             t.theta[i * t.th_ld] * ldnt(glb(t.U) + (i * t.ld + g));
             st_any(t.R, i * t.ld + g, t.r32, r);
             acc[i] += r * r;
         }
@@ -842,17 +885,21 @@ __global__ void __launch_bounds__(kT) resid_kernel(const ResTask *__restrict__ t
     }
 }
 
-__global__ void __launch_bounds__(64) resid_reduce_kernel(const ResTask *__restrict__ tasks,
+// one wave per column of the block (four per workgroup)
+__global__ void __launch_bounds__(kT) resid_reduce_kernel(const ResTask *__restrict__ tasks,
                                                           const double *__restrict__ partial)
 {
     const ResTask t = tasks[blockIdx.x];
-    if ((int)threadIdx.x >= t.k) return;
-    double s = 0;
-    for (int c = 0; c < t.nch; ++c) s += partial[t.part_off + (long long)threadIdx.x * t.nch + c];
-    t.out[threadIdx.x] = s;
-    if (t.gather) {
-        t.gather[threadIdx.x] = t.theta[threadIdx.x * t.th_ld];
-        t.gather[kBdMaxK + threadIdx.x] = s;
+    const int lane = threadIdx.x & 63;
+    for (int b = threadIdx.x >> 6; b < t.k; b += kT / 64) {
+        const double s = group_sum<64>(partial + t.part_off + (long long)b * t.nch, t.nch, lane);
+        if (lane == 0) {
+            t.out[b] = s;
+            if (t.gather) {
+                t.gather[b] = t.theta[b * t.th_ld];
+                t.gather[kBdMaxK + b] = s;
+            }
+        }
     }
 }
 
@@ -871,8 +918,8 @@ __device__ __forceinline__ void ritz_group(gptr<const T> A, gptr<const T> HA, lo
     for (int q = 0; q < G; ++q)
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
-            x[r][q] = A[(long long)(a + q) * N + gi[r]];
-            h[r][q] = HA[(long long)(a + q) * N + gi[r]];
+            x[r][q] = ldnt(A + ((long long)(a + q) * N + gi[r]));
+            h[r][q] = ldnt(HA + ((long long)(a + q) * N + gi[r]));
         }
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
@@ -974,21 +1021,19 @@ __global__ void __launch_bounds__(kT) ritz_kernel(const RitzTask *__restrict__ t
     }
 }
 
-__global__ void __launch_bounds__(64) ritz_reduce_kernel(const RitzTask *__restrict__ tasks,
+// one wave per Ritz column (four per workgroup)
+__global__ void __launch_bounds__(kT) ritz_reduce_kernel(const RitzTask *__restrict__ tasks,
                                                          const double *__restrict__ partial)
 {
     const RitzTask &t = tasks[blockIdx.x];
-    const int lane = threadIdx.x;
-    for (int b = 0; b < t.k; ++b) {
-        const double *p = partial + t.part_off + (long long)b * t.nch;
-        double sum = 0;
-        for (int c = lane; c < t.nch; c += 64) sum += p[c];
-        for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    const int lane = threadIdx.x & 63;
+    for (int b = threadIdx.x >> 6; b < t.k; b += kT / 64) {
+        const double s = group_sum<64>(partial + t.part_off + (long long)b * t.nch, t.nch, lane);
         if (lane == 0) {
-            t.out[b] = sum;
+            t.out[b] = s;
             if (t.gather) {
                 t.gather[b] = t.theta[b];
-                t.gather[kBdMaxK + b] = sum;
+                t.gather[kBdMaxK + b] = s;
             }
         }
     }
@@ -1460,7 +1505,8 @@ void bd_gram(const GramTask *d_tasks, int ntasks, int max_blocks, int max_blocks
     else
         hipLaunchKernelGGL(gram_kernel<1>, dim3(max_blocks, ntasks), dim3(kT), 0, s, d_tasks, d_partial, rows);
     if (reduce)
-        hipLaunchKernelGGL(gram_reduce_kernel, dim3((max_mamb + kT - 1) / kT, ntasks), dim3(kT), 0,
+        hipLaunchKernelGGL(gram_reduce_kernel,
+                           dim3((max_mamb + kT / kGramRedLanes - 1) / (kT / kGramRedLanes), ntasks), dim3(kT), 0,
                            s, d_tasks, d_partial);
     PC_HIP(hipGetLastError());
 }
@@ -1551,7 +1597,7 @@ void bd_ritz(const RitzTask *d_tasks, int ntasks, int max_nch, int k, double *d_
     else if (k <= 4) hipLaunchKernelGGL((ritz_kernel<4, 2>), grid, dim3(kT), 0, s, d_tasks, d_partial, rev);
     else if (k <= 8) hipLaunchKernelGGL((ritz_kernel<8, 1>), grid, dim3(kT), 0, s, d_tasks, d_partial, rev);
     else hipLaunchKernelGGL((ritz_kernel<16, 1>), grid, dim3(kT), 0, s, d_tasks, d_partial, rev);
-    hipLaunchKernelGGL(ritz_reduce_kernel, dim3(ntasks), dim3(64), 0, s, d_tasks, d_partial);
+    hipLaunchKernelGGL(ritz_reduce_kernel, dim3(ntasks), dim3(kT), 0, s, d_tasks, d_partial);
     PC_HIP(hipGetLastError());
 }
 
@@ -1580,7 +1626,7 @@ void bd_resid(const ResTask *d_tasks, int ntasks, int max_nch, double *d_partial
 {
     if (ntasks <= 0) return;
     hipLaunchKernelGGL(resid_kernel, dim3(max_nch, ntasks), dim3(kT), 0, s, d_tasks, d_partial);
-    hipLaunchKernelGGL(resid_reduce_kernel, dim3(ntasks), dim3(64), 0, s, d_tasks, d_partial);
+    hipLaunchKernelGGL(resid_reduce_kernel, dim3(ntasks), dim3(kT), 0, s, d_tasks, d_partial);
     PC_HIP(hipGetLastError());
 }
 

@@ -448,6 +448,46 @@ void copy_probe(const void *src, void *dst, long long bytes, hipStream_t s)
     PC_HIP(hipGetLastError());
 }
 
+// read-only stream (16-byte vectors, grid-stride, four loads in flight per
+// thread), with or without the nontemporal hint: the read bandwidth a plain
+// stream reaches under each cache policy.  The loaded words are folded into
+// one value per thread, stored only if it equals a pattern the uniform fill
+// does not produce, so the loads cannot be dropped.
+typedef unsigned probe_u32x4 __attribute__((ext_vector_type(4)));
+template <bool NT>
+__global__ void __launch_bounds__(256) stream_probe_kernel(const probe_u32x4 *__restrict__ src,
+                                                           long long n16, unsigned *__restrict__ sink)
+{
+    const long long stride = (long long)gridDim.x * 1024;
+    probe_u32x4 acc = {0u, 0u, 0u, 0u};
+    for (long long base = (long long)blockIdx.x * 1024; base < n16; base += stride) {
+        probe_u32x4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            long long i = base + u * 256 + threadIdx.x;
+            i = i < n16 ? i : n16 - 1;
+            if constexpr (NT) v[u] = __builtin_nontemporal_load(src + i);
+            else v[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc ^= v[u];
+    }
+    const unsigned x = acc.x ^ acc.y ^ acc.z ^ acc.w;
+    if (x == 0x9e3779b9u) sink[0] = x;
+}
+
+void stream_probe(const void *src, long long bytes, bool nontemporal, unsigned *sink, hipStream_t s)
+{
+    const long long n16 = bytes / 16;
+    const long long blocks = std::min<long long>((n16 + 1023) / 1024, 8192);
+    const probe_u32x4 *p = static_cast<const probe_u32x4 *>(src);
+    if (nontemporal)
+        hipLaunchKernelGGL(stream_probe_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, p, n16, sink);
+    else
+        hipLaunchKernelGGL(stream_probe_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, p, n16, sink);
+    PC_HIP(hipGetLastError());
+}
+
 void fill_uniform(void *x, long long n, bool f64, unsigned seed, hipStream_t s)
 {
     hipLaunchKernelGGL(fill_kernel, dim3(nblk(n)), dim3(kT), 0, s, x, n, f64 ? 1 : 0, seed);

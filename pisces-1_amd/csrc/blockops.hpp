@@ -51,6 +51,9 @@ void to_f64(const float *x, double *y, long long n, hipStream_t s);
 void fill_uniform(void *x, long long n, bool f64, unsigned seed, hipStream_t s);
 // dst = src, bytes a multiple of 16 (copy_probe_kernel)
 void copy_probe(const void *src, void *dst, long long bytes, hipStream_t s);
+// read-only stream over `bytes` (a multiple of 16), nontemporal loads if
+// asked (stream_probe_kernel); sink: one unsigned of device memory
+void stream_probe(const void *src, long long bytes, bool nontemporal, unsigned *sink, hipStream_t s);
 
 // Symmetric eigensolver on the host for the projected matrix (m <= 64):
 // cyclic Jacobi, ascending eigenvalues; reads the upper triangle of A

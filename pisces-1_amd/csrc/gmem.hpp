@@ -23,6 +23,20 @@ template <typename T> using cptr = __attribute__((address_space(4))) const T *;
 
 template <typename T> __device__ __forceinline__ cptr<T> cst(const T *p) { return (cptr<T>)p; }
 
+// Loads of block columns that one workgroup of a launch streams through once
+// carry the nontemporal hint (the nt bit): on MI355X a plain read stream of
+// 1 GiB measured 7.1 TB/s with it against 6.3 TB/s with the default policy
+// (pisces_stream_probe).  Columns that several workgroups of the same launch
+// read again from the L2 keep the default policy.
+template <typename T> __device__ __forceinline__ T ldnt(gptr<const T> p)
+{
+    return __builtin_nontemporal_load(p);
+}
+template <typename T> __device__ __forceinline__ T ldnt(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
 // 1/x and 1/sqrt(x) in FP64 to about one ulp: the hardware estimates refined
 // by two Newton-Raphson steps each (r <- r + r (1 - x r);
 // y <- y (3 - x y^2) / 2), each step squaring the relative error.
