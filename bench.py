@@ -16,7 +16,7 @@ Reported next to each other on the same snapshots:
   * the eigenvalue error of every state of every snapshot against the FP64
     GPU result (which matches the CPU oracle within 1e-8 Eh in the tests);
   * HIP-event timing of every batched launch family over one more step of
-    the batch on a single engine lane (the headline's two lanes overlap their
+    the batch on a single engine lane (the headline's three lanes overlap their
     launches), the roofline of the dominant one;
   * the CPU restatement of the reference path (its Davidson, dense-T H*x, the
     DPP potential) timed on the host cores on a bounded sample.
@@ -81,7 +81,7 @@ def run_single(P, geoms, precision):
 
 
 def run_server(P, geoms, precision, steps, warmup, prof=False, lanes=None):
-    """`lanes`: the server's engine lanes (None: its default, two)."""
+    """`lanes`: the server's engine lanes (None: its default, three)."""
     if lanes is None:
         os.environ.pop("PISCES_SERVER_LANES", None)
     else:
@@ -117,7 +117,7 @@ def run_server(P, geoms, precision, steps, warmup, prof=False, lanes=None):
 
 def roofline_pass(P, geoms, precision):
     """One step of the served batch on one engine lane with HIP-event timing of
-    every launch family.  The headline runs two lanes whose launches overlap,
+    every launch family.  The headline runs three lanes whose launches overlap,
     so there an event pair around a launch also spans the other lane's work;
     on one lane each family's time is its own."""
     res, dt, _, fam = run_server(P, geoms, precision, 1, 0, prof=True, lanes=1)
@@ -380,7 +380,7 @@ def main():
         "iters_mean": round(float(np.mean([r["n_iter"] for r in res[-1]])), 1),
         "roofline": roof,
         "launch_families": fam_out,
-        "lanes": 2,
+        "lanes": 3,
         "autotuned": P.tuning(),
         "hx_standalone_64cube_x8": hx,
         "hx_roofline": hx_roof,

@@ -260,6 +260,10 @@ __device__ __forceinline__ void own_load(const S *src_, long long base, const Ge
         }
 }
 
+// Row padding (BF16 elements) of the split planes in LDS: a row of a 64-point
+// plane spans 64 + kBf2Pad elements (16 and 24 measured no faster)
+constexpr int kBf2Pad = 8;
+
 // Exchange a dword with the neighbouring lane (DPP quad_perm 1,0,3,2).
 __device__ __forceinline__ unsigned swap_adj(unsigned x)
 {
@@ -306,7 +310,7 @@ __device__ __forceinline__ void own_stage(const float (&x)[NCT][NIT][4], const G
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
-    const int ntc = g.npc / 16, nti = g.npi / 16, pt = g.npc + 8, pc = g.npi + 8;
+    const int ntc = g.npc / 16, nti = g.npi / 16, pt = g.npc + kBf2Pad, pc = g.npi + kBf2Pad;
 #pragma unroll
     for (int u = 0; u < NCT; ++u) {
         const int tc = wave + 4 * u;
@@ -443,7 +447,7 @@ __device__ __forceinline__ void hx_bf2_planes(const HxJob &J, int p, int pstride
     const AxisMats &Mc = J.ax[PASS == 0 ? 1 : 2];
     Geo g;
     if (!geo_of(PASS, J.nx, J.ny, J.nz, J.nvec, J.ld, Mi.np, Mc.np, p, g)) return;
-    const int pt = g.npc + 8, pc = g.npi + 8;
+    const int pt = g.npc + kBf2Pad, pc = g.npi + kBf2Pad;
     uint16_t *Pth = reinterpret_cast<uint16_t *>(smem);
     uint16_t *Ptl = Pth + g.npi * pt;
     uint16_t *Pch = Ptl + g.npi * pt;
@@ -1012,7 +1016,7 @@ __device__ __forceinline__ void put_scaled(const f32x4 (&acc)[NCT][NIT], const G
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
-    const int ntc = g.npc / 16, nti = g.npi / 16, pt = g.npc + 8;
+    const int ntc = g.npc / 16, nti = g.npi / 16, pt = g.npc + kBf2Pad;
 #pragma unroll
     for (int u = 0; u < NCT; ++u) {
         const int tc = wave + 4 * u;
@@ -1085,7 +1089,7 @@ __device__ __forceinline__ void mm_c_all(const bf16x8 (&ah)[NCT][KS], const bf16
                                          f32x4 (&acc)[NCT][NIT])
 {
     const int wave = threadIdx.x >> 6;
-    const int ntc = g.npc / 16, nti = g.npi / 16, ksc = g.npc / 32, pt = g.npc + 8;
+    const int ntc = g.npc / 16, nti = g.npi / 16, ksc = g.npc / 32, pt = g.npc + kBf2Pad;
 #pragma unroll
     for (int u = 0; u < NCT; ++u) {
         if (wave + 4 * u >= ntc) break;
@@ -1106,7 +1110,7 @@ __device__ __forceinline__ void mm_i_held(const uint16_t *Pch, const uint16_t *P
                                           const Geo &g, f32x4 (&acc)[NCT][NIT])
 {
     const int wave = threadIdx.x >> 6;
-    const int ntc = g.npc / 16, nti = g.npi / 16, ksi = g.npi / 32, pc = g.npi + 8;
+    const int ntc = g.npc / 16, nti = g.npi / 16, ksi = g.npi / 32, pc = g.npi + kBf2Pad;
 #pragma unroll
     for (int u = 0; u < NCT; ++u) {
         const int tc = wave + 4 * u;
@@ -1129,7 +1133,7 @@ __device__ __forceinline__ void mm_i_all(const uint16_t *Pch, const uint16_t *Pc
                                          const uint16_t *Ml, const Geo &g, f32x4 (&acc)[NCT][NIT])
 {
     const int wave = threadIdx.x >> 6;
-    const int ntc = g.npc / 16, nti = g.npi / 16, ksi = g.npi / 32, pc = g.npi + 8;
+    const int ntc = g.npc / 16, nti = g.npi / 16, ksi = g.npi / 32, pc = g.npi + kBf2Pad;
 #pragma unroll
     for (int u = 0; u < NCT; ++u) {
         const int tc = wave + 4 * u;
@@ -1165,7 +1169,7 @@ __device__ __forceinline__ void fdm_bf2_planes(const FdmTask &t, int p, int pstr
     const AxisMats &Mc = PASS == 1 ? Mz : My;
     Geo g;
     if (!geo_of(kind, Mx.n, My.n, Mz.n, 1, 0, Mx.np, Mc.np, p, g)) return;
-    const int pt = g.npc + 8, pc = g.npi + 8;
+    const int pt = g.npc + kBf2Pad, pc = g.npi + kBf2Pad;
     uint16_t *Pth = reinterpret_cast<uint16_t *>(smem);
     uint16_t *Ptl = Pth + g.npi * pt;
     uint16_t *P2h = Ptl + g.npi * pt;  // c-major plane (passes 0, 2) or i-major plane B (pass 1)
@@ -1346,7 +1350,7 @@ void allow_lds(K kernel, size_t bytes = 160 * 1024)
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-size_t bf2_lds(int np) { return (size_t)2 * 2 * np * (np + 8) * sizeof(uint16_t); }
+size_t bf2_lds(int np) { return (size_t)2 * 2 * np * (np + kBf2Pad) * sizeof(uint16_t); }
 template <typename T> size_t f_lds(int np) { return (size_t)np * fpitch<T>(np) * sizeof(T); }
 
 template <int NPMAX>

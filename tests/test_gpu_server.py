@@ -95,11 +95,13 @@ def test_warm_start_chain(pisces):
     assert sum(w["n_iter"] for w in warm[1:]) < sum(c["n_iter"] for c in cold[1:])
 
 
-def test_warm_chains_cross_lanes(pisces):
-    """Two chains submitted together in alternating order: the lanes take the
-    requests in turn, so each chain's next snapshot starts on the other lane
-    from vectors the first lane wrote.  Warm results match cold solves."""
+def test_warm_chains_cross_lanes(pisces, monkeypatch):
+    """Two chains submitted together in alternating order on a two-lane server:
+    the lanes take the requests in turn, so each chain's next snapshot starts
+    on the other lane from vectors the first lane wrote.  Warm results match
+    cold solves."""
     import workload
+    monkeypatch.setenv("PISCES_SERVER_LANES", "2")
     rng = np.random.default_rng(11)
     n, L, nw, rad, seed, nr = SIZES[0]
     chains = [[_xyz(nw, rad, seed)], [_xyz(nw, rad, seed + 50)]]
@@ -129,7 +131,7 @@ def test_warm_chains_cross_lanes(pisces):
 
 def test_block_classes_and_lanes_in_one_mix(pisces):
     """Requests of 2, 6 and 4 states together: the 6-state one runs on a block-8
-    engine, the others on block-4 engines, spread over two lanes each; every
+    engine, the others on block-4 engines, spread over the lanes; every
     result equals the same request solved alone."""
     cases = [((32, 32, 32), 30.0, 6, 3.5, 4200, 2), ((40, 40, 40), 34.0, 10, 4.0, 4201, 6),
              ((32, 32, 32), 30.0, 8, 3.5, 4202, 4), ((36, 36, 36), 32.0, 7, 3.5, 4203, 6)]
