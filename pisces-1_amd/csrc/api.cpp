@@ -433,7 +433,9 @@ int pisces_build_cluster_potential(pisces_ctx *c, int nw, const double *xyz_ang,
             // (src/ClusterAnion.cpp:173; WaterWaterPolarizationEnergy,
             // src/DPP.cpp:1935-2024, summed over all pairs: -mu.Epc / 2)
             std::vector<double> epc((size_t)3 * ns);
-            PC_HIP(hipMemcpy(epc.data(), depc.p, sizeof(double) * epc.size(), hipMemcpyDeviceToHost));
+            PC_HIP(hipMemcpyAsync(epc.data(), depc.p, sizeof(double) * epc.size(), hipMemcpyDeviceToHost,
+                                  c->stream));
+            PC_HIP(hipStreamSynchronize(c->stream));
             const double a3 = 1.0 / (0.52917720859 * 0.52917720859 * 0.52917720859);
             std::vector<double> alpha((size_t)ns);
             std::vector<int> ip((size_t)ns);
@@ -542,7 +544,8 @@ int pisces_dual_grid_extend(pisces_ctx *c, const int nc[3], const double *coarse
         dbuf dc(sizeof(double) * ncg), df(sizeof(double) * c->ngp);
         PC_HIP(hipMemcpyAsync(dc.p, coarse, sizeof(double) * ncg, hipMemcpyHostToDevice, c->stream));
         dual_transfer(c, nc[0], dc.as<double>(), kind, nullptr, 0, 0.0, df.as<double>());
-        PC_HIP(hipMemcpy(fine, df.p, sizeof(double) * c->ngp, hipMemcpyDeviceToHost));
+        PC_HIP(hipMemcpyAsync(fine, df.p, sizeof(double) * c->ngp, hipMemcpyDeviceToHost, c->stream));
+        PC_HIP(hipStreamSynchronize(c->stream));
         return 0;
     });
 }

@@ -144,7 +144,11 @@ BdEngine::BdEngine(int device, hipStream_t s, int precision, int block, int maxs
         double *eye = reinterpret_cast<double *>(h.data() + kInitEye);
         for (int i = 0; i < k_; ++i) eye[(size_t)i * k_ + i] = 1.0;
         dinit_.alloc(h.size());
-        PC_HIP(hipMemcpy(dinit_.p, h.data(), h.size(), hipMemcpyHostToDevice));
+        // on the engine's stream: the library never touches the null stream,
+        // which would hold one of the device's few hardware queues for the
+        // rest of the process and leave two engine lanes sharing another
+        PC_HIP(hipMemcpyAsync(dinit_.p, h.data(), h.size(), hipMemcpyHostToDevice, s_));
+        PC_HIP(hipStreamSynchronize(s_));
     }
     const char *tr = std::getenv("PISCES_TRACE");
     trace_ = tr && tr[0] == '1';

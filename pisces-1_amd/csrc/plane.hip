@@ -1608,8 +1608,16 @@ bool hx_fused_enabled()
 
 unsigned hx_fused_spin_timeouts()
 {
+    // a stream of its own (released again), not the null stream
     unsigned v = 0;
-    PC_HIP(hipMemcpyFromSymbol(&v, HIP_SYMBOL(hx_spin_timeouts), sizeof(v), 0, hipMemcpyDeviceToHost));
+    hipStream_t s;
+    PC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const hipError_t e = hipMemcpyFromSymbolAsync(&v, HIP_SYMBOL(hx_spin_timeouts), sizeof(v), 0,
+                                                  hipMemcpyDeviceToHost, s);
+    const hipError_t w = hipStreamSynchronize(s);
+    static_cast<void>(hipStreamDestroy(s));
+    PC_HIP(e);
+    PC_HIP(w);
     return v;
 }
 
