@@ -543,16 +543,22 @@ __global__ void __launch_bounds__(kPT) hx_bf2_kernel(const HxJob *__restrict__ j
 // the X and Y that pass A left in that XCD's L2 instead of HBM.  The vectors
 // are split into eight contiguous runs of about equal work, one per XCD; the
 // workgroups of an XCD (read from the hardware's XCC_ID) take tickets from
-// that XCD's counter in order, and the ticket sequence of run s0, s1, ... is
+// that XCD's counter in order, `batch` consecutive tickets per counter
+// increment (one atomic per ticket serialised the 64 workgroups of an XCD on
+// the counter's cache line: ~0.5 us per draw), each batch worked through in
+// order, and the ticket sequence of run s0, s1, ... is
 //   A(s0) | A(s1) B(s0) | A(s2) B(s1) | ... | B(s_last)
 // (A(s): the n_z (i,j) planes of pass A of vector s; B(s): its n_y (i,k)
 // planes).  A workgroup that draws a plane of B(s) waits until every plane
 // of A(s) is stored: each A plane, once its stores have completed (all
 // memory counters drained), adds 1 to the vector's completion count, and the
-// wait polls that count in the XCD's L2.  Tickets are drawn in order, so
-// every plane of A(s) was drawn earlier by a resident workgroup that waits on
-// nothing: the wait always ends.  It is bounded all the same (a spin limit;
-// a workgroup past it counts the event in hx_spin_timeouts and goes on).
+// wait polls that count in the XCD's L2.  Every plane a ticket waits on has a
+// smaller ticket: either earlier in the waiting workgroup's own batch (done
+// already) or drawn earlier by a running workgroup, which works through its
+// batch in order and can itself wait only on smaller tickets; the smallest
+// unfinished ticket waits on nothing, so the wait always ends.  It is bounded
+// all the same (a spin limit; a workgroup past it counts the event in
+// hx_spin_timeouts and goes on).
 // X is not written by the launch and Y of a vector only by its own A and B
 // planes, so no stale copy of either can sit in a CU's L1: the L1 is written
 // through and holds only lines that the CU read after their last writer
@@ -586,7 +592,7 @@ __device__ __forceinline__ int xcd_read(int *p)
 
 template <int NPMAX>
 __global__ void __launch_bounds__(kPT) hx_bf2_fused_kernel(const HxJob *__restrict__ jobs,
-                                                           int *__restrict__ plan)
+                                                           int *__restrict__ plan, int batch)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_task[4];  // job, vector, pass, plane; -1 job: no tickets left
@@ -596,9 +602,14 @@ __global__ void __launch_bounds__(kPT) hx_bf2_fused_kernel(const HxJob *__restri
     const int *segs = plan + 26 + nvecs;
     const int seg_end = plan[16 + xcc + 1];
     int cursor = plan[16 + xcc];
+    int bt = 0, be = 0;  // thread 0: the next ticket of the workgroup's batch, its end
     for (;;) {
         if (threadIdx.x == 0) {
-            const int t = xcd_add(plan + xcc, 1);
+            if (bt >= be) {
+                bt = xcd_add(plan + xcc, batch);
+                be = bt + batch;
+            }
+            const int t = bt++;
             int job = -1;
             if (t < plan[8 + xcc]) {
                 while (cursor < seg_end && t >= segs[kSegInts * cursor + 3] + segs[kSegInts * cursor + 4])
@@ -1281,7 +1292,7 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_kernel(const FdmTask *__restrict_
 // pass, first ticket, planes, vector slot).
 template <int NPMAX>
 __global__ void __launch_bounds__(kPT) fdm_bf2_fused_kernel(const FdmTask *__restrict__ tasks,
-                                                            int *__restrict__ plan)
+                                                            int *__restrict__ plan, int batch)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_task[3];  // task, pass, plane; -1 task: no tickets left
@@ -1291,9 +1302,14 @@ __global__ void __launch_bounds__(kPT) fdm_bf2_fused_kernel(const FdmTask *__res
     const int *segs = plan + 26 + 2 * nvecs;
     const int seg_end = plan[16 + xcc + 1];
     int cursor = plan[16 + xcc];
+    int bt = 0, be = 0;  // thread 0: the next ticket of the workgroup's batch, its end
     for (;;) {
         if (threadIdx.x == 0) {
-            const int t = xcd_add(plan + xcc, 1);
+            if (bt >= be) {
+                bt = xcd_add(plan + xcc, batch);
+                be = bt + batch;
+            }
+            const int t = bt++;
             int task = -1;
             if (t < plan[8 + xcc]) {
                 while (cursor < seg_end && t >= segs[kSegInts * cursor + 3] + segs[kSegInts * cursor + 4])
@@ -1558,9 +1574,10 @@ void hx_fused_np_launch(const HxJob *d_jobs, int *d_plan, int np, int max_ticket
     std::call_once(once, [] { allow_lds(hx_bf2_fused_kernel<NPMAX>, bf2_lds(NPMAX)); });
     // workgroups per XCD: two per CU (32 CUs) unless an XCD has fewer planes
     static const int wpx = ppw_of("PISCES_HXF_WG", 64);
+    static const int batch = ppw_of("PISCES_HXF_BATCH", 4);
     const int w = std::max(1, std::min(wpx, max_tickets));
     hipLaunchKernelGGL((hx_bf2_fused_kernel<NPMAX>), dim3(kXcds * w), dim3(kPT), bf2_lds(np), s,
-                       d_jobs, d_plan);
+                       d_jobs, d_plan, batch);
 }
 }  // namespace
 
@@ -1592,11 +1609,13 @@ bool one_launch_available()
     return on;
 }
 
-// The one-launch forms are opt-in (PISCES_HX_FUSED=1, PISCES_FDM_FUSED=1): on
-// the served 64^3 batch they took 0.62 ms (H*X) and 0.80 ms (preconditioner)
-// per launch against 0.12 and 0.16 ms for the multi-launch forms (0.53 ms for
-// H*X at 4 waves per SIMD): each ticket puts a counter round trip, the plane's
-// load latency and a store drain in series.
+// The one-launch forms are opt-in (PISCES_HX_FUSED=1, PISCES_FDM_FUSED=1): at
+// 64^3 x 64 vectors they take 0.29 ms (H*X) and 0.43 ms (preconditioner)
+// against 0.086 and 0.129 ms for the multi-launch forms (0.49 and 0.67 ms with
+// one ticket per counter increment).  A ticket's plane takes ~14 us from draw
+// to counted store, and an XCD keeps 64 - 128 of them in flight, where the
+// multi-launch grids keep thousands of planes going; these kernels are bound
+// by that latency, not by the HBM traffic the L2 reuse saves.
 bool hx_fused_enabled()
 {
     static const bool on = [] {
@@ -1678,9 +1697,10 @@ void fdm_fused_np_launch(const FdmTask *d_tasks, int *d_plan, int np, int max_ti
     static std::once_flag once;
     std::call_once(once, [] { allow_lds(fdm_bf2_fused_kernel<NPMAX>, bf2_lds(NPMAX)); });
     static const int wpx = ppw_of("PISCES_FDMF_WG", 64);
+    static const int batch = ppw_of("PISCES_FDMF_BATCH", 4);
     const int w = std::max(1, std::min(wpx, max_tickets));
     hipLaunchKernelGGL((fdm_bf2_fused_kernel<NPMAX>), dim3(kXcds * w), dim3(kPT), bf2_lds(np), s,
-                       d_tasks, d_plan);
+                       d_tasks, d_plan, batch);
 }
 }  // namespace
 
