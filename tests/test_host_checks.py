@@ -1,6 +1,6 @@
 """Host-side logic of the library that runs without a GPU: the polarization
-matrix inverse behind PolType 3 (both of its methods) and the argument checks
-of the PolType 3 potential build's binding."""
+matrix inverse behind PolType 3 (both of its methods), the argument checks
+of the PolType 3 potential build's binding and of the bandwidth probes."""
 import numpy as np
 import pytest
 
@@ -109,3 +109,22 @@ def test_dual_coarse_grid_matches_oracle(pisces, oracle_mod):
         pisces.dual_coarse((32, 32, 33), (40.0,) * 3)
     with pytest.raises(pisces.PiscesError):
         pisces.dual_coarse((3, 3, 3), (40.0,) * 3)
+
+
+def test_probe_argument_checks(pisces):
+    """The copy and read-stream probes reject sizes that are not positive
+    multiples of 16 bytes, non-positive repetition counts and a missing result
+    pointer before they touch the device."""
+    import ctypes
+    L = pisces.lib()
+    ms = ctypes.c_double(0)
+    L.pisces_copy_probe.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_double)]
+    L.pisces_stream_probe.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double)]
+    for nbytes, reps in ((24, 1), (0, 1), (-16, 1), (1024, 0)):
+        assert L.pisces_copy_probe(0, nbytes, reps, ctypes.byref(ms)) == -1
+        for nt in (0, 1):
+            assert L.pisces_stream_probe(0, nbytes, reps, nt, ctypes.byref(ms)) == -1
+    assert L.pisces_copy_probe(0, 1024, 1, None) == -1
+    assert L.pisces_stream_probe(0, 1024, 1, 0, None) == -1
