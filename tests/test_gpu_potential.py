@@ -26,6 +26,31 @@ def test_build_potential_lists(pisces, oracle_mod, flagset):
     assert np.abs(V - Vo).max() <= 1e-13 * np.abs(Vo).max()
 
 
+@pytest.mark.parametrize("flagset", FLAGSETS)
+def test_site_lists_in_any_order(pisces, oracle_mod, flagset):
+    """The charge and dipole lists in the cluster's order and reversed (the
+    same terms, summed in the other order) give the oracle's potential to
+    rounding, and the same potential as each other."""
+    f = golden("potential_c1.npz")
+    cl = oracle_mod.Cluster(f["xyz"])
+    n, L = (24, 24, 24), (30.0, 30.0, 30.0)
+    g = oracle_mod.Grid(n, L)
+    flags, pp = f["flags_" + flagset], f["potpara_" + flagset]
+    Vo = cl.potential_grid(g, flags, pp)
+    iq, idd = np.asarray(cl.iq), np.asarray(cl.id)
+    assert np.all(np.diff(iq) > 0) and np.all(np.diff(idd) > 0)
+    d = np.asarray(cl.d).reshape(-1, 3)
+    with pisces.Context(n, L) as ctx:
+        ctx.build_potential(flags, pp, cl.sites, cl.q, iq, d, idd)
+        Va = ctx.get_potential()
+        ctx.build_potential(flags, pp, cl.sites, np.asarray(cl.q)[::-1], iq[::-1], d[::-1], idd[::-1])
+        Vr = ctx.get_potential()
+    scale = np.abs(Vo).max()
+    assert np.abs(Va - Vo).max() <= 1e-13 * scale
+    assert np.abs(Vr - Vo).max() <= 1e-13 * scale
+    assert np.abs(Va - Vr).max() <= 1e-13 * scale
+
+
 @pytest.mark.parametrize("which", ["iq", "id"])
 def test_site_on_a_grid_point(pisces, oracle_mod, which):
     """A charge site (which = iq) or a dipole site (id) exactly on a grid point
