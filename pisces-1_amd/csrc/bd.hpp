@@ -184,8 +184,10 @@ constexpr int kGTile = 8;           // VALU variant: 8 x 8 tiles of (A, B) colum
 constexpr int kGRowsValu = 256 * 32; // VALU variant: rows per chunk (32 per thread)
 // Narrow-B tasks (mb <= kGNarrowB) run on the VALU with kGNarrowA x kGNarrowB
 // register tiles, kGRowsNarrow rows per workgroup (PISCES_GRAM_NARROW=0 sends
-// them to the configured variant instead).
-constexpr int kGNarrowA = 8, kGNarrowB = 4, kGRowsNarrow = 256 * 32;
+// them to the configured variant instead).  4096 rows: a launch's last round
+// of workgroups is less empty than with 8192 (served (H2O)24 64^3 batch on
+// three lanes: 544 against 535 solves/s; 2048 rows: 538).
+constexpr int kGNarrowA = 8, kGNarrowB = 4, kGRowsNarrow = 256 * 16;
 bool gram_narrow(int mb);
 // Gram kernel variant (0 VALU register tiles, 1 matrix cores with 16-row
 // blocks, 2 matrix cores with 64-row runs) and rows per workgroup, and the
