@@ -326,6 +326,9 @@ typedef struct {
     int n_matvec_lo;  /* H*x products in the low precision (mixed forms)       */
     int n_iter;       /* Davidson iterations (subspace expansions)             */
     int n_macro;      /* macro iterations (restarts + 1)                      */
+    int n_orth_data;  /* block solver, FP32-stored forms: steps whose second
+                         orthogonalisation round ran on the stored columns
+                         (the rest took it from the Gram algebra); 0 otherwise */
     double seconds;   /* wall time of the solve                                */
     double resid[64]; /* final FP64 residual norms ||H u - lambda u||, per root */
 } pisces_stats;

@@ -1265,6 +1265,7 @@ __global__ void __launch_bounds__(64) orth2_kernel(const Orth2Task *__restrict__
         if (tid == 0) {
             *t.alg = 0;
             *t.skip2 = 0;
+            if (t.ndata) *t.ndata += 1;
         }
         return;
     }

@@ -110,6 +110,7 @@ struct Orth2Task {
     double tol1, tol2, amp2_max;
     int *alg; int *skip2;
     double *Sout;  // S itself (its new columns)
+    int *ndata;    // optional: incremented where the second round is left to the data
 };
 void bd_orth2(const Orth2Task *d_tasks, int ntasks, hipStream_t s);
 

@@ -45,7 +45,7 @@ constexpr int oHG = 9856;    // [Xc|C|T]^T T of an orthogonalisation round ((k +
 constexpr int oVf = oHG + (kBdMaxM + kBdMaxK) * kBdMaxK;  // eigenvectors of the last solve (m x m, ld kBdMaxM)
 constexpr int oS = oVf + kBdMaxM * kBdMaxM;  // S = V^T V of the stored basis [Xc | C] (ld kBdMaxM)
 constexpr int kSmall = oS + kBdMaxM * kBdMaxM;
-static_assert(oKeepT * 8 + (2 * kBdMaxK + 2) * 4 <= oGu * 8, "new-block mask and flags fit before oGu");
+static_assert(oKeepT * 8 + (2 * kBdMaxK + 3) * 4 <= oGu * 8, "new-block mask, flags and count fit before oGu");
 // |W - I|, |F| bound under which the second orthogonalisation round's
 // combination of FP32 columns is left out: it would move an element by about
 // (m + nt) * 1e-9 of the column scale, well below FP32 rounding (6e-8)
@@ -84,6 +84,8 @@ struct BdEngine::Inst {
     // its combination is a no-op / both rounds came from one Gram and S
     int *skip2() const { return keepT() + 2 * kBdMaxK; }
     int *alg() const { return keepT() + 2 * kBdMaxK + 1; }
+    // steps whose second orthogonalisation round was left to the stored data
+    int *ndata() const { return keepT() + 2 * kBdMaxK + 2; }
     double *Sv() const { return small.as<double>() + oS; }
 };
 
@@ -159,13 +161,16 @@ BdEngine::BdEngine(int device, hipStream_t s, int precision, int block, int maxs
     // The second orthogonalisation round of the FP32-stored forms comes from S
     // (no pass over the data) unless the first round's squared amplification
     // |T|^2 / |T - V h0|^2 of a kept column exceeds amp2_max: the algebraic
-    // round's error is ~1e-16 times that amplification (1.1-3.8 on the bench),
-    // so at the default 1e6 it stays below 1e-10, and an instance above it runs
-    // the round on its data (the launches are there and skip the others on the
-    // device).  PISCES_AMP2_MAX=<a> sets the threshold (1: always on the data;
-    // 0: never, whatever the amplification).
+    // round's error grows with that amplification (1.1-3.8 on the (H2O)24
+    // bench, which then never takes the data path), and an instance above it
+    // runs the round on its data (the launches are there and skip the others
+    // on the device).  The default 1e3: a zero potential's six lowest states
+    // (a split triplet, split-BF16 products) stalled with 1e5 and 1e6 and
+    // converged in 17 steps, one of them on the data, with 1e3 and 1e4.
+    // PISCES_AMP2_MAX=<a> sets the threshold (1: always on the data; 0: never,
+    // whatever the amplification).
     const char *am = std::getenv("PISCES_AMP2_MAX");
-    const double a2 = am ? std::atof(am) : 1e6;
+    const double a2 = am ? std::atof(am) : 1e3;
     data_round2_ = a2 > 0;
     amp2_max_ = a2 > 0 ? a2 : 1e300;
     gram_load_config();
@@ -241,6 +246,7 @@ int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, 
                           hipMemcpyDeviceToDevice, s_));
     PC_HIP(hipMemcpyAsync(in->sm() + oI, dinit_.as<unsigned char>() + kInitEye,
                           sizeof(double) * k_ * k_, hipMemcpyDeviceToDevice, s_));
+    PC_HIP(hipMemsetAsync(in->ndata(), 0, sizeof(int), s_));
     // the caller's host start vectors may go out of scope after the return
     if (nstart > 0 && !start_on_device) PC_HIP(hipStreamSynchronize(s_));
     in->res.nroots = nroots;
@@ -255,6 +261,14 @@ int BdEngine::add(const BdGrid &g, int nroots, const double *start, int nstart, 
 
 bool BdEngine::finished(int slot) const { return inst_.at(slot)->done; }
 const BdResult &BdEngine::result(int slot) const { return inst_.at(slot)->res; }
+
+int BdEngine::data_rounds(int slot)
+{
+    int v = 0;
+    PC_HIP(hipMemcpyAsync(&v, inst_.at(slot)->ndata(), sizeof(int), hipMemcpyDeviceToHost, s_));
+    PC_HIP(hipStreamSynchronize(s_));
+    return v;
+}
 
 void BdEngine::vectors(int slot, double *out)
 {
@@ -1029,7 +1043,7 @@ int BdEngine::step_b()
                     hot[0][i].noop = one;
                     h2t[n2++] = Orth2Task{nullptr, 0, 0, m, in->Sv(), kBdMaxM, nt, W, F, in->keepT(),
                                           in->keep() + k_ + in->mc, 1e-13, 1e-10, amp2_max_, in->alg(),
-                                          in->skip2(), in->Sv()};
+                                          in->skip2(), in->Sv(), in->ndata()};
                     o2_inst.push_back(i);
                 } else {
                     og[1].h[og[1].n - 1].skip = in->alg();

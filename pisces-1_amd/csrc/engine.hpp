@@ -76,6 +76,9 @@ public:
     hipStream_t stream() const { return s_; }
     bool finished(int slot) const;
     const BdResult &result(int slot) const;
+    // steps of the slot whose second orthogonalisation round ran on the stored
+    // data (the FP32-stored forms; waits for the stream)
+    int data_rounds(int slot);
     // Ritz vectors of a finished instance (nroots x N doubles, host).
     void vectors(int slot, double *out);
     // Device pointer to the Ritz vectors of a finished instance (valid until release).
@@ -103,9 +106,9 @@ private:
     bool trace_ = false;  // PISCES_TRACE=1: per-step Ritz values of the first instance
     double refresh_below_ = 1e-3;  // late restarts refresh H X in FP64 below this residual
     // second orthogonalisation round on the data where the first round's
-    // squared amplification exceeds amp2_max_ (default 1e6; PISCES_AMP2_MAX)
+    // squared amplification exceeds amp2_max_ (default 1e3; PISCES_AMP2_MAX)
     bool data_round2_ = true;
-    double amp2_max_ = 1e6;
+    double amp2_max_ = 1e3;
     void grow_part(size_t bytes);
     std::vector<dbuf> pool_;  // released instance buffers, reused by add()
     std::vector<Inst *> act_, rs_, lt_;  // the step between step_a() and step_b()

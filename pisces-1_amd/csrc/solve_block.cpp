@@ -46,6 +46,7 @@ int solve_block(pisces_ctx *c, int nroots, int block, int maxsub, int max_iter, 
         st->n_matvec_lo = r.n_mvlo;
         st->n_iter = r.n_iter;
         st->n_macro = r.n_restart;
+        st->n_orth_data = eng.data_rounds(slot);
         st->seconds = std::chrono::duration<double>(t1 - t0).count();
         for (int i = 0; i < nroots && i < 64; ++i) st->resid[i] = r.resid[i];
     }

@@ -33,6 +33,7 @@ class Stats(ctypes.Structure):
         ("n_matvec_lo", ctypes.c_int),
         ("n_iter", ctypes.c_int),
         ("n_macro", ctypes.c_int),
+        ("n_orth_data", ctypes.c_int),
         ("seconds", ctypes.c_double),
         ("resid", ctypes.c_double * 64),
     ]
@@ -505,7 +506,7 @@ def _solve_block(self, nroots, start, block=None, maxsub=32, max_iter=2000, ptol
                "pisces_solve_block")
     return dict(nconv=rc, evals=ev, vecs=vecs, n_matvec=st.n_matvec,
                 n_matvec_lo=st.n_matvec_lo, n_iter=st.n_iter, n_macro=st.n_macro,
-                seconds=st.seconds, resid=np.array(st.resid[:nroots]))
+                n_orth_data=st.n_orth_data, seconds=st.seconds, resid=np.array(st.resid[:nroots]))
 
 
 Context.solve_block = _solve_block
