@@ -361,7 +361,10 @@ int pisces_solve(pisces_ctx *ctx, int nroots, int maxsub, int maxmacro, int ptol
  * Ritz block and its H-image are FP64 throughout and, in the low-precision
  * forms, H*X of the Ritz block is recomputed in FP64 at every restart (defect
  * correction), so reported eigenvalues and residuals are FP64-exact.
- * Converged when ||H u_i - theta_i u_i|| < 10^-ptol for i < nroots.
+ * Converged when ||H u_i - theta_i u_i|| < 10^-ptol for i < nroots.  The
+ * FP32-stored forms converged to ptol 8 on every case tried; at ptol 9 they
+ * converge on most but stalled on a separable (degenerate) model potential,
+ * which the FP64 form solves: ask for FP64 where 1e-9 is required.
  * precond: PISCES_PRECOND_DIAG, the reference's Davidson correction
  * r / (theta - diag H) (src/Davidson.cpp:557-566), or PISCES_PRECOND_KINETIC,
  * t = -(T - theta)^{-1} r applied exactly through the eigenbases of the 1-D
