@@ -41,13 +41,9 @@ __device__ inline void st_any(void *p, long long i, int is32, double v)
     else glb(static_cast<double *>(p))[i] = v;
 }
 // Streamed outputs (combinations, Ritz vectors, their H-images, residuals)
-// are stored with the nontemporal hint: on the served batch the combination
-// family went from 4.93 to 5.13 TB/s and the Ritz/residual family from 5.69
-// to 5.92 TB/s.
-template <typename T> __device__ __forceinline__ void stnt(gptr<T> p, T v)
-{
-    __builtin_nontemporal_store(v, p);
-}
+// are stored with the nontemporal hint (stnt): on the served batch the
+// combination family went from 4.93 to 5.13 TB/s and the Ritz/residual family
+// from 5.69 to 5.92 TB/s.
 __device__ inline void st_any_nt(void *p, long long i, int is32, double v)
 {
     if (is32) stnt(glb(static_cast<float *>(p)) + i, (float)v);

@@ -37,6 +37,13 @@ template <typename T> __device__ __forceinline__ T ldnt(const T *p)
     return __builtin_nontemporal_load(p);
 }
 
+// Stores of outputs that no later work of the same launch reads again, with
+// the nontemporal hint.
+template <typename T> __device__ __forceinline__ void stnt(gptr<T> p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 // 1/x and 1/sqrt(x) in FP64 to about one ulp: the hardware estimates refined
 // by two Newton-Raphson steps each (r <- r + r (1 - x r);
 // y <- y (3 - x y^2) / 2), each step squaring the relative error.
