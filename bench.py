@@ -248,7 +248,7 @@ def main():
         print(json.dumps(pmc_summary(*a.pmc_summary), indent=1))
         return
     if a.gpus != 1:
-        print("bench.py: one GPU only; this build has no multi-GPU path (--gpus %d)" % a.gpus,
+        print("bench.py: one GPU only; the library has no multi-GPU path (--gpus %d)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
 
