@@ -1486,6 +1486,7 @@ void hx_mfma_launch(const HxJob *d_jobs, int njobs, int max_planes_a, int max_pl
     if (precision == PISCES_BF16X2) {
         if (np <= 32) hx_bf2_launch<32>(d_jobs, njobs, max_planes_a, max_planes_b, np, s);
         else if (np <= 64) hx_bf2_launch<64>(d_jobs, njobs, max_planes_a, max_planes_b, np, s);
+        else if (np <= 96) hx_bf2_launch<96>(d_jobs, njobs, max_planes_a, max_planes_b, np, s);
         else hx_bf2_launch<128>(d_jobs, njobs, max_planes_a, max_planes_b, np, s);
     } else if (precision == PISCES_FP32) {
         init_f();
@@ -1735,6 +1736,7 @@ void fdm_launch(const FdmTask *d_tasks, int ntasks, int max_n, int precision, hi
     // planes per pass: the largest of ny (pass 1) and nz (passes 0, 2) <= max_n
     if (np <= 32) fdm_np_launch<32>(d_tasks, ntasks, max_n, np, precision, s);
     else if (np <= 64) fdm_np_launch<64>(d_tasks, ntasks, max_n, np, precision, s);
+    else if (np <= 96) fdm_np_launch<96>(d_tasks, ntasks, max_n, np, precision, s);
     else fdm_np_launch<128>(d_tasks, ntasks, max_n, np, precision, s);
     PC_HIP(hipGetLastError());
 }
