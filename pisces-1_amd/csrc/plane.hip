@@ -1403,7 +1403,7 @@ void init_f()
 
 // Planes per workgroup (measured at 64^3 with 8 - 160 vectors): the H*X
 // kernels run fastest with one plane per workgroup, the split-BF16
-// preconditioner with four (one above 64-point planes).  PISCES_PPW_HX /
+// preconditioner with four (one for 32-, 96- and 128-point planes).  PISCES_PPW_HX /
 // PISCES_PPW_F64 / PISCES_PPW_FDM override.
 int ppw_of(const char *env, int def)
 {
@@ -1461,10 +1461,10 @@ void fdm_np_launch(const FdmTask *d, int n, int nplanes, int np, int precision, 
         hipLaunchKernelGGL((fdm_f64_kernel<2, NPMAX>), dim3(nplanes, n), dim3(kPT), lds, s, d);
     } else {
         const size_t lds = bf2_lds(np);
-        // four planes per workgroup up to 64-point planes; one above, where the
-        // multi-plane form holds one wave per SIMD (96^3 x 64 vectors: 0.64
-        // against 1.13 ms)
-        static const int ppw = ppw_of("PISCES_PPW_FDM", NPMAX <= 64 ? 4 : 1);
+        // four planes per workgroup for 64-point planes, one otherwise (64
+        // vectors: 96^3 0.64 against 1.13 ms, where the multi-plane form holds
+        // one wave per SIMD; 32^3 0.025 against 0.038 ms)
+        static const int ppw = ppw_of("PISCES_PPW_FDM", NPMAX == 64 ? 4 : 1);
         const dim3 grid((nplanes + ppw - 1) / ppw, n);
         if (ppw == 1) {
             hipLaunchKernelGGL((fdm_bf2_kernel<0, NPMAX, false>), grid, dim3(kPT), lds, s, d);
