@@ -14,9 +14,8 @@
  * Davidson.cpp, Potential.cpp, MolPolAux.cpp, DerivDDTensor.cpp, Gradients.cpp,
  * DPP.cpp and Water.cpp), the mathematics (tests/test_oracle_*.py), or "parity
  * unpinned" where neither reaches.  The table in oracle/README.md collects them.
- * Parity unpinned: or_piab_start (the half-box start vector; DVR.cpp, where it
- * lives, needs FFTW headers and does not build here; it changes only the
- * iteration path, and the reference fixtures were run from the same start).
+ * or_piab_start (the half-box start vector) is pinned by the mathematics only:
+ * DVR.cpp, where it lives, needs FFTW headers and does not build here.
  */
 #ifndef PISCES_ORACLE_H
 #define PISCES_ORACLE_H
@@ -66,8 +65,9 @@ void or_diag(int nx, int ny, int nz, const double *Tx, const double *Ty,
 /* Half-box particle-in-a-box start vector; src/DVR.cpp:1725-1805. */
 void or_piab_start(int nx, int ny, int nz, const double *xg, const double *yg,
                    const double *zg, double *wf);
-/* Parity unpinned: only a start vector (the reference fixtures were computed
- * from these same start vectors, so it fixes iteration paths, not results). */
+/* Pinned by its closed form on the symmetric grid: prod_d cos(2 pi q_d / Lg_d)
+ * inside the central half box |q| <= Lg / 4 (Lg = last - first grid point),
+ * zero outside (tests/test_oracle_math.py). */
 
 /* ---------------------------------------------------------------- a8 */
 /* Rigid DPP waters from O,H1,H2 Cartesian input (Angstrom, 9 doubles/water).
@@ -77,9 +77,10 @@ void or_piab_start(int nx, int ny, int nz, const double *xg, const double *yg,
  * (12 doubles/water) and the configuration (x,y,z,phi,theta,psi).
  * Returns 0, or -1 if a projection produced a non-finite value. */
 int or_water_sites(int nw, const double *xyz_ang, double *sites, double *conf);
-/* Pinned by invariants only (Water.cpp does not compile here): rigid input is
- * reproduced, M on the bisector 0.25 A from O, centre of mass with masses 16/1/1.
- * Parity unpinned: the Euler-angle conventions for non-rigid input. */
+/* Pinned: the reference's own WaterCluster projection (Water.cpp built with a
+ * one-token fix) on the C1, C2, C3 geometries and a non-rigid one (sites and
+ * Euler angles, tests/golden/water_*.npz), and invariants: rigid input is
+ * reproduced, M on the bisector 0.25 A from O, centre of mass 16/1/1. */
 
 /* Charge-dipole and dipole-dipole damping factors of the water model
  * (ElectrostaticDCDamp / ElectrostaticDDDamp, src/DPP.cpp:1798-1812):
@@ -126,8 +127,9 @@ int or_potential_points(const int *flags, const double *potpara, int nsites,
                         int npts, const double *pts, double *v, double *parts);
 /* Pinned: the reference's Potential::Evaluate on the same lists for nine flag
  * sets (potential fixtures), rigid-motion invariance, far field, Reff C1
- * continuity.  Parity unpinned: the ion term (no reference combines it with
- * waters; checked only against its closed form). */
+ * continuity.  The ion term (no reference combines it with waters) is pinned
+ * by invariants: alone it is -q erf(R / r_Na) / R, far from the waters it adds
+ * -q / R. */
 
 /* Whole grid, x fastest: points x_i, y_j, z_k from the 1-D grids. */
 int or_potential_grid(const int *flags, const double *potpara, int nsites,
@@ -181,8 +183,8 @@ int or_water_energy(int nw, const double *sites, const double *mu, double *out);
 
 int or_water_lists(int nw, const double *sites, const double *mu, double *q,
                    int *iq, double *d, int *id);
-/* Parity unpinned beyond charge neutrality of each water: the fixtures pass
- * these same lists to the reference. */
+/* Pinned: the reference's GetLists charges and site indices and its
+ * ReportInducedDipoles dipoles and sites (tests/golden/water_*.npz). */
 
 /* ---------------------------------------------------------------- a9, a10 */
 typedef void (*or_matvec_fn)(const double *x, double *y, int nvec, void *ctx);

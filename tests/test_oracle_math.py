@@ -178,3 +178,29 @@ def test_syev_matches_lapack(oracle_mod):
         w = np.linalg.eigvalsh(A)
         np.testing.assert_allclose(W, w, rtol=0, atol=1e-12 * max(1.0, np.abs(w).max()))
         np.testing.assert_allclose(A @ Vv, Vv * W[None, :], rtol=0, atol=1e-11 * max(1.0, np.abs(w).max()))
+
+
+def test_piab_start_is_the_central_half_box_ground_state(oracle_mod):
+    """The start vector (DVR::ParticleInAnDBoxWf, src/DVR.cpp:1725-1805) on the
+    symmetric sine-DVR grid: with Lg = x_last - x_first per axis it is the
+    lowest particle-in-a-box state of the central half box |q| <= Lg / 4,
+    prod_d cos(2 pi q_d / Lg_d) there and zero outside, x fastest.  A slip in
+    the shift (the full box), the factor 2, the mask or the axis order fails
+    this closed form."""
+    for n, L in [((12, 11, 10), (30.0, 26.0, 22.0)), ((33, 32, 17), (36.0, 36.0, 20.0))]:
+        g = oracle_mod.Grid(n, L)
+        wf = g.piab()
+        f = []
+        for d in range(3):
+            x = g.x[d]
+            Lg = x[-1] - x[0]
+            assert abs(x[0] + x[-1]) < 1e-13  # symmetric grid
+            f.append(np.where(np.abs(x) <= Lg / 4 + 1e-12, np.cos(2 * np.pi * x / Lg), 0.0))
+        expect = np.einsum("k,j,i->kji", f[2], f[1], f[0]).ravel()
+        np.testing.assert_allclose(wf, expect, rtol=0, atol=1e-14)
+        assert (wf >= -1e-15).all() and np.count_nonzero(np.abs(wf) > 1e-12) > 0
+    # a grid with a centre point on every axis peaks at exactly 1 there
+    g = oracle_mod.Grid((33, 33, 33), (36.0, 36.0, 36.0))
+    wf = g.piab()
+    assert abs(wf[16 + 33 * (16 + 33 * 16)] - 1.0) < 1e-15
+    assert wf.max() <= 1.0 + 1e-15
