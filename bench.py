@@ -2,7 +2,8 @@
 
 Workload: (H2O)24 anions, 24 rigid waters placed from a seed in a 6 A sphere
 (one seed per snapshot), DPP electron-water potential with polarization, a
-sine-DVR 64^3 grid on a 40 bohr cube, lowest 4 states (block 4, subspace 28,
+sine-DVR 64^3 grid on a 40 bohr cube, lowest 4 states (block 4, subspace 36
+for the mixed-precision form and 28 for the FP64 forms, each its measured best;
 residual tolerance 1e-7).  A step is one batch of B such snapshots submitted
 together to the continuous-batching server and solved to completion; the
 potential build (rigid-water projection, induced dipoles, grid potential) is
@@ -40,7 +41,9 @@ RADIUS = 6.0
 SEED = 2000
 NROOTS = 4
 BLOCK = 4        # block engine: block and subspace of the served solver's 4-state class
-MAXSUB = 28
+MAXSUB = 36       # served mixed-precision form (36: 564 solves/s; 28: 546, 44: 546)
+MAXSUB_FP64 = 28  # FP64 comparators (28: 267 batched solves/s; 36: 253)
+MAXSUB_MIX = 28   # the batched and served mixes of 32^3-96^3 grids
 FP64_MATRIX_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix
 REF_MAXSUB = 32  # the reference algorithm's subspace (cpu_baseline's matvec count)
 PTOL = 7
@@ -63,7 +66,7 @@ def hx_algorithmic(n, nvec, elem):
     return (2 * nvec + 1) * ngp * elem, (6 * n + 4) * ngp * nvec
 
 
-def run_single(P, geoms, precision):
+def run_single(P, geoms, precision, maxsub=MAXSUB_FP64):
     """One solve at a time on one context (single-instance block engine)."""
     n, L = (N_GRID,) * 3, (L_BOX,) * 3
     ctx = P.Context(n, L)
@@ -73,21 +76,24 @@ def run_single(P, geoms, precision):
         t0 = time.perf_counter()
         ctx.build_cluster_potential(xyz)
         r = ctx.solve_block(NROOTS, P.start_vectors(ctx, NROOTS, seed=i), block=BLOCK,
-                            maxsub=MAXSUB, ptol=PTOL, precision=precision)
+                            maxsub=maxsub, ptol=PTOL, precision=precision)
         lat.append(time.perf_counter() - t0)
         out.append(r)
     ctx.close()
     return out, lat
 
 
-def run_server(P, geoms, precision, steps, warmup, prof=False, lanes=None):
-    """`lanes`: the server's engine lanes (None: its default, three)."""
+def run_server(P, geoms, precision, steps, warmup, prof=False, lanes=None, maxsub=None):
+    """`lanes`: the server's engine lanes (None: its default, three); `maxsub`:
+    the subspace (None: MAXSUB for the mixed forms, MAXSUB_FP64 for FP64)."""
+    if maxsub is None:
+        maxsub = MAXSUB_FP64 if precision == P.FP64 else MAXSUB
     if lanes is None:
         os.environ.pop("PISCES_SERVER_LANES", None)
     else:
         os.environ["PISCES_SERVER_LANES"] = str(lanes)
     sv = P.Server(hbm_cap_bytes=200 << 30, max_slots=len(geoms), precision=precision,
-                  maxsub=MAXSUB, ptol=PTOL)
+                  maxsub=maxsub, ptol=PTOL)
     n, L = (N_GRID,) * 3, (L_BOX,) * 3
 
     def one_step():
@@ -366,8 +372,8 @@ def main():
                                "H*X %s" % (BLOCK, MAXSUB, PTOL, B, a.precision),
                    "batch": B, "parallelism": "single GPU",
                    "note": "the C2 configuration states block 8 and subspace 32; the served "
-                           "4-state class runs block 4, subspace 28 (fewer bytes per step; "
-                           "the FP64 comparator uses the same parameters)"},
+                           "4-state class runs block 4 and each form its measured best "
+                           "subspace: 36 mixed, 28 FP64 (fewer bytes per step)"},
         "fp64_gpu": {"single_solves_per_s": round(fp64_single_sps, 3),
                      "single_p50_ms": round(1000 * float(np.percentile(lat64, 50)), 3),
                      "batched_solves_per_s": round(fp64_batched_sps, 3)},
@@ -467,7 +473,7 @@ def bench_c4(P, workload, prec, a, count=512):
     rng = np.random.default_rng(SEED_C4)
     snaps = [workload.mixed_snapshot(i, rng, SEED_C4) for i in range(count)]
     log("c4: %d snapshots, grids %s" % (count, {n: sum(s[1] == n for s in snaps) for n, _ in workload.MIX_GRIDS}))
-    sv = P.Server(hbm_cap_bytes=200 << 30, max_slots=count, precision=prec, maxsub=MAXSUB,
+    sv = P.Server(hbm_cap_bytes=200 << 30, max_slots=count, precision=prec, maxsub=MAXSUB_MIX,
                   ptol=PTOL)
     for xyz, n, L, nr in snaps[:32]:  # warm-up: shapes, pools, kernels
         sv.submit((n,) * 3, (L,) * 3, xyz, nr)
@@ -507,7 +513,7 @@ def fp64_reference(P, items):
     """Eigenvalues of the plain FP64 form (cold starts, the same server code)
     for (xyz, n, L, nroots) items."""
     sv = P.Server(hbm_cap_bytes=200 << 30, max_slots=len(items), precision=P.FP64,
-                  maxsub=MAXSUB, ptol=PTOL)
+                  maxsub=MAXSUB_FP64, ptol=PTOL)
     ids = [sv.submit((n,) * 3, (L,) * 3, xyz, nr) for xyz, n, L, nr in items]
     res = {r["id"]: r for r in sv.run_until_done()}
     sv.close()
@@ -528,7 +534,7 @@ def bench_c5(P, workload, prec, a, nchains=64):
         if rng.uniform() < 0.2:
             n, L = workload.MIX_BIG
         chains.append({"xyz": xyz, "n": n, "L": L, "nr": nr})
-    sv = P.Server(hbm_cap_bytes=200 << 30, max_slots=512, precision=prec, maxsub=MAXSUB,
+    sv = P.Server(hbm_cap_bytes=200 << 30, max_slots=512, precision=prec, maxsub=MAXSUB_MIX,
                   ptol=PTOL)
     # warm-up and capacity estimate: one request per chain, all at once
     t0 = time.perf_counter()
