@@ -43,7 +43,7 @@ NROOTS = 4
 BLOCK = 4        # block engine: block and subspace of the served solver's 4-state class
 MAXSUB = 36       # served mixed-precision form (36: 564 solves/s; 28: 546, 44: 546)
 MAXSUB_FP64 = 28  # FP64 comparators (28: 267 batched solves/s; 36: 253)
-MAXSUB_MIX = 28   # the batched and served mixes of 32^3-96^3 grids
+MAXSUB_MIX = 28   # the 32^3-96^3 mixes (512-snapshot batch: 996-1000 solves/s; 36: 960-980)
 FP64_MATRIX_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix
 REF_MAXSUB = 32  # the reference algorithm's subspace (cpu_baseline's matvec count)
 PTOL = 7
